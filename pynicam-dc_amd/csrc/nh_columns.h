@@ -1,0 +1,176 @@
+// Column kernels of the dynamical-core step.
+//
+// Every kernel is a functor `f(t)` over a flat thread index t.  Regional
+// columns come first (t < lall*g*g, i fastest), pole columns after.  A thread
+// marches its own column in k, reading the six neighbour columns of the
+// icosahedral stencil at the same level, so vertical recurrences (half-level
+// interpolation, the tridiagonal solve) stay in registers and every level of a
+// wavefront is a coalesced row of i.
+//
+// Field layout in memory (one field = one array of NF doubles):
+//   regional  ((l*kall + k)*g + j)*g + i        pole  R + (lp*kall + k)*6 + v
+// so a column is (base, stride) with stride g*g (regional) or 6 (pole).
+//
+// The functors are plain structs of pointers and scalars, compiled for the GPU
+// (hipcc) and, in the test build, for the host.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+
+#ifndef NH_HD
+#define NH_HD __host__ __device__
+#endif
+
+#define NH_KMAX 80   // largest supported kall
+
+namespace nh {
+
+struct Dims {
+  int g;       // gall_1d
+  int kall, kmin, kmax;
+  int lall;
+  long nreg;   // lall*g*g   regional columns
+  long R;      // lall*kall*g*g regional points
+  long NF;     // R + 2*kall*6 points per field
+};
+
+struct Consts {
+  double RADIUS, OHM, GRAV, Rdry, CVdry, Rvap, CVW_qv, ALPHA;
+  double GAMMA_H, HDIFF_FACT_RHO, ALPHA_D;
+};
+
+struct Geo {
+  // regional, structure of arrays [comp][nreg]
+  const double* cdiv;    // 21 = d*7 + m
+  const double* cgrad;   // 21
+  const double* clap;    // 7
+  const double* cintp;   // 18 = (d*3 + vertex)*2 + T
+  const double* cdiff;   // 18 = d*6 + arc
+  const double* gx;      // 3  GRD_x [m]
+  // pole, [v][lp][...]
+  const double* cdiv_pl;   // (v*2+lp)*3+d
+  const double* cgrad_pl;  // (v*2+lp)*3+d
+  const double* clap_pl;   // v*2+lp
+  const double* cintp_pl;  // ((v*2+lp)*3+d)*3+vertex
+  const double* cdiff_pl;  // (v*2+lp)*3+d
+  const double* gx_pl;     // (v*2+lp)*3+d
+  const int* sgp;          // [lall] 1 on pentagon regions
+  // vertical profiles [kall]
+  const double *afact, *bfact, *cfact, *dfact, *rdgz, *rdgzh;
+  const double *c2w0, *c2w1, *w2c0, *w2c1, *phi;
+};
+
+enum { I_RHOG = 0, I_RHOGVX, I_RHOGVY, I_RHOGVZ, I_RHOGW, I_RHOGE };
+enum { I_PRE = 0, I_TEM, I_VX, I_VY, I_VZ, I_W };
+
+// ----------------------------------------------------------------------------
+// column addressing
+// ----------------------------------------------------------------------------
+struct Cell {
+  bool pole;    // pole centre (or pole point for all-column kernels)
+  bool inner;   // regional interior point (stencil output location)
+  int i, j, l, v, lp;
+  long base;    // column base address
+  long ks;      // level stride
+  long ci;      // regional cell index (l*g + j)*g + i
+};
+
+NH_HD inline Cell cell_of(const Dims& d, long t) {
+  Cell c;
+  if (t < d.nreg) {
+    long gg = (long)d.g * d.g;
+    c.l = (int)(t / gg);
+    long r = t - (long)c.l * gg;
+    c.j = (int)(r / d.g);
+    c.i = (int)(r - (long)c.j * d.g);
+    c.pole = false;
+    c.inner = (c.i >= 1 && c.i <= d.g - 2 && c.j >= 1 && c.j <= d.g - 2);
+    c.base = ((long)c.l * d.kall * d.g + c.j) * d.g + c.i;
+    c.ks = gg;
+    c.ci = t;
+    c.v = 0;
+    c.lp = 0;
+  } else {
+    long p = t - d.nreg;
+    c.lp = (int)(p / 6);
+    c.v = (int)(p - (long)c.lp * 6);
+    c.pole = true;
+    c.inner = false;
+    c.base = d.R + (long)c.lp * d.kall * 6 + c.v;
+    c.ks = 6;
+    c.ci = -1;
+    c.i = c.j = c.l = 0;
+  }
+  return c;
+}
+
+// stencil offsets in column-base units: (0,0) (+1,0) (+1,+1) (0,+1) (-1,0) (-1,-1) (0,-1)
+NH_HD inline void stencil_offsets(int g, long off[7]) {
+  off[0] = 0; off[1] = 1; off[2] = 1 + g; off[3] = g;
+  off[4] = -1; off[5] = -1 - g; off[6] = -g;
+}
+
+NH_HD inline double rc(const double* a, long nreg, int comp, long ci) { return a[comp * nreg + ci]; }
+
+// 7-point divergence of (ax, ay, az)(point) at one level for a regional cell
+template <class FX, class FY, class FZ>
+NH_HD inline double div7(const Geo& G, long nreg, long ci, FX fx, FY fy, FZ fz) {
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  // same association as the reference: sum over m for each direction, then x + y + z
+  sx = rc(G.cdiv, nreg, 0, ci) * fx(0);
+  for (int m = 1; m < 7; ++m) sx = sx + rc(G.cdiv, nreg, m, ci) * fx(m);
+  sy = rc(G.cdiv, nreg, 7, ci) * fy(0);
+  for (int m = 1; m < 7; ++m) sy = sy + rc(G.cdiv, nreg, 7 + m, ci) * fy(m);
+  sz = rc(G.cdiv, nreg, 14, ci) * fz(0);
+  for (int m = 1; m < 7; ++m) sz = sz + rc(G.cdiv, nreg, 14 + m, ci) * fz(m);
+  return sx + sy + sz;
+}
+
+// pole-centre divergence: sum over v = 0..5 of (cx*ax + cy*ay + cz*az)
+template <class FX, class FY, class FZ>
+NH_HD inline double div_pl(const Geo& G, int lp, FX fx, FY fy, FZ fz) {
+  double s = 0.0;
+  for (int v = 0; v < 6; ++v) {
+    const double* c = G.cdiv_pl + (v * 2 + lp) * 3;
+    double pv = c[0] * fx(v) + c[1] * fy(v) + c[2] * fz(v);
+    s = (v == 0) ? pv : s + pv;
+  }
+  return s;
+}
+
+// ----------------------------------------------------------------------------
+// halo gather
+// ----------------------------------------------------------------------------
+struct HaloGather {
+  Dims d;
+  const int* dst;   // column ids
+  const int* src;
+  long n;           // number of halo columns
+  double* f[16];
+  int nf;
+  NH_HD inline long colbase(long c, long* ks) const {
+    if (c < d.nreg) {
+      long gg = (long)d.g * d.g;
+      long l = c / gg;
+      long r = c - l * gg;
+      *ks = gg;
+      return l * d.kall * gg + r;
+    }
+    long p = c - d.nreg;
+    long lp = p / 6;
+    *ks = 6;
+    return d.R + lp * d.kall * 6 + (p - lp * 6);
+  }
+  // t over n * kall: halo column fastest so a wavefront copies consecutive halo slots
+  NH_HD inline void operator()(long t) const {
+    long h = t % n;
+    int k = (int)(t / n);
+    long ks_d, ks_s;
+    long bd = colbase(dst[h], &ks_d) + k * ks_d;
+    long bs = colbase(src[h], &ks_s) + k * ks_s;
+    for (int q = 0; q < nf; ++q) f[q][bd] = f[q][bs];
+  }
+};
+
+}  // namespace nh
