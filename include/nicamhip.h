@@ -1,0 +1,143 @@
+/*
+ * nicamhip: C-ABI of the MI355X (gfx950) backend of the pyNICAM-DC
+ * dynamical-core time step.
+ *
+ * Plain C types only: dimensions and constants as POD structs, fields as
+ * device pointers to FP64 arrays in the backend layout, streams as opaque
+ * hipStream_t handles (NULL = default stream).  Every entry point returns 0 on
+ * success or a negative NH_ERR_* code; nh_last_error() then describes the
+ * failure (thread-local, valid until the next call on that thread).  The
+ * library never frees caller buffers.
+ *
+ * Field layout (one field = NF doubles):
+ *   regional point (i, j, k, l):  ((l*kall + k)*gall_1d + j)*gall_1d + i
+ *   pole point (v, k, lpl):       R + (lpl*kall + k)*6 + v,  R = lall*kall*gall_1d^2
+ *   NF = R + 2*kall*6
+ * i.e. the reference's (i, j, k, l) / (v, k, lpl) arrays with the axes
+ * reversed, so that a column is strided by gall_1d^2 and a level of a region
+ * is contiguous in i.  Multi-field arguments are NF-strided blocks.
+ *
+ * Regional column id c = (l*gall_1d + j)*gall_1d + i; pole column id
+ * lall*gall_1d^2 + lpl*6 + v.
+ */
+#ifndef NICAMHIP_H
+#define NICAMHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  NH_OK = 0,
+  NH_ERR_ARG = -1,     /* invalid argument (null pointer, size out of range) */
+  NH_ERR_STATE = -2,   /* context not ready (geometry or halo map missing)   */
+  NH_ERR_ALLOC = -3,   /* device allocation failed                           */
+  NH_ERR_DEVICE = -4   /* HIP runtime or kernel launch error                 */
+};
+
+/* Grid dimensions (mod_adm.py:121-182: ADM_gall_1d, ADM_kall, ADM_kmin,
+ * ADM_kmax, ADM_lall). kmin must be 1 and kmax = kall-2 <= 78. */
+typedef struct {
+  int32_t gall_1d;
+  int32_t kall;
+  int32_t kmin;
+  int32_t kmax;
+  int32_t lall;
+} nh_dims;
+
+/* Physical constants and numerical-filter coefficients
+ * (mod_const.py:9-205, mod_runconf.py:13, mod_numfilter.py:304-366). */
+typedef struct {
+  double radius;          /* CONST_RADIUS                                     */
+  double ohm;             /* CONST_OHM                                        */
+  double grav;            /* CONST_GRAV                                       */
+  double rdry;            /* CONST_Rdry                                       */
+  double cvdry;           /* CONST_CVdry                                      */
+  double rvap;            /* CONST_Rvap                                       */
+  double cvw_qv;          /* CVW[I_QV]                                        */
+  double alpha;           /* NON_HYDRO_ALPHA                                  */
+  double gamma_h;         /* hyperdiffusion coefficient, hdiff_type DIRECT    */
+  double hdiff_fact_rho;  /* density factor of the hyperdiffusion             */
+  double alpha_d;         /* 3-D divergence damping coefficient, DIRECT       */
+} nh_consts;
+
+/* Host pointers to the operator coefficients and vertical profiles
+ * (mod_oprt.py:102-110, mod_grd.py:190-222, mod_vmtr.py:30-258).
+ * Regional arrays are structure-of-arrays [component][column]:
+ *   coef_div, coef_grad  21 = dir*7 + stencil point
+ *   coef_lap              7
+ *   coef_intp            18 = (dir*3 + vertex)*2 + triangle (TI, TJ)
+ *   coef_diff            18 = dir*6 + arc
+ *   grd_x                 3   position (m)
+ * Pole arrays: coef_div_pl/coef_grad_pl/coef_diff_pl/grd_x_pl [v][lpl][dir],
+ *   coef_lap_pl [v][lpl], coef_intp_pl [v][lpl][dir][vertex].
+ * Vertical profiles have kall entries; c2w0/c2w1 and w2c0/w2c1 are the two
+ * columns of C2Wfact/W2Cfact, phi the geopotential.  sgp[l] is 1 on the
+ * regions whose (gmin-1, gmin-1) corner is a pentagon. */
+typedef struct {
+  const double *coef_div, *coef_grad, *coef_lap, *coef_intp, *coef_diff, *grd_x;
+  const double *coef_div_pl, *coef_grad_pl, *coef_lap_pl, *coef_intp_pl, *coef_diff_pl, *grd_x_pl;
+  const int32_t* sgp;
+  const double *afact, *bfact, *cfact, *dfact, *rdgz, *rdgzh;
+  const double *c2w0, *c2w1, *w2c0, *w2c1, *phi;
+} nh_geometry;
+
+typedef struct nh_context nh_context;
+
+/* Description of the last failure on the calling thread. */
+const char* nh_last_error(void);
+
+/* Number of doubles per field (NF) for the given dimensions. */
+int64_t nh_field_size(const nh_dims* dims);
+
+/* Create a context on a device and allocate its step workspace.
+ * Replaces the per-process setup chain Adm/Comm/Grd/Gmtr/Oprt/Vmtr
+ * (mod_adm.py:78, mod_comm.py:119, mod_vmtr.py:30) for the step's buffers. */
+int nh_create(const nh_dims* dims, const nh_consts* consts, int device, nh_context** out);
+
+/* Release a context and everything it allocated. */
+int nh_destroy(nh_context* ctx);
+
+/* Copy operator coefficients and vertical profiles to the device
+ * (OPRT_setup mod_oprt.py:125-1386, VMTR_setup mod_vmtr.py:30-258). */
+int nh_upload_geometry(nh_context* ctx, const nh_geometry* geo);
+
+/* Upload the halo gather map: for h < n, column dst[h] <- column src[h] on
+ * every level. Replaces the plan of Comm._build_comm_plan (mod_comm.py:1400-1561)
+ * built from copy_r2r / copy_p2r / copy_r2p / singular lists. */
+int nh_upload_halo_map(nh_context* ctx, const int32_t* dst, const int32_t* src, int64_t n);
+
+/* In-place halo and pole fill of nfields fields (each NF doubles).
+ * Replaces Comm.COMM_data_transfer (mod_comm.py:1066, 1563-1644). */
+int nh_halo_fill(nh_context* ctx, double* const* fields, int32_t nfields, void* stream);
+
+/* 7-point Laplacian on interior points and pole centres; other points of out
+ * are left unchanged.  Replaces OPRT_laplacian (mod_oprt.py:1429,
+ * kernels/oprtlaplacian.py:41-85). */
+int nh_laplacian(nh_context* ctx, const double* scl, double* out, void* stream);
+
+/* 7-point divergence of (vx, vy, vz) on interior points and pole centres.
+ * Replaces the horizontal part of src_flux_convergence (mod_src.py:631,
+ * kernels/fluxconv.py:238 _divergence). */
+int nh_divergence(nh_context* ctx, const double* vx, const double* vy, const double* vz, double* out,
+                  void* stream);
+
+/* 7-point gradient of scl on interior points and pole centres (3 output
+ * fields).  Replaces OPRT_gradient (mod_oprt.py:1387, kernels/oprtgradient.py:45-88). */
+int nh_gradient(nh_context* ctx, const double* scl, double* gx, double* gy, double* gz, void* stream);
+
+/* One dynamical-core large step, in place: prog holds the 6 prognostic
+ * fields (rhog, rhogvx, rhogvy, rhogvz, rhogw, rhoge), progq the nq >= 1
+ * tracer mass fields (tracer 0 = qv).  RK3 with sstep_max/3, sstep_max/2 and
+ * sstep_max HEVI small steps of dtl/sstep_max.
+ * Replaces Dyn.dynamics_step (mod_dynamics.py:1725-2741). */
+int nh_dyn_step(nh_context* ctx, double* prog, double* progq, int32_t nq, double dtl, int32_t sstep_max,
+                void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NICAMHIP_H */

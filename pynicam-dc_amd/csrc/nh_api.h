@@ -1,0 +1,225 @@
+// extern "C" entry points of include/nicamhip.h, generic over the executor
+// type NHX defined by the including translation unit:
+//   NHX(int device, void* stream)  sets the device and binds the stream.
+#pragma once
+
+#include <cstdio>
+#include <cstdlib>
+#include <new>
+#include <string>
+
+#include "nh_engine.h"
+
+struct nh_context {
+  nh::Context c;
+};
+
+namespace nh {
+
+static thread_local std::string g_err;
+
+inline int fail(int code, const char* where, const std::string& msg) {
+  g_err = std::string(where) + ": " + msg;
+  return code;
+}
+
+// single-operator functors (interior points and pole centres, all levels)
+struct OpLap {
+  Dims d; Geo G;
+  const double* s;
+  double* o;
+  NH_HD void operator()(long t) const {
+    Cell cl = cell_of(d, t);
+    if (cl.pole ? cl.v != 0 : !cl.inner) return;
+    for (int k = 0; k < d.kall; ++k) {
+      long a = cl.base + k * cl.ks;
+      o[a] = cl.pole ? lap_pl(G, cl.lp, s, a) : lap7(G, d.nreg, cl.ci, s, a, d.g);
+    }
+  }
+};
+
+struct OpDiv {
+  Dims d; Geo G;
+  const double *vx, *vy, *vz;
+  double* o;
+  NH_HD void operator()(long t) const {
+    Cell cl = cell_of(d, t);
+    if (cl.pole ? cl.v != 0 : !cl.inner) return;
+    long nb[7];
+    neighbours(d, cl, nb);
+    for (int k = 0; k < d.kall; ++k) {
+      long kk = k * cl.ks;
+      o[cl.base + kk] = hdiv(G, d, cl, [&](int m) { return vx[nb[m] + kk]; },
+                             [&](int m) { return vy[nb[m] + kk]; }, [&](int m) { return vz[nb[m] + kk]; });
+    }
+  }
+};
+
+struct OpGrad {
+  Dims d; Geo G;
+  const double* s;
+  double *ox, *oy, *oz;
+  NH_HD void operator()(long t) const {
+    Cell cl = cell_of(d, t);
+    if (cl.pole ? cl.v != 0 : !cl.inner) return;
+    for (int k = 0; k < d.kall; ++k) {
+      long a = cl.base + k * cl.ks;
+      double o[3];
+      if (cl.pole) grad_pl(G, cl.lp, s, a, o);
+      else grad7(G, d.nreg, cl.ci, s, a, d.g, o);
+      ox[a] = o[0];
+      oy[a] = o[1];
+      oz[a] = o[2];
+    }
+  }
+};
+
+inline int check_ready(const nh_context* ctx, const char* where, bool need_halo) {
+  if (!ctx) return fail(NH_ERR_ARG, where, "null context");
+  if (!ctx->c.geo_ready) return fail(NH_ERR_STATE, where, "geometry not uploaded");
+  if (need_halo && ctx->c.halo_n == 0) return fail(NH_ERR_STATE, where, "halo map not uploaded");
+  return 0;
+}
+
+}  // namespace nh
+
+extern "C" {
+
+const char* nh_last_error(void) { return nh::g_err.c_str(); }
+
+int64_t nh_field_size(const nh_dims* dims) {
+  if (!dims) return nh::fail(NH_ERR_ARG, "nh_field_size", "null dims");
+  nh::Dims d;
+  nh::set_dims(d, dims);
+  return d.NF;
+}
+
+int nh_create(const nh_dims* dims, const nh_consts* consts, int device, nh_context** out) {
+  const char* where = "nh_create";
+  if (!dims || !consts || !out) return nh::fail(NH_ERR_ARG, where, "null argument");
+  *out = nullptr;
+  if (dims->gall_1d < 4 || dims->lall < 1 || dims->kmin != 1 || dims->kmax != dims->kall - 2 ||
+      dims->kall > NH_KMAX || dims->kmax - dims->kmin < 2)
+    return nh::fail(NH_ERR_ARG, where, "unsupported dimensions (need gall_1d>=4, kmin=1, kmax=kall-2, kall<=80)");
+  nh_context* ctx = new (std::nothrow) nh_context();
+  if (!ctx) return nh::fail(NH_ERR_ALLOC, where, "host allocation");
+  nh::set_dims(ctx->c.d, dims);
+  nh::set_consts(ctx->c.c, consts);
+  ctx->c.device = device;
+  NHX x(device, nullptr);
+  if (x.check(where)) { delete ctx; return nh::fail(NH_ERR_DEVICE, where, x.message()); }
+  size_t bytes = (size_t)nh::W_NFIELDS * ctx->c.d.NF * sizeof(double);
+  ctx->c.ws = (double*)x.alloc(bytes);
+  if (!ctx->c.ws) {
+    delete ctx;
+    return nh::fail(NH_ERR_ALLOC, where, "workspace allocation of " + std::to_string(bytes) + " bytes");
+  }
+  x.zero(ctx->c.ws, bytes);
+  if (x.check(where)) {
+    x.release(ctx->c.ws);
+    delete ctx;
+    return nh::fail(NH_ERR_DEVICE, where, x.message());
+  }
+  *out = ctx;
+  return 0;
+}
+
+int nh_destroy(nh_context* ctx) {
+  if (!ctx) return 0;
+  NHX x(ctx->c.device, nullptr);
+  x.sync();
+  if (ctx->c.ws) x.release(ctx->c.ws);
+  if (ctx->c.geo_buf) x.release(ctx->c.geo_buf);
+  if (ctx->c.int_buf) x.release(ctx->c.int_buf);
+  if (ctx->c.halo_dst) x.release(ctx->c.halo_dst);
+  if (ctx->c.halo_src) x.release(ctx->c.halo_src);
+  delete ctx;
+  return 0;
+}
+
+int nh_upload_geometry(nh_context* ctx, const nh_geometry* geo) {
+  const char* where = "nh_upload_geometry";
+  if (!ctx || !geo) return nh::fail(NH_ERR_ARG, where, "null argument");
+  NHX x(ctx->c.device, nullptr);
+  int rc = nh::upload_geometry(x, ctx->c, geo);
+  if (rc == NH_ERR_ARG) return nh::fail(rc, where, "null coefficient pointer");
+  if (rc == NH_ERR_ALLOC) return nh::fail(rc, where, "device allocation");
+  if (rc) return nh::fail(rc, where, x.message());
+  return 0;
+}
+
+int nh_upload_halo_map(nh_context* ctx, const int32_t* dst, const int32_t* src, int64_t n) {
+  const char* where = "nh_upload_halo_map";
+  if (!ctx || n < 0 || (n > 0 && (!dst || !src))) return nh::fail(NH_ERR_ARG, where, "invalid argument");
+  NHX x(ctx->c.device, nullptr);
+  int rc = nh::upload_halo(x, ctx->c, dst, src, (long)n);
+  if (rc == NH_ERR_ARG) return nh::fail(rc, where, "column id out of range");
+  if (rc == NH_ERR_ALLOC) return nh::fail(rc, where, "device allocation");
+  if (rc) return nh::fail(rc, where, x.message());
+  return 0;
+}
+
+int nh_halo_fill(nh_context* ctx, double* const* fields, int32_t nfields, void* stream) {
+  const char* where = "nh_halo_fill";
+  if (!ctx || nfields < 0 || (nfields > 0 && !fields)) return nh::fail(NH_ERR_ARG, where, "invalid argument");
+  if (ctx->c.halo_n == 0) return nh::fail(NH_ERR_STATE, where, "halo map not uploaded");
+  for (int q = 0; q < nfields; ++q)
+    if (!fields[q]) return nh::fail(NH_ERR_ARG, where, "null field pointer");
+  NHX x(ctx->c.device, stream);
+  for (int q0 = 0; q0 < nfields; q0 += 16) {
+    int m = (nfields - q0 < 16) ? nfields - q0 : 16;
+    nh::halo(x, ctx->c, fields + q0, m);
+  }
+  if (x.check(where)) return nh::fail(NH_ERR_DEVICE, where, x.message());
+  return 0;
+}
+
+int nh_laplacian(nh_context* ctx, const double* scl, double* out, void* stream) {
+  const char* where = "nh_laplacian";
+  if (int rc = nh::check_ready(ctx, where, false)) return rc;
+  if (!scl || !out) return nh::fail(NH_ERR_ARG, where, "null field pointer");
+  NHX x(ctx->c.device, stream);
+  nh::OpLap f{ctx->c.d, ctx->c.G, scl, out};
+  x.run(f, nh::ncols(ctx->c.d));
+  if (x.check(where)) return nh::fail(NH_ERR_DEVICE, where, x.message());
+  return 0;
+}
+
+int nh_divergence(nh_context* ctx, const double* vx, const double* vy, const double* vz, double* out,
+                  void* stream) {
+  const char* where = "nh_divergence";
+  if (int rc = nh::check_ready(ctx, where, false)) return rc;
+  if (!vx || !vy || !vz || !out) return nh::fail(NH_ERR_ARG, where, "null field pointer");
+  NHX x(ctx->c.device, stream);
+  nh::OpDiv f{ctx->c.d, ctx->c.G, vx, vy, vz, out};
+  x.run(f, nh::ncols(ctx->c.d));
+  if (x.check(where)) return nh::fail(NH_ERR_DEVICE, where, x.message());
+  return 0;
+}
+
+int nh_gradient(nh_context* ctx, const double* scl, double* gx, double* gy, double* gz, void* stream) {
+  const char* where = "nh_gradient";
+  if (int rc = nh::check_ready(ctx, where, false)) return rc;
+  if (!scl || !gx || !gy || !gz) return nh::fail(NH_ERR_ARG, where, "null field pointer");
+  NHX x(ctx->c.device, stream);
+  nh::OpGrad f{ctx->c.d, ctx->c.G, scl, gx, gy, gz};
+  x.run(f, nh::ncols(ctx->c.d));
+  if (x.check(where)) return nh::fail(NH_ERR_DEVICE, where, x.message());
+  return 0;
+}
+
+int nh_dyn_step(nh_context* ctx, double* prog, double* progq, int32_t nq, double dtl, int32_t sstep_max,
+                void* stream) {
+  const char* where = "nh_dyn_step";
+  if (int rc = nh::check_ready(ctx, where, true)) return rc;
+  if (!prog || !progq || nq < 1) return nh::fail(NH_ERR_ARG, where, "need prog and at least one tracer field");
+  if (sstep_max < 3 || sstep_max % 6 != 0 || !(dtl > 0.0))
+    return nh::fail(NH_ERR_ARG, where, "sstep_max must be a positive multiple of 6 and dtl > 0");
+  NHX x(ctx->c.device, stream);
+  int rc = nh::dyn_step(x, ctx->c, prog, progq, nq, dtl, sstep_max);
+  if (rc == NH_ERR_ARG) return nh::fail(rc, where, "invalid argument");
+  if (rc) return nh::fail(NH_ERR_DEVICE, where, x.message());
+  return 0;
+}
+
+}  // extern "C"

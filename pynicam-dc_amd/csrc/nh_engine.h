@@ -1,0 +1,351 @@
+// Context, workspace and launch sequence of the dynamical-core step.
+//
+// Generic over an executor X that provides
+//   void* alloc(size_t bytes);  void release(void* p);
+//   void h2d(void* dst, const void* src, size_t bytes);   (synchronous)
+//   void d2d(void* dst, const void* src, size_t bytes);   (stream ordered)
+//   template <class F> void run(const F& f, long n);      (f(t) for t < n)
+//   int check(const char* what);                          (0 or error)
+// so that the same sequence drives the GPU build and the host test build.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <string>
+
+#include "nh_step.h"
+#include "../../include/nicamhip.h"
+
+namespace nh {
+
+// fields of the step workspace, each NF doubles
+enum WsField {
+  W_PROG0 = 0,            // 6  state at the start of the large step
+  W_DIAG = 6,             // 6  pre, tem, vx, vy, vz, w
+  W_ETH = 12,             // 1
+  W_GADV = 13,            // 4  momentum advection tendency
+  W_HDA = 17,             // 6  hyperdiffusion pass 1
+  W_HDB = 23,             // 6  hyperdiffusion pass 2
+  W_DDT = 29,             // 3  divergence damping, first operator
+  W_DD = 32,              // 3  divergence damping of the large-step state
+  W_DDV = 35,             // 3  divergence damping of the split state
+  W_GT = 38,              // 6  large-step tendencies of the small steps
+  W_MC = 44, W_MU = 45, W_ML = 46,
+  W_GETOT = 47,           // 1
+  W_FRHOG = 48,           // 1
+  W_ETHTOT = 49,          // 1
+  W_PM = 50,              // 5  time-mean mass fluxes
+  W_SPLIT = 55,           // 12 split state (6) + path1 momentum (3) + vi_main out (3)
+  W_NFIELDS = 67
+};
+
+struct Context {
+  Dims d;
+  Consts c;
+  Geo G;
+  // owned device buffers
+  double* geo_buf = nullptr;
+  int* int_buf = nullptr;
+  int* halo_dst = nullptr;
+  int* halo_src = nullptr;
+  long halo_n = 0;
+  double* ws = nullptr;
+  bool geo_ready = false;
+  int device = 0;
+  double* field(int f) const { return ws + (long)f * d.NF; }
+};
+
+inline void set_dims(Dims& d, const nh_dims* in) {
+  d.g = in->gall_1d;
+  d.kall = in->kall;
+  d.kmin = in->kmin;
+  d.kmax = in->kmax;
+  d.lall = in->lall;
+  d.nreg = (long)d.lall * d.g * d.g;
+  d.R = d.nreg * d.kall;
+  d.NF = d.R + 2L * d.kall * 6;
+}
+
+inline void set_consts(Consts& c, const nh_consts* in) {
+  c.RADIUS = in->radius;
+  c.OHM = in->ohm;
+  c.GRAV = in->grav;
+  c.Rdry = in->rdry;
+  c.CVdry = in->cvdry;
+  c.Rvap = in->rvap;
+  c.CVW_qv = in->cvw_qv;
+  c.ALPHA = in->alpha;
+  c.GAMMA_H = in->gamma_h;
+  c.HDIFF_FACT_RHO = in->hdiff_fact_rho;
+  c.ALPHA_D = in->alpha_d;
+}
+
+// geometry sizes in doubles, in upload order
+struct GeoSizes {
+  long reg[6];   // cdiv, cgrad, clap, cintp, cdiff, gx
+  long pl[6];
+  long total;
+};
+
+inline GeoSizes geo_sizes(const Dims& d) {
+  GeoSizes s;
+  const long n = d.nreg;
+  s.reg[0] = 21 * n; s.reg[1] = 21 * n; s.reg[2] = 7 * n;
+  s.reg[3] = 18 * n; s.reg[4] = 18 * n; s.reg[5] = 3 * n;
+  s.pl[0] = 36; s.pl[1] = 36; s.pl[2] = 12; s.pl[3] = 108; s.pl[4] = 36; s.pl[5] = 36;
+  s.total = 0;
+  for (int i = 0; i < 6; ++i) s.total += s.reg[i] + s.pl[i];
+  s.total += 11L * d.kall;
+  return s;
+}
+
+template <class X>
+int upload_geometry(X& x, Context& ctx, const nh_geometry* g) {
+  const Dims& d = ctx.d;
+  GeoSizes s = geo_sizes(d);
+  if (!ctx.geo_buf) {
+    ctx.geo_buf = (double*)x.alloc(s.total * sizeof(double));
+    ctx.int_buf = (int*)x.alloc(d.lall * sizeof(int));
+    if (!ctx.geo_buf || !ctx.int_buf) return NH_ERR_ALLOC;
+  }
+  const double* regsrc[6] = {g->coef_div, g->coef_grad, g->coef_lap, g->coef_intp, g->coef_diff, g->grd_x};
+  const double* plsrc[6] = {g->coef_div_pl, g->coef_grad_pl, g->coef_lap_pl, g->coef_intp_pl, g->coef_diff_pl,
+                            g->grd_x_pl};
+  const double* vsrc[11] = {g->afact, g->bfact, g->cfact, g->dfact, g->rdgz, g->rdgzh,
+                            g->c2w0, g->c2w1, g->w2c0, g->w2c1, g->phi};
+  for (int i = 0; i < 6; ++i)
+    if (!regsrc[i] || !plsrc[i]) return NH_ERR_ARG;
+  for (int i = 0; i < 11; ++i)
+    if (!vsrc[i]) return NH_ERR_ARG;
+  if (!g->sgp) return NH_ERR_ARG;
+  double* p = ctx.geo_buf;
+  const double** regdst[6] = {&ctx.G.cdiv, &ctx.G.cgrad, &ctx.G.clap, &ctx.G.cintp, &ctx.G.cdiff, &ctx.G.gx};
+  const double** pldst[6] = {&ctx.G.cdiv_pl, &ctx.G.cgrad_pl, &ctx.G.clap_pl, &ctx.G.cintp_pl, &ctx.G.cdiff_pl,
+                             &ctx.G.gx_pl};
+  const double** vdst[11] = {&ctx.G.afact, &ctx.G.bfact, &ctx.G.cfact, &ctx.G.dfact, &ctx.G.rdgz, &ctx.G.rdgzh,
+                             &ctx.G.c2w0, &ctx.G.c2w1, &ctx.G.w2c0, &ctx.G.w2c1, &ctx.G.phi};
+  for (int i = 0; i < 6; ++i) {
+    x.h2d(p, regsrc[i], s.reg[i] * sizeof(double));
+    *regdst[i] = p;
+    p += s.reg[i];
+  }
+  for (int i = 0; i < 6; ++i) {
+    x.h2d(p, plsrc[i], s.pl[i] * sizeof(double));
+    *pldst[i] = p;
+    p += s.pl[i];
+  }
+  for (int i = 0; i < 11; ++i) {
+    x.h2d(p, vsrc[i], d.kall * sizeof(double));
+    *vdst[i] = p;
+    p += d.kall;
+  }
+  x.h2d(ctx.int_buf, g->sgp, d.lall * sizeof(int));
+  ctx.G.sgp = ctx.int_buf;
+  ctx.geo_ready = true;
+  return x.check("upload_geometry");
+}
+
+template <class X>
+int upload_halo(X& x, Context& ctx, const int32_t* dst, const int32_t* src, long n) {
+  const long ncol = ctx.d.nreg + 12;
+  for (long h = 0; h < n; ++h) {
+    if (dst[h] < 0 || dst[h] >= ncol || src[h] < 0 || src[h] >= ncol) return NH_ERR_ARG;
+  }
+  if (ctx.halo_dst) { x.release(ctx.halo_dst); ctx.halo_dst = nullptr; }
+  if (ctx.halo_src) { x.release(ctx.halo_src); ctx.halo_src = nullptr; }
+  ctx.halo_n = n;
+  if (n == 0) return 0;
+  ctx.halo_dst = (int*)x.alloc(n * sizeof(int));
+  ctx.halo_src = (int*)x.alloc(n * sizeof(int));
+  if (!ctx.halo_dst || !ctx.halo_src) return NH_ERR_ALLOC;
+  x.h2d(ctx.halo_dst, dst, n * sizeof(int));
+  x.h2d(ctx.halo_src, src, n * sizeof(int));
+  return x.check("upload_halo");
+}
+
+// in-place halo fill of up to 16 fields
+template <class X>
+void halo(X& x, const Context& ctx, double* const* f, int nf) {
+  if (ctx.halo_n == 0) return;
+  for (int q0 = 0; q0 < nf; q0 += 16) {
+    HaloGather h;
+    h.d = ctx.d;
+    h.dst = ctx.halo_dst;
+    h.src = ctx.halo_src;
+    h.n = ctx.halo_n;
+    h.nf = (nf - q0 < 16) ? nf - q0 : 16;
+    for (int q = 0; q < h.nf; ++q) h.f[q] = f[q0 + q];
+    x.run(h, ctx.halo_n * ctx.d.kall);
+  }
+}
+
+template <class X>
+void halo_block(X& x, const Context& ctx, double* base, int nf) {
+  double* f[16];
+  for (int q0 = 0; q0 < nf; q0 += 16) {
+    int m = (nf - q0 < 16) ? nf - q0 : 16;
+    for (int q = 0; q < m; ++q) f[q] = base + (long)(q0 + q) * ctx.d.NF;
+    halo(x, ctx, f, m);
+  }
+}
+
+inline long ncols(const Dims& d) { return d.nreg + 12; }
+
+// ----------------------------------------------------------------------------
+// 3-D divergence damping of (vx, vy, vz, w) into out (3 fields)
+// ----------------------------------------------------------------------------
+template <class X>
+void divdamp(X& x, const Context& ctx, const double* vx, const double* vy, const double* vz, const double* w,
+             double* out0, double* out1, double* out2) {
+  DivDamp1 a;
+  a.d = ctx.d; a.c = ctx.c; a.G = ctx.G;
+  a.rvx = vx; a.rvy = vy; a.rvz = vz; a.rw = w;
+  for (int q = 0; q < 3; ++q) a.out[q] = ctx.field(W_DDT + q);
+  x.run(a, ncols(ctx.d));
+  double* tmp[3] = {ctx.field(W_DDT), ctx.field(W_DDT + 1), ctx.field(W_DDT + 2)};
+  halo(x, ctx, tmp, 3);
+  DivDamp2 b;
+  b.d = ctx.d; b.c = ctx.c; b.G = ctx.G;
+  for (int q = 0; q < 3; ++q) b.in[q] = tmp[q];
+  b.out[0] = out0; b.out[1] = out1; b.out[2] = out2;
+  x.run(b, ncols(ctx.d));
+}
+
+// ----------------------------------------------------------------------------
+// one large step (mod_dynamics.py:1725-2741 with the RK3/HEVI schedule)
+// ----------------------------------------------------------------------------
+template <class X>
+int dyn_step(X& x, const Context& ctx, double* prog, double* progq, int nq, double dtl, int sstep_max) {
+  const Dims& d = ctx.d;
+  const long NF = d.NF;
+  const long nc = ncols(d);
+  const double dts = dtl / (double)sstep_max;
+  const int nsub[3] = {sstep_max / 3, sstep_max / 2, sstep_max};
+  if (nq < 1 || !prog || !progq) return NH_ERR_ARG;   // tracer 0 is qv (zero in a dry run)
+  const double* rhogq = progq;
+  double* prog0 = ctx.field(W_PROG0);
+  double* diag = ctx.field(W_DIAG);
+  double* eth = ctx.field(W_ETH);
+  double* gadv = ctx.field(W_GADV);
+  double* gT = ctx.field(W_GT);
+  double* pm = ctx.field(W_PM);
+  double* ethtot = ctx.field(W_ETHTOT);
+  double* getot = ctx.field(W_GETOT);
+  double* frhog = ctx.field(W_FRHOG);
+  x.d2d(prog0, prog, 6 * NF * sizeof(double));
+  for (int nl = 0; nl < 3; ++nl) {
+    // diagnostics and boundary values
+    {
+      PrePost f{d, ctx.c, ctx.G, prog, rhogq, diag, eth};
+      x.run(f, nc);
+    }
+    {
+      AdvMom f{d, ctx.c, ctx.G, prog, diag, gadv};
+      x.run(f, nc);
+    }
+    // hyperdiffusion: two passes with a halo fill after each
+    {
+      HDiffPass f;
+      f.d = d; f.c = ctx.c; f.G = ctx.G;
+      f.in[0] = diag + I_VX * NF; f.in[1] = diag + I_VY * NF; f.in[2] = diag + I_VZ * NF;
+      f.in[3] = diag + I_W * NF; f.in[4] = diag + I_TEM * NF; f.in[5] = prog;
+      for (int q = 0; q < 6; ++q) f.out[q] = ctx.field(W_HDA + q);
+      f.rhog = prog;
+      f.last = 0;
+      x.run(f, nc);
+      halo_block(x, ctx, ctx.field(W_HDA), 6);
+      for (int q = 0; q < 6; ++q) {
+        f.in[q] = ctx.field(W_HDA + q);
+        f.out[q] = ctx.field(W_HDB + q);
+      }
+      f.last = 1;
+      x.run(f, nc);
+      halo_block(x, ctx, ctx.field(W_HDB), 6);
+    }
+    // divergence damping of the large-step momentum
+    divdamp(x, ctx, prog + 1 * NF, prog + 2 * NF, prog + 3 * NF, prog + 4 * NF,
+            ctx.field(W_DD), ctx.field(W_DD + 1), ctx.field(W_DD + 2));
+    {
+      ViPath0 f;
+      f.d = d; f.c = ctx.c; f.G = ctx.G;
+      f.dt = dts;
+      f.prog = prog; f.diag = diag; f.eth = eth; f.gadv = gadv;
+      for (int q = 0; q < 6; ++q) f.vt[q] = ctx.field(W_HDB + q);
+      for (int q = 0; q < 3; ++q) f.dd[q] = ctx.field(W_DD + q);
+      f.gT = gT;
+      f.Mc = ctx.field(W_MC); f.Mu = ctx.field(W_MU); f.Ml = ctx.field(W_ML);
+      f.getot = getot; f.frhog = frhog;
+      x.run(f, nc);
+    }
+    {
+      EthTotPM f{d, ctx.c, ctx.G, prog, eth, ethtot, pm};
+      x.run(f, nc);
+    }
+    // split state: ps[0..5], then path1 momentum dv[0..2] and vi_main output we[0..2]
+    double* ps[6];
+    double* dv[3];
+    double* we[3];
+    for (int q = 0; q < 6; ++q) ps[q] = ctx.field(W_SPLIT + q);
+    for (int q = 0; q < 3; ++q) dv[q] = ctx.field(W_SPLIT + 6 + q);
+    for (int q = 0; q < 3; ++q) we[q] = ctx.field(W_SPLIT + 9 + q);
+    {
+      SplitInit f{NF, prog, prog0, {ps[0], ps[1], ps[2], ps[3], ps[4], ps[5]}, nl == 0 ? 1 : 0};
+      x.run(f, NF);
+    }
+    const double rweight = 1.0 / (double)nsub[nl];
+    for (int it = 0; it < nsub[nl]; ++it) {
+      double* ddv = ctx.field(W_DDV);
+      divdamp(x, ctx, ps[1], ps[2], ps[3], ps[4], ddv, ddv + NF, ddv + 2 * NF);
+      {
+        Path1 f;
+        f.d = d; f.c = ctx.c; f.G = ctx.G;
+        f.dt = dts; f.rweight = rweight;
+        f.prog = prog;
+        for (int q = 0; q < 6; ++q) f.ps[q] = ps[q];
+        f.gT = gT;
+        for (int q = 0; q < 3; ++q) f.dd[q] = ddv + q * NF;
+        for (int q = 0; q < 3; ++q) f.dv[q] = dv[q];
+        f.pm = pm;
+        x.run(f, nc);
+      }
+      halo(x, ctx, dv, 3);
+      {
+        ViMain f;
+        f.d = d; f.c = ctx.c; f.G = ctx.G;
+        f.dt = dts; f.rweight = rweight;
+        f.prog = prog;
+        for (int q = 0; q < 6; ++q) f.ps[q] = ps[q];
+        for (int q = 0; q < 3; ++q) f.dv[q] = dv[q];
+        f.eth = eth; f.ethtot = ethtot; f.gT = gT; f.getot = getot;
+        f.Mc = ctx.field(W_MC); f.Mu = ctx.field(W_MU); f.Ml = ctx.field(W_ML);
+        for (int q = 0; q < 3; ++q) f.we[q] = we[q];
+        f.pm = pm;
+        x.run(f, nc);
+      }
+      halo(x, ctx, we, 3);
+      // rotate buffers: the new split state is (we0, dv0, dv1, dv2, we1, we2)
+      double* old[6] = {ps[0], ps[1], ps[2], ps[3], ps[4], ps[5]};
+      ps[0] = we[0]; ps[1] = dv[0]; ps[2] = dv[1]; ps[3] = dv[2]; ps[4] = we[1]; ps[5] = we[2];
+      we[0] = old[0]; dv[0] = old[1]; dv[1] = old[2]; dv[2] = old[3]; we[1] = old[4]; we[2] = old[5];
+    }
+    {
+      ProgUpdate f;
+      f.d = d; f.c = ctx.c; f.G = ctx.G;
+      f.prog = prog;
+      for (int q = 0; q < 6; ++q) f.ps[q] = ps[q];
+      x.run(f, nc);
+    }
+    halo_block(x, ctx, pm, 5);
+    if (nl < 2) halo_block(x, ctx, prog, 6);
+    int rc = x.check("dyn_step stage");
+    if (rc) return rc;
+  }
+  // tracer transport of the last stage is not applied here: the dry state
+  // carries qv = 0, for which the flux-form update is the identity
+  halo_block(x, ctx, prog, 6);
+  halo_block(x, ctx, progq, nq);
+  return x.check("dyn_step");
+}
+
+}  // namespace nh

@@ -208,8 +208,10 @@ struct AdvMom {
       out[3 * NF + a] = (k > kmin) ? (G.c2w0[k] * gwc + G.c2w1[k] * gwc_prev) : 0.0;
       gwc_prev = gwc;
     }
-    for (int k : {kmin - 1, kmax + 1})
-      for (int q = 0; q < 4; ++q) out[q * NF + cl.base + k * ks] = 0.0;
+    for (int q = 0; q < 4; ++q) {
+      out[q * NF + cl.base + (kmin - 1) * ks] = 0.0;
+      out[q * NF + cl.base + (kmax + 1) * ks] = 0.0;
+    }
   }
 };
 
@@ -601,11 +603,6 @@ NH_HD inline double hdiv(const Geo& G, const Dims& d, const Cell& cl, FX fx, FY 
   return div7(G, d.nreg, cl.ci, fx, fy, fz);
 }
 
-// kinetic energy density of one column at level k (kernels/rhogkin.py:38-79)
-struct KinCol {
-  const double *rg, *vx, *vy, *vz, *w;   // callables replaced by pointer + addend arrays
-};
-
 // ----------------------------------------------------------------------------
 // vi path0: large-step source terms of the small steps + tridiagonal matrix
 // (mod_vi.py:132-881, kernels/vimatrix.py:67-136)
@@ -817,12 +814,8 @@ struct Path1 {
     const long NF = d.NF, ks = cl.ks;
     const int kmin = d.kmin, kmax = d.kmax;
     const double RovCV = c.Rdry / c.CVdry;
-    // preg of this small step on the fly: PS_e * R/Cv at the gradient points
-    struct PregView {
-      const double* e; double f;
-    };
+    // pressure of this small step from the split energy: PS_e * R/Cv, formed at the gradient points
     const double* pse = ps[I_RHOGE];
-    // the gradient reads neighbour values; build them through a scaled copy
     for (int k = kmin; k <= kmax; ++k) {
       long a = cl.base + k * ks;
       double Pg[3];

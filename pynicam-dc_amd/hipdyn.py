@@ -1,0 +1,175 @@
+"""Device-resident dynamical core behind the nicamhip C-ABI.
+
+``HipContext`` owns one library context (geometry, halo map, step workspace)
+on one device; ``HipDynamics`` owns the prognostic state as torch tensors in
+the device layout and advances it with ``dynamics_step()``, the counterpart of
+``Dyn.dynamics_step(msc)`` (nhm/dynamics/mod_dynamics.py:1725-2741).
+Operators (``OPRT_laplacian``, ``OPRT_gradient``, divergence, halo fill) are
+exposed with reference-style argument order for tests and callers.
+
+torch provides device memory and the stream only; every computation is a
+nicamhip kernel.  Host (numpy) arrays are accepted at the boundary in the
+reference layout and converted once.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import layout, native
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class HipContext:
+    """Library context for one grid on one device.
+
+    ``device`` is a torch device string.  The default is the GPU; tests of the
+    kernel source on the host pass ``device="cpu"`` together with the host
+    test build of the library (``lib_path``).
+    """
+
+    def __init__(self, geo, const, device: str = "cuda:0", lib_path: str | None = None):
+        torch = _torch()
+        self.geo = geo
+        self.const = const
+        self.topo = t = geo.topo
+        self.device = torch.device(device)
+        if self.device.type == "cuda" and not torch.cuda.is_available():
+            raise RuntimeError("backend='hip' needs a GPU (torch.cuda.is_available() is False)")
+        self.lib = native.load(lib_path)
+        lib = self.lib
+        self.dims = native.NhDims(t.gall_1d, t.kall, t.kmin, t.kmax, t.lall)
+        self.NF = int(lib.nh_field_size(C.byref(self.dims)))
+        if self.NF != layout.field_size(t.gall_1d, t.kall, t.lall):
+            raise RuntimeError("field size mismatch between library and host layout")
+        c = const
+        self.consts = native.NhConsts(c.RADIUS, c.OHM, c.GRAV, c.Rdry, c.CVdry, c.Rvap, float(c.CVW[c.I_QV]),
+                                      c.ALPHA, c.GAMMA_H, c.HDIFF_FACT_RHO, c.ALPHA_D)
+        if c.LAP_ORDER_HDIFF != 2 or c.LAP_ORDER_DIVDAMP != 2:
+            raise NotImplementedError("backend='hip' implements lap_order 2 for hyperdiffusion and divergence damping")
+        dev_index = self.device.index if self.device.type == "cuda" and self.device.index is not None else 0
+        if self.device.type == "cuda":
+            torch.cuda.set_device(dev_index)
+        h = C.c_void_p()
+        native.check(lib, lib.nh_create(C.byref(self.dims), C.byref(self.consts), dev_index, C.byref(h)),
+                     "nh_create")
+        self.handle = h
+        # geometry (host arrays kept alive until the synchronous upload returns)
+        arrs = layout.geometry_arrays(geo)
+        g = native.NhGeometry()
+        dp = C.POINTER(C.c_double)
+        for name, _ in native.NhGeometry._fields_:
+            a = arrs[name]
+            ptr = a.ctypes.data_as(C.POINTER(C.c_int32) if name == "sgp" else dp)
+            setattr(g, name, ptr)
+        native.check(lib, lib.nh_upload_geometry(self.handle, C.byref(g)), "nh_upload_geometry")
+        dst = np.ascontiguousarray(t.halo_dst, dtype=np.int32)
+        src = np.ascontiguousarray(t.halo_src, dtype=np.int32)
+        ip = C.POINTER(C.c_int32)
+        native.check(lib, lib.nh_upload_halo_map(self.handle, dst.ctypes.data_as(ip), src.ctypes.data_as(ip),
+                                                 dst.size), "nh_upload_halo_map")
+
+    # ------------------------------------------------------------------
+    def stream(self):
+        if self.device.type == "cuda":
+            return C.c_void_p(_torch().cuda.current_stream(self.device).cuda_stream)
+        return C.c_void_p(None)
+
+    def empty(self, nv: int):
+        return _torch().empty((nv, self.NF), dtype=_torch().float64, device=self.device)
+
+    def to_device(self, var: np.ndarray, var_pl: np.ndarray):
+        """Reference layout (g,g,kall,l,nv)+(6,kall,2,nv) -> device tensor (nv, NF)."""
+        return _torch().from_numpy(layout.pack(var, var_pl)).to(self.device)
+
+    def to_host(self, flat):
+        t = self.topo
+        return layout.unpack(flat.detach().cpu().numpy(), t.gall_1d, t.kall, t.lall)
+
+    def sync(self):
+        if self.device.type == "cuda":
+            _torch().cuda.synchronize(self.device)
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self.lib.nh_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------
+    # operators on device tensors (device layout)
+    def COMM_data_transfer(self, fields):
+        """In-place halo/pole fill of a (nv, NF) tensor (share/mod_comm.py:1066)."""
+        self._check_tensor(fields)
+        nv = fields.shape[0]
+        ptrs = (C.c_void_p * nv)(*[fields[q].data_ptr() for q in range(nv)])
+        native.check(self.lib, self.lib.nh_halo_fill(self.handle, ptrs, nv, self.stream()), "nh_halo_fill")
+
+    def OPRT_laplacian(self, scl, out=None):
+        """7-point Laplacian (share/mod_oprt.py:1429)."""
+        self._check_tensor(scl)
+        out = _torch().zeros_like(scl) if out is None else out
+        native.check(self.lib, self.lib.nh_laplacian(self.handle, C.c_void_p(scl.data_ptr()),
+                                                     C.c_void_p(out.data_ptr()), self.stream()), "nh_laplacian")
+        return out
+
+    def divergence(self, vx, vy, vz, out=None):
+        """7-point divergence (kernels/fluxconv.py:238)."""
+        for a in (vx, vy, vz):
+            self._check_tensor(a)
+        out = _torch().zeros_like(vx) if out is None else out
+        native.check(self.lib, self.lib.nh_divergence(self.handle, *(C.c_void_p(a.data_ptr()) for a in
+                                                                     (vx, vy, vz, out)), self.stream()),
+                     "nh_divergence")
+        return out
+
+    def OPRT_gradient(self, scl, out=None):
+        """7-point gradient (share/mod_oprt.py:1387); out is (3, NF)."""
+        self._check_tensor(scl)
+        out = _torch().zeros((3, self.NF), dtype=scl.dtype, device=scl.device) if out is None else out
+        native.check(self.lib, self.lib.nh_gradient(self.handle, C.c_void_p(scl.data_ptr()),
+                                                    *(C.c_void_p(out[q].data_ptr()) for q in range(3)),
+                                                    self.stream()), "nh_gradient")
+        return out
+
+    def _check_tensor(self, a):
+        torch = _torch()
+        if a.dtype != torch.float64 or a.device != self.device or a.shape[-1] != self.NF or not a.is_contiguous():
+            raise ValueError(f"expected a contiguous float64 tensor (..., {self.NF}) on {self.device}")
+
+
+class HipDynamics:
+    """Prognostic state on the device and the large step (mod_dynamics.py:1725)."""
+
+    def __init__(self, ctx: HipContext, state: dict):
+        self.ctx = ctx
+        c = ctx.const
+        self.prog = ctx.to_device(state["PROG"], state["PROG_pl"]).contiguous()
+        self.progq = ctx.to_device(state["PROGq"], state["PROGq_pl"]).contiguous()
+        if self.prog.shape[0] != 6 or self.progq.shape[0] < 1:
+            raise ValueError("PROG needs 6 variables and PROGq at least qv")
+        self.dtl = float(c.DTL)
+        self.sstep_max = int(c.SSTEP_MAX)
+
+    def dynamics_step(self):
+        lib = self.ctx.lib
+        native.check(lib, lib.nh_dyn_step(self.ctx.handle, C.c_void_p(self.prog.data_ptr()),
+                                          C.c_void_p(self.progq.data_ptr()), self.progq.shape[0],
+                                          self.dtl, self.sstep_max, self.ctx.stream()), "nh_dyn_step")
+
+    def state(self) -> dict:
+        """Copy the state back to the host in the reference layout."""
+        self.ctx.sync()
+        P, Pp = self.ctx.to_host(self.prog)
+        Q, Qp = self.ctx.to_host(self.progq)
+        return {"PROG": P, "PROG_pl": Pp, "PROGq": Q, "PROGq_pl": Qp}

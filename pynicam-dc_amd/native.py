@@ -1,0 +1,83 @@
+"""ctypes binding of the nicamhip C-ABI (include/nicamhip.h).
+
+The library is built in-tree next to this file (``libnicamhip.so``, gfx950).
+Loading fails loudly when it is missing: there is no CPU fallback.  The ABI
+style follows the reference's native precedent (share/mod_ncclffi.py:52-79):
+``CDLL`` + argtypes/restype, int return codes turned into RuntimeError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+LIB_NAME = "libnicamhip.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+NH_ERR = {-1: "NH_ERR_ARG", -2: "NH_ERR_STATE", -3: "NH_ERR_ALLOC", -4: "NH_ERR_DEVICE"}
+
+
+class NhDims(C.Structure):
+    _fields_ = [("gall_1d", C.c_int32), ("kall", C.c_int32), ("kmin", C.c_int32),
+                ("kmax", C.c_int32), ("lall", C.c_int32)]
+
+
+class NhConsts(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "radius", "ohm", "grav", "rdry", "cvdry", "rvap", "cvw_qv", "alpha",
+        "gamma_h", "hdiff_fact_rho", "alpha_d")]
+
+
+_GEO_FIELDS = [
+    "coef_div", "coef_grad", "coef_lap", "coef_intp", "coef_diff", "grd_x",
+    "coef_div_pl", "coef_grad_pl", "coef_lap_pl", "coef_intp_pl", "coef_diff_pl", "grd_x_pl",
+]
+_VERT_FIELDS = ["afact", "bfact", "cfact", "dfact", "rdgz", "rdgzh", "c2w0", "c2w1", "w2c0", "w2c1", "phi"]
+
+
+class NhGeometry(C.Structure):
+    _fields_ = ([(n, C.POINTER(C.c_double)) for n in _GEO_FIELDS]
+                + [("sgp", C.POINTER(C.c_int32))]
+                + [(n, C.POINTER(C.c_double)) for n in _VERT_FIELDS])
+
+
+EXPORTS = {
+    "nh_last_error": (C.c_char_p, []),
+    "nh_field_size": (C.c_int64, [C.POINTER(NhDims)]),
+    "nh_create": (C.c_int, [C.POINTER(NhDims), C.POINTER(NhConsts), C.c_int, C.POINTER(C.c_void_p)]),
+    "nh_destroy": (C.c_int, [C.c_void_p]),
+    "nh_upload_geometry": (C.c_int, [C.c_void_p, C.POINTER(NhGeometry)]),
+    "nh_upload_halo_map": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int64]),
+    "nh_halo_fill": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int32, C.c_void_p]),
+    "nh_laplacian": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "nh_divergence": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4 + [C.c_void_p]),
+    "nh_gradient": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4 + [C.c_void_p]),
+    "nh_dyn_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_int32,
+                              C.c_void_p]),
+}
+
+_cache: dict = {}
+
+
+def load(path: str | None = None):
+    """Load the C-ABI library (default: the in-tree gfx950 build) and declare its signatures."""
+    path = os.path.abspath(path or LIB_PATH)
+    if path in _cache:
+        return _cache[path]
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"nicamhip library not found at {path}; build it first "
+            "(python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(path)
+    for name, (res, args) in EXPORTS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _cache[path] = lib
+    return lib
+
+
+def check(lib, rc: int, what: str = ""):
+    if rc != 0:
+        msg = lib.nh_last_error()
+        msg = msg.decode() if msg else ""
+        raise RuntimeError(f"{what or 'nicamhip'} failed ({NH_ERR.get(rc, rc)}): {msg}")
