@@ -136,6 +136,16 @@ int nh_gradient(nh_context* ctx, const double* scl, double* gx, double* gy, doub
 int nh_dyn_step(nh_context* ctx, double* prog, double* progq, int32_t nq, double dtl, int32_t sstep_max,
                 void* stream);
 
+/* Kernel timers (the per-kernel counterpart of the section timers
+ * PROF_rapstart / PROF_rapend / PROF_rapreport, share/mod_prof.py:81-140).
+ * While enabled, every launch of a context is bracketed by two events on its
+ * stream.  nh_prof_report waits for the recorded launches, writes up to
+ * max_kernels entries (names: 32 bytes each, NUL-terminated; summed duration
+ * in ms; launch count), sets *n_kernels and clears the record. */
+int nh_prof_enable(nh_context* ctx, int32_t on);
+int nh_prof_report(nh_context* ctx, int32_t max_kernels, char* names, double* total_ms, int64_t* launches,
+                   int32_t* n_kernels);
+
 #ifdef __cplusplus
 }
 #endif

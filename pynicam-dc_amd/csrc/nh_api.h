@@ -1,6 +1,8 @@
 // extern "C" entry points of include/nicamhip.h, generic over the executor
 // type NHX defined by the including translation unit:
-//   NHX(int device, void* stream)  sets the device and binds the stream.
+//   NHX(int device, void* stream, Context* ctx)  sets the device, binds the
+//   stream and the context's kernel timers;
+//   prof_enable(bool), prof_report(...), prof_release() for the timers.
 #pragma once
 
 #include <cstdio>
@@ -25,6 +27,7 @@ inline int fail(int code, const char* where, const std::string& msg) {
 
 // single-operator functors (interior points and pole centres, all levels)
 struct OpLap {
+  static constexpr const char* kName = "oprt_laplacian";
   Dims d; Geo G;
   const double* s;
   double* o;
@@ -39,6 +42,7 @@ struct OpLap {
 };
 
 struct OpDiv {
+  static constexpr const char* kName = "oprt_divergence";
   Dims d; Geo G;
   const double *vx, *vy, *vz;
   double* o;
@@ -56,6 +60,7 @@ struct OpDiv {
 };
 
 struct OpGrad {
+  static constexpr const char* kName = "oprt_gradient";
   Dims d; Geo G;
   const double* s;
   double *ox, *oy, *oz;
@@ -106,7 +111,7 @@ int nh_create(const nh_dims* dims, const nh_consts* consts, int device, nh_conte
   nh::set_dims(ctx->c.d, dims);
   nh::set_consts(ctx->c.c, consts);
   ctx->c.device = device;
-  NHX x(device, nullptr);
+  NHX x(device, nullptr, nullptr);
   if (x.check(where)) { delete ctx; return nh::fail(NH_ERR_DEVICE, where, x.message()); }
   size_t bytes = (size_t)nh::W_NFIELDS * ctx->c.d.NF * sizeof(double);
   ctx->c.ws = (double*)x.alloc(bytes);
@@ -126,13 +131,14 @@ int nh_create(const nh_dims* dims, const nh_consts* consts, int device, nh_conte
 
 int nh_destroy(nh_context* ctx) {
   if (!ctx) return 0;
-  NHX x(ctx->c.device, nullptr);
+  NHX x(ctx->c.device, nullptr, &ctx->c);
   x.sync();
   if (ctx->c.ws) x.release(ctx->c.ws);
   if (ctx->c.geo_buf) x.release(ctx->c.geo_buf);
   if (ctx->c.int_buf) x.release(ctx->c.int_buf);
   if (ctx->c.halo_dst) x.release(ctx->c.halo_dst);
   if (ctx->c.halo_src) x.release(ctx->c.halo_src);
+  x.prof_release();
   delete ctx;
   return 0;
 }
@@ -140,7 +146,7 @@ int nh_destroy(nh_context* ctx) {
 int nh_upload_geometry(nh_context* ctx, const nh_geometry* geo) {
   const char* where = "nh_upload_geometry";
   if (!ctx || !geo) return nh::fail(NH_ERR_ARG, where, "null argument");
-  NHX x(ctx->c.device, nullptr);
+  NHX x(ctx->c.device, nullptr, &ctx->c);
   int rc = nh::upload_geometry(x, ctx->c, geo);
   if (rc == NH_ERR_ARG) return nh::fail(rc, where, "null coefficient pointer");
   if (rc == NH_ERR_ALLOC) return nh::fail(rc, where, "device allocation");
@@ -151,7 +157,7 @@ int nh_upload_geometry(nh_context* ctx, const nh_geometry* geo) {
 int nh_upload_halo_map(nh_context* ctx, const int32_t* dst, const int32_t* src, int64_t n) {
   const char* where = "nh_upload_halo_map";
   if (!ctx || n < 0 || (n > 0 && (!dst || !src))) return nh::fail(NH_ERR_ARG, where, "invalid argument");
-  NHX x(ctx->c.device, nullptr);
+  NHX x(ctx->c.device, nullptr, &ctx->c);
   int rc = nh::upload_halo(x, ctx->c, dst, src, (long)n);
   if (rc == NH_ERR_ARG) return nh::fail(rc, where, "column id out of range");
   if (rc == NH_ERR_ALLOC) return nh::fail(rc, where, "device allocation");
@@ -165,7 +171,7 @@ int nh_halo_fill(nh_context* ctx, double* const* fields, int32_t nfields, void* 
   if (ctx->c.halo_n == 0) return nh::fail(NH_ERR_STATE, where, "halo map not uploaded");
   for (int q = 0; q < nfields; ++q)
     if (!fields[q]) return nh::fail(NH_ERR_ARG, where, "null field pointer");
-  NHX x(ctx->c.device, stream);
+  NHX x(ctx->c.device, stream, &ctx->c);
   for (int q0 = 0; q0 < nfields; q0 += 16) {
     int m = (nfields - q0 < 16) ? nfields - q0 : 16;
     nh::halo(x, ctx->c, fields + q0, m);
@@ -178,7 +184,7 @@ int nh_laplacian(nh_context* ctx, const double* scl, double* out, void* stream) 
   const char* where = "nh_laplacian";
   if (int rc = nh::check_ready(ctx, where, false)) return rc;
   if (!scl || !out) return nh::fail(NH_ERR_ARG, where, "null field pointer");
-  NHX x(ctx->c.device, stream);
+  NHX x(ctx->c.device, stream, &ctx->c);
   nh::OpLap f{ctx->c.d, ctx->c.G, scl, out};
   x.run(f, nh::ncols(ctx->c.d));
   if (x.check(where)) return nh::fail(NH_ERR_DEVICE, where, x.message());
@@ -190,7 +196,7 @@ int nh_divergence(nh_context* ctx, const double* vx, const double* vy, const dou
   const char* where = "nh_divergence";
   if (int rc = nh::check_ready(ctx, where, false)) return rc;
   if (!vx || !vy || !vz || !out) return nh::fail(NH_ERR_ARG, where, "null field pointer");
-  NHX x(ctx->c.device, stream);
+  NHX x(ctx->c.device, stream, &ctx->c);
   nh::OpDiv f{ctx->c.d, ctx->c.G, vx, vy, vz, out};
   x.run(f, nh::ncols(ctx->c.d));
   if (x.check(where)) return nh::fail(NH_ERR_DEVICE, where, x.message());
@@ -201,7 +207,7 @@ int nh_gradient(nh_context* ctx, const double* scl, double* gx, double* gy, doub
   const char* where = "nh_gradient";
   if (int rc = nh::check_ready(ctx, where, false)) return rc;
   if (!scl || !gx || !gy || !gz) return nh::fail(NH_ERR_ARG, where, "null field pointer");
-  NHX x(ctx->c.device, stream);
+  NHX x(ctx->c.device, stream, &ctx->c);
   nh::OpGrad f{ctx->c.d, ctx->c.G, scl, gx, gy, gz};
   x.run(f, nh::ncols(ctx->c.d));
   if (x.check(where)) return nh::fail(NH_ERR_DEVICE, where, x.message());
@@ -215,10 +221,29 @@ int nh_dyn_step(nh_context* ctx, double* prog, double* progq, int32_t nq, double
   if (!prog || !progq || nq < 1) return nh::fail(NH_ERR_ARG, where, "need prog and at least one tracer field");
   if (sstep_max < 3 || sstep_max % 6 != 0 || !(dtl > 0.0))
     return nh::fail(NH_ERR_ARG, where, "sstep_max must be a positive multiple of 6 and dtl > 0");
-  NHX x(ctx->c.device, stream);
+  NHX x(ctx->c.device, stream, &ctx->c);
   int rc = nh::dyn_step(x, ctx->c, prog, progq, nq, dtl, sstep_max);
   if (rc == NH_ERR_ARG) return nh::fail(rc, where, "invalid argument");
   if (rc) return nh::fail(NH_ERR_DEVICE, where, x.message());
+  return 0;
+}
+
+int nh_prof_enable(nh_context* ctx, int32_t on) {
+  if (!ctx) return nh::fail(NH_ERR_ARG, "nh_prof_enable", "null context");
+  NHX x(ctx->c.device, nullptr, &ctx->c);
+  if (x.prof_enable(on != 0)) return nh::fail(NH_ERR_DEVICE, "nh_prof_enable", x.message());
+  return 0;
+}
+
+int nh_prof_report(nh_context* ctx, int32_t max_kernels, char* names, double* total_ms, int64_t* launches,
+                   int32_t* n_kernels) {
+  const char* where = "nh_prof_report";
+  if (!ctx || max_kernels < 0 || !n_kernels || (max_kernels > 0 && (!names || !total_ms || !launches)))
+    return nh::fail(NH_ERR_ARG, where, "invalid argument");
+  NHX x(ctx->c.device, nullptr, &ctx->c);
+  int n = x.prof_report(max_kernels, names, total_ms, launches);
+  if (n < 0) return nh::fail(NH_ERR_DEVICE, where, x.message());
+  *n_kernels = n;
   return 0;
 }
 

@@ -143,6 +143,7 @@ NH_HD inline double div_pl(const Geo& G, int lp, FX fx, FY fy, FZ fz) {
 // halo gather
 // ----------------------------------------------------------------------------
 struct HaloGather {
+  static constexpr const char* kName = "halo_gather";
   Dims d;
   const int* dst;   // column ids
   const int* src;

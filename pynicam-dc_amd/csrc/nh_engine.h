@@ -1,6 +1,7 @@
 // Context, workspace and launch sequence of the dynamical-core step.
 //
 // Generic over an executor X that provides
+//   X(int device, void* stream, Context* ctx);
 //   void* alloc(size_t bytes);  void release(void* p);
 //   void h2d(void* dst, const void* src, size_t bytes);   (synchronous)
 //   void d2d(void* dst, const void* src, size_t bytes);   (stream ordered)
@@ -53,6 +54,7 @@ struct Context {
   double* ws = nullptr;
   bool geo_ready = false;
   int device = 0;
+  void* prof = nullptr;   // executor-specific kernel timer state
   double* field(int f) const { return ws + (long)f * d.NF; }
 };
 
@@ -246,21 +248,22 @@ int dyn_step(X& x, const Context& ctx, double* prog, double* progq, int nq, doub
     }
     // hyperdiffusion: two passes with a halo fill after each
     {
-      HDiffPass f;
+      HDiffPass<0> f;
       f.d = d; f.c = ctx.c; f.G = ctx.G;
       f.in[0] = diag + I_VX * NF; f.in[1] = diag + I_VY * NF; f.in[2] = diag + I_VZ * NF;
       f.in[3] = diag + I_W * NF; f.in[4] = diag + I_TEM * NF; f.in[5] = prog;
       for (int q = 0; q < 6; ++q) f.out[q] = ctx.field(W_HDA + q);
       f.rhog = prog;
-      f.last = 0;
       x.run(f, nc);
       halo_block(x, ctx, ctx.field(W_HDA), 6);
+      HDiffPass<1> f2;
+      f2.d = d; f2.c = ctx.c; f2.G = ctx.G;
       for (int q = 0; q < 6; ++q) {
-        f.in[q] = ctx.field(W_HDA + q);
-        f.out[q] = ctx.field(W_HDB + q);
+        f2.in[q] = ctx.field(W_HDA + q);
+        f2.out[q] = ctx.field(W_HDB + q);
       }
-      f.last = 1;
-      x.run(f, nc);
+      f2.rhog = prog;
+      x.run(f2, nc);
       halo_block(x, ctx, ctx.field(W_HDB), 6);
     }
     // divergence damping of the large-step momentum

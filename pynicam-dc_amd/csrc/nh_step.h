@@ -15,6 +15,7 @@ namespace nh {
 // (kernels/diag.py:53-143, mod_bndcnd.py:153-238, kernels/thrmdyn.py:56-71)
 // ----------------------------------------------------------------------------
 struct PrePost {
+  static constexpr const char* kName = "prepost";
   Dims d; Consts c; Geo G;
   double* prog;        // 6 fields, in/out (ghost rows written)
   const double* rhogq; // qv * rhog
@@ -105,6 +106,7 @@ struct PrePost {
 // (mod_src.py:96-497, kernels/advconvmom.py:54-194, advconv.py, fluxconv.py)
 // ----------------------------------------------------------------------------
 struct AdvMom {
+  static constexpr const char* kName = "adv_mom";
   Dims d; Consts c; Geo G;
   const double* prog;
   const double* diag;
@@ -320,12 +322,14 @@ NH_HD inline double diffusion_pl(const Geo& G, int lp, const double* s, KH kh, l
 // One hyperdiffusion pass: out_q = -L(in_q) for six fields (mod_numfilter.py:1125-1264).
 // In the last pass components 4 (temperature) and 5 (density) use the
 // Kh-weighted diffusion with Kh*rhog*CVdry and Kh*rhog*hdiff_fact_rho.
+// LAST = 0: first pass (six Laplacians); LAST = 1: second pass.
+template <int LAST>
 struct HDiffPass {
+  static constexpr const char* kName = LAST ? "hdiff_pass2" : "hdiff_pass1";
   Dims d; Consts c; Geo G;
   const double* in[6];
   double* out[6];
   const double* rhog;
-  int last;
   NH_HD void operator()(long t) const {
     Cell cl = cell_of(d, t);
     if (cl.pole && cl.v != 0) return;
@@ -334,7 +338,7 @@ struct HDiffPass {
       long a = cl.base + k * cl.ks;
       for (int q = 0; q < 6; ++q) {
         double r;
-        if (last && q >= 4) {
+        if (LAST && q >= 4) {
           double fac = (q == 4) ? c.CVdry : c.HDIFF_FACT_RHO;
           const double* rg = rhog;
           const double gam = c.GAMMA_H;
@@ -383,6 +387,7 @@ NH_HD inline double hex_sum(const Geo& G, long nreg, long ci, int dd, const doub
 
 // First operator: triangle-mean 3-D divergence, gradient back to cells.
 struct DivDamp1 {
+  static constexpr const char* kName = "divdamp_3d";
   Dims d; Consts c; Geo G;
   const double *rvx, *rvy, *rvz, *rw;
   double* out[3];
@@ -456,6 +461,7 @@ struct DivDamp1 {
 // Second operator on -(first result), scaled by the damping coefficient and
 // projected onto the tangent plane.
 struct DivDamp2 {
+  static constexpr const char* kName = "divdamp_post";
   Dims d; Consts c; Geo G;
   const double* in[3];
   double* out[3];
@@ -608,6 +614,7 @@ NH_HD inline double hdiv(const Geo& G, const Dims& d, const Cell& cl, FX fx, FY 
 // (mod_vi.py:132-881, kernels/vimatrix.py:67-136)
 // ----------------------------------------------------------------------------
 struct ViPath0 {
+  static constexpr const char* kName = "vi_path0";
   Dims d; Consts c; Geo G;
   double dt;
   const double* prog;
@@ -765,6 +772,7 @@ NH_HD inline double rhogkin_k(const Geo& G, int kmin, int kmax, int k, F get) {
 // total specific energy of the large-step state (for the small-step energy
 // flux) and the running mean of the small-step mass fluxes, all columns
 struct EthTotPM {
+  static constexpr const char* kName = "ethtot_mean";
   Dims d; Consts c; Geo G;
   const double* prog;
   const double* eth;
@@ -785,6 +793,7 @@ struct EthTotPM {
 
 // split state at the start of an RK stage: 0 (first stage) or PROG0 - PROG
 struct SplitInit {
+  static constexpr const char* kName = "split_init";
   long NF;
   const double* prog;
   const double* prog0;
@@ -799,6 +808,7 @@ struct SplitInit {
 // small step path1: horizontally explicit momentum update (kernels/vipath1.py:68-160)
 // ----------------------------------------------------------------------------
 struct Path1 {
+  static constexpr const char* kName = "vi_path1";
   Dims d; Consts c; Geo G;
   double dt, rweight;
   const double* prog;
@@ -880,6 +890,7 @@ struct Path1 {
 // (mod_vi.py:1925-2432, kernels/vimain.py:127-262, kernels/virhowsolver.py:42-120)
 // ----------------------------------------------------------------------------
 struct ViMain {
+  static constexpr const char* kName = "vi_main";
   Dims d; Consts c; Geo G;
   double dt, rweight;
   const double* prog;
@@ -1021,6 +1032,7 @@ struct ViMain {
 // end of the small steps: PROG += PROG_split and horizontal projection (mod_vi.py:1572-1586)
 // ----------------------------------------------------------------------------
 struct ProgUpdate {
+  static constexpr const char* kName = "prog_update";
   Dims d; Consts c; Geo G;
   double* prog;
   const double* ps[6];

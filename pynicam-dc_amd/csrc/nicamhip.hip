@@ -5,7 +5,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
+#include <cstring>
 #include <string>
+#include <vector>
 
 #include "nh_step.h"
 #include "../../include/nicamhip.h"
@@ -18,15 +20,23 @@ __global__ __launch_bounds__(256) void k_run(F f, long n) {
   if (t < n) f(t);
 }
 
+struct Context;
+
+// Kernel timers: one event pair per launch while enabled, recorded on the
+// launch stream around the kernel; aggregated per kernel name on report.
+struct HipProf {
+  bool on = false;
+  std::vector<hipEvent_t> ev;          // pairs (start, stop)
+  size_t used = 0;                     // pairs in use
+  std::vector<const char*> name;       // per used pair
+};
+
 class HipExec {
  public:
-  HipExec(int device, void* stream) : s_((hipStream_t)stream) {
-    int cur = -1;
-    err_ = hipGetDevice(&cur);
-    if (err_ == hipSuccess && cur != device) err_ = hipSetDevice(device);
-    const char* e = std::getenv("NH_SYNC");
-    sync_each_ = (e && e[0] == '1');
-  }
+  HipExec(int device, void* stream, Context* ctx);
+  int prof_enable(bool on);
+  int prof_report(int max_kernels, char* names, double* total_ms, int64_t* launches);
+  void prof_release();
   void* alloc(size_t bytes) {
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, bytes);
@@ -51,8 +61,11 @@ class HipExec {
     if (n <= 0) return;
     const int bs = 256;
     long nb = (n + bs - 1) / bs;
+    hipEvent_t* pair = prof_ && prof_->on ? next_pair(F::kName) : nullptr;
+    if (pair) note(hipEventRecord(pair[0], s_), "hipEventRecord");
     hipLaunchKernelGGL(k_run<F>, dim3((unsigned)nb), dim3(bs), 0, s_, f, n);
     note(hipGetLastError(), "kernel launch");
+    if (pair) note(hipEventRecord(pair[1], s_), "hipEventRecord");
     if (sync_each_) note(hipStreamSynchronize(s_), "kernel execution");
   }
   void sync() { note(hipStreamSynchronize(s_), "hipStreamSynchronize"); }
@@ -71,6 +84,19 @@ class HipExec {
       where_ = where;
     }
   }
+  hipEvent_t* next_pair(const char* name) {
+    HipProf* p = prof_;
+    if (p->used * 2 == p->ev.size()) {
+      hipEvent_t a, b;
+      if (hipEventCreate(&a) != hipSuccess) return nullptr;
+      if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return nullptr; }
+      p->ev.push_back(a);
+      p->ev.push_back(b);
+    }
+    p->name.push_back(name);
+    return &p->ev[2 * p->used++];
+  }
+  HipProf* prof_ = nullptr;
   hipStream_t s_;
   hipError_t err_ = hipSuccess;
   const char* where_ = "";
@@ -82,3 +108,62 @@ class HipExec {
 using NHX = nh::HipExec;
 
 #include "nh_api.h"
+
+namespace nh {
+
+HipExec::HipExec(int device, void* stream, Context* ctx) : s_((hipStream_t)stream) {
+  int cur = -1;
+  err_ = hipGetDevice(&cur);
+  if (err_ == hipSuccess && cur != device) err_ = hipSetDevice(device);
+  const char* e = std::getenv("NH_SYNC");
+  sync_each_ = (e && e[0] == '1');
+  if (ctx) {
+    if (!ctx->prof) ctx->prof = new HipProf();
+    prof_ = static_cast<HipProf*>(ctx->prof);
+  }
+}
+
+int HipExec::prof_enable(bool on) {
+  if (!prof_) return NH_ERR_STATE;
+  prof_->on = on;
+  return 0;
+}
+
+int HipExec::prof_report(int max_kernels, char* names, double* total_ms, int64_t* launches) {
+  if (!prof_) return 0;
+  HipProf* p = prof_;
+  int n = 0;
+  for (size_t u = 0; u < p->used; ++u) {
+    hipEvent_t a = p->ev[2 * u], b = p->ev[2 * u + 1];
+    hipError_t e = hipEventSynchronize(b);
+    if (e != hipSuccess) { note(e, "hipEventSynchronize"); return -1; }
+    float ms = 0.0f;
+    e = hipEventElapsedTime(&ms, a, b);
+    if (e != hipSuccess) { note(e, "hipEventElapsedTime"); return -1; }
+    int slot = -1;
+    for (int q = 0; q < n; ++q)
+      if (std::strncmp(names + 32 * q, p->name[u], 31) == 0) { slot = q; break; }
+    if (slot < 0) {
+      if (n >= max_kernels) continue;
+      slot = n++;
+      std::strncpy(names + 32 * slot, p->name[u], 31);
+      names[32 * slot + 31] = 0;
+      total_ms[slot] = 0.0;
+      launches[slot] = 0;
+    }
+    total_ms[slot] += ms;
+    launches[slot] += 1;
+  }
+  p->used = 0;
+  p->name.clear();
+  return n;
+}
+
+void HipExec::prof_release() {
+  if (!prof_) return;
+  for (hipEvent_t ev : prof_->ev) (void)hipEventDestroy(ev);
+  delete prof_;
+  prof_ = nullptr;
+}
+
+}  // namespace nh

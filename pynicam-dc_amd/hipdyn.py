@@ -95,6 +95,25 @@ class HipContext:
         if self.device.type == "cuda":
             _torch().cuda.synchronize(self.device)
 
+    def prof_enable(self, on: bool = True):
+        """Start/stop per-launch kernel timers (events on the launch stream)."""
+        native.check(self.lib, self.lib.nh_prof_enable(self.handle, 1 if on else 0), "nh_prof_enable")
+
+    def prof_report(self, max_kernels: int = 64) -> dict:
+        """{kernel name: (total ms, launches)} of the launches timed since the last report."""
+        names = C.create_string_buffer(32 * max_kernels)
+        ms = (C.c_double * max_kernels)()
+        cnt = (C.c_int64 * max_kernels)()
+        n = C.c_int32()
+        native.check(self.lib, self.lib.nh_prof_report(self.handle, max_kernels, names, ms, cnt, C.byref(n)),
+                     "nh_prof_report")
+        raw = names.raw
+        out = {}
+        for q in range(n.value):
+            nm = raw[32 * q:32 * (q + 1)].split(b"\0", 1)[0].decode()
+            out[nm] = (ms[q], int(cnt[q]))
+        return out
+
     def close(self):
         if getattr(self, "handle", None) is not None and self.handle.value:
             self.lib.nh_destroy(self.handle)

@@ -53,6 +53,9 @@ EXPORTS = {
     "nh_gradient": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4 + [C.c_void_p]),
     "nh_dyn_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_int32,
                               C.c_void_p]),
+    "nh_prof_enable": (C.c_int, [C.c_void_p, C.c_int32]),
+    "nh_prof_report": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.POINTER(C.c_double),
+                                 C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
 }
 
 _cache: dict = {}

@@ -5,18 +5,57 @@
 // This library is test infrastructure: the product library is the gfx950
 // build of csrc/nicamhip.hip.
 #define NH_HD
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../pynicam-dc_amd/csrc/nh_step.h"
 #include "../../include/nicamhip.h"
 
 namespace nh {
 
+struct Context;
+
+struct HostProf {
+  bool on = false;
+  std::vector<std::pair<const char*, double>> rec;
+};
+
 class HostExec {
  public:
-  HostExec(int, void*) {}
+  HostExec(int, void*, Context* ctx);
+  int prof_enable(bool on) {
+    if (!prof_) return -2;
+    prof_->on = on;
+    return 0;
+  }
+  int prof_report(int max_kernels, char* names, double* total_ms, int64_t* launches) {
+    if (!prof_) return 0;
+    int n = 0;
+    for (auto& r : prof_->rec) {
+      int slot = -1;
+      for (int q = 0; q < n; ++q)
+        if (std::strncmp(names + 32 * q, r.first, 31) == 0) slot = q;
+      if (slot < 0) {
+        if (n >= max_kernels) continue;
+        slot = n++;
+        std::strncpy(names + 32 * slot, r.first, 31);
+        names[32 * slot + 31] = 0;
+        total_ms[slot] = 0.0;
+        launches[slot] = 0;
+      }
+      total_ms[slot] += r.second;
+      launches[slot] += 1;
+    }
+    prof_->rec.clear();
+    return n;
+  }
+  void prof_release() {
+    delete prof_;
+    prof_ = nullptr;
+  }
   void* alloc(size_t bytes) { return std::aligned_alloc(256, (bytes + 255) / 256 * 256); }
   void release(void* p) { std::free(p); }
   void zero(void* p, size_t bytes) { std::memset(p, 0, bytes); }
@@ -24,12 +63,19 @@ class HostExec {
   void d2d(void* dst, const void* src, size_t bytes) { std::memmove(dst, src, bytes); }
   template <class F>
   void run(const F& f, long n) {
+    auto t0 = std::chrono::steady_clock::now();
 #pragma omp parallel for schedule(static)
     for (long t = 0; t < n; ++t) f(t);
+    if (prof_ && prof_->on)
+      prof_->rec.emplace_back(F::kName,
+                              std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
   }
   void sync() {}
   int check(const char*) { return 0; }
   std::string message() const { return "host"; }
+
+ private:
+  HostProf* prof_ = nullptr;
 };
 
 }  // namespace nh
@@ -37,3 +83,10 @@ class HostExec {
 using NHX = nh::HostExec;
 
 #include "../../pynicam-dc_amd/csrc/nh_api.h"
+
+nh::HostExec::HostExec(int, void*, Context* ctx) {
+  if (ctx) {
+    if (!ctx->prof) ctx->prof = new HostProf();
+    prof_ = static_cast<HostProf*>(ctx->prof);
+  }
+}

@@ -1,0 +1,152 @@
+"""HIP (gfx950) path against the CPU oracle, through the C-ABI.
+
+Operators and halo fills are compared bit for bit; the dynamics step within a
+relative tolerance of 1e-10 of each prognostic's peak on interior points and
+pole centres of the physical levels (the kernels follow the oracle's
+operation order in FP64 without contraction; the remaining differences come
+from the terrain-gradient metric, which is round-off in the oracle and exactly
+zero on the device).  At the largest size only size-independent properties
+are checked (finiteness, global mass conservation).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import interior, mod, pole_centre, rel_err
+
+pytestmark = pytest.mark.gpu
+
+layout = mod("layout")
+hipdyn = mod("hipdyn")
+native = mod("native")
+
+STEP_TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def ctx5(gpu, case_gl05):
+    ctx = hipdyn.HipContext(case_gl05.geo, case_gl05.const, device="cuda:0")
+    yield ctx
+    ctx.close()
+
+
+def _rand_field(rng, t, halo=True):
+    g, kall, L = t.gall_1d, t.kall, t.lall
+    f = rng.standard_normal((g, g, kall, L))
+    fp = rng.standard_normal((6, kall, 2))
+    if halo:
+        t.halo_fill_host(f, fp)
+    return f, fp
+
+
+def test_halo_fill_bit_exact(gpu, ctx5, case_gl05):
+    t = case_gl05.topo
+    rng = np.random.default_rng(11)
+    fields = [_rand_field(rng, t, halo=False) for _ in range(7)]
+    T = gpu.from_numpy(np.stack([layout.pack1(a, b) for a, b in fields])).cuda().contiguous()
+    ctx5.COMM_data_transfer(T)
+    got = T.cpu().numpy()
+    for q, (a, b) in enumerate(fields):
+        t.halo_fill_host(a, b)
+        ga, gb = layout.unpack1(got[q], t.gall_1d, t.kall, t.lall)
+        assert np.array_equal(ga, a) and np.array_equal(gb, b)
+
+
+def test_operators_bit_exact(gpu, ctx5, case_gl05):
+    from oracle import dynamics as od
+    geo = case_gl05.geo
+    t = geo.topo
+    g, kall, L = t.gall_1d, t.kall, t.lall
+    octx = od.OracleContext(geo, case_gl05.const, None)
+    rng = np.random.default_rng(5)
+    f, fp = _rand_field(rng, t)
+    T = gpu.from_numpy(layout.pack1(f, fp)).cuda()
+    s = slice(1, g - 1)
+    lo, lop = od.laplacian(octx, f, fp)
+    a, ap = layout.unpack1(ctx5.OPRT_laplacian(T).cpu().numpy(), g, kall, L)
+    assert np.array_equal(a[s, s], lo[s, s]) and np.array_equal(ap[0], lop[0])
+    go, gop = od.gradient(octx, f, fp)
+    G = ctx5.OPRT_gradient(T).cpu().numpy()
+    for d in range(3):
+        a, ap = layout.unpack1(G[d], g, kall, L)
+        assert np.array_equal(a[s, s], go[s, s, ..., d]) and np.array_equal(ap[0], gop[0, ..., d])
+    V = [_rand_field(rng, t) for _ in range(3)]
+    do, dop = od.divergence(octx, V[0][0], V[1][0], V[2][0], V[0][1], V[1][1], V[2][1])
+    D = ctx5.divergence(*(gpu.from_numpy(layout.pack1(a, b)).cuda() for a, b in V))
+    a, ap = layout.unpack1(D.cpu().numpy(), g, kall, L)
+    assert np.array_equal(a[s, s], do[s, s]) and np.array_equal(ap[0], dop[0])
+
+
+def _compare_steps(case, ctx, nsteps, tol):
+    from oracle import dynamics as od
+    t = case.topo
+    ref = od.OracleDynamics(od.OracleContext(case.geo, case.const, None), case.state)
+    dyn = hipdyn.HipDynamics(ctx, case.state)
+    for n in range(nsteps):
+        print(f"  step {n + 1}/{nsteps}", flush=True)
+        ref.step()
+        dyn.dynamics_step()
+    got = dyn.state()
+    errs = []
+    for v in range(6):
+        a, b = interior(ref.PROG[..., v], t), interior(got["PROG"][..., v], t)
+        ap, bp = pole_centre(ref.PROG_pl[..., v], t), pole_centre(got["PROG_pl"][..., v], t)
+        errs.append(rel_err(a, b, ap, bp))
+    print("  relative errors (rhog, rhogvx, rhogvy, rhogvz, rhogw, rhoge):",
+          " ".join(f"{e:.2e}" for e in errs), flush=True)
+    assert max(errs) < tol, errs
+    assert np.array_equal(got["PROGq"], ref.PROGq)
+
+
+def test_dynamics_step_gl04_three_steps(gpu, case_gl04):
+    ctx = hipdyn.HipContext(case_gl04.geo, case_gl04.const, device="cuda:0")
+    try:
+        _compare_steps(case_gl04, ctx, 3, STEP_TOL)
+    finally:
+        ctx.close()
+
+
+def test_dynamics_step_gl05_z40(gpu, ctx5, case_gl05):
+    _compare_steps(case_gl05, ctx5, 1, STEP_TOL)
+
+
+def test_step_argument_errors(gpu, ctx5):
+    lib = ctx5.lib
+    prog = ctx5.empty(6)
+    rc = lib.nh_dyn_step(ctx5.handle, C.c_void_p(prog.data_ptr()), C.c_void_p(None), 0, 1200.0, 6,
+                         ctx5.stream())
+    assert rc == -1
+    rc = lib.nh_dyn_step(ctx5.handle, C.c_void_p(prog.data_ptr()), C.c_void_p(prog.data_ptr()), 1, 1200.0, 4,
+                         ctx5.stream())
+    assert rc == -1
+
+
+def test_mass_conservation_gl06(gpu):
+    """gl06 z40, two steps: global dry mass conserved to round-off, state finite."""
+    case = mod("setup").build_case(glevel=6, vlayer=40, dtl=600.0, seed=0)
+    geo, t = case.geo, case.topo
+    ctx = hipdyn.HipContext(geo, case.const, device="cuda:0")
+    try:
+        dyn = hipdyn.HipDynamics(ctx, case.state)
+
+        def mass(P, Pp):
+            vol = interior(geo.VOLUME, t)
+            return ((interior(P[..., 0], t) * vol).sum()
+                    + (pole_centre(Pp[..., 0], t) * pole_centre(geo.VOLUME_pl, t)).sum())
+        m0 = mass(case.state["PROG"], case.state["PROG_pl"])
+        for _ in range(2):
+            dyn.dynamics_step()
+        st = dyn.state()
+        assert np.all(np.isfinite(interior(st["PROG"], t)))
+        assert abs(mass(st["PROG"], st["PROG_pl"]) / m0 - 1.0) < 1e-13
+    finally:
+        ctx.close()
