@@ -54,10 +54,11 @@ def bump_centres(seed: int, n: int = 8):
     return list(zip(lat, lon))
 
 
-def baroclinic_bumps_state(geo: Geometry, c: Const, seed: int = 0, u0: float = 35.0):
+def baroclinic_bumps_state(geo: Geometry, c: Const, seed: int = 0, u0: float = 35.0, amp: float = 1.0):
     """Dry balanced state with potential-temperature bumps; returns prognostic arrays.
 
     PROG (g, g, kall, l, 6), PROG_pl (6, kall, 2, 6), PROGq / PROGq_pl zeros.
+    u0 = 0 and amp = 0 give the resting, horizontally uniform atmosphere.
     """
     t = geo.topo
     vg = geo.vgrid
@@ -71,7 +72,7 @@ def baroclinic_bumps_state(geo: Geometry, c: Const, seed: int = 0, u0: float = 3
         z = vg.gz.reshape((1,) * lat.ndim + (kall,))
         zz = z - shift[..., None]
         tem, pre = reference_profile(zz, grav=c.GRAV, rd=c.Rdry)
-        th_b = theta_bumps(lat, lon, centres, a=c.RADIUS)[..., None]
+        th_b = theta_bumps(lat, lon, centres, amp=amp, a=c.RADIUS)[..., None]
         kappa = c.Rdry / c.CPdry
         tem = tem + th_b * (pre / c.PRE00) ** kappa
         # discrete hydrostatic re-integration from the first level

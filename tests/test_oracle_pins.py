@@ -1,0 +1,216 @@
+"""Pins of the CPU oracle that do not come from the oracle's own formulas.
+
+Closed forms on the sphere (spherical harmonic Y2 = 3 (z/a)^2 - 1 with
+Laplacian -6/a^2 Y2, tangential gradients of linear functions, solid-body
+rotation), the reference's vertical-grid fixture, a dense linear solve for the
+tridiagonal sweep, and invariants of the full step (dry-mass conservation, a
+resting atmosphere staying at rest).  Discretisation errors are checked to
+shrink from gl04 to gl05 at least at first order (the benchmark grid is the
+plain recursive subdivision without spring smoothing, on which the NICAM
+operators converge at first order in the RMS norm), so a dropped term or a
+wrong sign, which leaves an O(1) error, fails.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, interior, mod, pole_centre
+
+grid = mod("grid")
+
+
+def test_vgrid_completion_matches_reference_fixture():
+    """Full levels and ghost levels from interfaces reproduce vgrid40_stretch_45km (mod_grd.py:190-222)."""
+    ref = json.load(open(os.path.join(GOLDEN, "vgrid40_stretch_45km.json")))
+    gzh_ref, gz_ref = np.array(ref["gzh"]), np.array(ref["gz"])
+    gz, gzh = grid.vgrid_from_interfaces(gzh_ref[1:])
+    assert np.allclose(gz, gz_ref, atol=2e-6, rtol=0) and np.allclose(gzh, gzh_ref, atol=2e-6, rtol=0)
+
+
+def test_stretched_interfaces():
+    zh = grid.stretched_interfaces(40, 30000.0, 100.0)
+    assert zh[0] == 0.0 and abs(zh[-1] - 30000.0) < 1e-6 and abs(zh[1] - 100.0) < 1e-9
+    dz = np.diff(zh)
+    assert np.all(np.diff(dz) > 0)
+
+
+@pytest.fixture(scope="module")
+def geos(case_gl04, case_gl05):
+    return {4: case_gl04.geo, 5: case_gl05.geo}
+
+
+def _octx(geo, const):
+    from oracle import dynamics as od
+    return od.OracleContext(geo, const, None)
+
+
+def _fields(geo):
+    """Y2, its tangential gradient, and positions on all points (k axis of size 1)."""
+    a = geo.radius
+    x = geo.GRD_x[:, :, 0]                # (g, g, l, 3)
+    xp = geo.GRD_x_pl[:, 0]               # (6, 2, 3)
+
+    def y2(p):
+        return 3.0 * (p[..., 2] / a) ** 2 - 1.0
+
+    def grad_y2(p):
+        g3 = np.zeros_like(p)
+        g3[..., 2] = 6.0 * p[..., 2] / a ** 2
+        r = p / np.linalg.norm(p, axis=-1, keepdims=True)
+        return g3 - np.sum(g3 * r, axis=-1, keepdims=True) * r
+    return x, xp, y2, grad_y2
+
+
+def _k1(a):
+    """(g, g, l, ...) -> (g, g, 1, l, ...) and (6, 2, ...) -> (6, 1, 2, ...)."""
+    return np.expand_dims(a, 2 if a.shape[0] == a.shape[1] and a.shape[0] != 6 else 1)
+
+
+def _errs(geo, num, num_pl, ana, ana_pl):
+    g = geo.topo.gall_1d
+    s = slice(1, g - 1)
+    e = np.concatenate([(num[s, s] - ana[s, s]).ravel(), (num_pl[0] - ana_pl[0]).ravel()])
+    r = np.concatenate([ana[s, s].ravel(), ana_pl[0].ravel()])
+    return np.sqrt(np.mean(e ** 2) / np.mean(r ** 2))
+
+
+def test_total_area_is_the_sphere(geos):
+    for gl, geo in geos.items():
+        g = geo.topo.gall_1d
+        A = geo.area[1:g - 1, 1:g - 1].sum() + geo.area_pl[0].sum()
+        assert abs(A / (4 * np.pi * geo.radius ** 2) - 1.0) < 1e-12, gl
+
+
+def test_laplacian_of_constant_and_harmonic(geos, case_gl05):
+    from oracle import dynamics as od
+    errs = {}
+    for gl, geo in geos.items():
+        ctx = _octx(geo, case_gl05.const)
+        x, xp, y2, _ = _fields(geo)
+        c = np.ones(x.shape[:3])[:, :, None]
+        lc, lcp = od.laplacian(ctx, c, np.ones((6, 1, 2)))
+        assert np.abs(lc).max() * geo.radius ** 2 < 1e-8 and np.abs(lcp[0]).max() * geo.radius ** 2 < 1e-8
+        f, fp = y2(x)[:, :, None], y2(xp)[:, None]
+        L, Lp = od.laplacian(ctx, f, fp)
+        errs[gl] = _errs(geo, L, Lp, -6.0 / geo.radius ** 2 * f, -6.0 / geo.radius ** 2 * fp)
+    assert errs[5] < 0.02 and errs[4] > 1.7 * errs[5], errs
+
+
+def test_gradient_of_harmonic(geos, case_gl05):
+    from oracle import dynamics as od
+    errs = {}
+    for gl, geo in geos.items():
+        ctx = _octx(geo, case_gl05.const)
+        x, xp, y2, gy2 = _fields(geo)
+        G, Gp = od.gradient(ctx, y2(x)[:, :, None], y2(xp)[:, None])
+        errs[gl] = _errs(geo, G[:, :, 0], Gp[:, 0], gy2(x), gy2(xp))
+    assert errs[5] < 0.01 and errs[4] > 1.7 * errs[5], errs
+
+
+def test_divergence_of_rotation_and_gradient_field(geos, case_gl05):
+    from oracle import dynamics as od
+    errs, rot_errs = {}, {}
+    for gl, geo in geos.items():
+        ctx = _octx(geo, case_gl05.const)
+        x, xp, y2, gy2 = _fields(geo)
+        # solid-body rotation about z with unit angular speed: divergence-free
+        rot = np.stack([-x[..., 1], x[..., 0], 0 * x[..., 0]], -1)
+        rotp = np.stack([-xp[..., 1], xp[..., 0], 0 * xp[..., 0]], -1)
+        D, Dp = od.divergence(ctx, *(rot[:, :, None, :, d] for d in range(3)), *(rotp[:, None, :, d] for d in range(3)))
+        g = geo.topo.gall_1d
+        rot_errs[gl] = max(np.abs(D[1:g - 1, 1:g - 1]).max(), np.abs(Dp[0]).max())
+        v, vp = gy2(x), gy2(xp)
+        D, Dp = od.divergence(ctx, *(v[:, :, None, :, d] for d in range(3)), *(vp[:, None, :, d] for d in range(3)))
+        f, fp = y2(x)[:, :, None], y2(xp)[:, None]
+        errs[gl] = _errs(geo, D, Dp, -6.0 / geo.radius ** 2 * f, -6.0 / geo.radius ** 2 * fp)
+    assert rot_errs[5] < 2e-3 and rot_errs[4] > 1.7 * rot_errs[5], rot_errs
+    assert errs[5] < 0.02 and errs[4] > 1.7 * errs[5], errs
+
+
+def test_kh_weighted_diffusion_is_a_laplacian(geos, case_gl05):
+    """OPRT_diffusion with kh = 1 approximates the Laplacian; kh scales it linearly."""
+    from oracle import dynamics as od
+    errs = {}
+    for gl, geo in geos.items():
+        ctx = _octx(geo, case_gl05.const)
+        x, xp, y2, _ = _fields(geo)
+        f, fp = y2(x)[:, :, None], y2(xp)[:, None]
+        one, onep = np.ones_like(f), np.ones_like(fp)
+        D, Dp = od.diffusion(ctx, f, fp, one, onep)
+        D3, D3p = od.diffusion(ctx, f, fp, 3.0 * one, 3.0 * onep)
+        g = geo.topo.gall_1d
+        assert np.allclose(D3[1:g - 1, 1:g - 1], 3.0 * D[1:g - 1, 1:g - 1], rtol=0, atol=1e-12 * np.abs(D3).max())
+        errs[gl] = _errs(geo, D, Dp, -6.0 / geo.radius ** 2 * f, -6.0 / geo.radius ** 2 * fp)
+    assert errs[5] < 0.03 and errs[4] > 1.7 * errs[5], errs
+
+
+def test_divdamp_operator_is_grad_div(geos, case_gl05):
+    """OPRT_divdamp(grad Y2), projected on the tangent plane, is grad(div grad Y2) = -6/a^2 grad Y2.
+
+    The pentagon cells (1, 1) of the pentagon regions are left out: like the
+    reference (kernels/oprtdivdamp.py:54-122) this operator applies no pentagon
+    mask, so the degenerate (0, 0) triangle enters there.
+    """
+    from oracle import dynamics as od
+    errs = {}
+    for gl, geo in geos.items():
+        ctx = _octx(geo, case_gl05.const)
+        x, xp, _, gy2 = _fields(geo)
+        v, vp = gy2(x), gy2(xp)
+        o, op = od.divdamp_op(ctx, *(v[:, :, None, :, d] for d in range(3)), *(vp[:, None, :, d] for d in range(3)))
+        od.horizontalize_inplace(ctx, *o, *op)
+        num, nump = np.stack(o, -1)[:, :, 0], np.stack(op, -1)[:, 0]
+        ana, anap = -6.0 / geo.radius ** 2 * v, -6.0 / geo.radius ** 2 * vp
+        sgp = geo.topo.have_sgp
+        num[1, 1, sgp] = ana[1, 1, sgp]
+        errs[gl] = _errs(geo, num, nump, ana, anap)
+    assert errs[5] < 0.05 and errs[4] > 1.7 * errs[5], errs
+
+
+def test_thomas_matches_dense_solve():
+    from oracle import dynamics as od
+    rng = np.random.default_rng(7)
+    N, ncol = 39, 5
+    Mc = 4.0 + rng.random((N, ncol))
+    Mu = rng.standard_normal((N, ncol))
+    Ml = rng.standard_normal((N, ncol))
+    S = rng.standard_normal((N, ncol))
+    x = np.array(od.thomas(list(S), list(Mc), list(Mu), list(Ml)))
+    for c in range(ncol):
+        A = np.diag(Mc[:, c]) + np.diag(Mu[:-1, c], 1) + np.diag(Ml[1:, c], -1)
+        assert np.allclose(x[:, c], np.linalg.solve(A, S[:, c]), rtol=1e-13, atol=1e-13)
+
+
+def _mass(geo, P, Pp):
+    t = geo.topo
+    return ((interior(P[..., 0], t) * interior(geo.VOLUME, t)).sum()
+            + (pole_centre(Pp[..., 0], t) * pole_centre(geo.VOLUME_pl, t)).sum())
+
+
+def test_step_conserves_dry_mass(case_gl04):
+    from oracle import dynamics as od
+    c = case_gl04
+    dyn = od.OracleDynamics(_octx(c.geo, c.const), c.state)
+    m0 = _mass(c.geo, dyn.PROG, dyn.PROG_pl)
+    for _ in range(2):
+        dyn.step()
+    assert abs(_mass(c.geo, dyn.PROG, dyn.PROG_pl) / m0 - 1.0) < 1e-13
+
+
+def test_resting_atmosphere_stays_at_rest(case_gl04):
+    """No wind, no bumps: the discretely balanced column must not start moving."""
+    from oracle import dynamics as od
+    c = case_gl04
+    state = mod("initial").baroclinic_bumps_state(c.geo, c.const, seed=0, u0=0.0, amp=0.0)
+    dyn = od.OracleDynamics(_octx(c.geo, c.const), state)
+    for _ in range(2):
+        dyn.step()
+    t = c.topo
+    rg = interior(dyn.PROG[..., 0], t)
+    for v in (1, 2, 3, 4):
+        speed = np.abs(interior(dyn.PROG[..., v], t) / rg).max()
+        assert speed < 1e-6, (v, speed)
+    drho = np.abs(interior(dyn.PROG[..., 0], t) - interior(state["PROG"][..., 0], t)).max() / rg.max()
+    assert drho < 1e-10
