@@ -48,7 +48,7 @@ typedef struct {
 } nh_dims;
 
 /* Physical constants and numerical-filter coefficients
- * (mod_const.py:9-205, mod_runconf.py:13, mod_numfilter.py:304-366). */
+ * (mod_const.py:9-205, mod_runconf.py:13-19, mod_numfilter.py:304-366). */
 typedef struct {
   double radius;          /* CONST_RADIUS                                     */
   double ohm;             /* CONST_OHM                                        */
@@ -61,6 +61,7 @@ typedef struct {
   double gamma_h;         /* hyperdiffusion coefficient, hdiff_type DIRECT    */
   double hdiff_fact_rho;  /* density factor of the hyperdiffusion             */
   double alpha_d;         /* 3-D divergence damping coefficient, DIRECT       */
+  int32_t thuburn_lim;    /* THUBURN_LIM: tracer limiters on (1) or off (0)   */
 } nh_consts;
 
 /* Host pointers to the operator coefficients and vertical profiles
@@ -82,6 +83,14 @@ typedef struct {
   const int32_t* sgp;
   const double *afact, *bfact, *cfact, *dfact, *rdgz, *rdgzh;
   const double *c2w0, *c2w1, *w2c0, *w2c1, *phi;
+  /* tracer transport geometry (mod_gmtr.py GMTR_t / GMTR_a / GMTR_p, mod_grd.py GRD_xr):
+   *   tri_w   6 = triangle*3 + w     interpolation weights W1..W3 of TI, TJ
+   *   arc_hn  9 = arc*3 + dir        hexagon-arc normals (AI, AIJ, AJ) times length
+   *   rarea   1                      1 / control-volume area
+   *   arc_xr  9 = arc*3 + dir        hexagon-arc chord midpoints
+   * pole: tri_w_pl [v][lpl][w], arc_hn_pl / arc_xr_pl [v][lpl][dir], rarea_pl [lpl] */
+  const double *tri_w, *arc_hn, *rarea, *arc_xr;
+  const double *tri_w_pl, *arc_hn_pl, *rarea_pl, *arc_xr_pl;
 } nh_geometry;
 
 typedef struct nh_context nh_context;
@@ -131,8 +140,11 @@ int nh_gradient(nh_context* ctx, const double* scl, double* gx, double* gy, doub
 /* One dynamical-core large step, in place: prog holds the 6 prognostic
  * fields (rhog, rhogvx, rhogvy, rhogvz, rhogw, rhoge), progq the nq >= 1
  * tracer mass fields (tracer 0 = qv).  RK3 with sstep_max/3, sstep_max/2 and
- * sstep_max HEVI small steps of dtl/sstep_max.
- * Replaces Dyn.dynamics_step (mod_dynamics.py:1725-2741). */
+ * sstep_max HEVI small steps of dtl/sstep_max; the tracers are transported
+ * in the last stage with the time-mean mass fluxes (MIURA2004, Thuburn
+ * limiters when consts.thuburn_lim).
+ * Replaces Dyn.dynamics_step (mod_dynamics.py:1725-2741) with
+ * Srctr.src_tracer_advection (mod_src_tracer.py:83-1228). */
 int nh_dyn_step(nh_context* ctx, double* prog, double* progq, int32_t nq, double dtl, int32_t sstep_max,
                 void* stream);
 

@@ -413,6 +413,12 @@ def thomas(S, Mc, Mu, Ml):
 # ======================================================================
 # the dynamics step
 # ======================================================================
+def default_tracer(dyn, rhog_in, rhog_in_p, PM, PMp, f_rhog, f_rhog_p, dtl):
+    """MIURA2004 transport of all tracers with the Thuburn limiters (mod_dynamics.py:2346-2461)."""
+    from .tracer import tracer_advection
+    tracer_advection(dyn.ctx, dyn.PROGq, dyn.PROGq_pl, rhog_in, rhog_in_p, PM, PMp, f_rhog, f_rhog_p, dtl,
+                     limiter=bool(dyn.ctx.c.THUBURN_LIM))
+
 class OracleDynamics:
     """One RK3/HEVI large step (mod_dynamics.py:1725-2741, numpy branches).
 
@@ -429,7 +435,7 @@ class OracleDynamics:
         g, kall, L = ctx.g, ctx.kall, ctx.topo.lall
         self.DIAG = np.zeros((g, g, kall, L, 6))
         self.DIAG_pl = np.zeros((6, kall, 2, 6))
-        self.tracer = None     # optional tracer-advection callable (last RK stage)
+        self.tracer = default_tracer   # tracer transport of the last RK stage (None: skip)
 
     # ------------------------------------------------------------------
     def prepost(self):

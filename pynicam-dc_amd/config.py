@@ -38,6 +38,8 @@ class Const:
     HDIFF_FACT_RHO: float = 1.0e-2
     ALPHA_D: float = 1.0e16            # DIRECT 3-D divergence damping coefficient
     LAP_ORDER_DIVDAMP: int = 2
+    # tracer transport (mod_runconf.py:19, TRC_ADV_TYPE = MIURA2004)
+    THUBURN_LIM: bool = True
     # time integration (mod_time.py:83-102)
     DTL: float = 1200.0
     SSTEP_MAX: int = 6

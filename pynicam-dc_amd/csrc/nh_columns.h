@@ -38,6 +38,7 @@ struct Dims {
 struct Consts {
   double RADIUS, OHM, GRAV, Rdry, CVdry, Rvap, CVW_qv, ALPHA;
   double GAMMA_H, HDIFF_FACT_RHO, ALPHA_D;
+  int THUBURN_LIM;
 };
 
 struct Geo {
@@ -59,6 +60,15 @@ struct Geo {
   // vertical profiles [kall]
   const double *afact, *bfact, *cfact, *dfact, *rdgz, *rdgzh;
   const double *c2w0, *c2w1, *w2c0, *w2c1, *phi;
+  // tracer transport: regional [comp][nreg], pole [v][lp][...]
+  const double* twt;      // 6 = T*3 + w
+  const double* ahn;      // 9 = arc*3 + d
+  const double* rarea;    // 1
+  const double* axr;      // 9 = arc*3 + d
+  const double* twt_pl;   // (v*2+lp)*3+w
+  const double* ahn_pl;   // (v*2+lp)*3+d
+  const double* rarea_pl; // lp
+  const double* axr_pl;   // (v*2+lp)*3+d
 };
 
 enum { I_RHOG = 0, I_RHOGVX, I_RHOGVY, I_RHOGVZ, I_RHOGW, I_RHOGE };

@@ -16,6 +16,7 @@
 #include <string>
 
 #include "nh_step.h"
+#include "nh_tracer.h"
 #include "../../include/nicamhip.h"
 
 namespace nh {
@@ -38,7 +39,17 @@ enum WsField {
   W_ETHTOT = 49,          // 1
   W_PM = 50,              // 5  time-mean mass fluxes
   W_SPLIT = 55,           // 12 split state (6) + path1 momentum (3) + vi_main out (3)
-  W_NFIELDS = 67
+  // tracer transport
+  W_FLXV = 67,            // 1  vertical mass flux of the half steps
+  W_RHOG1 = 68,           // 1  density after the first vertical half step
+  W_RHOG2 = 69,           // 1  density after the horizontal step
+  W_FA = 70,              // 3  arc mass fluxes (AI, AIJ, AJ)
+  W_XC = 73,              // 9  arc mass centroids
+  W_GQ = 82,              // 3  gradient of the mixing ratio
+  W_QA = 85,              // 3  upwind face values (faces 0..2)
+  W_QOUT = 88,            // 2  limiter outflow bounds
+  W_RQ = 90,              // 1  tracer after the horizontal step
+  W_NFIELDS = 91
 };
 
 struct Context {
@@ -81,23 +92,27 @@ inline void set_consts(Consts& c, const nh_consts* in) {
   c.GAMMA_H = in->gamma_h;
   c.HDIFF_FACT_RHO = in->hdiff_fact_rho;
   c.ALPHA_D = in->alpha_d;
+  c.THUBURN_LIM = in->thuburn_lim;
 }
 
 // geometry sizes in doubles, in upload order
 struct GeoSizes {
-  long reg[6];   // cdiv, cgrad, clap, cintp, cdiff, gx
-  long pl[6];
+  long reg[10];   // cdiv, cgrad, clap, cintp, cdiff, gx, twt, ahn, rarea, axr
+  long pl[10];
   long total;
 };
 
 inline GeoSizes geo_sizes(const Dims& d) {
   GeoSizes s;
   const long n = d.nreg;
-  s.reg[0] = 21 * n; s.reg[1] = 21 * n; s.reg[2] = 7 * n;
-  s.reg[3] = 18 * n; s.reg[4] = 18 * n; s.reg[5] = 3 * n;
-  s.pl[0] = 36; s.pl[1] = 36; s.pl[2] = 12; s.pl[3] = 108; s.pl[4] = 36; s.pl[5] = 36;
+  const long rc[10] = {21, 21, 7, 18, 18, 3, 6, 9, 1, 9};
+  const long pc[10] = {36, 36, 12, 108, 36, 36, 36, 36, 2, 36};
   s.total = 0;
-  for (int i = 0; i < 6; ++i) s.total += s.reg[i] + s.pl[i];
+  for (int i = 0; i < 10; ++i) {
+    s.reg[i] = rc[i] * n;
+    s.pl[i] = pc[i];
+    s.total += s.reg[i] + s.pl[i];
+  }
   s.total += 11L * d.kall;
   return s;
 }
@@ -111,28 +126,30 @@ int upload_geometry(X& x, Context& ctx, const nh_geometry* g) {
     ctx.int_buf = (int*)x.alloc(d.lall * sizeof(int));
     if (!ctx.geo_buf || !ctx.int_buf) return NH_ERR_ALLOC;
   }
-  const double* regsrc[6] = {g->coef_div, g->coef_grad, g->coef_lap, g->coef_intp, g->coef_diff, g->grd_x};
-  const double* plsrc[6] = {g->coef_div_pl, g->coef_grad_pl, g->coef_lap_pl, g->coef_intp_pl, g->coef_diff_pl,
-                            g->grd_x_pl};
+  const double* regsrc[10] = {g->coef_div, g->coef_grad, g->coef_lap, g->coef_intp, g->coef_diff, g->grd_x,
+                              g->tri_w, g->arc_hn, g->rarea, g->arc_xr};
+  const double* plsrc[10] = {g->coef_div_pl, g->coef_grad_pl, g->coef_lap_pl, g->coef_intp_pl, g->coef_diff_pl,
+                             g->grd_x_pl, g->tri_w_pl, g->arc_hn_pl, g->rarea_pl, g->arc_xr_pl};
   const double* vsrc[11] = {g->afact, g->bfact, g->cfact, g->dfact, g->rdgz, g->rdgzh,
                             g->c2w0, g->c2w1, g->w2c0, g->w2c1, g->phi};
-  for (int i = 0; i < 6; ++i)
+  for (int i = 0; i < 10; ++i)
     if (!regsrc[i] || !plsrc[i]) return NH_ERR_ARG;
   for (int i = 0; i < 11; ++i)
     if (!vsrc[i]) return NH_ERR_ARG;
   if (!g->sgp) return NH_ERR_ARG;
   double* p = ctx.geo_buf;
-  const double** regdst[6] = {&ctx.G.cdiv, &ctx.G.cgrad, &ctx.G.clap, &ctx.G.cintp, &ctx.G.cdiff, &ctx.G.gx};
-  const double** pldst[6] = {&ctx.G.cdiv_pl, &ctx.G.cgrad_pl, &ctx.G.clap_pl, &ctx.G.cintp_pl, &ctx.G.cdiff_pl,
-                             &ctx.G.gx_pl};
+  const double** regdst[10] = {&ctx.G.cdiv, &ctx.G.cgrad, &ctx.G.clap, &ctx.G.cintp, &ctx.G.cdiff, &ctx.G.gx,
+                               &ctx.G.twt, &ctx.G.ahn, &ctx.G.rarea, &ctx.G.axr};
+  const double** pldst[10] = {&ctx.G.cdiv_pl, &ctx.G.cgrad_pl, &ctx.G.clap_pl, &ctx.G.cintp_pl, &ctx.G.cdiff_pl,
+                              &ctx.G.gx_pl, &ctx.G.twt_pl, &ctx.G.ahn_pl, &ctx.G.rarea_pl, &ctx.G.axr_pl};
   const double** vdst[11] = {&ctx.G.afact, &ctx.G.bfact, &ctx.G.cfact, &ctx.G.dfact, &ctx.G.rdgz, &ctx.G.rdgzh,
                              &ctx.G.c2w0, &ctx.G.c2w1, &ctx.G.w2c0, &ctx.G.w2c1, &ctx.G.phi};
-  for (int i = 0; i < 6; ++i) {
+  for (int i = 0; i < 10; ++i) {
     x.h2d(p, regsrc[i], s.reg[i] * sizeof(double));
     *regdst[i] = p;
     p += s.reg[i];
   }
-  for (int i = 0; i < 6; ++i) {
+  for (int i = 0; i < 10; ++i) {
     x.h2d(p, plsrc[i], s.pl[i] * sizeof(double));
     *pldst[i] = p;
     p += s.pl[i];
@@ -212,6 +229,101 @@ void divdamp(X& x, const Context& ctx, const double* vx, const double* vy, const
   for (int q = 0; q < 3; ++q) b.in[q] = tmp[q];
   b.out[0] = out0; b.out[1] = out1; b.out[2] = out2;
   x.run(b, ncols(ctx.d));
+}
+
+// ----------------------------------------------------------------------------
+// tracer transport with the time-mean mass fluxes of the last stage
+// (mod_dynamics.py:2346-2461 -> mod_src_tracer.py:83-1228)
+// ----------------------------------------------------------------------------
+template <class X>
+void tracer_transport(X& x, const Context& ctx, double* progq, int nq, double dt) {
+  const Dims& d = ctx.d;
+  const long NF = d.NF, nc = ncols(d);
+  const double* pm = ctx.field(W_PM);
+  const double* frhog = ctx.field(W_FRHOG);
+  double* rhog1 = ctx.field(W_RHOG1);
+  double* rhog2 = ctx.field(W_RHOG2);
+  double* flxv = ctx.field(W_FLXV);
+  {
+    TrcVert1 f;
+    f.d = d; f.c = ctx.c; f.G = ctx.G;
+    f.dt = dt;
+    f.rw_mean = pm + 4 * NF;
+    f.rhog_in = ctx.field(W_PROG0);
+    f.frhog = frhog;
+    f.flxv = flxv; f.rhog1 = rhog1;
+    f.rhogq = progq; f.nq = nq;
+    f.limiter = ctx.c.THUBURN_LIM;
+    x.run(f, nc);
+  }
+  double* fa[3] = {ctx.field(W_FA), ctx.field(W_FA + 1), ctx.field(W_FA + 2)};
+  double* xc[9];
+  for (int q = 0; q < 9; ++q) xc[q] = ctx.field(W_XC + q);
+  {
+    TrcHFlux f;
+    f.d = d; f.c = ctx.c; f.G = ctx.G;
+    f.dt = dt; f.pm = pm;
+    for (int q = 0; q < 3; ++q) f.fa[q] = fa[q];
+    for (int q = 0; q < 9; ++q) f.xc[q] = xc[q];
+    x.run(f, nc);
+  }
+  {
+    TrcRhog2 f;
+    f.d = d; f.c = ctx.c; f.G = ctx.G;
+    f.dt = dt;
+    for (int q = 0; q < 3; ++q) f.fa[q] = fa[q];
+    f.rhog1 = rhog1; f.frhog = frhog; f.rhog2 = rhog2;
+    x.run(f, nc);
+  }
+  double* gq[3] = {ctx.field(W_GQ), ctx.field(W_GQ + 1), ctx.field(W_GQ + 2)};
+  double* qa[3] = {ctx.field(W_QA), ctx.field(W_QA + 1), ctx.field(W_QA + 2)};
+  double* qout[2] = {ctx.field(W_QOUT), ctx.field(W_QOUT + 1)};
+  double* rqh = ctx.field(W_RQ);
+  for (int iq = 0; iq < nq; ++iq) {
+    double* rq = progq + (long)iq * NF;
+    {
+      TrcGradQ f;
+      f.d = d; f.G = ctx.G;
+      f.rq = rq; f.rhog1 = rhog1;
+      for (int q = 0; q < 3; ++q) f.gq[q] = gq[q];
+      x.run(f, nc);
+    }
+    halo(x, ctx, gq, 3);
+    {
+      TrcRemap f;
+      f.d = d; f.c = ctx.c; f.G = ctx.G;
+      f.dt = dt; f.rq = rq; f.rhog1 = rhog1;
+      for (int q = 0; q < 3; ++q) { f.gq[q] = gq[q]; f.fa[q] = fa[q]; f.qa[q] = qa[q]; }
+      for (int q = 0; q < 9; ++q) f.xc[q] = xc[q];
+      x.run(f, nc);
+    }
+    if (ctx.c.THUBURN_LIM) {
+      TrcQout f;
+      f.d = d; f.c = ctx.c; f.G = ctx.G;
+      f.dt = dt; f.rq = rq; f.rhog1 = rhog1; f.frhog = frhog;
+      for (int q = 0; q < 3; ++q) f.fa[q] = fa[q];
+      f.qout[0] = qout[0]; f.qout[1] = qout[1];
+      x.run(f, nc);
+      halo(x, ctx, qout, 2);
+    }
+    {
+      TrcApply f;
+      f.d = d; f.c = ctx.c; f.G = ctx.G;
+      f.dt = dt; f.rq = rq; f.rhog1 = rhog1;
+      for (int q = 0; q < 3; ++q) { f.fa[q] = fa[q]; f.qa[q] = qa[q]; }
+      f.qout[0] = qout[0]; f.qout[1] = qout[1];
+      f.out = rqh;
+      x.run(f, nc);
+    }
+    {
+      TrcVert2 f;
+      f.d = d; f.c = ctx.c; f.G = ctx.G;
+      f.dt = dt; f.flxv = flxv; f.rhog2 = rhog2; f.frhog = frhog;
+      f.in = rqh; f.out = rq;
+      f.limiter = ctx.c.THUBURN_LIM;
+      x.run(f, nc);
+    }
+  }
 }
 
 // ----------------------------------------------------------------------------
@@ -344,8 +456,7 @@ int dyn_step(X& x, const Context& ctx, double* prog, double* progq, int nq, doub
     int rc = x.check("dyn_step stage");
     if (rc) return rc;
   }
-  // tracer transport of the last stage is not applied here: the dry state
-  // carries qv = 0, for which the flux-form update is the identity
+  tracer_transport(x, ctx, progq, nq, dtl);
   halo_block(x, ctx, prog, 6);
   halo_block(x, ctx, progq, nq);
   return x.check("dyn_step");

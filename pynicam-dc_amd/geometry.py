@@ -157,9 +157,34 @@ class Geometry:
             self._gmtr_t()
             self._gmtr_a()
             self._oprt()
+        self._arc_midpoints()
         self._vmtr()
 
     # ------------------------------------------------------------------
+    def _arc_midpoints(self):
+        """GRD_xr: chord midpoints of the hexagon arcs between triangle centres (mod_grd.py:625-660).
+
+        AI (i, j >= 1): TJ(i, j-1)..TI(i, j); AIJ: TI(i, j)..TJ(i, j); AJ (i >= 1, j):
+        TJ(i, j)..TI(i-1, j); pole arc v: triangle v-1 .. triangle v.  Used for the
+        mass-centroid positions of the tracer transport.
+        """
+        t = self.topo
+        g, gmax = t.gall_1d, t.gmax
+        xt = self.xt
+        xr = np.zeros((g, g, t.lall, 3, 3))
+        i0, j1, jm = slice(0, gmax + 1), slice(1, gmax + 1), slice(0, gmax)
+        xr[i0, j1, :, AI] = 0.5 * (xt[i0, jm, :, TJ] + xt[i0, j1, :, TI])
+        s = slice(0, gmax + 1)
+        xr[s, s, :, AIJ] = 0.5 * (xt[s, s, :, TI] + xt[s, s, :, TJ])
+        i1, im, j0 = slice(1, gmax + 1), slice(0, gmax), slice(0, gmax + 1)
+        xr[i1, j0, :, AJ] = 0.5 * (xt[i1, j0, :, TJ] + xt[im, j0, :, TI])
+        xr_pl = np.zeros((GALL_PL, 2, 3))
+        for l in range(2):
+            for v in range(GMIN_PL, GMAX_PL + 1):
+                vm = GMAX_PL if v == GMIN_PL else v - 1
+                xr_pl[v, l] = 0.5 * (self.xt_pl[vm, l] + self.xt_pl[v, l])
+        self.xr, self.xr_pl = xr, xr_pl
+
     def _gmtr_p(self):
         """Cell areas and local east/north unit vectors (GMTR_p_setup, mod_gmtr.py:185-335)."""
         t = self.topo

@@ -49,7 +49,8 @@ class HipContext:
             raise RuntimeError("field size mismatch between library and host layout")
         c = const
         self.consts = native.NhConsts(c.RADIUS, c.OHM, c.GRAV, c.Rdry, c.CVdry, c.Rvap, float(c.CVW[c.I_QV]),
-                                      c.ALPHA, c.GAMMA_H, c.HDIFF_FACT_RHO, c.ALPHA_D)
+                                      c.ALPHA, c.GAMMA_H, c.HDIFF_FACT_RHO, c.ALPHA_D,
+                                      1 if c.THUBURN_LIM else 0)
         if c.LAP_ORDER_HDIFF != 2 or c.LAP_ORDER_DIVDAMP != 2:
             raise NotImplementedError("backend='hip' implements lap_order 2 for hyperdiffusion and divergence damping")
         dev_index = self.device.index if self.device.type == "cuda" and self.device.index is not None else 0

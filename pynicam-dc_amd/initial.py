@@ -112,3 +112,40 @@ def baroclinic_bumps_state(geo: Geometry, c: Const, seed: int = 0, u0: float = 3
     t.halo_fill_host(out["PROG"], out["PROG_pl"])
     t.halo_fill_host(PROGq, PROGq_pl)
     return {"PROG": out["PROG"], "PROG_pl": out["PROG_pl"], "PROGq": PROGq, "PROGq_pl": PROGq_pl}
+
+
+def cosine_blobs(lat, lon, centres, radius_m=1.0e6, a=6.371e6):
+    """Sum of compact cosine bells 0.5 (1 + cos(pi d / r)) for d < r, capped at 1."""
+    out = np.zeros(np.shape(lat))
+    for (clat, clon) in centres:
+        cosd = np.sin(lat) * np.sin(clat) + np.cos(lat) * np.cos(clat) * np.cos(lon - clon)
+        d = a * np.arccos(np.clip(cosd, -1.0, 1.0))
+        out = out + np.where(d < radius_m, 0.5 * (1.0 + np.cos(np.pi * d / radius_m)), 0.0)
+    return np.minimum(out, 1.0)
+
+
+def blob_tracers(geo: Geometry, state: dict, ntracer: int = 4, seed: int = 0, nblob: int = 6,
+                 radius_m: float = 1.0e6, with_qv: bool = True):
+    """Passive tracers for the transport tests: each a seeded sum of compact cosine blobs in [0, 1].
+
+    Returns a copy of ``state`` whose PROGq/PROGq_pl hold qv = 0 (when
+    ``with_qv``) followed by ``ntracer`` tracer masses rhog * q on the physical
+    levels (zero on the ghost levels), halos filled.
+    """
+    t = geo.topo
+    kmin, kmax = t.kmin, t.kmax
+    rng = np.random.default_rng(seed + 1000)
+    P, Pp = state["PROG"], state["PROG_pl"]
+    nq = ntracer + (1 if with_qv else 0)
+    Q = np.zeros(P.shape[:-1] + (nq,))
+    Qp = np.zeros(Pp.shape[:-1] + (nq,))
+    k = slice(kmin, kmax + 1)
+    for n in range(ntracer):
+        centres = list(zip(np.arcsin(rng.uniform(-0.9, 0.9, nblob)), rng.uniform(-np.pi, np.pi, nblob)))
+        q = cosine_blobs(geo.lat, geo.lon, centres, radius_m, geo.radius)
+        qp = cosine_blobs(geo.lat_pl, geo.lon_pl, centres, radius_m, geo.radius)
+        iq = n + (1 if with_qv else 0)
+        Q[:, :, k, :, iq] = P[:, :, k, :, 0] * q[:, :, None, :]
+        Qp[:, k, :, iq] = Pp[:, k, :, 0] * qp[:, None, :]
+    t.halo_fill_host(Q, Qp)
+    return dict(state, PROGq=Q, PROGq_pl=Qp)

@@ -110,13 +110,24 @@ def geometry_arrays(geo) -> dict:
         "coef_diff_pl": np.ascontiguousarray(geo.coef_diff_pl).reshape(-1),   # (6, 2, 3)
         "grd_x_pl": np.ascontiguousarray(geo.GRD_x_pl[:, 0]).reshape(-1),     # (6, 2, 3)
         "sgp": np.ascontiguousarray(geo.topo.have_sgp.astype(np.int32)),
+        # tracer transport (GMTR_t weights W1..W3, GMTR_a hexagon normals, GMTR_p RAREA, GRD_xr)
+        "tri_w": _soa(geo.gmtr_t[..., 2:5]),                 # (g, g, l, 2, 3) -> 6
+        "arc_hn": _soa(geo.gmtr_a[..., :, 0:3]),             # (g, g, l, 3, 3) -> 9
+        "rarea": _soa(geo.gmtr_p[..., 1]),                   # 1
+        "arc_xr": _soa(geo.xr),                              # (g, g, l, 3, 3) -> 9
+        "tri_w_pl": np.ascontiguousarray(geo.gmtr_t_pl[..., 2:5]).reshape(-1),   # (6, 2, 3)
+        "arc_hn_pl": np.ascontiguousarray(geo.gmtr_a_pl[..., 0:3]).reshape(-1),  # (6, 2, 3)
+        "rarea_pl": np.ascontiguousarray(geo.gmtr_p_pl[0, :, 1]).reshape(-1),   # (2,)
+        "arc_xr_pl": np.ascontiguousarray(geo.xr_pl).reshape(-1),               # (6, 2, 3)
     }
     sizes = {"coef_div_pl": 36, "coef_grad_pl": 36, "coef_lap_pl": 12, "coef_intp_pl": 108,
-             "coef_diff_pl": 36, "grd_x_pl": 36}
+             "coef_diff_pl": 36, "grd_x_pl": 36, "tri_w_pl": 36, "arc_hn_pl": 36, "rarea_pl": 2,
+             "arc_xr_pl": 36}
     for k, n in sizes.items():
         if out[k].size != n:
             raise ValueError(f"{k} has {out[k].size} entries, expected {n}")
-    for k in ("coef_div", "coef_grad", "coef_lap", "coef_intp", "coef_diff", "grd_x"):
+    for k in ("coef_div", "coef_grad", "coef_lap", "coef_intp", "coef_diff", "grd_x", "tri_w", "arc_hn",
+              "rarea", "arc_xr", "tri_w_pl", "arc_hn_pl", "rarea_pl", "arc_xr_pl"):
         out[k] = np.ascontiguousarray(out[k], dtype=np.float64)
     out.update({k: np.ascontiguousarray(v, dtype=np.float64) for k, v in vertical_profiles(geo).items()})
     return out

@@ -24,7 +24,7 @@ class NhDims(C.Structure):
 class NhConsts(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "radius", "ohm", "grav", "rdry", "cvdry", "rvap", "cvw_qv", "alpha",
-        "gamma_h", "hdiff_fact_rho", "alpha_d")]
+        "gamma_h", "hdiff_fact_rho", "alpha_d")] + [("thuburn_lim", C.c_int32)]
 
 
 _GEO_FIELDS = [
@@ -34,10 +34,14 @@ _GEO_FIELDS = [
 _VERT_FIELDS = ["afact", "bfact", "cfact", "dfact", "rdgz", "rdgzh", "c2w0", "c2w1", "w2c0", "w2c1", "phi"]
 
 
+_TRC_FIELDS = ["tri_w", "arc_hn", "rarea", "arc_xr", "tri_w_pl", "arc_hn_pl", "rarea_pl", "arc_xr_pl"]
+
+
 class NhGeometry(C.Structure):
     _fields_ = ([(n, C.POINTER(C.c_double)) for n in _GEO_FIELDS]
                 + [("sgp", C.POINTER(C.c_int32))]
-                + [(n, C.POINTER(C.c_double)) for n in _VERT_FIELDS])
+                + [(n, C.POINTER(C.c_double)) for n in _VERT_FIELDS]
+                + [(n, C.POINTER(C.c_double)) for n in _TRC_FIELDS])
 
 
 EXPORTS = {

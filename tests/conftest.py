@@ -31,6 +31,12 @@ def case_gl04():
 
 
 @pytest.fixture(scope="session")
+def state_gl04_tracers(case_gl04):
+    """gl04 state with qv = 0 and three cosine-blob tracers."""
+    return mod("initial").blob_tracers(case_gl04.geo, case_gl04.state, ntracer=3, seed=0)
+
+
+@pytest.fixture(scope="session")
 def case_gl05():
     """gl05 z40, 30 km top, 100 m bottom layer, dt 1200 s: the CPU-runnable benchmark case."""
     return mod("setup").build_case(glevel=5, vlayer=40, dtl=1200.0, seed=0)

@@ -115,13 +115,13 @@ def test_host_build_operators_bit_exact(case_gl04, emu_ctx):
     assert np.array_equal(a[s, s], do[s, s]) and np.array_equal(ap[0], dop[0])
 
 
-def test_host_build_step_matches_oracle(case_gl04, emu_ctx):
-    """Two large steps of the kernel source (host build) against the numpy oracle."""
+def test_host_build_step_matches_oracle(case_gl04, state_gl04_tracers, emu_ctx):
+    """Two large steps (dynamics + tracer transport) of the kernel source, host build, against the oracle."""
     from oracle import dynamics as od
     c = case_gl04
     t = c.topo
-    ref = od.OracleDynamics(od.OracleContext(c.geo, c.const, None), c.state)
-    dyn = hipdyn.HipDynamics(emu_ctx, c.state)
+    ref = od.OracleDynamics(od.OracleContext(c.geo, c.const, None), state_gl04_tracers)
+    dyn = hipdyn.HipDynamics(emu_ctx, state_gl04_tracers)
     for _ in range(2):
         ref.step()
         dyn.dynamics_step()
@@ -131,3 +131,7 @@ def test_host_build_step_matches_oracle(case_gl04, emu_ctx):
         ap, bp = pole_centre(ref.PROG_pl[..., v], t), pole_centre(got["PROG_pl"][..., v], t)
         # rhogw is measured against its own (small) peak; round-off of the terrain terms enters there
         assert rel_err(a, b, ap, bp) < 1e-10, v
+    for q in range(1, got["PROGq"].shape[-1]):
+        a, b = interior(ref.PROGq[..., q], t), interior(got["PROGq"][..., q], t)
+        ap, bp = pole_centre(ref.PROGq_pl[..., q], t), pole_centre(got["PROGq_pl"][..., q], t)
+        assert rel_err(a, b, ap, bp) < 1e-12, q

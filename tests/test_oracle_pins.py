@@ -214,3 +214,39 @@ def test_resting_atmosphere_stays_at_rest(case_gl04):
         assert speed < 1e-6, (v, speed)
     drho = np.abs(interior(dyn.PROG[..., 0], t) - interior(state["PROG"][..., 0], t)).max() / rg.max()
     assert drho < 1e-10
+
+
+def _tracer_run(case, hdiff_rho, limiter, nsteps=2):
+    from oracle import dynamics as od
+    import copy
+    const = copy.copy(case.const)
+    const.HDIFF_FACT_RHO = hdiff_rho
+    const.THUBURN_LIM = limiter
+    st = mod("initial").blob_tracers(case.geo, case.state, ntracer=2, seed=3, radius_m=8.0e5)
+    t = case.topo
+    k = slice(t.kmin, t.kmax + 1)
+    # tracer 1: q = 1 everywhere
+    st["PROGq"][:, :, k, :, 1] = st["PROG"][:, :, k, :, 0]
+    st["PROGq_pl"][:, k, :, 1] = st["PROG_pl"][:, k, :, 0]
+    dyn = od.OracleDynamics(_octx(case.geo, const), st)
+    for _ in range(nsteps):
+        dyn.step()
+    q = interior(dyn.PROGq, t) / interior(dyn.PROG[..., 0], t)[..., None]
+    qp = pole_centre(dyn.PROGq_pl, t) / pole_centre(dyn.PROG_pl[..., 0], t)[..., None]
+    return q, qp
+
+
+def test_tracer_transport_keeps_constant_mixing_ratio(case_gl04):
+    """With the density hyperdiffusion off the tracer mass fluxes are the dynamics' mass fluxes:
+    q = 1 stays 1 to round-off (MIURA2004 consistency), and qv = 0 stays 0."""
+    q, qp = _tracer_run(case_gl04, 0.0, True)
+    assert np.abs(q[..., 1] - 1.0).max() < 1e-13 and np.abs(qp[..., 1] - 1.0).max() < 1e-13
+    assert np.all(q[..., 0] == 0.0)
+
+
+def test_tracer_limiters_keep_bounds(case_gl04):
+    """Thuburn limiters keep the blob tracer inside its initial range [0, 1]; without them it leaves it."""
+    q, qp = _tracer_run(case_gl04, 0.0, True)
+    assert q[..., 2].min() >= -1e-14 and q[..., 2].max() <= 1.0 + 1e-13
+    q_nl, _ = _tracer_run(case_gl04, 0.0, False)
+    assert q_nl[..., 2].min() < -1e-6 or q_nl[..., 2].max() > 1.0 + 1e-6
