@@ -25,60 +25,6 @@ inline int fail(int code, const char* where, const std::string& msg) {
   return code;
 }
 
-// single-operator functors (interior points and pole centres, all levels)
-struct OpLap {
-  static constexpr const char* kName = "oprt_laplacian";
-  Dims d; Geo G;
-  const double* s;
-  double* o;
-  NH_HD void operator()(long t) const {
-    Cell cl = cell_of(d, t);
-    if (cl.pole ? cl.v != 0 : !cl.inner) return;
-    for (int k = 0; k < d.kall; ++k) {
-      long a = cl.base + k * cl.ks;
-      o[a] = cl.pole ? lap_pl(G, cl.lp, s, a) : lap7(G, d.nreg, cl.ci, s, a, d.g);
-    }
-  }
-};
-
-struct OpDiv {
-  static constexpr const char* kName = "oprt_divergence";
-  Dims d; Geo G;
-  const double *vx, *vy, *vz;
-  double* o;
-  NH_HD void operator()(long t) const {
-    Cell cl = cell_of(d, t);
-    if (cl.pole ? cl.v != 0 : !cl.inner) return;
-    long nb[7];
-    neighbours(d, cl, nb);
-    for (int k = 0; k < d.kall; ++k) {
-      long kk = k * cl.ks;
-      o[cl.base + kk] = hdiv(G, d, cl, [&](int m) { return vx[nb[m] + kk]; },
-                             [&](int m) { return vy[nb[m] + kk]; }, [&](int m) { return vz[nb[m] + kk]; });
-    }
-  }
-};
-
-struct OpGrad {
-  static constexpr const char* kName = "oprt_gradient";
-  Dims d; Geo G;
-  const double* s;
-  double *ox, *oy, *oz;
-  NH_HD void operator()(long t) const {
-    Cell cl = cell_of(d, t);
-    if (cl.pole ? cl.v != 0 : !cl.inner) return;
-    for (int k = 0; k < d.kall; ++k) {
-      long a = cl.base + k * cl.ks;
-      double o[3];
-      if (cl.pole) grad_pl(G, cl.lp, s, a, o);
-      else grad7(G, d.nreg, cl.ci, s, a, d.g, o);
-      ox[a] = o[0];
-      oy[a] = o[1];
-      oz[a] = o[2];
-    }
-  }
-};
-
 inline int check_ready(const nh_context* ctx, const char* where, bool need_halo) {
   if (!ctx) return fail(NH_ERR_ARG, where, "null context");
   if (!ctx->c.geo_ready) return fail(NH_ERR_STATE, where, "geometry not uploaded");
@@ -185,8 +131,8 @@ int nh_laplacian(nh_context* ctx, const double* scl, double* out, void* stream) 
   if (int rc = nh::check_ready(ctx, where, false)) return rc;
   if (!scl || !out) return nh::fail(NH_ERR_ARG, where, "null field pointer");
   NHX x(ctx->c.device, stream, &ctx->c);
-  nh::OpLap f{ctx->c.d, ctx->c.G, scl, out};
-  x.run(f, nh::ncols(ctx->c.d));
+  nh::OpLapP f{ctx->c.d, ctx->c.G, scl, out};
+  x.run_pts(f, ctx->c.d);
   if (x.check(where)) return nh::fail(NH_ERR_DEVICE, where, x.message());
   return 0;
 }
@@ -197,8 +143,8 @@ int nh_divergence(nh_context* ctx, const double* vx, const double* vy, const dou
   if (int rc = nh::check_ready(ctx, where, false)) return rc;
   if (!vx || !vy || !vz || !out) return nh::fail(NH_ERR_ARG, where, "null field pointer");
   NHX x(ctx->c.device, stream, &ctx->c);
-  nh::OpDiv f{ctx->c.d, ctx->c.G, vx, vy, vz, out};
-  x.run(f, nh::ncols(ctx->c.d));
+  nh::OpDivP f{ctx->c.d, ctx->c.G, vx, vy, vz, out};
+  x.run_pts(f, ctx->c.d);
   if (x.check(where)) return nh::fail(NH_ERR_DEVICE, where, x.message());
   return 0;
 }
@@ -208,8 +154,8 @@ int nh_gradient(nh_context* ctx, const double* scl, double* gx, double* gy, doub
   if (int rc = nh::check_ready(ctx, where, false)) return rc;
   if (!scl || !gx || !gy || !gz) return nh::fail(NH_ERR_ARG, where, "null field pointer");
   NHX x(ctx->c.device, stream, &ctx->c);
-  nh::OpGrad f{ctx->c.d, ctx->c.G, scl, gx, gy, gz};
-  x.run(f, nh::ncols(ctx->c.d));
+  nh::OpGradP f{ctx->c.d, ctx->c.G, scl, gx, gy, gz};
+  x.run_pts(f, ctx->c.d);
   if (x.check(where)) return nh::fail(NH_ERR_DEVICE, where, x.message());
   return 0;
 }

@@ -115,6 +115,55 @@ NH_HD inline Cell cell_of(const Dims& d, long t) {
   return c;
 }
 
+// ----------------------------------------------------------------------------
+// point addressing: point kernels run on a 2-D grid, y = plane (region level
+// l*kall + k, then the 2*kall pole levels lp*kall + k), x = point in the plane
+// (j*g + i, or the pole point v).  The field address is plane*g*g + x
+// (regional) or R + plane_pole*6 + v.
+// ----------------------------------------------------------------------------
+struct Pt {
+  bool pole, inner;
+  int i, j, l, v, lp, k;
+  long a;       // field address of the point
+  long ks;      // level stride
+  long ci;      // regional column index (l*g + j)*g + i
+};
+
+NH_HD inline bool pt_of(const Dims& d, int plane, int x, Pt& p) {
+  const int g = d.g;
+  const int nrp = d.lall * d.kall;
+  if (plane < nrp) {
+    if (x >= g * g) return false;
+    p.pole = false;
+    p.l = plane / d.kall;
+    p.k = plane - p.l * d.kall;
+    p.j = x / g;
+    p.i = x - p.j * g;
+    p.inner = (p.i >= 1 && p.i <= g - 2 && p.j >= 1 && p.j <= g - 2);
+    p.ks = (long)g * g;
+    p.a = (long)plane * g * g + x;
+    p.ci = (long)p.l * g * g + x;
+    p.v = 0;
+    p.lp = 0;
+  } else {
+    if (x >= 6) return false;
+    const int pp = plane - nrp;
+    p.pole = true;
+    p.inner = false;
+    p.lp = pp / d.kall;
+    p.k = pp - p.lp * d.kall;
+    p.v = x;
+    p.ks = 6;
+    p.a = d.R + (long)pp * 6 + x;
+    p.ci = -1;
+    p.i = p.j = p.l = 0;
+  }
+  return true;
+}
+
+// the compute points of the horizontal operators: regional interior and pole centres
+NH_HD inline bool pt_compute(const Pt& p) { return p.pole ? p.v == 0 : p.inner; }
+
 // stencil offsets in column-base units: (0,0) (+1,0) (+1,+1) (0,+1) (-1,0) (-1,-1) (0,-1)
 NH_HD inline void stencil_offsets(int g, long off[7]) {
   off[0] = 0; off[1] = 1; off[2] = 1 + g; off[3] = g;

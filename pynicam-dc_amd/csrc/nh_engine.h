@@ -6,6 +6,8 @@
 //   void h2d(void* dst, const void* src, size_t bytes);   (synchronous)
 //   void d2d(void* dst, const void* src, size_t bytes);   (stream ordered)
 //   template <class F> void run(const F& f, long n);      (f(t) for t < n)
+//   template <class F> void run_pts(const F& f, const Dims& d);
+//                                                         (f(plane, x) on the point grid)
 //   int check(const char* what);                          (0 or error)
 // so that the same sequence drives the GPU build and the host test build.
 #pragma once
@@ -17,6 +19,7 @@
 
 #include "nh_step.h"
 #include "nh_tracer.h"
+#include "nh_point.h"
 #include "../../include/nicamhip.h"
 
 namespace nh {
@@ -49,7 +52,8 @@ enum WsField {
   W_QA = 85,              // 3  upwind face values (faces 0..2)
   W_QOUT = 88,            // 2  limiter outflow bounds
   W_RQ = 90,              // 1  tracer after the horizontal step
-  W_NFIELDS = 91
+  W_HV = 91,              // 3  horizontal divergences of the small-step momentum
+  W_NFIELDS = 94
 };
 
 struct Context {
@@ -217,18 +221,18 @@ inline long ncols(const Dims& d) { return d.nreg + 12; }
 template <class X>
 void divdamp(X& x, const Context& ctx, const double* vx, const double* vy, const double* vz, const double* w,
              double* out0, double* out1, double* out2) {
-  DivDamp1 a;
+  DivDamp1P a;
   a.d = ctx.d; a.c = ctx.c; a.G = ctx.G;
   a.rvx = vx; a.rvy = vy; a.rvz = vz; a.rw = w;
   for (int q = 0; q < 3; ++q) a.out[q] = ctx.field(W_DDT + q);
-  x.run(a, ncols(ctx.d));
+  x.run_pts(a, ctx.d);
   double* tmp[3] = {ctx.field(W_DDT), ctx.field(W_DDT + 1), ctx.field(W_DDT + 2)};
   halo(x, ctx, tmp, 3);
-  DivDamp2 b;
+  DivDamp2P b;
   b.d = ctx.d; b.c = ctx.c; b.G = ctx.G;
   for (int q = 0; q < 3; ++q) b.in[q] = tmp[q];
   b.out[0] = out0; b.out[1] = out1; b.out[2] = out2;
-  x.run(b, ncols(ctx.d));
+  x.run_pts(b, ctx.d);
 }
 
 // ----------------------------------------------------------------------------
@@ -351,38 +355,38 @@ int dyn_step(X& x, const Context& ctx, double* prog, double* progq, int nq, doub
   for (int nl = 0; nl < 3; ++nl) {
     // diagnostics and boundary values
     {
-      PrePost f{d, ctx.c, ctx.G, prog, rhogq, diag, eth};
-      x.run(f, nc);
+      PrePostP f{d, ctx.c, ctx.G, prog, rhogq, diag, eth};
+      x.run_pts(f, d);
     }
     {
-      AdvMom f{d, ctx.c, ctx.G, prog, diag, gadv};
-      x.run(f, nc);
+      AdvMomP f{d, ctx.c, ctx.G, prog, diag, gadv};
+      x.run_pts(f, d);
     }
     // hyperdiffusion: two passes with a halo fill after each
     {
-      HDiffPass<0> f;
+      HDiffPassP<0> f;
       f.d = d; f.c = ctx.c; f.G = ctx.G;
       f.in[0] = diag + I_VX * NF; f.in[1] = diag + I_VY * NF; f.in[2] = diag + I_VZ * NF;
       f.in[3] = diag + I_W * NF; f.in[4] = diag + I_TEM * NF; f.in[5] = prog;
       for (int q = 0; q < 6; ++q) f.out[q] = ctx.field(W_HDA + q);
       f.rhog = prog;
-      x.run(f, nc);
+      x.run_pts(f, d);
       halo_block(x, ctx, ctx.field(W_HDA), 6);
-      HDiffPass<1> f2;
+      HDiffPassP<1> f2;
       f2.d = d; f2.c = ctx.c; f2.G = ctx.G;
       for (int q = 0; q < 6; ++q) {
         f2.in[q] = ctx.field(W_HDA + q);
         f2.out[q] = ctx.field(W_HDB + q);
       }
       f2.rhog = prog;
-      x.run(f2, nc);
+      x.run_pts(f2, d);
       halo_block(x, ctx, ctx.field(W_HDB), 6);
     }
     // divergence damping of the large-step momentum
     divdamp(x, ctx, prog + 1 * NF, prog + 2 * NF, prog + 3 * NF, prog + 4 * NF,
             ctx.field(W_DD), ctx.field(W_DD + 1), ctx.field(W_DD + 2));
     {
-      ViPath0 f;
+      ViPath0P f;
       f.d = d; f.c = ctx.c; f.G = ctx.G;
       f.dt = dts;
       f.prog = prog; f.diag = diag; f.eth = eth; f.gadv = gadv;
@@ -391,11 +395,11 @@ int dyn_step(X& x, const Context& ctx, double* prog, double* progq, int nq, doub
       f.gT = gT;
       f.Mc = ctx.field(W_MC); f.Mu = ctx.field(W_MU); f.Ml = ctx.field(W_ML);
       f.getot = getot; f.frhog = frhog;
-      x.run(f, nc);
+      x.run_pts(f, d);
     }
     {
-      EthTotPM f{d, ctx.c, ctx.G, prog, eth, ethtot, pm};
-      x.run(f, nc);
+      EthTotPMP f{d, ctx.c, ctx.G, prog, eth, ethtot, pm};
+      x.run_pts(f, d);
     }
     // split state: ps[0..5], then path1 momentum dv[0..2] and vi_main output we[0..2]
     double* ps[6];
@@ -413,7 +417,7 @@ int dyn_step(X& x, const Context& ctx, double* prog, double* progq, int nq, doub
       double* ddv = ctx.field(W_DDV);
       divdamp(x, ctx, ps[1], ps[2], ps[3], ps[4], ddv, ddv + NF, ddv + 2 * NF);
       {
-        Path1 f;
+        Path1P f;
         f.d = d; f.c = ctx.c; f.G = ctx.G;
         f.dt = dts; f.rweight = rweight;
         f.prog = prog;
@@ -422,18 +426,29 @@ int dyn_step(X& x, const Context& ctx, double* prog, double* progq, int nq, doub
         for (int q = 0; q < 3; ++q) f.dd[q] = ddv + q * NF;
         for (int q = 0; q < 3; ++q) f.dv[q] = dv[q];
         f.pm = pm;
-        x.run(f, nc);
+        x.run_pts(f, d);
       }
       halo(x, ctx, dv, 3);
+      double* hv[3] = {ctx.field(W_HV), ctx.field(W_HV + 1), ctx.field(W_HV + 2)};
       {
-        ViMain f;
+        ViMainH f;
+        f.d = d; f.c = ctx.c; f.G = ctx.G;
+        f.prog = prog;
+        for (int q = 0; q < 3; ++q) f.dv[q] = dv[q];
+        f.eth = eth; f.ethtot = ethtot; f.gT = gT;
+        for (int q = 0; q < 3; ++q) f.hv[q] = hv[q];
+        x.run_pts(f, d);
+      }
+      {
+        ViMainV f;
         f.d = d; f.c = ctx.c; f.G = ctx.G;
         f.dt = dts; f.rweight = rweight;
         f.prog = prog;
         for (int q = 0; q < 6; ++q) f.ps[q] = ps[q];
         for (int q = 0; q < 3; ++q) f.dv[q] = dv[q];
-        f.eth = eth; f.ethtot = ethtot; f.gT = gT; f.getot = getot;
+        f.ethtot = ethtot; f.gT = gT; f.getot = getot;
         f.Mc = ctx.field(W_MC); f.Mu = ctx.field(W_MU); f.Ml = ctx.field(W_ML);
+        for (int q = 0; q < 3; ++q) f.hv[q] = hv[q];
         for (int q = 0; q < 3; ++q) f.we[q] = we[q];
         f.pm = pm;
         x.run(f, nc);
@@ -445,11 +460,11 @@ int dyn_step(X& x, const Context& ctx, double* prog, double* progq, int nq, doub
       we[0] = old[0]; dv[0] = old[1]; dv[1] = old[2]; dv[2] = old[3]; we[1] = old[4]; we[2] = old[5];
     }
     {
-      ProgUpdate f;
+      ProgUpdateP f;
       f.d = d; f.c = ctx.c; f.G = ctx.G;
       f.prog = prog;
       for (int q = 0; q < 6; ++q) f.ps[q] = ps[q];
-      x.run(f, nc);
+      x.run_pts(f, d);
     }
     halo_block(x, ctx, pm, 5);
     if (nl < 2) halo_block(x, ctx, prog, 6);

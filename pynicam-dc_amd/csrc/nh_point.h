@@ -1,0 +1,853 @@
+// Point kernels of the HEVI step: one thread per (column, level).
+//
+// Horizontal stencils have no vertical recurrence, so they run one thread per
+// point on the 2-D grid of nh_columns.h (plane = region level, x = point in
+// the plane).  Vertical couplings that reach only k-2..k+1 (half-level
+// interpolation, boundary rows, the tridiagonal coefficients) are recomputed
+// by the thread that needs them, which keeps every kernel free of per-thread
+// column arrays.  Only the tridiagonal solve (ViMainV) stays a column kernel.
+// The arithmetic of every value follows the reference kernels cited per
+// functor, in the reference's operation order.
+#pragma once
+
+#include "nh_step.h"
+
+namespace nh {
+
+NH_HD inline Cell cell_of_pt(const Pt& p) {
+  Cell c;
+  c.pole = p.pole;
+  c.inner = p.inner;
+  c.i = p.i; c.j = p.j; c.l = p.l; c.v = p.v; c.lp = p.lp;
+  c.ks = p.ks;
+  c.base = p.a - (long)p.k * p.ks;
+  c.ci = p.ci;
+  return c;
+}
+
+// ----------------------------------------------------------------------------
+// single operators (tests and callers of the C-ABI)
+// ----------------------------------------------------------------------------
+struct OpLapP {
+  static constexpr const char* kName = "oprt_laplacian";
+  Dims d; Geo G;
+  const double* s;
+  double* o;
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    o[p.a] = p.pole ? lap_pl(G, p.lp, s, p.a) : lap7(G, d.nreg, p.ci, s, p.a, d.g);
+  }
+};
+
+struct OpDivP {
+  static constexpr const char* kName = "oprt_divergence";
+  Dims d; Geo G;
+  const double *vx, *vy, *vz;
+  double* o;
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    const long a = p.a;
+    if (p.pole) {
+      o[a] = div_pl(G, p.lp, [&](int v) { return vx[a + v]; }, [&](int v) { return vy[a + v]; },
+                    [&](int v) { return vz[a + v]; });
+    } else {
+      long off[7];
+      stencil_offsets(d.g, off);
+      o[a] = div7(G, d.nreg, p.ci, [&](int m) { return vx[a + off[m]]; }, [&](int m) { return vy[a + off[m]]; },
+                  [&](int m) { return vz[a + off[m]]; });
+    }
+  }
+};
+
+struct OpGradP {
+  static constexpr const char* kName = "oprt_gradient";
+  Dims d; Geo G;
+  const double* s;
+  double *ox, *oy, *oz;
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    double o[3];
+    if (p.pole) grad_pl(G, p.lp, s, p.a, o);
+    else grad7(G, d.nreg, p.ci, s, p.a, d.g, o);
+    ox[p.a] = o[0];
+    oy[p.a] = o[1];
+    oz[p.a] = o[2];
+  }
+};
+
+// ----------------------------------------------------------------------------
+// Pre_Post (kernels/diag.py:53-143, mod_bndcnd.py:153-238, kernels/thrmdyn.py:56-71)
+// ----------------------------------------------------------------------------
+struct PrePostP {
+  static constexpr const char* kName = "prepost";
+  Dims d; Consts c; Geo G;
+  double* prog;
+  const double* rhogq;
+  double* diag;
+  double* eth;
+  struct Th { double rho, ein, tem, pre; };
+  NH_HD Th thermo(long a) const {
+    const long NF = d.NF;
+    Th t;
+    const double rhog = prog[a];
+    t.rho = rhog / 1.0;
+    t.ein = prog[5 * NF + a] / rhog;
+    const double q = rhogq[a] / rhog;
+    const double qd = 1.0 - q;
+    const double cv = q * c.CVW_qv + qd * c.CVdry;
+    t.tem = t.ein / cv;
+    t.pre = t.rho * t.tem * (qd * c.Rdry + q * c.Rvap);
+    return t;
+  }
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p)) return;
+    const long NF = d.NF, a = p.a, ks = p.ks;
+    const int kmin = d.kmin, kmax = d.kmax, k = p.k;
+    double* rg = prog;
+    double* rw = prog + 4 * NF;
+    double* pre = diag + I_PRE * NF;
+    double* tem = diag + I_TEM * NF;
+    double* w = diag + I_W * NF;
+    if (k >= kmin && k <= kmax) {
+      Th t = thermo(a);
+      const double rhog = rg[a];
+      diag[I_VX * NF + a] = prog[1 * NF + a] / rhog;
+      diag[I_VY * NF + a] = prog[2 * NF + a] / rhog;
+      diag[I_VZ * NF + a] = prog[3 * NF + a] / rhog;
+      tem[a] = t.tem;
+      pre[a] = t.pre;
+      if (k > kmin) {
+        w[a] = rw[a] / (G.c2w0[k] * rg[a] + G.c2w1[k] * rg[a - ks]);
+      } else {
+        // bottom: rigid lid, with the density of the ghost row after the boundary condition
+        const Th t1 = thermo(a + ks);
+        const double pre_b = t1.pre - t.rho * (G.phi[kmin - 1] - G.phi[kmin + 1]);
+        const double rho_b = pre_b / (c.Rdry * t.tem);
+        rw[a] = 0.0;
+        w[a] = rw[a] / (G.c2w0[kmin] * rg[a] + G.c2w1[kmin] * (rho_b * 1.0));
+      }
+      eth[a] = t.ein + t.pre / t.rho;
+      return;
+    }
+    // ghost rows: TEM thermodynamic condition, free-slip top, rigid bottom
+    const bool top = (k == kmax + 1);
+    const long an = top ? a - ks : a + ks;            // nearest physical level
+    const long an2 = top ? a - 2 * ks : a + 2 * ks;    // next one
+    const Th tn = thermo(an);
+    const Th tn2 = thermo(an2);
+    const double tem_g = tn.tem;
+    const double pre_g = top ? tn2.pre - tn.rho * (G.phi[kmax + 1] - G.phi[kmax - 1])
+                             : tn2.pre - tn.rho * (G.phi[kmin - 1] - G.phi[kmin + 1]);
+    const double rho_g = pre_g / (c.Rdry * tem_g);
+    const double rg_g = rho_g * 1.0;
+    const double ein_g = c.CVdry * tem_g;
+    tem[a] = tem_g;
+    pre[a] = pre_g;
+    rg[a] = rg_g;
+    prog[5 * NF + a] = rg_g * ein_g;
+    const double sgn = top ? 1.0 : -1.0;
+    for (int q = 0; q < 3; ++q) {
+      const double v = sgn * (prog[(1 + q) * NF + an] / rg[an]) * rg_g;
+      prog[(1 + q) * NF + a] = v;
+      diag[(I_VX + q) * NF + a] = v / rg_g;
+    }
+    if (top) {
+      rw[a] = -0.0;
+      w[a] = rw[a] / (G.c2w0[kmax + 1] * rg_g + G.c2w1[kmax + 1] * rg[an]);
+    } else {
+      rw[a] = 0.0;
+      w[a] = 0.0;
+    }
+    eth[a] = ein_g + pre_g / rho_g;
+  }
+};
+
+// ----------------------------------------------------------------------------
+// momentum advection, Coriolis, projection (mod_src.py:96-497, kernels/advconvmom.py)
+// out[0..2]: momentum tendencies; out[3]: the radial part gwc at full levels
+// (the half-level w tendency c2w0*gwc(k) + c2w1*gwc(k-1) is formed by its user)
+// ----------------------------------------------------------------------------
+struct AdvMomP {
+  static constexpr const char* kName = "adv_mom";
+  Dims d; Consts c; Geo G;
+  const double* prog;
+  const double* diag;
+  double* out;
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p)) return;
+    const long NF = d.NF, a = p.a, ks = p.ks;
+    const int kmin = d.kmin, kmax = d.kmax, k = p.k;
+    if (!pt_compute(p) || k < kmin || k > kmax) {
+      for (int q = 0; q < 4; ++q) out[q * NF + a] = 0.0;
+      return;
+    }
+    const double R = c.RADIUS;
+    const double* rvx = prog + 1 * NF;
+    const double* rvy = prog + 2 * NF;
+    const double* rvz = prog + 3 * NF;
+    const double* rw = prog + 4 * NF;
+    const double* rg = prog;
+    const double* vxd = diag + I_VX * NF;
+    const double* vyd = diag + I_VY * NF;
+    const double* vzd = diag + I_VZ * NF;
+    const double* wd = diag + I_W * NF;
+    long nb[7];
+    double px[7], py[7], pz[7];
+    const int nn = p.pole ? 6 : 7;
+    if (p.pole) {
+      for (int v = 0; v < 6; ++v) {
+        nb[v] = a + v;
+        px[v] = G.gx_pl[(v * 2 + p.lp) * 3 + 0];
+        py[v] = G.gx_pl[(v * 2 + p.lp) * 3 + 1];
+        pz[v] = G.gx_pl[(v * 2 + p.lp) * 3 + 2];
+      }
+    } else {
+      long off[7];
+      stencil_offsets(d.g, off);
+      for (int m = 0; m < 7; ++m) {
+        nb[m] = a + off[m];
+        const long cim = p.ci + off[m];
+        px[m] = G.gx[0 * d.nreg + cim];
+        py[m] = G.gx[1 * d.nreg + cim];
+        pz[m] = G.gx[2 * d.nreg + cim];
+      }
+    }
+    // merged velocity v + w_full r/|r| of neighbour m at level kk
+    auto vv3 = [&](int m, int kk, double* o) {
+      const long b = nb[m] + (long)(kk - k) * ks;
+      const double wc = G.cfact[kk] * wd[b + ks] + G.dfact[kk] * wd[b];
+      o[0] = vxd[b] + wc * px[m] / R;
+      o[1] = vyd[b] + wc * py[m] / R;
+      o[2] = vzd[b] + wc * pz[m] / R;
+    };
+    double vvm[7][3];
+    for (int m = 0; m < nn; ++m) vv3(m, k, vvm[m]);
+    double div[3];
+    for (int q = 0; q < 3; ++q) {
+      if (p.pole) {
+        div[q] = div_pl(G, p.lp, [&](int v) { return rvx[nb[v]] * vvm[v][q] * 1.0; },
+                        [&](int v) { return rvy[nb[v]] * vvm[v][q] * 1.0; },
+                        [&](int v) { return rvz[nb[v]] * vvm[v][q] * 1.0; });
+      } else {
+        div[q] = div7(G, d.nreg, p.ci, [&](int m) { return rvx[nb[m]] * vvm[m][q] * 1.0; },
+                      [&](int m) { return rvy[nb[m]] * vvm[m][q] * 1.0; },
+                      [&](int m) { return rvz[nb[m]] * vvm[m][q] * 1.0; });
+      }
+    }
+    // vertical fluxes at the half levels k+1 and k (zero at kmin and kmax+1)
+    double hi[3] = {0.0, 0.0, 0.0}, lo[3] = {0.0, 0.0, 0.0};
+    if (k + 1 <= kmax) {
+      double up[3];
+      vv3(0, k + 1, up);
+      const double wup = rw[a + ks];
+      for (int q = 0; q < 3; ++q) hi[q] = 0.0 * 1.0 + wup * (G.afact[k + 1] * up[q] + G.bfact[k + 1] * vvm[0][q]) * 1.0;
+    }
+    if (k >= kmin + 1) {
+      double dn[3];
+      vv3(0, k - 1, dn);
+      const double w0 = rw[a];
+      for (int q = 0; q < 3; ++q) lo[q] = 0.0 * 1.0 + w0 * (G.afact[k] * vvm[0][q] + G.bfact[k] * dn[q]) * 1.0;
+    }
+    double dvv[3];
+    for (int q = 0; q < 3; ++q) dvv[q] = -div[q] - (hi[q] - lo[q]) * G.rdgz[k];
+    const double gxc = px[0] / R, gyc = py[0] / R, gzc = pz[0] / R;
+    const double rh = rg[a];
+    const double dx = dvv[0] - (-2.0 * rh * (c.OHM * vvm[0][1]));
+    const double dy = dvv[1] - (2.0 * rh * (c.OHM * vvm[0][0]));
+    const double dz = dvv[2];
+    const double prd = dx * gxc + dy * gyc + dz * gzc;
+    out[0 * NF + a] = dx - prd * gxc;
+    out[1 * NF + a] = dy - prd * gyc;
+    out[2 * NF + a] = dz - prd * gzc;
+    out[3 * NF + a] = prd * c.ALPHA;
+  }
+};
+
+// ----------------------------------------------------------------------------
+// hyperdiffusion passes (mod_numfilter.py:1125-1264)
+// ----------------------------------------------------------------------------
+template <int LAST>
+struct HDiffPassP {
+  static constexpr const char* kName = LAST ? "hdiff_pass2" : "hdiff_pass1";
+  Dims d; Consts c; Geo G;
+  const double* in[6];
+  double* out[6];
+  const double* rhog;
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    const long a = p.a;
+    const Cell cl = cell_of_pt(p);
+    for (int q = 0; q < 6; ++q) {
+      double r;
+      if (LAST && q >= 4) {
+        const double fac = (q == 4) ? c.CVdry : c.HDIFF_FACT_RHO;
+        const double* rg = rhog;
+        const double gam = c.GAMMA_H;
+        auto kh = [rg, fac, gam](long xx) { return rg[xx] * fac * gam; };
+        r = p.pole ? diffusion_pl(G, p.lp, in[q], kh, a) : diffusion7(G, d, cl, in[q], kh, a);
+      } else {
+        r = p.pole ? lap_pl(G, p.lp, in[q], a) : lap7(G, d.nreg, p.ci, in[q], a, d.g);
+      }
+      out[q][a] = -r;
+    }
+  }
+};
+
+// ----------------------------------------------------------------------------
+// 3-D divergence damping (kernels/oprt3ddivdamp.py, oprtdivdamp.py, divdamppostcomm.py)
+// ----------------------------------------------------------------------------
+struct DivDamp1P {
+  static constexpr const char* kName = "divdamp_3d";
+  Dims d; Consts c; Geo G;
+  const double *rvx, *rvy, *rvz, *rw;
+  double* out[3];
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    const int kmin = d.kmin, kmax = d.kmax, g = d.g, k = p.k;
+    const long ks = p.ks, a = p.a;
+    if (k < kmin || k > kmax) {
+      for (int q = 0; q < 3; ++q) out[q][a] = 0.0;
+      return;
+    }
+    auto rwv = [&](long aa, int kk) { return (kk >= kmin + 1 && kk <= kmax) ? (0.0 * 1.0 + rw[aa] * 1.0) : 0.0; };
+    if (p.pole) {
+      double sclt[5];
+      const double* vv[3] = {rvx, rvy, rvz};
+      for (int v = 1; v <= 5; ++v) {
+        const int vp = (v == 5) ? 1 : v + 1;
+        double s = 0.0;
+        for (int dd = 0; dd < 3; ++dd) {
+          const double* cI = G.cintp_pl + ((v * 2 + p.lp) * 3 + dd) * 3;
+          const double t0 = cI[0] * (vv[dd][a] * 1.0), t1 = cI[1] * (vv[dd][a + v] * 1.0),
+                       t2 = cI[2] * (vv[dd][a + vp] * 1.0);
+          s = (dd == 0) ? (t0 + t1 + t2) : (s + t0 + t1 + t2);
+        }
+        const double up = (rwv(a + ks, k + 1) + rwv(a + v + ks, k + 1) + rwv(a + vp + ks, k + 1));
+        const double dn = (rwv(a, k) + rwv(a + v, k) + rwv(a + vp, k));
+        sclt[v - 1] = s + (up - dn) / 3.0 * G.rdgz[k];
+      }
+      for (int dd = 0; dd < 3; ++dd) {
+        double row = 0.0;
+        for (int v = 1; v <= 5; ++v) {
+          const int vm = (v == 1) ? 5 : v - 1;
+          const double cb = G.cdiff_pl[(v * 2 + p.lp) * 3 + dd] * (sclt[vm - 1] + sclt[v - 1]);
+          row = (v == 1) ? cb : row + cb;
+        }
+        out[dd][a] = row;
+      }
+      return;
+    }
+    const long nreg = d.nreg, ci = p.ci;
+    const long co[6] = {0, 0, -1, -1 - (long)g, -1 - (long)g, -(long)g};
+    const int ty[6] = {0, 1, 0, 1, 0, 1};
+    double T[6];
+    for (int n = 0; n < 6; ++n) {
+      const long aa = a + co[n];
+      const long cj = ci + co[n];
+      const long o1 = (ty[n] == 0) ? 1 : 1 + g;
+      const long o2 = (ty[n] == 0) ? 1 + g : g;
+      const double s = tri_div(G, nreg, cj, ty[n], rvx, rvy, rvz, aa, g, 1.0);
+      const double up = (rwv(aa + ks, k + 1) + rwv(aa + o1 + ks, k + 1) + rwv(aa + o2 + ks, k + 1));
+      const double dn = (rwv(aa, k) + rwv(aa + o1, k) + rwv(aa + o2, k));
+      T[n] = s + (up - dn) / 3.0 * G.rdgz[k];
+    }
+    if (p.i == 1 && p.j == 1) {
+      const double m0 = G.sgp[p.l] ? 0.0 : 1.0, m1 = G.sgp[p.l] ? 1.0 : 0.0;
+      T[4] = T[4] * m0 + T[5] * m1;
+    }
+    for (int dd = 0; dd < 3; ++dd) out[dd][a] = hex_sum(G, nreg, ci, dd, T);
+  }
+};
+
+struct DivDamp2P {
+  static constexpr const char* kName = "divdamp_post";
+  Dims d; Consts c; Geo G;
+  const double* in[3];
+  double* out[3];
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p)) return;
+    const long a = p.a;
+    const int kmin = d.kmin, kmax = d.kmax, g = d.g, k = p.k;
+    if (!pt_compute(p)) {
+      for (int q = 0; q < 3; ++q) out[q][a] = 0.0;
+      return;
+    }
+    const double R = c.RADIUS;
+    double o[3];
+    if (k < kmin || k > kmax) {
+      o[0] = o[1] = o[2] = 0.0;
+    } else if (p.pole) {
+      double sclt[5];
+      for (int v = 1; v <= 5; ++v) {
+        const int vp = (v == 5) ? 1 : v + 1;
+        double s = 0.0;
+        for (int dd = 0; dd < 3; ++dd) {
+          const double* cI = G.cintp_pl + ((v * 2 + p.lp) * 3 + dd) * 3;
+          const double t0 = cI[0] * (-in[dd][a]), t1 = cI[1] * (-in[dd][a + v]), t2 = cI[2] * (-in[dd][a + vp]);
+          s = (dd == 0) ? (t0 + t1 + t2) : (s + t0 + t1 + t2);
+        }
+        sclt[v - 1] = s;
+      }
+      for (int dd = 0; dd < 3; ++dd) {
+        double row = 0.0;
+        for (int v = 1; v <= 5; ++v) {
+          const int vm = (v == 1) ? 5 : v - 1;
+          const double cb = G.cdiff_pl[(v * 2 + p.lp) * 3 + dd] * (sclt[vm - 1] + sclt[v - 1]);
+          row = (v == 1) ? cb : row + cb;
+        }
+        o[dd] = row;
+      }
+    } else {
+      const long nreg = d.nreg, ci = p.ci;
+      const long co[6] = {0, 0, -1, -1 - (long)g, -1 - (long)g, -(long)g};
+      const int ty[6] = {0, 1, 0, 1, 0, 1};
+      double T[6];
+      for (int n = 0; n < 6; ++n) T[n] = tri_div(G, nreg, ci + co[n], ty[n], in[0], in[1], in[2], a + co[n], g, -1.0);
+      for (int dd = 0; dd < 3; ++dd) o[dd] = hex_sum(G, nreg, ci, dd, T);
+    }
+    const double gd[3] = {c.ALPHA_D * o[0], c.ALPHA_D * o[1], c.ALPHA_D * o[2]};
+    if (p.pole) {
+      const double gx = G.gx_pl[p.lp * 3 + 0], gy = G.gx_pl[p.lp * 3 + 1], gz = G.gx_pl[p.lp * 3 + 2];
+      const double prd = gd[0] * gx / R + gd[1] * gy / R + gd[2] * gz / R;
+      out[0][a] = gd[0] - prd * gx / R;
+      out[1][a] = gd[1] - prd * gy / R;
+      out[2][a] = gd[2] - prd * gz / R;
+    } else {
+      const double gx = G.gx[p.ci], gy = G.gx[d.nreg + p.ci], gz = G.gx[2 * d.nreg + p.ci];
+      const double prd = (gd[0] * gx + gd[1] * gy + gd[2] * gz) / R;
+      out[0][a] = gd[0] - prd * gx / R;
+      out[1][a] = gd[1] - prd * gy / R;
+      out[2][a] = gd[2] - prd * gz / R;
+    }
+  }
+};
+
+// ----------------------------------------------------------------------------
+// large-step source terms of the small steps and the tridiagonal coefficients
+// (mod_vi.py:132-881, kernels/vimatrix.py:67-136); column quantities of levels
+// k-1..k+1 are recomputed per point
+// ----------------------------------------------------------------------------
+struct ViPath0P {
+  static constexpr const char* kName = "vi_path0";
+  Dims d; Consts c; Geo G;
+  double dt;
+  const double* prog;
+  const double* diag;
+  const double* eth;
+  const double* gadv;   // 3 momentum tendencies + gwc at full levels
+  const double* vt[6];
+  const double* dd[3];
+  double* gT;
+  double* Mc; double* Mu; double* Ml;
+  double* getot;
+  double* frhog;
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    const long NF = d.NF, ks = p.ks, a = p.a;
+    const int kmin = d.kmin, kmax = d.kmax, k = p.k;
+    const double R = c.RADIUS;
+    const double* rg = prog;
+    const double* rvx = prog + NF;
+    const double* rvy = prog + 2 * NF;
+    const double* rvz = prog + 3 * NF;
+    const double* rw = prog + 4 * NF;
+    const double* pre = diag + I_PRE * NF;
+    const long b = a - (long)k * ks;     // column base
+    auto at = [&](int kk) { return b + (long)kk * ks; };
+    auto rhog_h = [&](int kk) {
+      const int k2 = (kk < kmin) ? kmin : kk;
+      return G.c2w0[k2] * rg[at(k2)] + G.c2w1[k2] * rg[at(k2 - 1)];
+    };
+    auto eth_h = [&](int kk) {
+      const int k2 = (kk < kmin) ? kmin : kk;
+      return G.afact[k2] * eth[at(k2)] + G.bfact[k2] * eth[at(k2 - 1)];
+    };
+    auto dpgw = [&](int kk) {
+      return (kk >= kmin + 1 && kk <= kmax) ? 1.0 * (pre[at(kk)] * 1.0 - pre[at(kk - 1)] * 1.0) * G.rdgzh[kk] : 0.0;
+    };
+    auto dbuo = [&](int kk) {
+      return (kk >= kmin + 1 && kk <= kmax)
+                 ? -c.GRAV * (G.c2w0[kk] * (rg[at(kk)] / 1.0 * 1.0) + G.c2w1[kk] * (rg[at(kk - 1)] / 1.0 * 1.0))
+                 : 0.0;
+    };
+    auto gzt = [&](int kk) { return c.GRAV - (dpgw(kk) - dbuo(kk)) / rhog_h(kk); };
+    auto pwh = [&](int kk) { return -gzt(kk) * rw[at(kk)]; };
+    double gx, gy, gz;
+    if (p.pole) {
+      gx = G.gx_pl[p.lp * 3 + 0]; gy = G.gx_pl[p.lp * 3 + 1]; gz = G.gx_pl[p.lp * 3 + 2];
+    } else {
+      gx = G.gx[p.ci]; gy = G.gx[d.nreg + p.ci]; gz = G.gx[2 * d.nreg + p.ci];
+    }
+    const double Kh = c.GAMMA_H;
+    // hyperdiffusion tendency + momentum advection = large-step tendency g0
+    const double rgk = rg[a];
+    double fvx = -(vt[0][a] * Kh + 0.0) * rgk;
+    double fvy = -(vt[1][a] * Kh + 0.0) * rgk;
+    double fvz = -(vt[2][a] * Kh + 0.0) * rgk;
+    const double rgh = (k >= kmin) ? G.c2w0[k] * rgk + G.c2w1[k] * rg[a - ks] : 0.0;
+    const double fw = -(vt[3][a] * Kh + 0.0) * rgh;
+    const double fe = -(vt[4][a] + 0.0);
+    const double fr = -(vt[5][a] + 0.0);
+    if (p.pole) {
+      const double prd = fvx * gx / R + fvy * gy / R + fvz * gz / R;
+      fvx = fvx - prd * gx / R; fvy = fvy - prd * gy / R; fvz = fvz - prd * gz / R;
+    } else {
+      const double prd = (fvx * gx + fvy * gy + fvz * gz) / R;
+      fvx = fvx - prd * gx / R; fvy = fvy - prd * gy / R; fvz = fvz - prd * gz / R;
+    }
+    const double gadv_w = (k > kmin && k <= kmax) ? (G.c2w0[k] * gadv[3 * NF + a] + G.c2w1[k] * gadv[3 * NF + a - ks]) : 0.0;
+    double g0[6];
+    g0[0] = 0.0 + fr;
+    g0[1] = gadv[0 * NF + a] + fvx;
+    g0[2] = gadv[1 * NF + a] + fvy;
+    g0[3] = gadv[2 * NF + a] + fvz;
+    g0[4] = gadv_w + fw;
+    g0[5] = 0.0 + fe;
+    getot[a] = g0[5];
+    frhog[a] = g0[0];
+    double drhog = 0.0, drhoge = 0.0, pw = 0.0;
+    double dpg[3] = {0.0, 0.0, 0.0};
+    if (k >= kmin && k <= kmax) {
+      const Cell cl = cell_of_pt(p);
+      long off[7];
+      stencil_offsets(d.g, off);
+      auto nbr = [&](int m) { return p.pole ? a + m : a + off[m]; };
+      const double hd = hdiv(G, d, cl, [&](int m) { return rvx[nbr(m)] * 1.0; },
+                             [&](int m) { return rvy[nbr(m)] * 1.0; }, [&](int m) { return rvz[nbr(m)] * 1.0; });
+      const double vmh_hi = (k + 1 <= kmax) ? 0.0 * 1.0 + 1.0 * rw[a + ks] * 1.0 : 0.0;
+      const double vmh_lo = (k >= kmin + 1) ? 0.0 * 1.0 + 1.0 * rw[a] * 1.0 : 0.0;
+      drhog = -hd - (vmh_hi - vmh_lo) * G.rdgz[k];
+      const double he = hdiv(G, d, cl, [&](int m) { return rvx[nbr(m)] * eth[nbr(m)] * 1.0; },
+                             [&](int m) { return rvy[nbr(m)] * eth[nbr(m)] * 1.0; },
+                             [&](int m) { return rvz[nbr(m)] * eth[nbr(m)] * 1.0; });
+      const double ws_hi = (k + 1 <= kmax)
+          ? 0.0 * 1.0 + 1.0 * (rw[a + ks] * (G.afact[k + 1] * eth[a + ks] + G.bfact[k + 1] * eth[a])) * 1.0 : 0.0;
+      const double ws_lo = (k >= kmin + 1)
+          ? 0.0 * 1.0 + 1.0 * (rw[a] * (G.afact[k] * eth[a] + G.bfact[k] * eth[a - ks])) * 1.0 : 0.0;
+      drhoge = -he - (ws_hi - ws_lo) * G.rdgz[k];
+      presgrad_h(G, d, cl, pre, k, R, dpg);
+      pw = diag[I_VX * NF + a] * dpg[0] + diag[I_VY * NF + a] * dpg[1] + diag[I_VZ * NF + a] * dpg[2]
+         + G.w2c0[k] * pwh(k + 1) + G.w2c1[k] * pwh(k);
+    }
+    gT[0 * NF + a] = g0[0] + drhog;
+    gT[1 * NF + a] = g0[1] - dpg[0] + dd[0][a] + 0.0;
+    gT[2 * NF + a] = g0[2] - dpg[1] + dd[1][a] + 0.0;
+    gT[3 * NF + a] = g0[3] - dpg[2] + dd[2][a] + 0.0;
+    gT[4 * NF + a] = g0[4] + 0.0 * c.ALPHA - dpgw(k) + dbuo(k);
+    gT[5 * NF + a] = g0[5] + drhoge + pw;
+    if (k >= kmin + 1 && k <= kmax) {
+      const double GCVovR = c.GRAV * c.CVdry / c.Rdry;
+      const double ACVovRt2 = c.ALPHA * c.CVdry / c.Rdry / (dt * dt);
+      const double Ca = (1.0 * G.rdgz[k] + 1.0 * G.rdgz[k - 1]) * 1.0;
+      const double Cb = G.dfact[k] - G.cfact[k - 1];
+      const double Cu1 = 1.0 * G.rdgz[k] * 1.0;
+      const double Cu2 = 1.0 * 1.0 * GCVovR;
+      const double Cl1 = 1.0 * G.rdgz[k] * 1.0;
+      const double Cl2 = 1.0 * 1.0 * GCVovR;
+      Mc[a] = ACVovRt2 / 1.0 + G.rdgzh[k] * (Ca * eth_h(k) - Cb * (gzt(k) + GCVovR));
+      Mu[a] = -G.rdgzh[k] * (Cu1 * eth_h(k + 1) + G.cfact[k] * (gzt(k + 1) + Cu2));
+      Ml[a] = -G.rdgzh[k] * (Cl1 * eth_h(k - 1) - G.dfact[k - 1] * (gzt(k - 1) + Cl2));
+    }
+  }
+};
+
+// total specific energy of the large-step state and the start of the time-mean fluxes
+struct EthTotPMP {
+  static constexpr const char* kName = "ethtot_mean";
+  Dims d; Consts c; Geo G;
+  const double* prog;
+  const double* eth;
+  double* ethtot;
+  double* pm;
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p)) return;
+    const long NF = d.NF, a = p.a, ks = p.ks;
+    const long b = a - (long)p.k * ks;
+    auto get = [&](int kk, int q) { return prog[q * NF + b + (long)kk * ks]; };
+    const double kin0 = rhogkin_k(G, d.kmin, d.kmax, p.k, get);
+    ethtot[a] = eth[a] + kin0 / prog[a] + G.phi[p.k];
+    for (int q = 0; q < 5; ++q) pm[q * NF + a] = prog[q * NF + a];
+  }
+};
+
+// ----------------------------------------------------------------------------
+// small step path1 (kernels/vipath1.py:68-160); the ghost rows recompute the
+// nearest physical level
+// ----------------------------------------------------------------------------
+struct Path1P {
+  static constexpr const char* kName = "vi_path1";
+  Dims d; Consts c; Geo G;
+  double dt, rweight;
+  const double* prog;
+  const double* ps[6];
+  const double* gT;
+  const double* dd[3];
+  double* dv[3];
+  double* pm;
+  NH_HD void level(const Pt& p, int k, long a, double out[3]) const {
+    const long NF = d.NF, ks = p.ks;
+    const int kmin = d.kmin;
+    const double RovCV = c.Rdry / c.CVdry;
+    const double R = c.RADIUS;
+    const double* pse = ps[I_RHOGE];
+    const int kk = (k == kmin) ? kmin + 1 : k;
+    const long ak = a + (long)(kk - k) * ks;
+    double gr[3];
+    if (p.pole) {
+      for (int dd2 = 0; dd2 < 3; ++dd2) {
+        double acc = 0.0;
+        for (int v = 0; v < 6; ++v) {
+          const double pv = G.cgrad_pl[(v * 2 + p.lp) * 3 + dd2] * ((pse[ak + v] * RovCV) * 1.0 * 1.0);
+          acc = (v == 0) ? pv : acc + pv;
+        }
+        gr[dd2] = acc;
+      }
+    } else {
+      long off[7];
+      stencil_offsets(d.g, off);
+      for (int dd2 = 0; dd2 < 3; ++dd2) {
+        double acc = rc(G.cgrad, d.nreg, dd2 * 7, p.ci) * ((pse[ak] * RovCV) * 1.0);
+        for (int m = 1; m < 7; ++m) acc = acc + rc(G.cgrad, d.nreg, dd2 * 7 + m, p.ci) * ((pse[ak + off[m]] * RovCV) * 1.0);
+        gr[dd2] = acc;
+      }
+    }
+    for (int q = 0; q < 3; ++q) gr[q] = gr[q] + 0.0;
+    double Pg[3];
+    if (p.pole) {
+      const double gx = G.gx_pl[p.lp * 3 + 0], gy = G.gx_pl[p.lp * 3 + 1], gz = G.gx_pl[p.lp * 3 + 2];
+      const double prd = gr[0] * gx / R + gr[1] * gy / R + gr[2] * gz / R;
+      Pg[0] = gr[0] - prd * gx / R; Pg[1] = gr[1] - prd * gy / R; Pg[2] = gr[2] - prd * gz / R;
+    } else {
+      const double gx = G.gx[p.ci], gy = G.gx[d.nreg + p.ci], gz = G.gx[2 * d.nreg + p.ci];
+      const double prd = (gr[0] * gx + gr[1] * gy + gr[2] * gz) / R;
+      Pg[0] = gr[0] - prd * gx / R; Pg[1] = gr[1] - prd * gy / R; Pg[2] = gr[2] - prd * gz / R;
+    }
+    for (int q = 0; q < 3; ++q) {
+      const double drv = gT[(1 + q) * NF + a] - Pg[q] + dd[q][a] + 0.0;
+      out[q] = ps[1 + q][a] + drv * dt;
+    }
+  }
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    const long NF = d.NF, a = p.a, ks = p.ks;
+    const int kmin = d.kmin, kmax = d.kmax, k = p.k;
+    double v[3];
+    if (k >= kmin && k <= kmax) {
+      level(p, k, a, v);
+    } else {
+      // free-slip top / rigid bottom, scaled by the large-step density
+      const bool top = (k == kmax + 1);
+      const long an = top ? a - ks : a + ks;
+      double vn[3];
+      level(p, top ? kmax : kmin, an, vn);
+      const double* rg = prog;
+      for (int q = 0; q < 3; ++q) v[q] = top ? 1.0 * (vn[q] / rg[an]) * rg[a] : -1.0 * (vn[q] / rg[an]) * rg[a];
+    }
+    for (int q = 0; q < 3; ++q) {
+      dv[q][a] = v[q];
+      pm[(1 + q) * NF + a] = pm[(1 + q) * NF + a] + v[q] * rweight;
+    }
+  }
+};
+
+// ----------------------------------------------------------------------------
+// small step vi_main, horizontal part: the three flux divergences of the new
+// momentum (mod_vi.py:1925-2432, kernels/vimain.py:127-262)
+//   hv[0] = div(dv), hv[1] = vertical-pressure source gpre, hv[2] = div((prog + dv) * ethtot)
+// ----------------------------------------------------------------------------
+struct ViMainH {
+  static constexpr const char* kName = "vi_main_h";
+  Dims d; Consts c; Geo G;
+  const double* prog;
+  const double* dv[3];
+  const double* eth;
+  const double* ethtot;
+  const double* gT;
+  double* hv[3];
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    const int kmin = d.kmin, kmax = d.kmax, k = p.k;
+    if (k < kmin || k > kmax) return;
+    const long NF = d.NF, a = p.a;
+    const Cell cl = cell_of_pt(p);
+    long off[7];
+    stencil_offsets(d.g, off);
+    auto nbr = [&](int m) { return p.pole ? a + m : a + off[m]; };
+    const double divs1 = hdiv(G, d, cl, [&](int m) { return dv[0][nbr(m)] * 1.0; },
+                              [&](int m) { return dv[1][nbr(m)] * 1.0; }, [&](int m) { return dv[2][nbr(m)] * 1.0; });
+    const double he = hdiv(G, d, cl, [&](int m) { return dv[0][nbr(m)] * eth[nbr(m)] * 1.0; },
+                           [&](int m) { return dv[1][nbr(m)] * eth[nbr(m)] * 1.0; },
+                           [&](int m) { return dv[2][nbr(m)] * eth[nbr(m)] * 1.0; });
+    const double drhoge = -he - (0.0 - 0.0) * G.rdgz[k];
+    const double grhoge1 = gT[5 * NF + a] + drhoge;
+    const double hd = hdiv(G, d, cl,
+                           [&](int m) { return (prog[1 * NF + nbr(m)] + dv[0][nbr(m)]) * ethtot[nbr(m)] * 1.0; },
+                           [&](int m) { return (prog[2 * NF + nbr(m)] + dv[1][nbr(m)]) * ethtot[nbr(m)] * 1.0; },
+                           [&](int m) { return (prog[3 * NF + nbr(m)] + dv[2][nbr(m)]) * ethtot[nbr(m)] * 1.0; });
+    hv[0][a] = divs1;
+    hv[1][a] = grhoge1 * c.Rdry / c.CVdry;
+    hv[2][a] = hd;
+  }
+};
+
+// vertical part: tridiagonal rhogw solve and the mass / energy update, one
+// thread per compute column, streamed in k (kernels/virhowsolver.py:42-120,
+// kernels/vimain.py:127-262).  The forward sweep keeps u and gamma in the
+// output fields we[1] and we[2] until the backward sweep has used them.
+struct ViMainV {
+  static constexpr const char* kName = "vi_main_v";
+  Dims d; Consts c; Geo G;
+  double dt, rweight;
+  const double* prog;
+  const double* ps[6];
+  const double* dv[3];
+  const double* ethtot;
+  const double* gT;
+  const double* getot;
+  const double* Mc; const double* Mu; const double* Ml;
+  const double* hv[3];
+  double* we[3];
+  double* pm;
+  NH_HD void operator()(long t) const {
+    Cell cl = cell_of(d, t);
+    if (cl.pole ? cl.v != 0 : !cl.inner) return;
+    const long NF = d.NF, ks = cl.ks, b = cl.base;
+    const int kmin = d.kmin, kmax = d.kmax;
+    const double RovCV = c.Rdry / c.CVdry;
+    auto at = [&](int kk) { return b + (long)kk * ks; };
+    const double* pg = ps[I_RHOG];
+    const double* pw = ps[I_RHOGW];
+    const double* pe = ps[I_RHOGE];
+    // split energy with the ghost rows copied from the nearest level
+    auto pe_k = [&](int kk) {
+      const int k2 = (kk < kmin) ? kmin : ((kk > kmax) ? kmax : kk);
+      return pe[at(k2)];
+    };
+    auto grhog1 = [&](int kk) {
+      const double drhog = -hv[0][at(kk)] - (0.0 - 0.0) * G.rdgz[kk];
+      return gT[0 * NF + at(kk)] + drhog;
+    };
+    const int N = kmax - kmin;
+    const double CVovRt2 = c.CVdry / c.Rdry / (dt * dt);
+    double* U = we[1];
+    double* Gm = we[2];
+    // source vector and forward sweep
+    double beta = 0.0, u_prev = 0.0;
+    for (int n = 0; n < N; ++n) {
+      const int k = kmin + 1 + n;
+      const long a = at(k);
+      const double sw = gT[4 * NF + a] + 0.0 * c.ALPHA;
+      const double p1 = pe_k(k) * RovCV, p0 = pe_k(k - 1) * RovCV;
+      const double A = (pw[a] * c.ALPHA + dt * sw) * 1.0;
+      const double B = (p1 + dt * hv[1][a]) * 1.0 - (p0 + dt * hv[1][a - ks]) * 1.0;
+      const double Cc = (pg[a] + dt * grhog1(k)) * 1.0 * G.afact[k] + (pg[a - ks] + dt * grhog1(k - 1)) * 1.0 * G.bfact[k];
+      double S = (A - B * dt * G.rdgzh[k] - Cc * dt * c.GRAV) * CVovRt2;
+      if (n == 0) S = S - Ml[at(kmin + 1)] * (0.0 * 1.0);
+      if (n == N - 1) S = S - Mu[at(kmax)] * (-0.0 * 1.0);
+      double u;
+      if (n == 0) {
+        beta = Mc[a];
+        u = S / beta;
+      } else {
+        const double gm = Mu[a - ks] / beta;
+        beta = Mc[a] - Ml[a] * gm;
+        u = (S - Ml[a] * u_prev) / beta;
+        Gm[a] = gm;
+      }
+      U[a] = u;
+      u_prev = u;
+    }
+    // backward sweep: rhogw split at kmin+1..kmax, boundary values
+    double xv = U[at(kmax)];
+    U[at(kmax)] = xv * 1.0;
+    for (int n = N - 2; n >= 0; --n) {
+      const long a = at(kmin + 1 + n);
+      xv = U[a] - Gm[a + ks] * xv;
+      U[a] = xv * 1.0;
+    }
+    U[at(kmin)] = 0.0 * 1.0;
+    U[at(kmax + 1)] = -0.0 * 1.0;
+    U[at(kmin - 1)] = 0.0;
+    const double* rw1 = U;
+    // new density split (all levels)
+    auto rs1 = [&](int kk) {
+      double drhog2 = 0.0;
+      if (kk >= kmin && kk <= kmax) {
+        const double hi = (kk + 1 <= kmax) ? 0.0 * 1.0 + 1.0 * rw1[at(kk + 1)] * 1.0 : 0.0;
+        const double lo = (kk >= kmin + 1) ? 0.0 * 1.0 + 1.0 * rw1[at(kk)] * 1.0 : 0.0;
+        drhog2 = -hv[0][at(kk)] - (hi - lo) * G.rdgz[kk];
+      }
+      return pg[at(kk)] + (gT[0 * NF + at(kk)] + drhog2) * dt;
+    };
+    auto g10 = [&](int kk, int q) { return prog[q * NF + at(kk)] + ps[q][at(kk)]; };
+    auto g11 = [&](int kk, int q) {
+      if (q == 0) return prog[at(kk)] + rs1(kk);
+      if (q == 4) return prog[4 * NF + at(kk)] + rw1[at(kk)];
+      return prog[q * NF + at(kk)] + dv[q - 1][at(kk)];
+    };
+    for (int k = 0; k < d.kall; ++k) {
+      const long a = at(k);
+      const double kin10 = rhogkin_k(G, kmin, kmax, k, g10);
+      const double kin11 = rhogkin_k(G, kmin, kmax, k, g11);
+      double dtot = 0.0;
+      if (k >= kmin && k <= kmax) {
+        double hi = 0.0, lo = 0.0;
+        if (k + 1 <= kmax)
+          hi = 0.0 * 1.0 + 1.0 * ((prog[4 * NF + a + ks] + rw1[a + ks]) *
+                                  (G.afact[k + 1] * ethtot[a + ks] + G.bfact[k + 1] * ethtot[a])) * 1.0;
+        if (k >= kmin + 1)
+          lo = 0.0 * 1.0 + 1.0 * ((prog[4 * NF + a] + rw1[a]) *
+                                  (G.afact[k] * ethtot[a] + G.bfact[k] * ethtot[a - ks])) * 1.0;
+        dtot = -hv[2][a] - (hi - lo) * G.rdgz[k];
+      }
+      const double r1 = rs1(k);
+      const double re1 = pe_k(k) + (getot[a] + dtot) * dt + (kin10 - kin11) + (pg[a] - r1) * G.phi[k];
+      we[2][a] = re1;
+      pm[0 * NF + a] = pm[0 * NF + a] + r1 * rweight;
+      pm[4 * NF + a] = pm[4 * NF + a] + rw1[a] * rweight;
+    }
+    for (int k = 0; k < d.kall; ++k) we[0][at(k)] = rs1(k);
+  }
+};
+
+// PROG += PROG_split and horizontal projection (mod_vi.py:1572-1586)
+struct ProgUpdateP {
+  static constexpr const char* kName = "prog_update";
+  Dims d; Consts c; Geo G;
+  double* prog;
+  const double* ps[6];
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p)) return;
+    const long NF = d.NF, a = p.a;
+    const double R = c.RADIUS;
+    double v[6];
+    for (int q = 0; q < 6; ++q) v[q] = prog[q * NF + a] + ps[q][a];
+    if (p.pole) {
+      const double gx = G.gx_pl[(p.v * 2 + p.lp) * 3 + 0];
+      const double gy = G.gx_pl[(p.v * 2 + p.lp) * 3 + 1];
+      const double gz = G.gx_pl[(p.v * 2 + p.lp) * 3 + 2];
+      const double prd = v[1] * gx / R + v[2] * gy / R + v[3] * gz / R;
+      v[1] = v[1] - prd * gx / R; v[2] = v[2] - prd * gy / R; v[3] = v[3] - prd * gz / R;
+    } else if (p.inner) {
+      const double gx = G.gx[p.ci], gy = G.gx[d.nreg + p.ci], gz = G.gx[2 * d.nreg + p.ci];
+      const double prd = (v[1] * gx + v[2] * gy + v[3] * gz) / R;
+      v[1] = v[1] - prd * gx / R; v[2] = v[2] - prd * gy / R; v[3] = v[3] - prd * gz / R;
+    }
+    for (int q = 0; q < 6; ++q) prog[q * NF + a] = v[q];
+  }
+};
+
+}  // namespace nh

@@ -20,6 +20,12 @@ __global__ __launch_bounds__(256) void k_run(F f, long n) {
   if (t < n) f(t);
 }
 
+// point grid: x = point of the plane, y = plane (region level or pole level)
+template <class F>
+__global__ __launch_bounds__(256) void k_pts(F f) {
+  f((int)blockIdx.y, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
 struct Context;
 
 // Kernel timers: one event pair per launch while enabled, recorded on the
@@ -64,6 +70,18 @@ class HipExec {
     hipEvent_t* pair = prof_ && prof_->on ? next_pair(F::kName) : nullptr;
     if (pair) note(hipEventRecord(pair[0], s_), "hipEventRecord");
     hipLaunchKernelGGL(k_run<F>, dim3((unsigned)nb), dim3(bs), 0, s_, f, n);
+    note(hipGetLastError(), "kernel launch");
+    if (pair) note(hipEventRecord(pair[1], s_), "hipEventRecord");
+    if (sync_each_) note(hipStreamSynchronize(s_), "kernel execution");
+  }
+  template <class F>
+  void run_pts(const F& f, const Dims& d) {
+    const int bs = 256;
+    const int nx = (d.g * d.g + bs - 1) / bs;
+    const int ny = d.lall * d.kall + 2 * d.kall;
+    hipEvent_t* pair = prof_ && prof_->on ? next_pair(F::kName) : nullptr;
+    if (pair) note(hipEventRecord(pair[0], s_), "hipEventRecord");
+    hipLaunchKernelGGL(k_pts<F>, dim3(nx, ny), dim3(bs), 0, s_, f);
     note(hipGetLastError(), "kernel launch");
     if (pair) note(hipEventRecord(pair[1], s_), "hipEventRecord");
     if (sync_each_) note(hipStreamSynchronize(s_), "kernel execution");

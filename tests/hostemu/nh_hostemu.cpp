@@ -70,6 +70,18 @@ class HostExec {
       prof_->rec.emplace_back(F::kName,
                               std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
   }
+  template <class F>
+  void run_pts(const F& f, const Dims& d) {
+    auto t0 = std::chrono::steady_clock::now();
+    const int ny = d.lall * d.kall + 2 * d.kall;
+    const int nx = d.g * d.g;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) f(y, x);
+    if (prof_ && prof_->on)
+      prof_->rec.emplace_back(F::kName,
+                              std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+  }
   void sync() {}
   int check(const char*) { return 0; }
   std::string message() const { return "host"; }
