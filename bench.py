@@ -9,7 +9,9 @@ resident in HBM.  SDPD = dt / wall time per step.
 
 Prints one JSON line (rank 0).  The dominant kernel's achieved HBM bandwidth
 is its algorithmic bytes per launch divided by its average launch time,
-measured with events on the launch stream during the timed steps.  The CPU
+measured with events on the launch stream during the timed steps; its HBM
+traffic per launch comes from two one-step rocprofv3 counter passes
+(FETCH_SIZE, WRITE_SIZE) run as child processes first (--no-traffic skips them).  The CPU
 baseline times the numpy FP64 oracle on one gl05 z40 step on the host and
 scales it to gl07 by the cell ratio (the step's work is proportional to the
 number of cells).
@@ -35,7 +37,7 @@ def pkg(name):
     return importlib.import_module(f"pynicam-dc_amd.{name}")
 
 
-def kernel_bytes(t) -> dict:
+def kernel_bytes(t, nq: int = 1) -> dict:
     """Algorithmic HBM bytes per launch of each kernel (FP64).
 
     Each distinct (i, j, k, l) field read or written once counts F bytes; each
@@ -56,9 +58,82 @@ def kernel_bytes(t) -> dict:
         "ethtot_mean": 12 * F,                  # prog 5 + eth in; ethtot + 5 means out
         "split_init": 15 * F,                   # prog0 6 + prog 6 in (2 of 3 stages), 6 out
         "vi_path1": 20 * F + 24 * Cc,           # 14 in, 6 out; grad + xyz
-        "vi_main": 30 * F + 21 * Cc,            # 25 in, 5 out; div coefs
+        "vi_main_h": 12 * F + 21 * Cc,          # dv 3, prog 3, eth, ethtot, gT_e in; 3 out; div coefs
+        "vi_main_v": 32 * F,                    # 27 in (tendencies, split, state, matrix, divergences), 5 out
         "prog_update": 18 * F + 3 * Cc,         # 12 in, 6 out
+        "trc_vert1": (5 + 2 * nq) * F,          # w mean, rhog_in, frhog + tracers in; flux, rhog1 + tracers out
+        "trc_hflux": 16 * F + 24 * Cc,          # 4 mean fluxes in; 3 arc fluxes + 9 centroids out
+        "trc_rhog2": 6 * F + 1 * Cc,
+        "trc_gradq": 5 * F + 21 * Cc,
+        "trc_remap": 20 * F + 4 * Cc,
+        "trc_qout": 8 * F + 1 * Cc,
+        "trc_apply": 11 * F + 1 * Cc,
+        "trc_vert2": 5 * F,
     }
+
+
+# rocprof kernel name -> library kernel name (the functor type inside k_run / k_pts)
+KERNEL_TYPES = {
+    "PrePostP": "prepost", "AdvMomP": "adv_mom", "HDiffPassP<0>": "hdiff_pass1", "HDiffPassP<1>": "hdiff_pass2",
+    "DivDamp1P": "divdamp_3d", "DivDamp2P": "divdamp_post", "ViPath0P": "vi_path0", "EthTotPMP": "ethtot_mean",
+    "SplitInit": "split_init", "Path1P": "vi_path1", "ViMainH": "vi_main_h", "ViMainV": "vi_main_v",
+    "ProgUpdateP": "prog_update", "HaloGather": "halo_gather", "TrcVert1": "trc_vert1", "TrcVert2": "trc_vert2",
+    "TrcHFlux": "trc_hflux", "TrcRhog2": "trc_rhog2", "TrcGradQ": "trc_gradq", "TrcRemap": "trc_remap",
+    "TrcQout": "trc_qout", "TrcApply": "trc_apply",
+}
+
+
+def _kernel_key(name: str):
+    import re
+    m = re.search(r"k_(?:pts|run)<nh::(\w+(?:<\d+>)?)\s*>", name)
+    return KERNEL_TYPES.get(m.group(1)) if m else None
+
+
+def hbm_traffic(glevel: int, timeout: float = 300.0):
+    """HBM bytes per dispatch of each kernel from two rocprofv3 counter passes.
+
+    Runs this benchmark for one step under ``rocprofv3 --pmc FETCH_SIZE`` and
+    ``--pmc WRITE_SIZE`` (one counter per pass) as child processes, before this
+    process touches the GPU.  Each counter is scaled by what it reads for the
+    512 MiB device-to-device copy of the copy probe in the same run.  Returns
+    ({kernel: bytes}, {counter: factor}) or (None, reason).
+    """
+    import csv
+    import glob
+    import subprocess
+    import tempfile
+    per = {}
+    factors = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as td:
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(td, ctr)
+            cmd = ["rocprofv3", "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "pmc", "--",
+                   sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0",
+                   "--glevel", str(glevel), "--no-cpu-baseline", "--no-traffic", "--copy-probe"]
+            print(f"bench: counter pass {ctr}", file=sys.stderr, flush=True)
+            try:
+                r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), capture_output=True,
+                                   text=True, timeout=timeout)
+            except (OSError, subprocess.TimeoutExpired) as e:
+                return None, f"{ctr} pass failed: {e}"
+            files = glob.glob(os.path.join(d, "**", "pmc_counter_collection.csv"), recursive=True)
+            if r.returncode != 0 or not files:
+                return None, f"{ctr} pass failed (rc {r.returncode})"
+            rows = list(csv.DictReader(open(files[0])))
+            copies = [row for row in rows if "copyBuffer" in row["Kernel_Name"]]
+            if not copies:
+                return None, "copy probe not found"
+            probe = max(copies, key=lambda row: int(row["Dispatch_Id"]))
+            factors[ctr] = 524288.0 / float(probe["Counter_Value"])
+            acc = {}
+            for row in rows:
+                key = _kernel_key(row["Kernel_Name"])
+                if key:
+                    acc.setdefault(key, []).append(float(row["Counter_Value"]))
+            per[ctr] = {k: sum(v) / len(v) for k, v in acc.items()}
+    traffic = {k: 1024.0 * (per["FETCH_SIZE"][k] * factors["FETCH_SIZE"] + per["WRITE_SIZE"].get(k, 0.0)
+                            * factors["WRITE_SIZE"]) for k in per["FETCH_SIZE"]}
+    return traffic, factors
 
 
 def cpu_baseline(seconds_cap: float = 60.0) -> dict:
@@ -91,6 +166,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--glevel", type=int, default=7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure HBM traffic per kernel")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the run, copy 512 MiB device-to-device once (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -98,11 +175,15 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU path is a design, not a run", file=sys.stderr)
         sys.exit(2)
+    gl = args.glevel
+    traffic, factors = (None, "not measured")
+    if not args.no_traffic:
+        # before this process initialises the GPU
+        traffic, factors = hbm_traffic(gl)
     import torch
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
-    gl = args.glevel
     dt = {5: 1200.0, 6: 600.0, 7: 300.0, 8: 150.0}[gl]
     print(f"bench: setting up gl{gl:02d} z40 dt {dt:g} s", file=sys.stderr, flush=True)
     case = pkg("setup").build_case(glevel=gl, vlayer=40, dtl=dt, seed=1)
@@ -140,6 +221,10 @@ def main():
                    "avg_us": 1000.0 * v[0] / v[1],
                    "GBps": (kb[k] / (v[0] / v[1] / 1000.0) / 1e9) if k in kb else None}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
+    if isinstance(traffic, dict):
+        for k, v in kernels.items():
+            if k in traffic:
+                v["traffic_GB"] = traffic[k] / 1e9
     print("bench: kernels " + json.dumps(kernels), file=sys.stderr, flush=True)
     out = {
         "metric": METRIC,
@@ -159,7 +244,11 @@ def main():
                    "glevel": gl, "vlayer": 40, "dt": dt, "cells": 10 * 4 ** gl + 2, "form": "fp64",
                    "parallelism": "single GPU, 40 regions"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (traffic.get(dom) if isinstance(traffic, dict) else None),
+                     "traffic_note": (f"rocprofv3 FETCH_SIZE x {factors['FETCH_SIZE']:.3f} + WRITE_SIZE x "
+                                      f"{factors['WRITE_SIZE']:.3f} per dispatch, factors from a 512 MiB copy"
+                                      if isinstance(traffic, dict) else factors),
                      "algorithmic_bytes_per_launch": kb[dom], "avg_launch_us": avg_s * 1e6},
     }
     if args.copy_probe:
