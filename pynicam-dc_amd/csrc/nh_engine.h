@@ -8,6 +8,8 @@
 //   template <class F> void run(const F& f, long n);      (f(t) for t < n)
 //   template <class F> void run_pts(const F& f, const Dims& d);
 //                                                         (f(plane, x) on the point grid)
+//   template <class F> void run_tiles(const F& f, int nblocks, int lds_doubles);
+//                                                         (f.phase1, barrier, f.phase2 per block)
 //   int check(const char* what);                          (0 or error)
 // so that the same sequence drives the GPU build and the host test build.
 #pragma once
@@ -20,6 +22,7 @@
 #include "nh_step.h"
 #include "nh_tracer.h"
 #include "nh_point.h"
+#include "nh_tile.h"
 #include "../../include/nicamhip.h"
 
 namespace nh {
@@ -221,18 +224,19 @@ inline long ncols(const Dims& d) { return d.nreg + 12; }
 template <class X>
 void divdamp(X& x, const Context& ctx, const double* vx, const double* vy, const double* vz, const double* w,
              double* out0, double* out1, double* out2) {
-  DivDamp1P a;
-  a.d = ctx.d; a.c = ctx.c; a.G = ctx.G;
+  const TileGeom tg = tile_geom(ctx.d, 2);
+  DivDamp1T a;
+  a.d = ctx.d; a.c = ctx.c; a.G = ctx.G; a.tg = tg;
   a.rvx = vx; a.rvy = vy; a.rvz = vz; a.rw = w;
   for (int q = 0; q < 3; ++q) a.out[q] = ctx.field(W_DDT + q);
-  x.run_pts(a, ctx.d);
+  x.run_tiles(a, tg.nblocks, tg.lds);
   double* tmp[3] = {ctx.field(W_DDT), ctx.field(W_DDT + 1), ctx.field(W_DDT + 2)};
   halo(x, ctx, tmp, 3);
-  DivDamp2P b;
-  b.d = ctx.d; b.c = ctx.c; b.G = ctx.G;
+  DivDamp2T b;
+  b.d = ctx.d; b.c = ctx.c; b.G = ctx.G; b.tg = tg;
   for (int q = 0; q < 3; ++q) b.in[q] = tmp[q];
   b.out[0] = out0; b.out[1] = out1; b.out[2] = out2;
-  x.run_pts(b, ctx.d);
+  x.run_tiles(b, tg.nblocks, tg.lds);
 }
 
 // ----------------------------------------------------------------------------

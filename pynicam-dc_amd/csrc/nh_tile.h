@@ -1,0 +1,179 @@
+// LDS-tiled kernels for the triangle-based operators (3-D divergence damping).
+//
+// A block owns a tile of one region level: all interior i (W = g-2 points,
+// one or more wavefronts) times TH rows.  Phase 1 computes the two triangle
+// values of every tile corner (i', j'), i' in 0..W, j' in j0-1..j0+TH-1, once
+// into LDS; phase 2 forms each cell's hexagon sum from the six triangles
+// around it.  Each triangle value is shared by three cells, so this does a
+// third of the work and loads of the per-point form.  Blocks past the
+// regional tiles handle the pole levels (the pole centre's five triangles).
+// On the GPU the phases are separated by a workgroup barrier; the host test
+// build runs them one after the other.
+#pragma once
+
+#include "nh_point.h"
+
+namespace nh {
+
+struct TileGeom {
+  int W, TH;          // tile width (interior points of a row) and height
+  int ntp;            // tiles per region level
+  int ntiles;         // regional tiles (lall * kall * ntp)
+  int nblocks;        // + 2 * kall pole blocks
+  int lds;            // LDS doubles per block
+};
+
+inline TileGeom tile_geom(const Dims& d, int per_corner) {
+  TileGeom t;
+  t.W = d.g - 2;
+  t.TH = 256 / t.W;
+  if (t.TH < 1) t.TH = 1;
+  t.ntp = (t.W + t.TH - 1) / t.TH;
+  t.ntiles = d.lall * d.kall * t.ntp;
+  t.nblocks = t.ntiles + 2 * d.kall;
+  t.lds = per_corner * (t.W + 1) * (t.TH + 1);
+  return t;
+}
+
+// tile decode: region l, level k, first row j0
+struct TileAt {
+  int l, k, j0, rows;
+};
+
+NH_HD inline TileAt tile_at(const Dims& d, const TileGeom& tg, int blk) {
+  TileAt t;
+  const int plane = blk / tg.ntp;
+  const int tile = blk - plane * tg.ntp;
+  t.l = plane / d.kall;
+  t.k = plane - t.l * d.kall;
+  t.j0 = 1 + tile * tg.TH;
+  t.rows = (d.g - 1 - t.j0 < tg.TH) ? d.g - 1 - t.j0 : tg.TH;   // rows j0..j0+rows-1 <= g-2
+  return t;
+}
+
+// ----------------------------------------------------------------------------
+// first operator: triangle-mean 3-D divergence, gradient back to the cells
+// (kernels/oprt3ddivdamp.py:50-173)
+// ----------------------------------------------------------------------------
+struct DivDamp1T {
+  static constexpr const char* kName = "divdamp_3d";
+  Dims d; Consts c; Geo G; TileGeom tg;
+  const double *rvx, *rvy, *rvz, *rw;
+  double* out[3];
+  NH_HD double rwv(long aa, int kk) const {
+    return (kk >= d.kmin + 1 && kk <= d.kmax) ? (0.0 * 1.0 + rw[aa] * 1.0) : 0.0;
+  }
+  NH_HD bool active(int k) const { return k >= d.kmin && k <= d.kmax; }
+  NH_HD void phase1(int blk, int tid, double* lds) const {
+    if (blk >= tg.ntiles) return;
+    const TileAt t = tile_at(d, tg, blk);
+    if (!active(t.k)) return;
+    const int g = d.g, W1 = tg.W + 1;
+    const long gg = (long)g * g, ks = gg;
+    const int nc = W1 * (t.rows + 1);
+    for (int cidx = tid; cidx < nc; cidx += 256) {
+      const int ci_ = cidx % W1, cj_ = cidx / W1;
+      const int i = ci_, j = t.j0 - 1 + cj_;
+      const long cj = ((long)t.l * g + j) * g + i;
+      const long a = ((long)(t.l * d.kall + t.k) * g + j) * g + i;
+      for (int T = 0; T < 2; ++T) {
+        const long o1 = (T == 0) ? 1 : 1 + g;
+        const long o2 = (T == 0) ? 1 + g : g;
+        const double s = tri_div(G, d.nreg, cj, T, rvx, rvy, rvz, a, g, 1.0);
+        const double up = (rwv(a + ks, t.k + 1) + rwv(a + o1 + ks, t.k + 1) + rwv(a + o2 + ks, t.k + 1));
+        const double dn = (rwv(a, t.k) + rwv(a + o1, t.k) + rwv(a + o2, t.k));
+        lds[(T * (t.rows + 1) + cj_) * W1 + ci_] = s + (up - dn) / 3.0 * G.rdgz[t.k];
+      }
+    }
+  }
+  NH_HD void phase2(int blk, int tid, const double* lds) const {
+    const int g = d.g;
+    if (blk >= tg.ntiles) {
+      pole(blk - tg.ntiles, tid);
+      return;
+    }
+    const TileAt t = tile_at(d, tg, blk);
+    const int W1 = tg.W + 1;
+    const int ti = tid % tg.W, tj = tid / tg.W;
+    if (tj >= t.rows) return;
+    const int i = ti + 1, j = t.j0 + tj;
+    const long ci = ((long)t.l * g + j) * g + i;
+    const long a = ((long)(t.l * d.kall + t.k) * g + j) * g + i;
+    if (!active(t.k)) {
+      for (int q = 0; q < 3; ++q) out[q][a] = 0.0;
+      return;
+    }
+    const int R = t.rows + 1;
+    auto L = [&](int T, int ii, int jj) { return lds[(T * R + (jj - t.j0 + 1)) * W1 + ii]; };
+    double T6[6] = {L(0, i, j), L(1, i, j), L(0, i - 1, j), L(1, i - 1, j - 1), L(0, i - 1, j - 1), L(1, i, j - 1)};
+    if (i == 1 && j == 1) {
+      const double m0 = G.sgp[t.l] ? 0.0 : 1.0, m1 = G.sgp[t.l] ? 1.0 : 0.0;
+      T6[4] = T6[4] * m0 + T6[5] * m1;
+    }
+    for (int dd = 0; dd < 3; ++dd) out[dd][a] = hex_sum(G, d.nreg, ci, dd, T6);
+  }
+  NH_HD void pole(int pb, int tid) const {
+    if (tid != 0) return;
+    DivDamp1P p{d, c, G, rvx, rvy, rvz, rw, {out[0], out[1], out[2]}};
+    p(d.lall * d.kall + pb, 0);
+  }
+};
+
+// second operator on -(first result), coefficient, projection
+// (kernels/oprtdivdamp.py:54-173, kernels/divdamppostcomm.py)
+struct DivDamp2T {
+  static constexpr const char* kName = "divdamp_post";
+  Dims d; Consts c; Geo G; TileGeom tg;
+  const double* in[3];
+  double* out[3];
+  NH_HD bool active(int k) const { return k >= d.kmin && k <= d.kmax; }
+  NH_HD void phase1(int blk, int tid, double* lds) const {
+    if (blk >= tg.ntiles) return;
+    const TileAt t = tile_at(d, tg, blk);
+    if (!active(t.k)) return;
+    const int g = d.g, W1 = tg.W + 1;
+    const int nc = W1 * (t.rows + 1);
+    for (int cidx = tid; cidx < nc; cidx += 256) {
+      const int ci_ = cidx % W1, cj_ = cidx / W1;
+      const int i = ci_, j = t.j0 - 1 + cj_;
+      const long cj = ((long)t.l * g + j) * g + i;
+      const long a = ((long)(t.l * d.kall + t.k) * g + j) * g + i;
+      for (int T = 0; T < 2; ++T)
+        lds[(T * (t.rows + 1) + cj_) * W1 + ci_] = tri_div(G, d.nreg, cj, T, in[0], in[1], in[2], a, g, -1.0);
+    }
+  }
+  NH_HD void phase2(int blk, int tid, const double* lds) const {
+    const int g = d.g;
+    if (blk >= tg.ntiles) {
+      if (tid == 0) {
+        DivDamp2P p{d, c, G, {in[0], in[1], in[2]}, {out[0], out[1], out[2]}};
+        p(d.lall * d.kall + (blk - tg.ntiles), 0);
+      }
+      return;
+    }
+    const TileAt t = tile_at(d, tg, blk);
+    const int W1 = tg.W + 1;
+    const int ti = tid % tg.W, tj = tid / tg.W;
+    if (tj >= t.rows) return;
+    const int i = ti + 1, j = t.j0 + tj;
+    const long ci = ((long)t.l * g + j) * g + i;
+    const long a = ((long)(t.l * d.kall + t.k) * g + j) * g + i;
+    double o[3] = {0.0, 0.0, 0.0};
+    if (active(t.k)) {
+      const int R = t.rows + 1;
+      auto L = [&](int T, int ii, int jj) { return lds[(T * R + (jj - t.j0 + 1)) * W1 + ii]; };
+      const double T6[6] = {L(0, i, j), L(1, i, j), L(0, i - 1, j), L(1, i - 1, j - 1), L(0, i - 1, j - 1),
+                            L(1, i, j - 1)};
+      for (int dd = 0; dd < 3; ++dd) o[dd] = hex_sum(G, d.nreg, ci, dd, T6);
+    }
+    const double Rr = c.RADIUS;
+    const double gd[3] = {c.ALPHA_D * o[0], c.ALPHA_D * o[1], c.ALPHA_D * o[2]};
+    const double gx = G.gx[ci], gy = G.gx[d.nreg + ci], gz = G.gx[2 * d.nreg + ci];
+    const double prd = (gd[0] * gx + gd[1] * gy + gd[2] * gz) / Rr;
+    out[0][a] = gd[0] - prd * gx / Rr;
+    out[1][a] = gd[1] - prd * gy / Rr;
+    out[2][a] = gd[2] - prd * gz / Rr;
+  }
+};
+
+}  // namespace nh

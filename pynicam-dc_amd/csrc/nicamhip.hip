@@ -20,10 +20,26 @@ __global__ __launch_bounds__(256) void k_run(F f, long n) {
   if (t < n) f(t);
 }
 
-// point grid: x = point of the plane, y = plane (region level or pole level)
+// point grid: nx blocks per plane, ny planes (region levels, then pole levels),
+// launched as one linear grid.  The hardware deals workgroups to the 8 XCDs
+// round-robin; the remap gives each XCD a contiguous range of blocks, so the
+// neighbouring rows (j +- 1) and levels (k +- 1) a stencil reads sit in the
+// same XCD's L2.
+constexpr int kXcds = 8;
+
 template <class F>
-__global__ __launch_bounds__(256) void k_pts(F f) {
-  f((int)blockIdx.y, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+__global__ __launch_bounds__(256) void k_pts(F f, int nx, int nblk) {
+  const int L = blockIdx.x;
+  const int per = (nblk + kXcds - 1) / kXcds;
+  const int xcd = L % kXcds, idx = L / kXcds;
+  // the first (nblk % kXcds) XCDs get 'per' blocks, the others one fewer
+  const int rem = nblk % kXcds;
+  int logical;
+  if (rem == 0 || xcd < rem) logical = xcd * per + idx;
+  else logical = rem * per + (xcd - rem) * (per - 1) + idx;
+  const int plane = logical / nx;
+  const int xb = logical - plane * nx;
+  f(plane, xb * (int)blockDim.x + (int)threadIdx.x);
 }
 
 struct Context;
@@ -36,6 +52,20 @@ struct HipProf {
   size_t used = 0;                     // pairs in use
   std::vector<const char*> name;       // per used pair
 };
+
+// tiled two-phase kernels: LDS shared by the block, barrier between the phases
+template <class F>
+__global__ __launch_bounds__(256) void k_tiles(F f, int nblk) {
+  extern __shared__ double lds[];
+  const int L = blockIdx.x;
+  const int per = (nblk + kXcds - 1) / kXcds;
+  const int xcd = L % kXcds, idx = L / kXcds;
+  const int rem = nblk % kXcds;
+  const int blk = (rem == 0 || xcd < rem) ? xcd * per + idx : rem * per + (xcd - rem) * (per - 1) + idx;
+  f.phase1(blk, (int)threadIdx.x, lds);
+  __syncthreads();
+  f.phase2(blk, (int)threadIdx.x, lds);
+}
 
 class HipExec {
  public:
@@ -79,9 +109,19 @@ class HipExec {
     const int bs = 256;
     const int nx = (d.g * d.g + bs - 1) / bs;
     const int ny = d.lall * d.kall + 2 * d.kall;
+    const int nblk = nx * ny;
     hipEvent_t* pair = prof_ && prof_->on ? next_pair(F::kName) : nullptr;
     if (pair) note(hipEventRecord(pair[0], s_), "hipEventRecord");
-    hipLaunchKernelGGL(k_pts<F>, dim3(nx, ny), dim3(bs), 0, s_, f);
+    hipLaunchKernelGGL(k_pts<F>, dim3(nblk), dim3(bs), 0, s_, f, nx, nblk);
+    note(hipGetLastError(), "kernel launch");
+    if (pair) note(hipEventRecord(pair[1], s_), "hipEventRecord");
+    if (sync_each_) note(hipStreamSynchronize(s_), "kernel execution");
+  }
+  template <class F>
+  void run_tiles(const F& f, int nblocks, int lds_doubles) {
+    hipEvent_t* pair = prof_ && prof_->on ? next_pair(F::kName) : nullptr;
+    if (pair) note(hipEventRecord(pair[0], s_), "hipEventRecord");
+    hipLaunchKernelGGL(k_tiles<F>, dim3(nblocks), dim3(256), (size_t)lds_doubles * sizeof(double), s_, f, nblocks);
     note(hipGetLastError(), "kernel launch");
     if (pair) note(hipEventRecord(pair[1], s_), "hipEventRecord");
     if (sync_each_) note(hipStreamSynchronize(s_), "kernel execution");

@@ -82,6 +82,22 @@ class HostExec {
       prof_->rec.emplace_back(F::kName,
                               std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
   }
+  template <class F>
+  void run_tiles(const F& f, int nblocks, int lds_doubles) {
+    auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel
+    {
+      std::vector<double> lds((size_t)lds_doubles + 1, 0.0);
+#pragma omp for schedule(static)
+      for (int b = 0; b < nblocks; ++b) {
+        for (int t = 0; t < 256; ++t) f.phase1(b, t, lds.data());
+        for (int t = 0; t < 256; ++t) f.phase2(b, t, lds.data());
+      }
+    }
+    if (prof_ && prof_->on)
+      prof_->rec.emplace_back(F::kName,
+                              std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+  }
   void sync() {}
   int check(const char*) { return 0; }
   std::string message() const { return "host"; }
