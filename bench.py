@@ -59,7 +59,8 @@ def kernel_bytes(t, nq: int = 1) -> dict:
         "split_init": 15 * F,                   # prog0 6 + prog 6 in (2 of 3 stages), 6 out
         "vi_path1": 20 * F + 24 * Cc,           # 14 in, 6 out; grad + xyz
         "vi_main_h": 12 * F + 21 * Cc,          # dv 3, prog 3, eth, ethtot, gT_e in; 3 out; div coefs
-        "vi_main_v": 32 * F,                    # 27 in (tendencies, split, state, matrix, divergences), 5 out
+        "vi_main_v": 13 * F,                    # split rho/w/e, gT rho/w, matrix 3, divergences 2 in; rhogw out
+        "vi_main_post": 26 * F,                 # state 5, split 6, dv 3, ethtot, gT, getot, div 2, rhogw, means 2 in; 4 out
         "prog_update": 18 * F + 3 * Cc,         # 12 in, 6 out
         "trc_vert1": (5 + 2 * nq) * F,          # w mean, rhog_in, frhog + tracers in; flux, rhog1 + tracers out
         "trc_hflux": 16 * F + 24 * Cc,          # 4 mean fluxes in; 3 arc fluxes + 9 centroids out
@@ -77,15 +78,17 @@ KERNEL_TYPES = {
     "PrePostP": "prepost", "AdvMomP": "adv_mom", "HDiffPassP<0>": "hdiff_pass1", "HDiffPassP<1>": "hdiff_pass2",
     "DivDamp1P": "divdamp_3d", "DivDamp2P": "divdamp_post", "ViPath0P": "vi_path0", "EthTotPMP": "ethtot_mean",
     "SplitInit": "split_init", "Path1P": "vi_path1", "ViMainH": "vi_main_h", "ViMainV": "vi_main_v",
+    "ViMainPost": "vi_main_post",
     "ProgUpdateP": "prog_update", "HaloGather": "halo_gather", "TrcVert1": "trc_vert1", "TrcVert2": "trc_vert2",
     "TrcHFlux": "trc_hflux", "TrcRhog2": "trc_rhog2", "TrcGradQ": "trc_gradq", "TrcRemap": "trc_remap",
-    "TrcQout": "trc_qout", "TrcApply": "trc_apply",
+    "TrcQout": "trc_qout", "TrcApply": "trc_apply", "DivDamp1T": "divdamp_3d", "DivDamp2T": "divdamp_post",
+    "HDiffPass2T": "hdiff_pass2",
 }
 
 
 def _kernel_key(name: str):
     import re
-    m = re.search(r"k_(?:pts|run)<nh::(\w+(?:<\d+>)?)\s*>", name)
+    m = re.search(r"k_(?:pts|run|tiles)<nh::(\w+(?:<\d+>)?)\s*>", name)
     return KERNEL_TYPES.get(m.group(1)) if m else None
 
 
@@ -130,10 +133,15 @@ def hbm_traffic(glevel: int, timeout: float = 300.0):
                 key = _kernel_key(row["Kernel_Name"])
                 if key:
                     acc.setdefault(key, []).append(float(row["Counter_Value"]))
+                    RESOURCES[key] = {"vgpr": int(row["VGPR_Count"]), "scratch": int(row["Scratch_Size"]),
+                                      "lds": int(row["LDS_Block_Size"])}
             per[ctr] = {k: sum(v) / len(v) for k, v in acc.items()}
     traffic = {k: 1024.0 * (per["FETCH_SIZE"][k] * factors["FETCH_SIZE"] + per["WRITE_SIZE"].get(k, 0.0)
                             * factors["WRITE_SIZE"]) for k in per["FETCH_SIZE"]}
     return traffic, factors
+
+
+RESOURCES: dict = {}
 
 
 def cpu_baseline(seconds_cap: float = 60.0) -> dict:
@@ -225,6 +233,8 @@ def main():
         for k, v in kernels.items():
             if k in traffic:
                 v["traffic_GB"] = traffic[k] / 1e9
+            if k in RESOURCES:
+                v.update(RESOURCES[k])
     print("bench: kernels " + json.dumps(kernels), file=sys.stderr, flush=True)
     out = {
         "metric": METRIC,

@@ -376,14 +376,14 @@ int dyn_step(X& x, const Context& ctx, double* prog, double* progq, int nq, doub
       f.rhog = prog;
       x.run_pts(f, d);
       halo_block(x, ctx, ctx.field(W_HDA), 6);
-      HDiffPassP<1> f2;
-      f2.d = d; f2.c = ctx.c; f2.G = ctx.G;
+      HDiffPass2T f2;
+      f2.d = d; f2.c = ctx.c; f2.G = ctx.G; f2.tg = tile_geom(d, 12);
       for (int q = 0; q < 6; ++q) {
         f2.in[q] = ctx.field(W_HDA + q);
         f2.out[q] = ctx.field(W_HDB + q);
       }
       f2.rhog = prog;
-      x.run_pts(f2, d);
+      x.run_tiles(f2, f2.tg.nblocks, f2.tg.lds);
       halo_block(x, ctx, ctx.field(W_HDB), 6);
     }
     // divergence damping of the large-step momentum
@@ -446,16 +446,26 @@ int dyn_step(X& x, const Context& ctx, double* prog, double* progq, int nq, doub
       {
         ViMainV f;
         f.d = d; f.c = ctx.c; f.G = ctx.G;
+        f.dt = dts;
+        for (int q = 0; q < 6; ++q) f.ps[q] = ps[q];
+        f.gT = gT;
+        f.Mc = ctx.field(W_MC); f.Mu = ctx.field(W_MU); f.Ml = ctx.field(W_ML);
+        for (int q = 0; q < 3; ++q) f.hv[q] = hv[q];
+        for (int q = 0; q < 3; ++q) f.we[q] = we[q];
+        x.run(f, nc);
+      }
+      {
+        ViMainPost f;
+        f.d = d; f.c = ctx.c; f.G = ctx.G;
         f.dt = dts; f.rweight = rweight;
         f.prog = prog;
         for (int q = 0; q < 6; ++q) f.ps[q] = ps[q];
         for (int q = 0; q < 3; ++q) f.dv[q] = dv[q];
         f.ethtot = ethtot; f.gT = gT; f.getot = getot;
-        f.Mc = ctx.field(W_MC); f.Mu = ctx.field(W_MU); f.Ml = ctx.field(W_ML);
         for (int q = 0; q < 3; ++q) f.hv[q] = hv[q];
         for (int q = 0; q < 3; ++q) f.we[q] = we[q];
         f.pm = pm;
-        x.run(f, nc);
+        x.run_pts(f, d);
       }
       halo(x, ctx, we, 3);
       // rotate buffers: the new split state is (we0, dv0, dv1, dv2, we1, we2)

@@ -702,12 +702,95 @@ struct ViMainH {
   }
 };
 
-// vertical part: tridiagonal rhogw solve and the mass / energy update, one
-// thread per compute column, streamed in k (kernels/virhowsolver.py:42-120,
-// kernels/vimain.py:127-262).  The forward sweep keeps u and gamma in the
-// output fields we[1] and we[2] until the backward sweep has used them.
+// vertical part, one thread per compute column: source vector of the
+// tridiagonal rhogw system and the Thomas sweeps, streamed in k with the
+// previous level kept in registers (kernels/virhowsolver.py:42-120).  The
+// forward sweep keeps u and gamma in the output fields we[1] and we[2]; on
+// return we[1] holds the new rhogw split on every level.
 struct ViMainV {
   static constexpr const char* kName = "vi_main_v";
+  Dims d; Consts c; Geo G;
+  double dt;
+  const double* ps[6];
+  const double* gT;
+  const double* Mc; const double* Mu; const double* Ml;
+  const double* hv[3];
+  double* we[3];
+  NH_HD void operator()(long t) const {
+    Cell cl = cell_of(d, t);
+    if (cl.pole ? cl.v != 0 : !cl.inner) return;
+    const long NF = d.NF, ks = cl.ks, b = cl.base;
+    const int kmin = d.kmin, kmax = d.kmax;
+    const double RovCV = c.Rdry / c.CVdry;
+    const double CVovRt2 = c.CVdry / c.Rdry / (dt * dt);
+    const double* __restrict__ pg = ps[I_RHOG];
+    const double* __restrict__ pw = ps[I_RHOGW];
+    const double* __restrict__ pe = ps[I_RHOGE];
+    const double* __restrict__ gT0 = gT;
+    const double* __restrict__ gT4 = gT + 4 * NF;
+    const double* __restrict__ hv0 = hv[0];
+    const double* __restrict__ hv1 = hv[1];
+    const double* __restrict__ mc = Mc;
+    const double* __restrict__ mu = Mu;
+    const double* __restrict__ ml = Ml;
+    double* __restrict__ U = we[1];
+    double* __restrict__ Gm = we[2];
+    const int N = kmax - kmin;
+    // level k-1 values, starting at kmin
+    long am = b + (long)kmin * ks;
+    double pe_m = pe[am], hv1_m = hv1[am], pg_m = pg[am];
+    double gr_m = gT0[am] + (-hv0[am] - (0.0 - 0.0) * G.rdgz[kmin]);
+    double beta = 0.0, u_prev = 0.0, mu_m = 0.0;
+    for (int n = 0; n < N; ++n) {
+      const int k = kmin + 1 + n;
+      const long a = am + ks;
+      const double pe_k = pe[a], hv1_k = hv1[a], pg_k = pg[a];
+      const double gr_k = gT0[a] + (-hv0[a] - (0.0 - 0.0) * G.rdgz[k]);
+      const double sw = gT4[a] + 0.0 * c.ALPHA;
+      const double A = (pw[a] * c.ALPHA + dt * sw) * 1.0;
+      const double B = (pe_k * RovCV + dt * hv1_k) * 1.0 - (pe_m * RovCV + dt * hv1_m) * 1.0;
+      const double Cc = (pg_k + dt * gr_k) * 1.0 * G.afact[k] + (pg_m + dt * gr_m) * 1.0 * G.bfact[k];
+      double S = (A - B * dt * G.rdgzh[k] - Cc * dt * c.GRAV) * CVovRt2;
+      const double mc_k = mc[a], ml_k = ml[a], mu_k = mu[a];
+      if (n == 0) S = S - ml_k * (0.0 * 1.0);
+      if (n == N - 1) S = S - mu_k * (-0.0 * 1.0);
+      double u;
+      if (n == 0) {
+        beta = mc_k;
+        u = S / beta;
+      } else {
+        const double gm = mu_m / beta;
+        beta = mc_k - ml_k * gm;
+        u = (S - ml_k * u_prev) / beta;
+        Gm[a] = gm;
+      }
+      U[a] = u;
+      u_prev = u;
+      mu_m = mu_k;
+      pe_m = pe_k; hv1_m = hv1_k; pg_m = pg_k; gr_m = gr_k;
+      am = a;
+    }
+    // backward sweep
+    long a = b + (long)kmax * ks;
+    double xv = u_prev;
+    U[a] = xv * 1.0;
+    double gm_next = Gm[a];
+    for (int n = N - 2; n >= 0; --n) {
+      a -= ks;
+      xv = U[a] - gm_next * xv;
+      gm_next = Gm[a];
+      U[a] = xv * 1.0;
+    }
+    U[b + (long)kmin * ks] = 0.0 * 1.0;
+    U[b + (long)(kmax + 1) * ks] = -0.0 * 1.0;
+    U[b + (long)(kmin - 1) * ks] = 0.0;
+  }
+};
+
+// mass and energy update of the small step from the new rhogw split, one
+// thread per point (kernels/vimain.py:190-262, rhogkin.py:38-79)
+struct ViMainPost {
+  static constexpr const char* kName = "vi_main_post";
   Dims d; Consts c; Geo G;
   double dt, rweight;
   const double* prog;
@@ -716,72 +799,19 @@ struct ViMainV {
   const double* ethtot;
   const double* gT;
   const double* getot;
-  const double* Mc; const double* Mu; const double* Ml;
   const double* hv[3];
-  double* we[3];
+  double* we[3];       // we[1] (rhogw split) is read
   double* pm;
-  NH_HD void operator()(long t) const {
-    Cell cl = cell_of(d, t);
-    if (cl.pole ? cl.v != 0 : !cl.inner) return;
-    const long NF = d.NF, ks = cl.ks, b = cl.base;
-    const int kmin = d.kmin, kmax = d.kmax;
-    const double RovCV = c.Rdry / c.CVdry;
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    const long NF = d.NF, ks = p.ks, a = p.a;
+    const int kmin = d.kmin, kmax = d.kmax, k = p.k;
+    const long b = a - (long)k * ks;
     auto at = [&](int kk) { return b + (long)kk * ks; };
     const double* pg = ps[I_RHOG];
-    const double* pw = ps[I_RHOGW];
     const double* pe = ps[I_RHOGE];
-    // split energy with the ghost rows copied from the nearest level
-    auto pe_k = [&](int kk) {
-      const int k2 = (kk < kmin) ? kmin : ((kk > kmax) ? kmax : kk);
-      return pe[at(k2)];
-    };
-    auto grhog1 = [&](int kk) {
-      const double drhog = -hv[0][at(kk)] - (0.0 - 0.0) * G.rdgz[kk];
-      return gT[0 * NF + at(kk)] + drhog;
-    };
-    const int N = kmax - kmin;
-    const double CVovRt2 = c.CVdry / c.Rdry / (dt * dt);
-    double* U = we[1];
-    double* Gm = we[2];
-    // source vector and forward sweep
-    double beta = 0.0, u_prev = 0.0;
-    for (int n = 0; n < N; ++n) {
-      const int k = kmin + 1 + n;
-      const long a = at(k);
-      const double sw = gT[4 * NF + a] + 0.0 * c.ALPHA;
-      const double p1 = pe_k(k) * RovCV, p0 = pe_k(k - 1) * RovCV;
-      const double A = (pw[a] * c.ALPHA + dt * sw) * 1.0;
-      const double B = (p1 + dt * hv[1][a]) * 1.0 - (p0 + dt * hv[1][a - ks]) * 1.0;
-      const double Cc = (pg[a] + dt * grhog1(k)) * 1.0 * G.afact[k] + (pg[a - ks] + dt * grhog1(k - 1)) * 1.0 * G.bfact[k];
-      double S = (A - B * dt * G.rdgzh[k] - Cc * dt * c.GRAV) * CVovRt2;
-      if (n == 0) S = S - Ml[at(kmin + 1)] * (0.0 * 1.0);
-      if (n == N - 1) S = S - Mu[at(kmax)] * (-0.0 * 1.0);
-      double u;
-      if (n == 0) {
-        beta = Mc[a];
-        u = S / beta;
-      } else {
-        const double gm = Mu[a - ks] / beta;
-        beta = Mc[a] - Ml[a] * gm;
-        u = (S - Ml[a] * u_prev) / beta;
-        Gm[a] = gm;
-      }
-      U[a] = u;
-      u_prev = u;
-    }
-    // backward sweep: rhogw split at kmin+1..kmax, boundary values
-    double xv = U[at(kmax)];
-    U[at(kmax)] = xv * 1.0;
-    for (int n = N - 2; n >= 0; --n) {
-      const long a = at(kmin + 1 + n);
-      xv = U[a] - Gm[a + ks] * xv;
-      U[a] = xv * 1.0;
-    }
-    U[at(kmin)] = 0.0 * 1.0;
-    U[at(kmax + 1)] = -0.0 * 1.0;
-    U[at(kmin - 1)] = 0.0;
-    const double* rw1 = U;
-    // new density split (all levels)
+    const double* rw1 = we[1];
     auto rs1 = [&](int kk) {
       double drhog2 = 0.0;
       if (kk >= kmin && kk <= kmax) {
@@ -797,28 +827,27 @@ struct ViMainV {
       if (q == 4) return prog[4 * NF + at(kk)] + rw1[at(kk)];
       return prog[q * NF + at(kk)] + dv[q - 1][at(kk)];
     };
-    for (int k = 0; k < d.kall; ++k) {
-      const long a = at(k);
-      const double kin10 = rhogkin_k(G, kmin, kmax, k, g10);
-      const double kin11 = rhogkin_k(G, kmin, kmax, k, g11);
-      double dtot = 0.0;
-      if (k >= kmin && k <= kmax) {
-        double hi = 0.0, lo = 0.0;
-        if (k + 1 <= kmax)
-          hi = 0.0 * 1.0 + 1.0 * ((prog[4 * NF + a + ks] + rw1[a + ks]) *
-                                  (G.afact[k + 1] * ethtot[a + ks] + G.bfact[k + 1] * ethtot[a])) * 1.0;
-        if (k >= kmin + 1)
-          lo = 0.0 * 1.0 + 1.0 * ((prog[4 * NF + a] + rw1[a]) *
-                                  (G.afact[k] * ethtot[a] + G.bfact[k] * ethtot[a - ks])) * 1.0;
-        dtot = -hv[2][a] - (hi - lo) * G.rdgz[k];
-      }
-      const double r1 = rs1(k);
-      const double re1 = pe_k(k) + (getot[a] + dtot) * dt + (kin10 - kin11) + (pg[a] - r1) * G.phi[k];
-      we[2][a] = re1;
-      pm[0 * NF + a] = pm[0 * NF + a] + r1 * rweight;
-      pm[4 * NF + a] = pm[4 * NF + a] + rw1[a] * rweight;
+    const double kin10 = rhogkin_k(G, kmin, kmax, k, g10);
+    const double kin11 = rhogkin_k(G, kmin, kmax, k, g11);
+    double dtot = 0.0;
+    if (k >= kmin && k <= kmax) {
+      double hi = 0.0, lo = 0.0;
+      if (k + 1 <= kmax)
+        hi = 0.0 * 1.0 + 1.0 * ((prog[4 * NF + a + ks] + rw1[a + ks]) *
+                                (G.afact[k + 1] * ethtot[a + ks] + G.bfact[k + 1] * ethtot[a])) * 1.0;
+      if (k >= kmin + 1)
+        lo = 0.0 * 1.0 + 1.0 * ((prog[4 * NF + a] + rw1[a]) *
+                                (G.afact[k] * ethtot[a] + G.bfact[k] * ethtot[a - ks])) * 1.0;
+      dtot = -hv[2][a] - (hi - lo) * G.rdgz[k];
     }
-    for (int k = 0; k < d.kall; ++k) we[0][at(k)] = rs1(k);
+    const double r1 = rs1(k);
+    const int ke = (k < kmin) ? kmin : ((k > kmax) ? kmax : k);
+    const double pe_k = pe[at(ke)];
+    const double re1 = pe_k + (getot[a] + dtot) * dt + (kin10 - kin11) + (pg[a] - r1) * G.phi[k];
+    we[0][a] = r1;
+    we[2][a] = re1;
+    pm[0 * NF + a] = pm[0 * NF + a] + r1 * rweight;
+    pm[4 * NF + a] = pm[4 * NF + a] + rw1[a] * rweight;
   }
 };
 

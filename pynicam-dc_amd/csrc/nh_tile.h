@@ -1,4 +1,5 @@
-// LDS-tiled kernels for the triangle-based operators (3-D divergence damping).
+// LDS-tiled kernels for the triangle-based operators (3-D divergence damping,
+// the Kh-weighted diffusion of the second hyperdiffusion pass).
 //
 // A block owns a tile of one region level: all interior i (W = g-2 points,
 // one or more wavefronts) times TH rows.  Phase 1 computes the two triangle
@@ -173,6 +174,94 @@ struct DivDamp2T {
     out[0][a] = gd[0] - prd * gx / Rr;
     out[1][a] = gd[1] - prd * gy / Rr;
     out[2][a] = gd[2] - prd * gz / Rr;
+  }
+};
+
+// ----------------------------------------------------------------------------
+// second hyperdiffusion pass (mod_numfilter.py:1125-1264): Laplacians of the
+// velocity components and w, Kh-weighted diffusion (kernels/oprtdiffusion.py:59-177)
+// of temperature and density; the triangle gradients of the two diffused
+// fields (3 components x TI/TJ) go through LDS
+// ----------------------------------------------------------------------------
+struct HDiffPass2T {
+  static constexpr const char* kName = "hdiff_pass2";
+  Dims d; Consts c; Geo G; TileGeom tg;
+  const double* in[6];
+  double* out[6];
+  const double* rhog;
+  // LDS layout: [field f (0,1)][triangle T][component dd][row][col]
+  NH_HD int lidx(const TileAt& t, int f, int T, int dd, int cj_, int ci_) const {
+    const int W1 = tg.W + 1, R = t.rows + 1;
+    return (((f * 2 + T) * 3 + dd) * R + cj_) * W1 + ci_;
+  }
+  NH_HD void phase1(int blk, int tid, double* lds) const {
+    if (blk >= tg.ntiles) return;
+    const TileAt t = tile_at(d, tg, blk);
+    const int g = d.g, W1 = tg.W + 1;
+    const int nc = W1 * (t.rows + 1);
+    for (int cidx = tid; cidx < nc; cidx += 256) {
+      const int ci_ = cidx % W1, cj_ = cidx / W1;
+      const int i = ci_, j = t.j0 - 1 + cj_;
+      const long cj = ((long)t.l * g + j) * g + i;
+      const long a = ((long)(t.l * d.kall + t.k) * g + j) * g + i;
+      for (int f = 0; f < 2; ++f)
+        for (int T = 0; T < 2; ++T) {
+          double o3[3];
+          tri_grad(G, d.nreg, cj, T, in[4 + f], a, g, o3);
+          for (int dd = 0; dd < 3; ++dd) lds[lidx(t, f, T, dd, cj_, ci_)] = o3[dd];
+        }
+    }
+  }
+  NH_HD void phase2(int blk, int tid, const double* lds) const {
+    const int g = d.g;
+    if (blk >= tg.ntiles) {
+      if (tid == 0) {
+        HDiffPassP<1> p;
+        p.d = d; p.c = c; p.G = G; p.rhog = rhog;
+        for (int q = 0; q < 6; ++q) { p.in[q] = in[q]; p.out[q] = out[q]; }
+        p(d.lall * d.kall + (blk - tg.ntiles), 0);
+      }
+      return;
+    }
+    const TileAt t = tile_at(d, tg, blk);
+    const int ti = tid % tg.W, tj = tid / tg.W;
+    if (tj >= t.rows) return;
+    const int i = ti + 1, j = t.j0 + tj;
+    const long nreg = d.nreg;
+    const long ci = ((long)t.l * g + j) * g + i;
+    const long a = ((long)(t.l * d.kall + t.k) * g + j) * g + i;
+    for (int q = 0; q < 4; ++q) out[q][a] = -lap7(G, nreg, ci, in[q], a, g);
+    const bool pent = (i == 1 && j == 1);
+    const double m0 = G.sgp[t.l] ? 0.0 : 1.0, m1 = G.sgp[t.l] ? 1.0 : 0.0;
+    // corner offsets of the six triangles: TI(i,j) TJ(i,j) TI(i-1,j) TJ(i-1,j-1) TI(i-1,j-1) TJ(i,j-1)
+    const int cdi[6] = {0, 0, -1, -1, -1, 0}, cdj[6] = {0, 0, 0, -1, -1, -1}, cT[6] = {0, 1, 0, 1, 0, 1};
+    for (int f = 0; f < 2; ++f) {
+      const double fac = (f == 0) ? c.CVdry : c.HDIFF_FACT_RHO;
+      const double gam = c.GAMMA_H;
+      auto kh = [&](long x) { return rhog[x] * fac * gam; };
+      double tv[6][3];
+      for (int n = 0; n < 6; ++n)
+        for (int dd = 0; dd < 3; ++dd)
+          tv[n][dd] = lds[lidx(t, f, cT[n], dd, j + cdj[n] - t.j0 + 1, i + cdi[n])];
+      if (pent)
+        for (int dd = 0; dd < 3; ++dd) tv[4][dd] = tv[4][dd] * m0 + tv[5][dd] * m1;
+      const double k0 = kh(a);
+      const double kf1 = 0.5 * (k0 + kh(a + 1 + g));
+      const double kf2 = 0.5 * (k0 + kh(a + g));
+      const double kf3 = 0.5 * (k0 + kh(a - 1));
+      const double kf4 = 0.5 * (k0 + kh(a - 1 - g));
+      const double kf5 = 0.5 * (k0 + kh(a - g));
+      const double kf6 = 0.5 * (k0 + kh(a + 1));
+      double inner = 0.0;
+      for (int dd = 0; dd < 3; ++dd) {
+        const double* cd = G.cdiff + (long)(dd * 6) * nreg + ci;
+        const double blk2 = kf1 * cd[0] * (tv[0][dd] + tv[1][dd]) + kf2 * cd[nreg] * (tv[1][dd] + tv[2][dd])
+                          + kf3 * cd[2 * nreg] * (tv[2][dd] + tv[3][dd]) + kf4 * cd[3 * nreg] * (tv[3][dd] + tv[4][dd])
+                          + kf5 * cd[4 * nreg] * (tv[4][dd] + tv[5][dd]) + kf6 * cd[5 * nreg] * (tv[5][dd] + tv[0][dd]);
+        inner = (dd == 0) ? blk2 : inner + blk2;
+      }
+      out[4 + f][a] = -inner;
+    }
   }
 };
 
