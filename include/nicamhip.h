@@ -91,6 +91,9 @@ typedef struct {
    * pole: tri_w_pl [v][lpl][w], arc_hn_pl / arc_xr_pl [v][lpl][dir], rarea_pl [lpl] */
   const double *tri_w, *arc_hn, *rarea, *arc_xr;
   const double *tri_w_pl, *arc_hn_pl, *rarea_pl, *arc_xr_pl;
+  /* control-volume area (GMTR_p P_AREA, one per column) and layer thickness
+   * dgz [kall]: VOLUME = area * GSGAM2 * dgz (mod_vmtr.py) for the budgets */
+  const double *area, *dgz;
 } nh_geometry;
 
 typedef struct nh_context nh_context;
@@ -147,6 +150,15 @@ int nh_gradient(nh_context* ctx, const double* scl, double* gx, double* gy, doub
  * Srctr.src_tracer_advection (mod_src_tracer.py:83-1228). */
 int nh_dyn_step(nh_context* ctx, double* prog, double* progq, int32_t nq, double dtl, int32_t sstep_max,
                 void* stream);
+
+/* Global mass and energy sums (Embudget._store_var, mod_embudget.py:82-139):
+ * sum of var * VOLUME over the regional interior points of the physical
+ * levels (pole cells omitted, as in _gsum, mod_embudget.py:74-80), with
+ * compensated FP64 summation.  Tracer 0 of progq is qv, lhv its latent heat.
+ * out[0..5] = potential, internal, kinetic and total energy [J], dry-air
+ * mass and water-vapour mass [kg].  Synchronises the stream. */
+int nh_budget(nh_context* ctx, const double* prog, const double* progq, int32_t nq, double lhv, double* out,
+              void* stream);
 
 /* Kernel timers (the per-kernel counterpart of the section timers
  * PROF_rapstart / PROF_rapend / PROF_rapreport, share/mod_prof.py:81-140).

@@ -38,6 +38,11 @@ class Const:
     HDIFF_FACT_RHO: float = 1.0e-2
     ALPHA_D: float = 1.0e16            # DIRECT 3-D divergence damping coefficient
     LAP_ORDER_DIVDAMP: int = 2
+    # latent heat of vaporisation for the energy budget (mod_const.py:53-81, 183-192)
+    LHV0: float = 2.5008e6
+    CL: float = 4218.0
+    TEM00: float = 273.15
+    THERMODYN_TYPE: str = "SIMPLE"
     # tracer transport (mod_runconf.py:19, TRC_ADV_TYPE = MIURA2004)
     THUBURN_LIM: bool = True
     # time integration (mod_time.py:83-102)
@@ -56,6 +61,15 @@ class Const:
     @property
     def CVvap(self) -> float:
         return self.CPvap - self.Rvap
+
+    @property
+    def LHV(self) -> float:
+        """LHV0 for the SIMPLE energy type, LHV00 = LHV0 - (CPvap - CL) TEM00 for EXACT."""
+        if self.THERMODYN_TYPE == "EXACT":
+            return self.LHV0 - (self.CPvap - self.CL) * self.TEM00
+        if self.THERMODYN_TYPE in ("SIMPLE", "SIMPLE2"):
+            return self.LHV0
+        raise ValueError(f"unknown thermodynamics type {self.THERMODYN_TYPE!r}")
 
     @property
     def DTS(self) -> float:

@@ -174,6 +174,18 @@ int nh_dyn_step(nh_context* ctx, double* prog, double* progq, int32_t nq, double
   return 0;
 }
 
+int nh_budget(nh_context* ctx, const double* prog, const double* progq, int32_t nq, double lhv, double* out,
+              void* stream) {
+  const char* where = "nh_budget";
+  if (int rc = nh::check_ready(ctx, where, true)) return rc;
+  if (!prog || !progq || nq < 1 || !out) return nh::fail(NH_ERR_ARG, where, "need prog, qv and an output array");
+  NHX x(ctx->c.device, stream, &ctx->c);
+  int rc = nh::budget(x, ctx->c, prog, progq, lhv, out);
+  if (rc == NH_ERR_ARG) return nh::fail(rc, where, "grid too small for the partial-sum buffer");
+  if (rc) return nh::fail(NH_ERR_DEVICE, where, x.message());
+  return 0;
+}
+
 int nh_prof_enable(nh_context* ctx, int32_t on) {
   if (!ctx) return nh::fail(NH_ERR_ARG, "nh_prof_enable", "null context");
   NHX x(ctx->c.device, nullptr, &ctx->c);

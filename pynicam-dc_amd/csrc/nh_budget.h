@@ -1,0 +1,80 @@
+// Global mass and energy sums of the prognostic state
+// (Embudget._store_var, share/mod_embudget.py:82-139).
+//
+// One thread per interior point of the physical levels (i, j in 1..g-2,
+// k in kmin..kmax, all regions; pole cells are left out as in _gsum,
+// mod_embudget.py:74-80).  Phase 1 writes the six volume-weighted terms of
+// the point into LDS; phase 2 lets six threads sum one term each over the
+// block with Neumaier's compensated summation and store (sum, correction) as
+// the block's partial.  The host adds the partials, again compensated.
+#pragma once
+
+#include "nh_step.h"
+
+namespace nh {
+
+// Neumaier's improved Kahan summation: s + c carries the running sum
+NH_HD inline void neu_add(double& s, double& c, double v) {
+  const double t = s + v;
+  if (fabs(s) >= fabs(v)) c += (s - t) + v;
+  else c += (v - t) + s;
+  s = t;
+}
+
+enum BudgetTerm { B_POT = 0, B_INT, B_KIN, B_DRY, B_VAP, B_NTERMS };
+
+struct BudgetT {
+  static constexpr const char* kName = "budget";
+  Dims d; Consts c; Geo G;
+  const double* prog;
+  const double* rhogq;   // qv
+  double lhv;
+  long npts;             // interior points of the physical levels
+  double* part;          // [nblocks][B_NTERMS][2]
+
+  NH_HD void phase1(int blk, int t, double* lds) const {
+    const long n = (long)blk * 256 + t;
+    double v[B_NTERMS] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (n < npts) {
+      const int W = d.g - 2;
+      const int nz = d.kmax - d.kmin + 1;
+      const int i = 1 + (int)(n % W);
+      const int j = 1 + (int)((n / W) % W);
+      const long r = n / ((long)W * W);
+      const int k = d.kmin + (int)(r % nz);
+      const int l = (int)(r / nz);
+      const long ks = (long)d.g * d.g;
+      const long a = (((long)l * d.kall + k) * d.g + j) * d.g + i;
+      const long col = ((long)l * d.g + j) * d.g + i;
+      const long NF = d.NF;
+      const double rhog = prog[a];
+      const double rho = rhog / 1.0;                  // GSGAM2 = 1
+      const double ein = prog[5 * NF + a] / rhog;
+      const double q = rhogq[a] / rhog;
+      const double qd = 1.0 - q;
+      const double cv = q * c.CVW_qv + qd * c.CVdry;
+      const double tem = ein / cv;
+      const double vol = G.area[col] * 1.0 * G.dgz[k];  // area * GSGAM2 * dgz
+      double e = qd * c.CVdry * tem;
+      e = e + q * c.CVW_qv * tem;
+      e = e + q * lhv;
+      const double kin = rhogkin_k(G, d.kmin, d.kmax, k,
+                                   [&](int kk, int qq) { return prog[qq * NF + a + (long)(kk - k) * ks]; });
+      v[B_POT] = rho * G.phi[k] * vol;
+      v[B_INT] = rho * e * vol;
+      v[B_KIN] = kin * 1.0 * vol;                     // RGSGAM2 = 1
+      v[B_DRY] = rho * qd * vol;
+      v[B_VAP] = rho * q * vol;
+    }
+    for (int m = 0; m < B_NTERMS; ++m) lds[m * 256 + t] = v[m];
+  }
+  NH_HD void phase2(int blk, int t, double* lds) const {
+    if (t >= B_NTERMS) return;
+    double s = 0.0, cc = 0.0;
+    for (int u = 0; u < 256; ++u) neu_add(s, cc, lds[t * 256 + u]);
+    part[((long)blk * B_NTERMS + t) * 2] = s;
+    part[((long)blk * B_NTERMS + t) * 2 + 1] = cc;
+  }
+};
+
+}  // namespace nh

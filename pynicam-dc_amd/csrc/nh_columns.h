@@ -69,6 +69,9 @@ struct Geo {
   const double* ahn_pl;   // (v*2+lp)*3+d
   const double* rarea_pl; // lp
   const double* axr_pl;   // (v*2+lp)*3+d
+  // budgets: control-volume area per column, layer thickness per level
+  const double* area;     // 1
+  const double* dgz;      // [kall]
 };
 
 enum { I_RHOG = 0, I_RHOGVX, I_RHOGVY, I_RHOGVZ, I_RHOGW, I_RHOGE };

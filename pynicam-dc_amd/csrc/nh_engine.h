@@ -18,11 +18,13 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "nh_step.h"
 #include "nh_tracer.h"
 #include "nh_point.h"
 #include "nh_tile.h"
+#include "nh_budget.h"
 #include "../../include/nicamhip.h"
 
 namespace nh {
@@ -104,7 +106,7 @@ inline void set_consts(Consts& c, const nh_consts* in) {
 
 // geometry sizes in doubles, in upload order
 struct GeoSizes {
-  long reg[10];   // cdiv, cgrad, clap, cintp, cdiff, gx, twt, ahn, rarea, axr
+  long reg[11];   // cdiv, cgrad, clap, cintp, cdiff, gx, twt, ahn, rarea, axr, area
   long pl[10];
   long total;
 };
@@ -112,15 +114,14 @@ struct GeoSizes {
 inline GeoSizes geo_sizes(const Dims& d) {
   GeoSizes s;
   const long n = d.nreg;
-  const long rc[10] = {21, 21, 7, 18, 18, 3, 6, 9, 1, 9};
+  const long rc[11] = {21, 21, 7, 18, 18, 3, 6, 9, 1, 9, 1};
   const long pc[10] = {36, 36, 12, 108, 36, 36, 36, 36, 2, 36};
   s.total = 0;
-  for (int i = 0; i < 10; ++i) {
-    s.reg[i] = rc[i] * n;
-    s.pl[i] = pc[i];
-    s.total += s.reg[i] + s.pl[i];
-  }
-  s.total += 11L * d.kall;
+  for (int i = 0; i < 11; ++i) s.reg[i] = rc[i] * n;
+  for (int i = 0; i < 10; ++i) s.pl[i] = pc[i];
+  for (int i = 0; i < 11; ++i) s.total += s.reg[i];
+  for (int i = 0; i < 10; ++i) s.total += s.pl[i];
+  s.total += 12L * d.kall;
   return s;
 }
 
@@ -133,25 +134,27 @@ int upload_geometry(X& x, Context& ctx, const nh_geometry* g) {
     ctx.int_buf = (int*)x.alloc(d.lall * sizeof(int));
     if (!ctx.geo_buf || !ctx.int_buf) return NH_ERR_ALLOC;
   }
-  const double* regsrc[10] = {g->coef_div, g->coef_grad, g->coef_lap, g->coef_intp, g->coef_diff, g->grd_x,
-                              g->tri_w, g->arc_hn, g->rarea, g->arc_xr};
+  const double* regsrc[11] = {g->coef_div, g->coef_grad, g->coef_lap, g->coef_intp, g->coef_diff, g->grd_x,
+                              g->tri_w, g->arc_hn, g->rarea, g->arc_xr, g->area};
   const double* plsrc[10] = {g->coef_div_pl, g->coef_grad_pl, g->coef_lap_pl, g->coef_intp_pl, g->coef_diff_pl,
                              g->grd_x_pl, g->tri_w_pl, g->arc_hn_pl, g->rarea_pl, g->arc_xr_pl};
-  const double* vsrc[11] = {g->afact, g->bfact, g->cfact, g->dfact, g->rdgz, g->rdgzh,
-                            g->c2w0, g->c2w1, g->w2c0, g->w2c1, g->phi};
-  for (int i = 0; i < 10; ++i)
-    if (!regsrc[i] || !plsrc[i]) return NH_ERR_ARG;
+  const double* vsrc[12] = {g->afact, g->bfact, g->cfact, g->dfact, g->rdgz, g->rdgzh,
+                            g->c2w0, g->c2w1, g->w2c0, g->w2c1, g->phi, g->dgz};
   for (int i = 0; i < 11; ++i)
+    if (!regsrc[i]) return NH_ERR_ARG;
+  for (int i = 0; i < 10; ++i)
+    if (!plsrc[i]) return NH_ERR_ARG;
+  for (int i = 0; i < 12; ++i)
     if (!vsrc[i]) return NH_ERR_ARG;
   if (!g->sgp) return NH_ERR_ARG;
   double* p = ctx.geo_buf;
-  const double** regdst[10] = {&ctx.G.cdiv, &ctx.G.cgrad, &ctx.G.clap, &ctx.G.cintp, &ctx.G.cdiff, &ctx.G.gx,
-                               &ctx.G.twt, &ctx.G.ahn, &ctx.G.rarea, &ctx.G.axr};
+  const double** regdst[11] = {&ctx.G.cdiv, &ctx.G.cgrad, &ctx.G.clap, &ctx.G.cintp, &ctx.G.cdiff, &ctx.G.gx,
+                               &ctx.G.twt, &ctx.G.ahn, &ctx.G.rarea, &ctx.G.axr, &ctx.G.area};
   const double** pldst[10] = {&ctx.G.cdiv_pl, &ctx.G.cgrad_pl, &ctx.G.clap_pl, &ctx.G.cintp_pl, &ctx.G.cdiff_pl,
                               &ctx.G.gx_pl, &ctx.G.twt_pl, &ctx.G.ahn_pl, &ctx.G.rarea_pl, &ctx.G.axr_pl};
-  const double** vdst[11] = {&ctx.G.afact, &ctx.G.bfact, &ctx.G.cfact, &ctx.G.dfact, &ctx.G.rdgz, &ctx.G.rdgzh,
-                             &ctx.G.c2w0, &ctx.G.c2w1, &ctx.G.w2c0, &ctx.G.w2c1, &ctx.G.phi};
-  for (int i = 0; i < 10; ++i) {
+  const double** vdst[12] = {&ctx.G.afact, &ctx.G.bfact, &ctx.G.cfact, &ctx.G.dfact, &ctx.G.rdgz, &ctx.G.rdgzh,
+                             &ctx.G.c2w0, &ctx.G.c2w1, &ctx.G.w2c0, &ctx.G.w2c1, &ctx.G.phi, &ctx.G.dgz};
+  for (int i = 0; i < 11; ++i) {
     x.h2d(p, regsrc[i], s.reg[i] * sizeof(double));
     *regdst[i] = p;
     p += s.reg[i];
@@ -161,7 +164,7 @@ int upload_geometry(X& x, Context& ctx, const nh_geometry* g) {
     *pldst[i] = p;
     p += s.pl[i];
   }
-  for (int i = 0; i < 11; ++i) {
+  for (int i = 0; i < 12; ++i) {
     x.h2d(p, vsrc[i], d.kall * sizeof(double));
     *vdst[i] = p;
     p += d.kall;
@@ -337,6 +340,39 @@ void tracer_transport(X& x, const Context& ctx, double* progq, int nq, double dt
 // ----------------------------------------------------------------------------
 // one large step (mod_dynamics.py:1725-2741 with the RK3/HEVI schedule)
 // ----------------------------------------------------------------------------
+// global budgets (Embudget._store_var, mod_embudget.py:82-139): block partials
+// on the device into the W_GT scratch field, compensated host sum of them
+template <class X>
+int budget(X& x, const Context& ctx, const double* prog, const double* progq, double lhv, double out[6]) {
+  const Dims& d = ctx.d;
+  const int W = d.g - 2;
+  const long npts = (long)d.lall * (d.kmax - d.kmin + 1) * W * W;
+  const int nblocks = (int)((npts + 255) / 256);
+  if ((long)nblocks * B_NTERMS * 2 > 6 * d.NF) return NH_ERR_ARG;
+  double* part = ctx.field(W_GT);
+  BudgetT f{d, ctx.c, ctx.G, prog, progq, lhv, npts, part};
+  x.run_tiles(f, nblocks, B_NTERMS * 256);
+  std::vector<double> h((size_t)nblocks * B_NTERMS * 2);
+  x.d2h(h.data(), part, h.size() * sizeof(double));
+  if (x.check("budget")) return NH_ERR_DEVICE;
+  double sum[B_NTERMS];
+  for (int m = 0; m < B_NTERMS; ++m) {
+    double s = 0.0, c = 0.0;
+    for (int b = 0; b < nblocks; ++b) {
+      neu_add(s, c, h[((size_t)b * B_NTERMS + m) * 2]);
+      neu_add(s, c, h[((size_t)b * B_NTERMS + m) * 2 + 1]);
+    }
+    sum[m] = s + c;
+  }
+  out[0] = sum[B_POT];
+  out[1] = sum[B_INT];
+  out[2] = sum[B_KIN];
+  out[3] = sum[B_POT] + sum[B_INT] + sum[B_KIN];   // etot = pot + eint + ekin
+  out[4] = sum[B_DRY];
+  out[5] = sum[B_VAP];
+  return 0;
+}
+
 template <class X>
 int dyn_step(X& x, const Context& ctx, double* prog, double* progq, int nq, double dtl, int sstep_max) {
   const Dims& d = ctx.d;

@@ -92,6 +92,11 @@ class HipExec {
   void d2d(void* dst, const void* src, size_t bytes) {
     note(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s_), "hipMemcpyAsync D2D");
   }
+  // ordered after the stream's work; returns when the bytes are on the host
+  void d2h(void* dst, const void* src, size_t bytes) {
+    note(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s_), "hipMemcpyAsync D2H");
+    note(hipStreamSynchronize(s_), "hipStreamSynchronize");
+  }
   template <class F>
   void run(const F& f, long n) {
     if (n <= 0) return;

@@ -187,6 +187,20 @@ class HipDynamics:
                                           C.c_void_p(self.progq.data_ptr()), self.progq.shape[0],
                                           self.dtl, self.sstep_max, self.ctx.stream()), "nh_dyn_step")
 
+    def budget(self, lhv: float | None = None) -> dict:
+        """Global energies and masses of the current state (Embudget._store_var, mod_embudget.py:82-139).
+
+        Keys as the reference's monitor: pot, int, kin, tot [J], dry, vap [kg].
+        ``lhv`` defaults to the constant set's latent heat (CONST_LHV).
+        """
+        lib, c = self.ctx.lib, self.ctx.const
+        out = (C.c_double * 6)()
+        lhv = float(c.LHV if lhv is None else lhv)
+        native.check(lib, lib.nh_budget(self.ctx.handle, C.c_void_p(self.prog.data_ptr()),
+                                        C.c_void_p(self.progq.data_ptr()), self.progq.shape[0], lhv, out,
+                                        self.ctx.stream()), "nh_budget")
+        return dict(zip(("pot", "int", "kin", "tot", "dry", "vap"), list(out)))
+
     def state(self) -> dict:
         """Copy the state back to the host in the reference layout."""
         self.ctx.sync()

@@ -119,6 +119,9 @@ def geometry_arrays(geo) -> dict:
         "arc_hn_pl": np.ascontiguousarray(geo.gmtr_a_pl[..., 0:3]).reshape(-1),  # (6, 2, 3)
         "rarea_pl": np.ascontiguousarray(geo.gmtr_p_pl[0, :, 1]).reshape(-1),   # (2,)
         "arc_xr_pl": np.ascontiguousarray(geo.xr_pl).reshape(-1),               # (6, 2, 3)
+        # budgets (GMTR_p P_AREA per column, layer thickness)
+        "area": _soa(geo.gmtr_p[..., 0]),
+        "dgz": np.ascontiguousarray(geo.vgrid.dgz, dtype=np.float64),
     }
     sizes = {"coef_div_pl": 36, "coef_grad_pl": 36, "coef_lap_pl": 12, "coef_intp_pl": 108,
              "coef_diff_pl": 36, "grd_x_pl": 36, "tri_w_pl": 36, "arc_hn_pl": 36, "rarea_pl": 2,
@@ -127,7 +130,7 @@ def geometry_arrays(geo) -> dict:
         if out[k].size != n:
             raise ValueError(f"{k} has {out[k].size} entries, expected {n}")
     for k in ("coef_div", "coef_grad", "coef_lap", "coef_intp", "coef_diff", "grd_x", "tri_w", "arc_hn",
-              "rarea", "arc_xr", "tri_w_pl", "arc_hn_pl", "rarea_pl", "arc_xr_pl"):
+              "rarea", "arc_xr", "tri_w_pl", "arc_hn_pl", "rarea_pl", "arc_xr_pl", "area"):
         out[k] = np.ascontiguousarray(out[k], dtype=np.float64)
     out.update({k: np.ascontiguousarray(v, dtype=np.float64) for k, v in vertical_profiles(geo).items()})
     return out

@@ -34,7 +34,8 @@ _GEO_FIELDS = [
 _VERT_FIELDS = ["afact", "bfact", "cfact", "dfact", "rdgz", "rdgzh", "c2w0", "c2w1", "w2c0", "w2c1", "phi"]
 
 
-_TRC_FIELDS = ["tri_w", "arc_hn", "rarea", "arc_xr", "tri_w_pl", "arc_hn_pl", "rarea_pl", "arc_xr_pl"]
+_TRC_FIELDS = ["tri_w", "arc_hn", "rarea", "arc_xr", "tri_w_pl", "arc_hn_pl", "rarea_pl", "arc_xr_pl",
+               "area", "dgz"]
 
 
 class NhGeometry(C.Structure):
@@ -57,6 +58,8 @@ EXPORTS = {
     "nh_gradient": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4 + [C.c_void_p]),
     "nh_dyn_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_int32,
                               C.c_void_p]),
+    "nh_budget": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double,
+                            C.POINTER(C.c_double), C.c_void_p]),
     "nh_prof_enable": (C.c_int, [C.c_void_p, C.c_int32]),
     "nh_prof_report": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.POINTER(C.c_double),
                                  C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),

@@ -61,6 +61,7 @@ class HostExec {
   void zero(void* p, size_t bytes) { std::memset(p, 0, bytes); }
   void h2d(void* dst, const void* src, size_t bytes) { std::memcpy(dst, src, bytes); }
   void d2d(void* dst, const void* src, size_t bytes) { std::memmove(dst, src, bytes); }
+  void d2h(void* dst, const void* src, size_t bytes) { std::memcpy(dst, src, bytes); }
   template <class F>
   void run(const F& f, long n) {
     auto t0 = std::chrono::steady_clock::now();

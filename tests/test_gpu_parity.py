@@ -157,5 +157,27 @@ def test_mass_conservation_gl06(gpu):
         st = dyn.state()
         assert np.all(np.isfinite(interior(st["PROG"], t)))
         assert abs(mass(st["PROG"], st["PROG_pl"]) / m0 - 1.0) < 1e-13
+        # the device budgets of the stepped state against the oracle's sums of it
+        from oracle import budget as ob
+        from oracle import dynamics as od
+        want = ob.store_var(od.OracleContext(geo, case.const, None), st["PROG"], st["PROGq"], case.const.LHV)
+        got = dyn.budget()
+        for key, v in want.items():
+            assert abs(got[key] - v) <= 1e-13 * max(abs(v), abs(want["tot"]) * 1e-6), (key, got[key], v)
     finally:
         ctx.close()
+
+
+def test_budget_gl05(gpu, ctx5, case_gl05):
+    """Global energies and masses of a moist gl05 z40 state against the oracle."""
+    from oracle import budget as ob
+    from oracle import dynamics as od
+    rng = np.random.default_rng(11)
+    s = {k: v.copy() for k, v in case_gl05.state.items()}
+    s["PROGq"][..., 0] = s["PROG"][..., 0] * 2e-3 * (1.0 + rng.random(s["PROG"].shape[:-1]))
+    s["PROGq_pl"][..., 0] = s["PROG_pl"][..., 0] * 2e-3
+    want = ob.store_var(od.OracleContext(case_gl05.geo, case_gl05.const, None), s["PROG"], s["PROGq"],
+                        case_gl05.const.LHV)
+    got = hipdyn.HipDynamics(ctx5, s).budget()
+    for key, v in want.items():
+        assert abs(got[key] - v) <= 1e-13 * abs(v), (key, got[key], v)

@@ -135,3 +135,33 @@ def test_host_build_step_matches_oracle(case_gl04, state_gl04_tracers, emu_ctx):
         a, b = interior(ref.PROGq[..., q], t), interior(got["PROGq"][..., q], t)
         ap, bp = pole_centre(ref.PROGq_pl[..., q], t), pole_centre(got["PROGq_pl"][..., q], t)
         assert rel_err(a, b, ap, bp) < 1e-12, q
+
+
+def _moist(state, seed=3):
+    """The gl04 state with a positive, varying qv."""
+    rng = np.random.default_rng(seed)
+    s = {k: v.copy() for k, v in state.items()}
+    s["PROGq"][..., 0] = s["PROG"][..., 0] * 1e-3 * (1.0 + 0.5 * rng.random(s["PROG"].shape[:-1]))
+    s["PROGq_pl"][..., 0] = s["PROG_pl"][..., 0] * 1e-3
+    return s
+
+
+def test_host_build_budget_matches_oracle(case_gl04, state_gl04_tracers, emu_ctx):
+    from oracle import budget as ob
+    from oracle import dynamics as od
+    c = case_gl04
+    s = _moist(state_gl04_tracers)
+    want = ob.store_var(od.OracleContext(c.geo, c.const, None), s["PROG"], s["PROGq"], c.const.LHV)
+    got = hipdyn.HipDynamics(emu_ctx, s).budget()
+    assert set(got) == set(want)
+    for key, v in want.items():
+        assert abs(got[key] - v) <= 1e-13 * abs(v), (key, got[key], v)
+
+
+def test_budget_rejects_missing_arguments(emu_ctx):
+    lib = emu_ctx.lib
+    prog = emu_ctx.empty(6)
+    out = (C.c_double * 6)()
+    rc = lib.nh_budget(emu_ctx.handle, C.c_void_p(prog.data_ptr()), C.c_void_p(None), 1, 2.5e6, out,
+                       C.c_void_p(None))
+    assert rc == -1 and b"nh_budget" in lib.nh_last_error()

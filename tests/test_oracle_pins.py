@@ -250,3 +250,64 @@ def test_tracer_limiters_keep_bounds(case_gl04):
     assert q[..., 2].min() >= -1e-14 and q[..., 2].max() <= 1.0 + 1e-13
     q_nl, _ = _tracer_run(case_gl04, 0.0, False)
     assert q_nl[..., 2].min() < -1e-6 or q_nl[..., 2].max() > 1.0 + 1e-6
+
+
+def _budget_state(geo, topo, rho_k, q, T, U, w0, const):
+    """Horizontally uniform state: rho(k), qv = q, temperature T, wind U, rhogw = rho w0 on inner faces."""
+    g, kall, L = topo.gall_1d, topo.kall, topo.lall
+    cv = (1.0 - q) * const.CVdry + q * const.CVW[0]
+    P = np.zeros((g, g, kall, L, 6))
+    Pp = np.zeros((6, kall, 2, 6))
+    for arr in (P, Pp):
+        r = rho_k.reshape((1, 1, kall, 1) if arr is P else (1, kall, 1))
+        arr[..., 0] = r
+        for d in range(3):
+            arr[..., 1 + d] = r * U[d]
+        arr[..., 5] = r * cv * T
+    kf = slice(topo.kmin + 1, topo.kmax + 1)
+    P[:, :, kf, :, 4] = rho_k[kf].reshape(1, 1, -1, 1) * w0
+    Q = (q * P[..., 0])[..., None]
+    return P, Pp, Q
+
+
+def test_budget_closed_forms(case_gl04):
+    """Energies and masses of horizontally uniform states against their closed forms.
+
+    With GSGAM2 = 1 and level centres midway between the faces, the volume sums
+    reduce to (interior area) x (sums over the vertical grid):
+    dry/vapour mass  A sum_k rho_k (1-q) dz_k, A sum_k rho_k q dz_k;
+    potential energy A sum_k rho_k g z_k dz_k;
+    internal energy  A sum_k rho_k (cv T + q LHV) dz_k;
+    kinetic energy   A sum_k rho_k |U|^2/2 dz_k, plus, for uniform rho and
+    rhogw = rho w0 on the inner faces, rho w0^2/2 A (z_kmax - z_kmin).
+    """
+    from oracle import budget as ob
+    c = case_gl04
+    geo, topo, const = c.geo, c.topo, c.const
+    vg = geo.vgrid
+    k = slice(topo.kmin, topo.kmax + 1)
+    assert np.array_equal(vg.gz[k], 0.5 * (vg.gzh[k] + vg.gzh[topo.kmin + 1:topo.kmax + 2]))
+    g = topo.gall_1d
+    A = geo.area[1:g - 1, 1:g - 1].sum()
+    dz, z = vg.dgz[k], vg.gz[k]
+    ctx = _octx(geo, const)
+    lhv, q, T = const.LHV, 4e-3, 263.0
+    cv = (1.0 - q) * const.CVdry + q * const.CVW[0]
+    U = np.array([12.0, -7.0, 3.0])
+    rho_k = 1.2 * np.exp(-vg.gz / 8000.0)
+    P, Pp, Q = _budget_state(geo, topo, rho_k, q, T, U, 0.0, const)
+    b = ob.store_var(ctx, P, Q, lhv)
+    rz = rho_k[k] * dz
+    want = {"dry": A * np.sum(rz * (1.0 - q)), "vap": A * np.sum(rz * q),
+            "pot": A * np.sum(rz * const.GRAV * z), "int": A * np.sum(rz * (cv * T + q * lhv)),
+            "kin": A * np.sum(rz) * 0.5 * U @ U}
+    for key, v in want.items():
+        assert abs(b[key] / v - 1.0) < 1e-12, key
+    assert b["tot"] == b["pot"] + b["int"] + b["kin"]
+    # vertical motion only, uniform density
+    rho0, w0 = 0.9, 2.5
+    P, Pp, Q = _budget_state(geo, topo, np.full(topo.kall, rho0), 0.0, T, np.zeros(3), w0, const)
+    b = ob.store_var(ctx, P, Q, lhv)
+    want_w = 0.5 * rho0 * w0 ** 2 * A * (vg.gz[topo.kmax] - vg.gz[topo.kmin])
+    assert abs(b["kin"] / want_w - 1.0) < 1e-12
+    assert b["vap"] == 0.0
