@@ -94,6 +94,10 @@ typedef struct {
   /* control-volume area (GMTR_p P_AREA, one per column) and layer thickness
    * dgz [kall]: VOLUME = area * GSGAM2 * dgz (mod_vmtr.py) for the budgets */
   const double *area, *dgz;
+  /* prescribed tracer winds: latitude/longitude [rad] per regional column
+   * (GRD_LAT/GRD_LON) and [v][lpl] at the poles, full- and half-level
+   * heights gz, gzh [kall] (GRD_vz Z/ZH under flat topography) */
+  const double *lat, *lon, *lat_pl, *lon_pl, *gz, *gzh;
 } nh_geometry;
 
 typedef struct nh_context nh_context;
@@ -150,6 +154,25 @@ int nh_gradient(nh_context* ctx, const double* scl, double* gx, double* gy, doub
  * Srctr.src_tracer_advection (mod_src_tracer.py:83-1228). */
 int nh_dyn_step(nh_context* ctx, double* prog, double* progq, int32_t nq, double dtl, int32_t sstep_max,
                 void* stream);
+
+/* Offline tracer step (TIME_integ_type 'TRCADV', Dyn.dynamics_step,
+ * mod_dynamics.py:2030-2076): MIURA2004 transport of progq
+ * (src_tracer_advection, mod_src_tracer.py:83-1228) by the mass fluxes held in
+ * prog (rhog, rhogvx..vz, rhogw; rhog is both the initial and the mean
+ * density), with zero density tendency, then the halo fill of progq. */
+int nh_tracer_step(nh_context* ctx, const double* prog, double* progq, int32_t nq, double dtl, void* stream);
+
+/* Prescribed winds of the offline tracer runs (Forcing.forcing_update,
+ * mod_forcing.py:113-206): rhogv = v(t) rhog, rhogw = w(t) rhog at every
+ * point, prog's density unchanged.
+ * kind NH_WIND_DCMIP11: Trcadv.test11_velocity (mod_af_trcadv.py:23-98),
+ *   params unused;
+ * kind NH_WIND_DEFORM: non-divergent, time-reversing deformational flow
+ *   with vertical motion, params = {period [s], umax [m/s], w0 [m/s],
+ *   ztop [m]}. */
+enum { NH_WIND_DCMIP11 = 0, NH_WIND_DEFORM = 1 };
+int nh_trcadv_winds(nh_context* ctx, double* prog, double time, int32_t kind, const double* params,
+                    void* stream);
 
 /* Global mass and energy sums (Embudget._store_var, mod_embudget.py:82-139):
  * sum of var * VOLUME over the regional interior points of the physical

@@ -174,10 +174,35 @@ int nh_dyn_step(nh_context* ctx, double* prog, double* progq, int32_t nq, double
   return 0;
 }
 
+int nh_tracer_step(nh_context* ctx, const double* prog, double* progq, int32_t nq, double dtl, void* stream) {
+  const char* where = "nh_tracer_step";
+  if (int rc = nh::check_ready(ctx, where, true)) return rc;
+  if (!prog || !progq || nq < 1) return nh::fail(NH_ERR_ARG, where, "need prog and at least one tracer field");
+  if (!(dtl > 0.0)) return nh::fail(NH_ERR_ARG, where, "dtl must be > 0");
+  NHX x(ctx->c.device, stream, &ctx->c);
+  int rc = nh::tracer_step(x, ctx->c, prog, progq, nq, dtl);
+  if (rc) return nh::fail(NH_ERR_DEVICE, where, x.message());
+  return 0;
+}
+
+int nh_trcadv_winds(nh_context* ctx, double* prog, double time, int32_t kind, const double* params,
+                    void* stream) {
+  const char* where = "nh_trcadv_winds";
+  if (int rc = nh::check_ready(ctx, where, false)) return rc;
+  if (!prog) return nh::fail(NH_ERR_ARG, where, "null prog");
+  if (kind != NH_WIND_DCMIP11 && kind != NH_WIND_DEFORM) return nh::fail(NH_ERR_ARG, where, "unknown wind kind");
+  if (kind == NH_WIND_DEFORM && (!params || !(params[0] > 0.0) || !(params[3] > 0.0)))
+    return nh::fail(NH_ERR_ARG, where, "deformational flow needs period > 0 and ztop > 0");
+  NHX x(ctx->c.device, stream, &ctx->c);
+  int rc = nh::trcadv_winds(x, ctx->c, prog, time, kind, params);
+  if (rc) return nh::fail(NH_ERR_DEVICE, where, x.message());
+  return 0;
+}
+
 int nh_budget(nh_context* ctx, const double* prog, const double* progq, int32_t nq, double lhv, double* out,
               void* stream) {
   const char* where = "nh_budget";
-  if (int rc = nh::check_ready(ctx, where, true)) return rc;
+  if (int rc = nh::check_ready(ctx, where, false)) return rc;
   if (!prog || !progq || nq < 1 || !out) return nh::fail(NH_ERR_ARG, where, "need prog, qv and an output array");
   NHX x(ctx->c.device, stream, &ctx->c);
   int rc = nh::budget(x, ctx->c, prog, progq, lhv, out);

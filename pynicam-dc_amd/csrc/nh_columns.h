@@ -72,6 +72,12 @@ struct Geo {
   // budgets: control-volume area per column, layer thickness per level
   const double* area;     // 1
   const double* dgz;      // [kall]
+  // prescribed winds: latitude, longitude per column, heights per level
+  const double* lat;      // 1
+  const double* lon;      // 1
+  const double* lat_pl;   // v*2+lp
+  const double* lon_pl;   // v*2+lp
+  const double *gz, *gzh; // [kall]
 };
 
 enum { I_RHOG = 0, I_RHOGVX, I_RHOGVY, I_RHOGVZ, I_RHOGW, I_RHOGE };

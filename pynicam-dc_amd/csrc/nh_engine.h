@@ -25,6 +25,7 @@
 #include "nh_point.h"
 #include "nh_tile.h"
 #include "nh_budget.h"
+#include "nh_trcadv.h"
 #include "../../include/nicamhip.h"
 
 namespace nh {
@@ -104,73 +105,69 @@ inline void set_consts(Consts& c, const nh_consts* in) {
   c.THUBURN_LIM = in->thuburn_lim;
 }
 
-// geometry sizes in doubles, in upload order
-struct GeoSizes {
-  long reg[11];   // cdiv, cgrad, clap, cintp, cdiff, gx, twt, ahn, rarea, axr, area
-  long pl[10];
-  long total;
-};
+// geometry arrays in upload order: regional [comp][nreg], pole [v][lp][...],
+// vertical profiles [kall]
+constexpr int kNReg = 13, kNPl = 12, kNVert = 14;
 
-inline GeoSizes geo_sizes(const Dims& d) {
-  GeoSizes s;
-  const long n = d.nreg;
-  const long rc[11] = {21, 21, 7, 18, 18, 3, 6, 9, 1, 9, 1};
-  const long pc[10] = {36, 36, 12, 108, 36, 36, 36, 36, 2, 36};
-  s.total = 0;
-  for (int i = 0; i < 11; ++i) s.reg[i] = rc[i] * n;
-  for (int i = 0; i < 10; ++i) s.pl[i] = pc[i];
-  for (int i = 0; i < 11; ++i) s.total += s.reg[i];
-  for (int i = 0; i < 10; ++i) s.total += s.pl[i];
-  s.total += 12L * d.kall;
-  return s;
+inline long geo_total(const Dims& d) {
+  static const long rc[kNReg] = {21, 21, 7, 18, 18, 3, 6, 9, 1, 9, 1, 1, 1};
+  static const long pc[kNPl] = {36, 36, 12, 108, 36, 36, 36, 36, 2, 36, 12, 12};
+  long t = (long)kNVert * d.kall;
+  for (int i = 0; i < kNReg; ++i) t += rc[i] * d.nreg;
+  for (int i = 0; i < kNPl; ++i) t += pc[i];
+  return t;
 }
 
 template <class X>
 int upload_geometry(X& x, Context& ctx, const nh_geometry* g) {
   const Dims& d = ctx.d;
-  GeoSizes s = geo_sizes(d);
+  static const long rc[kNReg] = {21, 21, 7, 18, 18, 3, 6, 9, 1, 9, 1, 1, 1};
+  static const long pc[kNPl] = {36, 36, 12, 108, 36, 36, 36, 36, 2, 36, 12, 12};
   if (!ctx.geo_buf) {
-    ctx.geo_buf = (double*)x.alloc(s.total * sizeof(double));
+    ctx.geo_buf = (double*)x.alloc(geo_total(d) * sizeof(double));
     ctx.int_buf = (int*)x.alloc(d.lall * sizeof(int));
     if (!ctx.geo_buf || !ctx.int_buf) return NH_ERR_ALLOC;
   }
-  const double* regsrc[11] = {g->coef_div, g->coef_grad, g->coef_lap, g->coef_intp, g->coef_diff, g->grd_x,
-                              g->tri_w, g->arc_hn, g->rarea, g->arc_xr, g->area};
-  const double* plsrc[10] = {g->coef_div_pl, g->coef_grad_pl, g->coef_lap_pl, g->coef_intp_pl, g->coef_diff_pl,
-                             g->grd_x_pl, g->tri_w_pl, g->arc_hn_pl, g->rarea_pl, g->arc_xr_pl};
-  const double* vsrc[12] = {g->afact, g->bfact, g->cfact, g->dfact, g->rdgz, g->rdgzh,
-                            g->c2w0, g->c2w1, g->w2c0, g->w2c1, g->phi, g->dgz};
-  for (int i = 0; i < 11; ++i)
+  const double* regsrc[kNReg] = {g->coef_div, g->coef_grad, g->coef_lap, g->coef_intp, g->coef_diff,
+                                 g->grd_x,    g->tri_w,     g->arc_hn,   g->rarea,      g->arc_xr,
+                                 g->area,     g->lat,       g->lon};
+  const double* plsrc[kNPl] = {g->coef_div_pl, g->coef_grad_pl, g->coef_lap_pl, g->coef_intp_pl,
+                               g->coef_diff_pl, g->grd_x_pl,    g->tri_w_pl,    g->arc_hn_pl,
+                               g->rarea_pl,     g->arc_xr_pl,   g->lat_pl,      g->lon_pl};
+  const double* vsrc[kNVert] = {g->afact, g->bfact, g->cfact, g->dfact, g->rdgz, g->rdgzh, g->c2w0,
+                                g->c2w1,  g->w2c0,  g->w2c1,  g->phi,   g->dgz,  g->gz,   g->gzh};
+  for (int i = 0; i < kNReg; ++i)
     if (!regsrc[i]) return NH_ERR_ARG;
-  for (int i = 0; i < 10; ++i)
+  for (int i = 0; i < kNPl; ++i)
     if (!plsrc[i]) return NH_ERR_ARG;
-  for (int i = 0; i < 12; ++i)
+  for (int i = 0; i < kNVert; ++i)
     if (!vsrc[i]) return NH_ERR_ARG;
   if (!g->sgp) return NH_ERR_ARG;
   double* p = ctx.geo_buf;
-  const double** regdst[11] = {&ctx.G.cdiv, &ctx.G.cgrad, &ctx.G.clap, &ctx.G.cintp, &ctx.G.cdiff, &ctx.G.gx,
-                               &ctx.G.twt, &ctx.G.ahn, &ctx.G.rarea, &ctx.G.axr, &ctx.G.area};
-  const double** pldst[10] = {&ctx.G.cdiv_pl, &ctx.G.cgrad_pl, &ctx.G.clap_pl, &ctx.G.cintp_pl, &ctx.G.cdiff_pl,
-                              &ctx.G.gx_pl, &ctx.G.twt_pl, &ctx.G.ahn_pl, &ctx.G.rarea_pl, &ctx.G.axr_pl};
-  const double** vdst[12] = {&ctx.G.afact, &ctx.G.bfact, &ctx.G.cfact, &ctx.G.dfact, &ctx.G.rdgz, &ctx.G.rdgzh,
-                             &ctx.G.c2w0, &ctx.G.c2w1, &ctx.G.w2c0, &ctx.G.w2c1, &ctx.G.phi, &ctx.G.dgz};
-  for (int i = 0; i < 11; ++i) {
-    x.h2d(p, regsrc[i], s.reg[i] * sizeof(double));
+  Geo& G = ctx.G;
+  const double** regdst[kNReg] = {&G.cdiv, &G.cgrad, &G.clap, &G.cintp, &G.cdiff, &G.gx,  &G.twt,
+                                  &G.ahn,  &G.rarea, &G.axr,  &G.area,  &G.lat,   &G.lon};
+  const double** pldst[kNPl] = {&G.cdiv_pl, &G.cgrad_pl, &G.clap_pl,  &G.cintp_pl, &G.cdiff_pl, &G.gx_pl,
+                                &G.twt_pl,  &G.ahn_pl,   &G.rarea_pl, &G.axr_pl,   &G.lat_pl,   &G.lon_pl};
+  const double** vdst[kNVert] = {&G.afact, &G.bfact, &G.cfact, &G.dfact, &G.rdgz, &G.rdgzh, &G.c2w0,
+                                 &G.c2w1,  &G.w2c0,  &G.w2c1,  &G.phi,   &G.dgz,  &G.gz,    &G.gzh};
+  for (int i = 0; i < kNReg; ++i) {
+    x.h2d(p, regsrc[i], rc[i] * d.nreg * sizeof(double));
     *regdst[i] = p;
-    p += s.reg[i];
+    p += rc[i] * d.nreg;
   }
-  for (int i = 0; i < 10; ++i) {
-    x.h2d(p, plsrc[i], s.pl[i] * sizeof(double));
+  for (int i = 0; i < kNPl; ++i) {
+    x.h2d(p, plsrc[i], pc[i] * sizeof(double));
     *pldst[i] = p;
-    p += s.pl[i];
+    p += pc[i];
   }
-  for (int i = 0; i < 12; ++i) {
+  for (int i = 0; i < kNVert; ++i) {
     x.h2d(p, vsrc[i], d.kall * sizeof(double));
     *vdst[i] = p;
     p += d.kall;
   }
   x.h2d(ctx.int_buf, g->sgp, d.lall * sizeof(int));
-  ctx.G.sgp = ctx.int_buf;
+  G.sgp = ctx.int_buf;
   ctx.geo_ready = true;
   return x.check("upload_geometry");
 }
@@ -246,12 +243,13 @@ void divdamp(X& x, const Context& ctx, const double* vx, const double* vy, const
 // tracer transport with the time-mean mass fluxes of the last stage
 // (mod_dynamics.py:2346-2461 -> mod_src_tracer.py:83-1228)
 // ----------------------------------------------------------------------------
+// pm: mass fluxes (rhog, rhogvx, rhogvy, rhogvz, rhogw) with field stride NF;
+// rhog_in: density at the start of the step; frhog: density tendency
 template <class X>
-void tracer_transport(X& x, const Context& ctx, double* progq, int nq, double dt) {
+void tracer_transport(X& x, const Context& ctx, double* progq, int nq, double dt, const double* pm,
+                      const double* rhog_in, const double* frhog) {
   const Dims& d = ctx.d;
   const long NF = d.NF, nc = ncols(d);
-  const double* pm = ctx.field(W_PM);
-  const double* frhog = ctx.field(W_FRHOG);
   double* rhog1 = ctx.field(W_RHOG1);
   double* rhog2 = ctx.field(W_RHOG2);
   double* flxv = ctx.field(W_FLXV);
@@ -260,7 +258,7 @@ void tracer_transport(X& x, const Context& ctx, double* progq, int nq, double dt
     f.d = d; f.c = ctx.c; f.G = ctx.G;
     f.dt = dt;
     f.rw_mean = pm + 4 * NF;
-    f.rhog_in = ctx.field(W_PROG0);
+    f.rhog_in = rhog_in;
     f.frhog = frhog;
     f.flxv = flxv; f.rhog1 = rhog1;
     f.rhogq = progq; f.nq = nq;
@@ -335,6 +333,28 @@ void tracer_transport(X& x, const Context& ctx, double* progq, int nq, double dt
       x.run(f, nc);
     }
   }
+}
+
+// offline tracer step (TIME_integ_type 'TRCADV', mod_dynamics.py:2030-2076):
+// the fluxes are prog's own momentum, rhog is the initial and the mean density,
+// the density tendency is zero; the step ends with the halo fill of progq
+template <class X>
+int tracer_step(X& x, const Context& ctx, const double* prog, double* progq, int nq, double dtl) {
+  const Dims& d = ctx.d;
+  double* frhog = ctx.field(W_FRHOG);
+  x.run(FillZero{frhog}, d.NF);
+  tracer_transport(x, ctx, progq, nq, dtl, prog, prog, frhog);
+  halo_block(x, ctx, progq, nq);
+  return x.check("tracer_step");
+}
+
+template <class X>
+int trcadv_winds(X& x, const Context& ctx, double* prog, double time, int kind, const double* par) {
+  TrcadvWindsP f;
+  f.d = ctx.d; f.G = ctx.G; f.prog = prog; f.time = time; f.kind = kind;
+  for (int i = 0; i < 4; ++i) f.par[i] = par ? par[i] : 0.0;
+  x.run_pts(f, ctx.d);
+  return x.check("trcadv_winds");
 }
 
 // ----------------------------------------------------------------------------
@@ -521,7 +541,7 @@ int dyn_step(X& x, const Context& ctx, double* prog, double* progq, int nq, doub
     int rc = x.check("dyn_step stage");
     if (rc) return rc;
   }
-  tracer_transport(x, ctx, progq, nq, dtl);
+  tracer_transport(x, ctx, progq, nq, dtl, ctx.field(W_PM), ctx.field(W_PROG0), ctx.field(W_FRHOG));
   halo_block(x, ctx, prog, 6);
   halo_block(x, ctx, progq, nq);
   return x.check("dyn_step");

@@ -207,3 +207,45 @@ class HipDynamics:
         P, Pp = self.ctx.to_host(self.prog)
         Q, Qp = self.ctx.to_host(self.progq)
         return {"PROG": P, "PROG_pl": Pp, "PROGq": Q, "PROGq_pl": Qp}
+
+
+WIND_KINDS = {"dcmip11": 0, "deform": 1}
+
+
+class HipOfflineTracer(HipDynamics):
+    """Offline tracer run: ``TIME_integ_type = 'TRCADV'`` of ``Dyn.dynamics_step``.
+
+    Each step transports the tracers with the mass fluxes held in PROG
+    (mod_dynamics.py:2030-2068) and then writes the prescribed winds of the
+    new time into PROG (``Forcing.forcing_update``, mod_forcing.py:113-206).
+    ``wind`` is ``"dcmip11"`` (Trcadv.test11_velocity, mod_af_trcadv.py:23-98)
+    or ``"deform"`` with ``params = (period, umax, w0, ztop)``: the
+    non-divergent deformational flow u = umax sin^2(lon) sin(2 lat) cos(pi t/T),
+    v = umax sin(2 lon) cos(lat) cos(pi t/T), w = w0 sin(pi z/ztop) cos(2 pi t/T).
+    The winds of ``time0`` are written at construction.
+    """
+
+    def __init__(self, ctx: HipContext, state: dict, wind: str = "dcmip11", params=None, time0: float = 0.0):
+        if wind not in WIND_KINDS:
+            raise ValueError(f"unknown wind {wind!r}; expected one of {sorted(WIND_KINDS)}")
+        p = [0.0] * 4 if params is None else [float(v) for v in params]
+        if len(p) != 4:
+            raise ValueError("params = (period, umax, w0, ztop)")
+        super().__init__(ctx, state)
+        self.kind = WIND_KINDS[wind]
+        self._params = (C.c_double * 4)(*p)
+        self.time = float(time0)
+        self.set_winds(self.time)
+
+    def set_winds(self, time: float):
+        lib = self.ctx.lib
+        native.check(lib, lib.nh_trcadv_winds(self.ctx.handle, C.c_void_p(self.prog.data_ptr()), float(time),
+                                              self.kind, self._params, self.ctx.stream()), "nh_trcadv_winds")
+
+    def dynamics_step(self):
+        lib = self.ctx.lib
+        native.check(lib, lib.nh_tracer_step(self.ctx.handle, C.c_void_p(self.prog.data_ptr()),
+                                             C.c_void_p(self.progq.data_ptr()), self.progq.shape[0], self.dtl,
+                                             self.ctx.stream()), "nh_tracer_step")
+        self.time += self.dtl
+        self.set_winds(self.time)

@@ -122,15 +122,23 @@ def geometry_arrays(geo) -> dict:
         # budgets (GMTR_p P_AREA per column, layer thickness)
         "area": _soa(geo.gmtr_p[..., 0]),
         "dgz": np.ascontiguousarray(geo.vgrid.dgz, dtype=np.float64),
+        # prescribed tracer winds (GRD_LAT/GRD_LON, level heights)
+        "lat": _soa(geo.lat),
+        "lon": _soa(geo.lon),
+        "lat_pl": np.ascontiguousarray(geo.lat_pl).reshape(-1),                 # (6, 2)
+        "lon_pl": np.ascontiguousarray(geo.lon_pl).reshape(-1),
+        "gz": np.ascontiguousarray(geo.vgrid.gz, dtype=np.float64),
+        "gzh": np.ascontiguousarray(geo.vgrid.gzh, dtype=np.float64),
     }
     sizes = {"coef_div_pl": 36, "coef_grad_pl": 36, "coef_lap_pl": 12, "coef_intp_pl": 108,
              "coef_diff_pl": 36, "grd_x_pl": 36, "tri_w_pl": 36, "arc_hn_pl": 36, "rarea_pl": 2,
-             "arc_xr_pl": 36}
+             "arc_xr_pl": 36, "lat_pl": 12, "lon_pl": 12}
     for k, n in sizes.items():
         if out[k].size != n:
             raise ValueError(f"{k} has {out[k].size} entries, expected {n}")
     for k in ("coef_div", "coef_grad", "coef_lap", "coef_intp", "coef_diff", "grd_x", "tri_w", "arc_hn",
-              "rarea", "arc_xr", "tri_w_pl", "arc_hn_pl", "rarea_pl", "arc_xr_pl", "area"):
+              "rarea", "arc_xr", "tri_w_pl", "arc_hn_pl", "rarea_pl", "arc_xr_pl", "area", "lat", "lon",
+              "lat_pl", "lon_pl"):
         out[k] = np.ascontiguousarray(out[k], dtype=np.float64)
     out.update({k: np.ascontiguousarray(v, dtype=np.float64) for k, v in vertical_profiles(geo).items()})
     return out

@@ -35,7 +35,7 @@ _VERT_FIELDS = ["afact", "bfact", "cfact", "dfact", "rdgz", "rdgzh", "c2w0", "c2
 
 
 _TRC_FIELDS = ["tri_w", "arc_hn", "rarea", "arc_xr", "tri_w_pl", "arc_hn_pl", "rarea_pl", "arc_xr_pl",
-               "area", "dgz"]
+               "area", "dgz", "lat", "lon", "lat_pl", "lon_pl", "gz", "gzh"]
 
 
 class NhGeometry(C.Structure):
@@ -58,6 +58,9 @@ EXPORTS = {
     "nh_gradient": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4 + [C.c_void_p]),
     "nh_dyn_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_int32,
                               C.c_void_p]),
+    "nh_tracer_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_void_p]),
+    "nh_trcadv_winds": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int32, C.POINTER(C.c_double),
+                                  C.c_void_p]),
     "nh_budget": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double,
                             C.POINTER(C.c_double), C.c_void_p]),
     "nh_prof_enable": (C.c_int, [C.c_void_p, C.c_int32]),

@@ -181,3 +181,52 @@ def test_budget_gl05(gpu, ctx5, case_gl05):
     got = hipdyn.HipDynamics(ctx5, s).budget()
     for key, v in want.items():
         assert abs(got[key] - v) <= 1e-13 * abs(v), (key, got[key], v)
+
+
+@pytest.mark.parametrize("wind,params", [("dcmip11", None), ("deform", (12 * 86400.0, 40.0, 0.1, 30000.0))])
+def test_offline_tracer_gl05(gpu, ctx5, case_gl05, wind, params):
+    """Three TRCADV steps (prescribed winds + MIURA2004 transport) at gl05 z40 against the oracle."""
+    from oracle import dynamics as od
+    from oracle import trcadv as ot
+    t = case_gl05.topo
+    print(f"  offline tracer gl05 wind={wind}", flush=True)
+    st = mod("initial").blob_tracers(case_gl05.geo, case_gl05.state, ntracer=3, seed=0)
+    ref = ot.OfflineTracer(od.OracleContext(case_gl05.geo, case_gl05.const, None), st, wind, params)
+    run = hipdyn.HipOfflineTracer(ctx5, st, wind, params)
+    for _ in range(3):
+        ref.step()
+        run.dynamics_step()
+    got = run.state()
+    for v in range(1, 5):
+        assert rel_err(ref.PROG[..., v], got["PROG"][..., v], ref.PROG_pl[..., v], got["PROG_pl"][..., v]) < 1e-13
+    for q in range(1, got["PROGq"].shape[-1]):
+        a, b = interior(ref.PROGq[..., q], t), interior(got["PROGq"][..., q], t)
+        ap, bp = pole_centre(ref.PROGq_pl[..., q], t), pole_centre(got["PROGq_pl"][..., q], t)
+        assert rel_err(a, b, ap, bp) < 1e-12, q
+
+
+def test_offline_tracer_gl06_z60_bounds_and_mass(gpu):
+    """The tracer benchmark grid (gl06, 60 levels to 12 km, 4 tracers of 6 blobs, dt = 900 s,
+    deformational flow): 12 steps keep every tracer's global mass to round-off and its mixing
+    ratio within the initial range [0, 1]."""
+    case = mod("setup").build_case(glevel=6, vlayer=60, ztop=12000.0, dz_bottom=200.0, dtl=900.0, seed=0)
+    geo, t = case.geo, case.topo
+    st = mod("initial").blob_tracers(geo, case.state, ntracer=4, seed=7, nblob=6, radius_m=1.0e6)
+    ctx = hipdyn.HipContext(geo, case.const, device="cuda:0")
+    try:
+        run = hipdyn.HipOfflineTracer(ctx, st, "deform", (12 * 86400.0, 40.0, 0.1, 12000.0))
+
+        def tmass(Q, Qp):
+            vol = interior(geo.VOLUME, t)
+            return ((interior(Q, t) * vol[..., None]).sum(axis=(0, 1, 2, 3))
+                    + (pole_centre(Qp, t) * pole_centre(geo.VOLUME_pl, t)[..., None]).sum(axis=(0, 1)))
+        m0 = tmass(st["PROGq"], st["PROGq_pl"])
+        for _ in range(12):
+            run.dynamics_step()
+        got = run.state()
+        m1 = tmass(got["PROGq"], got["PROGq_pl"])
+        assert np.all(np.abs(m1[1:] / m0[1:] - 1.0) < 1e-13), m1 / m0
+        q = interior(got["PROGq"][..., 1:], t) / interior(got["PROG"][..., 0], t)[..., None]
+        assert q.min() >= -1e-14 and q.max() <= 1.0 + 1e-13, (q.min(), q.max())
+    finally:
+        ctx.close()

@@ -165,3 +165,39 @@ def test_budget_rejects_missing_arguments(emu_ctx):
     rc = lib.nh_budget(emu_ctx.handle, C.c_void_p(prog.data_ptr()), C.c_void_p(None), 1, 2.5e6, out,
                        C.c_void_p(None))
     assert rc == -1 and b"nh_budget" in lib.nh_last_error()
+
+
+DEFORM = (12 * 86400.0, 40.0, 0.1, 10000.0)
+
+
+@pytest.mark.parametrize("wind,params", [("dcmip11", None), ("deform", DEFORM)])
+def test_host_build_offline_tracer_matches_oracle(case_gl04, state_gl04_tracers, emu_ctx, wind, params):
+    """Prescribed winds and three TRCADV steps of the kernel source, host build, against the oracle."""
+    from oracle import dynamics as od
+    from oracle import trcadv as ot
+    c = case_gl04
+    t = c.topo
+    ref = ot.OfflineTracer(od.OracleContext(c.geo, c.const, None), state_gl04_tracers, wind, params)
+    run = hipdyn.HipOfflineTracer(emu_ctx, state_gl04_tracers, wind, params)
+    got = run.state()
+    for v in range(1, 5):
+        assert rel_err(ref.PROG[..., v], got["PROG"][..., v], ref.PROG_pl[..., v], got["PROG_pl"][..., v]) < 1e-13
+    for _ in range(3):
+        ref.step()
+        run.dynamics_step()
+    got = run.state()
+    assert run.time == ref.time
+    for q in range(1, got["PROGq"].shape[-1]):
+        a, b = interior(ref.PROGq[..., q], t), interior(got["PROGq"][..., q], t)
+        ap, bp = pole_centre(ref.PROGq_pl[..., q], t), pole_centre(got["PROGq_pl"][..., q], t)
+        assert rel_err(a, b, ap, bp) < 1e-12, q
+
+
+def test_trcadv_winds_argument_errors(emu_ctx):
+    lib = emu_ctx.lib
+    prog = emu_ctx.empty(6)
+    bad = (C.c_double * 4)(0.0, 40.0, 0.1, 1e4)
+    assert lib.nh_trcadv_winds(emu_ctx.handle, C.c_void_p(prog.data_ptr()), 0.0, 1, bad, C.c_void_p(None)) == -1
+    assert lib.nh_trcadv_winds(emu_ctx.handle, C.c_void_p(prog.data_ptr()), 0.0, 7, None, C.c_void_p(None)) == -1
+    with pytest.raises(ValueError):
+        hipdyn.HipOfflineTracer(emu_ctx, {"PROG": np.zeros(1)}, "nope")

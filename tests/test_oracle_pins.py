@@ -311,3 +311,109 @@ def test_budget_closed_forms(case_gl04):
     want_w = 0.5 * rho0 * w0 ** 2 * A * (vg.gz[topo.kmax] - vg.gz[topo.kmin])
     assert abs(b["kin"] / want_w - 1.0) < 1e-12
     assert b["vap"] == 0.0
+
+
+def _east_north_components(vx, vy, vz, lon, lat):
+    """Project a Cartesian vector field on the local east and north directions (geometric construction)."""
+    r = np.stack([np.cos(lat) * np.cos(lon), np.cos(lat) * np.sin(lon), np.sin(lat)], axis=-1)
+    zhat = np.array([0.0, 0.0, 1.0])
+    e = np.cross(zhat, r)
+    e /= np.linalg.norm(e, axis=-1, keepdims=True)
+    n = np.cross(r, e)
+    v = np.stack([vx, vy, vz], axis=-1)
+    return (v * e).sum(-1), (v * n).sum(-1), (v * r).sum(-1)
+
+
+def _sphere_divergence(uv, lon, lat, a, h=1e-5):
+    """(1/(a cos lat)) (du/dlon + d(v cos lat)/dlat) by centred differences of uv(lon, lat) -> (u, v)."""
+    du = (uv(lon + h, lat)[0] - uv(lon - h, lat)[0]) / (2 * h)
+    dv = (uv(lon, lat + h)[1] * np.cos(lat + h) - uv(lon, lat - h)[1] * np.cos(lat - h)) / (2 * h)
+    return (du + dv) / (a * np.cos(lat))
+
+
+def test_dcmip11_winds_are_consistent_with_their_pressure_continuity():
+    """DCMIP 1-1 winds (Trcadv.test11_velocity): tangent to the sphere, w = 0 at the surface and at
+    12 km, and the 3-D flow is non-divergent in pressure coordinates,
+    div_p(u, v) + d(omega)/dp = 0 with omega = -w g p / (Rd T0) and p = p0 exp(-z g / (Rd T0))."""
+    from oracle import trcadv as ot
+    rng = np.random.default_rng(2)
+    lon = rng.uniform(0, 2 * np.pi, 200)
+    lat = rng.uniform(-1.3, 1.3, 200)
+    z = rng.uniform(500.0, 11500.0, 200)
+    a, Rd, g, T0, p0 = 6371220.0, 287.0, 9.80616, 300.0, 1.0e5
+    H = Rd * T0 / g
+    for time in (0.0, 3.3 * 86400.0, 7.0 * 86400.0):
+        vx, vy, vz, w = ot.test11_velocity(time, lon, lat, z, z)
+        u, v, rad = _east_north_components(vx, vy, vz, lon, lat)
+        assert np.abs(rad).max() < 1e-12 * np.abs(u).max()
+
+        def uv(lo, la):
+            X = ot.test11_velocity(time, lo, la, z, z)
+            return _east_north_components(*X[:3], lo, la)[:2]
+        div = _sphere_divergence(uv, lon, lat, a)
+        p = p0 * np.exp(-z / H)
+        dp = 1e-4 * p
+
+        def omega(pp):
+            zz = -H * np.log(pp / p0)
+            return -ot.test11_velocity(time, lon, lat, zz, zz)[3] * g * pp / (Rd * T0)
+        domega = (omega(p + dp) - omega(p - dp)) / (2 * dp)
+        scale = np.abs(domega).max()
+        assert np.abs(div + domega).max() < 1e-5 * scale, time
+        zb = np.array([0.0, 12000.0])
+        wb = ot.test11_velocity(time, lon[:2], lat[:2], zb, zb)[3]
+        assert np.abs(wb).max() < 1e-12 * np.abs(w).max()
+
+
+def test_deformational_winds_closed_forms():
+    """The benchmark flow: tangent, non-divergent, peak speed umax, reverses at T/2 (v(T - t) = -v(t)),
+    w = w0 sin(pi z/ztop) cos(2 pi t/T) on inner half levels and 0 elsewhere."""
+    from oracle import trcadv as ot
+    T, umax, w0, ztop = 12 * 86400.0, 40.0, 0.1, 12000.0
+    par = (T, umax, w0, ztop)
+    rng = np.random.default_rng(3)
+    lon = rng.uniform(0, 2 * np.pi, 300)
+    lat = rng.uniform(-1.4, 1.4, 300)
+    z = rng.uniform(0.0, ztop, 300)
+    inner = np.ones(300, bool)
+    t = 2.1 * 86400.0
+    vx, vy, vz, w = ot.deform_velocity(t, lon, lat, z, inner, par)
+    u, v, rad = _east_north_components(vx, vy, vz, lon, lat)
+    assert np.abs(rad).max() < 1e-12 * umax
+
+    def uv(lo, la):
+        return _east_north_components(*ot.deform_velocity(t, lo, la, z, inner, par)[:3], lo, la)[:2]
+    assert np.abs(_sphere_divergence(uv, lon, lat, 6.371e6)).max() < 1e-9 * umax / 6.371e6
+    # peak speed at t = 0 on a dense grid
+    LO, LA = np.meshgrid(np.linspace(0, 2 * np.pi, 721), np.linspace(-np.pi / 2, np.pi / 2, 361))
+    V = ot.deform_velocity(0.0, LO, LA, np.zeros_like(LO), np.zeros_like(LO, bool), par)
+    assert abs(np.sqrt(V[0] ** 2 + V[1] ** 2 + V[2] ** 2).max() / umax - 1.0) < 1e-6
+    Vr = ot.deform_velocity(T - t, lon, lat, z, inner, par)
+    assert np.allclose(Vr[0], -vx, atol=1e-12 * umax) and np.allclose(Vr[3], w, atol=1e-15)
+    assert np.allclose(w, w0 * np.sin(np.pi * z / ztop) * np.cos(2 * np.pi * t / T), rtol=0, atol=1e-16)
+    assert np.all(ot.deform_velocity(t, lon, lat, z, ~inner, par)[3] == 0.0)
+
+
+def test_offline_tracer_conserves_tracer_mass_and_bounds(case_gl04):
+    """TRCADV steps with the DCMIP 1-1 winds: flux-form transport keeps the global tracer mass
+    (sum of rhogq * VOLUME, interior and pole cells) to round-off and the limiters keep q in [0, 1]."""
+    from oracle import trcadv as ot
+    c = case_gl04
+    geo, t = c.geo, c.topo
+    st = mod("initial").blob_tracers(geo, c.state, ntracer=3, seed=0)
+    run = ot.OfflineTracer(_octx(geo, c.const), st, "dcmip11")
+
+    def tmass(Q, Qp, iq):
+        g = t.gall_1d
+        s, k = slice(1, g - 1), slice(t.kmin, t.kmax + 1)
+        return ((Q[s, s, k, :, iq] * geo.VOLUME[s, s, k]).sum()
+                + (Qp[0, k, :, iq] * geo.VOLUME_pl[0, k]).sum())
+    m0 = [tmass(run.PROGq, run.PROGq_pl, q) for q in (1, 2)]
+    for _ in range(3):
+        run.step()
+    for q, m in zip((1, 2), m0):
+        assert abs(tmass(run.PROGq, run.PROGq_pl, q) / m - 1.0) < 1e-13
+    g = t.gall_1d
+    s, k = slice(1, g - 1), slice(t.kmin, t.kmax + 1)
+    qq = run.PROGq[s, s, k, :, 2] / run.PROG[s, s, k, :, 0]
+    assert qq.min() >= -1e-14 and qq.max() <= 1.0 + 1e-13
