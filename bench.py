@@ -2,10 +2,12 @@
 
 Workload: glevel 7 (163,842 cells, 40 regions of 66 x 66 points), 40 layers
 up to 30 km (100 m at the surface), dt = 300 s, RK3 with 2/3/6 HEVI small
-steps, DIRECT hyperdiffusion and 3-D divergence damping; dry balanced zonal
-flow with seeded potential-temperature bumps (seed 1), flat topography; plain
-FP64 HIP form.  A step is one large step of the whole globe with the state
-resident in HBM.  SDPD = dt / wall time per step.
+steps, DIRECT hyperdiffusion and 3-D divergence damping, tracer transport of
+qv; dry balanced zonal flow with seeded potential-temperature bumps (seed 1),
+flat topography.  A step is one large step of the whole globe with the state
+resident in HBM.  SDPD = dt / wall time per step.  Both forms are timed on the
+same state: the mixed form (FP32 fields and stencils, FP64 tridiagonal solve,
+the value) and the plain FP64 form (the drop-in, the comparison).
 
 Prints one JSON line (rank 0).  The dominant kernel's achieved HBM bandwidth
 is its algorithmic bytes per launch divided by its average launch time,
@@ -37,16 +39,21 @@ def pkg(name):
     return importlib.import_module(f"pynicam-dc_amd.{name}")
 
 
-def kernel_bytes(t, nq: int = 1) -> dict:
-    """Algorithmic HBM bytes per launch of each kernel (FP64).
+def kernel_bytes(t, nq: int = 1, elem: int = 8) -> dict:
+    """Algorithmic HBM bytes per launch of each kernel.
 
-    Each distinct (i, j, k, l) field read or written once counts F bytes; each
-    per-column coefficient component counts C = F / kall; k-only profiles are
-    ignored.  Field counts follow the arrays each kernel touches.
+    Each distinct (i, j, k, l) field read or written once counts F bytes
+    (elem bytes per point: 8 in the FP64 form, 4 in the mixed form), each of
+    the FP64 fields of the tridiagonal solve (matrix, forward-sweep scratch)
+    D = 8 bytes per point; each per-column coefficient component counts
+    C = elem bytes per column; k-only profiles are ignored.  Field counts
+    follow the arrays each kernel touches.
     """
     ncell = t.lall * t.gall_1d ** 2
-    F = 8.0 * (ncell * t.kall + 12 * t.kall)
-    Cc = 8.0 * ncell
+    npts = ncell * t.kall + 12 * t.kall
+    F = float(elem) * npts
+    D = 8.0 * npts
+    Cc = float(elem) * ncell
     return {
         "prepost": 14 * F,                      # prog 6 + qv in; diag 6 + eth out
         "adv_mom": 13 * F + 24 * Cc,            # prog 5 + diag 4 in; 4 out; div + xyz coefs
@@ -54,12 +61,13 @@ def kernel_bytes(t, nq: int = 1) -> dict:
         "hdiff_pass2": 13 * F + 43 * Cc,        # 6 + rhog in, 6 out; lap + intp + diff coefs
         "divdamp_3d": 7 * F + 36 * Cc,          # 4 in, 3 out; intp + diff coefs
         "divdamp_post": 6 * F + 39 * Cc,        # 3 in, 3 out; intp + diff + xyz
-        "vi_path0": 34 * F + 45 * Cc,           # 23 in, 11 out; div + grad + xyz
+        "vi_path0": 31 * F + 3 * D + 45 * Cc,   # 23 in, 8 out + FP64 matrix 3 out; div + grad + xyz
         "ethtot_mean": 12 * F,                  # prog 5 + eth in; ethtot + 5 means out
         "split_init": 15 * F,                   # prog0 6 + prog 6 in (2 of 3 stages), 6 out
         "vi_path1": 20 * F + 24 * Cc,           # 14 in, 6 out; grad + xyz
         "vi_main_h": 12 * F + 21 * Cc,          # dv 3, prog 3, eth, ethtot, gT_e in; 3 out; div coefs
-        "vi_main_v": 13 * F,                    # split rho/w/e, gT rho/w, matrix 3, divergences 2 in; rhogw out
+        "vi_main_v": 8 * F + 5 * D,             # split rho/w/e, gT rho/w, divergences 2 in, rhogw out;
+                                                # FP64 matrix 3 in, sweep scratch u, gamma
         "vi_main_post": 26 * F,                 # state 5, split 6, dv 3, ethtot, gT, getot, div 2, rhogw, means 2 in; 4 out
         "prog_update": 18 * F + 3 * Cc,         # 12 in, 6 out
         "trc_vert1": (5 + 2 * nq) * F,          # w mean, rhog_in, frhog + tracers in; flux, rhog1 + tracers out
@@ -88,11 +96,11 @@ KERNEL_TYPES = {
 
 def _kernel_key(name: str):
     import re
-    m = re.search(r"k_(?:pts|run|tiles)<nh::(\w+(?:<\d+>)?)\s*>", name)
+    m = re.search(r"k_(?:pts|run|tiles)<nh::f(?:64|32)::(\w+(?:<\d+>)?)\s*>", name)
     return KERNEL_TYPES.get(m.group(1)) if m else None
 
 
-def hbm_traffic(glevel: int, timeout: float = 300.0):
+def hbm_traffic(glevel: int, precision: str, timeout: float = 300.0):
     """HBM bytes per dispatch of each kernel from two rocprofv3 counter passes.
 
     Runs this benchmark for one step under ``rocprofv3 --pmc FETCH_SIZE`` and
@@ -112,7 +120,8 @@ def hbm_traffic(glevel: int, timeout: float = 300.0):
             d = os.path.join(td, ctr)
             cmd = ["rocprofv3", "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "pmc", "--",
                    sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0",
-                   "--glevel", str(glevel), "--no-cpu-baseline", "--no-traffic", "--copy-probe"]
+                   "--glevel", str(glevel), "--precision", precision, "--no-cpu-baseline", "--no-traffic",
+                   "--copy-probe"]
             print(f"bench: counter pass {ctr}", file=sys.stderr, flush=True)
             try:
                 r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), capture_output=True,
@@ -167,12 +176,49 @@ def cpu_baseline(seconds_cap: float = 60.0) -> dict:
     }
 
 
+FORMS = {"float64": ("fp64", "f64", 8), "mixed": ("mixed", "f32+f64-solve", 4)}
+
+
+def run_form(hipdyn, case, precision: str, steps: int, warmup: int):
+    """Warm up, then time `steps` large steps of one form with the kernel timers on."""
+    import numpy as np
+    import torch
+    ctx = hipdyn.HipContext(case.geo, case.const, device="cuda:0", precision=precision)
+    dyn = hipdyn.HipDynamics(ctx, case.state)
+    for _ in range(warmup):
+        dyn.dynamics_step()
+    ctx.sync()
+    print(f"bench: {precision}: {warmup} warmup steps done", file=sys.stderr, flush=True)
+    ctx.prof_report()                      # clear
+    ctx.prof_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dyn.dynamics_step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.prof_enable(False)
+    prof = ctx.prof_report()
+    st = dyn.state()
+    budget = dyn.budget()
+    ctx.close()
+    g = case.topo.gall_1d
+    if not np.all(np.isfinite(st["PROG"][1:g - 1, 1:g - 1, 1:-1])):
+        print(f"bench: {precision}: state is not finite", file=sys.stderr)
+        sys.exit(1)
+    ms = (t1 - t0) * 1000.0 / steps
+    print(f"bench: {precision}: {ms:.2f} ms per step", file=sys.stderr, flush=True)
+    return ms, prof, st, budget
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--glevel", type=int, default=7)
+    ap.add_argument("--precision", choices=("both", "mixed", "float64"), default="both",
+                    help="forms to time; the value is the mixed form's SDPD when it is timed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure HBM traffic per kernel")
@@ -184,10 +230,12 @@ def main():
               "the multi-GPU path is a design, not a run", file=sys.stderr)
         sys.exit(2)
     gl = args.glevel
+    forms = ["float64", "mixed"] if args.precision == "both" else [args.precision]
+    main_form = forms[-1]
     traffic, factors = (None, "not measured")
     if not args.no_traffic:
         # before this process initialises the GPU
-        traffic, factors = hbm_traffic(gl)
+        traffic, factors = hbm_traffic(gl, main_form)
     import torch
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
@@ -196,31 +244,11 @@ def main():
     print(f"bench: setting up gl{gl:02d} z40 dt {dt:g} s", file=sys.stderr, flush=True)
     case = pkg("setup").build_case(glevel=gl, vlayer=40, dtl=dt, seed=1)
     hipdyn = pkg("hipdyn")
-    ctx = hipdyn.HipContext(case.geo, case.const, device="cuda:0")
-    dyn = hipdyn.HipDynamics(ctx, case.state)
-    for w in range(args.warmup):
-        dyn.dynamics_step()
-    ctx.sync()
-    print(f"bench: {args.warmup} warmup steps done", file=sys.stderr, flush=True)
-    ctx.prof_report()                      # clear
-    ctx.prof_enable(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        dyn.dynamics_step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ctx.prof_enable(False)
-    prof = ctx.prof_report()
-    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    res = {f: run_form(hipdyn, case, f, args.steps, args.warmup) for f in forms}
+    ms_per_step, prof, st, budget = res[main_form]
     sdpd = dt / (ms_per_step / 1000.0)
-    st = dyn.state()
-    import numpy as np
-    g = case.topo.gall_1d
-    if not np.all(np.isfinite(st["PROG"][1:g - 1, 1:g - 1, 1:-1])):
-        print("bench: state is not finite", file=sys.stderr)
-        sys.exit(1)
-    kb = kernel_bytes(case.topo)
+    tag, dtype, elem = FORMS[main_form]
+    kb = kernel_bytes(case.topo, elem=elem)
     dom = max((k for k in prof if k in kb), key=lambda k: prof[k][0])
     tot, n = prof[dom]
     avg_s = tot / n / 1000.0
@@ -235,7 +263,10 @@ def main():
                 v["traffic_GB"] = traffic[k] / 1e9
             if k in RESOURCES:
                 v.update(RESOURCES[k])
-    print("bench: kernels " + json.dumps(kernels), file=sys.stderr, flush=True)
+    print(f"bench: {main_form} kernels " + json.dumps(kernels), file=sys.stderr, flush=True)
+    form_info = {}
+    for f, (ms, _, _, b) in res.items():
+        form_info[FORMS[f][0]] = {"ms_per_step": ms, "sdpd": dt / (ms / 1000.0), "budget": b}
     out = {
         "metric": METRIC,
         "value": sdpd,
@@ -247,12 +278,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": dtype,
         "data": "synthetic",
-        "config": {"workload": f"gl{gl:02d} z40 dynamics step, dt {dt:g} s, plain FP64 HIP form, "
+        "config": {"workload": f"gl{gl:02d} z40 dynamics step, dt {dt:g} s, {tag} HIP form, "
                                "dry balanced zonal flow + seeded theta bumps (seed 1), flat topography",
-                   "glevel": gl, "vlayer": 40, "dt": dt, "cells": 10 * 4 ** gl + 2, "form": "fp64",
+                   "glevel": gl, "vlayer": 40, "dt": dt, "cells": 10 * 4 ** gl + 2, "form": tag,
                    "parallelism": "single GPU, 40 regions"},
+        "forms": form_info,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (traffic.get(dom) if isinstance(traffic, dict) else None),
@@ -261,6 +293,16 @@ def main():
                                       if isinstance(traffic, dict) else factors),
                      "algorithmic_bytes_per_launch": kb[dom], "avg_launch_us": avg_s * 1e6},
     }
+    if "fp64" in form_info and "mixed" in form_info:
+        out["speedup_mixed_vs_fp64"] = form_info["mixed"]["sdpd"] / form_info["fp64"]["sdpd"]
+        import numpy as np
+        t = case.topo
+        a = res["float64"][2]["PROG"][1:-1, 1:-1, t.kmin:t.kmax + 1]
+        b = res["mixed"][2]["PROG"][1:-1, 1:-1, t.kmin:t.kmax + 1].astype(np.float64)
+        scale = np.abs(a).reshape(-1, 6).max(axis=0)
+        err = np.abs(a - b).reshape(-1, 6).max(axis=0) / scale
+        out["mixed_vs_fp64_max_rel_err"] = dict(zip(("rhog", "rhogvx", "rhogvy", "rhogvz", "rhogw", "rhoge"),
+                                                    [float(e) for e in err]))
     if args.copy_probe:
         a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda:0")
         b = torch.empty_like(a)
@@ -271,7 +313,6 @@ def main():
     if not args.no_cpu_baseline:
         print("bench: timing the CPU oracle", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline()
-    ctx.close()
     print(json.dumps(out), flush=True)
 
 

@@ -3,13 +3,23 @@
  * dynamical-core time step.
  *
  * Plain C types only: dimensions and constants as POD structs, fields as
- * device pointers to FP64 arrays in the backend layout, streams as opaque
- * hipStream_t handles (NULL = default stream).  Every entry point returns 0 on
+ * device pointers to arrays in the backend layout, streams as opaque
+ * hipStream_t handles (NULL = default stream).
+ *
+ * Two forms, chosen when a context is created:
+ *   NH_PREC_FP64   fields are double; the drop-in for the reference's float64
+ *                  path (backend precision "float64");
+ *   NH_PREC_MIXED  fields are float: stencils, advection, hyperdiffusion and
+ *                  tracer transport run in FP32 with FP32 accumulation, while
+ *                  the tridiagonal matrix and rhogw solve, the global sums and
+ *                  the time steps stay FP64.  Entry points that take fields
+ *                  come in a double form and an _f32 form; calling the form
+ *                  that does not match the context returns NH_ERR_STATE.  Every entry point returns 0 on
  * success or a negative NH_ERR_* code; nh_last_error() then describes the
  * failure (thread-local, valid until the next call on that thread).  The
  * library never frees caller buffers.
  *
- * Field layout (one field = NF doubles):
+ * Field layout (one field = NF values):
  *   regional point (i, j, k, l):  ((l*kall + k)*gall_1d + j)*gall_1d + i
  *   pole point (v, k, lpl):       R + (lpl*kall + k)*6 + v,  R = lall*kall*gall_1d^2
  *   NF = R + 2*kall*6
@@ -102,16 +112,28 @@ typedef struct {
 
 typedef struct nh_context nh_context;
 
+enum { NH_PREC_FP64 = 0, NH_PREC_MIXED = 1 };
+
 /* Description of the last failure on the calling thread. */
 const char* nh_last_error(void);
 
-/* Number of doubles per field (NF) for the given dimensions. */
+/* Number of values per field (NF) for the given dimensions. */
 int64_t nh_field_size(const nh_dims* dims);
 
 /* Create a context on a device and allocate its step workspace.
  * Replaces the per-process setup chain Adm/Comm/Grd/Gmtr/Oprt/Vmtr
  * (mod_adm.py:78, mod_comm.py:119, mod_vmtr.py:30) for the step's buffers. */
 int nh_create(const nh_dims* dims, const nh_consts* consts, int device, nh_context** out);
+
+/* nh_create with the form of the fields: NH_PREC_FP64 (as nh_create) or
+ * NH_PREC_MIXED.  The precision switch of Backend.configure(name, precision)
+ * (share/mod_backend.py:175-283); the geometry is uploaded from the same
+ * double arrays and rounded to the field precision on the device, with the
+ * vertical profiles kept in FP64 as well. */
+int nh_create_ex(const nh_dims* dims, const nh_consts* consts, int device, int32_t precision, nh_context** out);
+
+/* The context's form (NH_PREC_FP64 or NH_PREC_MIXED), or NH_ERR_ARG. */
+int32_t nh_precision(const nh_context* ctx);
 
 /* Release a context and everything it allocated. */
 int nh_destroy(nh_context* ctx);
@@ -128,21 +150,26 @@ int nh_upload_halo_map(nh_context* ctx, const int32_t* dst, const int32_t* src, 
 /* In-place halo and pole fill of nfields fields (each NF doubles).
  * Replaces Comm.COMM_data_transfer (mod_comm.py:1066, 1563-1644). */
 int nh_halo_fill(nh_context* ctx, double* const* fields, int32_t nfields, void* stream);
+int nh_halo_fill_f32(nh_context* ctx, float* const* fields, int32_t nfields, void* stream);
 
 /* 7-point Laplacian on interior points and pole centres; other points of out
  * are left unchanged.  Replaces OPRT_laplacian (mod_oprt.py:1429,
  * kernels/oprtlaplacian.py:41-85). */
 int nh_laplacian(nh_context* ctx, const double* scl, double* out, void* stream);
+int nh_laplacian_f32(nh_context* ctx, const float* scl, float* out, void* stream);
 
 /* 7-point divergence of (vx, vy, vz) on interior points and pole centres.
  * Replaces the horizontal part of src_flux_convergence (mod_src.py:631,
  * kernels/fluxconv.py:238 _divergence). */
 int nh_divergence(nh_context* ctx, const double* vx, const double* vy, const double* vz, double* out,
                   void* stream);
+int nh_divergence_f32(nh_context* ctx, const float* vx, const float* vy, const float* vz, float* out,
+                      void* stream);
 
 /* 7-point gradient of scl on interior points and pole centres (3 output
  * fields).  Replaces OPRT_gradient (mod_oprt.py:1387, kernels/oprtgradient.py:45-88). */
 int nh_gradient(nh_context* ctx, const double* scl, double* gx, double* gy, double* gz, void* stream);
+int nh_gradient_f32(nh_context* ctx, const float* scl, float* gx, float* gy, float* gz, void* stream);
 
 /* One dynamical-core large step, in place: prog holds the 6 prognostic
  * fields (rhog, rhogvx, rhogvy, rhogvz, rhogw, rhoge), progq the nq >= 1
@@ -154,6 +181,8 @@ int nh_gradient(nh_context* ctx, const double* scl, double* gx, double* gy, doub
  * Srctr.src_tracer_advection (mod_src_tracer.py:83-1228). */
 int nh_dyn_step(nh_context* ctx, double* prog, double* progq, int32_t nq, double dtl, int32_t sstep_max,
                 void* stream);
+int nh_dyn_step_f32(nh_context* ctx, float* prog, float* progq, int32_t nq, double dtl, int32_t sstep_max,
+                    void* stream);
 
 /* Offline tracer step (TIME_integ_type 'TRCADV', Dyn.dynamics_step,
  * mod_dynamics.py:2030-2076): MIURA2004 transport of progq
@@ -161,6 +190,7 @@ int nh_dyn_step(nh_context* ctx, double* prog, double* progq, int32_t nq, double
  * prog (rhog, rhogvx..vz, rhogw; rhog is both the initial and the mean
  * density), with zero density tendency, then the halo fill of progq. */
 int nh_tracer_step(nh_context* ctx, const double* prog, double* progq, int32_t nq, double dtl, void* stream);
+int nh_tracer_step_f32(nh_context* ctx, const float* prog, float* progq, int32_t nq, double dtl, void* stream);
 
 /* Prescribed winds of the offline tracer runs (Forcing.forcing_update,
  * mod_forcing.py:113-206): rhogv = v(t) rhog, rhogw = w(t) rhog at every
@@ -173,6 +203,8 @@ int nh_tracer_step(nh_context* ctx, const double* prog, double* progq, int32_t n
 enum { NH_WIND_DCMIP11 = 0, NH_WIND_DEFORM = 1 };
 int nh_trcadv_winds(nh_context* ctx, double* prog, double time, int32_t kind, const double* params,
                     void* stream);
+int nh_trcadv_winds_f32(nh_context* ctx, float* prog, double time, int32_t kind, const double* params,
+                        void* stream);
 
 /* Global mass and energy sums (Embudget._store_var, mod_embudget.py:82-139):
  * sum of var * VOLUME over the regional interior points of the physical
@@ -182,6 +214,8 @@ int nh_trcadv_winds(nh_context* ctx, double* prog, double time, int32_t kind, co
  * mass and water-vapour mass [kg].  Synchronises the stream. */
 int nh_budget(nh_context* ctx, const double* prog, const double* progq, int32_t nq, double lhv, double* out,
               void* stream);
+int nh_budget_f32(nh_context* ctx, const float* prog, const float* progq, int32_t nq, double lhv, double* out,
+                  void* stream);
 
 /* Kernel timers (the per-kernel counterpart of the section timers
  * PROF_rapstart / PROF_rapend / PROF_rapreport, share/mod_prof.py:81-140).

@@ -1,19 +1,17 @@
-// extern "C" entry points of include/nicamhip.h, generic over the executor
-// type NHX defined by the including translation unit:
-//   NHX(int device, void* stream, Context* ctx)  sets the device, binds the
-//   stream and the context's kernel timers;
-//   prof_enable(bool), prof_report(...), prof_release() for the timers.
+// extern "C" entry points of include/nicamhip.h: argument checks, then
+// dispatch to the context's precision variant (nh_variant.h).  Included by
+// exactly one translation unit of a library build.
 #pragma once
 
-#include <cstdio>
-#include <cstdlib>
 #include <new>
 #include <string>
 
-#include "nh_engine.h"
+#include "nh_variant.h"
 
 struct nh_context {
-  nh::Context c;
+  int precision;            // NH_PREC_FP64 or NH_PREC_MIXED
+  nh::f64::Context* c64;
+  nh::f32::Context* c32;
 };
 
 namespace nh {
@@ -25,11 +23,112 @@ inline int fail(int code, const char* where, const std::string& msg) {
   return code;
 }
 
-inline int check_ready(const nh_context* ctx, const char* where, bool need_halo) {
+// the field precision an entry point was called for: 0 double, 1 float
+inline int check_ctx(const nh_context* ctx, const char* where, int want_f32) {
   if (!ctx) return fail(NH_ERR_ARG, where, "null context");
-  if (!ctx->c.geo_ready) return fail(NH_ERR_STATE, where, "geometry not uploaded");
-  if (need_halo && ctx->c.halo_n == 0) return fail(NH_ERR_STATE, where, "halo map not uploaded");
+  const int is_f32 = (ctx->precision == NH_PREC_MIXED);
+  if (want_f32 >= 0 && want_f32 != is_f32)
+    return fail(NH_ERR_STATE, where,
+                is_f32 ? "the context holds float fields: call the _f32 entry point"
+                       : "the context holds double fields: call the entry point without _f32");
   return 0;
+}
+
+inline bool geo_ready(const nh_context* ctx) {
+  return ctx->c64 ? f64::v_geo_ready(ctx->c64) : f32::v_geo_ready(ctx->c32);
+}
+
+inline long halo_n(const nh_context* ctx) { return ctx->c64 ? f64::v_halo_n(ctx->c64) : f32::v_halo_n(ctx->c32); }
+
+inline int check_ready(const nh_context* ctx, const char* where, int want_f32, bool need_halo) {
+  if (int rc = check_ctx(ctx, where, want_f32)) return rc;
+  if (!geo_ready(ctx)) return fail(NH_ERR_STATE, where, "geometry not uploaded");
+  if (need_halo && halo_n(ctx) == 0) return fail(NH_ERR_STATE, where, "halo map not uploaded");
+  return 0;
+}
+
+inline int done(int rc, const char* where, const std::string& err) { return rc < 0 ? fail(rc, where, err) : 0; }
+
+// call the variant of ctx with the same arguments
+#define NH_DISPATCH(ctx, fn, ...) \
+  ((ctx)->c64 ? nh::f64::fn((ctx)->c64, __VA_ARGS__) : nh::f32::fn((ctx)->c32, __VA_ARGS__))
+
+inline int halo_fill(nh_context* ctx, void* const* fields, int32_t nfields, void* stream, int f32) {
+  const char* where = f32 ? "nh_halo_fill_f32" : "nh_halo_fill";
+  if (int rc = check_ctx(ctx, where, f32)) return rc;
+  if (nfields < 0 || (nfields > 0 && !fields)) return fail(NH_ERR_ARG, where, "invalid argument");
+  if (halo_n(ctx) == 0) return fail(NH_ERR_STATE, where, "halo map not uploaded");
+  for (int q = 0; q < nfields; ++q)
+    if (!fields[q]) return fail(NH_ERR_ARG, where, "null field pointer");
+  std::string err;
+  return done(NH_DISPATCH(ctx, v_halo_fill, fields, nfields, stream, &err), where, err);
+}
+
+inline int laplacian(nh_context* ctx, const void* scl, void* out, void* stream, int f32) {
+  const char* where = f32 ? "nh_laplacian_f32" : "nh_laplacian";
+  if (int rc = check_ready(ctx, where, f32, false)) return rc;
+  if (!scl || !out) return fail(NH_ERR_ARG, where, "null field pointer");
+  std::string err;
+  return done(NH_DISPATCH(ctx, v_laplacian, scl, out, stream, &err), where, err);
+}
+
+inline int divergence(nh_context* ctx, const void* vx, const void* vy, const void* vz, void* out, void* stream,
+                      int f32) {
+  const char* where = f32 ? "nh_divergence_f32" : "nh_divergence";
+  if (int rc = check_ready(ctx, where, f32, false)) return rc;
+  if (!vx || !vy || !vz || !out) return fail(NH_ERR_ARG, where, "null field pointer");
+  std::string err;
+  return done(NH_DISPATCH(ctx, v_divergence, vx, vy, vz, out, stream, &err), where, err);
+}
+
+inline int gradient(nh_context* ctx, const void* scl, void* gx, void* gy, void* gz, void* stream, int f32) {
+  const char* where = f32 ? "nh_gradient_f32" : "nh_gradient";
+  if (int rc = check_ready(ctx, where, f32, false)) return rc;
+  if (!scl || !gx || !gy || !gz) return fail(NH_ERR_ARG, where, "null field pointer");
+  std::string err;
+  return done(NH_DISPATCH(ctx, v_gradient, scl, gx, gy, gz, stream, &err), where, err);
+}
+
+inline int dyn_step(nh_context* ctx, void* prog, void* progq, int32_t nq, double dtl, int32_t sstep_max,
+                    void* stream, int f32) {
+  const char* where = f32 ? "nh_dyn_step_f32" : "nh_dyn_step";
+  if (int rc = check_ready(ctx, where, f32, true)) return rc;
+  if (!prog || !progq || nq < 1) return fail(NH_ERR_ARG, where, "need prog and at least one tracer field");
+  if (sstep_max < 3 || sstep_max % 6 != 0 || !(dtl > 0.0))
+    return fail(NH_ERR_ARG, where, "sstep_max must be a positive multiple of 6 and dtl > 0");
+  std::string err;
+  return done(NH_DISPATCH(ctx, v_dyn_step, prog, progq, nq, dtl, sstep_max, stream, &err), where, err);
+}
+
+inline int tracer_step(nh_context* ctx, const void* prog, void* progq, int32_t nq, double dtl, void* stream,
+                       int f32) {
+  const char* where = f32 ? "nh_tracer_step_f32" : "nh_tracer_step";
+  if (int rc = check_ready(ctx, where, f32, true)) return rc;
+  if (!prog || !progq || nq < 1) return fail(NH_ERR_ARG, where, "need prog and at least one tracer field");
+  if (!(dtl > 0.0)) return fail(NH_ERR_ARG, where, "dtl must be > 0");
+  std::string err;
+  return done(NH_DISPATCH(ctx, v_tracer_step, prog, progq, nq, dtl, stream, &err), where, err);
+}
+
+inline int trcadv_winds(nh_context* ctx, void* prog, double time, int32_t kind, const double* params, void* stream,
+                        int f32) {
+  const char* where = f32 ? "nh_trcadv_winds_f32" : "nh_trcadv_winds";
+  if (int rc = check_ready(ctx, where, f32, false)) return rc;
+  if (!prog) return fail(NH_ERR_ARG, where, "null prog");
+  if (kind != NH_WIND_DCMIP11 && kind != NH_WIND_DEFORM) return fail(NH_ERR_ARG, where, "unknown wind kind");
+  if (kind == NH_WIND_DEFORM && (!params || !(params[0] > 0.0) || !(params[3] > 0.0)))
+    return fail(NH_ERR_ARG, where, "deformational flow needs period > 0 and ztop > 0");
+  std::string err;
+  return done(NH_DISPATCH(ctx, v_trcadv_winds, prog, time, kind, params, stream, &err), where, err);
+}
+
+inline int budget(nh_context* ctx, const void* prog, const void* progq, int32_t nq, double lhv, double* out,
+                  void* stream, int f32) {
+  const char* where = f32 ? "nh_budget_f32" : "nh_budget";
+  if (int rc = check_ready(ctx, where, f32, false)) return rc;
+  if (!prog || !progq || nq < 1 || !out) return fail(NH_ERR_ARG, where, "need prog, qv and an output array");
+  std::string err;
+  return done(NH_DISPATCH(ctx, v_budget, prog, progq, lhv, out, stream, &err), where, err);
 }
 
 }  // namespace nh
@@ -40,51 +139,45 @@ const char* nh_last_error(void) { return nh::g_err.c_str(); }
 
 int64_t nh_field_size(const nh_dims* dims) {
   if (!dims) return nh::fail(NH_ERR_ARG, "nh_field_size", "null dims");
-  nh::Dims d;
-  nh::set_dims(d, dims);
-  return d.NF;
+  const long nreg = (long)dims->lall * dims->gall_1d * dims->gall_1d;
+  return nreg * dims->kall + 2L * dims->kall * 6;
 }
 
-int nh_create(const nh_dims* dims, const nh_consts* consts, int device, nh_context** out) {
+int nh_create_ex(const nh_dims* dims, const nh_consts* consts, int device, int32_t precision, nh_context** out) {
   const char* where = "nh_create";
   if (!dims || !consts || !out) return nh::fail(NH_ERR_ARG, where, "null argument");
   *out = nullptr;
+  if (precision != NH_PREC_FP64 && precision != NH_PREC_MIXED)
+    return nh::fail(NH_ERR_ARG, where, "precision must be NH_PREC_FP64 or NH_PREC_MIXED");
   if (dims->gall_1d < 4 || dims->lall < 1 || dims->kmin != 1 || dims->kmax != dims->kall - 2 ||
       dims->kall > NH_KMAX || dims->kmax - dims->kmin < 2)
     return nh::fail(NH_ERR_ARG, where, "unsupported dimensions (need gall_1d>=4, kmin=1, kmax=kall-2, kall<=80)");
-  nh_context* ctx = new (std::nothrow) nh_context();
+  nh_context* ctx = new (std::nothrow) nh_context{precision, nullptr, nullptr};
   if (!ctx) return nh::fail(NH_ERR_ALLOC, where, "host allocation");
-  nh::set_dims(ctx->c.d, dims);
-  nh::set_consts(ctx->c.c, consts);
-  ctx->c.device = device;
-  NHX x(device, nullptr, nullptr);
-  if (x.check(where)) { delete ctx; return nh::fail(NH_ERR_DEVICE, where, x.message()); }
-  size_t bytes = (size_t)nh::W_NFIELDS * ctx->c.d.NF * sizeof(double);
-  ctx->c.ws = (double*)x.alloc(bytes);
-  if (!ctx->c.ws) {
+  std::string err;
+  const int rc = (precision == NH_PREC_FP64) ? nh::f64::v_create(dims, consts, device, &ctx->c64, &err)
+                                             : nh::f32::v_create(dims, consts, device, &ctx->c32, &err);
+  if (rc) {
     delete ctx;
-    return nh::fail(NH_ERR_ALLOC, where, "workspace allocation of " + std::to_string(bytes) + " bytes");
-  }
-  x.zero(ctx->c.ws, bytes);
-  if (x.check(where)) {
-    x.release(ctx->c.ws);
-    delete ctx;
-    return nh::fail(NH_ERR_DEVICE, where, x.message());
+    return nh::fail(rc, where, err);
   }
   *out = ctx;
   return 0;
 }
 
+int nh_create(const nh_dims* dims, const nh_consts* consts, int device, nh_context** out) {
+  return nh_create_ex(dims, consts, device, NH_PREC_FP64, out);
+}
+
+int32_t nh_precision(const nh_context* ctx) {
+  if (!ctx) return nh::fail(NH_ERR_ARG, "nh_precision", "null context");
+  return ctx->precision;
+}
+
 int nh_destroy(nh_context* ctx) {
   if (!ctx) return 0;
-  NHX x(ctx->c.device, nullptr, &ctx->c);
-  x.sync();
-  if (ctx->c.ws) x.release(ctx->c.ws);
-  if (ctx->c.geo_buf) x.release(ctx->c.geo_buf);
-  if (ctx->c.int_buf) x.release(ctx->c.int_buf);
-  if (ctx->c.halo_dst) x.release(ctx->c.halo_dst);
-  if (ctx->c.halo_src) x.release(ctx->c.halo_src);
-  x.prof_release();
+  if (ctx->c64) nh::f64::v_destroy(ctx->c64);
+  if (ctx->c32) nh::f32::v_destroy(ctx->c32);
   delete ctx;
   return 0;
 }
@@ -92,130 +185,85 @@ int nh_destroy(nh_context* ctx) {
 int nh_upload_geometry(nh_context* ctx, const nh_geometry* geo) {
   const char* where = "nh_upload_geometry";
   if (!ctx || !geo) return nh::fail(NH_ERR_ARG, where, "null argument");
-  NHX x(ctx->c.device, nullptr, &ctx->c);
-  int rc = nh::upload_geometry(x, ctx->c, geo);
-  if (rc == NH_ERR_ARG) return nh::fail(rc, where, "null coefficient pointer");
-  if (rc == NH_ERR_ALLOC) return nh::fail(rc, where, "device allocation");
-  if (rc) return nh::fail(rc, where, x.message());
-  return 0;
+  std::string err;
+  return nh::done(NH_DISPATCH(ctx, v_upload_geometry, geo, &err), where, err);
 }
 
 int nh_upload_halo_map(nh_context* ctx, const int32_t* dst, const int32_t* src, int64_t n) {
   const char* where = "nh_upload_halo_map";
   if (!ctx || n < 0 || (n > 0 && (!dst || !src))) return nh::fail(NH_ERR_ARG, where, "invalid argument");
-  NHX x(ctx->c.device, nullptr, &ctx->c);
-  int rc = nh::upload_halo(x, ctx->c, dst, src, (long)n);
-  if (rc == NH_ERR_ARG) return nh::fail(rc, where, "column id out of range");
-  if (rc == NH_ERR_ALLOC) return nh::fail(rc, where, "device allocation");
-  if (rc) return nh::fail(rc, where, x.message());
-  return 0;
+  std::string err;
+  return nh::done(NH_DISPATCH(ctx, v_upload_halo, dst, src, (long)n, &err), where, err);
 }
 
 int nh_halo_fill(nh_context* ctx, double* const* fields, int32_t nfields, void* stream) {
-  const char* where = "nh_halo_fill";
-  if (!ctx || nfields < 0 || (nfields > 0 && !fields)) return nh::fail(NH_ERR_ARG, where, "invalid argument");
-  if (ctx->c.halo_n == 0) return nh::fail(NH_ERR_STATE, where, "halo map not uploaded");
-  for (int q = 0; q < nfields; ++q)
-    if (!fields[q]) return nh::fail(NH_ERR_ARG, where, "null field pointer");
-  NHX x(ctx->c.device, stream, &ctx->c);
-  for (int q0 = 0; q0 < nfields; q0 += 16) {
-    int m = (nfields - q0 < 16) ? nfields - q0 : 16;
-    nh::halo(x, ctx->c, fields + q0, m);
-  }
-  if (x.check(where)) return nh::fail(NH_ERR_DEVICE, where, x.message());
-  return 0;
+  return nh::halo_fill(ctx, (void* const*)fields, nfields, stream, 0);
+}
+int nh_halo_fill_f32(nh_context* ctx, float* const* fields, int32_t nfields, void* stream) {
+  return nh::halo_fill(ctx, (void* const*)fields, nfields, stream, 1);
 }
 
 int nh_laplacian(nh_context* ctx, const double* scl, double* out, void* stream) {
-  const char* where = "nh_laplacian";
-  if (int rc = nh::check_ready(ctx, where, false)) return rc;
-  if (!scl || !out) return nh::fail(NH_ERR_ARG, where, "null field pointer");
-  NHX x(ctx->c.device, stream, &ctx->c);
-  nh::OpLapP f{ctx->c.d, ctx->c.G, scl, out};
-  x.run_pts(f, ctx->c.d);
-  if (x.check(where)) return nh::fail(NH_ERR_DEVICE, where, x.message());
-  return 0;
+  return nh::laplacian(ctx, scl, out, stream, 0);
+}
+int nh_laplacian_f32(nh_context* ctx, const float* scl, float* out, void* stream) {
+  return nh::laplacian(ctx, scl, out, stream, 1);
 }
 
 int nh_divergence(nh_context* ctx, const double* vx, const double* vy, const double* vz, double* out,
                   void* stream) {
-  const char* where = "nh_divergence";
-  if (int rc = nh::check_ready(ctx, where, false)) return rc;
-  if (!vx || !vy || !vz || !out) return nh::fail(NH_ERR_ARG, where, "null field pointer");
-  NHX x(ctx->c.device, stream, &ctx->c);
-  nh::OpDivP f{ctx->c.d, ctx->c.G, vx, vy, vz, out};
-  x.run_pts(f, ctx->c.d);
-  if (x.check(where)) return nh::fail(NH_ERR_DEVICE, where, x.message());
-  return 0;
+  return nh::divergence(ctx, vx, vy, vz, out, stream, 0);
+}
+int nh_divergence_f32(nh_context* ctx, const float* vx, const float* vy, const float* vz, float* out,
+                      void* stream) {
+  return nh::divergence(ctx, vx, vy, vz, out, stream, 1);
 }
 
 int nh_gradient(nh_context* ctx, const double* scl, double* gx, double* gy, double* gz, void* stream) {
-  const char* where = "nh_gradient";
-  if (int rc = nh::check_ready(ctx, where, false)) return rc;
-  if (!scl || !gx || !gy || !gz) return nh::fail(NH_ERR_ARG, where, "null field pointer");
-  NHX x(ctx->c.device, stream, &ctx->c);
-  nh::OpGradP f{ctx->c.d, ctx->c.G, scl, gx, gy, gz};
-  x.run_pts(f, ctx->c.d);
-  if (x.check(where)) return nh::fail(NH_ERR_DEVICE, where, x.message());
-  return 0;
+  return nh::gradient(ctx, scl, gx, gy, gz, stream, 0);
+}
+int nh_gradient_f32(nh_context* ctx, const float* scl, float* gx, float* gy, float* gz, void* stream) {
+  return nh::gradient(ctx, scl, gx, gy, gz, stream, 1);
 }
 
 int nh_dyn_step(nh_context* ctx, double* prog, double* progq, int32_t nq, double dtl, int32_t sstep_max,
                 void* stream) {
-  const char* where = "nh_dyn_step";
-  if (int rc = nh::check_ready(ctx, where, true)) return rc;
-  if (!prog || !progq || nq < 1) return nh::fail(NH_ERR_ARG, where, "need prog and at least one tracer field");
-  if (sstep_max < 3 || sstep_max % 6 != 0 || !(dtl > 0.0))
-    return nh::fail(NH_ERR_ARG, where, "sstep_max must be a positive multiple of 6 and dtl > 0");
-  NHX x(ctx->c.device, stream, &ctx->c);
-  int rc = nh::dyn_step(x, ctx->c, prog, progq, nq, dtl, sstep_max);
-  if (rc == NH_ERR_ARG) return nh::fail(rc, where, "invalid argument");
-  if (rc) return nh::fail(NH_ERR_DEVICE, where, x.message());
-  return 0;
+  return nh::dyn_step(ctx, prog, progq, nq, dtl, sstep_max, stream, 0);
+}
+int nh_dyn_step_f32(nh_context* ctx, float* prog, float* progq, int32_t nq, double dtl, int32_t sstep_max,
+                    void* stream) {
+  return nh::dyn_step(ctx, prog, progq, nq, dtl, sstep_max, stream, 1);
 }
 
 int nh_tracer_step(nh_context* ctx, const double* prog, double* progq, int32_t nq, double dtl, void* stream) {
-  const char* where = "nh_tracer_step";
-  if (int rc = nh::check_ready(ctx, where, true)) return rc;
-  if (!prog || !progq || nq < 1) return nh::fail(NH_ERR_ARG, where, "need prog and at least one tracer field");
-  if (!(dtl > 0.0)) return nh::fail(NH_ERR_ARG, where, "dtl must be > 0");
-  NHX x(ctx->c.device, stream, &ctx->c);
-  int rc = nh::tracer_step(x, ctx->c, prog, progq, nq, dtl);
-  if (rc) return nh::fail(NH_ERR_DEVICE, where, x.message());
-  return 0;
+  return nh::tracer_step(ctx, prog, progq, nq, dtl, stream, 0);
+}
+int nh_tracer_step_f32(nh_context* ctx, const float* prog, float* progq, int32_t nq, double dtl, void* stream) {
+  return nh::tracer_step(ctx, prog, progq, nq, dtl, stream, 1);
 }
 
 int nh_trcadv_winds(nh_context* ctx, double* prog, double time, int32_t kind, const double* params,
                     void* stream) {
-  const char* where = "nh_trcadv_winds";
-  if (int rc = nh::check_ready(ctx, where, false)) return rc;
-  if (!prog) return nh::fail(NH_ERR_ARG, where, "null prog");
-  if (kind != NH_WIND_DCMIP11 && kind != NH_WIND_DEFORM) return nh::fail(NH_ERR_ARG, where, "unknown wind kind");
-  if (kind == NH_WIND_DEFORM && (!params || !(params[0] > 0.0) || !(params[3] > 0.0)))
-    return nh::fail(NH_ERR_ARG, where, "deformational flow needs period > 0 and ztop > 0");
-  NHX x(ctx->c.device, stream, &ctx->c);
-  int rc = nh::trcadv_winds(x, ctx->c, prog, time, kind, params);
-  if (rc) return nh::fail(NH_ERR_DEVICE, where, x.message());
-  return 0;
+  return nh::trcadv_winds(ctx, prog, time, kind, params, stream, 0);
+}
+int nh_trcadv_winds_f32(nh_context* ctx, float* prog, double time, int32_t kind, const double* params,
+                        void* stream) {
+  return nh::trcadv_winds(ctx, prog, time, kind, params, stream, 1);
 }
 
 int nh_budget(nh_context* ctx, const double* prog, const double* progq, int32_t nq, double lhv, double* out,
               void* stream) {
-  const char* where = "nh_budget";
-  if (int rc = nh::check_ready(ctx, where, false)) return rc;
-  if (!prog || !progq || nq < 1 || !out) return nh::fail(NH_ERR_ARG, where, "need prog, qv and an output array");
-  NHX x(ctx->c.device, stream, &ctx->c);
-  int rc = nh::budget(x, ctx->c, prog, progq, lhv, out);
-  if (rc == NH_ERR_ARG) return nh::fail(rc, where, "grid too small for the partial-sum buffer");
-  if (rc) return nh::fail(NH_ERR_DEVICE, where, x.message());
-  return 0;
+  return nh::budget(ctx, prog, progq, nq, lhv, out, stream, 0);
+}
+int nh_budget_f32(nh_context* ctx, const float* prog, const float* progq, int32_t nq, double lhv, double* out,
+                  void* stream) {
+  return nh::budget(ctx, prog, progq, nq, lhv, out, stream, 1);
 }
 
 int nh_prof_enable(nh_context* ctx, int32_t on) {
   if (!ctx) return nh::fail(NH_ERR_ARG, "nh_prof_enable", "null context");
-  NHX x(ctx->c.device, nullptr, &ctx->c);
-  if (x.prof_enable(on != 0)) return nh::fail(NH_ERR_DEVICE, "nh_prof_enable", x.message());
-  return 0;
+  std::string err;
+  return nh::done(NH_DISPATCH(ctx, v_prof_enable, on != 0, &err), "nh_prof_enable", err);
 }
 
 int nh_prof_report(nh_context* ctx, int32_t max_kernels, char* names, double* total_ms, int64_t* launches,
@@ -223,9 +271,9 @@ int nh_prof_report(nh_context* ctx, int32_t max_kernels, char* names, double* to
   const char* where = "nh_prof_report";
   if (!ctx || max_kernels < 0 || !n_kernels || (max_kernels > 0 && (!names || !total_ms || !launches)))
     return nh::fail(NH_ERR_ARG, where, "invalid argument");
-  NHX x(ctx->c.device, nullptr, &ctx->c);
-  int n = x.prof_report(max_kernels, names, total_ms, launches);
-  if (n < 0) return nh::fail(NH_ERR_DEVICE, where, x.message());
+  std::string err;
+  const int n = NH_DISPATCH(ctx, v_prof_report, max_kernels, names, total_ms, launches, &err);
+  if (n < 0) return nh::fail(n, where, err);
   *n_kernels = n;
   return 0;
 }

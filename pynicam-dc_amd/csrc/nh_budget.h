@@ -6,17 +6,18 @@
 // mod_embudget.py:74-80).  Phase 1 writes the six volume-weighted terms of
 // the point into LDS; phase 2 lets six threads sum one term each over the
 // block with Neumaier's compensated summation and store (sum, correction) as
-// the block's partial.  The host adds the partials, again compensated.
+// the block's partial.  The host adds the partials, again compensated.  All
+// arithmetic is FP64 in every form (field values are widened on load).
 #pragma once
 
 #include "nh_step.h"
 
-namespace nh {
+NH_BEGIN
 
 // Neumaier's improved Kahan summation: s + c carries the running sum
 NH_HD inline void neu_add(double& s, double& c, double v) {
   const double t = s + v;
-  if (fabs(s) >= fabs(v)) c += (s - t) + v;
+  if (nabs(s) >= nabs(v)) c += (s - t) + v;
   else c += (v - t) + s;
   s = t;
 }
@@ -24,10 +25,11 @@ NH_HD inline void neu_add(double& s, double& c, double v) {
 enum BudgetTerm { B_POT = 0, B_INT, B_KIN, B_DRY, B_VAP, B_NTERMS };
 
 struct BudgetT {
+  using LdsT = double;
   static constexpr const char* kName = "budget";
   Dims d; Consts c; Geo G;
-  const double* prog;
-  const double* rhogq;   // qv
+  const real* prog;
+  const real* rhogq;     // qv
   double lhv;
   long npts;             // interior points of the physical levels
   double* part;          // [nblocks][B_NTERMS][2]
@@ -47,20 +49,22 @@ struct BudgetT {
       const long a = (((long)l * d.kall + k) * d.g + j) * d.g + i;
       const long col = ((long)l * d.g + j) * d.g + i;
       const long NF = d.NF;
+      const ConstsT<double>& C = c.d;
       const double rhog = prog[a];
       const double rho = rhog / 1.0;                  // GSGAM2 = 1
-      const double ein = prog[5 * NF + a] / rhog;
-      const double q = rhogq[a] / rhog;
+      const double ein = (double)prog[5 * NF + a] / rhog;
+      const double q = (double)rhogq[a] / rhog;
       const double qd = 1.0 - q;
-      const double cv = q * c.CVW_qv + qd * c.CVdry;
+      const double cv = q * C.CVW_qv + qd * C.CVdry;
       const double tem = ein / cv;
-      const double vol = G.area[col] * 1.0 * G.dgz[k];  // area * GSGAM2 * dgz
-      double e = qd * c.CVdry * tem;
-      e = e + q * c.CVW_qv * tem;
+      const double vol = G.area[col] * 1.0 * G.v64.dgz[k];  // area * GSGAM2 * dgz
+      double e = qd * C.CVdry * tem;
+      e = e + q * C.CVW_qv * tem;
       e = e + q * lhv;
-      const double kin = rhogkin_k(G, d.kmin, d.kmax, k,
-                                   [&](int kk, int qq) { return prog[qq * NF + a + (long)(kk - k) * ks]; });
-      v[B_POT] = rho * G.phi[k] * vol;
+      const double kin = rhogkin_k(G.v64, d.kmin, d.kmax, k, [&](int kk, int qq) {
+        return (double)prog[qq * NF + a + (long)(kk - k) * ks];
+      });
+      v[B_POT] = rho * G.v64.phi[k] * vol;
       v[B_INT] = rho * e * vol;
       v[B_KIN] = kin * 1.0 * vol;                     // RGSGAM2 = 1
       v[B_DRY] = rho * qd * vol;
@@ -77,4 +81,4 @@ struct BudgetT {
   }
 };
 
-}  // namespace nh
+NH_END

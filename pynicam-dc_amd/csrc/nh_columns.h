@@ -13,6 +13,12 @@
 //
 // The functors are plain structs of pointers and scalars, compiled for the GPU
 // (hipcc) and, in the test build, for the host.
+//
+// Precision: `real` is the type of the fields and of the stencil arithmetic.
+// The sources are compiled once per variant, each in its own namespace
+// (nh::f64 with real = double, the drop-in form; nh::f32 with real = float,
+// the mixed form).  Values the mixed form keeps in FP64 (the tridiagonal
+// matrix and solve, the global sums, time steps) are written as `double`.
 #pragma once
 
 #include <cmath>
@@ -22,9 +28,25 @@
 #define NH_HD __host__ __device__
 #endif
 
+#ifndef NH_REAL
+#define NH_REAL double
+#define NH_VARIANT f64
+#endif
+
 #define NH_KMAX 80   // largest supported kall
 
-namespace nh {
+#define NH_BEGIN namespace nh { namespace NH_VARIANT {
+#define NH_END } }
+
+NH_BEGIN
+
+using real = NH_REAL;
+
+// type-exact absolute value and sign transfer (no promotion of float arguments)
+NH_HD inline float nabs(float x) { return fabsf(x); }
+NH_HD inline double nabs(double x) { return fabs(x); }
+NH_HD inline float nsign(float m, float s) { return copysignf(m, s); }
+NH_HD inline double nsign(double m, double s) { return copysign(m, s); }
 
 struct Dims {
   int g;       // gall_1d
@@ -35,49 +57,61 @@ struct Dims {
   long NF;     // R + 2*kall*6 points per field
 };
 
-struct Consts {
-  double RADIUS, OHM, GRAV, Rdry, CVdry, Rvap, CVW_qv, ALPHA;
-  double GAMMA_H, HDIFF_FACT_RHO, ALPHA_D;
+template <class T>
+struct ConstsT {
+  T RADIUS, OHM, GRAV, Rdry, CVdry, Rvap, CVW_qv, ALPHA;
+  T GAMMA_H, HDIFF_FACT_RHO, ALPHA_D;
+};
+
+// physical constants in the field precision, and in FP64 (d) for the parts
+// the mixed form keeps in double
+struct Consts : ConstsT<real> {
+  ConstsT<double> d;
   int THUBURN_LIM;
 };
 
-struct Geo {
+// vertical profiles [kall]
+template <class T>
+struct VProf {
+  const T *afact, *bfact, *cfact, *dfact, *rdgz, *rdgzh;
+  const T *c2w0, *c2w1, *w2c0, *w2c1, *phi;
+  const T *dgz, *gz, *gzh;
+};
+
+struct Geo : VProf<real> {
   // regional, structure of arrays [comp][nreg]
-  const double* cdiv;    // 21 = d*7 + m
-  const double* cgrad;   // 21
-  const double* clap;    // 7
-  const double* cintp;   // 18 = (d*3 + vertex)*2 + T
-  const double* cdiff;   // 18 = d*6 + arc
-  const double* gx;      // 3  GRD_x [m]
+  const real* cdiv;    // 21 = d*7 + m
+  const real* cgrad;   // 21
+  const real* clap;    // 7
+  const real* cintp;   // 18 = (d*3 + vertex)*2 + T
+  const real* cdiff;   // 18 = d*6 + arc
+  const real* gx;      // 3  GRD_x [m]
   // pole, [v][lp][...]
-  const double* cdiv_pl;   // (v*2+lp)*3+d
-  const double* cgrad_pl;  // (v*2+lp)*3+d
-  const double* clap_pl;   // v*2+lp
-  const double* cintp_pl;  // ((v*2+lp)*3+d)*3+vertex
-  const double* cdiff_pl;  // (v*2+lp)*3+d
-  const double* gx_pl;     // (v*2+lp)*3+d
-  const int* sgp;          // [lall] 1 on pentagon regions
-  // vertical profiles [kall]
-  const double *afact, *bfact, *cfact, *dfact, *rdgz, *rdgzh;
-  const double *c2w0, *c2w1, *w2c0, *w2c1, *phi;
+  const real* cdiv_pl;   // (v*2+lp)*3+d
+  const real* cgrad_pl;  // (v*2+lp)*3+d
+  const real* clap_pl;   // v*2+lp
+  const real* cintp_pl;  // ((v*2+lp)*3+d)*3+vertex
+  const real* cdiff_pl;  // (v*2+lp)*3+d
+  const real* gx_pl;     // (v*2+lp)*3+d
+  const int* sgp;        // [lall] 1 on pentagon regions
   // tracer transport: regional [comp][nreg], pole [v][lp][...]
-  const double* twt;      // 6 = T*3 + w
-  const double* ahn;      // 9 = arc*3 + d
-  const double* rarea;    // 1
-  const double* axr;      // 9 = arc*3 + d
-  const double* twt_pl;   // (v*2+lp)*3+w
-  const double* ahn_pl;   // (v*2+lp)*3+d
-  const double* rarea_pl; // lp
-  const double* axr_pl;   // (v*2+lp)*3+d
-  // budgets: control-volume area per column, layer thickness per level
+  const real* twt;      // 6 = T*3 + w
+  const real* ahn;      // 9 = arc*3 + d
+  const real* rarea;    // 1
+  const real* axr;      // 9 = arc*3 + d
+  const real* twt_pl;   // (v*2+lp)*3+w
+  const real* ahn_pl;   // (v*2+lp)*3+d
+  const real* rarea_pl; // lp
+  const real* axr_pl;   // (v*2+lp)*3+d
+  // FP64: budgets (control-volume area per column), prescribed winds
+  // (latitude, longitude per column and at the poles)
   const double* area;     // 1
-  const double* dgz;      // [kall]
-  // prescribed winds: latitude, longitude per column, heights per level
   const double* lat;      // 1
   const double* lon;      // 1
   const double* lat_pl;   // v*2+lp
   const double* lon_pl;   // v*2+lp
-  const double *gz, *gzh; // [kall]
+  // the vertical profiles in FP64
+  VProf<double> v64;
 };
 
 enum { I_RHOG = 0, I_RHOGVX, I_RHOGVY, I_RHOGVZ, I_RHOGW, I_RHOGE };
@@ -179,12 +213,13 @@ NH_HD inline void stencil_offsets(int g, long off[7]) {
   off[4] = -1; off[5] = -1 - g; off[6] = -g;
 }
 
-NH_HD inline double rc(const double* a, long nreg, int comp, long ci) { return a[comp * nreg + ci]; }
+template <class T>
+NH_HD inline T rc(const T* a, long nreg, int comp, long ci) { return a[comp * nreg + ci]; }
 
 // 7-point divergence of (ax, ay, az)(point) at one level for a regional cell
 template <class FX, class FY, class FZ>
-NH_HD inline double div7(const Geo& G, long nreg, long ci, FX fx, FY fy, FZ fz) {
-  double sx = 0.0, sy = 0.0, sz = 0.0;
+NH_HD inline real div7(const Geo& G, long nreg, long ci, FX fx, FY fy, FZ fz) {
+  real sx, sy, sz;
   // same association as the reference: sum over m for each direction, then x + y + z
   sx = rc(G.cdiv, nreg, 0, ci) * fx(0);
   for (int m = 1; m < 7; ++m) sx = sx + rc(G.cdiv, nreg, m, ci) * fx(m);
@@ -197,11 +232,11 @@ NH_HD inline double div7(const Geo& G, long nreg, long ci, FX fx, FY fy, FZ fz) 
 
 // pole-centre divergence: sum over v = 0..5 of (cx*ax + cy*ay + cz*az)
 template <class FX, class FY, class FZ>
-NH_HD inline double div_pl(const Geo& G, int lp, FX fx, FY fy, FZ fz) {
-  double s = 0.0;
+NH_HD inline real div_pl(const Geo& G, int lp, FX fx, FY fy, FZ fz) {
+  real s = real(0.0);
   for (int v = 0; v < 6; ++v) {
-    const double* c = G.cdiv_pl + (v * 2 + lp) * 3;
-    double pv = c[0] * fx(v) + c[1] * fy(v) + c[2] * fz(v);
+    const real* c = G.cdiv_pl + (v * 2 + lp) * 3;
+    real pv = c[0] * fx(v) + c[1] * fy(v) + c[2] * fz(v);
     s = (v == 0) ? pv : s + pv;
   }
   return s;
@@ -216,7 +251,7 @@ struct HaloGather {
   const int* dst;   // column ids
   const int* src;
   long n;           // number of halo columns
-  double* f[16];
+  real* f[16];
   int nf;
   NH_HD inline long colbase(long c, long* ks) const {
     if (c < d.nreg) {
@@ -242,4 +277,4 @@ struct HaloGather {
   }
 };
 
-}  // namespace nh
+NH_END

@@ -28,9 +28,9 @@
 #include "nh_trcadv.h"
 #include "../../include/nicamhip.h"
 
-namespace nh {
+NH_BEGIN
 
-// fields of the step workspace, each NF doubles
+// fields of the step workspace, each NF values of the field precision
 enum WsField {
   W_PROG0 = 0,            // 6  state at the start of the large step
   W_DIAG = 6,             // 6  pre, tem, vx, vy, vz, w
@@ -42,41 +42,47 @@ enum WsField {
   W_DD = 32,              // 3  divergence damping of the large-step state
   W_DDV = 35,             // 3  divergence damping of the split state
   W_GT = 38,              // 6  large-step tendencies of the small steps
-  W_MC = 44, W_MU = 45, W_ML = 46,
-  W_GETOT = 47,           // 1
-  W_FRHOG = 48,           // 1
-  W_ETHTOT = 49,          // 1
-  W_PM = 50,              // 5  time-mean mass fluxes
-  W_SPLIT = 55,           // 12 split state (6) + path1 momentum (3) + vi_main out (3)
+  W_GETOT = 44,           // 1
+  W_FRHOG = 45,           // 1
+  W_ETHTOT = 46,          // 1
+  W_PM = 47,              // 5  time-mean mass fluxes
+  W_SPLIT = 52,           // 12 split state (6) + path1 momentum (3) + vi_main out (3)
   // tracer transport
-  W_FLXV = 67,            // 1  vertical mass flux of the half steps
-  W_RHOG1 = 68,           // 1  density after the first vertical half step
-  W_RHOG2 = 69,           // 1  density after the horizontal step
-  W_FA = 70,              // 3  arc mass fluxes (AI, AIJ, AJ)
-  W_XC = 73,              // 9  arc mass centroids
-  W_GQ = 82,              // 3  gradient of the mixing ratio
-  W_QA = 85,              // 3  upwind face values (faces 0..2)
-  W_QOUT = 88,            // 2  limiter outflow bounds
-  W_RQ = 90,              // 1  tracer after the horizontal step
-  W_HV = 91,              // 3  horizontal divergences of the small-step momentum
-  W_NFIELDS = 94
+  W_FLXV = 64,            // 1  vertical mass flux of the half steps
+  W_RHOG1 = 65,           // 1  density after the first vertical half step
+  W_RHOG2 = 66,           // 1  density after the horizontal step
+  W_FA = 67,              // 3  arc mass fluxes (AI, AIJ, AJ)
+  W_XC = 70,              // 9  arc mass centroids
+  W_GQ = 79,              // 3  gradient of the mixing ratio
+  W_QA = 82,              // 3  upwind face values (faces 0..2)
+  W_QOUT = 85,            // 2  limiter outflow bounds
+  W_RQ = 87,              // 1  tracer after the horizontal step
+  W_HV = 88,              // 3  horizontal divergences of the small-step momentum
+  W_NFIELDS = 91
 };
+
+// FP64 fields (every form): the tridiagonal matrix, the forward-sweep scratch
+// of the solve; the budgets' block partials reuse them between steps
+enum DwsField { D_MC = 0, D_MU = 1, D_ML = 2, D_U = 3, D_GM = 4, D_NFIELDS = 5 };
 
 struct Context {
   Dims d;
   Consts c;
   Geo G;
   // owned device buffers
-  double* geo_buf = nullptr;
+  real* geo_buf = nullptr;      // coefficients and profiles in the field precision
+  double* geo64_buf = nullptr;  // FP64 geometry (profiles, areas, latitude/longitude)
   int* int_buf = nullptr;
   int* halo_dst = nullptr;
   int* halo_src = nullptr;
   long halo_n = 0;
-  double* ws = nullptr;
+  real* ws = nullptr;
+  double* dws = nullptr;
   bool geo_ready = false;
   int device = 0;
   void* prof = nullptr;   // executor-specific kernel timer state
-  double* field(int f) const { return ws + (long)f * d.NF; }
+  real* field(int f) const { return ws + (long)f * d.NF; }
+  double* dfield(int f) const { return dws + (long)f * d.NF; }
 };
 
 inline void set_dims(Dims& d, const nh_dims* in) {
@@ -90,81 +96,128 @@ inline void set_dims(Dims& d, const nh_dims* in) {
   d.NF = d.R + 2L * d.kall * 6;
 }
 
+template <class T>
+inline void set_consts_t(ConstsT<T>& c, const nh_consts* in) {
+  c.RADIUS = (T)in->radius;
+  c.OHM = (T)in->ohm;
+  c.GRAV = (T)in->grav;
+  c.Rdry = (T)in->rdry;
+  c.CVdry = (T)in->cvdry;
+  c.Rvap = (T)in->rvap;
+  c.CVW_qv = (T)in->cvw_qv;
+  c.ALPHA = (T)in->alpha;
+  c.GAMMA_H = (T)in->gamma_h;
+  c.HDIFF_FACT_RHO = (T)in->hdiff_fact_rho;
+  c.ALPHA_D = (T)in->alpha_d;
+}
+
 inline void set_consts(Consts& c, const nh_consts* in) {
-  c.RADIUS = in->radius;
-  c.OHM = in->ohm;
-  c.GRAV = in->grav;
-  c.Rdry = in->rdry;
-  c.CVdry = in->cvdry;
-  c.Rvap = in->rvap;
-  c.CVW_qv = in->cvw_qv;
-  c.ALPHA = in->alpha;
-  c.GAMMA_H = in->gamma_h;
-  c.HDIFF_FACT_RHO = in->hdiff_fact_rho;
-  c.ALPHA_D = in->alpha_d;
+  set_consts_t<real>(c, in);
+  set_consts_t<double>(c.d, in);
   c.THUBURN_LIM = in->thuburn_lim;
 }
 
-// geometry arrays in upload order: regional [comp][nreg], pole [v][lp][...],
-// vertical profiles [kall]
-constexpr int kNReg = 13, kNPl = 12, kNVert = 14;
+// geometry arrays in upload order: regional [comp][nreg] and pole [v][lp][...]
+// arrays in the field precision, vertical profiles [kall] in both precisions,
+// then the FP64-only arrays
+constexpr int kNReg = 10, kNPl = 10, kNVert = 14, kNReg64 = 3, kNPl64 = 2;
+constexpr long kRegComp[kNReg] = {21, 21, 7, 18, 18, 3, 6, 9, 1, 9};
+constexpr long kPlSize[kNPl] = {36, 36, 12, 108, 36, 36, 36, 36, 2, 36};
 
 inline long geo_total(const Dims& d) {
-  static const long rc[kNReg] = {21, 21, 7, 18, 18, 3, 6, 9, 1, 9, 1, 1, 1};
-  static const long pc[kNPl] = {36, 36, 12, 108, 36, 36, 36, 36, 2, 36, 12, 12};
   long t = (long)kNVert * d.kall;
-  for (int i = 0; i < kNReg; ++i) t += rc[i] * d.nreg;
-  for (int i = 0; i < kNPl; ++i) t += pc[i];
+  for (int i = 0; i < kNReg; ++i) t += kRegComp[i] * d.nreg;
+  for (int i = 0; i < kNPl; ++i) t += kPlSize[i];
   return t;
+}
+
+inline long geo64_total(const Dims& d) { return (long)kNVert * d.kall + kNReg64 * d.nreg + kNPl64 * 12L; }
+
+// copy n doubles to the device in the field precision
+template <class X>
+void h2d_real(X& x, real* dst, const double* src, long n) {
+  if (sizeof(real) == sizeof(double)) {
+    x.h2d(dst, src, n * sizeof(double));
+    return;
+  }
+  std::vector<real> tmp((size_t)n);
+  for (long i = 0; i < n; ++i) tmp[(size_t)i] = (real)src[i];
+  x.h2d(dst, tmp.data(), n * sizeof(real));
 }
 
 template <class X>
 int upload_geometry(X& x, Context& ctx, const nh_geometry* g) {
   const Dims& d = ctx.d;
-  static const long rc[kNReg] = {21, 21, 7, 18, 18, 3, 6, 9, 1, 9, 1, 1, 1};
-  static const long pc[kNPl] = {36, 36, 12, 108, 36, 36, 36, 36, 2, 36, 12, 12};
   if (!ctx.geo_buf) {
-    ctx.geo_buf = (double*)x.alloc(geo_total(d) * sizeof(double));
+    ctx.geo_buf = (real*)x.alloc(geo_total(d) * sizeof(real));
+    ctx.geo64_buf = (double*)x.alloc(geo64_total(d) * sizeof(double));
     ctx.int_buf = (int*)x.alloc(d.lall * sizeof(int));
-    if (!ctx.geo_buf || !ctx.int_buf) return NH_ERR_ALLOC;
+    if (!ctx.geo_buf || !ctx.geo64_buf || !ctx.int_buf) return NH_ERR_ALLOC;
   }
   const double* regsrc[kNReg] = {g->coef_div, g->coef_grad, g->coef_lap, g->coef_intp, g->coef_diff,
-                                 g->grd_x,    g->tri_w,     g->arc_hn,   g->rarea,      g->arc_xr,
-                                 g->area,     g->lat,       g->lon};
+                                 g->grd_x,    g->tri_w,     g->arc_hn,   g->rarea,      g->arc_xr};
   const double* plsrc[kNPl] = {g->coef_div_pl, g->coef_grad_pl, g->coef_lap_pl, g->coef_intp_pl,
                                g->coef_diff_pl, g->grd_x_pl,    g->tri_w_pl,    g->arc_hn_pl,
-                               g->rarea_pl,     g->arc_xr_pl,   g->lat_pl,      g->lon_pl};
+                               g->rarea_pl,     g->arc_xr_pl};
   const double* vsrc[kNVert] = {g->afact, g->bfact, g->cfact, g->dfact, g->rdgz, g->rdgzh, g->c2w0,
                                 g->c2w1,  g->w2c0,  g->w2c1,  g->phi,   g->dgz,  g->gz,   g->gzh};
+  const double* reg64[kNReg64] = {g->area, g->lat, g->lon};
+  const double* pl64[kNPl64] = {g->lat_pl, g->lon_pl};
   for (int i = 0; i < kNReg; ++i)
     if (!regsrc[i]) return NH_ERR_ARG;
   for (int i = 0; i < kNPl; ++i)
     if (!plsrc[i]) return NH_ERR_ARG;
   for (int i = 0; i < kNVert; ++i)
     if (!vsrc[i]) return NH_ERR_ARG;
+  for (int i = 0; i < kNReg64; ++i)
+    if (!reg64[i]) return NH_ERR_ARG;
+  for (int i = 0; i < kNPl64; ++i)
+    if (!pl64[i]) return NH_ERR_ARG;
   if (!g->sgp) return NH_ERR_ARG;
-  double* p = ctx.geo_buf;
   Geo& G = ctx.G;
-  const double** regdst[kNReg] = {&G.cdiv, &G.cgrad, &G.clap, &G.cintp, &G.cdiff, &G.gx,  &G.twt,
-                                  &G.ahn,  &G.rarea, &G.axr,  &G.area,  &G.lat,   &G.lon};
-  const double** pldst[kNPl] = {&G.cdiv_pl, &G.cgrad_pl, &G.clap_pl,  &G.cintp_pl, &G.cdiff_pl, &G.gx_pl,
-                                &G.twt_pl,  &G.ahn_pl,   &G.rarea_pl, &G.axr_pl,   &G.lat_pl,   &G.lon_pl};
-  const double** vdst[kNVert] = {&G.afact, &G.bfact, &G.cfact, &G.dfact, &G.rdgz, &G.rdgzh, &G.c2w0,
-                                 &G.c2w1,  &G.w2c0,  &G.w2c1,  &G.phi,   &G.dgz,  &G.gz,    &G.gzh};
+  const real** regdst[kNReg] = {&G.cdiv, &G.cgrad, &G.clap, &G.cintp, &G.cdiff,
+                                &G.gx,   &G.twt,   &G.ahn,  &G.rarea, &G.axr};
+  const real** pldst[kNPl] = {&G.cdiv_pl, &G.cgrad_pl, &G.clap_pl, &G.cintp_pl, &G.cdiff_pl,
+                              &G.gx_pl,   &G.twt_pl,   &G.ahn_pl,  &G.rarea_pl, &G.axr_pl};
+  VProf<real>& VR = G;
+  const real** vdst[kNVert] = {&VR.afact, &VR.bfact, &VR.cfact, &VR.dfact, &VR.rdgz, &VR.rdgzh, &VR.c2w0,
+                               &VR.c2w1,  &VR.w2c0,  &VR.w2c1,  &VR.phi,   &VR.dgz,  &VR.gz,    &VR.gzh};
+  VProf<double>& VD = G.v64;
+  const double** v64dst[kNVert] = {&VD.afact, &VD.bfact, &VD.cfact, &VD.dfact, &VD.rdgz, &VD.rdgzh, &VD.c2w0,
+                                   &VD.c2w1,  &VD.w2c0,  &VD.w2c1,  &VD.phi,   &VD.dgz,  &VD.gz,    &VD.gzh};
+  const double** reg64dst[kNReg64] = {&G.area, &G.lat, &G.lon};
+  const double** pl64dst[kNPl64] = {&G.lat_pl, &G.lon_pl};
+  real* p = ctx.geo_buf;
   for (int i = 0; i < kNReg; ++i) {
-    x.h2d(p, regsrc[i], rc[i] * d.nreg * sizeof(double));
+    h2d_real(x, p, regsrc[i], kRegComp[i] * d.nreg);
     *regdst[i] = p;
-    p += rc[i] * d.nreg;
+    p += kRegComp[i] * d.nreg;
   }
   for (int i = 0; i < kNPl; ++i) {
-    x.h2d(p, plsrc[i], pc[i] * sizeof(double));
+    h2d_real(x, p, plsrc[i], kPlSize[i]);
     *pldst[i] = p;
-    p += pc[i];
+    p += kPlSize[i];
   }
   for (int i = 0; i < kNVert; ++i) {
-    x.h2d(p, vsrc[i], d.kall * sizeof(double));
+    h2d_real(x, p, vsrc[i], d.kall);
     *vdst[i] = p;
     p += d.kall;
+  }
+  double* q = ctx.geo64_buf;
+  for (int i = 0; i < kNVert; ++i) {
+    x.h2d(q, vsrc[i], d.kall * sizeof(double));
+    *v64dst[i] = q;
+    q += d.kall;
+  }
+  for (int i = 0; i < kNReg64; ++i) {
+    x.h2d(q, reg64[i], d.nreg * sizeof(double));
+    *reg64dst[i] = q;
+    q += d.nreg;
+  }
+  for (int i = 0; i < kNPl64; ++i) {
+    x.h2d(q, pl64[i], 12 * sizeof(double));
+    *pl64dst[i] = q;
+    q += 12;
   }
   x.h2d(ctx.int_buf, g->sgp, d.lall * sizeof(int));
   G.sgp = ctx.int_buf;
@@ -192,7 +245,7 @@ int upload_halo(X& x, Context& ctx, const int32_t* dst, const int32_t* src, long
 
 // in-place halo fill of up to 16 fields
 template <class X>
-void halo(X& x, const Context& ctx, double* const* f, int nf) {
+void halo(X& x, const Context& ctx, real* const* f, int nf) {
   if (ctx.halo_n == 0) return;
   for (int q0 = 0; q0 < nf; q0 += 16) {
     HaloGather h;
@@ -207,8 +260,8 @@ void halo(X& x, const Context& ctx, double* const* f, int nf) {
 }
 
 template <class X>
-void halo_block(X& x, const Context& ctx, double* base, int nf) {
-  double* f[16];
+void halo_block(X& x, const Context& ctx, real* base, int nf) {
+  real* f[16];
   for (int q0 = 0; q0 < nf; q0 += 16) {
     int m = (nf - q0 < 16) ? nf - q0 : 16;
     for (int q = 0; q < m; ++q) f[q] = base + (long)(q0 + q) * ctx.d.NF;
@@ -222,15 +275,15 @@ inline long ncols(const Dims& d) { return d.nreg + 12; }
 // 3-D divergence damping of (vx, vy, vz, w) into out (3 fields)
 // ----------------------------------------------------------------------------
 template <class X>
-void divdamp(X& x, const Context& ctx, const double* vx, const double* vy, const double* vz, const double* w,
-             double* out0, double* out1, double* out2) {
+void divdamp(X& x, const Context& ctx, const real* vx, const real* vy, const real* vz, const real* w,
+             real* out0, real* out1, real* out2) {
   const TileGeom tg = tile_geom(ctx.d, 2);
   DivDamp1T a;
   a.d = ctx.d; a.c = ctx.c; a.G = ctx.G; a.tg = tg;
   a.rvx = vx; a.rvy = vy; a.rvz = vz; a.rw = w;
   for (int q = 0; q < 3; ++q) a.out[q] = ctx.field(W_DDT + q);
   x.run_tiles(a, tg.nblocks, tg.lds);
-  double* tmp[3] = {ctx.field(W_DDT), ctx.field(W_DDT + 1), ctx.field(W_DDT + 2)};
+  real* tmp[3] = {ctx.field(W_DDT), ctx.field(W_DDT + 1), ctx.field(W_DDT + 2)};
   halo(x, ctx, tmp, 3);
   DivDamp2T b;
   b.d = ctx.d; b.c = ctx.c; b.G = ctx.G; b.tg = tg;
@@ -246,17 +299,17 @@ void divdamp(X& x, const Context& ctx, const double* vx, const double* vy, const
 // pm: mass fluxes (rhog, rhogvx, rhogvy, rhogvz, rhogw) with field stride NF;
 // rhog_in: density at the start of the step; frhog: density tendency
 template <class X>
-void tracer_transport(X& x, const Context& ctx, double* progq, int nq, double dt, const double* pm,
-                      const double* rhog_in, const double* frhog) {
+void tracer_transport(X& x, const Context& ctx, real* progq, int nq, double dt, const real* pm,
+                      const real* rhog_in, const real* frhog) {
   const Dims& d = ctx.d;
   const long NF = d.NF, nc = ncols(d);
-  double* rhog1 = ctx.field(W_RHOG1);
-  double* rhog2 = ctx.field(W_RHOG2);
-  double* flxv = ctx.field(W_FLXV);
+  real* rhog1 = ctx.field(W_RHOG1);
+  real* rhog2 = ctx.field(W_RHOG2);
+  real* flxv = ctx.field(W_FLXV);
   {
     TrcVert1 f;
     f.d = d; f.c = ctx.c; f.G = ctx.G;
-    f.dt = dt;
+    f.dt = (real)dt;
     f.rw_mean = pm + 4 * NF;
     f.rhog_in = rhog_in;
     f.frhog = frhog;
@@ -265,13 +318,13 @@ void tracer_transport(X& x, const Context& ctx, double* progq, int nq, double dt
     f.limiter = ctx.c.THUBURN_LIM;
     x.run(f, nc);
   }
-  double* fa[3] = {ctx.field(W_FA), ctx.field(W_FA + 1), ctx.field(W_FA + 2)};
-  double* xc[9];
+  real* fa[3] = {ctx.field(W_FA), ctx.field(W_FA + 1), ctx.field(W_FA + 2)};
+  real* xc[9];
   for (int q = 0; q < 9; ++q) xc[q] = ctx.field(W_XC + q);
   {
     TrcHFlux f;
     f.d = d; f.c = ctx.c; f.G = ctx.G;
-    f.dt = dt; f.pm = pm;
+    f.dt = (real)dt; f.pm = pm;
     for (int q = 0; q < 3; ++q) f.fa[q] = fa[q];
     for (int q = 0; q < 9; ++q) f.xc[q] = xc[q];
     x.run(f, nc);
@@ -279,17 +332,17 @@ void tracer_transport(X& x, const Context& ctx, double* progq, int nq, double dt
   {
     TrcRhog2 f;
     f.d = d; f.c = ctx.c; f.G = ctx.G;
-    f.dt = dt;
+    f.dt = (real)dt;
     for (int q = 0; q < 3; ++q) f.fa[q] = fa[q];
     f.rhog1 = rhog1; f.frhog = frhog; f.rhog2 = rhog2;
     x.run(f, nc);
   }
-  double* gq[3] = {ctx.field(W_GQ), ctx.field(W_GQ + 1), ctx.field(W_GQ + 2)};
-  double* qa[3] = {ctx.field(W_QA), ctx.field(W_QA + 1), ctx.field(W_QA + 2)};
-  double* qout[2] = {ctx.field(W_QOUT), ctx.field(W_QOUT + 1)};
-  double* rqh = ctx.field(W_RQ);
+  real* gq[3] = {ctx.field(W_GQ), ctx.field(W_GQ + 1), ctx.field(W_GQ + 2)};
+  real* qa[3] = {ctx.field(W_QA), ctx.field(W_QA + 1), ctx.field(W_QA + 2)};
+  real* qout[2] = {ctx.field(W_QOUT), ctx.field(W_QOUT + 1)};
+  real* rqh = ctx.field(W_RQ);
   for (int iq = 0; iq < nq; ++iq) {
-    double* rq = progq + (long)iq * NF;
+    real* rq = progq + (long)iq * NF;
     {
       TrcGradQ f;
       f.d = d; f.G = ctx.G;
@@ -301,7 +354,7 @@ void tracer_transport(X& x, const Context& ctx, double* progq, int nq, double dt
     {
       TrcRemap f;
       f.d = d; f.c = ctx.c; f.G = ctx.G;
-      f.dt = dt; f.rq = rq; f.rhog1 = rhog1;
+      f.dt = (real)dt; f.rq = rq; f.rhog1 = rhog1;
       for (int q = 0; q < 3; ++q) { f.gq[q] = gq[q]; f.fa[q] = fa[q]; f.qa[q] = qa[q]; }
       for (int q = 0; q < 9; ++q) f.xc[q] = xc[q];
       x.run(f, nc);
@@ -309,7 +362,7 @@ void tracer_transport(X& x, const Context& ctx, double* progq, int nq, double dt
     if (ctx.c.THUBURN_LIM) {
       TrcQout f;
       f.d = d; f.c = ctx.c; f.G = ctx.G;
-      f.dt = dt; f.rq = rq; f.rhog1 = rhog1; f.frhog = frhog;
+      f.dt = (real)dt; f.rq = rq; f.rhog1 = rhog1; f.frhog = frhog;
       for (int q = 0; q < 3; ++q) f.fa[q] = fa[q];
       f.qout[0] = qout[0]; f.qout[1] = qout[1];
       x.run(f, nc);
@@ -318,7 +371,7 @@ void tracer_transport(X& x, const Context& ctx, double* progq, int nq, double dt
     {
       TrcApply f;
       f.d = d; f.c = ctx.c; f.G = ctx.G;
-      f.dt = dt; f.rq = rq; f.rhog1 = rhog1;
+      f.dt = (real)dt; f.rq = rq; f.rhog1 = rhog1;
       for (int q = 0; q < 3; ++q) { f.fa[q] = fa[q]; f.qa[q] = qa[q]; }
       f.qout[0] = qout[0]; f.qout[1] = qout[1];
       f.out = rqh;
@@ -327,7 +380,7 @@ void tracer_transport(X& x, const Context& ctx, double* progq, int nq, double dt
     {
       TrcVert2 f;
       f.d = d; f.c = ctx.c; f.G = ctx.G;
-      f.dt = dt; f.flxv = flxv; f.rhog2 = rhog2; f.frhog = frhog;
+      f.dt = (real)dt; f.flxv = flxv; f.rhog2 = rhog2; f.frhog = frhog;
       f.in = rqh; f.out = rq;
       f.limiter = ctx.c.THUBURN_LIM;
       x.run(f, nc);
@@ -339,9 +392,9 @@ void tracer_transport(X& x, const Context& ctx, double* progq, int nq, double dt
 // the fluxes are prog's own momentum, rhog is the initial and the mean density,
 // the density tendency is zero; the step ends with the halo fill of progq
 template <class X>
-int tracer_step(X& x, const Context& ctx, const double* prog, double* progq, int nq, double dtl) {
+int tracer_step(X& x, const Context& ctx, const real* prog, real* progq, int nq, double dtl) {
   const Dims& d = ctx.d;
-  double* frhog = ctx.field(W_FRHOG);
+  real* frhog = ctx.field(W_FRHOG);
   x.run(FillZero{frhog}, d.NF);
   tracer_transport(x, ctx, progq, nq, dtl, prog, prog, frhog);
   halo_block(x, ctx, progq, nq);
@@ -349,7 +402,7 @@ int tracer_step(X& x, const Context& ctx, const double* prog, double* progq, int
 }
 
 template <class X>
-int trcadv_winds(X& x, const Context& ctx, double* prog, double time, int kind, const double* par) {
+int trcadv_winds(X& x, const Context& ctx, real* prog, double time, int kind, const double* par) {
   TrcadvWindsP f;
   f.d = ctx.d; f.G = ctx.G; f.prog = prog; f.time = time; f.kind = kind;
   for (int i = 0; i < 4; ++i) f.par[i] = par ? par[i] : 0.0;
@@ -357,19 +410,16 @@ int trcadv_winds(X& x, const Context& ctx, double* prog, double time, int kind, 
   return x.check("trcadv_winds");
 }
 
-// ----------------------------------------------------------------------------
-// one large step (mod_dynamics.py:1725-2741 with the RK3/HEVI schedule)
-// ----------------------------------------------------------------------------
-// global budgets (Embudget._store_var, mod_embudget.py:82-139): block partials
-// on the device into the W_GT scratch field, compensated host sum of them
+// global budgets (Embudget._store_var, mod_embudget.py:82-139): FP64 block
+// partials on the device into the FP64 scratch, compensated host sum of them
 template <class X>
-int budget(X& x, const Context& ctx, const double* prog, const double* progq, double lhv, double out[6]) {
+int budget(X& x, const Context& ctx, const real* prog, const real* progq, double lhv, double out[6]) {
   const Dims& d = ctx.d;
   const int W = d.g - 2;
   const long npts = (long)d.lall * (d.kmax - d.kmin + 1) * W * W;
   const int nblocks = (int)((npts + 255) / 256);
-  if ((long)nblocks * B_NTERMS * 2 > 6 * d.NF) return NH_ERR_ARG;
-  double* part = ctx.field(W_GT);
+  if ((long)nblocks * B_NTERMS * 2 > D_NFIELDS * d.NF) return NH_ERR_ARG;
+  double* part = ctx.dfield(0);
   BudgetT f{d, ctx.c, ctx.G, prog, progq, lhv, npts, part};
   x.run_tiles(f, nblocks, B_NTERMS * 256);
   std::vector<double> h((size_t)nblocks * B_NTERMS * 2);
@@ -393,25 +443,28 @@ int budget(X& x, const Context& ctx, const double* prog, const double* progq, do
   return 0;
 }
 
+// ----------------------------------------------------------------------------
+// one large step (mod_dynamics.py:1725-2741 with the RK3/HEVI schedule)
+// ----------------------------------------------------------------------------
 template <class X>
-int dyn_step(X& x, const Context& ctx, double* prog, double* progq, int nq, double dtl, int sstep_max) {
+int dyn_step(X& x, const Context& ctx, real* prog, real* progq, int nq, double dtl, int sstep_max) {
   const Dims& d = ctx.d;
   const long NF = d.NF;
   const long nc = ncols(d);
   const double dts = dtl / (double)sstep_max;
   const int nsub[3] = {sstep_max / 3, sstep_max / 2, sstep_max};
   if (nq < 1 || !prog || !progq) return NH_ERR_ARG;   // tracer 0 is qv (zero in a dry run)
-  const double* rhogq = progq;
-  double* prog0 = ctx.field(W_PROG0);
-  double* diag = ctx.field(W_DIAG);
-  double* eth = ctx.field(W_ETH);
-  double* gadv = ctx.field(W_GADV);
-  double* gT = ctx.field(W_GT);
-  double* pm = ctx.field(W_PM);
-  double* ethtot = ctx.field(W_ETHTOT);
-  double* getot = ctx.field(W_GETOT);
-  double* frhog = ctx.field(W_FRHOG);
-  x.d2d(prog0, prog, 6 * NF * sizeof(double));
+  const real* rhogq = progq;
+  real* prog0 = ctx.field(W_PROG0);
+  real* diag = ctx.field(W_DIAG);
+  real* eth = ctx.field(W_ETH);
+  real* gadv = ctx.field(W_GADV);
+  real* gT = ctx.field(W_GT);
+  real* pm = ctx.field(W_PM);
+  real* ethtot = ctx.field(W_ETHTOT);
+  real* getot = ctx.field(W_GETOT);
+  real* frhog = ctx.field(W_FRHOG);
+  x.d2d(prog0, prog, 6 * NF * sizeof(real));
   for (int nl = 0; nl < 3; ++nl) {
     // diagnostics and boundary values
     {
@@ -453,7 +506,7 @@ int dyn_step(X& x, const Context& ctx, double* prog, double* progq, int nq, doub
       for (int q = 0; q < 6; ++q) f.vt[q] = ctx.field(W_HDB + q);
       for (int q = 0; q < 3; ++q) f.dd[q] = ctx.field(W_DD + q);
       f.gT = gT;
-      f.Mc = ctx.field(W_MC); f.Mu = ctx.field(W_MU); f.Ml = ctx.field(W_ML);
+      f.Mc = ctx.dfield(D_MC); f.Mu = ctx.dfield(D_MU); f.Ml = ctx.dfield(D_ML);
       f.getot = getot; f.frhog = frhog;
       x.run_pts(f, d);
     }
@@ -462,9 +515,9 @@ int dyn_step(X& x, const Context& ctx, double* prog, double* progq, int nq, doub
       x.run_pts(f, d);
     }
     // split state: ps[0..5], then path1 momentum dv[0..2] and vi_main output we[0..2]
-    double* ps[6];
-    double* dv[3];
-    double* we[3];
+    real* ps[6];
+    real* dv[3];
+    real* we[3];
     for (int q = 0; q < 6; ++q) ps[q] = ctx.field(W_SPLIT + q);
     for (int q = 0; q < 3; ++q) dv[q] = ctx.field(W_SPLIT + 6 + q);
     for (int q = 0; q < 3; ++q) we[q] = ctx.field(W_SPLIT + 9 + q);
@@ -474,12 +527,12 @@ int dyn_step(X& x, const Context& ctx, double* prog, double* progq, int nq, doub
     }
     const double rweight = 1.0 / (double)nsub[nl];
     for (int it = 0; it < nsub[nl]; ++it) {
-      double* ddv = ctx.field(W_DDV);
+      real* ddv = ctx.field(W_DDV);
       divdamp(x, ctx, ps[1], ps[2], ps[3], ps[4], ddv, ddv + NF, ddv + 2 * NF);
       {
         Path1P f;
         f.d = d; f.c = ctx.c; f.G = ctx.G;
-        f.dt = dts; f.rweight = rweight;
+        f.dt = (real)dts; f.rweight = (real)rweight;
         f.prog = prog;
         for (int q = 0; q < 6; ++q) f.ps[q] = ps[q];
         f.gT = gT;
@@ -489,7 +542,7 @@ int dyn_step(X& x, const Context& ctx, double* prog, double* progq, int nq, doub
         x.run_pts(f, d);
       }
       halo(x, ctx, dv, 3);
-      double* hv[3] = {ctx.field(W_HV), ctx.field(W_HV + 1), ctx.field(W_HV + 2)};
+      real* hv[3] = {ctx.field(W_HV), ctx.field(W_HV + 1), ctx.field(W_HV + 2)};
       {
         ViMainH f;
         f.d = d; f.c = ctx.c; f.G = ctx.G;
@@ -505,15 +558,16 @@ int dyn_step(X& x, const Context& ctx, double* prog, double* progq, int nq, doub
         f.dt = dts;
         for (int q = 0; q < 6; ++q) f.ps[q] = ps[q];
         f.gT = gT;
-        f.Mc = ctx.field(W_MC); f.Mu = ctx.field(W_MU); f.Ml = ctx.field(W_ML);
+        f.Mc = ctx.dfield(D_MC); f.Mu = ctx.dfield(D_MU); f.Ml = ctx.dfield(D_ML);
         for (int q = 0; q < 3; ++q) f.hv[q] = hv[q];
-        for (int q = 0; q < 3; ++q) f.we[q] = we[q];
+        f.U = ctx.dfield(D_U); f.Gm = ctx.dfield(D_GM);
+        f.out = we[1];
         x.run(f, nc);
       }
       {
         ViMainPost f;
         f.d = d; f.c = ctx.c; f.G = ctx.G;
-        f.dt = dts; f.rweight = rweight;
+        f.dt = (real)dts; f.rweight = (real)rweight;
         f.prog = prog;
         for (int q = 0; q < 6; ++q) f.ps[q] = ps[q];
         for (int q = 0; q < 3; ++q) f.dv[q] = dv[q];
@@ -525,7 +579,7 @@ int dyn_step(X& x, const Context& ctx, double* prog, double* progq, int nq, doub
       }
       halo(x, ctx, we, 3);
       // rotate buffers: the new split state is (we0, dv0, dv1, dv2, we1, we2)
-      double* old[6] = {ps[0], ps[1], ps[2], ps[3], ps[4], ps[5]};
+      real* old[6] = {ps[0], ps[1], ps[2], ps[3], ps[4], ps[5]};
       ps[0] = we[0]; ps[1] = dv[0]; ps[2] = dv[1]; ps[3] = dv[2]; ps[4] = we[1]; ps[5] = we[2];
       we[0] = old[0]; dv[0] = old[1]; dv[1] = old[2]; dv[2] = old[3]; we[1] = old[4]; we[2] = old[5];
     }
@@ -547,4 +601,4 @@ int dyn_step(X& x, const Context& ctx, double* prog, double* progq, int nq, doub
   return x.check("dyn_step");
 }
 
-}  // namespace nh
+NH_END

@@ -1,0 +1,166 @@
+// One precision variant of the library behind the interface of nh_variant.h,
+// generic over the executor type NHX defined by the including translation unit:
+//   NHX(int device, void* stream, void** prof_slot)  sets the device, binds the
+//   stream and the context's kernel-timer state;
+//   alloc/release/zero/h2d/d2h/d2d, run/run_pts/run_tiles, check, message,
+//   prof_enable/prof_report/prof_release.
+#pragma once
+
+#include <new>
+#include <string>
+
+#include "nh_variant.h"
+#include "nh_engine.h"
+
+NH_BEGIN
+
+namespace {
+inline int failv(std::string* err, int code, const std::string& msg) {
+  if (err) *err = msg;
+  return code;
+}
+}  // namespace
+
+int v_create(const nh_dims* dims, const nh_consts* consts, int device, Context** out, std::string* err) {
+  Context* c = new (std::nothrow) Context();
+  if (!c) return failv(err, NH_ERR_ALLOC, "host allocation");
+  set_dims(c->d, dims);
+  set_consts(c->c, consts);
+  c->device = device;
+  NHX x(device, nullptr, nullptr);
+  if (x.check("nh_create")) { delete c; return failv(err, NH_ERR_DEVICE, x.message()); }
+  const size_t bytes = (size_t)W_NFIELDS * c->d.NF * sizeof(real);
+  const size_t dbytes = (size_t)D_NFIELDS * c->d.NF * sizeof(double);
+  c->ws = (real*)x.alloc(bytes);
+  c->dws = (double*)x.alloc(dbytes);
+  if (!c->ws || !c->dws) {
+    if (c->ws) x.release(c->ws);
+    if (c->dws) x.release(c->dws);
+    delete c;
+    return failv(err, NH_ERR_ALLOC, "workspace allocation of " + std::to_string(bytes + dbytes) + " bytes");
+  }
+  x.zero(c->ws, bytes);
+  x.zero(c->dws, dbytes);
+  if (x.check("nh_create")) {
+    x.release(c->ws);
+    x.release(c->dws);
+    delete c;
+    return failv(err, NH_ERR_DEVICE, x.message());
+  }
+  *out = c;
+  return 0;
+}
+
+void v_destroy(Context* c) {
+  if (!c) return;
+  NHX x(c->device, nullptr, &c->prof);
+  x.sync();
+  void* bufs[7] = {c->ws, c->dws, c->geo_buf, c->geo64_buf, c->int_buf, c->halo_dst, c->halo_src};
+  for (void* b : bufs)
+    if (b) x.release(b);
+  x.prof_release();
+  delete c;
+}
+
+bool v_geo_ready(const Context* c) { return c->geo_ready; }
+long v_halo_n(const Context* c) { return c->halo_n; }
+
+int v_upload_geometry(Context* c, const nh_geometry* g, std::string* err) {
+  NHX x(c->device, nullptr, &c->prof);
+  int rc = upload_geometry(x, *c, g);
+  if (rc == NH_ERR_ARG) return failv(err, rc, "null coefficient pointer");
+  if (rc == NH_ERR_ALLOC) return failv(err, rc, "device allocation");
+  if (rc) return failv(err, rc, x.message());
+  return 0;
+}
+
+int v_upload_halo(Context* c, const int32_t* dst, const int32_t* src, long n, std::string* err) {
+  NHX x(c->device, nullptr, &c->prof);
+  int rc = upload_halo(x, *c, dst, src, n);
+  if (rc == NH_ERR_ARG) return failv(err, rc, "column id out of range");
+  if (rc == NH_ERR_ALLOC) return failv(err, rc, "device allocation");
+  if (rc) return failv(err, rc, x.message());
+  return 0;
+}
+
+int v_halo_fill(Context* c, void* const* fields, int nf, void* stream, std::string* err) {
+  NHX x(c->device, stream, &c->prof);
+  real* const* f = reinterpret_cast<real* const*>(fields);
+  for (int q0 = 0; q0 < nf; q0 += 16) halo(x, *c, f + q0, (nf - q0 < 16) ? nf - q0 : 16);
+  if (x.check("nh_halo_fill")) return failv(err, NH_ERR_DEVICE, x.message());
+  return 0;
+}
+
+int v_laplacian(Context* c, const void* scl, void* out, void* stream, std::string* err) {
+  NHX x(c->device, stream, &c->prof);
+  OpLapP f{c->d, c->G, (const real*)scl, (real*)out};
+  x.run_pts(f, c->d);
+  if (x.check("nh_laplacian")) return failv(err, NH_ERR_DEVICE, x.message());
+  return 0;
+}
+
+int v_divergence(Context* c, const void* vx, const void* vy, const void* vz, void* out, void* stream,
+                 std::string* err) {
+  NHX x(c->device, stream, &c->prof);
+  OpDivP f{c->d, c->G, (const real*)vx, (const real*)vy, (const real*)vz, (real*)out};
+  x.run_pts(f, c->d);
+  if (x.check("nh_divergence")) return failv(err, NH_ERR_DEVICE, x.message());
+  return 0;
+}
+
+int v_gradient(Context* c, const void* scl, void* gx, void* gy, void* gz, void* stream, std::string* err) {
+  NHX x(c->device, stream, &c->prof);
+  OpGradP f{c->d, c->G, (const real*)scl, (real*)gx, (real*)gy, (real*)gz};
+  x.run_pts(f, c->d);
+  if (x.check("nh_gradient")) return failv(err, NH_ERR_DEVICE, x.message());
+  return 0;
+}
+
+int v_dyn_step(Context* c, void* prog, void* progq, int nq, double dtl, int sstep_max, void* stream,
+               std::string* err) {
+  NHX x(c->device, stream, &c->prof);
+  int rc = dyn_step(x, *c, (real*)prog, (real*)progq, nq, dtl, sstep_max);
+  if (rc == NH_ERR_ARG) return failv(err, rc, "invalid argument");
+  if (rc) return failv(err, NH_ERR_DEVICE, x.message());
+  return 0;
+}
+
+int v_tracer_step(Context* c, const void* prog, void* progq, int nq, double dtl, void* stream, std::string* err) {
+  NHX x(c->device, stream, &c->prof);
+  int rc = tracer_step(x, *c, (const real*)prog, (real*)progq, nq, dtl);
+  if (rc) return failv(err, NH_ERR_DEVICE, x.message());
+  return 0;
+}
+
+int v_trcadv_winds(Context* c, void* prog, double time, int kind, const double* params, void* stream,
+                   std::string* err) {
+  NHX x(c->device, stream, &c->prof);
+  int rc = trcadv_winds(x, *c, (real*)prog, time, kind, params);
+  if (rc) return failv(err, NH_ERR_DEVICE, x.message());
+  return 0;
+}
+
+int v_budget(Context* c, const void* prog, const void* progq, double lhv, double* out, void* stream,
+             std::string* err) {
+  NHX x(c->device, stream, &c->prof);
+  int rc = budget(x, *c, (const real*)prog, (const real*)progq, lhv, out);
+  if (rc == NH_ERR_ARG) return failv(err, rc, "grid too small for the partial-sum buffer");
+  if (rc) return failv(err, NH_ERR_DEVICE, x.message());
+  return 0;
+}
+
+int v_prof_enable(Context* c, bool on, std::string* err) {
+  NHX x(c->device, nullptr, &c->prof);
+  if (x.prof_enable(on)) return failv(err, NH_ERR_DEVICE, x.message());
+  return 0;
+}
+
+int v_prof_report(Context* c, int max_kernels, char* names, double* total_ms, int64_t* launches,
+                  std::string* err) {
+  NHX x(c->device, nullptr, &c->prof);
+  int n = x.prof_report(max_kernels, names, total_ms, launches);
+  if (n < 0) return failv(err, NH_ERR_DEVICE, x.message());
+  return n;
+}
+
+NH_END

@@ -12,7 +12,7 @@
 
 #include "nh_step.h"
 
-namespace nh {
+NH_BEGIN
 
 NH_HD inline Cell cell_of_pt(const Pt& p) {
   Cell c;
@@ -31,8 +31,8 @@ NH_HD inline Cell cell_of_pt(const Pt& p) {
 struct OpLapP {
   static constexpr const char* kName = "oprt_laplacian";
   Dims d; Geo G;
-  const double* s;
-  double* o;
+  const real* s;
+  real* o;
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
@@ -43,8 +43,8 @@ struct OpLapP {
 struct OpDivP {
   static constexpr const char* kName = "oprt_divergence";
   Dims d; Geo G;
-  const double *vx, *vy, *vz;
-  double* o;
+  const real *vx, *vy, *vz;
+  real* o;
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
@@ -64,12 +64,12 @@ struct OpDivP {
 struct OpGradP {
   static constexpr const char* kName = "oprt_gradient";
   Dims d; Geo G;
-  const double* s;
-  double *ox, *oy, *oz;
+  const real* s;
+  real *ox, *oy, *oz;
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
-    double o[3];
+    real o[3];
     if (p.pole) grad_pl(G, p.lp, s, p.a, o);
     else grad7(G, d.nreg, p.ci, s, p.a, d.g, o);
     ox[p.a] = o[0];
@@ -84,20 +84,20 @@ struct OpGradP {
 struct PrePostP {
   static constexpr const char* kName = "prepost";
   Dims d; Consts c; Geo G;
-  double* prog;
-  const double* rhogq;
-  double* diag;
-  double* eth;
-  struct Th { double rho, ein, tem, pre; };
+  real* prog;
+  const real* rhogq;
+  real* diag;
+  real* eth;
+  struct Th { real rho, ein, tem, pre; };
   NH_HD Th thermo(long a) const {
     const long NF = d.NF;
     Th t;
-    const double rhog = prog[a];
-    t.rho = rhog / 1.0;
+    const real rhog = prog[a];
+    t.rho = rhog / real(1.0);
     t.ein = prog[5 * NF + a] / rhog;
-    const double q = rhogq[a] / rhog;
-    const double qd = 1.0 - q;
-    const double cv = q * c.CVW_qv + qd * c.CVdry;
+    const real q = rhogq[a] / rhog;
+    const real qd = real(1.0) - q;
+    const real cv = q * c.CVW_qv + qd * c.CVdry;
     t.tem = t.ein / cv;
     t.pre = t.rho * t.tem * (qd * c.Rdry + q * c.Rvap);
     return t;
@@ -107,14 +107,14 @@ struct PrePostP {
     if (!pt_of(d, plane, x, p)) return;
     const long NF = d.NF, a = p.a, ks = p.ks;
     const int kmin = d.kmin, kmax = d.kmax, k = p.k;
-    double* rg = prog;
-    double* rw = prog + 4 * NF;
-    double* pre = diag + I_PRE * NF;
-    double* tem = diag + I_TEM * NF;
-    double* w = diag + I_W * NF;
+    real* rg = prog;
+    real* rw = prog + 4 * NF;
+    real* pre = diag + I_PRE * NF;
+    real* tem = diag + I_TEM * NF;
+    real* w = diag + I_W * NF;
     if (k >= kmin && k <= kmax) {
       Th t = thermo(a);
-      const double rhog = rg[a];
+      const real rhog = rg[a];
       diag[I_VX * NF + a] = prog[1 * NF + a] / rhog;
       diag[I_VY * NF + a] = prog[2 * NF + a] / rhog;
       diag[I_VZ * NF + a] = prog[3 * NF + a] / rhog;
@@ -125,10 +125,10 @@ struct PrePostP {
       } else {
         // bottom: rigid lid, with the density of the ghost row after the boundary condition
         const Th t1 = thermo(a + ks);
-        const double pre_b = t1.pre - t.rho * (G.phi[kmin - 1] - G.phi[kmin + 1]);
-        const double rho_b = pre_b / (c.Rdry * t.tem);
-        rw[a] = 0.0;
-        w[a] = rw[a] / (G.c2w0[kmin] * rg[a] + G.c2w1[kmin] * (rho_b * 1.0));
+        const real pre_b = t1.pre - t.rho * (G.phi[kmin - 1] - G.phi[kmin + 1]);
+        const real rho_b = pre_b / (c.Rdry * t.tem);
+        rw[a] = real(0.0);
+        w[a] = rw[a] / (G.c2w0[kmin] * rg[a] + G.c2w1[kmin] * (rho_b * real(1.0)));
       }
       eth[a] = t.ein + t.pre / t.rho;
       return;
@@ -139,28 +139,28 @@ struct PrePostP {
     const long an2 = top ? a - 2 * ks : a + 2 * ks;    // next one
     const Th tn = thermo(an);
     const Th tn2 = thermo(an2);
-    const double tem_g = tn.tem;
-    const double pre_g = top ? tn2.pre - tn.rho * (G.phi[kmax + 1] - G.phi[kmax - 1])
+    const real tem_g = tn.tem;
+    const real pre_g = top ? tn2.pre - tn.rho * (G.phi[kmax + 1] - G.phi[kmax - 1])
                              : tn2.pre - tn.rho * (G.phi[kmin - 1] - G.phi[kmin + 1]);
-    const double rho_g = pre_g / (c.Rdry * tem_g);
-    const double rg_g = rho_g * 1.0;
-    const double ein_g = c.CVdry * tem_g;
+    const real rho_g = pre_g / (c.Rdry * tem_g);
+    const real rg_g = rho_g * real(1.0);
+    const real ein_g = c.CVdry * tem_g;
     tem[a] = tem_g;
     pre[a] = pre_g;
     rg[a] = rg_g;
     prog[5 * NF + a] = rg_g * ein_g;
-    const double sgn = top ? 1.0 : -1.0;
+    const real sgn = top ? real(1.0) : -real(1.0);
     for (int q = 0; q < 3; ++q) {
-      const double v = sgn * (prog[(1 + q) * NF + an] / rg[an]) * rg_g;
+      const real v = sgn * (prog[(1 + q) * NF + an] / rg[an]) * rg_g;
       prog[(1 + q) * NF + a] = v;
       diag[(I_VX + q) * NF + a] = v / rg_g;
     }
     if (top) {
-      rw[a] = -0.0;
+      rw[a] = -real(0.0);
       w[a] = rw[a] / (G.c2w0[kmax + 1] * rg_g + G.c2w1[kmax + 1] * rg[an]);
     } else {
-      rw[a] = 0.0;
-      w[a] = 0.0;
+      rw[a] = real(0.0);
+      w[a] = real(0.0);
     }
     eth[a] = ein_g + pre_g / rho_g;
   }
@@ -174,30 +174,30 @@ struct PrePostP {
 struct AdvMomP {
   static constexpr const char* kName = "adv_mom";
   Dims d; Consts c; Geo G;
-  const double* prog;
-  const double* diag;
-  double* out;
+  const real* prog;
+  const real* diag;
+  real* out;
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p)) return;
     const long NF = d.NF, a = p.a, ks = p.ks;
     const int kmin = d.kmin, kmax = d.kmax, k = p.k;
     if (!pt_compute(p) || k < kmin || k > kmax) {
-      for (int q = 0; q < 4; ++q) out[q * NF + a] = 0.0;
+      for (int q = 0; q < 4; ++q) out[q * NF + a] = real(0.0);
       return;
     }
-    const double R = c.RADIUS;
-    const double* rvx = prog + 1 * NF;
-    const double* rvy = prog + 2 * NF;
-    const double* rvz = prog + 3 * NF;
-    const double* rw = prog + 4 * NF;
-    const double* rg = prog;
-    const double* vxd = diag + I_VX * NF;
-    const double* vyd = diag + I_VY * NF;
-    const double* vzd = diag + I_VZ * NF;
-    const double* wd = diag + I_W * NF;
+    const real R = c.RADIUS;
+    const real* rvx = prog + 1 * NF;
+    const real* rvy = prog + 2 * NF;
+    const real* rvz = prog + 3 * NF;
+    const real* rw = prog + 4 * NF;
+    const real* rg = prog;
+    const real* vxd = diag + I_VX * NF;
+    const real* vyd = diag + I_VY * NF;
+    const real* vzd = diag + I_VZ * NF;
+    const real* wd = diag + I_W * NF;
     long nb[7];
-    double px[7], py[7], pz[7];
+    real px[7], py[7], pz[7];
     const int nn = p.pole ? 6 : 7;
     if (p.pole) {
       for (int v = 0; v < 6; ++v) {
@@ -218,49 +218,49 @@ struct AdvMomP {
       }
     }
     // merged velocity v + w_full r/|r| of neighbour m at level kk
-    auto vv3 = [&](int m, int kk, double* o) {
+    auto vv3 = [&](int m, int kk, real* o) {
       const long b = nb[m] + (long)(kk - k) * ks;
-      const double wc = G.cfact[kk] * wd[b + ks] + G.dfact[kk] * wd[b];
+      const real wc = G.cfact[kk] * wd[b + ks] + G.dfact[kk] * wd[b];
       o[0] = vxd[b] + wc * px[m] / R;
       o[1] = vyd[b] + wc * py[m] / R;
       o[2] = vzd[b] + wc * pz[m] / R;
     };
-    double vvm[7][3];
+    real vvm[7][3];
     for (int m = 0; m < nn; ++m) vv3(m, k, vvm[m]);
-    double div[3];
+    real div[3];
     for (int q = 0; q < 3; ++q) {
       if (p.pole) {
-        div[q] = div_pl(G, p.lp, [&](int v) { return rvx[nb[v]] * vvm[v][q] * 1.0; },
-                        [&](int v) { return rvy[nb[v]] * vvm[v][q] * 1.0; },
-                        [&](int v) { return rvz[nb[v]] * vvm[v][q] * 1.0; });
+        div[q] = div_pl(G, p.lp, [&](int v) { return rvx[nb[v]] * vvm[v][q] * real(1.0); },
+                        [&](int v) { return rvy[nb[v]] * vvm[v][q] * real(1.0); },
+                        [&](int v) { return rvz[nb[v]] * vvm[v][q] * real(1.0); });
       } else {
-        div[q] = div7(G, d.nreg, p.ci, [&](int m) { return rvx[nb[m]] * vvm[m][q] * 1.0; },
-                      [&](int m) { return rvy[nb[m]] * vvm[m][q] * 1.0; },
-                      [&](int m) { return rvz[nb[m]] * vvm[m][q] * 1.0; });
+        div[q] = div7(G, d.nreg, p.ci, [&](int m) { return rvx[nb[m]] * vvm[m][q] * real(1.0); },
+                      [&](int m) { return rvy[nb[m]] * vvm[m][q] * real(1.0); },
+                      [&](int m) { return rvz[nb[m]] * vvm[m][q] * real(1.0); });
       }
     }
     // vertical fluxes at the half levels k+1 and k (zero at kmin and kmax+1)
-    double hi[3] = {0.0, 0.0, 0.0}, lo[3] = {0.0, 0.0, 0.0};
+    real hi[3] = {real(0.0), real(0.0), real(0.0)}, lo[3] = {real(0.0), real(0.0), real(0.0)};
     if (k + 1 <= kmax) {
-      double up[3];
+      real up[3];
       vv3(0, k + 1, up);
-      const double wup = rw[a + ks];
-      for (int q = 0; q < 3; ++q) hi[q] = 0.0 * 1.0 + wup * (G.afact[k + 1] * up[q] + G.bfact[k + 1] * vvm[0][q]) * 1.0;
+      const real wup = rw[a + ks];
+      for (int q = 0; q < 3; ++q) hi[q] = real(0.0) * real(1.0) + wup * (G.afact[k + 1] * up[q] + G.bfact[k + 1] * vvm[0][q]) * real(1.0);
     }
     if (k >= kmin + 1) {
-      double dn[3];
+      real dn[3];
       vv3(0, k - 1, dn);
-      const double w0 = rw[a];
-      for (int q = 0; q < 3; ++q) lo[q] = 0.0 * 1.0 + w0 * (G.afact[k] * vvm[0][q] + G.bfact[k] * dn[q]) * 1.0;
+      const real w0 = rw[a];
+      for (int q = 0; q < 3; ++q) lo[q] = real(0.0) * real(1.0) + w0 * (G.afact[k] * vvm[0][q] + G.bfact[k] * dn[q]) * real(1.0);
     }
-    double dvv[3];
+    real dvv[3];
     for (int q = 0; q < 3; ++q) dvv[q] = -div[q] - (hi[q] - lo[q]) * G.rdgz[k];
-    const double gxc = px[0] / R, gyc = py[0] / R, gzc = pz[0] / R;
-    const double rh = rg[a];
-    const double dx = dvv[0] - (-2.0 * rh * (c.OHM * vvm[0][1]));
-    const double dy = dvv[1] - (2.0 * rh * (c.OHM * vvm[0][0]));
-    const double dz = dvv[2];
-    const double prd = dx * gxc + dy * gyc + dz * gzc;
+    const real gxc = px[0] / R, gyc = py[0] / R, gzc = pz[0] / R;
+    const real rh = rg[a];
+    const real dx = dvv[0] - (-real(2.0) * rh * (c.OHM * vvm[0][1]));
+    const real dy = dvv[1] - (real(2.0) * rh * (c.OHM * vvm[0][0]));
+    const real dz = dvv[2];
+    const real prd = dx * gxc + dy * gyc + dz * gzc;
     out[0 * NF + a] = dx - prd * gxc;
     out[1 * NF + a] = dy - prd * gyc;
     out[2 * NF + a] = dz - prd * gzc;
@@ -275,20 +275,20 @@ template <int LAST>
 struct HDiffPassP {
   static constexpr const char* kName = LAST ? "hdiff_pass2" : "hdiff_pass1";
   Dims d; Consts c; Geo G;
-  const double* in[6];
-  double* out[6];
-  const double* rhog;
+  const real* in[6];
+  real* out[6];
+  const real* rhog;
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
     const long a = p.a;
     const Cell cl = cell_of_pt(p);
     for (int q = 0; q < 6; ++q) {
-      double r;
+      real r;
       if (LAST && q >= 4) {
-        const double fac = (q == 4) ? c.CVdry : c.HDIFF_FACT_RHO;
-        const double* rg = rhog;
-        const double gam = c.GAMMA_H;
+        const real fac = (q == 4) ? c.CVdry : c.HDIFF_FACT_RHO;
+        const real* rg = rhog;
+        const real gam = c.GAMMA_H;
         auto kh = [rg, fac, gam](long xx) { return rg[xx] * fac * gam; };
         r = p.pole ? diffusion_pl(G, p.lp, in[q], kh, a) : diffusion7(G, d, cl, in[q], kh, a);
       } else {
@@ -305,39 +305,39 @@ struct HDiffPassP {
 struct DivDamp1P {
   static constexpr const char* kName = "divdamp_3d";
   Dims d; Consts c; Geo G;
-  const double *rvx, *rvy, *rvz, *rw;
-  double* out[3];
+  const real *rvx, *rvy, *rvz, *rw;
+  real* out[3];
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
     const int kmin = d.kmin, kmax = d.kmax, g = d.g, k = p.k;
     const long ks = p.ks, a = p.a;
     if (k < kmin || k > kmax) {
-      for (int q = 0; q < 3; ++q) out[q][a] = 0.0;
+      for (int q = 0; q < 3; ++q) out[q][a] = real(0.0);
       return;
     }
-    auto rwv = [&](long aa, int kk) { return (kk >= kmin + 1 && kk <= kmax) ? (0.0 * 1.0 + rw[aa] * 1.0) : 0.0; };
+    auto rwv = [&](long aa, int kk) { return (kk >= kmin + 1 && kk <= kmax) ? (real(0.0) * real(1.0) + rw[aa] * real(1.0)) : real(0.0); };
     if (p.pole) {
-      double sclt[5];
-      const double* vv[3] = {rvx, rvy, rvz};
+      real sclt[5];
+      const real* vv[3] = {rvx, rvy, rvz};
       for (int v = 1; v <= 5; ++v) {
         const int vp = (v == 5) ? 1 : v + 1;
-        double s = 0.0;
+        real s = real(0.0);
         for (int dd = 0; dd < 3; ++dd) {
-          const double* cI = G.cintp_pl + ((v * 2 + p.lp) * 3 + dd) * 3;
-          const double t0 = cI[0] * (vv[dd][a] * 1.0), t1 = cI[1] * (vv[dd][a + v] * 1.0),
-                       t2 = cI[2] * (vv[dd][a + vp] * 1.0);
+          const real* cI = G.cintp_pl + ((v * 2 + p.lp) * 3 + dd) * 3;
+          const real t0 = cI[0] * (vv[dd][a] * real(1.0)), t1 = cI[1] * (vv[dd][a + v] * real(1.0)),
+                       t2 = cI[2] * (vv[dd][a + vp] * real(1.0));
           s = (dd == 0) ? (t0 + t1 + t2) : (s + t0 + t1 + t2);
         }
-        const double up = (rwv(a + ks, k + 1) + rwv(a + v + ks, k + 1) + rwv(a + vp + ks, k + 1));
-        const double dn = (rwv(a, k) + rwv(a + v, k) + rwv(a + vp, k));
-        sclt[v - 1] = s + (up - dn) / 3.0 * G.rdgz[k];
+        const real up = (rwv(a + ks, k + 1) + rwv(a + v + ks, k + 1) + rwv(a + vp + ks, k + 1));
+        const real dn = (rwv(a, k) + rwv(a + v, k) + rwv(a + vp, k));
+        sclt[v - 1] = s + (up - dn) / real(3.0) * G.rdgz[k];
       }
       for (int dd = 0; dd < 3; ++dd) {
-        double row = 0.0;
+        real row = real(0.0);
         for (int v = 1; v <= 5; ++v) {
           const int vm = (v == 1) ? 5 : v - 1;
-          const double cb = G.cdiff_pl[(v * 2 + p.lp) * 3 + dd] * (sclt[vm - 1] + sclt[v - 1]);
+          const real cb = G.cdiff_pl[(v * 2 + p.lp) * 3 + dd] * (sclt[vm - 1] + sclt[v - 1]);
           row = (v == 1) ? cb : row + cb;
         }
         out[dd][a] = row;
@@ -347,19 +347,19 @@ struct DivDamp1P {
     const long nreg = d.nreg, ci = p.ci;
     const long co[6] = {0, 0, -1, -1 - (long)g, -1 - (long)g, -(long)g};
     const int ty[6] = {0, 1, 0, 1, 0, 1};
-    double T[6];
+    real T[6];
     for (int n = 0; n < 6; ++n) {
       const long aa = a + co[n];
       const long cj = ci + co[n];
       const long o1 = (ty[n] == 0) ? 1 : 1 + g;
       const long o2 = (ty[n] == 0) ? 1 + g : g;
-      const double s = tri_div(G, nreg, cj, ty[n], rvx, rvy, rvz, aa, g, 1.0);
-      const double up = (rwv(aa + ks, k + 1) + rwv(aa + o1 + ks, k + 1) + rwv(aa + o2 + ks, k + 1));
-      const double dn = (rwv(aa, k) + rwv(aa + o1, k) + rwv(aa + o2, k));
-      T[n] = s + (up - dn) / 3.0 * G.rdgz[k];
+      const real s = tri_div(G, nreg, cj, ty[n], rvx, rvy, rvz, aa, g, real(1.0));
+      const real up = (rwv(aa + ks, k + 1) + rwv(aa + o1 + ks, k + 1) + rwv(aa + o2 + ks, k + 1));
+      const real dn = (rwv(aa, k) + rwv(aa + o1, k) + rwv(aa + o2, k));
+      T[n] = s + (up - dn) / real(3.0) * G.rdgz[k];
     }
     if (p.i == 1 && p.j == 1) {
-      const double m0 = G.sgp[p.l] ? 0.0 : 1.0, m1 = G.sgp[p.l] ? 1.0 : 0.0;
+      const real m0 = G.sgp[p.l] ? real(0.0) : real(1.0), m1 = G.sgp[p.l] ? real(1.0) : real(0.0);
       T[4] = T[4] * m0 + T[5] * m1;
     }
     for (int dd = 0; dd < 3; ++dd) out[dd][a] = hex_sum(G, nreg, ci, dd, T);
@@ -369,38 +369,38 @@ struct DivDamp1P {
 struct DivDamp2P {
   static constexpr const char* kName = "divdamp_post";
   Dims d; Consts c; Geo G;
-  const double* in[3];
-  double* out[3];
+  const real* in[3];
+  real* out[3];
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p)) return;
     const long a = p.a;
     const int kmin = d.kmin, kmax = d.kmax, g = d.g, k = p.k;
     if (!pt_compute(p)) {
-      for (int q = 0; q < 3; ++q) out[q][a] = 0.0;
+      for (int q = 0; q < 3; ++q) out[q][a] = real(0.0);
       return;
     }
-    const double R = c.RADIUS;
-    double o[3];
+    const real R = c.RADIUS;
+    real o[3];
     if (k < kmin || k > kmax) {
-      o[0] = o[1] = o[2] = 0.0;
+      o[0] = o[1] = o[2] = real(0.0);
     } else if (p.pole) {
-      double sclt[5];
+      real sclt[5];
       for (int v = 1; v <= 5; ++v) {
         const int vp = (v == 5) ? 1 : v + 1;
-        double s = 0.0;
+        real s = real(0.0);
         for (int dd = 0; dd < 3; ++dd) {
-          const double* cI = G.cintp_pl + ((v * 2 + p.lp) * 3 + dd) * 3;
-          const double t0 = cI[0] * (-in[dd][a]), t1 = cI[1] * (-in[dd][a + v]), t2 = cI[2] * (-in[dd][a + vp]);
+          const real* cI = G.cintp_pl + ((v * 2 + p.lp) * 3 + dd) * 3;
+          const real t0 = cI[0] * (-in[dd][a]), t1 = cI[1] * (-in[dd][a + v]), t2 = cI[2] * (-in[dd][a + vp]);
           s = (dd == 0) ? (t0 + t1 + t2) : (s + t0 + t1 + t2);
         }
         sclt[v - 1] = s;
       }
       for (int dd = 0; dd < 3; ++dd) {
-        double row = 0.0;
+        real row = real(0.0);
         for (int v = 1; v <= 5; ++v) {
           const int vm = (v == 1) ? 5 : v - 1;
-          const double cb = G.cdiff_pl[(v * 2 + p.lp) * 3 + dd] * (sclt[vm - 1] + sclt[v - 1]);
+          const real cb = G.cdiff_pl[(v * 2 + p.lp) * 3 + dd] * (sclt[vm - 1] + sclt[v - 1]);
           row = (v == 1) ? cb : row + cb;
         }
         o[dd] = row;
@@ -409,20 +409,20 @@ struct DivDamp2P {
       const long nreg = d.nreg, ci = p.ci;
       const long co[6] = {0, 0, -1, -1 - (long)g, -1 - (long)g, -(long)g};
       const int ty[6] = {0, 1, 0, 1, 0, 1};
-      double T[6];
-      for (int n = 0; n < 6; ++n) T[n] = tri_div(G, nreg, ci + co[n], ty[n], in[0], in[1], in[2], a + co[n], g, -1.0);
+      real T[6];
+      for (int n = 0; n < 6; ++n) T[n] = tri_div(G, nreg, ci + co[n], ty[n], in[0], in[1], in[2], a + co[n], g, -real(1.0));
       for (int dd = 0; dd < 3; ++dd) o[dd] = hex_sum(G, nreg, ci, dd, T);
     }
-    const double gd[3] = {c.ALPHA_D * o[0], c.ALPHA_D * o[1], c.ALPHA_D * o[2]};
+    const real gd[3] = {c.ALPHA_D * o[0], c.ALPHA_D * o[1], c.ALPHA_D * o[2]};
     if (p.pole) {
-      const double gx = G.gx_pl[p.lp * 3 + 0], gy = G.gx_pl[p.lp * 3 + 1], gz = G.gx_pl[p.lp * 3 + 2];
-      const double prd = gd[0] * gx / R + gd[1] * gy / R + gd[2] * gz / R;
+      const real gx = G.gx_pl[p.lp * 3 + 0], gy = G.gx_pl[p.lp * 3 + 1], gz = G.gx_pl[p.lp * 3 + 2];
+      const real prd = gd[0] * gx / R + gd[1] * gy / R + gd[2] * gz / R;
       out[0][a] = gd[0] - prd * gx / R;
       out[1][a] = gd[1] - prd * gy / R;
       out[2][a] = gd[2] - prd * gz / R;
     } else {
-      const double gx = G.gx[p.ci], gy = G.gx[d.nreg + p.ci], gz = G.gx[2 * d.nreg + p.ci];
-      const double prd = (gd[0] * gx + gd[1] * gy + gd[2] * gz) / R;
+      const real gx = G.gx[p.ci], gy = G.gx[d.nreg + p.ci], gz = G.gx[2 * d.nreg + p.ci];
+      const real prd = (gd[0] * gx + gd[1] * gy + gd[2] * gz) / R;
       out[0][a] = gd[0] - prd * gx / R;
       out[1][a] = gd[1] - prd * gy / R;
       out[2][a] = gd[2] - prd * gz / R;
@@ -439,123 +439,142 @@ struct ViPath0P {
   static constexpr const char* kName = "vi_path0";
   Dims d; Consts c; Geo G;
   double dt;
-  const double* prog;
-  const double* diag;
-  const double* eth;
-  const double* gadv;   // 3 momentum tendencies + gwc at full levels
-  const double* vt[6];
-  const double* dd[3];
-  double* gT;
-  double* Mc; double* Mu; double* Ml;
-  double* getot;
-  double* frhog;
+  const real* prog;
+  const real* diag;
+  const real* eth;
+  const real* gadv;   // 3 momentum tendencies + gwc at full levels
+  const real* vt[6];
+  const real* dd[3];
+  real* gT;
+  double* Mc; double* Mu; double* Ml;   // FP64 in every form
+  real* getot;
+  real* frhog;
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
     const long NF = d.NF, ks = p.ks, a = p.a;
     const int kmin = d.kmin, kmax = d.kmax, k = p.k;
-    const double R = c.RADIUS;
-    const double* rg = prog;
-    const double* rvx = prog + NF;
-    const double* rvy = prog + 2 * NF;
-    const double* rvz = prog + 3 * NF;
-    const double* rw = prog + 4 * NF;
-    const double* pre = diag + I_PRE * NF;
+    const real R = c.RADIUS;
+    const real* rg = prog;
+    const real* rvx = prog + NF;
+    const real* rvy = prog + 2 * NF;
+    const real* rvz = prog + 3 * NF;
+    const real* rw = prog + 4 * NF;
+    const real* pre = diag + I_PRE * NF;
     const long b = a - (long)k * ks;     // column base
     auto at = [&](int kk) { return b + (long)kk * ks; };
     auto rhog_h = [&](int kk) {
       const int k2 = (kk < kmin) ? kmin : kk;
       return G.c2w0[k2] * rg[at(k2)] + G.c2w1[k2] * rg[at(k2 - 1)];
     };
-    auto eth_h = [&](int kk) {
-      const int k2 = (kk < kmin) ? kmin : kk;
-      return G.afact[k2] * eth[at(k2)] + G.bfact[k2] * eth[at(k2 - 1)];
-    };
     auto dpgw = [&](int kk) {
-      return (kk >= kmin + 1 && kk <= kmax) ? 1.0 * (pre[at(kk)] * 1.0 - pre[at(kk - 1)] * 1.0) * G.rdgzh[kk] : 0.0;
+      return (kk >= kmin + 1 && kk <= kmax) ? real(1.0) * (pre[at(kk)] * real(1.0) - pre[at(kk - 1)] * real(1.0)) * G.rdgzh[kk] : real(0.0);
     };
     auto dbuo = [&](int kk) {
       return (kk >= kmin + 1 && kk <= kmax)
-                 ? -c.GRAV * (G.c2w0[kk] * (rg[at(kk)] / 1.0 * 1.0) + G.c2w1[kk] * (rg[at(kk - 1)] / 1.0 * 1.0))
-                 : 0.0;
+                 ? -c.GRAV * (G.c2w0[kk] * (rg[at(kk)] / real(1.0) * real(1.0)) + G.c2w1[kk] * (rg[at(kk - 1)] / real(1.0) * real(1.0)))
+                 : real(0.0);
     };
     auto gzt = [&](int kk) { return c.GRAV - (dpgw(kk) - dbuo(kk)) / rhog_h(kk); };
     auto pwh = [&](int kk) { return -gzt(kk) * rw[at(kk)]; };
-    double gx, gy, gz;
+    real gx, gy, gz;
     if (p.pole) {
       gx = G.gx_pl[p.lp * 3 + 0]; gy = G.gx_pl[p.lp * 3 + 1]; gz = G.gx_pl[p.lp * 3 + 2];
     } else {
       gx = G.gx[p.ci]; gy = G.gx[d.nreg + p.ci]; gz = G.gx[2 * d.nreg + p.ci];
     }
-    const double Kh = c.GAMMA_H;
+    const real Kh = c.GAMMA_H;
     // hyperdiffusion tendency + momentum advection = large-step tendency g0
-    const double rgk = rg[a];
-    double fvx = -(vt[0][a] * Kh + 0.0) * rgk;
-    double fvy = -(vt[1][a] * Kh + 0.0) * rgk;
-    double fvz = -(vt[2][a] * Kh + 0.0) * rgk;
-    const double rgh = (k >= kmin) ? G.c2w0[k] * rgk + G.c2w1[k] * rg[a - ks] : 0.0;
-    const double fw = -(vt[3][a] * Kh + 0.0) * rgh;
-    const double fe = -(vt[4][a] + 0.0);
-    const double fr = -(vt[5][a] + 0.0);
+    const real rgk = rg[a];
+    real fvx = -(vt[0][a] * Kh + real(0.0)) * rgk;
+    real fvy = -(vt[1][a] * Kh + real(0.0)) * rgk;
+    real fvz = -(vt[2][a] * Kh + real(0.0)) * rgk;
+    const real rgh = (k >= kmin) ? G.c2w0[k] * rgk + G.c2w1[k] * rg[a - ks] : real(0.0);
+    const real fw = -(vt[3][a] * Kh + real(0.0)) * rgh;
+    const real fe = -(vt[4][a] + real(0.0));
+    const real fr = -(vt[5][a] + real(0.0));
     if (p.pole) {
-      const double prd = fvx * gx / R + fvy * gy / R + fvz * gz / R;
+      const real prd = fvx * gx / R + fvy * gy / R + fvz * gz / R;
       fvx = fvx - prd * gx / R; fvy = fvy - prd * gy / R; fvz = fvz - prd * gz / R;
     } else {
-      const double prd = (fvx * gx + fvy * gy + fvz * gz) / R;
+      const real prd = (fvx * gx + fvy * gy + fvz * gz) / R;
       fvx = fvx - prd * gx / R; fvy = fvy - prd * gy / R; fvz = fvz - prd * gz / R;
     }
-    const double gadv_w = (k > kmin && k <= kmax) ? (G.c2w0[k] * gadv[3 * NF + a] + G.c2w1[k] * gadv[3 * NF + a - ks]) : 0.0;
-    double g0[6];
-    g0[0] = 0.0 + fr;
+    const real gadv_w = (k > kmin && k <= kmax) ? (G.c2w0[k] * gadv[3 * NF + a] + G.c2w1[k] * gadv[3 * NF + a - ks]) : real(0.0);
+    real g0[6];
+    g0[0] = real(0.0) + fr;
     g0[1] = gadv[0 * NF + a] + fvx;
     g0[2] = gadv[1 * NF + a] + fvy;
     g0[3] = gadv[2 * NF + a] + fvz;
     g0[4] = gadv_w + fw;
-    g0[5] = 0.0 + fe;
+    g0[5] = real(0.0) + fe;
     getot[a] = g0[5];
     frhog[a] = g0[0];
-    double drhog = 0.0, drhoge = 0.0, pw = 0.0;
-    double dpg[3] = {0.0, 0.0, 0.0};
+    real drhog = real(0.0), drhoge = real(0.0), pw = real(0.0);
+    real dpg[3] = {real(0.0), real(0.0), real(0.0)};
     if (k >= kmin && k <= kmax) {
       const Cell cl = cell_of_pt(p);
       long off[7];
       stencil_offsets(d.g, off);
       auto nbr = [&](int m) { return p.pole ? a + m : a + off[m]; };
-      const double hd = hdiv(G, d, cl, [&](int m) { return rvx[nbr(m)] * 1.0; },
-                             [&](int m) { return rvy[nbr(m)] * 1.0; }, [&](int m) { return rvz[nbr(m)] * 1.0; });
-      const double vmh_hi = (k + 1 <= kmax) ? 0.0 * 1.0 + 1.0 * rw[a + ks] * 1.0 : 0.0;
-      const double vmh_lo = (k >= kmin + 1) ? 0.0 * 1.0 + 1.0 * rw[a] * 1.0 : 0.0;
+      const real hd = hdiv(G, d, cl, [&](int m) { return rvx[nbr(m)] * real(1.0); },
+                             [&](int m) { return rvy[nbr(m)] * real(1.0); }, [&](int m) { return rvz[nbr(m)] * real(1.0); });
+      const real vmh_hi = (k + 1 <= kmax) ? real(0.0) * real(1.0) + real(1.0) * rw[a + ks] * real(1.0) : real(0.0);
+      const real vmh_lo = (k >= kmin + 1) ? real(0.0) * real(1.0) + real(1.0) * rw[a] * real(1.0) : real(0.0);
       drhog = -hd - (vmh_hi - vmh_lo) * G.rdgz[k];
-      const double he = hdiv(G, d, cl, [&](int m) { return rvx[nbr(m)] * eth[nbr(m)] * 1.0; },
-                             [&](int m) { return rvy[nbr(m)] * eth[nbr(m)] * 1.0; },
-                             [&](int m) { return rvz[nbr(m)] * eth[nbr(m)] * 1.0; });
-      const double ws_hi = (k + 1 <= kmax)
-          ? 0.0 * 1.0 + 1.0 * (rw[a + ks] * (G.afact[k + 1] * eth[a + ks] + G.bfact[k + 1] * eth[a])) * 1.0 : 0.0;
-      const double ws_lo = (k >= kmin + 1)
-          ? 0.0 * 1.0 + 1.0 * (rw[a] * (G.afact[k] * eth[a] + G.bfact[k] * eth[a - ks])) * 1.0 : 0.0;
+      const real he = hdiv(G, d, cl, [&](int m) { return rvx[nbr(m)] * eth[nbr(m)] * real(1.0); },
+                             [&](int m) { return rvy[nbr(m)] * eth[nbr(m)] * real(1.0); },
+                             [&](int m) { return rvz[nbr(m)] * eth[nbr(m)] * real(1.0); });
+      const real ws_hi = (k + 1 <= kmax)
+          ? real(0.0) * real(1.0) + real(1.0) * (rw[a + ks] * (G.afact[k + 1] * eth[a + ks] + G.bfact[k + 1] * eth[a])) * real(1.0) : real(0.0);
+      const real ws_lo = (k >= kmin + 1)
+          ? real(0.0) * real(1.0) + real(1.0) * (rw[a] * (G.afact[k] * eth[a] + G.bfact[k] * eth[a - ks])) * real(1.0) : real(0.0);
       drhoge = -he - (ws_hi - ws_lo) * G.rdgz[k];
       presgrad_h(G, d, cl, pre, k, R, dpg);
       pw = diag[I_VX * NF + a] * dpg[0] + diag[I_VY * NF + a] * dpg[1] + diag[I_VZ * NF + a] * dpg[2]
          + G.w2c0[k] * pwh(k + 1) + G.w2c1[k] * pwh(k);
     }
     gT[0 * NF + a] = g0[0] + drhog;
-    gT[1 * NF + a] = g0[1] - dpg[0] + dd[0][a] + 0.0;
-    gT[2 * NF + a] = g0[2] - dpg[1] + dd[1][a] + 0.0;
-    gT[3 * NF + a] = g0[3] - dpg[2] + dd[2][a] + 0.0;
-    gT[4 * NF + a] = g0[4] + 0.0 * c.ALPHA - dpgw(k) + dbuo(k);
+    gT[1 * NF + a] = g0[1] - dpg[0] + dd[0][a] + real(0.0);
+    gT[2 * NF + a] = g0[2] - dpg[1] + dd[1][a] + real(0.0);
+    gT[3 * NF + a] = g0[3] - dpg[2] + dd[2][a] + real(0.0);
+    gT[4 * NF + a] = g0[4] + real(0.0) * c.ALPHA - dpgw(k) + dbuo(k);
     gT[5 * NF + a] = g0[5] + drhoge + pw;
     if (k >= kmin + 1 && k <= kmax) {
-      const double GCVovR = c.GRAV * c.CVdry / c.Rdry;
-      const double ACVovRt2 = c.ALPHA * c.CVdry / c.Rdry / (dt * dt);
-      const double Ca = (1.0 * G.rdgz[k] + 1.0 * G.rdgz[k - 1]) * 1.0;
-      const double Cb = G.dfact[k] - G.cfact[k - 1];
-      const double Cu1 = 1.0 * G.rdgz[k] * 1.0;
+      // tridiagonal coefficients, FP64 in every form: the column quantities of
+      // levels k-1..k+1 again, from the stored fields, in double
+      const VProf<double>& V = G.v64;
+      const ConstsT<double>& C = c.d;
+      auto rhog_h64 = [&](int kk) {
+        const int k2 = (kk < kmin) ? kmin : kk;
+        return V.c2w0[k2] * (double)rg[at(k2)] + V.c2w1[k2] * (double)rg[at(k2 - 1)];
+      };
+      auto eth_h64 = [&](int kk) {
+        const int k2 = (kk < kmin) ? kmin : kk;
+        return V.afact[k2] * (double)eth[at(k2)] + V.bfact[k2] * (double)eth[at(k2 - 1)];
+      };
+      auto dpgw64 = [&](int kk) {
+        return (kk >= kmin + 1 && kk <= kmax)
+                   ? 1.0 * ((double)pre[at(kk)] * 1.0 - (double)pre[at(kk - 1)] * 1.0) * V.rdgzh[kk] : 0.0;
+      };
+      auto dbuo64 = [&](int kk) {
+        return (kk >= kmin + 1 && kk <= kmax)
+                   ? -C.GRAV * (V.c2w0[kk] * ((double)rg[at(kk)] / 1.0 * 1.0) +
+                                V.c2w1[kk] * ((double)rg[at(kk - 1)] / 1.0 * 1.0))
+                   : 0.0;
+      };
+      auto gzt64 = [&](int kk) { return C.GRAV - (dpgw64(kk) - dbuo64(kk)) / rhog_h64(kk); };
+      const double GCVovR = C.GRAV * C.CVdry / C.Rdry;
+      const double ACVovRt2 = C.ALPHA * C.CVdry / C.Rdry / (dt * dt);
+      const double Ca = (1.0 * V.rdgz[k] + 1.0 * V.rdgz[k - 1]) * 1.0;
+      const double Cb = V.dfact[k] - V.cfact[k - 1];
+      const double Cu1 = 1.0 * V.rdgz[k] * 1.0;
       const double Cu2 = 1.0 * 1.0 * GCVovR;
-      const double Cl1 = 1.0 * G.rdgz[k] * 1.0;
+      const double Cl1 = 1.0 * V.rdgz[k] * 1.0;
       const double Cl2 = 1.0 * 1.0 * GCVovR;
-      Mc[a] = ACVovRt2 / 1.0 + G.rdgzh[k] * (Ca * eth_h(k) - Cb * (gzt(k) + GCVovR));
-      Mu[a] = -G.rdgzh[k] * (Cu1 * eth_h(k + 1) + G.cfact[k] * (gzt(k + 1) + Cu2));
-      Ml[a] = -G.rdgzh[k] * (Cl1 * eth_h(k - 1) - G.dfact[k - 1] * (gzt(k - 1) + Cl2));
+      Mc[a] = ACVovRt2 / 1.0 + V.rdgzh[k] * (Ca * eth_h64(k) - Cb * (gzt64(k) + GCVovR));
+      Mu[a] = -V.rdgzh[k] * (Cu1 * eth_h64(k + 1) + V.cfact[k] * (gzt64(k + 1) + Cu2));
+      Ml[a] = -V.rdgzh[k] * (Cl1 * eth_h64(k - 1) - V.dfact[k - 1] * (gzt64(k - 1) + Cl2));
     }
   }
 };
@@ -564,17 +583,17 @@ struct ViPath0P {
 struct EthTotPMP {
   static constexpr const char* kName = "ethtot_mean";
   Dims d; Consts c; Geo G;
-  const double* prog;
-  const double* eth;
-  double* ethtot;
-  double* pm;
+  const real* prog;
+  const real* eth;
+  real* ethtot;
+  real* pm;
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p)) return;
     const long NF = d.NF, a = p.a, ks = p.ks;
     const long b = a - (long)p.k * ks;
     auto get = [&](int kk, int q) { return prog[q * NF + b + (long)kk * ks]; };
-    const double kin0 = rhogkin_k(G, d.kmin, d.kmax, p.k, get);
+    const real kin0 = rhogkin_k(G, d.kmin, d.kmax, p.k, get);
     ethtot[a] = eth[a] + kin0 / prog[a] + G.phi[p.k];
     for (int q = 0; q < 5; ++q) pm[q * NF + a] = prog[q * NF + a];
   }
@@ -587,27 +606,27 @@ struct EthTotPMP {
 struct Path1P {
   static constexpr const char* kName = "vi_path1";
   Dims d; Consts c; Geo G;
-  double dt, rweight;
-  const double* prog;
-  const double* ps[6];
-  const double* gT;
-  const double* dd[3];
-  double* dv[3];
-  double* pm;
-  NH_HD void level(const Pt& p, int k, long a, double out[3]) const {
+  real dt, rweight;
+  const real* prog;
+  const real* ps[6];
+  const real* gT;
+  const real* dd[3];
+  real* dv[3];
+  real* pm;
+  NH_HD void level(const Pt& p, int k, long a, real out[3]) const {
     const long NF = d.NF, ks = p.ks;
     const int kmin = d.kmin;
-    const double RovCV = c.Rdry / c.CVdry;
-    const double R = c.RADIUS;
-    const double* pse = ps[I_RHOGE];
+    const real RovCV = c.Rdry / c.CVdry;
+    const real R = c.RADIUS;
+    const real* pse = ps[I_RHOGE];
     const int kk = (k == kmin) ? kmin + 1 : k;
     const long ak = a + (long)(kk - k) * ks;
-    double gr[3];
+    real gr[3];
     if (p.pole) {
       for (int dd2 = 0; dd2 < 3; ++dd2) {
-        double acc = 0.0;
+        real acc = real(0.0);
         for (int v = 0; v < 6; ++v) {
-          const double pv = G.cgrad_pl[(v * 2 + p.lp) * 3 + dd2] * ((pse[ak + v] * RovCV) * 1.0 * 1.0);
+          const real pv = G.cgrad_pl[(v * 2 + p.lp) * 3 + dd2] * ((pse[ak + v] * RovCV) * real(1.0) * real(1.0));
           acc = (v == 0) ? pv : acc + pv;
         }
         gr[dd2] = acc;
@@ -616,24 +635,24 @@ struct Path1P {
       long off[7];
       stencil_offsets(d.g, off);
       for (int dd2 = 0; dd2 < 3; ++dd2) {
-        double acc = rc(G.cgrad, d.nreg, dd2 * 7, p.ci) * ((pse[ak] * RovCV) * 1.0);
-        for (int m = 1; m < 7; ++m) acc = acc + rc(G.cgrad, d.nreg, dd2 * 7 + m, p.ci) * ((pse[ak + off[m]] * RovCV) * 1.0);
+        real acc = rc(G.cgrad, d.nreg, dd2 * 7, p.ci) * ((pse[ak] * RovCV) * real(1.0));
+        for (int m = 1; m < 7; ++m) acc = acc + rc(G.cgrad, d.nreg, dd2 * 7 + m, p.ci) * ((pse[ak + off[m]] * RovCV) * real(1.0));
         gr[dd2] = acc;
       }
     }
-    for (int q = 0; q < 3; ++q) gr[q] = gr[q] + 0.0;
-    double Pg[3];
+    for (int q = 0; q < 3; ++q) gr[q] = gr[q] + real(0.0);
+    real Pg[3];
     if (p.pole) {
-      const double gx = G.gx_pl[p.lp * 3 + 0], gy = G.gx_pl[p.lp * 3 + 1], gz = G.gx_pl[p.lp * 3 + 2];
-      const double prd = gr[0] * gx / R + gr[1] * gy / R + gr[2] * gz / R;
+      const real gx = G.gx_pl[p.lp * 3 + 0], gy = G.gx_pl[p.lp * 3 + 1], gz = G.gx_pl[p.lp * 3 + 2];
+      const real prd = gr[0] * gx / R + gr[1] * gy / R + gr[2] * gz / R;
       Pg[0] = gr[0] - prd * gx / R; Pg[1] = gr[1] - prd * gy / R; Pg[2] = gr[2] - prd * gz / R;
     } else {
-      const double gx = G.gx[p.ci], gy = G.gx[d.nreg + p.ci], gz = G.gx[2 * d.nreg + p.ci];
-      const double prd = (gr[0] * gx + gr[1] * gy + gr[2] * gz) / R;
+      const real gx = G.gx[p.ci], gy = G.gx[d.nreg + p.ci], gz = G.gx[2 * d.nreg + p.ci];
+      const real prd = (gr[0] * gx + gr[1] * gy + gr[2] * gz) / R;
       Pg[0] = gr[0] - prd * gx / R; Pg[1] = gr[1] - prd * gy / R; Pg[2] = gr[2] - prd * gz / R;
     }
     for (int q = 0; q < 3; ++q) {
-      const double drv = gT[(1 + q) * NF + a] - Pg[q] + dd[q][a] + 0.0;
+      const real drv = gT[(1 + q) * NF + a] - Pg[q] + dd[q][a] + real(0.0);
       out[q] = ps[1 + q][a] + drv * dt;
     }
   }
@@ -642,17 +661,17 @@ struct Path1P {
     if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
     const long NF = d.NF, a = p.a, ks = p.ks;
     const int kmin = d.kmin, kmax = d.kmax, k = p.k;
-    double v[3];
+    real v[3];
     if (k >= kmin && k <= kmax) {
       level(p, k, a, v);
     } else {
       // free-slip top / rigid bottom, scaled by the large-step density
       const bool top = (k == kmax + 1);
       const long an = top ? a - ks : a + ks;
-      double vn[3];
+      real vn[3];
       level(p, top ? kmax : kmin, an, vn);
-      const double* rg = prog;
-      for (int q = 0; q < 3; ++q) v[q] = top ? 1.0 * (vn[q] / rg[an]) * rg[a] : -1.0 * (vn[q] / rg[an]) * rg[a];
+      const real* rg = prog;
+      for (int q = 0; q < 3; ++q) v[q] = top ? real(1.0) * (vn[q] / rg[an]) * rg[a] : -real(1.0) * (vn[q] / rg[an]) * rg[a];
     }
     for (int q = 0; q < 3; ++q) {
       dv[q][a] = v[q];
@@ -669,12 +688,12 @@ struct Path1P {
 struct ViMainH {
   static constexpr const char* kName = "vi_main_h";
   Dims d; Consts c; Geo G;
-  const double* prog;
-  const double* dv[3];
-  const double* eth;
-  const double* ethtot;
-  const double* gT;
-  double* hv[3];
+  const real* prog;
+  const real* dv[3];
+  const real* eth;
+  const real* ethtot;
+  const real* gT;
+  real* hv[3];
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
@@ -685,17 +704,17 @@ struct ViMainH {
     long off[7];
     stencil_offsets(d.g, off);
     auto nbr = [&](int m) { return p.pole ? a + m : a + off[m]; };
-    const double divs1 = hdiv(G, d, cl, [&](int m) { return dv[0][nbr(m)] * 1.0; },
-                              [&](int m) { return dv[1][nbr(m)] * 1.0; }, [&](int m) { return dv[2][nbr(m)] * 1.0; });
-    const double he = hdiv(G, d, cl, [&](int m) { return dv[0][nbr(m)] * eth[nbr(m)] * 1.0; },
-                           [&](int m) { return dv[1][nbr(m)] * eth[nbr(m)] * 1.0; },
-                           [&](int m) { return dv[2][nbr(m)] * eth[nbr(m)] * 1.0; });
-    const double drhoge = -he - (0.0 - 0.0) * G.rdgz[k];
-    const double grhoge1 = gT[5 * NF + a] + drhoge;
-    const double hd = hdiv(G, d, cl,
-                           [&](int m) { return (prog[1 * NF + nbr(m)] + dv[0][nbr(m)]) * ethtot[nbr(m)] * 1.0; },
-                           [&](int m) { return (prog[2 * NF + nbr(m)] + dv[1][nbr(m)]) * ethtot[nbr(m)] * 1.0; },
-                           [&](int m) { return (prog[3 * NF + nbr(m)] + dv[2][nbr(m)]) * ethtot[nbr(m)] * 1.0; });
+    const real divs1 = hdiv(G, d, cl, [&](int m) { return dv[0][nbr(m)] * real(1.0); },
+                              [&](int m) { return dv[1][nbr(m)] * real(1.0); }, [&](int m) { return dv[2][nbr(m)] * real(1.0); });
+    const real he = hdiv(G, d, cl, [&](int m) { return dv[0][nbr(m)] * eth[nbr(m)] * real(1.0); },
+                           [&](int m) { return dv[1][nbr(m)] * eth[nbr(m)] * real(1.0); },
+                           [&](int m) { return dv[2][nbr(m)] * eth[nbr(m)] * real(1.0); });
+    const real drhoge = -he - (real(0.0) - real(0.0)) * G.rdgz[k];
+    const real grhoge1 = gT[5 * NF + a] + drhoge;
+    const real hd = hdiv(G, d, cl,
+                           [&](int m) { return (prog[1 * NF + nbr(m)] + dv[0][nbr(m)]) * ethtot[nbr(m)] * real(1.0); },
+                           [&](int m) { return (prog[2 * NF + nbr(m)] + dv[1][nbr(m)]) * ethtot[nbr(m)] * real(1.0); },
+                           [&](int m) { return (prog[3 * NF + nbr(m)] + dv[2][nbr(m)]) * ethtot[nbr(m)] * real(1.0); });
     hv[0][a] = divs1;
     hv[1][a] = grhoge1 * c.Rdry / c.CVdry;
     hv[2][a] = hd;
@@ -704,53 +723,56 @@ struct ViMainH {
 
 // vertical part, one thread per compute column: source vector of the
 // tridiagonal rhogw system and the Thomas sweeps, streamed in k with the
-// previous level kept in registers (kernels/virhowsolver.py:42-120).  The
-// forward sweep keeps u and gamma in the output fields we[1] and we[2]; on
-// return we[1] holds the new rhogw split on every level.
+// previous level kept in registers (kernels/virhowsolver.py:42-120).  FP64 in
+// every form: the field values are widened on load.  The forward sweep keeps
+// u and gamma in the FP64 scratch U and Gm; the backward sweep writes the new
+// rhogw split to out on every level (U may be out itself when real is double).
 struct ViMainV {
   static constexpr const char* kName = "vi_main_v";
   Dims d; Consts c; Geo G;
   double dt;
-  const double* ps[6];
-  const double* gT;
+  const real* ps[6];
+  const real* gT;
   const double* Mc; const double* Mu; const double* Ml;
-  const double* hv[3];
-  double* we[3];
+  const real* hv[3];
+  double* U;
+  double* Gm;
+  real* out;
   NH_HD void operator()(long t) const {
     Cell cl = cell_of(d, t);
     if (cl.pole ? cl.v != 0 : !cl.inner) return;
     const long NF = d.NF, ks = cl.ks, b = cl.base;
     const int kmin = d.kmin, kmax = d.kmax;
-    const double RovCV = c.Rdry / c.CVdry;
-    const double CVovRt2 = c.CVdry / c.Rdry / (dt * dt);
-    const double* __restrict__ pg = ps[I_RHOG];
-    const double* __restrict__ pw = ps[I_RHOGW];
-    const double* __restrict__ pe = ps[I_RHOGE];
-    const double* __restrict__ gT0 = gT;
-    const double* __restrict__ gT4 = gT + 4 * NF;
-    const double* __restrict__ hv0 = hv[0];
-    const double* __restrict__ hv1 = hv[1];
+    const VProf<double>& V = G.v64;
+    const ConstsT<double>& C = c.d;
+    const double RovCV = C.Rdry / C.CVdry;
+    const double CVovRt2 = C.CVdry / C.Rdry / (dt * dt);
+    const real* __restrict__ pg = ps[I_RHOG];
+    const real* __restrict__ pw = ps[I_RHOGW];
+    const real* __restrict__ pe = ps[I_RHOGE];
+    const real* __restrict__ gT0 = gT;
+    const real* __restrict__ gT4 = gT + 4 * NF;
+    const real* __restrict__ hv0 = hv[0];
+    const real* __restrict__ hv1 = hv[1];
     const double* __restrict__ mc = Mc;
     const double* __restrict__ mu = Mu;
     const double* __restrict__ ml = Ml;
-    double* __restrict__ U = we[1];
-    double* __restrict__ Gm = we[2];
     const int N = kmax - kmin;
     // level k-1 values, starting at kmin
     long am = b + (long)kmin * ks;
     double pe_m = pe[am], hv1_m = hv1[am], pg_m = pg[am];
-    double gr_m = gT0[am] + (-hv0[am] - (0.0 - 0.0) * G.rdgz[kmin]);
+    double gr_m = (double)gT0[am] + (-(double)hv0[am] - (0.0 - 0.0) * V.rdgz[kmin]);
     double beta = 0.0, u_prev = 0.0, mu_m = 0.0;
     for (int n = 0; n < N; ++n) {
       const int k = kmin + 1 + n;
       const long a = am + ks;
       const double pe_k = pe[a], hv1_k = hv1[a], pg_k = pg[a];
-      const double gr_k = gT0[a] + (-hv0[a] - (0.0 - 0.0) * G.rdgz[k]);
-      const double sw = gT4[a] + 0.0 * c.ALPHA;
-      const double A = (pw[a] * c.ALPHA + dt * sw) * 1.0;
+      const double gr_k = (double)gT0[a] + (-(double)hv0[a] - (0.0 - 0.0) * V.rdgz[k]);
+      const double sw = (double)gT4[a] + 0.0 * C.ALPHA;
+      const double A = ((double)pw[a] * C.ALPHA + dt * sw) * 1.0;
       const double B = (pe_k * RovCV + dt * hv1_k) * 1.0 - (pe_m * RovCV + dt * hv1_m) * 1.0;
-      const double Cc = (pg_k + dt * gr_k) * 1.0 * G.afact[k] + (pg_m + dt * gr_m) * 1.0 * G.bfact[k];
-      double S = (A - B * dt * G.rdgzh[k] - Cc * dt * c.GRAV) * CVovRt2;
+      const double Cc = (pg_k + dt * gr_k) * 1.0 * V.afact[k] + (pg_m + dt * gr_m) * 1.0 * V.bfact[k];
+      double S = (A - B * dt * V.rdgzh[k] - Cc * dt * C.GRAV) * CVovRt2;
       const double mc_k = mc[a], ml_k = ml[a], mu_k = mu[a];
       if (n == 0) S = S - ml_k * (0.0 * 1.0);
       if (n == N - 1) S = S - mu_k * (-0.0 * 1.0);
@@ -773,17 +795,17 @@ struct ViMainV {
     // backward sweep
     long a = b + (long)kmax * ks;
     double xv = u_prev;
-    U[a] = xv * 1.0;
+    out[a] = (real)(xv * 1.0);
     double gm_next = Gm[a];
     for (int n = N - 2; n >= 0; --n) {
       a -= ks;
       xv = U[a] - gm_next * xv;
       gm_next = Gm[a];
-      U[a] = xv * 1.0;
+      out[a] = (real)(xv * 1.0);
     }
-    U[b + (long)kmin * ks] = 0.0 * 1.0;
-    U[b + (long)(kmax + 1) * ks] = -0.0 * 1.0;
-    U[b + (long)(kmin - 1) * ks] = 0.0;
+    out[b + (long)kmin * ks] = (real)(0.0 * 1.0);
+    out[b + (long)(kmax + 1) * ks] = (real)(-0.0 * 1.0);
+    out[b + (long)(kmin - 1) * ks] = real(0.0);
   }
 };
 
@@ -792,16 +814,16 @@ struct ViMainV {
 struct ViMainPost {
   static constexpr const char* kName = "vi_main_post";
   Dims d; Consts c; Geo G;
-  double dt, rweight;
-  const double* prog;
-  const double* ps[6];
-  const double* dv[3];
-  const double* ethtot;
-  const double* gT;
-  const double* getot;
-  const double* hv[3];
-  double* we[3];       // we[1] (rhogw split) is read
-  double* pm;
+  real dt, rweight;
+  const real* prog;
+  const real* ps[6];
+  const real* dv[3];
+  const real* ethtot;
+  const real* gT;
+  const real* getot;
+  const real* hv[3];
+  real* we[3];       // we[1] (rhogw split) is read
+  real* pm;
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
@@ -809,14 +831,14 @@ struct ViMainPost {
     const int kmin = d.kmin, kmax = d.kmax, k = p.k;
     const long b = a - (long)k * ks;
     auto at = [&](int kk) { return b + (long)kk * ks; };
-    const double* pg = ps[I_RHOG];
-    const double* pe = ps[I_RHOGE];
-    const double* rw1 = we[1];
+    const real* pg = ps[I_RHOG];
+    const real* pe = ps[I_RHOGE];
+    const real* rw1 = we[1];
     auto rs1 = [&](int kk) {
-      double drhog2 = 0.0;
+      real drhog2 = real(0.0);
       if (kk >= kmin && kk <= kmax) {
-        const double hi = (kk + 1 <= kmax) ? 0.0 * 1.0 + 1.0 * rw1[at(kk + 1)] * 1.0 : 0.0;
-        const double lo = (kk >= kmin + 1) ? 0.0 * 1.0 + 1.0 * rw1[at(kk)] * 1.0 : 0.0;
+        const real hi = (kk + 1 <= kmax) ? real(0.0) * real(1.0) + real(1.0) * rw1[at(kk + 1)] * real(1.0) : real(0.0);
+        const real lo = (kk >= kmin + 1) ? real(0.0) * real(1.0) + real(1.0) * rw1[at(kk)] * real(1.0) : real(0.0);
         drhog2 = -hv[0][at(kk)] - (hi - lo) * G.rdgz[kk];
       }
       return pg[at(kk)] + (gT[0 * NF + at(kk)] + drhog2) * dt;
@@ -827,23 +849,23 @@ struct ViMainPost {
       if (q == 4) return prog[4 * NF + at(kk)] + rw1[at(kk)];
       return prog[q * NF + at(kk)] + dv[q - 1][at(kk)];
     };
-    const double kin10 = rhogkin_k(G, kmin, kmax, k, g10);
-    const double kin11 = rhogkin_k(G, kmin, kmax, k, g11);
-    double dtot = 0.0;
+    const real kin10 = rhogkin_k(G, kmin, kmax, k, g10);
+    const real kin11 = rhogkin_k(G, kmin, kmax, k, g11);
+    real dtot = real(0.0);
     if (k >= kmin && k <= kmax) {
-      double hi = 0.0, lo = 0.0;
+      real hi = real(0.0), lo = real(0.0);
       if (k + 1 <= kmax)
-        hi = 0.0 * 1.0 + 1.0 * ((prog[4 * NF + a + ks] + rw1[a + ks]) *
-                                (G.afact[k + 1] * ethtot[a + ks] + G.bfact[k + 1] * ethtot[a])) * 1.0;
+        hi = real(0.0) * real(1.0) + real(1.0) * ((prog[4 * NF + a + ks] + rw1[a + ks]) *
+                                (G.afact[k + 1] * ethtot[a + ks] + G.bfact[k + 1] * ethtot[a])) * real(1.0);
       if (k >= kmin + 1)
-        lo = 0.0 * 1.0 + 1.0 * ((prog[4 * NF + a] + rw1[a]) *
-                                (G.afact[k] * ethtot[a] + G.bfact[k] * ethtot[a - ks])) * 1.0;
+        lo = real(0.0) * real(1.0) + real(1.0) * ((prog[4 * NF + a] + rw1[a]) *
+                                (G.afact[k] * ethtot[a] + G.bfact[k] * ethtot[a - ks])) * real(1.0);
       dtot = -hv[2][a] - (hi - lo) * G.rdgz[k];
     }
-    const double r1 = rs1(k);
+    const real r1 = rs1(k);
     const int ke = (k < kmin) ? kmin : ((k > kmax) ? kmax : k);
-    const double pe_k = pe[at(ke)];
-    const double re1 = pe_k + (getot[a] + dtot) * dt + (kin10 - kin11) + (pg[a] - r1) * G.phi[k];
+    const real pe_k = pe[at(ke)];
+    const real re1 = pe_k + (getot[a] + dtot) * dt + (kin10 - kin11) + (pg[a] - r1) * G.phi[k];
     we[0][a] = r1;
     we[2][a] = re1;
     pm[0 * NF + a] = pm[0 * NF + a] + r1 * rweight;
@@ -855,28 +877,28 @@ struct ViMainPost {
 struct ProgUpdateP {
   static constexpr const char* kName = "prog_update";
   Dims d; Consts c; Geo G;
-  double* prog;
-  const double* ps[6];
+  real* prog;
+  const real* ps[6];
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p)) return;
     const long NF = d.NF, a = p.a;
-    const double R = c.RADIUS;
-    double v[6];
+    const real R = c.RADIUS;
+    real v[6];
     for (int q = 0; q < 6; ++q) v[q] = prog[q * NF + a] + ps[q][a];
     if (p.pole) {
-      const double gx = G.gx_pl[(p.v * 2 + p.lp) * 3 + 0];
-      const double gy = G.gx_pl[(p.v * 2 + p.lp) * 3 + 1];
-      const double gz = G.gx_pl[(p.v * 2 + p.lp) * 3 + 2];
-      const double prd = v[1] * gx / R + v[2] * gy / R + v[3] * gz / R;
+      const real gx = G.gx_pl[(p.v * 2 + p.lp) * 3 + 0];
+      const real gy = G.gx_pl[(p.v * 2 + p.lp) * 3 + 1];
+      const real gz = G.gx_pl[(p.v * 2 + p.lp) * 3 + 2];
+      const real prd = v[1] * gx / R + v[2] * gy / R + v[3] * gz / R;
       v[1] = v[1] - prd * gx / R; v[2] = v[2] - prd * gy / R; v[3] = v[3] - prd * gz / R;
     } else if (p.inner) {
-      const double gx = G.gx[p.ci], gy = G.gx[d.nreg + p.ci], gz = G.gx[2 * d.nreg + p.ci];
-      const double prd = (v[1] * gx + v[2] * gy + v[3] * gz) / R;
+      const real gx = G.gx[p.ci], gy = G.gx[d.nreg + p.ci], gz = G.gx[2 * d.nreg + p.ci];
+      const real prd = (v[1] * gx + v[2] * gy + v[3] * gz) / R;
       v[1] = v[1] - prd * gx / R; v[2] = v[2] - prd * gy / R; v[3] = v[3] - prd * gz / R;
     }
     for (int q = 0; q < 6; ++q) prog[q * NF + a] = v[q];
   }
 };
 
-}  // namespace nh
+NH_END

@@ -14,14 +14,14 @@
 
 #include "nh_point.h"
 
-namespace nh {
+NH_BEGIN
 
 struct TileGeom {
   int W, TH;          // tile width (interior points of a row) and height
   int ntp;            // tiles per region level
   int ntiles;         // regional tiles (lall * kall * ntp)
   int nblocks;        // + 2 * kall pole blocks
-  int lds;            // LDS doubles per block
+  int lds;            // LDS elements (real) per block
 };
 
 inline TileGeom tile_geom(const Dims& d, int per_corner) {
@@ -57,15 +57,16 @@ NH_HD inline TileAt tile_at(const Dims& d, const TileGeom& tg, int blk) {
 // (kernels/oprt3ddivdamp.py:50-173)
 // ----------------------------------------------------------------------------
 struct DivDamp1T {
+  using LdsT = real;
   static constexpr const char* kName = "divdamp_3d";
   Dims d; Consts c; Geo G; TileGeom tg;
-  const double *rvx, *rvy, *rvz, *rw;
-  double* out[3];
-  NH_HD double rwv(long aa, int kk) const {
-    return (kk >= d.kmin + 1 && kk <= d.kmax) ? (0.0 * 1.0 + rw[aa] * 1.0) : 0.0;
+  const real *rvx, *rvy, *rvz, *rw;
+  real* out[3];
+  NH_HD real rwv(long aa, int kk) const {
+    return (kk >= d.kmin + 1 && kk <= d.kmax) ? (real(0.0) * real(1.0) + rw[aa] * real(1.0)) : real(0.0);
   }
   NH_HD bool active(int k) const { return k >= d.kmin && k <= d.kmax; }
-  NH_HD void phase1(int blk, int tid, double* lds) const {
+  NH_HD void phase1(int blk, int tid, real* lds) const {
     if (blk >= tg.ntiles) return;
     const TileAt t = tile_at(d, tg, blk);
     if (!active(t.k)) return;
@@ -80,14 +81,14 @@ struct DivDamp1T {
       for (int T = 0; T < 2; ++T) {
         const long o1 = (T == 0) ? 1 : 1 + g;
         const long o2 = (T == 0) ? 1 + g : g;
-        const double s = tri_div(G, d.nreg, cj, T, rvx, rvy, rvz, a, g, 1.0);
-        const double up = (rwv(a + ks, t.k + 1) + rwv(a + o1 + ks, t.k + 1) + rwv(a + o2 + ks, t.k + 1));
-        const double dn = (rwv(a, t.k) + rwv(a + o1, t.k) + rwv(a + o2, t.k));
-        lds[(T * (t.rows + 1) + cj_) * W1 + ci_] = s + (up - dn) / 3.0 * G.rdgz[t.k];
+        const real s = tri_div(G, d.nreg, cj, T, rvx, rvy, rvz, a, g, real(1.0));
+        const real up = (rwv(a + ks, t.k + 1) + rwv(a + o1 + ks, t.k + 1) + rwv(a + o2 + ks, t.k + 1));
+        const real dn = (rwv(a, t.k) + rwv(a + o1, t.k) + rwv(a + o2, t.k));
+        lds[(T * (t.rows + 1) + cj_) * W1 + ci_] = s + (up - dn) / real(3.0) * G.rdgz[t.k];
       }
     }
   }
-  NH_HD void phase2(int blk, int tid, const double* lds) const {
+  NH_HD void phase2(int blk, int tid, const real* lds) const {
     const int g = d.g;
     if (blk >= tg.ntiles) {
       pole(blk - tg.ntiles, tid);
@@ -101,14 +102,14 @@ struct DivDamp1T {
     const long ci = ((long)t.l * g + j) * g + i;
     const long a = ((long)(t.l * d.kall + t.k) * g + j) * g + i;
     if (!active(t.k)) {
-      for (int q = 0; q < 3; ++q) out[q][a] = 0.0;
+      for (int q = 0; q < 3; ++q) out[q][a] = real(0.0);
       return;
     }
     const int R = t.rows + 1;
     auto L = [&](int T, int ii, int jj) { return lds[(T * R + (jj - t.j0 + 1)) * W1 + ii]; };
-    double T6[6] = {L(0, i, j), L(1, i, j), L(0, i - 1, j), L(1, i - 1, j - 1), L(0, i - 1, j - 1), L(1, i, j - 1)};
+    real T6[6] = {L(0, i, j), L(1, i, j), L(0, i - 1, j), L(1, i - 1, j - 1), L(0, i - 1, j - 1), L(1, i, j - 1)};
     if (i == 1 && j == 1) {
-      const double m0 = G.sgp[t.l] ? 0.0 : 1.0, m1 = G.sgp[t.l] ? 1.0 : 0.0;
+      const real m0 = G.sgp[t.l] ? real(0.0) : real(1.0), m1 = G.sgp[t.l] ? real(1.0) : real(0.0);
       T6[4] = T6[4] * m0 + T6[5] * m1;
     }
     for (int dd = 0; dd < 3; ++dd) out[dd][a] = hex_sum(G, d.nreg, ci, dd, T6);
@@ -123,12 +124,13 @@ struct DivDamp1T {
 // second operator on -(first result), coefficient, projection
 // (kernels/oprtdivdamp.py:54-173, kernels/divdamppostcomm.py)
 struct DivDamp2T {
+  using LdsT = real;
   static constexpr const char* kName = "divdamp_post";
   Dims d; Consts c; Geo G; TileGeom tg;
-  const double* in[3];
-  double* out[3];
+  const real* in[3];
+  real* out[3];
   NH_HD bool active(int k) const { return k >= d.kmin && k <= d.kmax; }
-  NH_HD void phase1(int blk, int tid, double* lds) const {
+  NH_HD void phase1(int blk, int tid, real* lds) const {
     if (blk >= tg.ntiles) return;
     const TileAt t = tile_at(d, tg, blk);
     if (!active(t.k)) return;
@@ -140,10 +142,10 @@ struct DivDamp2T {
       const long cj = ((long)t.l * g + j) * g + i;
       const long a = ((long)(t.l * d.kall + t.k) * g + j) * g + i;
       for (int T = 0; T < 2; ++T)
-        lds[(T * (t.rows + 1) + cj_) * W1 + ci_] = tri_div(G, d.nreg, cj, T, in[0], in[1], in[2], a, g, -1.0);
+        lds[(T * (t.rows + 1) + cj_) * W1 + ci_] = tri_div(G, d.nreg, cj, T, in[0], in[1], in[2], a, g, -real(1.0));
     }
   }
-  NH_HD void phase2(int blk, int tid, const double* lds) const {
+  NH_HD void phase2(int blk, int tid, const real* lds) const {
     const int g = d.g;
     if (blk >= tg.ntiles) {
       if (tid == 0) {
@@ -159,18 +161,18 @@ struct DivDamp2T {
     const int i = ti + 1, j = t.j0 + tj;
     const long ci = ((long)t.l * g + j) * g + i;
     const long a = ((long)(t.l * d.kall + t.k) * g + j) * g + i;
-    double o[3] = {0.0, 0.0, 0.0};
+    real o[3] = {real(0.0), real(0.0), real(0.0)};
     if (active(t.k)) {
       const int R = t.rows + 1;
       auto L = [&](int T, int ii, int jj) { return lds[(T * R + (jj - t.j0 + 1)) * W1 + ii]; };
-      const double T6[6] = {L(0, i, j), L(1, i, j), L(0, i - 1, j), L(1, i - 1, j - 1), L(0, i - 1, j - 1),
+      const real T6[6] = {L(0, i, j), L(1, i, j), L(0, i - 1, j), L(1, i - 1, j - 1), L(0, i - 1, j - 1),
                             L(1, i, j - 1)};
       for (int dd = 0; dd < 3; ++dd) o[dd] = hex_sum(G, d.nreg, ci, dd, T6);
     }
-    const double Rr = c.RADIUS;
-    const double gd[3] = {c.ALPHA_D * o[0], c.ALPHA_D * o[1], c.ALPHA_D * o[2]};
-    const double gx = G.gx[ci], gy = G.gx[d.nreg + ci], gz = G.gx[2 * d.nreg + ci];
-    const double prd = (gd[0] * gx + gd[1] * gy + gd[2] * gz) / Rr;
+    const real Rr = c.RADIUS;
+    const real gd[3] = {c.ALPHA_D * o[0], c.ALPHA_D * o[1], c.ALPHA_D * o[2]};
+    const real gx = G.gx[ci], gy = G.gx[d.nreg + ci], gz = G.gx[2 * d.nreg + ci];
+    const real prd = (gd[0] * gx + gd[1] * gy + gd[2] * gz) / Rr;
     out[0][a] = gd[0] - prd * gx / Rr;
     out[1][a] = gd[1] - prd * gy / Rr;
     out[2][a] = gd[2] - prd * gz / Rr;
@@ -184,17 +186,18 @@ struct DivDamp2T {
 // fields (3 components x TI/TJ) go through LDS
 // ----------------------------------------------------------------------------
 struct HDiffPass2T {
+  using LdsT = real;
   static constexpr const char* kName = "hdiff_pass2";
   Dims d; Consts c; Geo G; TileGeom tg;
-  const double* in[6];
-  double* out[6];
-  const double* rhog;
+  const real* in[6];
+  real* out[6];
+  const real* rhog;
   // LDS layout: [field f (0,1)][triangle T][component dd][row][col]
   NH_HD int lidx(const TileAt& t, int f, int T, int dd, int cj_, int ci_) const {
     const int W1 = tg.W + 1, R = t.rows + 1;
     return (((f * 2 + T) * 3 + dd) * R + cj_) * W1 + ci_;
   }
-  NH_HD void phase1(int blk, int tid, double* lds) const {
+  NH_HD void phase1(int blk, int tid, real* lds) const {
     if (blk >= tg.ntiles) return;
     const TileAt t = tile_at(d, tg, blk);
     const int g = d.g, W1 = tg.W + 1;
@@ -206,13 +209,13 @@ struct HDiffPass2T {
       const long a = ((long)(t.l * d.kall + t.k) * g + j) * g + i;
       for (int f = 0; f < 2; ++f)
         for (int T = 0; T < 2; ++T) {
-          double o3[3];
+          real o3[3];
           tri_grad(G, d.nreg, cj, T, in[4 + f], a, g, o3);
           for (int dd = 0; dd < 3; ++dd) lds[lidx(t, f, T, dd, cj_, ci_)] = o3[dd];
         }
     }
   }
-  NH_HD void phase2(int blk, int tid, const double* lds) const {
+  NH_HD void phase2(int blk, int tid, const real* lds) const {
     const int g = d.g;
     if (blk >= tg.ntiles) {
       if (tid == 0) {
@@ -232,30 +235,30 @@ struct HDiffPass2T {
     const long a = ((long)(t.l * d.kall + t.k) * g + j) * g + i;
     for (int q = 0; q < 4; ++q) out[q][a] = -lap7(G, nreg, ci, in[q], a, g);
     const bool pent = (i == 1 && j == 1);
-    const double m0 = G.sgp[t.l] ? 0.0 : 1.0, m1 = G.sgp[t.l] ? 1.0 : 0.0;
+    const real m0 = G.sgp[t.l] ? real(0.0) : real(1.0), m1 = G.sgp[t.l] ? real(1.0) : real(0.0);
     // corner offsets of the six triangles: TI(i,j) TJ(i,j) TI(i-1,j) TJ(i-1,j-1) TI(i-1,j-1) TJ(i,j-1)
     const int cdi[6] = {0, 0, -1, -1, -1, 0}, cdj[6] = {0, 0, 0, -1, -1, -1}, cT[6] = {0, 1, 0, 1, 0, 1};
     for (int f = 0; f < 2; ++f) {
-      const double fac = (f == 0) ? c.CVdry : c.HDIFF_FACT_RHO;
-      const double gam = c.GAMMA_H;
+      const real fac = (f == 0) ? c.CVdry : c.HDIFF_FACT_RHO;
+      const real gam = c.GAMMA_H;
       auto kh = [&](long x) { return rhog[x] * fac * gam; };
-      double tv[6][3];
+      real tv[6][3];
       for (int n = 0; n < 6; ++n)
         for (int dd = 0; dd < 3; ++dd)
           tv[n][dd] = lds[lidx(t, f, cT[n], dd, j + cdj[n] - t.j0 + 1, i + cdi[n])];
       if (pent)
         for (int dd = 0; dd < 3; ++dd) tv[4][dd] = tv[4][dd] * m0 + tv[5][dd] * m1;
-      const double k0 = kh(a);
-      const double kf1 = 0.5 * (k0 + kh(a + 1 + g));
-      const double kf2 = 0.5 * (k0 + kh(a + g));
-      const double kf3 = 0.5 * (k0 + kh(a - 1));
-      const double kf4 = 0.5 * (k0 + kh(a - 1 - g));
-      const double kf5 = 0.5 * (k0 + kh(a - g));
-      const double kf6 = 0.5 * (k0 + kh(a + 1));
-      double inner = 0.0;
+      const real k0 = kh(a);
+      const real kf1 = real(0.5) * (k0 + kh(a + 1 + g));
+      const real kf2 = real(0.5) * (k0 + kh(a + g));
+      const real kf3 = real(0.5) * (k0 + kh(a - 1));
+      const real kf4 = real(0.5) * (k0 + kh(a - 1 - g));
+      const real kf5 = real(0.5) * (k0 + kh(a - g));
+      const real kf6 = real(0.5) * (k0 + kh(a + 1));
+      real inner = real(0.0);
       for (int dd = 0; dd < 3; ++dd) {
-        const double* cd = G.cdiff + (long)(dd * 6) * nreg + ci;
-        const double blk2 = kf1 * cd[0] * (tv[0][dd] + tv[1][dd]) + kf2 * cd[nreg] * (tv[1][dd] + tv[2][dd])
+        const real* cd = G.cdiff + (long)(dd * 6) * nreg + ci;
+        const real blk2 = kf1 * cd[0] * (tv[0][dd] + tv[1][dd]) + kf2 * cd[nreg] * (tv[1][dd] + tv[2][dd])
                           + kf3 * cd[2 * nreg] * (tv[2][dd] + tv[3][dd]) + kf4 * cd[3 * nreg] * (tv[3][dd] + tv[4][dd])
                           + kf5 * cd[4 * nreg] * (tv[4][dd] + tv[5][dd]) + kf6 * cd[5 * nreg] * (tv[5][dd] + tv[0][dd]);
         inner = (dd == 0) ? blk2 : inner + blk2;
@@ -265,4 +268,4 @@ struct HDiffPass2T {
   }
 };
 
-}  // namespace nh
+NH_END

@@ -16,63 +16,72 @@
 
 #include "nh_step.h"
 
-namespace nh {
+NH_BEGIN
 
-#define NH_EPS 2.220446049250313e-16
-#define NH_BIG 1.7976931348623157e308
+// CONST_EPS / CONST_HUGE of the field precision (np.finfo(rdtype).eps / .max,
+// mod_const.py:170-172)
+template <class T> struct Lim;
+template <> struct Lim<double> {
+  static constexpr double eps = 2.220446049250313e-16, big = 1.7976931348623157e308;   // NH_KEEP64
+};
+template <> struct Lim<float> {
+  static constexpr float eps = 1.1920928955078125e-07f, big = 3.4028234663852886e+38f;
+};
+#define NH_EPS (Lim<real>::eps)
+#define NH_BIG (Lim<real>::big)
 
-NH_HD inline double dmin(double a, double b) { return (a < b) ? a : b; }
-NH_HD inline double dmax(double a, double b) { return (a > b) ? a : b; }
-NH_HD inline double clip(double x, double lo, double hi) { return dmin(dmax(x, lo), hi); }
-NH_HD inline double upwind_mask(double ch) { return 0.5 - copysign(0.5, ch - NH_EPS); }
+NH_HD inline real dmin(real a, real b) { return (a < b) ? a : b; }
+NH_HD inline real dmax(real a, real b) { return (a > b) ? a : b; }
+NH_HD inline real clip(real x, real lo, real hi) { return dmin(dmax(x, lo), hi); }
+NH_HD inline real upwind_mask(real ch) { return real(0.5) - nsign(real(0.5), ch - NH_EPS); }
 
 // ----------------------------------------------------------------------------
 // vertical half step of one tracer column (mod_src_tracer.py:420-533 and the
 // limiter mod_src_tracer.py:1909-2074), streamed in k
 // ----------------------------------------------------------------------------
 template <class DEN, class FLX, class DV>
-NH_HD inline void vert_sweep(const Geo& G, int kmin, int kmax, const double* in, double* out, long base, long ks,
+NH_HD inline void vert_sweep(const Geo& G, int kmin, int kmax, const real* in, real* out, long base, long ks,
                              DEN den, FLX flx, DV dv, bool limiter, bool clipneg) {
-  auto ck0 = [&](int k) -> double { return (k >= kmin && k <= kmax) ? -flx(k) / den(k) * G.rdgz[k] : 0.0; };
-  double r_km1 = in[base + (kmin - 1) * ks];
-  double r_k = in[base + kmin * ks];
-  double q_km1 = r_km1 / den(kmin - 1);
-  double q_k = r_k / den(kmin);
-  double omin_prev = 0.0, omax_prev = 0.0;
-  double qh_prev = 0.0;      // limited face value at k-1 (zero at kmin)
-  double r_prev = r_km1;
-  double c0_k = ck0(kmin);
+  auto ck0 = [&](int k) -> real { return (k >= kmin && k <= kmax) ? -flx(k) / den(k) * G.rdgz[k] : real(0.0); };
+  real r_km1 = in[base + (kmin - 1) * ks];
+  real r_k = in[base + kmin * ks];
+  real q_km1 = r_km1 / den(kmin - 1);
+  real q_k = r_k / den(kmin);
+  real omin_prev = real(0.0), omax_prev = real(0.0);
+  real qh_prev = real(0.0);      // limited face value at k-1 (zero at kmin)
+  real r_prev = r_km1;
+  real c0_k = ck0(kmin);
   for (int k = kmin; k <= kmax; ++k) {
-    const double r_kp1 = in[base + (k + 1) * ks];
-    const double q_kp1 = r_kp1 / den(k + 1);
-    const double c0_kp1 = ck0(k + 1);
-    const double c1_k = flx(k + 1) / den(k) * G.rdgz[k];
-    double qh = G.afact[k] * q_k + G.bfact[k] * q_km1;
-    double omin = 0.0, omax = 0.0;
+    const real r_kp1 = in[base + (k + 1) * ks];
+    const real q_kp1 = r_kp1 / den(k + 1);
+    const real c0_kp1 = ck0(k + 1);
+    const real c1_k = flx(k + 1) / den(k) * G.rdgz[k];
+    real qh = G.afact[k] * q_k + G.bfact[k] * q_km1;
+    real omin = real(0.0), omax = real(0.0);
     if (limiter) {
-      const double inL = 0.5 - copysign(0.5, c0_k);
-      const double inU = 0.5 + copysign(0.5, c0_kp1);
-      const double minL = (inL == 1.0) ? dmin(q_k, q_km1) : NH_BIG;
-      const double minU = (inU == 1.0) ? dmin(q_k, q_kp1) : NH_BIG;
-      const double maxL = (inL == 1.0) ? dmax(q_k, q_km1) : -NH_BIG;
-      const double maxU = (inU == 1.0) ? dmax(q_k, q_kp1) : -NH_BIG;
-      const double nmin = dmin(dmin(minL, minU), q_k);
-      const double nmax = dmax(dmax(maxL, maxU), q_k);
-      const double Cin = inL * c0_k + inU * c1_k;
-      const double Cout = (1.0 - inL) * c0_k + (1.0 - inU) * c1_k;
-      const double CQmin = inL * c0_k * minL + inU * c1_k * minU;
-      const double CQmax = inL * c0_k * maxL + inU * c1_k * maxU;
-      const double zsw = 0.5 - copysign(0.5, fabs(Cout) - NH_EPS);
-      const double dk = dv(k);
-      omin = ((q_k - nmax) + nmax * (Cin + Cout - dk) - CQmax) / (Cout + zsw) * (1.0 - zsw) + q_k * zsw;
-      omax = ((q_k - nmin) + nmin * (Cin + Cout - dk) - CQmin) / (Cout + zsw) * (1.0 - zsw) + q_k * zsw;
-      if (k > kmin) qh = inL * clip(qh, omin_prev, omax_prev) + (1.0 - inL) * clip(qh, omin, omax);
+      const real inL = real(0.5) - nsign(real(0.5), c0_k);
+      const real inU = real(0.5) + nsign(real(0.5), c0_kp1);
+      const real minL = (inL == real(1.0)) ? dmin(q_k, q_km1) : NH_BIG;
+      const real minU = (inU == real(1.0)) ? dmin(q_k, q_kp1) : NH_BIG;
+      const real maxL = (inL == real(1.0)) ? dmax(q_k, q_km1) : -NH_BIG;
+      const real maxU = (inU == real(1.0)) ? dmax(q_k, q_kp1) : -NH_BIG;
+      const real nmin = dmin(dmin(minL, minU), q_k);
+      const real nmax = dmax(dmax(maxL, maxU), q_k);
+      const real Cin = inL * c0_k + inU * c1_k;
+      const real Cout = (real(1.0) - inL) * c0_k + (real(1.0) - inU) * c1_k;
+      const real CQmin = inL * c0_k * minL + inU * c1_k * minU;
+      const real CQmax = inL * c0_k * maxL + inU * c1_k * maxU;
+      const real zsw = real(0.5) - nsign(real(0.5), nabs(Cout) - NH_EPS);
+      const real dk = dv(k);
+      omin = ((q_k - nmax) + nmax * (Cin + Cout - dk) - CQmax) / (Cout + zsw) * (real(1.0) - zsw) + q_k * zsw;
+      omax = ((q_k - nmin) + nmin * (Cin + Cout - dk) - CQmin) / (Cout + zsw) * (real(1.0) - zsw) + q_k * zsw;
+      if (k > kmin) qh = inL * clip(qh, omin_prev, omax_prev) + (real(1.0) - inL) * clip(qh, omin, omax);
     }
-    if (k == kmin) qh = 0.0;
+    if (k == kmin) qh = real(0.0);
     if (k > kmin) {
       // update of level k-1 with the face values at k-1 and k
-      double o = r_prev - (flx(k) * qh - flx(k - 1) * qh_prev) * G.rdgz[k - 1];
-      if (clipneg && o > -1.0e-10 && o < 0.0) o = 0.0;
+      real o = r_prev - (flx(k) * qh - flx(k - 1) * qh_prev) * G.rdgz[k - 1];
+      if (clipneg && o > -real(1.0e-10) && o < real(0.0)) o = real(0.0);
       out[base + (k - 1) * ks] = o;
     }
     omin_prev = omin;
@@ -85,12 +94,12 @@ NH_HD inline void vert_sweep(const Geo& G, int kmin, int kmax, const double* in,
     c0_k = c0_kp1;
   }
   {
-    double o = r_prev - (flx(kmax + 1) * 0.0 - flx(kmax) * qh_prev) * G.rdgz[kmax];
-    if (clipneg && o > -1.0e-10 && o < 0.0) o = 0.0;
+    real o = r_prev - (flx(kmax + 1) * real(0.0) - flx(kmax) * qh_prev) * G.rdgz[kmax];
+    if (clipneg && o > -real(1.0e-10) && o < real(0.0)) o = real(0.0);
     out[base + kmax * ks] = o;
   }
-  out[base + (kmin - 1) * ks] = 0.0;
-  out[base + (kmax + 1) * ks] = 0.0;
+  out[base + (kmin - 1) * ks] = real(0.0);
+  out[base + (kmax + 1) * ks] = real(0.0);
 }
 
 // Phase 1: vertical flux of the mean mass flux, half-step density and the
@@ -100,31 +109,31 @@ NH_HD inline void vert_sweep(const Geo& G, int kmin, int kmax, const double* in,
 struct TrcVert1 {
   static constexpr const char* kName = "trc_vert1";
   Dims d; Consts c; Geo G;
-  double dt;
-  const double* rw_mean;    // time-mean rhogw
-  const double* rhog_in;    // density at the start of the large step
-  const double* frhog;      // hyperdiffusion density tendency
-  double* flxv;             // out
-  double* rhog1;            // out
-  double* rhogq;            // nq tracer fields, in place
+  real dt;
+  const real* rw_mean;    // time-mean rhogw
+  const real* rhog_in;    // density at the start of the large step
+  const real* frhog;      // hyperdiffusion density tendency
+  real* flxv;             // out
+  real* rhog1;            // out
+  real* rhogq;            // nq tracer fields, in place
   int nq;
   int limiter;
   NH_HD void operator()(long t) const {
     Cell cl = cell_of(d, t);
     const long ks = cl.ks, b = cl.base;
     const int kmin = d.kmin, kmax = d.kmax;
-    auto flx = [&](int k) -> double {
-      return (k >= kmin + 1 && k <= kmax) ? (0.0 + rw_mean[b + k * ks] * 1.0) * 0.5 * dt : 0.0;
+    auto flx = [&](int k) -> real {
+      return (k >= kmin + 1 && k <= kmax) ? (real(0.0) + rw_mean[b + k * ks] * real(1.0)) * real(0.5) * dt : real(0.0);
     };
     for (int k = 0; k < d.kall; ++k) flxv[b + k * ks] = flx(k);
     for (int k = kmin; k <= kmax; ++k)
-      rhog1[b + k * ks] = rhog_in[b + k * ks] - (flx(k + 1) - flx(k)) * G.rdgz[k] + 0.0 * frhog[b + k * ks] * dt;
+      rhog1[b + k * ks] = rhog_in[b + k * ks] - (flx(k + 1) - flx(k)) * G.rdgz[k] + real(0.0) * frhog[b + k * ks] * dt;
     rhog1[b + (kmin - 1) * ks] = rhog_in[b + kmin * ks];
     rhog1[b + (kmax + 1) * ks] = rhog_in[b + kmax * ks];
     auto den = [&](int k) { return rhog_in[b + k * ks]; };
-    auto dv = [&](int k) { return 0.0 * frhog[b + k * ks] / rhog_in[b + k * ks] * dt; };
+    auto dv = [&](int k) { return real(0.0) * frhog[b + k * ks] / rhog_in[b + k * ks] * dt; };
     for (int q = 0; q < nq; ++q) {
-      double* f = rhogq + (long)q * d.NF;
+      real* f = rhogq + (long)q * d.NF;
       vert_sweep(G, kmin, kmax, f, f, b, ks, den, flx, dv, limiter != 0, false);
     }
   }
@@ -136,12 +145,12 @@ struct TrcVert1 {
 struct TrcVert2 {
   static constexpr const char* kName = "trc_vert2";
   Dims d; Consts c; Geo G;
-  double dt;
-  const double* flxv;
-  const double* rhog2;
-  const double* frhog;
-  const double* in;     // tracer after the horizontal step
-  double* out;          // tracer field
+  real dt;
+  const real* flxv;
+  const real* rhog2;
+  const real* frhog;
+  const real* in;     // tracer after the horizontal step
+  real* out;          // tracer field
   int limiter;
   NH_HD void operator()(long t) const {
     Cell cl = cell_of(d, t);
@@ -149,7 +158,7 @@ struct TrcVert2 {
     const long ks = cl.ks, b = cl.base;
     auto flx = [&](int k) { return flxv[b + k * ks]; };
     auto den = [&](int k) { return rhog2[b + k * ks]; };
-    auto dv = [&](int k) { return 0.0 * frhog[b + k * ks] / rhog2[b + k * ks] * dt; };
+    auto dv = [&](int k) { return real(0.0) * frhog[b + k * ks] / rhog2[b + k * ks] * dt; };
     vert_sweep(G, d.kmin, d.kmax, in, out, b, ks, den, flx, dv, limiter != 0, true);
   }
 };
@@ -161,10 +170,10 @@ struct TrcVert2 {
 struct TrcHFlux {
   static constexpr const char* kName = "trc_hflux";
   Dims d; Consts c; Geo G;
-  double dt;
-  const double* pm;     // 5 mean-flux fields: rhog, rhogvx, rhogvy, rhogvz, rhogw
-  double* fa[3];
-  double* xc[9];
+  real dt;
+  const real* pm;     // 5 mean-flux fields: rhog, rhogvx, rhogvy, rhogvz, rhogw
+  real* fa[3];
+  real* xc[9];
   NH_HD void operator()(long t) const {
     Cell cl = cell_of(d, t);
     const long NF = d.NF;
@@ -173,25 +182,25 @@ struct TrcHFlux {
       const int lp = cl.lp;
       for (int k = 0; k < d.kall; ++k) {
         const long a0 = cl.base + k * 6;
-        double rt[6][4];
+        real rt[6][4];
         for (int v = 1; v <= 5; ++v) {
           int vp = (v == 5) ? 1 : v + 1;
-          const double* w = G.twt_pl + (v * 2 + lp) * 3;
+          const real* w = G.twt_pl + (v * 2 + lp) * 3;
           for (int q = 0; q < 4; ++q) {
-            const double* f = pm + q * NF;
+            const real* f = pm + q * NF;
             rt[v][q] = f[a0] * w[0] + f[a0 + v] * w[1] + f[a0 + vp] * w[2];
           }
         }
         for (int v = 1; v <= 5; ++v) {
           int vm = (v == 1) ? 5 : v - 1;
-          const double rr = 1.0 / dmax(rt[vm][0] + rt[v][0], NH_EPS);
-          const double v2x = rt[vm][1] + rt[v][1], v2y = rt[vm][2] + rt[v][2], v2z = rt[vm][3] + rt[v][3];
-          const double* hn = G.ahn_pl + (v * 2 + lp) * 3;
-          fa[0][a0 + v] = 0.5 * (v2x * hn[0] + v2y * hn[1] + v2z * hn[2]);
-          const double* xr = G.axr_pl + (v * 2 + lp) * 3;
-          xc[0][a0 + v] = xr[0] - v2x * rr * dt * 0.5;
-          xc[1][a0 + v] = xr[1] - v2y * rr * dt * 0.5;
-          xc[2][a0 + v] = xr[2] - v2z * rr * dt * 0.5;
+          const real rr = real(1.0) / dmax(rt[vm][0] + rt[v][0], NH_EPS);
+          const real v2x = rt[vm][1] + rt[v][1], v2y = rt[vm][2] + rt[v][2], v2z = rt[vm][3] + rt[v][3];
+          const real* hn = G.ahn_pl + (v * 2 + lp) * 3;
+          fa[0][a0 + v] = real(0.5) * (v2x * hn[0] + v2y * hn[1] + v2z * hn[2]);
+          const real* xr = G.axr_pl + (v * 2 + lp) * 3;
+          xc[0][a0 + v] = xr[0] - v2x * rr * dt * real(0.5);
+          xc[1][a0 + v] = xr[1] - v2y * rr * dt * real(0.5);
+          xc[2][a0 + v] = xr[2] - v2z * rr * dt * real(0.5);
         }
       }
       return;
@@ -199,26 +208,26 @@ struct TrcHFlux {
     const int g = d.g, i = cl.i, j = cl.j;
     if (i > g - 2 || j > g - 2) return;
     const long nreg = d.nreg, ci = cl.ci, ks = cl.ks;
-    const double m0 = G.sgp[cl.l] ? 0.0 : 1.0, m1 = G.sgp[cl.l] ? 1.0 : 0.0;
+    const real m0 = G.sgp[cl.l] ? real(0.0) : real(1.0), m1 = G.sgp[cl.l] ? real(1.0) : real(0.0);
     for (int k = 0; k < d.kall; ++k) {
       const long a = cl.base + k * ks;
       // triangle value of type T with corner at offset (di, dj) from this cell
-      auto tri = [&](int T, int di, int dj, int q) -> double {
-        const double* f = pm + q * NF;
+      auto tri = [&](int T, int di, int dj, int q) -> real {
+        const real* f = pm + q * NF;
         const long o = a + di + (long)dj * g;
         const long cc = ci + di + (long)dj * g;
-        const double w1 = rc(G.twt, nreg, T * 3 + 0, cc), w2 = rc(G.twt, nreg, T * 3 + 1, cc),
+        const real w1 = rc(G.twt, nreg, T * 3 + 0, cc), w2 = rc(G.twt, nreg, T * 3 + 1, cc),
                      w3 = rc(G.twt, nreg, T * 3 + 2, cc);
         if (T == 0) {
-          double s = f[o] * w1 + f[o + 1] * w2;
+          real s = f[o] * w1 + f[o + 1] * w2;
           return s + f[o + 1 + g] * w3;
         }
-        double s = f[o] * w1;
+        real s = f[o] * w1;
         return s + (f[o + 1 + g] * w2 + f[o + g] * w3);
       };
       // the (0, 0) TI triangle is replaced by TJ(1, 0) on pentagon regions (arithmetic mask, all regions)
-      auto triv = [&](int T, int di, int dj, int q) -> double {
-        double v = tri(T, di, dj, q);
+      auto triv = [&](int T, int di, int dj, int q) -> real {
+        real v = tri(T, di, dj, q);
         if (T == 0 && i + di == 0 && j + dj == 0) v = v * m0 + tri(1, 1 - i, -j, q) * m1;
         return v;
       };
@@ -230,29 +239,29 @@ struct TrcHFlux {
         else if (arc == 1) { Ta = 0; dia = 0; dja = 0; Tb = 1; dib = 0; djb = 0; }
         else { Ta = 1; dia = 0; dja = 0; Tb = 0; dib = -1; djb = 0; valid = (i >= 1); }
         if (!valid) {
-          fa[arc][a] = 0.0;
-          for (int dd = 0; dd < 3; ++dd) xc[arc * 3 + dd][a] = 0.0;
+          fa[arc][a] = real(0.0);
+          for (int dd = 0; dd < 3; ++dd) xc[arc * 3 + dd][a] = real(0.0);
           continue;
         }
-        const double rs = triv(Ta, dia, dja, 0) + triv(Tb, dib, djb, 0);
-        const double rr = 1.0 / dmax(rs, NH_EPS);
-        double v2[3];
+        const real rs = triv(Ta, dia, dja, 0) + triv(Tb, dib, djb, 0);
+        const real rr = real(1.0) / dmax(rs, NH_EPS);
+        real v2[3];
         for (int q = 0; q < 3; ++q) v2[q] = triv(Ta, dia, dja, q + 1) + triv(Tb, dib, djb, q + 1);
-        const double hx = rc(G.ahn, nreg, arc * 3 + 0, ci), hy = rc(G.ahn, nreg, arc * 3 + 1, ci),
+        const real hx = rc(G.ahn, nreg, arc * 3 + 0, ci), hy = rc(G.ahn, nreg, arc * 3 + 1, ci),
                      hz = rc(G.ahn, nreg, arc * 3 + 2, ci);
-        fa[arc][a] = 0.5 * (v2[0] * hx + v2[1] * hy + v2[2] * hz);
+        fa[arc][a] = real(0.5) * (v2[0] * hx + v2[1] * hy + v2[2] * hz);
         for (int dd = 0; dd < 3; ++dd)
-          xc[arc * 3 + dd][a] = rc(G.axr, nreg, arc * 3 + dd, ci) - v2[dd] * rr * dt * 0.5;
+          xc[arc * 3 + dd][a] = rc(G.axr, nreg, arc * 3 + dd, ci) - v2[dd] * rr * dt * real(0.5);
       }
     }
   }
 };
 
 // face mass flux of face m (0..5) of regional cell at address a / column ci
-NH_HD inline double face_flux(const Geo& G, const Dims& d, const double* const* fa, long a, long ci, int i, int j,
-                              int l, int m, double dt) {
+NH_HD inline real face_flux(const Geo& G, const Dims& d, const real* const* fa, long a, long ci, int i, int j,
+                              int l, int m, real dt) {
   const int g = d.g;
-  const double ra = rc(G.rarea, d.nreg, 0, ci);
+  const real ra = rc(G.rarea, d.nreg, 0, ci);
   switch (m) {
     case 0: return fa[0][a] * ra * dt;
     case 1: return fa[1][a] * ra * dt;
@@ -260,8 +269,8 @@ NH_HD inline double face_flux(const Geo& G, const Dims& d, const double* const* 
     case 3: return -fa[0][a - 1] * ra * dt;
     case 4: return -fa[1][a - 1 - g] * ra * dt;
     default: {
-      double f = -fa[2][a - g] * ra * dt;
-      if (i == 1 && j == 1) f *= (G.sgp[l] ? 0.0 : 1.0);
+      real f = -fa[2][a - g] * ra * dt;
+      if (i == 1 && j == 1) f *= (G.sgp[l] ? real(0.0) : real(1.0));
       return f;
     }
   }
@@ -271,25 +280,25 @@ NH_HD inline double face_flux(const Geo& G, const Dims& d, const double* const* 
 struct TrcRhog2 {
   static constexpr const char* kName = "trc_rhog2";
   Dims d; Consts c; Geo G;
-  double dt;
-  const double* fa[3];
-  const double* rhog1;
-  const double* frhog;
-  double* rhog2;
+  real dt;
+  const real* fa[3];
+  const real* rhog1;
+  const real* frhog;
+  real* rhog2;
   NH_HD void operator()(long t) const {
     Cell cl = cell_of(d, t);
     if (cl.pole ? cl.v != 0 : !cl.inner) return;
     for (int k = 0; k < d.kall; ++k) {
       const long a = cl.base + k * cl.ks;
       if (cl.pole) {
-        double r = rhog1[a];
+        real r = rhog1[a];
         for (int v = 1; v <= 5; ++v) r -= fa[0][a + v] * G.rarea_pl[cl.lp] * dt;
-        r += 1.0 * frhog[a] * dt;
+        r += real(1.0) * frhog[a] * dt;
         rhog2[a] = r;
       } else {
-        double f[6];
+        real f[6];
         for (int m = 0; m < 6; ++m) f[m] = face_flux(G, d, fa, a, cl.ci, cl.i, cl.j, cl.l, m, dt);
-        rhog2[a] = rhog1[a] - ((((f[0] + f[1]) + f[2]) + f[3]) + f[4] + f[5] - (1.0 * frhog[a] * dt));
+        rhog2[a] = rhog1[a] - ((((f[0] + f[1]) + f[2]) + f[3]) + f[4] + f[5] - (real(1.0) * frhog[a] * dt));
       }
     }
   }
@@ -299,9 +308,9 @@ struct TrcRhog2 {
 struct TrcGradQ {
   static constexpr const char* kName = "trc_gradq";
   Dims d; Geo G;
-  const double* rq;
-  const double* rhog1;
-  double* gq[3];
+  const real* rq;
+  const real* rhog1;
+  real* gq[3];
   NH_HD void operator()(long t) const {
     Cell cl = cell_of(d, t);
     if (cl.pole ? cl.v != 0 : !cl.inner) return;
@@ -309,14 +318,14 @@ struct TrcGradQ {
     int nn = neighbours(d, cl, nb);
     for (int k = 0; k < d.kall; ++k) {
       const long kk = k * cl.ks;
-      double qv[7];
+      real qv[7];
       for (int m = 0; m < nn; ++m) qv[m] = rq[nb[m] + kk] / rhog1[nb[m] + kk];
       for (int dd = 0; dd < 3; ++dd) {
-        double acc;
+        real acc;
         if (cl.pole) {
-          acc = 0.0;
+          acc = real(0.0);
           for (int v = 0; v < 6; ++v) {
-            double pv = G.cgrad_pl[(v * 2 + cl.lp) * 3 + dd] * qv[v];
+            real pv = G.cgrad_pl[(v * 2 + cl.lp) * 3 + dd] * qv[v];
             acc = (v == 0) ? pv : acc + pv;
           }
         } else {
@@ -335,13 +344,13 @@ struct TrcGradQ {
 struct TrcRemap {
   static constexpr const char* kName = "trc_remap";
   Dims d; Consts c; Geo G;
-  double dt;
-  const double* rq;
-  const double* rhog1;
-  const double* gq[3];
-  const double* fa[3];
-  const double* xc[9];
-  double* qa[3];
+  real dt;
+  const real* rq;
+  const real* rhog1;
+  const real* gq[3];
+  const real* fa[3];
+  const real* xc[9];
+  real* qa[3];
   NH_HD void operator()(long t) const {
     Cell cl = cell_of(d, t);
     if (cl.pole) {
@@ -349,17 +358,17 @@ struct TrcRemap {
       // pole arc v: upwind value from the pole centre or spoke v
       const int lp = cl.lp, v = cl.v;
       const long R0 = d.R + (long)lp * d.kall * 6;
-      const double* xp0 = G.gx_pl + (0 * 2 + lp) * 3;
-      const double* xpv = G.gx_pl + (v * 2 + lp) * 3;
+      const real* xp0 = G.gx_pl + (0 * 2 + lp) * 3;
+      const real* xpv = G.gx_pl + (v * 2 + lp) * 3;
       for (int k = 0; k < d.kall; ++k) {
         const long a0 = R0 + k * 6, av = a0 + v;
-        const double q0 = rq[a0] / rhog1[a0], qv = rq[av] / rhog1[av];
-        const double xcx = xc[0][av], xcy = xc[1][av], xcz = xc[2][av];
-        const double qap = q0 + gq[0][a0] * (xcx - xp0[0]) + gq[1][a0] * (xcy - xp0[1]) + gq[2][a0] * (xcz - xp0[2]);
-        const double qam = qv + gq[0][av] * (xcx - xpv[0]) + gq[1][av] * (xcy - xpv[1]) + gq[2][av] * (xcz - xpv[2]);
-        const double ch = fa[0][av] * G.rarea_pl[lp] * dt / rhog1[a0];
-        const double cm = upwind_mask(ch);
-        qa[0][av] = cm * qam + (1.0 - cm) * qap;
+        const real q0 = rq[a0] / rhog1[a0], qv = rq[av] / rhog1[av];
+        const real xcx = xc[0][av], xcy = xc[1][av], xcz = xc[2][av];
+        const real qap = q0 + gq[0][a0] * (xcx - xp0[0]) + gq[1][a0] * (xcy - xp0[1]) + gq[2][a0] * (xcz - xp0[2]);
+        const real qam = qv + gq[0][av] * (xcx - xpv[0]) + gq[1][av] * (xcy - xpv[1]) + gq[2][av] * (xcz - xpv[2]);
+        const real ch = fa[0][av] * G.rarea_pl[lp] * dt / rhog1[a0];
+        const real cm = upwind_mask(ch);
+        qa[0][av] = cm * qam + (real(1.0) - cm) * qap;
       }
       return;
     }
@@ -372,18 +381,18 @@ struct TrcRemap {
     for (int k = 0; k < d.kall; ++k) {
       const long a = cl.base + k * ks;
       for (int arc = 0; arc < 3; ++arc) {
-        if (!valid[arc]) { qa[arc][a] = 0.0; continue; }
+        if (!valid[arc]) { qa[arc][a] = real(0.0); continue; }
         const long an = a + noff[arc];
         const long cn = ci + noff[arc];
-        const double xcx = xc[arc * 3 + 0][a], xcy = xc[arc * 3 + 1][a], xcz = xc[arc * 3 + 2][a];
-        const double qc = rq[a] / rhog1[a], qn = rq[an] / rhog1[an];
-        const double qap = qc + gq[0][a] * (xcx - rc(G.gx, nreg, 0, ci)) + gq[1][a] * (xcy - rc(G.gx, nreg, 1, ci))
+        const real xcx = xc[arc * 3 + 0][a], xcy = xc[arc * 3 + 1][a], xcz = xc[arc * 3 + 2][a];
+        const real qc = rq[a] / rhog1[a], qn = rq[an] / rhog1[an];
+        const real qap = qc + gq[0][a] * (xcx - rc(G.gx, nreg, 0, ci)) + gq[1][a] * (xcy - rc(G.gx, nreg, 1, ci))
                          + gq[2][a] * (xcz - rc(G.gx, nreg, 2, ci));
-        const double qam = qn + gq[0][an] * (xcx - rc(G.gx, nreg, 0, cn)) + gq[1][an] * (xcy - rc(G.gx, nreg, 1, cn))
+        const real qam = qn + gq[0][an] * (xcx - rc(G.gx, nreg, 0, cn)) + gq[1][an] * (xcy - rc(G.gx, nreg, 1, cn))
                          + gq[2][an] * (xcz - rc(G.gx, nreg, 2, cn));
-        const double ch = fa[arc][a] * rc(G.rarea, nreg, 0, ci) * dt / rhog1[a];
-        const double cm = upwind_mask(ch);
-        qa[arc][a] = cm * qam + (1.0 - cm) * qap;
+        const real ch = fa[arc][a] * rc(G.rarea, nreg, 0, ci) * dt / rhog1[a];
+        const real cm = upwind_mask(ch);
+        qa[arc][a] = cm * qam + (real(1.0) - cm) * qap;
       }
     }
   }
@@ -396,12 +405,12 @@ struct TrcRemap {
 // six faces.  Stencil: 0 (i,j) 1 (i+1,j) 2 (i+1,j+1) 3 (i,j+1) 4 (i-1,j)
 // 5 (i-1,j-1) 6 (i,j-1).
 struct FaceBounds {
-  double mn[6], mx[6];      // min/max of q over the four cells around each face
-  double cm[6];             // upwind mask of the face's owner
+  real mn[6], mx[6];      // min/max of q over the four cells around each face
+  real cm[6];             // upwind mask of the face's owner
   bool pent;                // face 4 uses the pentagon override
 };
 
-NH_HD inline void face_bounds(const double q[7], const double cmo[6], bool pentagon, FaceBounds& fb) {
+NH_HD inline void face_bounds(const real q[7], const real cmo[6], bool pentagon, FaceBounds& fb) {
   // owner sets: own AI {0,6,1,2}, own AIJ {0,1,2,3}, own AJ {0,2,3,4},
   // (i-1,j) AI {4,5,0,3}, (i-1,j-1) AIJ {5,6,0,4} (pentagon: {5,0,1,4}), (i,j-1) AJ {6,1,0,5}
   const int sets[6][4] = {{0, 6, 1, 2}, {0, 1, 2, 3}, {0, 2, 3, 4}, {4, 5, 0, 3}, {5, 6, 0, 4}, {6, 1, 0, 5}};
@@ -409,7 +418,7 @@ NH_HD inline void face_bounds(const double q[7], const double cmo[6], bool penta
     const int* s = sets[m];
     int s1 = s[1], s2 = s[2];
     if (m == 4 && pentagon) { s1 = 0; s2 = 1; }
-    const double a = q[s[0]], b = q[s1], c = q[s2], e = q[s[3]];
+    const real a = q[s[0]], b = q[s1], c = q[s2], e = q[s[3]];
     fb.mn[m] = dmin(dmin(dmin(a, b), c), e);
     fb.mx[m] = dmax(dmax(dmax(a, b), c), e);
     fb.cm[m] = cmo[m];
@@ -419,42 +428,42 @@ NH_HD inline void face_bounds(const double q[7], const double cmo[6], bool penta
 
 // inflow bounds as seen from this cell (Qin(i, j, *, m)): faces 0..2 are owned by the
 // cell, faces 3..5 by the neighbour and see the complementary mask
-NH_HD inline void qin_of_cell(const FaceBounds& fb, int m, double& lo, double& hi) {
-  const double c = fb.cm[m];
+NH_HD inline void qin_of_cell(const FaceBounds& fb, int m, real& lo, real& hi) {
+  const real c = fb.cm[m];
   if (m < 3) {
-    lo = c * fb.mn[m] + (1.0 - c) * NH_BIG;
-    hi = c * fb.mx[m] + (1.0 - c) * (-NH_BIG);
+    lo = c * fb.mn[m] + (real(1.0) - c) * NH_BIG;
+    hi = c * fb.mx[m] + (real(1.0) - c) * (-NH_BIG);
   } else if (m == 4 && fb.pent) {
-    lo = (c == 1.0) ? NH_BIG : fb.mn[m];
-    hi = (c == 1.0) ? -NH_BIG : fb.mx[m];
+    lo = (c == real(1.0)) ? NH_BIG : fb.mn[m];
+    hi = (c == real(1.0)) ? -NH_BIG : fb.mx[m];
   } else {
-    lo = c * NH_BIG + (1.0 - c) * fb.mn[m];
-    hi = c * (-NH_BIG) + (1.0 - c) * fb.mx[m];
+    lo = c * NH_BIG + (real(1.0) - c) * fb.mn[m];
+    hi = c * (-NH_BIG) + (real(1.0) - c) * fb.mx[m];
   }
 }
 
 // inflow bounds of the same face as seen from its owner (m >= 3 only)
-NH_HD inline void qin_of_owner(const FaceBounds& fb, int m, double& lo, double& hi) {
-  const double c = fb.cm[m];
+NH_HD inline void qin_of_owner(const FaceBounds& fb, int m, real& lo, real& hi) {
+  const real c = fb.cm[m];
   if (m == 4 && fb.pent) {
-    lo = (c == 1.0) ? fb.mn[m] : NH_BIG;
-    hi = (c == 1.0) ? fb.mx[m] : -NH_BIG;
+    lo = (c == real(1.0)) ? fb.mn[m] : NH_BIG;
+    hi = (c == real(1.0)) ? fb.mx[m] : -NH_BIG;
   } else {
-    lo = c * fb.mn[m] + (1.0 - c) * NH_BIG;
-    hi = c * fb.mx[m] + (1.0 - c) * (-NH_BIG);
+    lo = c * fb.mn[m] + (real(1.0) - c) * NH_BIG;
+    hi = c * fb.mx[m] + (real(1.0) - c) * (-NH_BIG);
   }
 }
 
 // shared per-cell setup: q on the stencil, owner masks, own Courant numbers
 struct TrcCellCtx {
-  double q[7];
-  double cmo[6];    // owner mask per face
-  double ch[6];     // this cell's Courant numbers
-  double f[6];      // this cell's face fluxes
+  real q[7];
+  real cmo[6];    // owner mask per face
+  real ch[6];     // this cell's Courant numbers
+  real f[6];      // this cell's face fluxes
 };
 
-NH_HD inline void trc_cell(const Geo& G, const Dims& d, const Cell& cl, long a, double dt, const double* rq,
-                           const double* rhog1, const double* const* fa, TrcCellCtx& x) {
+NH_HD inline void trc_cell(const Geo& G, const Dims& d, const Cell& cl, long a, real dt, const real* rq,
+                           const real* rhog1, const real* const* fa, TrcCellCtx& x) {
   long off[7];
   stencil_offsets(d.g, off);
   for (int m = 0; m < 7; ++m) x.q[m] = rq[a + off[m]] / rhog1[a + off[m]];
@@ -473,69 +482,69 @@ NH_HD inline void trc_cell(const Geo& G, const Dims& d, const Cell& cl, long a, 
 struct TrcQout {
   static constexpr const char* kName = "trc_qout";
   Dims d; Consts c; Geo G;
-  double dt;
-  const double* rq;
-  const double* rhog1;
-  const double* frhog;
-  const double* fa[3];
-  double* qout[2];
+  real dt;
+  const real* rq;
+  const real* rhog1;
+  const real* frhog;
+  const real* fa[3];
+  real* qout[2];
   NH_HD void operator()(long t) const {
     Cell cl = cell_of(d, t);
     if (cl.pole ? cl.v != 0 : !cl.inner) return;
     for (int k = 0; k < d.kall; ++k) {
       const long a = cl.base + k * cl.ks;
-      const double dk = 1.0 * frhog[a] / rhog1[a] * dt;
+      const real dk = real(1.0) * frhog[a] / rhog1[a] * dt;
       if (cl.pole) {
-        const double ra = G.rarea_pl[cl.lp];
-        const double q0 = rq[a] / rhog1[a];
-        double qv[6];
+        const real ra = G.rarea_pl[cl.lp];
+        const real q0 = rq[a] / rhog1[a];
+        real qv[6];
         for (int v = 1; v <= 5; ++v) qv[v] = rq[a + v] / rhog1[a + v];
-        double nmin = q0, nmax = q0, Cin = 0.0, Cout = 0.0, CQmin = 0.0, CQmax = 0.0;
+        real nmin = q0, nmax = q0, Cin = real(0.0), Cout = real(0.0), CQmin = real(0.0), CQmax = real(0.0);
         for (int v = 1; v <= 5; ++v) {
           const int vp = (v == 5) ? 1 : v + 1, vm = (v == 1) ? 5 : v - 1;
-          const double mn = dmin(dmin(dmin(q0, qv[v]), qv[vm]), qv[vp]);
-          const double mx = dmax(dmax(dmax(q0, qv[v]), qv[vm]), qv[vp]);
-          const double ch = fa[0][a + v] * ra * dt / rhog1[a];
-          const double cm = upwind_mask(ch);
-          const double lo = (cm == 1.0) ? mn : NH_BIG;
-          const double hi = (cm == 1.0) ? mx : -NH_BIG;
+          const real mn = dmin(dmin(dmin(q0, qv[v]), qv[vm]), qv[vp]);
+          const real mx = dmax(dmax(dmax(q0, qv[v]), qv[vm]), qv[vp]);
+          const real ch = fa[0][a + v] * ra * dt / rhog1[a];
+          const real cm = upwind_mask(ch);
+          const real lo = (cm == real(1.0)) ? mn : NH_BIG;
+          const real hi = (cm == real(1.0)) ? mx : -NH_BIG;
           nmin = dmin(nmin, lo);
           nmax = dmax(nmax, hi);
-          const double chm = cm * ch;
+          const real chm = cm * ch;
           Cin = Cin + chm;
           Cout = Cout + (ch - chm);
           CQmin = CQmin + chm * lo;
           CQmax = CQmax + chm * hi;
         }
-        const double zsw = 0.5 - copysign(0.5, fabs(Cout) - NH_EPS);
-        const double den = Cout + zsw, fac = 1.0 - zsw;
-        qout[0][a] = (q0 - CQmax - nmax * (1.0 - Cin - Cout + dk)) / den * fac + q0 * zsw;
-        qout[1][a] = (q0 - CQmin - nmin * (1.0 - Cin - Cout + dk)) / den * fac + q0 * zsw;
+        const real zsw = real(0.5) - nsign(real(0.5), nabs(Cout) - NH_EPS);
+        const real den = Cout + zsw, fac = real(1.0) - zsw;
+        qout[0][a] = (q0 - CQmax - nmax * (real(1.0) - Cin - Cout + dk)) / den * fac + q0 * zsw;
+        qout[1][a] = (q0 - CQmin - nmin * (real(1.0) - Cin - Cout + dk)) / den * fac + q0 * zsw;
         continue;
       }
       TrcCellCtx x;
       trc_cell(G, d, cl, a, dt, rq, rhog1, fa, x);
       FaceBounds fb;
       face_bounds(x.q, x.cmo, cl.i == 1 && cl.j == 1 && G.sgp[cl.l], fb);
-      double nmin = x.q[0], nmax = x.q[0];
-      double lo[6], hi[6];
+      real nmin = x.q[0], nmax = x.q[0];
+      real lo[6], hi[6];
       for (int m = 0; m < 6; ++m) {
         qin_of_cell(fb, m, lo[m], hi[m]);
         nmin = dmin(nmin, lo[m]);
         nmax = dmax(nmax, hi[m]);
       }
-      double Cin = -0.0, Cout = -0.0, CQmin = -0.0, CQmax = -0.0;
+      real Cin = -real(0.0), Cout = -real(0.0), CQmin = -real(0.0), CQmax = -real(0.0);
       for (int m = 0; m < 6; ++m) {
-        const double chm = (x.ch[m] < 0.0) ? x.ch[m] : 0.0;
+        const real chm = (x.ch[m] < real(0.0)) ? x.ch[m] : real(0.0);
         Cin = Cin + chm;
         Cout = Cout + (x.ch[m] - chm);
         CQmin = CQmin + chm * lo[m];
         CQmax = CQmax + chm * hi[m];
       }
-      const double zsw = 0.5 - copysign(0.5, fabs(Cout) - NH_EPS);
-      const double qq = x.q[0];
-      qout[0][a] = (qq - CQmax - nmax * (1.0 - Cin - Cout + dk)) / (Cout + zsw) * (1.0 - zsw) + qq * zsw;
-      qout[1][a] = (qq - CQmin - nmin * (1.0 - Cin - Cout + dk)) / (Cout + zsw) * (1.0 - zsw) + qq * zsw;
+      const real zsw = real(0.5) - nsign(real(0.5), nabs(Cout) - NH_EPS);
+      const real qq = x.q[0];
+      qout[0][a] = (qq - CQmax - nmax * (real(1.0) - Cin - Cout + dk)) / (Cout + zsw) * (real(1.0) - zsw) + qq * zsw;
+      qout[1][a] = (qq - CQmin - nmin * (real(1.0) - Cin - Cout + dk)) / (Cout + zsw) * (real(1.0) - zsw) + qq * zsw;
     }
   }
 };
@@ -544,13 +553,13 @@ struct TrcQout {
 struct TrcApply {
   static constexpr const char* kName = "trc_apply";
   Dims d; Consts c; Geo G;
-  double dt;
-  const double* rq;
-  const double* rhog1;
-  const double* fa[3];
-  const double* qa[3];
-  const double* qout[2];
-  double* out;
+  real dt;
+  const real* rq;
+  const real* rhog1;
+  const real* fa[3];
+  const real* qa[3];
+  const real* qout[2];
+  real* out;
   NH_HD void operator()(long t) const {
     Cell cl = cell_of(d, t);
     if (cl.pole ? cl.v != 0 : !cl.inner) return;
@@ -558,25 +567,25 @@ struct TrcApply {
     for (int k = 0; k < d.kall; ++k) {
       const long a = cl.base + k * cl.ks;
       if (cl.pole) {
-        const double ra = G.rarea_pl[cl.lp];
-        const double q0 = rq[a] / rhog1[a];
-        double qv[6];
+        const real ra = G.rarea_pl[cl.lp];
+        const real q0 = rq[a] / rhog1[a];
+        real qv[6];
         for (int v = 1; v <= 5; ++v) qv[v] = rq[a + v] / rhog1[a + v];
-        double r = rq[a];
+        real r = rq[a];
         for (int v = 1; v <= 5; ++v) {
           const int vp = (v == 5) ? 1 : v + 1, vm = (v == 1) ? 5 : v - 1;
-          const double mn = dmin(dmin(dmin(q0, qv[v]), qv[vm]), qv[vp]);
-          const double mx = dmax(dmax(dmax(q0, qv[v]), qv[vm]), qv[vp]);
-          const double fl = fa[0][a + v] * ra * dt;
-          const double cm = upwind_mask(fl / rhog1[a]);
-          const double lo0 = (cm == 1.0) ? mn : NH_BIG, hi0 = (cm == 1.0) ? mx : -NH_BIG;
-          const double lo1 = (cm == 1.0) ? NH_BIG : mn, hi1 = (cm == 1.0) ? -NH_BIG : mx;
-          double s = qa[0][a + v];
-          const double s0 = clip(s, lo0, hi0), s1 = clip(s, lo1, hi1);
-          s = cm * s0 + (1.0 - cm) * s1;
-          const double s2 = dmax(dmin(s, qout[1][a + v]), qout[0][a + v]);
-          const double s3 = dmax(dmin(s, qout[1][a]), qout[0][a]);
-          s = cm * s2 + (1.0 - cm) * s3;
+          const real mn = dmin(dmin(dmin(q0, qv[v]), qv[vm]), qv[vp]);
+          const real mx = dmax(dmax(dmax(q0, qv[v]), qv[vm]), qv[vp]);
+          const real fl = fa[0][a + v] * ra * dt;
+          const real cm = upwind_mask(fl / rhog1[a]);
+          const real lo0 = (cm == real(1.0)) ? mn : NH_BIG, hi0 = (cm == real(1.0)) ? mx : -NH_BIG;
+          const real lo1 = (cm == real(1.0)) ? NH_BIG : mn, hi1 = (cm == real(1.0)) ? -NH_BIG : mx;
+          real s = qa[0][a + v];
+          const real s0 = clip(s, lo0, hi0), s1 = clip(s, lo1, hi1);
+          s = cm * s0 + (real(1.0) - cm) * s1;
+          const real s2 = dmax(dmin(s, qout[1][a + v]), qout[0][a + v]);
+          const real s3 = dmax(dmin(s, qout[1][a]), qout[0][a]);
+          s = cm * s2 + (real(1.0) - cm) * s3;
           r -= fl * s;
         }
         out[a] = r;
@@ -590,23 +599,23 @@ struct TrcApply {
       const long own[6] = {a, a, a, a - 1, a - 1 - g, a - g};
       const long dst[6] = {a + 1, a + 1 + g, a + g, a, a, a};
       const int arc[6] = {0, 1, 2, 0, 1, 2};
-      double s_face[6];
+      real s_face[6];
       for (int m = 0; m < 6; ++m) {
-        const double cm = fb.cm[m];
-        double loA, hiA, loB, hiB;     // A: owner side, B: downstream side
+        const real cm = fb.cm[m];
+        real loA, hiA, loB, hiB;     // A: owner side, B: downstream side
         if (m < 3) {
           qin_of_cell(fb, m, loA, hiA);
           // the downstream cell sees the complementary mask
-          loB = cm * NH_BIG + (1.0 - cm) * fb.mn[m];
-          hiB = cm * (-NH_BIG) + (1.0 - cm) * fb.mx[m];
+          loB = cm * NH_BIG + (real(1.0) - cm) * fb.mn[m];
+          hiB = cm * (-NH_BIG) + (real(1.0) - cm) * fb.mx[m];
         } else {
           qin_of_owner(fb, m, loA, hiA);
           qin_of_cell(fb, m, loB, hiB);
         }
-        const double s = qa[arc[m]][own[m]];
-        const double v1 = cm * clip(s, loA, hiA) + (1.0 - cm) * clip(s, loB, hiB);
+        const real s = qa[arc[m]][own[m]];
+        const real v1 = cm * clip(s, loA, hiA) + (real(1.0) - cm) * clip(s, loB, hiB);
         const long o = own[m], dn = dst[m];
-        s_face[m] = cm * dmax(dmin(v1, qout[1][dn]), qout[0][dn]) + (1.0 - cm) * dmax(dmin(v1, qout[1][o]), qout[0][o]);
+        s_face[m] = cm * dmax(dmin(v1, qout[1][dn]), qout[0][dn]) + (real(1.0) - cm) * dmax(dmin(v1, qout[1][o]), qout[0][o]);
       }
       out[a] = rq[a] - (x.f[0] * s_face[0] + x.f[1] * s_face[1] + x.f[2] * s_face[2] + x.f[3] * s_face[3]
                         + x.f[4] * s_face[4] + x.f[5] * s_face[5]);
@@ -614,4 +623,4 @@ struct TrcApply {
   }
 };
 
-}  // namespace nh
+NH_END

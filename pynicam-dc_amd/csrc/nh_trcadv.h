@@ -2,16 +2,17 @@
 // nhm/forcing/mod_forcing.py:113-206): the velocity of an analytic flow at
 // time t is written into prog as rhogv = v rhog, rhogw = w rhog at every
 // point (halo and ghost levels included); the density is left unchanged.
+// The flow is evaluated in FP64 and rounded to the field precision on store.
 #pragma once
 
 #include "nh_columns.h"
 
-namespace nh {
+NH_BEGIN
 
 struct TrcadvWindsP {
   static constexpr const char* kName = "trcadv_winds";
   Dims d; Geo G;
-  double* prog;
+  real* prog;
   double time;
   int kind;              // 0: DCMIP 1-1, 1: deformational flow
   double par[4];         // kind 1: period, umax, w0, ztop
@@ -34,7 +35,7 @@ struct TrcadvWindsP {
     const double ptop = p0 * exp(-12000.0 / H);
     const double bp = bs * ptop;
     {
-      const double p = p0 * exp(-G.gz[k] / H);
+      const double p = p0 * exp(-G.v64.gz[k] / H);
       const double cl = cos(lat);
       const double ud = (omega0 * a) / bp * cos(lonp) * (cl * cl) * cos(dlon) *
                         (-exp((p - p0) / bp) + exp((ptop - p) / bp));
@@ -43,7 +44,7 @@ struct TrcadvWindsP {
       v = k0 * sin(2.0 * lonp) * cl * cos(0.5 * dlon);
     }
     {
-      const double p = p0 * exp(-G.gzh[k] / H);
+      const double p = p0 * exp(-G.v64.gzh[k] / H);
       const double s = 1.0 + exp((ptop - p0) / bp) - exp((p - p0) / bp) - exp((ptop - p) / bp);
       w = -(Rd * T0) / (grav * p) * omega0 * sin(lonp) * cos(lat) * cos(dlon) * s;
     }
@@ -58,7 +59,7 @@ struct TrcadvWindsP {
     const double sl = sin(lon);
     u = umax * (sl * sl) * sin(2.0 * lat) * ct;
     v = umax * sin(2.0 * lon) * cos(lat) * ct;
-    w = (k > d.kmin && k <= d.kmax) ? w0 * sin(pi * G.gzh[k] / ztop) * cos(2.0 * pi * time / T) : 0.0;
+    w = (k > d.kmin && k <= d.kmax) ? w0 * sin(pi * G.v64.gzh[k] / ztop) * cos(2.0 * pi * time / T) : 0.0;
   }
 
   NH_HD void operator()(int plane, int x) const {
@@ -80,18 +81,18 @@ struct TrcadvWindsP {
     const double nx = -sin(lat) * cos(lon), ny = -sin(lat) * sin(lon), nz = cos(lat);
     const long NF = d.NF, a = p.a;
     const double rhog = prog[a];
-    prog[1 * NF + a] = (ex * u + nx * v) * rhog;
-    prog[2 * NF + a] = (ey * u + ny * v) * rhog;
-    prog[3 * NF + a] = (ez * u + nz * v) * rhog;
-    prog[4 * NF + a] = w * rhog;
+    prog[1 * NF + a] = (real)((ex * u + nx * v) * rhog);
+    prog[2 * NF + a] = (real)((ey * u + ny * v) * rhog);
+    prog[3 * NF + a] = (real)((ez * u + nz * v) * rhog);
+    prog[4 * NF + a] = (real)(w * rhog);
   }
 };
 
 // zero a field (density tendency of the offline tracer step)
 struct FillZero {
   static constexpr const char* kName = "fill_zero";
-  double* p;
-  NH_HD void operator()(long t) const { p[t] = 0.0; }
+  real* p;
+  NH_HD void operator()(long t) const { p[t] = real(0.0); }
 };
 
-}  // namespace nh
+NH_END

@@ -1,0 +1,42 @@
+// The interface each precision variant of the library implements (nh_impl.h,
+// compiled once per variant) and the C-ABI layer (nh_api.h) dispatches to.
+// Field pointers are untyped here: the variant knows its field precision.
+// Every function returns 0 or a negative NH_ERR_* code and sets *err.
+#pragma once
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/nicamhip.h"
+
+#define NH_VARIANT_API(NS)                                                                                  \
+  namespace nh {                                                                                            \
+  namespace NS {                                                                                            \
+  struct Context;                                                                                           \
+  int v_create(const nh_dims* dims, const nh_consts* consts, int device, Context** out, std::string* err); \
+  void v_destroy(Context* c);                                                                               \
+  bool v_geo_ready(const Context* c);                                                                       \
+  long v_halo_n(const Context* c);                                                                          \
+  int v_upload_geometry(Context* c, const nh_geometry* g, std::string* err);                               \
+  int v_upload_halo(Context* c, const int32_t* dst, const int32_t* src, long n, std::string* err);         \
+  int v_halo_fill(Context* c, void* const* fields, int nf, void* stream, std::string* err);                \
+  int v_laplacian(Context* c, const void* scl, void* out, void* stream, std::string* err);                 \
+  int v_divergence(Context* c, const void* vx, const void* vy, const void* vz, void* out, void* stream,     \
+                   std::string* err);                                                                       \
+  int v_gradient(Context* c, const void* scl, void* gx, void* gy, void* gz, void* stream, std::string* err); \
+  int v_dyn_step(Context* c, void* prog, void* progq, int nq, double dtl, int sstep_max, void* stream,      \
+                 std::string* err);                                                                         \
+  int v_tracer_step(Context* c, const void* prog, void* progq, int nq, double dtl, void* stream,            \
+                    std::string* err);                                                                      \
+  int v_trcadv_winds(Context* c, void* prog, double time, int kind, const double* params, void* stream,     \
+                     std::string* err);                                                                     \
+  int v_budget(Context* c, const void* prog, const void* progq, double lhv, double* out, void* stream,      \
+               std::string* err);                                                                           \
+  int v_prof_enable(Context* c, bool on, std::string* err);                                                 \
+  int v_prof_report(Context* c, int max_kernels, char* names, double* total_ms, int64_t* launches,          \
+                    std::string* err);                                                                      \
+  }                                                                                                         \
+  }
+
+NH_VARIANT_API(f64)
+NH_VARIANT_API(f32)

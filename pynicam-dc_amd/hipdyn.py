@@ -1,7 +1,9 @@
 """Device-resident dynamical core behind the nicamhip C-ABI.
 
 ``HipContext`` owns one library context (geometry, halo map, step workspace)
-on one device; ``HipDynamics`` owns the prognostic state as torch tensors in
+on one device, in one of two forms: ``precision="float64"`` (the drop-in) or
+``"mixed"`` (float fields and stencils; FP64 tridiagonal solve, global sums and
+time steps).  ``HipDynamics`` owns the prognostic state as torch tensors in
 the device layout and advances it with ``dynamics_step()``, the counterpart of
 ``Dyn.dynamics_step(msc)`` (nhm/dynamics/mod_dynamics.py:1725-2741).
 Operators (``OPRT_laplacian``, ``OPRT_gradient``, divergence, halo fill) are
@@ -33,8 +35,16 @@ class HipContext:
     test build of the library (``lib_path``).
     """
 
-    def __init__(self, geo, const, device: str = "cuda:0", lib_path: str | None = None):
+    PRECISIONS = {"float64": native.PREC_FP64, "mixed": native.PREC_MIXED}
+
+    def __init__(self, geo, const, device: str = "cuda:0", lib_path: str | None = None,
+                 precision: str = "float64"):
         torch = _torch()
+        if precision not in self.PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(self.PRECISIONS)}, not {precision!r}")
+        self.precision = precision
+        self.mixed = precision == "mixed"
+        self.dtype = torch.float32 if self.mixed else torch.float64
         self.geo = geo
         self.const = const
         self.topo = t = geo.topo
@@ -57,8 +67,8 @@ class HipContext:
         if self.device.type == "cuda":
             torch.cuda.set_device(dev_index)
         h = C.c_void_p()
-        native.check(lib, lib.nh_create(C.byref(self.dims), C.byref(self.consts), dev_index, C.byref(h)),
-                     "nh_create")
+        native.check(lib, lib.nh_create_ex(C.byref(self.dims), C.byref(self.consts), dev_index,
+                                           self.PRECISIONS[precision], C.byref(h)), "nh_create")
         self.handle = h
         # geometry (host arrays kept alive until the synchronous upload returns)
         arrs = layout.geometry_arrays(geo)
@@ -76,17 +86,21 @@ class HipContext:
                                                  dst.size), "nh_upload_halo_map")
 
     # ------------------------------------------------------------------
+    def fn(self, name: str):
+        """The library entry point for this context's field precision (``name`` or ``name_f32``)."""
+        return getattr(self.lib, name + "_f32" if self.mixed else name)
+
     def stream(self):
         if self.device.type == "cuda":
             return C.c_void_p(_torch().cuda.current_stream(self.device).cuda_stream)
         return C.c_void_p(None)
 
     def empty(self, nv: int):
-        return _torch().empty((nv, self.NF), dtype=_torch().float64, device=self.device)
+        return _torch().empty((nv, self.NF), dtype=self.dtype, device=self.device)
 
     def to_device(self, var: np.ndarray, var_pl: np.ndarray):
         """Reference layout (g,g,kall,l,nv)+(6,kall,2,nv) -> device tensor (nv, NF)."""
-        return _torch().from_numpy(layout.pack(var, var_pl)).to(self.device)
+        return _torch().from_numpy(layout.pack(var, var_pl)).to(device=self.device, dtype=self.dtype)
 
     def to_host(self, flat):
         t = self.topo
@@ -133,13 +147,13 @@ class HipContext:
         self._check_tensor(fields)
         nv = fields.shape[0]
         ptrs = (C.c_void_p * nv)(*[fields[q].data_ptr() for q in range(nv)])
-        native.check(self.lib, self.lib.nh_halo_fill(self.handle, ptrs, nv, self.stream()), "nh_halo_fill")
+        native.check(self.lib, self.fn("nh_halo_fill")(self.handle, ptrs, nv, self.stream()), "nh_halo_fill")
 
     def OPRT_laplacian(self, scl, out=None):
         """7-point Laplacian (share/mod_oprt.py:1429)."""
         self._check_tensor(scl)
         out = _torch().zeros_like(scl) if out is None else out
-        native.check(self.lib, self.lib.nh_laplacian(self.handle, C.c_void_p(scl.data_ptr()),
+        native.check(self.lib, self.fn("nh_laplacian")(self.handle, C.c_void_p(scl.data_ptr()),
                                                      C.c_void_p(out.data_ptr()), self.stream()), "nh_laplacian")
         return out
 
@@ -148,7 +162,7 @@ class HipContext:
         for a in (vx, vy, vz):
             self._check_tensor(a)
         out = _torch().zeros_like(vx) if out is None else out
-        native.check(self.lib, self.lib.nh_divergence(self.handle, *(C.c_void_p(a.data_ptr()) for a in
+        native.check(self.lib, self.fn("nh_divergence")(self.handle, *(C.c_void_p(a.data_ptr()) for a in
                                                                      (vx, vy, vz, out)), self.stream()),
                      "nh_divergence")
         return out
@@ -157,15 +171,15 @@ class HipContext:
         """7-point gradient (share/mod_oprt.py:1387); out is (3, NF)."""
         self._check_tensor(scl)
         out = _torch().zeros((3, self.NF), dtype=scl.dtype, device=scl.device) if out is None else out
-        native.check(self.lib, self.lib.nh_gradient(self.handle, C.c_void_p(scl.data_ptr()),
+        native.check(self.lib, self.fn("nh_gradient")(self.handle, C.c_void_p(scl.data_ptr()),
                                                     *(C.c_void_p(out[q].data_ptr()) for q in range(3)),
                                                     self.stream()), "nh_gradient")
         return out
 
     def _check_tensor(self, a):
         torch = _torch()
-        if a.dtype != torch.float64 or a.device != self.device or a.shape[-1] != self.NF or not a.is_contiguous():
-            raise ValueError(f"expected a contiguous float64 tensor (..., {self.NF}) on {self.device}")
+        if a.dtype != self.dtype or a.device != self.device or a.shape[-1] != self.NF or not a.is_contiguous():
+            raise ValueError(f"expected a contiguous {self.dtype} tensor (..., {self.NF}) on {self.device}")
 
 
 class HipDynamics:
@@ -183,7 +197,7 @@ class HipDynamics:
 
     def dynamics_step(self):
         lib = self.ctx.lib
-        native.check(lib, lib.nh_dyn_step(self.ctx.handle, C.c_void_p(self.prog.data_ptr()),
+        native.check(lib, self.ctx.fn("nh_dyn_step")(self.ctx.handle, C.c_void_p(self.prog.data_ptr()),
                                           C.c_void_p(self.progq.data_ptr()), self.progq.shape[0],
                                           self.dtl, self.sstep_max, self.ctx.stream()), "nh_dyn_step")
 
@@ -196,7 +210,7 @@ class HipDynamics:
         lib, c = self.ctx.lib, self.ctx.const
         out = (C.c_double * 6)()
         lhv = float(c.LHV if lhv is None else lhv)
-        native.check(lib, lib.nh_budget(self.ctx.handle, C.c_void_p(self.prog.data_ptr()),
+        native.check(lib, self.ctx.fn("nh_budget")(self.ctx.handle, C.c_void_p(self.prog.data_ptr()),
                                         C.c_void_p(self.progq.data_ptr()), self.progq.shape[0], lhv, out,
                                         self.ctx.stream()), "nh_budget")
         return dict(zip(("pot", "int", "kin", "tot", "dry", "vap"), list(out)))
@@ -239,12 +253,12 @@ class HipOfflineTracer(HipDynamics):
 
     def set_winds(self, time: float):
         lib = self.ctx.lib
-        native.check(lib, lib.nh_trcadv_winds(self.ctx.handle, C.c_void_p(self.prog.data_ptr()), float(time),
+        native.check(lib, self.ctx.fn("nh_trcadv_winds")(self.ctx.handle, C.c_void_p(self.prog.data_ptr()), float(time),
                                               self.kind, self._params, self.ctx.stream()), "nh_trcadv_winds")
 
     def dynamics_step(self):
         lib = self.ctx.lib
-        native.check(lib, lib.nh_tracer_step(self.ctx.handle, C.c_void_p(self.prog.data_ptr()),
+        native.check(lib, self.ctx.fn("nh_tracer_step")(self.ctx.handle, C.c_void_p(self.prog.data_ptr()),
                                              C.c_void_p(self.progq.data_ptr()), self.progq.shape[0], self.dtl,
                                              self.ctx.stream()), "nh_tracer_step")
         self.time += self.dtl

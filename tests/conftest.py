@@ -12,7 +12,8 @@ if ROOT not in sys.path:
 
 PKG = "pynicam-dc_amd"
 GOLDEN = os.path.join(ROOT, "tests", "golden")
-HOSTEMU_SRC = os.path.join(ROOT, "tests", "hostemu", "nh_hostemu.cpp")
+HOSTEMU_SRC = [os.path.join(ROOT, "tests", "hostemu", f) for f in ("nh_hostemu.cpp", "nh_hostemu_f32.cpp")]
+HOSTEMU_HDR = os.path.join(ROOT, "tests", "hostemu", "nh_hostexec.h")
 HOSTEMU_LIB = os.path.join(ROOT, "tests", "_build", "libnh_hostemu.so")
 
 
@@ -50,13 +51,13 @@ def _newer(target, *deps):
 def hostemu_lib():
     """Host (g++) build of the kernel sources, for CPU tests of the kernel logic."""
     csrc = os.path.join(ROOT, PKG, "csrc")
-    deps = [HOSTEMU_SRC, os.path.join(ROOT, "include", "nicamhip.h")] + [
+    deps = HOSTEMU_SRC + [HOSTEMU_HDR, os.path.join(ROOT, "include", "nicamhip.h")] + [
         os.path.join(csrc, f) for f in os.listdir(csrc) if f.endswith(".h")]
     if not _newer(HOSTEMU_LIB, *deps):
         os.makedirs(os.path.dirname(HOSTEMU_LIB), exist_ok=True)
         tmp = HOSTEMU_LIB + ".tmp"
         subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared",
-                        "-o", tmp, HOSTEMU_SRC], check=True)
+                        "-o", tmp, *HOSTEMU_SRC], check=True)
         os.replace(tmp, HOSTEMU_LIB)
     return HOSTEMU_LIB
 

@@ -230,3 +230,48 @@ def test_offline_tracer_gl06_z60_bounds_and_mass(gpu):
         assert q.min() >= -1e-14 and q.max() <= 1.0 + 1e-13, (q.min(), q.max())
     finally:
         ctx.close()
+
+
+# Mixed form (float fields and stencils, FP64 tridiagonal solve) against the FP64
+# oracle: largest error relative to the largest value of each prognostic, after
+# one gl05 z40 step (tolerances as in the host-build test of the same form).
+MIXED_TOL = {0: 1e-5, 1: 3e-5, 2: 3e-5, 3: 3e-3, 4: 1e-3, 5: 1e-5}
+
+
+@pytest.fixture(scope="module")
+def ctx5_mixed(gpu, case_gl05):
+    ctx = hipdyn.HipContext(case_gl05.geo, case_gl05.const, device="cuda:0", precision="mixed")
+    yield ctx
+    ctx.close()
+
+
+def test_mixed_step_gl05_close_to_oracle(gpu, ctx5_mixed, case_gl05):
+    from oracle import dynamics as od
+    t = case_gl05.topo
+    print("  mixed gl05 z40 step with blob tracers", flush=True)
+    state = mod("initial").blob_tracers(case_gl05.geo, case_gl05.state, ntracer=3, seed=0)
+    ref = od.OracleDynamics(od.OracleContext(case_gl05.geo, case_gl05.const, None), state)
+    dyn = hipdyn.HipDynamics(ctx5_mixed, state)
+    assert dyn.prog.dtype == gpu.float32
+    ref.step()
+    dyn.dynamics_step()
+    got = dyn.state()
+    errs = {}
+    for v, tol in MIXED_TOL.items():
+        a, b = interior(ref.PROG[..., v], t), interior(got["PROG"][..., v].astype(np.float64), t)
+        errs[v] = np.abs(a - b).max() / np.abs(a).max()
+    print("  mixed max rel errors", errs, flush=True)
+    for v, tol in MIXED_TOL.items():
+        assert errs[v] < tol, (v, errs[v])
+    for q in range(1, got["PROGq"].shape[-1]):
+        a, b = interior(ref.PROGq[..., q], t), interior(got["PROGq"][..., q].astype(np.float64), t)
+        assert np.abs(a - b).max() / np.abs(a).max() < 1e-5, q
+    # the FP64 global sums of the float state against the oracle's sums of the same float state
+    from oracle import budget as ob
+    st64 = {k: v.astype(np.float64) for k, v in got.items()}
+    want = ob.store_var(od.OracleContext(case_gl05.geo, case_gl05.const, None), st64["PROG"], st64["PROGq"],
+                        case_gl05.const.LHV)
+    b = dyn.budget()
+    for key, v in want.items():
+        if v != 0.0:
+            assert abs(b[key] - v) <= 1e-13 * abs(v), (key, b[key], v)

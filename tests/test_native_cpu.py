@@ -201,3 +201,78 @@ def test_trcadv_winds_argument_errors(emu_ctx):
     assert lib.nh_trcadv_winds(emu_ctx.handle, C.c_void_p(prog.data_ptr()), 0.0, 7, None, C.c_void_p(None)) == -1
     with pytest.raises(ValueError):
         hipdyn.HipOfflineTracer(emu_ctx, {"PROG": np.zeros(1)}, "nope")
+
+
+# Mixed form (float fields, FP64 solve) against the FP64 oracle: largest error
+# relative to the largest value of each prognostic.  Measured on the host build
+# after two gl04 steps: rhog 2.7e-7, rhogvx/vy 1.0e-6, rhogvz 1.4e-4 (a small
+# component: meridional flow only), rhogw 4.1e-5, rhoge 2.1e-7; tracers 2.6e-7.
+MIXED_TOL = {0: 1e-5, 1: 3e-5, 2: 3e-5, 3: 3e-3, 4: 1e-3, 5: 1e-5}
+MIXED_QTOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def emu_mixed(case_gl04, hostemu_lib):
+    ctx = hipdyn.HipContext(case_gl04.geo, case_gl04.const, device="cpu", lib_path=hostemu_lib, precision="mixed")
+    yield ctx
+    ctx.close()
+
+
+def test_host_build_mixed_step_close_to_oracle(case_gl04, state_gl04_tracers, emu_mixed):
+    from oracle import dynamics as od
+    c = case_gl04
+    t = c.topo
+    ref = od.OracleDynamics(od.OracleContext(c.geo, c.const, None), state_gl04_tracers)
+    dyn = hipdyn.HipDynamics(emu_mixed, state_gl04_tracers)
+    assert dyn.prog.dtype == torch_dtype("float32")
+    for _ in range(2):
+        ref.step()
+        dyn.dynamics_step()
+    got = dyn.state()
+    for v, tol in MIXED_TOL.items():
+        a, b = interior(ref.PROG[..., v], t), interior(got["PROG"][..., v].astype(np.float64), t)
+        assert np.abs(a - b).max() / np.abs(a).max() < tol, v
+    for q in range(1, got["PROGq"].shape[-1]):
+        a, b = interior(ref.PROGq[..., q], t), interior(got["PROGq"][..., q].astype(np.float64), t)
+        assert np.abs(a - b).max() / np.abs(a).max() < MIXED_QTOL, q
+
+
+def test_host_build_mixed_operators(case_gl04, emu_mixed):
+    """FP32 Laplacian and gradient against the FP64 oracle: relative 1e-5 of the result's range."""
+    import torch
+    from oracle import dynamics as od
+    geo = case_gl04.geo
+    t = geo.topo
+    g, kall, L = t.gall_1d, t.kall, t.lall
+    ctx = od.OracleContext(geo, case_gl04.const, None)
+    rng = np.random.default_rng(2)
+    f = rng.standard_normal((g, g, kall, L))
+    fp = rng.standard_normal((6, kall, 2))
+    T = torch.from_numpy(layout.pack1(f, fp)).to(torch.float32)
+    s = slice(1, g - 1)
+    lo, _ = od.laplacian(ctx, f, fp)
+    a, _ = layout.unpack1(emu_mixed.OPRT_laplacian(T).double().numpy(), g, kall, L)
+    assert np.abs(a[s, s] - lo[s, s]).max() < 1e-5 * np.abs(lo[s, s]).max()
+    go, _ = od.gradient(ctx, f, fp)
+    G = emu_mixed.OPRT_gradient(T).double().numpy()
+    a, _ = layout.unpack1(G[0], g, kall, L)
+    assert np.abs(a[s, s] - go[s, s, ..., 0]).max() < 1e-5 * np.abs(go[s, s, ..., 0]).max()
+
+
+def test_entry_point_must_match_the_context_precision(emu_ctx, emu_mixed):
+    lib = emu_ctx.lib
+    p64, p32 = emu_ctx.empty(6), emu_mixed.empty(6)
+    rc = lib.nh_dyn_step_f32(emu_ctx.handle, C.c_void_p(p64.data_ptr()), C.c_void_p(p64.data_ptr()), 1, 1200.0, 6,
+                             C.c_void_p(None))
+    assert rc == -2 and b"_f32" in lib.nh_last_error()
+    rc = lib.nh_dyn_step(emu_mixed.handle, C.c_void_p(p32.data_ptr()), C.c_void_p(p32.data_ptr()), 1, 1200.0, 6,
+                         C.c_void_p(None))
+    assert rc == -2
+    assert lib.nh_precision(emu_ctx.handle) == 0 and lib.nh_precision(emu_mixed.handle) == 1
+    with pytest.raises(ValueError):
+        hipdyn.HipContext(None, None, device="cpu", precision="half")
+
+
+def torch_dtype(name):
+    import torch
+    return getattr(torch, name)
