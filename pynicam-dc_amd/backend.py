@@ -23,6 +23,7 @@ class Backend:
         self.xp = None
         self.ndtype = None
         self.dtype = None
+        self.precision = None
         self.lib = None
 
     def configure(self, backend_name: str, precision: str, lib_path: str | None = None):
@@ -31,10 +32,13 @@ class Backend:
         if backend_name != "hip":
             raise NotImplementedError(
                 f"backend {backend_name!r} is the reference's own; this package provides backend='hip'")
-        if precision != "float64":
-            raise NotImplementedError("backend='hip' runs the plain FP64 form (precision='float64')")
-        self.ndtype = np.float64
-        self.dtype = np.float64
+        if precision not in ("float64", "mixed", "float32"):
+            raise ValueError(f"backend='hip' precision must be 'float64' or 'mixed', not {precision!r}")
+        # 'float32' selects the mixed form: float fields, with the tridiagonal
+        # solve and the global sums kept in FP64
+        self.precision = "float64" if precision == "float64" else "mixed"
+        self.ndtype = np.float64 if self.precision == "float64" else np.float32
+        self.dtype = self.ndtype
         self.type = "hip"
         self.xp = np
         self.lib = native.load(lib_path)

@@ -185,7 +185,11 @@ def test_budget_gl05(gpu, ctx5, case_gl05):
 
 @pytest.mark.parametrize("wind,params", [("dcmip11", None), ("deform", (12 * 86400.0, 40.0, 0.1, 30000.0))])
 def test_offline_tracer_gl05(gpu, ctx5, case_gl05, wind, params):
-    """Three TRCADV steps (prescribed winds + MIURA2004 transport) at gl05 z40 against the oracle."""
+    """Three TRCADV steps (prescribed winds + MIURA2004 transport) at gl05 z40 against the oracle.
+
+    The winds are evaluated with the device's sin/cos/exp and agree with numpy's to 1e-13; the
+    limiters' switches and min/max selections are not smooth in the winds, so the tracers are held
+    to 1e-10 here (1e-12 where the winds are bitwise equal, as in the dynamics tests)."""
     from oracle import dynamics as od
     from oracle import trcadv as ot
     t = case_gl05.topo
@@ -202,7 +206,7 @@ def test_offline_tracer_gl05(gpu, ctx5, case_gl05, wind, params):
     for q in range(1, got["PROGq"].shape[-1]):
         a, b = interior(ref.PROGq[..., q], t), interior(got["PROGq"][..., q], t)
         ap, bp = pole_centre(ref.PROGq_pl[..., q], t), pole_centre(got["PROGq_pl"][..., q], t)
-        assert rel_err(a, b, ap, bp) < 1e-12, q
+        assert rel_err(a, b, ap, bp) < 1e-10, q
 
 
 def test_offline_tracer_gl06_z60_bounds_and_mass(gpu):

@@ -276,3 +276,16 @@ def test_entry_point_must_match_the_context_precision(emu_ctx, emu_mixed):
 def torch_dtype(name):
     import torch
     return getattr(torch, name)
+
+
+def test_backend_selection(hostemu_lib):
+    """Backend.configure (mod_backend.py:175-283) with the 'hip' branch: both forms, loud failures."""
+    backend = mod("backend")
+    b = backend.Backend().configure("hip", "float64", lib_path=hostemu_lib)
+    assert b.type == "hip" and b.precision == "float64" and b.ndtype == np.float64
+    b = backend.Backend().configure("hip", "float32", lib_path=hostemu_lib)
+    assert b.precision == "mixed" and b.ndtype == np.float32
+    with pytest.raises(ValueError):
+        backend.Backend().configure("cuda", "float64")
+    with pytest.raises(ValueError):
+        backend.Backend().configure("hip", "float16", lib_path=hostemu_lib)
