@@ -5,9 +5,10 @@ TEST INFRASTRUCTURE ONLY: the checker for ``nh_trcadv_winds`` and
 
 * ``test11_velocity`` restates Trcadv.test11_velocity
   (nhm/forcing/mod_af_trcadv.py:23-98, DCMIP 2012 case 1-1);
-* ``deform_velocity`` is the deformational flow of the tracer benchmark
-  (u = U sin^2(lon) sin(2 lat) cos(pi t/T), v = U sin(2 lon) cos(lat) cos(pi t/T),
-  w = w0 sin(pi z/ztop) cos(2 pi t/T) on the inner half levels);
+* ``deform_velocity`` is the deformational flow of the tracer benchmark, the
+  non-divergent case 1 of Nair and Lauritzen (2010) with peak speed U:
+  u = U sin^2(lon/2) sin(2 lat) cos(pi t/T), v = U/2 sin(lon) cos(lat) cos(pi t/T),
+  w = w0 sin(pi z/ztop) cos(2 pi t/T) on the inner half levels;
 * ``set_winds`` restates Forcing.forcing_update (nhm/forcing/mod_forcing.py:113-206);
 * ``OfflineTracer.step`` restates the TRCADV branch of Dyn.dynamics_step
   (nhm/dynamics/mod_dynamics.py:2030-2076) followed by the closing halo fill.
@@ -61,8 +62,8 @@ def deform_velocity(time, lon, lat, zh, inner, params):
     """The benchmark's deformational flow; ``inner`` marks the half levels kmin+1..kmax."""
     T, umax, w0, ztop = params
     ct = np.cos(PI * time / T)
-    u = umax * np.sin(lon) ** 2 * np.sin(2.0 * lat) * ct
-    v = umax * np.sin(2.0 * lon) * np.cos(lat) * ct
+    u = umax * np.sin(0.5 * lon) ** 2 * np.sin(2.0 * lat) * ct
+    v = 0.5 * umax * np.sin(lon) * np.cos(lat) * ct
     east, nrth = _unit_east_north(lon, lat)
     vxyz = [east[..., d] * u + nrth[..., d] * v for d in range(3)]
     w = np.where(inner, w0 * np.sin(PI * zh / ztop) * np.cos(2.0 * PI * time / T), 0.0)

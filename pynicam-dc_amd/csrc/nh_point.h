@@ -726,7 +726,8 @@ struct ViMainH {
 // previous level kept in registers (kernels/virhowsolver.py:42-120).  FP64 in
 // every form: the field values are widened on load.  The forward sweep keeps
 // u and gamma in the FP64 scratch U and Gm; the backward sweep writes the new
-// rhogw split to out on every level (U may be out itself when real is double).
+// rhogw split to out on every level.  U, Gm and out are distinct buffers, so
+// the loads of the next levels can be issued ahead of this level's stores.
 struct ViMainV {
   static constexpr const char* kName = "vi_main_v";
   Dims d; Consts c; Geo G;
@@ -757,12 +758,16 @@ struct ViMainV {
     const double* __restrict__ mc = Mc;
     const double* __restrict__ mu = Mu;
     const double* __restrict__ ml = Ml;
+    double* __restrict__ Uo = U;
+    double* __restrict__ Go = Gm;
+    real* __restrict__ Xo = out;
     const int N = kmax - kmin;
     // level k-1 values, starting at kmin
     long am = b + (long)kmin * ks;
     double pe_m = pe[am], hv1_m = hv1[am], pg_m = pg[am];
     double gr_m = (double)gT0[am] + (-(double)hv0[am] - (0.0 - 0.0) * V.rdgz[kmin]);
     double beta = 0.0, u_prev = 0.0, mu_m = 0.0;
+#pragma unroll 4
     for (int n = 0; n < N; ++n) {
       const int k = kmin + 1 + n;
       const long a = am + ks;
@@ -784,9 +789,9 @@ struct ViMainV {
         const double gm = mu_m / beta;
         beta = mc_k - ml_k * gm;
         u = (S - ml_k * u_prev) / beta;
-        Gm[a] = gm;
+        Go[a] = gm;
       }
-      U[a] = u;
+      Uo[a] = u;
       u_prev = u;
       mu_m = mu_k;
       pe_m = pe_k; hv1_m = hv1_k; pg_m = pg_k; gr_m = gr_k;
@@ -795,17 +800,18 @@ struct ViMainV {
     // backward sweep
     long a = b + (long)kmax * ks;
     double xv = u_prev;
-    out[a] = (real)(xv * 1.0);
-    double gm_next = Gm[a];
+    Xo[a] = (real)(xv * 1.0);
+    double gm_next = Go[a];
+#pragma unroll 4
     for (int n = N - 2; n >= 0; --n) {
       a -= ks;
-      xv = U[a] - gm_next * xv;
-      gm_next = Gm[a];
-      out[a] = (real)(xv * 1.0);
+      xv = Uo[a] - gm_next * xv;
+      gm_next = Go[a];
+      Xo[a] = (real)(xv * 1.0);
     }
-    out[b + (long)kmin * ks] = (real)(0.0 * 1.0);
-    out[b + (long)(kmax + 1) * ks] = (real)(-0.0 * 1.0);
-    out[b + (long)(kmin - 1) * ks] = real(0.0);
+    Xo[b + (long)kmin * ks] = (real)(0.0 * 1.0);
+    Xo[b + (long)(kmax + 1) * ks] = (real)(-0.0 * 1.0);
+    Xo[b + (long)(kmin - 1) * ks] = real(0.0);
   }
 };
 

@@ -50,15 +50,16 @@ struct TrcadvWindsP {
     }
   }
 
-  // non-divergent deformation, reversing at half period, peak speed umax;
-  // w = w0 sin(pi z / ztop) cos(2 pi t / T) on the inner half levels
+  // non-divergent deformation (Nair and Lauritzen 2010, case 1), reversing at
+  // half period, peak speed umax; w = w0 sin(pi z / ztop) cos(2 pi t / T) on
+  // the inner half levels
   NH_HD void deform(double lat, double lon, int k, double& u, double& v, double& w) const {
     const double pi = 3.141592653589793238;
     const double T = par[0], umax = par[1], w0 = par[2], ztop = par[3];
     const double ct = cos(pi * time / T);
-    const double sl = sin(lon);
-    u = umax * (sl * sl) * sin(2.0 * lat) * ct;
-    v = umax * sin(2.0 * lon) * cos(lat) * ct;
+    const double sh = sin(0.5 * lon);
+    u = umax * (sh * sh) * sin(2.0 * lat) * ct;
+    v = 0.5 * umax * sin(lon) * cos(lat) * ct;
     w = (k > d.kmin && k <= d.kmax) ? w0 * sin(pi * G.v64.gzh[k] / ztop) * cos(2.0 * pi * time / T) : 0.0;
   }
 

@@ -234,8 +234,9 @@ class HipOfflineTracer(HipDynamics):
     new time into PROG (``Forcing.forcing_update``, mod_forcing.py:113-206).
     ``wind`` is ``"dcmip11"`` (Trcadv.test11_velocity, mod_af_trcadv.py:23-98)
     or ``"deform"`` with ``params = (period, umax, w0, ztop)``: the
-    non-divergent deformational flow u = umax sin^2(lon) sin(2 lat) cos(pi t/T),
-    v = umax sin(2 lon) cos(lat) cos(pi t/T), w = w0 sin(pi z/ztop) cos(2 pi t/T).
+    non-divergent deformational flow (Nair and Lauritzen 2010, case 1)
+    u = umax sin^2(lon/2) sin(2 lat) cos(pi t/T), v = umax/2 sin(lon) cos(lat) cos(pi t/T),
+    w = w0 sin(pi z/ztop) cos(2 pi t/T).
     The winds of ``time0`` are written at construction.
     """
 

@@ -201,8 +201,13 @@ def test_offline_tracer_gl05(gpu, ctx5, case_gl05, wind, params):
         ref.step()
         run.dynamics_step()
     got = run.state()
-    for v in range(1, 5):
-        assert rel_err(ref.PROG[..., v], got["PROG"][..., v], ref.PROG_pl[..., v], got["PROG_pl"][..., v]) < 1e-13
+    # momentum components against the largest momentum component (a component can vanish identically)
+    mom = max(np.abs(ref.PROG[..., 1:4]).max(), np.abs(ref.PROG_pl[..., 1:4]).max())
+    for v in range(1, 4):
+        e = max(np.abs(ref.PROG[..., v] - got["PROG"][..., v]).max(),
+                np.abs(ref.PROG_pl[..., v] - got["PROG_pl"][..., v]).max())
+        assert e < 1e-13 * mom, v
+    assert rel_err(ref.PROG[..., 4], got["PROG"][..., 4], ref.PROG_pl[..., 4], got["PROG_pl"][..., 4]) < 1e-13
     for q in range(1, got["PROGq"].shape[-1]):
         a, b = interior(ref.PROGq[..., q], t), interior(got["PROGq"][..., q], t)
         ap, bp = pole_centre(ref.PROGq_pl[..., q], t), pole_centre(got["PROGq_pl"][..., q], t)

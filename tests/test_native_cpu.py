@@ -180,8 +180,13 @@ def test_host_build_offline_tracer_matches_oracle(case_gl04, state_gl04_tracers,
     ref = ot.OfflineTracer(od.OracleContext(c.geo, c.const, None), state_gl04_tracers, wind, params)
     run = hipdyn.HipOfflineTracer(emu_ctx, state_gl04_tracers, wind, params)
     got = run.state()
-    for v in range(1, 5):
-        assert rel_err(ref.PROG[..., v], got["PROG"][..., v], ref.PROG_pl[..., v], got["PROG_pl"][..., v]) < 1e-13
+    # momentum components against the largest momentum component (a component can vanish identically)
+    mom = max(np.abs(ref.PROG[..., 1:4]).max(), np.abs(ref.PROG_pl[..., 1:4]).max())
+    for v in range(1, 4):
+        e = max(np.abs(ref.PROG[..., v] - got["PROG"][..., v]).max(),
+                np.abs(ref.PROG_pl[..., v] - got["PROG_pl"][..., v]).max())
+        assert e < 1e-13 * mom, v
+    assert rel_err(ref.PROG[..., 4], got["PROG"][..., 4], ref.PROG_pl[..., 4], got["PROG_pl"][..., 4]) < 1e-13
     for _ in range(3):
         ref.step()
         run.dynamics_step()
