@@ -173,34 +173,35 @@ struct Pt {
 };
 
 NH_HD inline bool pt_of(const Dims& d, int plane, int x, Pt& p) {
+  // every field is computed into a local on both paths and stored once, so
+  // the struct stays in registers
   const int g = d.g;
   const int nrp = d.lall * d.kall;
-  if (plane < nrp) {
-    if (x >= g * g) return false;
-    p.pole = false;
-    p.l = plane / d.kall;
-    p.k = plane - p.l * d.kall;
-    p.j = x / g;
-    p.i = x - p.j * g;
-    p.inner = (p.i >= 1 && p.i <= g - 2 && p.j >= 1 && p.j <= g - 2);
-    p.ks = (long)g * g;
-    p.a = (long)plane * g * g + x;
-    p.ci = (long)p.l * g * g + x;
-    p.v = 0;
-    p.lp = 0;
+  const bool pole = plane >= nrp;
+  if (x >= (pole ? 6 : g * g)) return false;
+  int l = 0, k, i = 0, j = 0, v = 0, lp = 0;
+  long a, ks, ci;
+  if (!pole) {
+    l = plane / d.kall;
+    k = plane - l * d.kall;
+    j = x / g;
+    i = x - j * g;
+    ks = (long)g * g;
+    a = (long)plane * g * g + x;
+    ci = (long)l * g * g + x;
   } else {
-    if (x >= 6) return false;
     const int pp = plane - nrp;
-    p.pole = true;
-    p.inner = false;
-    p.lp = pp / d.kall;
-    p.k = pp - p.lp * d.kall;
-    p.v = x;
-    p.ks = 6;
-    p.a = d.R + (long)pp * 6 + x;
-    p.ci = -1;
-    p.i = p.j = p.l = 0;
+    lp = pp / d.kall;
+    k = pp - lp * d.kall;
+    v = x;
+    ks = 6;
+    a = d.R + (long)pp * 6 + x;
+    ci = -1;
   }
+  p.pole = pole;
+  p.inner = !pole && i >= 1 && i <= g - 2 && j >= 1 && j <= g - 2;
+  p.i = i; p.j = j; p.l = l; p.v = v; p.lp = lp; p.k = k;
+  p.a = a; p.ks = ks; p.ci = ci;
   return true;
 }
 

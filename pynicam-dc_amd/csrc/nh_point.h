@@ -180,12 +180,22 @@ struct AdvMomP {
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p)) return;
-    const long NF = d.NF, a = p.a, ks = p.ks;
+    const long NF = d.NF, a = p.a;
     const int kmin = d.kmin, kmax = d.kmax, k = p.k;
     if (!pt_compute(p) || k < kmin || k > kmax) {
       for (int q = 0; q < 4; ++q) out[q * NF + a] = real(0.0);
       return;
     }
+    // separate instances for the 6-point pole and 7-point regional stencils:
+    // fixed trip counts keep the per-neighbour arrays in registers
+    if (p.pole) point<true>(p);
+    else point<false>(p);
+  }
+  template <bool POLE>
+  NH_HD void point(const Pt& p) const {
+    constexpr int NN = POLE ? 6 : 7;
+    const long NF = d.NF, a = p.a, ks = p.ks;
+    const int kmin = d.kmin, kmax = d.kmax, k = p.k;
     const real R = c.RADIUS;
     const real* rvx = prog + 1 * NF;
     const real* rvy = prog + 2 * NF;
@@ -196,69 +206,92 @@ struct AdvMomP {
     const real* vyd = diag + I_VY * NF;
     const real* vzd = diag + I_VZ * NF;
     const real* wd = diag + I_W * NF;
-    long nb[7];
-    real px[7], py[7], pz[7];
-    const int nn = p.pole ? 6 : 7;
-    if (p.pole) {
-      for (int v = 0; v < 6; ++v) {
-        nb[v] = a + v;
-        px[v] = G.gx_pl[(v * 2 + p.lp) * 3 + 0];
-        py[v] = G.gx_pl[(v * 2 + p.lp) * 3 + 1];
-        pz[v] = G.gx_pl[(v * 2 + p.lp) * 3 + 2];
-      }
-    } else {
-      long off[7];
-      stencil_offsets(d.g, off);
-      for (int m = 0; m < 7; ++m) {
-        nb[m] = a + off[m];
-        const long cim = p.ci + off[m];
-        px[m] = G.gx[0 * d.nreg + cim];
-        py[m] = G.gx[1 * d.nreg + cim];
-        pz[m] = G.gx[2 * d.nreg + cim];
-      }
-    }
-    // merged velocity v + w_full r/|r| of neighbour m at level kk
-    auto vv3 = [&](int m, int kk, real* o) {
-      const long b = nb[m] + (long)(kk - k) * ks;
-      const real wc = G.cfact[kk] * wd[b + ks] + G.dfact[kk] * wd[b];
-      o[0] = vxd[b] + wc * px[m] / R;
-      o[1] = vyd[b] + wc * py[m] / R;
-      o[2] = vzd[b] + wc * pz[m] / R;
+    const long g = d.g;
+    // stencil offset of neighbour m: (di, dj) + 1 packed two bits per point,
+    // (0,0) (+1,0) (+1,+1) (0,+1) (-1,0) (-1,-1) (0,-1)
+    auto off = [&](int m) -> long {
+      const int di = ((4201 >> (2 * m)) & 3) - 1, dj = ((421 >> (2 * m)) & 3) - 1;
+      return di + dj * g;
     };
-    real vvm[7][3];
-    for (int m = 0; m < nn; ++m) vv3(m, k, vvm[m]);
-    real div[3];
-    for (int q = 0; q < 3; ++q) {
-      if (p.pole) {
-        div[q] = div_pl(G, p.lp, [&](int v) { return rvx[nb[v]] * vvm[v][q] * real(1.0); },
-                        [&](int v) { return rvy[nb[v]] * vvm[v][q] * real(1.0); },
-                        [&](int v) { return rvz[nb[v]] * vvm[v][q] * real(1.0); });
+    // address and position of neighbour m
+    auto nbr = [&](int m) { return POLE ? a + m : a + off(m); };
+    auto pos = [&](int m, real* x) {
+      if (POLE) {
+        for (int q = 0; q < 3; ++q) x[q] = G.gx_pl[(m * 2 + p.lp) * 3 + q];
       } else {
-        div[q] = div7(G, d.nreg, p.ci, [&](int m) { return rvx[nb[m]] * vvm[m][q] * real(1.0); },
-                      [&](int m) { return rvy[nb[m]] * vvm[m][q] * real(1.0); },
-                      [&](int m) { return rvz[nb[m]] * vvm[m][q] * real(1.0); });
+        for (int q = 0; q < 3; ++q) x[q] = G.gx[q * d.nreg + p.ci + off(m)];
+      }
+    };
+    // merged velocity v + w_full r/|r| at address b (level kk) with position x
+    auto vv3 = [&](long b, int kk, const real* x, real* o) {
+      const real wc = G.cfact[kk] * wd[b + ks] + G.dfact[kk] * wd[b];
+      o[0] = vxd[b] + wc * x[0] / R;
+      o[1] = vyd[b] + wc * x[1] / R;
+      o[2] = vzd[b] + wc * x[2] / R;
+    };
+    real x0[3];
+    pos(0, x0);
+    // divergence of rhogv * vv_q, one neighbour at a time: per direction and
+    // component the sum over the neighbours runs in stencil order, as in
+    // div7 / div_pl, without keeping every neighbour's values live
+    real vvm0[3];
+    real sd[3][3];   // [direction][q] (regional); the pole sums the directions per point
+    real div[3];
+#ifdef NH_ADV_UNROLL
+#pragma unroll
+#else
+#pragma unroll 1
+#endif
+    for (int m = 0; m < NN; ++m) {
+      real xm[3], vv[3];
+      pos(m, xm);
+      const long bm = nbr(m);
+      vv3(bm, k, xm, vv);
+      if (m == 0)
+        for (int q = 0; q < 3; ++q) vvm0[q] = vv[q];
+      const real fx0 = rvx[bm], fy0 = rvy[bm], fz0 = rvz[bm];
+      if (POLE) {
+        const real* cc = G.cdiv_pl + (m * 2 + p.lp) * 3;
+        for (int q = 0; q < 3; ++q) {
+          const real pv = cc[0] * (fx0 * vv[q] * real(1.0)) + cc[1] * (fy0 * vv[q] * real(1.0)) +
+                          cc[2] * (fz0 * vv[q] * real(1.0));
+          div[q] = (m == 0) ? pv : div[q] + pv;
+        }
+      } else {
+        const real cx = rc(G.cdiv, d.nreg, m, p.ci), cy = rc(G.cdiv, d.nreg, 7 + m, p.ci),
+                   cz = rc(G.cdiv, d.nreg, 14 + m, p.ci);
+        for (int q = 0; q < 3; ++q) {
+          const real tx = cx * (fx0 * vv[q] * real(1.0));
+          const real ty = cy * (fy0 * vv[q] * real(1.0));
+          const real tz = cz * (fz0 * vv[q] * real(1.0));
+          sd[0][q] = (m == 0) ? tx : sd[0][q] + tx;
+          sd[1][q] = (m == 0) ? ty : sd[1][q] + ty;
+          sd[2][q] = (m == 0) ? tz : sd[2][q] + tz;
+        }
       }
     }
+    if (!POLE)
+      for (int q = 0; q < 3; ++q) div[q] = sd[0][q] + sd[1][q] + sd[2][q];
     // vertical fluxes at the half levels k+1 and k (zero at kmin and kmax+1)
     real hi[3] = {real(0.0), real(0.0), real(0.0)}, lo[3] = {real(0.0), real(0.0), real(0.0)};
     if (k + 1 <= kmax) {
       real up[3];
-      vv3(0, k + 1, up);
+      vv3(a + ks, k + 1, x0, up);
       const real wup = rw[a + ks];
-      for (int q = 0; q < 3; ++q) hi[q] = real(0.0) * real(1.0) + wup * (G.afact[k + 1] * up[q] + G.bfact[k + 1] * vvm[0][q]) * real(1.0);
+      for (int q = 0; q < 3; ++q) hi[q] = real(0.0) * real(1.0) + wup * (G.afact[k + 1] * up[q] + G.bfact[k + 1] * vvm0[q]) * real(1.0);
     }
     if (k >= kmin + 1) {
       real dn[3];
-      vv3(0, k - 1, dn);
+      vv3(a - ks, k - 1, x0, dn);
       const real w0 = rw[a];
-      for (int q = 0; q < 3; ++q) lo[q] = real(0.0) * real(1.0) + w0 * (G.afact[k] * vvm[0][q] + G.bfact[k] * dn[q]) * real(1.0);
+      for (int q = 0; q < 3; ++q) lo[q] = real(0.0) * real(1.0) + w0 * (G.afact[k] * vvm0[q] + G.bfact[k] * dn[q]) * real(1.0);
     }
     real dvv[3];
     for (int q = 0; q < 3; ++q) dvv[q] = -div[q] - (hi[q] - lo[q]) * G.rdgz[k];
-    const real gxc = px[0] / R, gyc = py[0] / R, gzc = pz[0] / R;
+    const real gxc = x0[0] / R, gyc = x0[1] / R, gzc = x0[2] / R;
     const real rh = rg[a];
-    const real dx = dvv[0] - (-real(2.0) * rh * (c.OHM * vvm[0][1]));
-    const real dy = dvv[1] - (real(2.0) * rh * (c.OHM * vvm[0][0]));
+    const real dx = dvv[0] - (-real(2.0) * rh * (c.OHM * vvm0[1]));
+    const real dy = dvv[1] - (real(2.0) * rh * (c.OHM * vvm0[0]));
     const real dz = dvv[2];
     const real prd = dx * gxc + dy * gyc + dz * gzc;
     out[0 * NF + a] = dx - prd * gxc;
