@@ -547,17 +547,21 @@ struct ViPath0P {
     real dpg[3] = {real(0.0), real(0.0), real(0.0)};
     if (k >= kmin && k <= kmax) {
       const Cell cl = cell_of_pt(p);
-      long off[7];
-      stencil_offsets(d.g, off);
-      auto nbr = [&](int m) { return p.pole ? a + m : a + off[m]; };
-      const real hd = hdiv(G, d, cl, [&](int m) { return rvx[nbr(m)] * real(1.0); },
-                             [&](int m) { return rvy[nbr(m)] * real(1.0); }, [&](int m) { return rvz[nbr(m)] * real(1.0); });
+      // div(rhogv), div(rhogv eth)
+      real dvs[2];
+      hdiv_multi<2>(G, d, p, [&](int, long b, real (*v)[3]) {
+        const real e = eth[b];
+        const real r3[3] = {rvx[b], rvy[b], rvz[b]};
+        for (int q = 0; q < 3; ++q) {
+          v[0][q] = r3[q] * real(1.0);
+          v[1][q] = r3[q] * e * real(1.0);
+        }
+      }, dvs);
+      const real hd = dvs[0];
       const real vmh_hi = (k + 1 <= kmax) ? real(0.0) * real(1.0) + real(1.0) * rw[a + ks] * real(1.0) : real(0.0);
       const real vmh_lo = (k >= kmin + 1) ? real(0.0) * real(1.0) + real(1.0) * rw[a] * real(1.0) : real(0.0);
       drhog = -hd - (vmh_hi - vmh_lo) * G.rdgz[k];
-      const real he = hdiv(G, d, cl, [&](int m) { return rvx[nbr(m)] * eth[nbr(m)] * real(1.0); },
-                             [&](int m) { return rvy[nbr(m)] * eth[nbr(m)] * real(1.0); },
-                             [&](int m) { return rvz[nbr(m)] * eth[nbr(m)] * real(1.0); });
+      const real he = dvs[1];
       const real ws_hi = (k + 1 <= kmax)
           ? real(0.0) * real(1.0) + real(1.0) * (rw[a + ks] * (G.afact[k + 1] * eth[a + ks] + G.bfact[k + 1] * eth[a])) * real(1.0) : real(0.0);
       const real ws_lo = (k >= kmin + 1)
@@ -733,21 +737,20 @@ struct ViMainH {
     const int kmin = d.kmin, kmax = d.kmax, k = p.k;
     if (k < kmin || k > kmax) return;
     const long NF = d.NF, a = p.a;
-    const Cell cl = cell_of_pt(p);
-    long off[7];
-    stencil_offsets(d.g, off);
-    auto nbr = [&](int m) { return p.pole ? a + m : a + off[m]; };
-    const real divs1 = hdiv(G, d, cl, [&](int m) { return dv[0][nbr(m)] * real(1.0); },
-                              [&](int m) { return dv[1][nbr(m)] * real(1.0); }, [&](int m) { return dv[2][nbr(m)] * real(1.0); });
-    const real he = hdiv(G, d, cl, [&](int m) { return dv[0][nbr(m)] * eth[nbr(m)] * real(1.0); },
-                           [&](int m) { return dv[1][nbr(m)] * eth[nbr(m)] * real(1.0); },
-                           [&](int m) { return dv[2][nbr(m)] * eth[nbr(m)] * real(1.0); });
+    // div(dv), div(dv eth), div((prog + dv) ethtot)
+    real dvs[3];
+    hdiv_multi<3>(G, d, p, [&](int, long b, real (*v)[3]) {
+      const real e = eth[b], et = ethtot[b];
+      for (int q = 0; q < 3; ++q) {
+        const real dq = dv[q][b];
+        v[0][q] = dq * real(1.0);
+        v[1][q] = dq * e * real(1.0);
+        v[2][q] = (prog[(1 + q) * NF + b] + dq) * et * real(1.0);
+      }
+    }, dvs);
+    const real divs1 = dvs[0], he = dvs[1], hd = dvs[2];
     const real drhoge = -he - (real(0.0) - real(0.0)) * G.rdgz[k];
     const real grhoge1 = gT[5 * NF + a] + drhoge;
-    const real hd = hdiv(G, d, cl,
-                           [&](int m) { return (prog[1 * NF + nbr(m)] + dv[0][nbr(m)]) * ethtot[nbr(m)] * real(1.0); },
-                           [&](int m) { return (prog[2 * NF + nbr(m)] + dv[1][nbr(m)]) * ethtot[nbr(m)] * real(1.0); },
-                           [&](int m) { return (prog[3 * NF + nbr(m)] + dv[2][nbr(m)]) * ethtot[nbr(m)] * real(1.0); });
     hv[0][a] = divs1;
     hv[1][a] = grhoge1 * c.Rdry / c.CVdry;
     hv[2][a] = hd;

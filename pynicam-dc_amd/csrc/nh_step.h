@@ -223,6 +223,47 @@ NH_HD inline real hdiv(const Geo& G, const Dims& d, const Cell& cl, FX fx, FY fy
 }
 
 
+// NDIV horizontal divergences at one compute point, one neighbour at a time:
+// f(m, addr, v) fills v[i][dir] for divergence i at neighbour m (address
+// addr).  Per divergence and direction the sum over the neighbours runs in
+// stencil order, then x + y + z, as div7 does (the pole: per point
+// cx*fx + cy*fy + cz*fz, summed over v, as div_pl).  The neighbour loop is not
+// unrolled so that only one neighbour's values are live.
+template <int NDIV, class F>
+NH_HD inline void hdiv_multi(const Geo& G, const Dims& d, const Pt& p, F f, real out[NDIV]) {
+  real v[NDIV][3];
+  if (p.pole) {
+#pragma unroll 1
+    for (int m = 0; m < 6; ++m) {
+      f(m, p.a + m, v);
+      const real* c = G.cdiv_pl + (m * 2 + p.lp) * 3;
+      for (int i = 0; i < NDIV; ++i) {
+        const real pv = c[0] * v[i][0] + c[1] * v[i][1] + c[2] * v[i][2];
+        out[i] = (m == 0) ? pv : out[i] + pv;
+      }
+    }
+    return;
+  }
+  real s[NDIV][3];
+  const long g = d.g;
+#pragma unroll 1
+  for (int m = 0; m < 7; ++m) {
+    // (di, dj) + 1 of neighbour m packed two bits per point:
+    // (0,0) (+1,0) (+1,+1) (0,+1) (-1,0) (-1,-1) (0,-1)
+    const long off = (((4201 >> (2 * m)) & 3) - 1) + (((421 >> (2 * m)) & 3) - 1) * g;
+    f(m, p.a + off, v);
+    const real cx = rc(G.cdiv, d.nreg, m, p.ci), cy = rc(G.cdiv, d.nreg, 7 + m, p.ci),
+               cz = rc(G.cdiv, d.nreg, 14 + m, p.ci);
+    for (int i = 0; i < NDIV; ++i) {
+      const real tx = cx * v[i][0], ty = cy * v[i][1], tz = cz * v[i][2];
+      s[i][0] = (m == 0) ? tx : s[i][0] + tx;
+      s[i][1] = (m == 0) ? ty : s[i][1] + ty;
+      s[i][2] = (m == 0) ? tz : s[i][2] + tz;
+    }
+  }
+  for (int i = 0; i < NDIV; ++i) out[i] = s[i][0] + s[i][1] + s[i][2];
+}
+
 // kinetic energy at level k of a column given accessors (kernels/rhogkin.py:38-79),
 // in the precision T of the vertical profiles V (the field precision, or FP64
 // for the global sums)
