@@ -216,8 +216,10 @@ def test_offline_tracer_gl05(gpu, ctx5, case_gl05, wind, params):
 
 def test_offline_tracer_gl06_z60_bounds_and_mass(gpu):
     """The tracer benchmark grid (gl06, 60 levels to 12 km, 4 tracers of 6 blobs, dt = 900 s,
-    deformational flow): 12 steps keep every tracer's global mass to round-off and its mixing
-    ratio within the initial range [0, 1]."""
+    deformational flow): 12 steps keep every tracer's global mass and its mixing ratio within the
+    initial range [0, 1].  The flux-form update conserves mass to round-off; the reference's
+    clipping of small negative masses (-1e-10 < rhogq < 0 -> 0, mod_src_tracer.py:1180-1204)
+    adds a little, so the mass is held to 1e-10."""
     case = mod("setup").build_case(glevel=6, vlayer=60, ztop=12000.0, dz_bottom=200.0, dtl=900.0, seed=0)
     geo, t = case.geo, case.topo
     st = mod("initial").blob_tracers(geo, case.state, ntracer=4, seed=7, nblob=6, radius_m=1.0e6)
@@ -234,7 +236,7 @@ def test_offline_tracer_gl06_z60_bounds_and_mass(gpu):
             run.dynamics_step()
         got = run.state()
         m1 = tmass(got["PROGq"], got["PROGq_pl"])
-        assert np.all(np.abs(m1[1:] / m0[1:] - 1.0) < 1e-13), m1 / m0
+        assert np.all(np.abs(m1[1:] / m0[1:] - 1.0) < 1e-10), m1 / m0
         q = interior(got["PROGq"][..., 1:], t) / interior(got["PROG"][..., 0], t)[..., None]
         assert q.min() >= -1e-14 and q.max() <= 1.0 + 1e-13, (q.min(), q.max())
     finally:
