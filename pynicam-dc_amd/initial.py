@@ -125,18 +125,21 @@ def cosine_blobs(lat, lon, centres, radius_m=1.0e6, a=6.371e6):
 
 
 def blob_tracers(geo: Geometry, state: dict, ntracer: int = 4, seed: int = 0, nblob: int = 6,
-                 radius_m: float = 1.0e6, with_qv: bool = True):
+                 radius_m: float = 1.0e6, with_qv: bool = True, with_constant: bool = False):
     """Passive tracers for the transport tests: each a seeded sum of compact cosine blobs in [0, 1].
 
     Returns a copy of ``state`` whose PROGq/PROGq_pl hold qv = 0 (when
     ``with_qv``) followed by ``ntracer`` tracer masses rhog * q on the physical
-    levels (zero on the ghost levels), halos filled.
+    levels (zero on the ghost levels), halos filled.  ``with_constant`` appends
+    a tracer with q = 1: transported with the same mass fluxes it carries the
+    density the transport implies, the reference for mixing-ratio bounds when
+    the prescribed fluxes do not conserve the state's own density.
     """
     t = geo.topo
     kmin, kmax = t.kmin, t.kmax
     rng = np.random.default_rng(seed + 1000)
     P, Pp = state["PROG"], state["PROG_pl"]
-    nq = ntracer + (1 if with_qv else 0)
+    nq = ntracer + (1 if with_qv else 0) + (1 if with_constant else 0)
     Q = np.zeros(P.shape[:-1] + (nq,))
     Qp = np.zeros(Pp.shape[:-1] + (nq,))
     k = slice(kmin, kmax + 1)
@@ -147,5 +150,8 @@ def blob_tracers(geo: Geometry, state: dict, ntracer: int = 4, seed: int = 0, nb
         iq = n + (1 if with_qv else 0)
         Q[:, :, k, :, iq] = P[:, :, k, :, 0] * q[:, :, None, :]
         Qp[:, k, :, iq] = Pp[:, k, :, 0] * qp[:, None, :]
+    if with_constant:
+        Q[:, :, k, :, -1] = P[:, :, k, :, 0]
+        Qp[:, k, :, -1] = Pp[:, k, :, 0]
     t.halo_fill_host(Q, Qp)
     return dict(state, PROGq=Q, PROGq_pl=Qp)
