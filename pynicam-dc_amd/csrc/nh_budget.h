@@ -26,6 +26,7 @@ enum BudgetTerm { B_POT = 0, B_INT, B_KIN, B_DRY, B_VAP, B_NTERMS };
 
 struct BudgetT {
   using LdsT = double;
+  static constexpr bool kStage = false;
   static constexpr const char* kName = "budget";
   Dims d; Consts c; Geo G;
   const real* prog;

@@ -9,7 +9,7 @@
 //   template <class F> void run_pts(const F& f, const Dims& d);
 //                                                         (f(plane, x) on the point grid)
 //   template <class F> void run_tiles(const F& f, int nblocks, int lds_doubles);
-//                                                         (f.phase1, barrier, f.phase2 per block)
+//                                                         ([f.phase0, barrier,] f.phase1, barrier, f.phase2)
 //   int check(const char* what);                          (0 or error)
 // so that the same sequence drives the GPU build and the host test build.
 #pragma once
@@ -282,14 +282,14 @@ void divdamp(X& x, const Context& ctx, const real* vx, const real* vy, const rea
   a.d = ctx.d; a.c = ctx.c; a.G = ctx.G; a.tg = tg;
   a.rvx = vx; a.rvy = vy; a.rvz = vz; a.rw = w;
   for (int q = 0; q < 3; ++q) a.out[q] = ctx.field(W_DDT + q);
-  x.run_tiles(a, tg.nblocks, tg.lds);
+  x.run_tiles(a, tg.nblocks, 5 * (tg.TH + 2) * ctx.d.g + tg.lds);   // staged fields + triangles
   real* tmp[3] = {ctx.field(W_DDT), ctx.field(W_DDT + 1), ctx.field(W_DDT + 2)};
   halo(x, ctx, tmp, 3);
   DivDamp2T b;
   b.d = ctx.d; b.c = ctx.c; b.G = ctx.G; b.tg = tg;
   for (int q = 0; q < 3; ++q) b.in[q] = tmp[q];
   b.out[0] = out0; b.out[1] = out1; b.out[2] = out2;
-  x.run_tiles(b, tg.nblocks, tg.lds);
+  x.run_tiles(b, tg.nblocks, 3 * (tg.TH + 2) * ctx.d.g + tg.lds);   // staged fields + triangles
 }
 
 // ----------------------------------------------------------------------------

@@ -58,8 +58,8 @@ struct HipProf {
   std::vector<const char*> name;       // per used pair
 };
 
-// tiled two-phase kernels: LDS shared by the block (elements of F::LdsT),
-// barrier between the phases
+// tiled kernels: LDS shared by the block (elements of F::LdsT), an optional
+// staging phase (F::kStage), then two phases, with a barrier between phases
 template <class F>
 __global__ __launch_bounds__(256) void k_tiles(F f, int nblk) {
   extern __shared__ __align__(16) unsigned char nh_lds_bytes[];
@@ -69,6 +69,10 @@ __global__ __launch_bounds__(256) void k_tiles(F f, int nblk) {
   const int xcd = L % kXcds, idx = L / kXcds;
   const int rem = nblk % kXcds;
   const int blk = (rem == 0 || xcd < rem) ? xcd * per + idx : rem * per + (xcd - rem) * (per - 1) + idx;
+  if constexpr (F::kStage) {
+    f.phase0(blk, (int)threadIdx.x, lds);
+    __syncthreads();
+  }
   f.phase1(blk, (int)threadIdx.x, lds);
   __syncthreads();
   f.phase2(blk, (int)threadIdx.x, lds);

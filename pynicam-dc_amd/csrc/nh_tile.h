@@ -54,37 +54,79 @@ NH_HD inline TileAt tile_at(const Dims& d, const TileGeom& tg, int blk) {
 
 // ----------------------------------------------------------------------------
 // first operator: triangle-mean 3-D divergence, gradient back to the cells
-// (kernels/oprt3ddivdamp.py:50-173)
+// (kernels/oprt3ddivdamp.py:50-173).  Three phases: the tile's points of
+// (vx, vy, vz) and of rhogw at the two bounding half levels are staged in LDS
+// (rows j0-1 .. j0+rows, columns 0 .. g-1, each value loaded once); the
+// triangle values of every corner are formed from LDS; then the hexagon sums.
 // ----------------------------------------------------------------------------
 struct DivDamp1T {
   using LdsT = real;
+  static constexpr bool kStage = true;
   static constexpr const char* kName = "divdamp_3d";
   Dims d; Consts c; Geo G; TileGeom tg;
   const real *rvx, *rvy, *rvz, *rw;
   real* out[3];
-  NH_HD real rwv(long aa, int kk) const {
-    return (kk >= d.kmin + 1 && kk <= d.kmax) ? (real(0.0) * real(1.0) + rw[aa] * real(1.0)) : real(0.0);
-  }
   NH_HD bool active(int k) const { return k >= d.kmin && k <= d.kmax; }
+  // LDS: staged fields [f][r][col] (f: vx, vy, vz, rhogw(k), rhogw(k+1)), then the triangles
+  NH_HD int sf() const { return (tg.TH + 2) * d.g; }
+  NH_HD int tbase() const { return 5 * sf(); }
+  NH_HD void phase0(int blk, int tid, real* lds) const {
+    if (blk >= tg.ntiles) return;
+    const TileAt t = tile_at(d, tg, blk);
+    if (!active(t.k)) return;
+    const int g = d.g;
+    const long gg = (long)g * g;
+    const long base = (long)(t.l * d.kall + t.k) * gg;
+    const int npl = (t.rows + 2) * g;
+    const real* v3[3] = {rvx, rvy, rvz};
+    for (int idx = tid; idx < 5 * npl; idx += 256) {
+      const int f = idx / npl, rem = idx - f * npl;
+      const int r = rem / g, col = rem - r * g;
+      const long a = base + (long)(t.j0 - 1 + r) * g + col;
+      real v;
+      if (f < 3) {
+        v = v3[f][a];
+      } else {
+        const int kk = t.k + (f - 3);     // rhogw at half levels k and k+1, zero outside kmin+1..kmax
+        v = (kk >= d.kmin + 1 && kk <= d.kmax) ? (real(0.0) * real(1.0) + rw[a + (long)(f - 3) * gg] * real(1.0))
+                                               : real(0.0);
+      }
+      lds[f * sf() + r * g + col] = v;
+    }
+  }
   NH_HD void phase1(int blk, int tid, real* lds) const {
     if (blk >= tg.ntiles) return;
     const TileAt t = tile_at(d, tg, blk);
     if (!active(t.k)) return;
     const int g = d.g, W1 = tg.W + 1;
-    const long gg = (long)g * g, ks = gg;
     const int nc = W1 * (t.rows + 1);
+    const real* S = lds;
+    real* Tl = lds + tbase();
     for (int cidx = tid; cidx < nc; cidx += 256) {
       const int ci_ = cidx % W1, cj_ = cidx / W1;
       const int i = ci_, j = t.j0 - 1 + cj_;
       const long cj = ((long)t.l * g + j) * g + i;
-      const long a = ((long)(t.l * d.kall + t.k) * g + j) * g + i;
+      // staged points of corner (i, j): p0 (i, j), then (i+1, j), (i+1, j+1), (i, j+1)
+      const int s00 = cj_ * g + i, s10 = s00 + 1, s11 = s00 + 1 + g, s01 = s00 + g;
       for (int T = 0; T < 2; ++T) {
-        const long o1 = (T == 0) ? 1 : 1 + g;
-        const long o2 = (T == 0) ? 1 + g : g;
-        const real s = tri_div(G, d.nreg, cj, T, rvx, rvy, rvz, a, g, real(1.0));
-        const real up = (rwv(a + ks, t.k + 1) + rwv(a + o1 + ks, t.k + 1) + rwv(a + o2 + ks, t.k + 1));
-        const real dn = (rwv(a, t.k) + rwv(a + o1, t.k) + rwv(a + o2, t.k));
-        lds[(T * (t.rows + 1) + cj_) * W1 + ci_] = s + (up - dn) / real(3.0) * G.rdgz[t.k];
+        const int p1 = (T == 0) ? s10 : s11;
+        const int p2 = (T == 0) ? s11 : s01;
+        real sv = real(0.0);
+        for (int dd = 0; dd < 3; ++dd) {
+          const real c0 = rc(G.cintp, d.nreg, (dd * 3 + 0) * 2 + T, cj);
+          const real c1 = rc(G.cintp, d.nreg, (dd * 3 + 1) * 2 + T, cj);
+          const real c2 = rc(G.cintp, d.nreg, (dd * 3 + 2) * 2 + T, cj);
+          const real* F = S + dd * sf();
+          const real t0 = c0 * (real(1.0) * F[s00]);
+          const real t1 = c1 * (real(1.0) * F[p1]);
+          const real t2 = c2 * (real(1.0) * F[p2]);
+          sv = (dd == 0) ? (t0 + t1 + t2) : (sv + t0 + t1 + t2);
+        }
+        const real* Wd = S + 3 * sf();
+        const real* Wu = S + 4 * sf();
+        const real up = (Wu[s00] + Wu[p1] + Wu[p2]);
+        const real dn = (Wd[s00] + Wd[p1] + Wd[p2]);
+        Tl[(T * (t.rows + 1) + cj_) * W1 + ci_] = sv + (up - dn) / real(3.0) * G.rdgz[t.k];
       }
     }
   }
@@ -106,7 +148,8 @@ struct DivDamp1T {
       return;
     }
     const int R = t.rows + 1;
-    auto L = [&](int T, int ii, int jj) { return lds[(T * R + (jj - t.j0 + 1)) * W1 + ii]; };
+    const real* Tl = lds + tbase();
+    auto L = [&](int T, int ii, int jj) { return Tl[(T * R + (jj - t.j0 + 1)) * W1 + ii]; };
     real T6[6] = {L(0, i, j), L(1, i, j), L(0, i - 1, j), L(1, i - 1, j - 1), L(0, i - 1, j - 1), L(1, i, j - 1)};
     if (i == 1 && j == 1) {
       const real m0 = G.sgp[t.l] ? real(0.0) : real(1.0), m1 = G.sgp[t.l] ? real(1.0) : real(0.0);
@@ -125,24 +168,57 @@ struct DivDamp1T {
 // (kernels/oprtdivdamp.py:54-173, kernels/divdamppostcomm.py)
 struct DivDamp2T {
   using LdsT = real;
+  static constexpr bool kStage = true;
   static constexpr const char* kName = "divdamp_post";
   Dims d; Consts c; Geo G; TileGeom tg;
   const real* in[3];
   real* out[3];
   NH_HD bool active(int k) const { return k >= d.kmin && k <= d.kmax; }
+  // LDS: the three staged input fields [f][r][col], rows j0-1 .. j0+rows, then the triangles
+  NH_HD int sf() const { return (tg.TH + 2) * d.g; }
+  NH_HD int tbase() const { return 3 * sf(); }
+  NH_HD void phase0(int blk, int tid, real* lds) const {
+    if (blk >= tg.ntiles) return;
+    const TileAt t = tile_at(d, tg, blk);
+    if (!active(t.k)) return;
+    const int g = d.g;
+    const long base = (long)(t.l * d.kall + t.k) * g * g;
+    const int npl = (t.rows + 2) * g;
+    for (int idx = tid; idx < 3 * npl; idx += 256) {
+      const int f = idx / npl, rem = idx - f * npl;
+      const int r = rem / g, col = rem - r * g;
+      lds[f * sf() + r * g + col] = in[f][base + (long)(t.j0 - 1 + r) * g + col];
+    }
+  }
   NH_HD void phase1(int blk, int tid, real* lds) const {
     if (blk >= tg.ntiles) return;
     const TileAt t = tile_at(d, tg, blk);
     if (!active(t.k)) return;
     const int g = d.g, W1 = tg.W + 1;
     const int nc = W1 * (t.rows + 1);
+    real* Tl = lds + tbase();
     for (int cidx = tid; cidx < nc; cidx += 256) {
       const int ci_ = cidx % W1, cj_ = cidx / W1;
       const int i = ci_, j = t.j0 - 1 + cj_;
       const long cj = ((long)t.l * g + j) * g + i;
-      const long a = ((long)(t.l * d.kall + t.k) * g + j) * g + i;
-      for (int T = 0; T < 2; ++T)
-        lds[(T * (t.rows + 1) + cj_) * W1 + ci_] = tri_div(G, d.nreg, cj, T, in[0], in[1], in[2], a, g, -real(1.0));
+      const int s00 = cj_ * g + i, s10 = s00 + 1, s11 = s00 + 1 + g, s01 = s00 + g;
+      for (int T = 0; T < 2; ++T) {
+        // tri_div of -(in) at the corner, from the staged values
+        const int p1 = (T == 0) ? s10 : s11;
+        const int p2 = (T == 0) ? s11 : s01;
+        real sv = real(0.0);
+        for (int dd = 0; dd < 3; ++dd) {
+          const real c0 = rc(G.cintp, d.nreg, (dd * 3 + 0) * 2 + T, cj);
+          const real c1 = rc(G.cintp, d.nreg, (dd * 3 + 1) * 2 + T, cj);
+          const real c2 = rc(G.cintp, d.nreg, (dd * 3 + 2) * 2 + T, cj);
+          const real* F = lds + dd * sf();
+          const real t0 = c0 * (-real(1.0) * F[s00]);
+          const real t1 = c1 * (-real(1.0) * F[p1]);
+          const real t2 = c2 * (-real(1.0) * F[p2]);
+          sv = (dd == 0) ? (t0 + t1 + t2) : (sv + t0 + t1 + t2);
+        }
+        Tl[(T * (t.rows + 1) + cj_) * W1 + ci_] = sv;
+      }
     }
   }
   NH_HD void phase2(int blk, int tid, const real* lds) const {
@@ -164,7 +240,8 @@ struct DivDamp2T {
     real o[3] = {real(0.0), real(0.0), real(0.0)};
     if (active(t.k)) {
       const int R = t.rows + 1;
-      auto L = [&](int T, int ii, int jj) { return lds[(T * R + (jj - t.j0 + 1)) * W1 + ii]; };
+      const real* Tl = lds + tbase();
+      auto L = [&](int T, int ii, int jj) { return Tl[(T * R + (jj - t.j0 + 1)) * W1 + ii]; };
       const real T6[6] = {L(0, i, j), L(1, i, j), L(0, i - 1, j), L(1, i - 1, j - 1), L(0, i - 1, j - 1),
                             L(1, i, j - 1)};
       for (int dd = 0; dd < 3; ++dd) o[dd] = hex_sum(G, d.nreg, ci, dd, T6);
@@ -187,6 +264,7 @@ struct DivDamp2T {
 // ----------------------------------------------------------------------------
 struct HDiffPass2T {
   using LdsT = real;
+  static constexpr bool kStage = false;
   static constexpr const char* kName = "hdiff_pass2";
   Dims d; Consts c; Geo G; TileGeom tg;
   const real* in[6];

@@ -93,6 +93,8 @@ class HostExec {
       std::vector<typename F::LdsT> lds((size_t)lds_elems + 1);
 #pragma omp for schedule(static)
       for (int b = 0; b < nblocks; ++b) {
+        if constexpr (F::kStage)
+          for (int t = 0; t < 256; ++t) f.phase0(b, t, lds.data());
         for (int t = 0; t < 256; ++t) f.phase1(b, t, lds.data());
         for (int t = 0; t < 256; ++t) f.phase2(b, t, lds.data());
       }
