@@ -96,7 +96,8 @@ KERNEL_TYPES = {
 
 def _kernel_key(name: str):
     import re
-    m = re.search(r"k_(?:pts|run|tiles)<nh::f(?:64|32)::(\w+(?:<\d+>)?)\s*>", name)
+    m = re.search(r"k_(?:pts|run|tiles)<nh::f(?:64|32)::(?:StagedPointT<nh::f(?:64|32)::)?(\w+(?:<\d+>)?)\s*>",
+                  name)
     return KERNEL_TYPES.get(m.group(1)) if m else None
 
 

@@ -282,14 +282,14 @@ void divdamp(X& x, const Context& ctx, const real* vx, const real* vy, const rea
   a.d = ctx.d; a.c = ctx.c; a.G = ctx.G; a.tg = tg;
   a.rvx = vx; a.rvy = vy; a.rvz = vz; a.rw = w;
   for (int q = 0; q < 3; ++q) a.out[q] = ctx.field(W_DDT + q);
-  x.run_tiles(a, tg.nblocks, 5 * (tg.TH + 2) * ctx.d.g + tg.lds);   // staged fields + triangles
+  x.run_tiles(a, tg.nblocks, DivDamp1T::lds_elems(tg, ctx.d.g));
   real* tmp[3] = {ctx.field(W_DDT), ctx.field(W_DDT + 1), ctx.field(W_DDT + 2)};
   halo(x, ctx, tmp, 3);
   DivDamp2T b;
   b.d = ctx.d; b.c = ctx.c; b.G = ctx.G; b.tg = tg;
   for (int q = 0; q < 3; ++q) b.in[q] = tmp[q];
   b.out[0] = out0; b.out[1] = out1; b.out[2] = out2;
-  x.run_tiles(b, tg.nblocks, 3 * (tg.TH + 2) * ctx.d.g + tg.lds);   // staged fields + triangles
+  x.run_tiles(b, tg.nblocks, DivDamp2T::lds_elems(tg, ctx.d.g));
 }
 
 // ----------------------------------------------------------------------------
@@ -471,10 +471,7 @@ int dyn_step(X& x, const Context& ctx, real* prog, real* progq, int nq, double d
       PrePostP f{d, ctx.c, ctx.G, prog, rhogq, diag, eth};
       x.run_pts(f, d);
     }
-    {
-      AdvMomP f{d, ctx.c, ctx.G, prog, diag, gadv};
-      x.run_pts(f, d);
-    }
+    run_staged(x, AdvMomP{d, ctx.c, ctx.G, prog, diag, gadv});
     // hyperdiffusion: two passes with a halo fill after each
     {
       HDiffPassP<0> f;
@@ -483,7 +480,7 @@ int dyn_step(X& x, const Context& ctx, real* prog, real* progq, int nq, double d
       f.in[3] = diag + I_W * NF; f.in[4] = diag + I_TEM * NF; f.in[5] = prog;
       for (int q = 0; q < 6; ++q) f.out[q] = ctx.field(W_HDA + q);
       f.rhog = prog;
-      x.run_pts(f, d);
+      run_staged(x, f);
       halo_block(x, ctx, ctx.field(W_HDA), 6);
       HDiffPass2T f2;
       f2.d = d; f2.c = ctx.c; f2.G = ctx.G; f2.tg = tile_geom(d, 12);
@@ -508,7 +505,7 @@ int dyn_step(X& x, const Context& ctx, real* prog, real* progq, int nq, double d
       f.gT = gT;
       f.Mc = ctx.dfield(D_MC); f.Mu = ctx.dfield(D_MU); f.Ml = ctx.dfield(D_ML);
       f.getot = getot; f.frhog = frhog;
-      x.run_pts(f, d);
+      run_staged(x, f);
     }
     {
       EthTotPMP f{d, ctx.c, ctx.G, prog, eth, ethtot, pm};
@@ -539,7 +536,7 @@ int dyn_step(X& x, const Context& ctx, real* prog, real* progq, int nq, double d
         for (int q = 0; q < 3; ++q) f.dd[q] = ddv + q * NF;
         for (int q = 0; q < 3; ++q) f.dv[q] = dv[q];
         f.pm = pm;
-        x.run_pts(f, d);
+        run_staged(x, f);
       }
       halo(x, ctx, dv, 3);
       real* hv[3] = {ctx.field(W_HV), ctx.field(W_HV + 1), ctx.field(W_HV + 2)};
@@ -550,7 +547,7 @@ int dyn_step(X& x, const Context& ctx, real* prog, real* progq, int nq, double d
         for (int q = 0; q < 3; ++q) f.dv[q] = dv[q];
         f.eth = eth; f.ethtot = ethtot; f.gT = gT;
         for (int q = 0; q < 3; ++q) f.hv[q] = hv[q];
-        x.run_pts(f, d);
+        run_staged(x, f);
       }
       {
         ViMainV f;

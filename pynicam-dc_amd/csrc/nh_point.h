@@ -173,33 +173,54 @@ struct PrePostP {
 // ----------------------------------------------------------------------------
 struct AdvMomP {
   static constexpr const char* kName = "adv_mom";
+  // fields read on the stencil: vx, vy, vz, w(k), w(k+1), rhogvx, rhogvy, rhogvz
+  static constexpr int kNStage = 8;
   Dims d; Consts c; Geo G;
   const real* prog;
   const real* diag;
   real* out;
+  // field f of the stencil set, positioned so that the point's own level reads it
+  NH_HD const real* stage_ptr(int f, int) const {
+    const long NF = d.NF;
+    switch (f) {
+      case 0: return diag + I_VX * NF;
+      case 1: return diag + I_VY * NF;
+      case 2: return diag + I_VZ * NF;
+      case 3: return diag + I_W * NF;
+      case 4: return diag + I_W * NF + (long)d.g * d.g;   // next level of the region
+      default: return prog + (long)(f - 4) * NF;
+    }
+  }
+  NH_HD bool stage_level(int k) const { return k >= d.kmin && k <= d.kmax; }
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p)) return;
+    if (!pt_compute(p)) {
+      for (int q = 0; q < 4; ++q) out[q * d.NF + p.a] = real(0.0);
+      return;
+    }
+    const long ks = p.ks;
+    at(p, [&](int f, long b) { return f == 4 ? diag[I_W * d.NF + b + ks] : stage_ptr(f, p.k)[b]; });
+  }
+  template <class LD>
+  NH_HD void at(const Pt& p, LD ld) const {
     const long NF = d.NF, a = p.a;
     const int kmin = d.kmin, kmax = d.kmax, k = p.k;
-    if (!pt_compute(p) || k < kmin || k > kmax) {
+    if (k < kmin || k > kmax) {
       for (int q = 0; q < 4; ++q) out[q * NF + a] = real(0.0);
       return;
     }
     // separate instances for the 6-point pole and 7-point regional stencils:
     // fixed trip counts keep the per-neighbour arrays in registers
-    if (p.pole) point<true>(p);
-    else point<false>(p);
+    if (p.pole) point<true>(p, ld);
+    else point<false>(p, ld);
   }
-  template <bool POLE>
-  NH_HD void point(const Pt& p) const {
+  template <bool POLE, class LD>
+  NH_HD void point(const Pt& p, LD ld) const {
     constexpr int NN = POLE ? 6 : 7;
     const long NF = d.NF, a = p.a, ks = p.ks;
     const int kmin = d.kmin, kmax = d.kmax, k = p.k;
     const real R = c.RADIUS;
-    const real* rvx = prog + 1 * NF;
-    const real* rvy = prog + 2 * NF;
-    const real* rvz = prog + 3 * NF;
     const real* rw = prog + 4 * NF;
     const real* rg = prog;
     const real* vxd = diag + I_VX * NF;
@@ -246,10 +267,15 @@ struct AdvMomP {
       real xm[3], vv[3];
       pos(m, xm);
       const long bm = nbr(m);
-      vv3(bm, k, xm, vv);
+      {
+        const real wc = G.cfact[k] * ld(4, bm) + G.dfact[k] * ld(3, bm);
+        vv[0] = ld(0, bm) + wc * xm[0] / R;
+        vv[1] = ld(1, bm) + wc * xm[1] / R;
+        vv[2] = ld(2, bm) + wc * xm[2] / R;
+      }
       if (m == 0)
         for (int q = 0; q < 3; ++q) vvm0[q] = vv[q];
-      const real fx0 = rvx[bm], fy0 = rvy[bm], fz0 = rvz[bm];
+      const real fx0 = ld(5, bm), fy0 = ld(6, bm), fz0 = ld(7, bm);
       if (POLE) {
         const real* cc = G.cdiv_pl + (m * 2 + p.lp) * 3;
         for (int q = 0; q < 3; ++q) {
@@ -307,18 +333,25 @@ struct AdvMomP {
 template <int LAST>
 struct HDiffPassP {
   static constexpr const char* kName = LAST ? "hdiff_pass2" : "hdiff_pass1";
+  static constexpr int kNStage = 6;    // the six filtered fields (first pass)
   Dims d; Consts c; Geo G;
   const real* in[6];
   real* out[6];
   const real* rhog;
+  NH_HD const real* stage_ptr(int f, int) const { return in[f]; }
+  NH_HD bool stage_level(int) const { return true; }
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    if (!LAST) {
+      at(p, [&](int f, long b) { return in[f][b]; });
+      return;
+    }
     const long a = p.a;
     const Cell cl = cell_of_pt(p);
     for (int q = 0; q < 6; ++q) {
       real r;
-      if (LAST && q >= 4) {
+      if (q >= 4) {
         const real fac = (q == 4) ? c.CVdry : c.HDIFF_FACT_RHO;
         const real* rg = rhog;
         const real gam = c.GAMMA_H;
@@ -326,6 +359,29 @@ struct HDiffPassP {
         r = p.pole ? diffusion_pl(G, p.lp, in[q], kh, a) : diffusion7(G, d, cl, in[q], kh, a);
       } else {
         r = p.pole ? lap_pl(G, p.lp, in[q], a) : lap7(G, d.nreg, p.ci, in[q], a, d.g);
+      }
+      out[q][a] = -r;
+    }
+  }
+  // first pass: minus the Laplacian of each field, fields read through ld(f, address)
+  template <class LD>
+  NH_HD void at(const Pt& p, LD ld) const {
+    const long a = p.a;
+    for (int q = 0; q < 6; ++q) {
+      real r;
+      if (p.pole) {
+        r = real(0.0);
+        for (int v = 0; v < 6; ++v) {
+          const real pv = G.clap_pl[v * 2 + p.lp] * ld(q, a + v);
+          r = (v == 0) ? pv : r + pv;
+        }
+      } else {
+        const long g = d.g;
+        r = rc(G.clap, d.nreg, 0, p.ci) * ld(q, a);
+        for (int m = 1; m < 7; ++m) {
+          const long off = (((4201 >> (2 * m)) & 3) - 1) + (((421 >> (2 * m)) & 3) - 1) * g;
+          r = r + rc(G.clap, d.nreg, m, p.ci) * ld(q, a + off);
+        }
       }
       out[q][a] = -r;
     }
@@ -482,34 +538,46 @@ struct ViPath0P {
   double* Mc; double* Mu; double* Ml;   // FP64 in every form
   real* getot;
   real* frhog;
+  // fields read on the stencil: rhogvx, rhogvy, rhogvz, eth at the point's
+  // level, the pressure at the level of its gradient (kmin -> kmin+1)
+  static constexpr int kNStage = 5;
+  NH_HD const real* stage_ptr(int f, int k) const {
+    const long NF = d.NF;
+    if (f < 3) return prog + (long)(1 + f) * NF;
+    if (f == 3) return eth;
+    return diag + I_PRE * NF + (k == d.kmin ? (long)d.g * d.g : 0L);
+  }
+  NH_HD bool stage_level(int k) const { return k >= d.kmin && k <= d.kmax; }
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    const long sh = (p.k == d.kmin) ? p.ks : 0;   // the level stride differs at the poles
+    at(p, [&](int f, long b) { return f < 4 ? stage_ptr(f, p.k)[b] : diag[I_PRE * d.NF + b + sh]; });
+  }
+  template <class LD>
+  NH_HD void at(const Pt& p, LD ld) const {
     const long NF = d.NF, ks = p.ks, a = p.a;
     const int kmin = d.kmin, kmax = d.kmax, k = p.k;
     const real R = c.RADIUS;
     const real* rg = prog;
-    const real* rvx = prog + NF;
-    const real* rvy = prog + 2 * NF;
-    const real* rvz = prog + 3 * NF;
     const real* rw = prog + 4 * NF;
     const real* pre = diag + I_PRE * NF;
     const long b = a - (long)k * ks;     // column base
-    auto at = [&](int kk) { return b + (long)kk * ks; };
+    auto lev = [&](int kk) { return b + (long)kk * ks; };
     auto rhog_h = [&](int kk) {
       const int k2 = (kk < kmin) ? kmin : kk;
-      return G.c2w0[k2] * rg[at(k2)] + G.c2w1[k2] * rg[at(k2 - 1)];
+      return G.c2w0[k2] * rg[lev(k2)] + G.c2w1[k2] * rg[lev(k2 - 1)];
     };
     auto dpgw = [&](int kk) {
-      return (kk >= kmin + 1 && kk <= kmax) ? real(1.0) * (pre[at(kk)] * real(1.0) - pre[at(kk - 1)] * real(1.0)) * G.rdgzh[kk] : real(0.0);
+      return (kk >= kmin + 1 && kk <= kmax) ? real(1.0) * (pre[lev(kk)] * real(1.0) - pre[lev(kk - 1)] * real(1.0)) * G.rdgzh[kk] : real(0.0);
     };
     auto dbuo = [&](int kk) {
       return (kk >= kmin + 1 && kk <= kmax)
-                 ? -c.GRAV * (G.c2w0[kk] * (rg[at(kk)] / real(1.0) * real(1.0)) + G.c2w1[kk] * (rg[at(kk - 1)] / real(1.0) * real(1.0)))
+                 ? -c.GRAV * (G.c2w0[kk] * (rg[lev(kk)] / real(1.0) * real(1.0)) + G.c2w1[kk] * (rg[lev(kk - 1)] / real(1.0) * real(1.0)))
                  : real(0.0);
     };
     auto gzt = [&](int kk) { return c.GRAV - (dpgw(kk) - dbuo(kk)) / rhog_h(kk); };
-    auto pwh = [&](int kk) { return -gzt(kk) * rw[at(kk)]; };
+    auto pwh = [&](int kk) { return -gzt(kk) * rw[lev(kk)]; };
     real gx, gy, gz;
     if (p.pole) {
       gx = G.gx_pl[p.lp * 3 + 0]; gy = G.gx_pl[p.lp * 3 + 1]; gz = G.gx_pl[p.lp * 3 + 2];
@@ -546,12 +614,11 @@ struct ViPath0P {
     real drhog = real(0.0), drhoge = real(0.0), pw = real(0.0);
     real dpg[3] = {real(0.0), real(0.0), real(0.0)};
     if (k >= kmin && k <= kmax) {
-      const Cell cl = cell_of_pt(p);
       // div(rhogv), div(rhogv eth)
       real dvs[2];
-      hdiv_multi<2>(G, d, p, [&](int, long b, real (*v)[3]) {
-        const real e = eth[b];
-        const real r3[3] = {rvx[b], rvy[b], rvz[b]};
+      hdiv_multi<2>(G, d, p, [&](int, long bn, real (*v)[3]) {
+        const real e = ld(3, bn);
+        const real r3[3] = {ld(0, bn), ld(1, bn), ld(2, bn)};
         for (int q = 0; q < 3; ++q) {
           v[0][q] = r3[q] * real(1.0);
           v[1][q] = r3[q] * e * real(1.0);
@@ -567,7 +634,7 @@ struct ViPath0P {
       const real ws_lo = (k >= kmin + 1)
           ? real(0.0) * real(1.0) + real(1.0) * (rw[a] * (G.afact[k] * eth[a] + G.bfact[k] * eth[a - ks])) * real(1.0) : real(0.0);
       drhoge = -he - (ws_hi - ws_lo) * G.rdgz[k];
-      presgrad_h(G, d, cl, pre, k, R, dpg);
+      presgrad_ld(p, ld, R, dpg);
       pw = diag[I_VX * NF + a] * dpg[0] + diag[I_VY * NF + a] * dpg[1] + diag[I_VZ * NF + a] * dpg[2]
          + G.w2c0[k] * pwh(k + 1) + G.w2c1[k] * pwh(k);
     }
@@ -584,20 +651,20 @@ struct ViPath0P {
       const ConstsT<double>& C = c.d;
       auto rhog_h64 = [&](int kk) {
         const int k2 = (kk < kmin) ? kmin : kk;
-        return V.c2w0[k2] * (double)rg[at(k2)] + V.c2w1[k2] * (double)rg[at(k2 - 1)];
+        return V.c2w0[k2] * (double)rg[lev(k2)] + V.c2w1[k2] * (double)rg[lev(k2 - 1)];
       };
       auto eth_h64 = [&](int kk) {
         const int k2 = (kk < kmin) ? kmin : kk;
-        return V.afact[k2] * (double)eth[at(k2)] + V.bfact[k2] * (double)eth[at(k2 - 1)];
+        return V.afact[k2] * (double)eth[lev(k2)] + V.bfact[k2] * (double)eth[lev(k2 - 1)];
       };
       auto dpgw64 = [&](int kk) {
         return (kk >= kmin + 1 && kk <= kmax)
-                   ? 1.0 * ((double)pre[at(kk)] * 1.0 - (double)pre[at(kk - 1)] * 1.0) * V.rdgzh[kk] : 0.0;
+                   ? 1.0 * ((double)pre[lev(kk)] * 1.0 - (double)pre[lev(kk - 1)] * 1.0) * V.rdgzh[kk] : 0.0;
       };
       auto dbuo64 = [&](int kk) {
         return (kk >= kmin + 1 && kk <= kmax)
-                   ? -C.GRAV * (V.c2w0[kk] * ((double)rg[at(kk)] / 1.0 * 1.0) +
-                                V.c2w1[kk] * ((double)rg[at(kk - 1)] / 1.0 * 1.0))
+                   ? -C.GRAV * (V.c2w0[kk] * ((double)rg[lev(kk)] / 1.0 * 1.0) +
+                                V.c2w1[kk] * ((double)rg[lev(kk - 1)] / 1.0 * 1.0))
                    : 0.0;
       };
       auto gzt64 = [&](int kk) { return C.GRAV - (dpgw64(kk) - dbuo64(kk)) / rhog_h64(kk); };
@@ -612,6 +679,50 @@ struct ViPath0P {
       Mc[a] = ACVovRt2 / 1.0 + V.rdgzh[k] * (Ca * eth_h64(k) - Cb * (gzt64(k) + GCVovR));
       Mu[a] = -V.rdgzh[k] * (Cu1 * eth_h64(k + 1) + V.cfact[k] * (gzt64(k + 1) + Cu2));
       Ml[a] = -V.rdgzh[k] * (Cl1 * eth_h64(k - 1) - V.dfact[k - 1] * (gzt64(k - 1) + Cl2));
+    }
+  }
+  // horizontal pressure-gradient force at the point with the first-layer
+  // remedy (presgrad_h), the pressure of the gradient's level read as ld(4, .)
+  template <class LD>
+  NH_HD void presgrad_ld(const Pt& p, LD ld, real R, real o[3]) const {
+    const long a = p.a;
+    real gr[3];
+    if (p.pole) {
+      for (int dd2 = 0; dd2 < 3; ++dd2) {
+        real acc = real(0.0);
+        for (int v = 0; v < 6; ++v) {
+          const real pv = G.cgrad_pl[(v * 2 + p.lp) * 3 + dd2] * (ld(4, a + v) * real(1.0) * real(1.0));
+          acc = (v == 0) ? pv : acc + pv;
+        }
+        gr[dd2] = acc;
+      }
+    } else {
+      const long g = d.g;
+      for (int dd2 = 0; dd2 < 3; ++dd2) {
+        real acc = rc(G.cgrad, d.nreg, dd2 * 7 + 0, p.ci) * (ld(4, a) * real(1.0));
+        for (int m = 1; m < 7; ++m) {
+          const long off = (((4201 >> (2 * m)) & 3) - 1) + (((421 >> (2 * m)) & 3) - 1) * g;
+          acc = acc + rc(G.cgrad, d.nreg, dd2 * 7 + m, p.ci) * (ld(4, a + off) * real(1.0));
+        }
+        gr[dd2] = acc;
+      }
+    }
+    for (int q = 0; q < 3; ++q) gr[q] = gr[q] + real(0.0);
+    real gx, gy, gz;
+    if (p.pole) {
+      gx = G.gx_pl[(0 * 2 + p.lp) * 3 + 0];
+      gy = G.gx_pl[(0 * 2 + p.lp) * 3 + 1];
+      gz = G.gx_pl[(0 * 2 + p.lp) * 3 + 2];
+      const real prd = gr[0] * gx / R + gr[1] * gy / R + gr[2] * gz / R;
+      o[0] = gr[0] - prd * gx / R;
+      o[1] = gr[1] - prd * gy / R;
+      o[2] = gr[2] - prd * gz / R;
+    } else {
+      gx = G.gx[p.ci]; gy = G.gx[d.nreg + p.ci]; gz = G.gx[2 * d.nreg + p.ci];
+      const real prd = (gr[0] * gx + gr[1] * gy + gr[2] * gz) / R;
+      o[0] = gr[0] - prd * gx / R;
+      o[1] = gr[1] - prd * gy / R;
+      o[2] = gr[2] - prd * gz / R;
     }
   }
 };
@@ -642,6 +753,9 @@ struct EthTotPMP {
 // ----------------------------------------------------------------------------
 struct Path1P {
   static constexpr const char* kName = "vi_path1";
+  // the split rhoge is read on the stencil, at the level whose pressure
+  // gradient the point needs (stage_shift levels above the point's own)
+  static constexpr int kNStage = 1;
   Dims d; Consts c; Geo G;
   real dt, rweight;
   const real* prog;
@@ -650,33 +764,31 @@ struct Path1P {
   const real* dd[3];
   real* dv[3];
   real* pm;
-  NH_HD void level(const Pt& p, int k, long a, real out[3]) const {
-    const long NF = d.NF, ks = p.ks;
-    const int kmin = d.kmin;
-    const real RovCV = c.Rdry / c.CVdry;
+  // level of the gradient for a point at level k: kmin -> kmin+1 (first-layer
+  // remedy); the ghost rows use the nearest physical level's value, kmin-1 ->
+  // kmin (gradient at kmin+1) and kmax+1 -> kmax
+  NH_HD int stage_shift(int k) const {
+    const int kmin = d.kmin, kmax = d.kmax;
+    if (k == kmin || k == kmin - 1) return kmin + 1 - k;
+    if (k == kmax + 1) return -1;
+    return 0;
+  }
+  NH_HD const real* stage_ptr(int, int k) const { return ps[I_RHOGE] + (long)stage_shift(k) * d.g * d.g; }
+  NH_HD bool stage_level(int) const { return true; }
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    const long sh = (long)stage_shift(p.k) * p.ks;
+    at(p, [&](int, long b) { return ps[I_RHOGE][b + sh]; });
+  }
+  // momentum of the small step at level k (column address a); the gradient of
+  // the split pressure comes through grad(m, dir)
+  template <class GR>
+  NH_HD void level(const Pt& p, long a, GR grad, real out[3]) const {
+    const long NF = d.NF;
     const real R = c.RADIUS;
-    const real* pse = ps[I_RHOGE];
-    const int kk = (k == kmin) ? kmin + 1 : k;
-    const long ak = a + (long)(kk - k) * ks;
     real gr[3];
-    if (p.pole) {
-      for (int dd2 = 0; dd2 < 3; ++dd2) {
-        real acc = real(0.0);
-        for (int v = 0; v < 6; ++v) {
-          const real pv = G.cgrad_pl[(v * 2 + p.lp) * 3 + dd2] * ((pse[ak + v] * RovCV) * real(1.0) * real(1.0));
-          acc = (v == 0) ? pv : acc + pv;
-        }
-        gr[dd2] = acc;
-      }
-    } else {
-      long off[7];
-      stencil_offsets(d.g, off);
-      for (int dd2 = 0; dd2 < 3; ++dd2) {
-        real acc = rc(G.cgrad, d.nreg, dd2 * 7, p.ci) * ((pse[ak] * RovCV) * real(1.0));
-        for (int m = 1; m < 7; ++m) acc = acc + rc(G.cgrad, d.nreg, dd2 * 7 + m, p.ci) * ((pse[ak + off[m]] * RovCV) * real(1.0));
-        gr[dd2] = acc;
-      }
-    }
+    for (int dd2 = 0; dd2 < 3; ++dd2) gr[dd2] = grad(dd2);
     for (int q = 0; q < 3; ++q) gr[q] = gr[q] + real(0.0);
     real Pg[3];
     if (p.pole) {
@@ -693,20 +805,39 @@ struct Path1P {
       out[q] = ps[1 + q][a] + drv * dt;
     }
   }
-  NH_HD void operator()(int plane, int x) const {
-    Pt p;
-    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+  // ld(0, address on the point's level): split rhoge at the gradient's level
+  template <class LD>
+  NH_HD void at(const Pt& p, LD ld) const {
     const long NF = d.NF, a = p.a, ks = p.ks;
     const int kmin = d.kmin, kmax = d.kmax, k = p.k;
+    const real RovCV = c.Rdry / c.CVdry;
+    auto grad = [&](int dd2) {
+      real acc;
+      if (p.pole) {
+        acc = real(0.0);
+        for (int v = 0; v < 6; ++v) {
+          const real pv = G.cgrad_pl[(v * 2 + p.lp) * 3 + dd2] * ((ld(0, a + v) * RovCV) * real(1.0) * real(1.0));
+          acc = (v == 0) ? pv : acc + pv;
+        }
+      } else {
+        const long g = d.g;
+        acc = rc(G.cgrad, d.nreg, dd2 * 7, p.ci) * ((ld(0, a) * RovCV) * real(1.0));
+        for (int m = 1; m < 7; ++m) {
+          const long off = (((4201 >> (2 * m)) & 3) - 1) + (((421 >> (2 * m)) & 3) - 1) * g;
+          acc = acc + rc(G.cgrad, d.nreg, dd2 * 7 + m, p.ci) * ((ld(0, a + off) * RovCV) * real(1.0));
+        }
+      }
+      return acc;
+    };
     real v[3];
     if (k >= kmin && k <= kmax) {
-      level(p, k, a, v);
+      level(p, a, grad, v);
     } else {
       // free-slip top / rigid bottom, scaled by the large-step density
       const bool top = (k == kmax + 1);
       const long an = top ? a - ks : a + ks;
       real vn[3];
-      level(p, top ? kmax : kmin, an, vn);
+      level(p, an, grad, vn);
       const real* rg = prog;
       for (int q = 0; q < 3; ++q) v[q] = top ? real(1.0) * (vn[q] / rg[an]) * rg[a] : -real(1.0) * (vn[q] / rg[an]) * rg[a];
     }
@@ -724,6 +855,7 @@ struct Path1P {
 // ----------------------------------------------------------------------------
 struct ViMainH {
   static constexpr const char* kName = "vi_main_h";
+  static constexpr int kNStage = 8;    // fields read on the stencil: dv 0..2, eth, ethtot, prog 1..3
   Dims d; Consts c; Geo G;
   const real* prog;
   const real* dv[3];
@@ -731,21 +863,30 @@ struct ViMainH {
   const real* ethtot;
   const real* gT;
   real* hv[3];
+  NH_HD const real* stage_ptr(int f, int) const {
+    return f < 3 ? dv[f] : (f == 3 ? eth : (f == 4 ? ethtot : prog + (long)(f - 4) * d.NF));
+  }
+  NH_HD bool stage_level(int k) const { return k >= d.kmin && k <= d.kmax; }
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    at(p, [&](int f, long b) { return stage_ptr(f, p.k)[b]; });
+  }
+  // ld(f, address): field f of the stencil set at an address of the point's stencil
+  template <class LD>
+  NH_HD void at(const Pt& p, LD ld) const {
     const int kmin = d.kmin, kmax = d.kmax, k = p.k;
     if (k < kmin || k > kmax) return;
     const long NF = d.NF, a = p.a;
     // div(dv), div(dv eth), div((prog + dv) ethtot)
     real dvs[3];
     hdiv_multi<3>(G, d, p, [&](int, long b, real (*v)[3]) {
-      const real e = eth[b], et = ethtot[b];
+      const real e = ld(3, b), et = ld(4, b);
       for (int q = 0; q < 3; ++q) {
-        const real dq = dv[q][b];
+        const real dq = ld(q, b);
         v[0][q] = dq * real(1.0);
         v[1][q] = dq * e * real(1.0);
-        v[2][q] = (prog[(1 + q) * NF + b] + dq) * et * real(1.0);
+        v[2][q] = (ld(5 + q, b) + dq) * et * real(1.0);
       }
     }, dvs);
     const real divs1 = dvs[0], he = dvs[1], hd = dvs[2];
