@@ -85,7 +85,8 @@ def kernel_bytes(t, nq: int = 1, elem: int = 8) -> dict:
 KERNEL_TYPES = {
     "PrePostP": "prepost", "AdvMomP": "adv_mom", "HDiffPassP<0>": "hdiff_pass1", "HDiffPassP<1>": "hdiff_pass2",
     "DivDamp1P": "divdamp_3d", "DivDamp2P": "divdamp_post", "ViPath0P": "vi_path0", "EthTotPMP": "ethtot_mean",
-    "SplitInit": "split_init", "Path1P": "vi_path1", "ViMainH": "vi_main_h", "ViMainV": "vi_main_v",
+    "SplitInit": "split_init", "Path1P": "vi_path1", "ViMainH": "vi_main_h", "ViMainV<39>": "vi_main_v",
+    "ViMainV<0>": "vi_main_v",
     "ViMainPost": "vi_main_post",
     "ProgUpdateP": "prog_update", "HaloGather": "halo_gather", "TrcVert1": "trc_vert1", "TrcVert2": "trc_vert2",
     "TrcHFlux": "trc_hflux", "TrcRhog2": "trc_rhog2", "TrcGradQ": "trc_gradq", "TrcRemap": "trc_remap",

@@ -550,16 +550,20 @@ int dyn_step(X& x, const Context& ctx, real* prog, real* progq, int nq, double d
         run_staged(x, f);
       }
       {
-        ViMainV f;
-        f.d = d; f.c = ctx.c; f.G = ctx.G;
-        f.dt = dts;
-        for (int q = 0; q < 6; ++q) f.ps[q] = ps[q];
-        f.gT = gT;
-        f.Mc = ctx.dfield(D_MC); f.Mu = ctx.dfield(D_MU); f.Ml = ctx.dfield(D_ML);
-        for (int q = 0; q < 3; ++q) f.hv[q] = hv[q];
-        f.U = ctx.dfield(D_U); f.Gm = ctx.dfield(D_GM);
-        f.out = we[1];
-        x.run(f, nc);
+        // the z40 column (39 unknowns) keeps the sweep in registers
+        auto solve = [&](auto f) {
+          f.d = d; f.c = ctx.c; f.G = ctx.G;
+          f.dt = dts;
+          for (int q = 0; q < 6; ++q) f.ps[q] = ps[q];
+          f.gT = gT;
+          f.Mc = ctx.dfield(D_MC); f.Mu = ctx.dfield(D_MU); f.Ml = ctx.dfield(D_ML);
+          for (int q = 0; q < 3; ++q) f.hv[q] = hv[q];
+          f.U = ctx.dfield(D_U); f.Gm = ctx.dfield(D_GM);
+          f.out = we[1];
+          x.run(f, nc);
+        };
+        if (d.kmax - d.kmin == 39) solve(ViMainV<39>{});
+        else solve(ViMainV<0>{});
       }
       {
         ViMainPost f;

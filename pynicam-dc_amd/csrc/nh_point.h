@@ -175,6 +175,7 @@ struct AdvMomP {
   static constexpr const char* kName = "adv_mom";
   // fields read on the stencil: vx, vy, vz, w(k), w(k+1), rhogvx, rhogvy, rhogvz
   static constexpr int kNStage = 8;
+  static constexpr bool kStageF32 = false;
   Dims d; Consts c; Geo G;
   const real* prog;
   const real* diag;
@@ -334,6 +335,7 @@ template <int LAST>
 struct HDiffPassP {
   static constexpr const char* kName = LAST ? "hdiff_pass2" : "hdiff_pass1";
   static constexpr int kNStage = 6;    // the six filtered fields (first pass)
+  static constexpr bool kStageF32 = false;
   Dims d; Consts c; Geo G;
   const real* in[6];
   real* out[6];
@@ -541,6 +543,7 @@ struct ViPath0P {
   // fields read on the stencil: rhogvx, rhogvy, rhogvz, eth at the point's
   // level, the pressure at the level of its gradient (kmin -> kmin+1)
   static constexpr int kNStage = 5;
+  static constexpr bool kStageF32 = false;
   NH_HD const real* stage_ptr(int f, int k) const {
     const long NF = d.NF;
     if (f < 3) return prog + (long)(1 + f) * NF;
@@ -756,6 +759,7 @@ struct Path1P {
   // the split rhoge is read on the stencil, at the level whose pressure
   // gradient the point needs (stage_shift levels above the point's own)
   static constexpr int kNStage = 1;
+  static constexpr bool kStageF32 = true;
   Dims d; Consts c; Geo G;
   real dt, rweight;
   const real* prog;
@@ -856,6 +860,7 @@ struct Path1P {
 struct ViMainH {
   static constexpr const char* kName = "vi_main_h";
   static constexpr int kNStage = 8;    // fields read on the stencil: dv 0..2, eth, ethtot, prog 1..3
+  static constexpr bool kStageF32 = false;
   Dims d; Consts c; Geo G;
   const real* prog;
   const real* dv[3];
@@ -901,12 +906,14 @@ struct ViMainH {
 // vertical part, one thread per compute column: source vector of the
 // tridiagonal rhogw system and the Thomas sweeps, streamed in k with the
 // previous level kept in registers (kernels/virhowsolver.py:42-120).  FP64 in
-// every form: the field values are widened on load.  The forward sweep keeps
-// u and gamma in the FP64 scratch U and Gm; the backward sweep writes the new
-// rhogw split to out on every level.  U, Gm and out are distinct buffers, so
-// the loads of the next levels can be issued ahead of this level's stores.
+// every form: the field values are widened on load.  NK > 0 is the number of
+// unknowns (kmax - kmin) fixed at compile time: u and gamma of the forward
+// sweep then stay in registers; NK = 0 keeps them in the FP64 scratch U, Gm.
+// The backward sweep writes the new rhogw split to out on every level.
+template <int NK>
 struct ViMainV {
   static constexpr const char* kName = "vi_main_v";
+  static constexpr int kUnroll = NK > 0 ? NK : 4;
   Dims d; Consts c; Geo G;
   double dt;
   const real* ps[6];
@@ -938,13 +945,14 @@ struct ViMainV {
     double* __restrict__ Uo = U;
     double* __restrict__ Go = Gm;
     real* __restrict__ Xo = out;
-    const int N = kmax - kmin;
+    const int N = (NK > 0) ? NK : kmax - kmin;
+    double ub[NK > 0 ? NK : 1], gb[NK > 0 ? NK : 1];
     // level k-1 values, starting at kmin
     long am = b + (long)kmin * ks;
     double pe_m = pe[am], hv1_m = hv1[am], pg_m = pg[am];
     double gr_m = (double)gT0[am] + (-(double)hv0[am] - (0.0 - 0.0) * V.rdgz[kmin]);
     double beta = 0.0, u_prev = 0.0, mu_m = 0.0;
-#pragma unroll 4
+#pragma unroll kUnroll
     for (int n = 0; n < N; ++n) {
       const int k = kmin + 1 + n;
       const long a = am + ks;
@@ -966,9 +974,11 @@ struct ViMainV {
         const double gm = mu_m / beta;
         beta = mc_k - ml_k * gm;
         u = (S - ml_k * u_prev) / beta;
-        Go[a] = gm;
+        if (NK > 0) gb[NK > 0 ? n : 0] = gm;
+        else Go[a] = gm;
       }
-      Uo[a] = u;
+      if (NK > 0) ub[NK > 0 ? n : 0] = u;
+      else Uo[a] = u;
       u_prev = u;
       mu_m = mu_k;
       pe_m = pe_k; hv1_m = hv1_k; pg_m = pg_k; gr_m = gr_k;
@@ -978,13 +988,22 @@ struct ViMainV {
     long a = b + (long)kmax * ks;
     double xv = u_prev;
     Xo[a] = (real)(xv * 1.0);
-    double gm_next = Go[a];
+    if (NK > 0) {
+#pragma unroll
+      for (int n = N - 2; n >= 0; --n) {
+        a -= ks;
+        xv = ub[NK > 0 ? n : 0] - gb[NK > 0 ? n + 1 : 0] * xv;
+        Xo[a] = (real)(xv * 1.0);
+      }
+    } else {
+      double gm_next = Go[a];
 #pragma unroll 4
-    for (int n = N - 2; n >= 0; --n) {
-      a -= ks;
-      xv = Uo[a] - gm_next * xv;
-      gm_next = Go[a];
-      Xo[a] = (real)(xv * 1.0);
+      for (int n = N - 2; n >= 0; --n) {
+        a -= ks;
+        xv = Uo[a] - gm_next * xv;
+        gm_next = Go[a];
+        Xo[a] = (real)(xv * 1.0);
+      }
     }
     Xo[b + (long)kmin * ks] = (real)(0.0 * 1.0);
     Xo[b + (long)(kmax + 1) * ks] = (real)(-0.0 * 1.0);

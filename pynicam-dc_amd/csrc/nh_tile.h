@@ -108,226 +108,21 @@ struct StagedPointT {
   }
 };
 
+// Staging pays where the stencil loads are 8-byte (the FP64 form: 10-20 %
+// faster on MI355X at gl07); with 4-byte fields the direct point kernel is as
+// fast or faster, except for kernels that set P::kStageF32.
 template <class X, class P>
 void run_staged(X& x, const P& pk) {
-#ifdef NH_NO_STAGE
-  x.run_pts(pk, pk.d);
-#else
+  if (sizeof(real) == sizeof(float) && !P::kStageF32) {
+    x.run_pts(pk, pk.d);
+    return;
+  }
   StagedPointT<P> f;
   f.pk = pk;
   f.tg = tile_geom(pk.d, 0);
   x.run_tiles(f, f.tg.nblocks, P::kNStage * (f.tg.TH + 2) * pk.d.g);
-#endif
 }
 
-#ifdef NH_DD_STAGE
-// ----------------------------------------------------------------------------
-// first operator: triangle-mean 3-D divergence, gradient back to the cells
-// (kernels/oprt3ddivdamp.py:50-173).  Three phases: the tile's points of
-// (vx, vy, vz) and of rhogw at the two bounding half levels are staged in LDS
-// (rows j0-1 .. j0+rows, columns 0 .. g-1, each value loaded once); the
-// triangle values of every corner are formed from LDS; then the hexagon sums.
-// ----------------------------------------------------------------------------
-struct DivDamp1T {
-  using LdsT = real;
-  static constexpr bool kStage = true;
-  static constexpr const char* kName = "divdamp_3d";
-  static int lds_elems(const TileGeom& tg, int g) { return 5 * (tg.TH + 2) * g + tg.lds; }
-  Dims d; Consts c; Geo G; TileGeom tg;
-  const real *rvx, *rvy, *rvz, *rw;
-  real* out[3];
-  NH_HD bool active(int k) const { return k >= d.kmin && k <= d.kmax; }
-  // LDS: staged fields [f][r][col] (f: vx, vy, vz, rhogw(k), rhogw(k+1)), then the triangles
-  NH_HD int sf() const { return (tg.TH + 2) * d.g; }
-  NH_HD int tbase() const { return 5 * sf(); }
-  NH_HD void phase0(int blk, int tid, real* lds) const {
-    if (blk >= tg.ntiles) return;
-    const TileAt t = tile_at(d, tg, blk);
-    if (!active(t.k)) return;
-    const int g = d.g;
-    const long gg = (long)g * g;
-    const long base = (long)(t.l * d.kall + t.k) * gg;
-    const int npl = (t.rows + 2) * g;
-    const real* v3[3] = {rvx, rvy, rvz};
-    for (int idx = tid; idx < 5 * npl; idx += 256) {
-      const int f = idx / npl, rem = idx - f * npl;
-      const int r = rem / g, col = rem - r * g;
-      const long a = base + (long)(t.j0 - 1 + r) * g + col;
-      real v;
-      if (f < 3) {
-        v = v3[f][a];
-      } else {
-        const int kk = t.k + (f - 3);     // rhogw at half levels k and k+1, zero outside kmin+1..kmax
-        v = (kk >= d.kmin + 1 && kk <= d.kmax) ? (real(0.0) * real(1.0) + rw[a + (long)(f - 3) * gg] * real(1.0))
-                                               : real(0.0);
-      }
-      lds[f * sf() + r * g + col] = v;
-    }
-  }
-  NH_HD void phase1(int blk, int tid, real* lds) const {
-    if (blk >= tg.ntiles) return;
-    const TileAt t = tile_at(d, tg, blk);
-    if (!active(t.k)) return;
-    const int g = d.g, W1 = tg.W + 1;
-    const int nc = W1 * (t.rows + 1);
-    const real* S = lds;
-    real* Tl = lds + tbase();
-    for (int cidx = tid; cidx < nc; cidx += 256) {
-      const int ci_ = cidx % W1, cj_ = cidx / W1;
-      const int i = ci_, j = t.j0 - 1 + cj_;
-      const long cj = ((long)t.l * g + j) * g + i;
-      // staged points of corner (i, j): p0 (i, j), then (i+1, j), (i+1, j+1), (i, j+1)
-      const int s00 = cj_ * g + i, s10 = s00 + 1, s11 = s00 + 1 + g, s01 = s00 + g;
-      for (int T = 0; T < 2; ++T) {
-        const int p1 = (T == 0) ? s10 : s11;
-        const int p2 = (T == 0) ? s11 : s01;
-        real sv = real(0.0);
-        for (int dd = 0; dd < 3; ++dd) {
-          const real c0 = rc(G.cintp, d.nreg, (dd * 3 + 0) * 2 + T, cj);
-          const real c1 = rc(G.cintp, d.nreg, (dd * 3 + 1) * 2 + T, cj);
-          const real c2 = rc(G.cintp, d.nreg, (dd * 3 + 2) * 2 + T, cj);
-          const real* F = S + dd * sf();
-          const real t0 = c0 * (real(1.0) * F[s00]);
-          const real t1 = c1 * (real(1.0) * F[p1]);
-          const real t2 = c2 * (real(1.0) * F[p2]);
-          sv = (dd == 0) ? (t0 + t1 + t2) : (sv + t0 + t1 + t2);
-        }
-        const real* Wd = S + 3 * sf();
-        const real* Wu = S + 4 * sf();
-        const real up = (Wu[s00] + Wu[p1] + Wu[p2]);
-        const real dn = (Wd[s00] + Wd[p1] + Wd[p2]);
-        Tl[(T * (t.rows + 1) + cj_) * W1 + ci_] = sv + (up - dn) / real(3.0) * G.rdgz[t.k];
-      }
-    }
-  }
-  NH_HD void phase2(int blk, int tid, const real* lds) const {
-    const int g = d.g;
-    if (blk >= tg.ntiles) {
-      pole(blk - tg.ntiles, tid);
-      return;
-    }
-    const TileAt t = tile_at(d, tg, blk);
-    const int W1 = tg.W + 1;
-    const int ti = tid % tg.W, tj = tid / tg.W;
-    if (tj >= t.rows) return;
-    const int i = ti + 1, j = t.j0 + tj;
-    const long ci = ((long)t.l * g + j) * g + i;
-    const long a = ((long)(t.l * d.kall + t.k) * g + j) * g + i;
-    if (!active(t.k)) {
-      for (int q = 0; q < 3; ++q) out[q][a] = real(0.0);
-      return;
-    }
-    const int R = t.rows + 1;
-    const real* Tl = lds + tbase();
-    auto L = [&](int T, int ii, int jj) { return Tl[(T * R + (jj - t.j0 + 1)) * W1 + ii]; };
-    real T6[6] = {L(0, i, j), L(1, i, j), L(0, i - 1, j), L(1, i - 1, j - 1), L(0, i - 1, j - 1), L(1, i, j - 1)};
-    if (i == 1 && j == 1) {
-      const real m0 = G.sgp[t.l] ? real(0.0) : real(1.0), m1 = G.sgp[t.l] ? real(1.0) : real(0.0);
-      T6[4] = T6[4] * m0 + T6[5] * m1;
-    }
-    for (int dd = 0; dd < 3; ++dd) out[dd][a] = hex_sum(G, d.nreg, ci, dd, T6);
-  }
-  NH_HD void pole(int pb, int tid) const {
-    if (tid != 0) return;
-    DivDamp1P p{d, c, G, rvx, rvy, rvz, rw, {out[0], out[1], out[2]}};
-    p(d.lall * d.kall + pb, 0);
-  }
-};
-
-// second operator on -(first result), coefficient, projection
-// (kernels/oprtdivdamp.py:54-173, kernels/divdamppostcomm.py)
-struct DivDamp2T {
-  using LdsT = real;
-  static constexpr bool kStage = true;
-  static constexpr const char* kName = "divdamp_post";
-  static int lds_elems(const TileGeom& tg, int g) { return 3 * (tg.TH + 2) * g + tg.lds; }
-  Dims d; Consts c; Geo G; TileGeom tg;
-  const real* in[3];
-  real* out[3];
-  NH_HD bool active(int k) const { return k >= d.kmin && k <= d.kmax; }
-  // LDS: the three staged input fields [f][r][col], rows j0-1 .. j0+rows, then the triangles
-  NH_HD int sf() const { return (tg.TH + 2) * d.g; }
-  NH_HD int tbase() const { return 3 * sf(); }
-  NH_HD void phase0(int blk, int tid, real* lds) const {
-    if (blk >= tg.ntiles) return;
-    const TileAt t = tile_at(d, tg, blk);
-    if (!active(t.k)) return;
-    const int g = d.g;
-    const long base = (long)(t.l * d.kall + t.k) * g * g;
-    const int npl = (t.rows + 2) * g;
-    for (int idx = tid; idx < 3 * npl; idx += 256) {
-      const int f = idx / npl, rem = idx - f * npl;
-      const int r = rem / g, col = rem - r * g;
-      lds[f * sf() + r * g + col] = in[f][base + (long)(t.j0 - 1 + r) * g + col];
-    }
-  }
-  NH_HD void phase1(int blk, int tid, real* lds) const {
-    if (blk >= tg.ntiles) return;
-    const TileAt t = tile_at(d, tg, blk);
-    if (!active(t.k)) return;
-    const int g = d.g, W1 = tg.W + 1;
-    const int nc = W1 * (t.rows + 1);
-    real* Tl = lds + tbase();
-    for (int cidx = tid; cidx < nc; cidx += 256) {
-      const int ci_ = cidx % W1, cj_ = cidx / W1;
-      const int i = ci_, j = t.j0 - 1 + cj_;
-      const long cj = ((long)t.l * g + j) * g + i;
-      const int s00 = cj_ * g + i, s10 = s00 + 1, s11 = s00 + 1 + g, s01 = s00 + g;
-      for (int T = 0; T < 2; ++T) {
-        // tri_div of -(in) at the corner, from the staged values
-        const int p1 = (T == 0) ? s10 : s11;
-        const int p2 = (T == 0) ? s11 : s01;
-        real sv = real(0.0);
-        for (int dd = 0; dd < 3; ++dd) {
-          const real c0 = rc(G.cintp, d.nreg, (dd * 3 + 0) * 2 + T, cj);
-          const real c1 = rc(G.cintp, d.nreg, (dd * 3 + 1) * 2 + T, cj);
-          const real c2 = rc(G.cintp, d.nreg, (dd * 3 + 2) * 2 + T, cj);
-          const real* F = lds + dd * sf();
-          const real t0 = c0 * (-real(1.0) * F[s00]);
-          const real t1 = c1 * (-real(1.0) * F[p1]);
-          const real t2 = c2 * (-real(1.0) * F[p2]);
-          sv = (dd == 0) ? (t0 + t1 + t2) : (sv + t0 + t1 + t2);
-        }
-        Tl[(T * (t.rows + 1) + cj_) * W1 + ci_] = sv;
-      }
-    }
-  }
-  NH_HD void phase2(int blk, int tid, const real* lds) const {
-    const int g = d.g;
-    if (blk >= tg.ntiles) {
-      if (tid == 0) {
-        DivDamp2P p{d, c, G, {in[0], in[1], in[2]}, {out[0], out[1], out[2]}};
-        p(d.lall * d.kall + (blk - tg.ntiles), 0);
-      }
-      return;
-    }
-    const TileAt t = tile_at(d, tg, blk);
-    const int W1 = tg.W + 1;
-    const int ti = tid % tg.W, tj = tid / tg.W;
-    if (tj >= t.rows) return;
-    const int i = ti + 1, j = t.j0 + tj;
-    const long ci = ((long)t.l * g + j) * g + i;
-    const long a = ((long)(t.l * d.kall + t.k) * g + j) * g + i;
-    real o[3] = {real(0.0), real(0.0), real(0.0)};
-    if (active(t.k)) {
-      const int R = t.rows + 1;
-      const real* Tl = lds + tbase();
-      auto L = [&](int T, int ii, int jj) { return Tl[(T * R + (jj - t.j0 + 1)) * W1 + ii]; };
-      const real T6[6] = {L(0, i, j), L(1, i, j), L(0, i - 1, j), L(1, i - 1, j - 1), L(0, i - 1, j - 1),
-                            L(1, i, j - 1)};
-      for (int dd = 0; dd < 3; ++dd) o[dd] = hex_sum(G, d.nreg, ci, dd, T6);
-    }
-    const real Rr = c.RADIUS;
-    const real gd[3] = {c.ALPHA_D * o[0], c.ALPHA_D * o[1], c.ALPHA_D * o[2]};
-    const real gx = G.gx[ci], gy = G.gx[d.nreg + ci], gz = G.gx[2 * d.nreg + ci];
-    const real prd = (gd[0] * gx + gd[1] * gy + gd[2] * gz) / Rr;
-    out[0][a] = gd[0] - prd * gx / Rr;
-    out[1][a] = gd[1] - prd * gy / Rr;
-    out[2][a] = gd[2] - prd * gz / Rr;
-  }
-};
-
-#else
 // ----------------------------------------------------------------------------
 // first operator: triangle-mean 3-D divergence, gradient back to the cells
 // (kernels/oprt3ddivdamp.py:50-173)
@@ -459,7 +254,6 @@ struct DivDamp2T {
   }
 };
 
-#endif
 
 // ----------------------------------------------------------------------------
 // second hyperdiffusion pass (mod_numfilter.py:1125-1264): Laplacians of the
