@@ -277,39 +277,19 @@ inline long ncols(const Dims& d) { return d.nreg + 12; }
 template <class X>
 void divdamp(X& x, const Context& ctx, const real* vx, const real* vy, const real* vz, const real* w,
              real* out0, real* out1, real* out2) {
-  real* tmp[3] = {ctx.field(W_DDT), ctx.field(W_DDT + 1), ctx.field(W_DDT + 2)};
-#ifdef NH_DD_KB
-  const KbGeom kg = kb_geom(ctx.d, NH_DD_KB);
-  DivDampKb<1> a;
-  a.d = ctx.d; a.c = ctx.c; a.G = ctx.G; a.kg = kg;
-  a.in[0] = vx; a.in[1] = vy; a.in[2] = vz; a.in[3] = w;
-  for (int q = 0; q < 3; ++q) a.out[q] = tmp[q];
-  x.run_tiles_kb(a, kg.ntiles, kg.tg.lds);
-  x.run(PoleCentres<DivDamp1P>{DivDamp1P{ctx.d, ctx.c, ctx.G, vx, vy, vz, w, {tmp[0], tmp[1], tmp[2]}}},
-        2L * ctx.d.kall);
-  halo(x, ctx, tmp, 3);
-  DivDampKb<2> b;
-  b.d = ctx.d; b.c = ctx.c; b.G = ctx.G; b.kg = kg;
-  for (int q = 0; q < 3; ++q) b.in[q] = tmp[q];
-  b.in[3] = nullptr;
-  b.out[0] = out0; b.out[1] = out1; b.out[2] = out2;
-  x.run_tiles_kb(b, kg.ntiles, kg.tg.lds);
-  x.run(PoleCentres<DivDamp2P>{DivDamp2P{ctx.d, ctx.c, ctx.G, {tmp[0], tmp[1], tmp[2]}, {out0, out1, out2}}},
-        2L * ctx.d.kall);
-#else
   const TileGeom tg = tile_geom(ctx.d, 2);
   DivDamp1T a;
   a.d = ctx.d; a.c = ctx.c; a.G = ctx.G; a.tg = tg;
   a.rvx = vx; a.rvy = vy; a.rvz = vz; a.rw = w;
-  for (int q = 0; q < 3; ++q) a.out[q] = tmp[q];
+  for (int q = 0; q < 3; ++q) a.out[q] = ctx.field(W_DDT + q);
   x.run_tiles(a, tg.nblocks, DivDamp1T::lds_elems(tg, ctx.d.g));
+  real* tmp[3] = {ctx.field(W_DDT), ctx.field(W_DDT + 1), ctx.field(W_DDT + 2)};
   halo(x, ctx, tmp, 3);
   DivDamp2T b;
   b.d = ctx.d; b.c = ctx.c; b.G = ctx.G; b.tg = tg;
   for (int q = 0; q < 3; ++q) b.in[q] = tmp[q];
   b.out[0] = out0; b.out[1] = out1; b.out[2] = out2;
   x.run_tiles(b, tg.nblocks, DivDamp2T::lds_elems(tg, ctx.d.g));
-#endif
 }
 
 // ----------------------------------------------------------------------------

@@ -78,30 +78,6 @@ __global__ __launch_bounds__(256) void k_tiles(F f, int nblk) {
   f.phase2(blk, (int)threadIdx.x, lds);
 }
 
-// level-batched tiles: per-thread registers F::Regs set once by f.init, then
-// for each of the block's levels phase 1 and phase 2 with barriers
-template <class F>
-__global__ __launch_bounds__(256) void k_tiles_kb(F f, int nblk) {
-  extern __shared__ __align__(16) unsigned char nh_lds_bytes[];
-  typename F::LdsT* lds = reinterpret_cast<typename F::LdsT*>(nh_lds_bytes);
-  const int L = blockIdx.x;
-  const int per = (nblk + kXcds - 1) / kXcds;
-  const int xcd = L % kXcds, idx = L / kXcds;
-  const int rem = nblk % kXcds;
-  const int blk = (rem == 0 || xcd < rem) ? xcd * per + idx : rem * per + (xcd - rem) * (per - 1) + idx;
-  const int tid = (int)threadIdx.x;
-  typename F::Regs r;
-  f.init(blk, tid, r);
-  const int nk = f.nlev(blk);
-#pragma unroll 1
-  for (int kb = 0; kb < nk; ++kb) {
-    f.phase1(blk, kb, tid, r, lds);
-    __syncthreads();
-    f.phase2(blk, kb, tid, r, lds);
-    __syncthreads();
-  }
-}
-
 class HipExec {
  public:
   HipExec(int device, void* stream, void** prof_slot);
@@ -163,16 +139,6 @@ class HipExec {
     if (pair) note(hipEventRecord(pair[0], s_), "hipEventRecord");
     hipLaunchKernelGGL(k_tiles<F>, dim3(nblocks), dim3(256), (size_t)lds_elems * sizeof(typename F::LdsT), s_, f,
                        nblocks);
-    note(hipGetLastError(), "kernel launch");
-    if (pair) note(hipEventRecord(pair[1], s_), "hipEventRecord");
-    if (sync_each_) note(hipStreamSynchronize(s_), "kernel execution");
-  }
-  template <class F>
-  void run_tiles_kb(const F& f, int nblocks, int lds_elems) {
-    hipEvent_t* pair = prof_ && prof_->on ? next_pair(F::kName) : nullptr;
-    if (pair) note(hipEventRecord(pair[0], s_), "hipEventRecord");
-    hipLaunchKernelGGL(k_tiles_kb<F>, dim3(nblocks), dim3(256), (size_t)lds_elems * sizeof(typename F::LdsT), s_,
-                       f, nblocks);
     note(hipGetLastError(), "kernel launch");
     if (pair) note(hipEventRecord(pair[1], s_), "hipEventRecord");
     if (sync_each_) note(hipStreamSynchronize(s_), "kernel execution");

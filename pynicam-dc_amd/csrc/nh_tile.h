@@ -346,170 +346,3 @@ struct HDiffPass2T {
 };
 
 NH_END
-
-NH_BEGIN
-
-// ----------------------------------------------------------------------------
-// Divergence damping on level-batched tiles: a block owns a tile of one region
-// for NKB consecutive levels.  Each thread loads the interpolation weights of
-// its corners and the arc coefficients of its cell once (Regs) and reuses them
-// on every level: per level, phase 1 forms the corners' triangle values into
-// LDS and phase 2 the cells' hexagon sums (kernels/oprt3ddivdamp.py:50-173,
-// kernels/oprtdivdamp.py:54-173, kernels/divdamppostcomm.py).  The arithmetic
-// of each value is that of DivDamp1T / DivDamp2T.  The pole centres run in a
-// separate launch of the point versions (PoleCentres), which keeps their
-// registers out of this kernel.
-// ----------------------------------------------------------------------------
-struct KbGeom {
-  TileGeom tg;
-  int nkb;        // levels per block
-  int ngroups;    // level groups per region
-  int ntiles;     // regional blocks
-  int nblocks;    // + 2 * kall pole blocks
-};
-
-inline KbGeom kb_geom(const Dims& d, int nkb) {
-  KbGeom k;
-  k.tg = tile_geom(d, 2);
-  k.nkb = nkb;
-  k.ngroups = (d.kall + nkb - 1) / nkb;
-  k.ntiles = d.lall * k.ngroups * k.tg.ntp;
-  k.nblocks = k.ntiles + 2 * d.kall;
-  return k;
-}
-
-struct KbAt {
-  int l, k0, nk, j0, rows;
-};
-
-NH_HD inline KbAt kb_at(const Dims& d, const KbGeom& kg, int blk) {
-  KbAt t;
-  const int pg = blk / kg.tg.ntp;
-  const int tile = blk - pg * kg.tg.ntp;
-  t.l = pg / kg.ngroups;
-  t.k0 = (pg - t.l * kg.ngroups) * kg.nkb;
-  t.nk = (d.kall - t.k0 < kg.nkb) ? d.kall - t.k0 : kg.nkb;
-  t.j0 = 1 + tile * kg.tg.TH;
-  t.rows = (d.g - 1 - t.j0 < kg.tg.TH) ? d.g - 1 - t.j0 : kg.tg.TH;
-  return t;
-}
-
-// the pole-centre points of a point functor F, one thread per pole level
-template <class F>
-struct PoleCentres {
-  static constexpr const char* kName = F::kName;
-  F f;
-  NH_HD void operator()(long t) const { f(f.d.lall * f.d.kall + (int)t, 0); }
-};
-
-template <int OP>   // 1: first operator (3-D divergence of the momentum), 2: second (on -first, projected)
-struct DivDampKb {
-  using LdsT = real;
-  static constexpr const char* kName = OP == 1 ? "divdamp_3d" : "divdamp_post";
-  struct Regs {
-    real ci_w[18];      // interpolation weights of the thread's first corner [(dd*3+v)*2+T]
-    real cd[18];        // arc coefficients of the thread's cell [dd*6+arc]
-    real gx[3];         // position of the thread's cell (second operator)
-  };
-  Dims d; Consts c; Geo G; KbGeom kg;
-  const real* in[4];    // OP 1: rhogvx, rhogvy, rhogvz, rhogw; OP 2: the first operator's 3 fields
-  real* out[3];
-  NH_HD int nlev(int blk) const { return kb_at(d, kg, blk).nk; }
-  NH_HD bool active(int k) const { return k >= d.kmin && k <= d.kmax; }
-  NH_HD void init(int blk, int tid, Regs& r) const {
-    const KbAt t = kb_at(d, kg, blk);
-    const int g = d.g, W1 = kg.tg.W + 1, nc = W1 * (t.rows + 1);
-    if (tid < nc) {
-      const int i = tid % W1, j = t.j0 - 1 + tid / W1;
-      const long cj = ((long)t.l * g + j) * g + i;
-      for (int q = 0; q < 18; ++q) r.ci_w[q] = rc(G.cintp, d.nreg, q, cj);
-    }
-    const int ti = tid % kg.tg.W, tj = tid / kg.tg.W;
-    if (tj < t.rows) {
-      const long ci = ((long)t.l * g + t.j0 + tj) * g + ti + 1;
-      for (int q = 0; q < 18; ++q) r.cd[q] = rc(G.cdiff, d.nreg, q, ci);
-      for (int q = 0; q < 3; ++q) r.gx[q] = rc(G.gx, d.nreg, q, ci);
-    }
-  }
-  NH_HD real rwv(long aa, int kk) const {
-    return (kk >= d.kmin + 1 && kk <= d.kmax) ? (real(0.0) * real(1.0) + in[3][aa] * real(1.0)) : real(0.0);
-  }
-  NH_HD void phase1(int blk, int kb, int tid, const Regs& r, real* lds) const {
-    const KbAt t = kb_at(d, kg, blk);
-    const int k = t.k0 + kb;
-    if (!active(k)) return;
-    const int g = d.g, W1 = kg.tg.W + 1, nc = W1 * (t.rows + 1);
-    const long ks = (long)g * g;
-    const real sgn = (OP == 1) ? real(1.0) : -real(1.0);
-    for (int u = 0; u < 2; ++u) {
-      const int cidx = tid + 256 * u;
-      if (cidx >= nc) continue;
-      const int ci_ = cidx % W1, cj_ = cidx / W1;
-      const int i = ci_, j = t.j0 - 1 + cj_;
-      const long a = ((long)(t.l * d.kall + k) * g + j) * g + i;
-      const long cj = ((long)t.l * g + j) * g + i;
-      // the first corner's weights are in registers; a second corner reads them
-      auto w = [&](int q) { return u == 0 ? r.ci_w[q] : rc(G.cintp, d.nreg, q, cj); };
-      for (int T = 0; T < 2; ++T) {
-        const long o1 = (T == 0) ? 1 : 1 + g;
-        const long o2 = (T == 0) ? 1 + g : g;
-        real s = real(0.0);
-        for (int dd = 0; dd < 3; ++dd) {
-          const real c0 = w((dd * 3 + 0) * 2 + T);
-          const real c1 = w((dd * 3 + 1) * 2 + T);
-          const real c2 = w((dd * 3 + 2) * 2 + T);
-          const real t0 = c0 * (sgn * in[dd][a]);
-          const real t1 = c1 * (sgn * in[dd][a + o1]);
-          const real t2 = c2 * (sgn * in[dd][a + o2]);
-          s = (dd == 0) ? (t0 + t1 + t2) : (s + t0 + t1 + t2);
-        }
-        if (OP == 1) {
-          const real up = (rwv(a + ks, k + 1) + rwv(a + o1 + ks, k + 1) + rwv(a + o2 + ks, k + 1));
-          const real dn = (rwv(a, k) + rwv(a + o1, k) + rwv(a + o2, k));
-          s = s + (up - dn) / real(3.0) * G.rdgz[k];
-        }
-        lds[(T * (t.rows + 1) + cj_) * W1 + ci_] = s;
-      }
-    }
-  }
-  NH_HD void phase2(int blk, int kb, int tid, const Regs& r, const real* lds) const {
-    const KbAt t = kb_at(d, kg, blk);
-    const int k = t.k0 + kb;
-    const int g = d.g, W1 = kg.tg.W + 1;
-    const int ti = tid % kg.tg.W, tj = tid / kg.tg.W;
-    if (tj >= t.rows) return;
-    const int i = ti + 1, j = t.j0 + tj;
-    const long a = ((long)(t.l * d.kall + k) * g + j) * g + i;
-    real o[3] = {real(0.0), real(0.0), real(0.0)};
-    if (active(k)) {
-      const int R = t.rows + 1;
-      auto L = [&](int T, int ii, int jj) { return lds[(T * R + (jj - t.j0 + 1)) * W1 + ii]; };
-      real T6[6] = {L(0, i, j), L(1, i, j), L(0, i - 1, j), L(1, i - 1, j - 1), L(0, i - 1, j - 1), L(1, i, j - 1)};
-      if (OP == 1 && i == 1 && j == 1) {
-        const real m0 = G.sgp[t.l] ? real(0.0) : real(1.0), m1 = G.sgp[t.l] ? real(1.0) : real(0.0);
-        T6[4] = T6[4] * m0 + T6[5] * m1;
-      }
-      for (int dd = 0; dd < 3; ++dd) {
-        const real* cd = r.cd + dd * 6;
-        o[dd] = cd[0] * (T6[0] + T6[1]) + cd[1] * (T6[1] + T6[2]) + cd[2] * (T6[2] + T6[3])
-              + cd[3] * (T6[3] + T6[4]) + cd[4] * (T6[4] + T6[5]) + cd[5] * (T6[5] + T6[0]);
-      }
-    } else if (OP == 1) {
-      for (int q = 0; q < 3; ++q) out[q][a] = real(0.0);
-      return;
-    }
-    if (OP == 1) {
-      for (int q = 0; q < 3; ++q) out[q][a] = o[q];
-      return;
-    }
-    const real Rr = c.RADIUS;
-    const real gd[3] = {c.ALPHA_D * o[0], c.ALPHA_D * o[1], c.ALPHA_D * o[2]};
-    const real gx = r.gx[0], gy = r.gx[1], gz = r.gx[2];
-    const real prd = (gd[0] * gx + gd[1] * gy + gd[2] * gz) / Rr;
-    out[0][a] = gd[0] - prd * gx / Rr;
-    out[1][a] = gd[1] - prd * gy / Rr;
-    out[2][a] = gd[2] - prd * gz / Rr;
-  }
-};
-
-NH_END

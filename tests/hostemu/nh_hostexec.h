@@ -103,27 +103,6 @@ class HostExec {
       prof_->rec.emplace_back(F::kName,
                               std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
   }
-  template <class F>
-  void run_tiles_kb(const F& f, int nblocks, int lds_elems) {
-    auto t0 = std::chrono::steady_clock::now();
-#pragma omp parallel
-    {
-      std::vector<typename F::LdsT> lds((size_t)lds_elems + 1);
-      std::vector<typename F::Regs> regs(256);
-#pragma omp for schedule(static)
-      for (int b = 0; b < nblocks; ++b) {
-        for (int t = 0; t < 256; ++t) f.init(b, t, regs[t]);
-        const int nk = f.nlev(b);
-        for (int kb = 0; kb < nk; ++kb) {
-          for (int t = 0; t < 256; ++t) f.phase1(b, kb, t, regs[t], lds.data());
-          for (int t = 0; t < 256; ++t) f.phase2(b, kb, t, regs[t], lds.data());
-        }
-      }
-    }
-    if (prof_ && prof_->on)
-      prof_->rec.emplace_back(F::kName,
-                              std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
-  }
   void sync() {}
   int check(const char*) { return 0; }
   std::string message() const { return "host"; }
