@@ -217,6 +217,33 @@ int nh_budget(nh_context* ctx, const double* prog, const double* progq, int32_t 
 int nh_budget_f32(nh_context* ctx, const float* prog, const float* progq, int32_t nq, double lhv, double* out,
                   void* stream);
 
+/* Adjoint of one nh_dyn_step (no counterpart in the reference, which has no
+ * adjoint; the forward call it transposes is Dyn.dynamics_step,
+ * nhm/dynamics/mod_dynamics.py:1725).  prog, progq: the state the forward step
+ * starts from (unchanged).  prog_b: on entry the gradient of a scalar with
+ * respect to the state after the step, on exit its gradient with respect to
+ * prog.  The step is recomputed inside from prog (checkpoints of every RK
+ * stage and small step in a workspace the first call allocates).  Tracer
+ * masses are held at progq: the result is the full derivative when they are
+ * zero (dry runs).  Needs the halo map with unique destinations that are
+ * never sources (else NH_ERR_STATE). */
+int nh_dyn_step_adj(nh_context* ctx, const double* prog, const double* progq, int32_t nq, double dtl,
+                    int32_t sstep_max, double* prog_b, void* stream);
+int nh_dyn_step_adj_f32(nh_context* ctx, const float* prog, const float* progq, int32_t nq, double dtl,
+                        int32_t sstep_max, float* prog_b, void* stream);
+
+/* Surface pressure per column (Bndc.BNDCND_pre_sfc, nhm/share/mod_bndcnd.py:521-555,
+ * with a flat surface z_srf = 0): density extrapolated quadratically from the
+ * lowest three levels, hydrostatic pressure from kmin.  ps: nreg + 12 doubles
+ * (regional columns (l*g + j)*g + i, then the pole columns lp*6 + v).  The _adj
+ * form adds the transpose applied to ps_b (per column) into prog_b. */
+int nh_sfc_pressure(nh_context* ctx, const double* prog, const double* progq, double* ps, void* stream);
+int nh_sfc_pressure_f32(nh_context* ctx, const float* prog, const float* progq, double* ps, void* stream);
+int nh_sfc_pressure_adj(nh_context* ctx, const double* prog, const double* progq, const double* ps_b,
+                        double* prog_b, void* stream);
+int nh_sfc_pressure_adj_f32(nh_context* ctx, const float* prog, const float* progq, const double* ps_b,
+                            float* prog_b, void* stream);
+
 /* Kernel timers (the per-kernel counterpart of the section timers
  * PROF_rapstart / PROF_rapend / PROF_rapreport, share/mod_prof.py:81-140).
  * While enabled, every launch of a context is bracketed by two events on its

@@ -1,0 +1,1498 @@
+// Adjoint (transposed) kernels of the dynamics step.
+//
+// Each functor XxxA is the transpose of the Jacobian of the forward functor
+// Xxx at the forward state it is given: it reads the adjoint of Xxx's outputs
+// ("bars"), consumes them (sets them to zero, so that a buffer reused by the
+// forward step starts clean for its previous use), and adds the adjoint of
+// Xxx's inputs into the input bars.  The forward values a kernel needs (for
+// products, quotients) are read from the same fields the forward kernel read;
+// the step driver recomputes them before each adjoint kernel.
+//
+// Stencil kernels are written in scatter form: every compute point adds its
+// contribution to the bars of the inputs it read, with atomic adds, so that a
+// kernel's adjoint follows the forward code line by line.  Column and
+// point-local kernels (the tridiagonal solve, the pointwise updates) write
+// without atomics.  The linearisation is exact for every smooth term;
+// sign/branch switches (ghost-row selection, k ranges) follow the forward.
+//
+// Tracer masses enter the dynamics only through q = rhogq / rhog in the
+// thermodynamics; they are held at their trajectory values here, so the
+// adjoint is the full derivative when the tracers are zero (the dry runs).
+#pragma once
+
+#include "nh_point.h"
+#include "nh_tile.h"
+
+NH_BEGIN
+
+NH_HD inline void aadd(float* p, float v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  unsafeAtomicAdd(p, v);
+#else
+#pragma omp atomic
+  *p += v;
+#endif
+}
+NH_HD inline void aadd(double* p, double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  unsafeAtomicAdd(p, v);
+#else
+#pragma omp atomic
+  *p += v;
+#endif
+}
+
+// regional stencil offset of neighbour m: (0,0) (+1,0) (+1,+1) (0,+1) (-1,0) (-1,-1) (0,-1)
+NH_HD inline long nb_off(int m, long g) {
+  return (((4201 >> (2 * m)) & 3) - 1) + (((421 >> (2 * m)) & 3) - 1) * g;
+}
+
+// position of a compute point for the horizontal projections: the pole
+// centre's GRD_x (pole) or the cell's (regional)
+NH_HD inline void point_pos(const Geo& G, const Dims& d, const Pt& p, real x[3]) {
+  if (p.pole) {
+    for (int q = 0; q < 3; ++q) x[q] = G.gx_pl[p.lp * 3 + q];
+  } else {
+    for (int q = 0; q < 3; ++q) x[q] = G.gx[q * d.nreg + p.ci];
+  }
+}
+
+// transpose of o = v - ((v.x)/R) x/R (symmetric): in place on v
+NH_HD inline void proj_t(const real x[3], real R, real v[3]) {
+  const real prd = (v[0] * x[0] + v[1] * x[1] + v[2] * x[2]) / R;
+  for (int q = 0; q < 3; ++q) v[q] = v[q] - prd * x[q] / R;
+}
+
+// ----------------------------------------------------------------------------
+// halo gather: f[dst] = f[src]  ->  fb[src] += fb[dst]; fb[dst] = 0
+// (the host checks that destinations are unique and are never sources)
+// ----------------------------------------------------------------------------
+struct HaloScatterA {
+  static constexpr const char* kName = "halo_adj";
+  Dims d;
+  const int* dst;
+  const int* src;
+  long n;
+  real* f[16];
+  int nf;
+  NH_HD void operator()(long t) const {
+    HaloGather h;
+    h.d = d;
+    const long hh = t % n;
+    const int k = (int)(t / n);
+    long ks_d, ks_s;
+    const long bd = h.colbase(dst[hh], &ks_d) + k * ks_d;
+    const long bs = h.colbase(src[hh], &ks_s) + k * ks_s;
+    for (int q = 0; q < nf; ++q) {
+      const real v = f[q][bd];
+      f[q][bd] = real(0.0);
+      if (v != real(0.0)) aadd(&f[q][bs], v);
+    }
+  }
+};
+
+// zero an FP64 range
+struct FillZero64 {
+  static constexpr const char* kName = "fill_zero";
+  double* p;
+  NH_HD void operator()(long t) const { p[t] = 0.0; }
+};
+
+// y += x over a flat range
+struct AxpyP {
+  static constexpr const char* kName = "adj_accumulate";
+  real* y;
+  const real* x;
+  NH_HD void operator()(long t) const { y[t] += x[t]; }
+};
+
+// ----------------------------------------------------------------------------
+// split state at a stage start: ps = prog0 - prog (nl > 0) or 0
+// ----------------------------------------------------------------------------
+struct SplitInitA {
+  static constexpr const char* kName = "split_init_adj";
+  long NF;
+  real* prog_b;
+  real* prog0_b;
+  real* ps_b[6];
+  int zero;
+  NH_HD void operator()(long t) const {
+    for (int q = 0; q < 6; ++q) {
+      const real v = ps_b[q][t];
+      ps_b[q][t] = real(0.0);
+      if (!zero) {
+        prog0_b[q * NF + t] += v;
+        prog_b[q * NF + t] -= v;
+      }
+    }
+  }
+};
+
+// ----------------------------------------------------------------------------
+// PROG = P(PROG + PROG_split): the bar of the updated PROG becomes the bar of
+// the stage's PROG and of the final split state (set, not added)
+// ----------------------------------------------------------------------------
+struct ProgUpdateA {
+  static constexpr const char* kName = "prog_update_adj";
+  Dims d; Consts c; Geo G;
+  real* prog_b;
+  real* ps_b[6];
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p)) return;
+    const long NF = d.NF, a = p.a;
+    const real R = c.RADIUS;
+    real v[6];
+    for (int q = 0; q < 6; ++q) v[q] = prog_b[q * NF + a];
+    if (p.pole || p.inner) {
+      real xx[3];
+      if (p.pole) {
+        for (int q = 0; q < 3; ++q) xx[q] = G.gx_pl[(p.v * 2 + p.lp) * 3 + q];
+      } else {
+        for (int q = 0; q < 3; ++q) xx[q] = G.gx[q * d.nreg + p.ci];
+      }
+      proj_t(xx, R, v + 1);
+    }
+    for (int q = 0; q < 6; ++q) {
+      prog_b[q * NF + a] = v[q];
+      ps_b[q][a] = v[q];
+    }
+  }
+};
+
+// ----------------------------------------------------------------------------
+// divergence damping (transposes of DivDamp1P / DivDamp2P and their tile forms)
+// ----------------------------------------------------------------------------
+// transpose of tri_div: sbar -> vb[dd] at the triangle's three corners
+NH_HD inline void tri_div_t(const Geo& G, long nreg, long cj, int T, real* const* vb, long a, int g, real sgn,
+                            real sbar) {
+  const long o1 = (T == 0) ? 1 : 1 + g;
+  const long o2 = (T == 0) ? 1 + g : g;
+  for (int dd = 0; dd < 3; ++dd) {
+    const real c0 = rc(G.cintp, nreg, (dd * 3 + 0) * 2 + T, cj);
+    const real c1 = rc(G.cintp, nreg, (dd * 3 + 1) * 2 + T, cj);
+    const real c2 = rc(G.cintp, nreg, (dd * 3 + 2) * 2 + T, cj);
+    aadd(&vb[dd][a], c0 * sgn * sbar);
+    aadd(&vb[dd][a + o1], c1 * sgn * sbar);
+    aadd(&vb[dd][a + o2], c2 * sgn * sbar);
+  }
+}
+
+// transpose of hex_sum over the three directions: o[dd] -> Tb[6]
+NH_HD inline void hex_sum_t(const Geo& G, long nreg, long ci, const real o[3], real Tb[6]) {
+  for (int n = 0; n < 6; ++n) Tb[n] = real(0.0);
+  for (int dd = 0; dd < 3; ++dd) {
+    const real* cd = G.cdiff + (long)(dd * 6) * nreg + ci;
+    const real c0 = cd[0], c1 = cd[nreg], c2 = cd[2 * nreg], c3 = cd[3 * nreg], c4 = cd[4 * nreg],
+               c5 = cd[5 * nreg];
+    Tb[0] += (c0 + c5) * o[dd];
+    Tb[1] += (c0 + c1) * o[dd];
+    Tb[2] += (c1 + c2) * o[dd];
+    Tb[3] += (c2 + c3) * o[dd];
+    Tb[4] += (c3 + c4) * o[dd];
+    Tb[5] += (c4 + c5) * o[dd];
+  }
+}
+
+// transpose of the pole ring sum row[dd] = sum_v cdiff_pl(v, dd) (sclt[v-1] + sclt[v-2])
+NH_HD inline void pole_ring_t(const Geo& G, int lp, const real o[3], real sb[5]) {
+  for (int v = 0; v < 5; ++v) sb[v] = real(0.0);
+  for (int v = 1; v <= 5; ++v) {
+    const int vm = (v == 1) ? 5 : v - 1;
+    real s = real(0.0);
+    for (int dd = 0; dd < 3; ++dd) s += G.cdiff_pl[(v * 2 + lp) * 3 + dd] * o[dd];
+    sb[vm - 1] += s;
+    sb[v - 1] += s;
+  }
+}
+
+struct DivDamp1A {
+  static constexpr const char* kName = "divdamp_3d_adj";
+  Dims d; Consts c; Geo G;
+  real* ob[3];    // bars of the operator's output (consumed)
+  real* vb[4];    // bars of rhogvx, rhogvy, rhogvz, rhogw
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    const int kmin = d.kmin, kmax = d.kmax, g = d.g, k = p.k;
+    const long ks = p.ks, a = p.a;
+    real o[3];
+    for (int q = 0; q < 3; ++q) {
+      o[q] = ob[q][a];
+      ob[q][a] = real(0.0);
+    }
+    if (k < kmin || k > kmax) return;
+    const real rz = G.rdgz[k] / real(3.0);
+    auto rwb = [&](long aa, int kk, real v) {
+      if (kk >= kmin + 1 && kk <= kmax) aadd(&vb[3][aa], v);
+    };
+    if (p.pole) {
+      real sb[5];
+      pole_ring_t(G, p.lp, o, sb);
+      for (int v = 1; v <= 5; ++v) {
+        const int vp = (v == 5) ? 1 : v + 1;
+        const real s = sb[v - 1];
+        for (int dd = 0; dd < 3; ++dd) {
+          const real* cI = G.cintp_pl + ((v * 2 + p.lp) * 3 + dd) * 3;
+          aadd(&vb[dd][a], cI[0] * s);
+          aadd(&vb[dd][a + v], cI[1] * s);
+          aadd(&vb[dd][a + vp], cI[2] * s);
+        }
+        rwb(a + ks, k + 1, s * rz); rwb(a + v + ks, k + 1, s * rz); rwb(a + vp + ks, k + 1, s * rz);
+        rwb(a, k, -s * rz); rwb(a + v, k, -s * rz); rwb(a + vp, k, -s * rz);
+      }
+      return;
+    }
+    const long nreg = d.nreg, ci = p.ci;
+    const long co[6] = {0, 0, -1, -1 - (long)g, -1 - (long)g, -(long)g};
+    const int ty[6] = {0, 1, 0, 1, 0, 1};
+    real Tb[6];
+    hex_sum_t(G, nreg, ci, o, Tb);
+    if (p.i == 1 && p.j == 1) {
+      const real m0 = G.sgp[p.l] ? real(0.0) : real(1.0), m1 = G.sgp[p.l] ? real(1.0) : real(0.0);
+      Tb[5] += Tb[4] * m1;
+      Tb[4] = Tb[4] * m0;
+    }
+    for (int n = 0; n < 6; ++n) {
+      const long aa = a + co[n];
+      const long o1 = (ty[n] == 0) ? 1 : 1 + g;
+      const long o2 = (ty[n] == 0) ? 1 + g : g;
+      tri_div_t(G, nreg, ci + co[n], ty[n], vb, aa, g, real(1.0), Tb[n]);
+      const real s = Tb[n] * rz;
+      rwb(aa + ks, k + 1, s); rwb(aa + o1 + ks, k + 1, s); rwb(aa + o2 + ks, k + 1, s);
+      rwb(aa, k, -s); rwb(aa + o1, k, -s); rwb(aa + o2, k, -s);
+    }
+  }
+};
+
+struct DivDamp2A {
+  static constexpr const char* kName = "divdamp_post_adj";
+  Dims d; Consts c; Geo G;
+  real* ob[3];    // bars of the damping force (consumed)
+  real* ib[3];    // bars of the first operator's (halo-filled) output
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    const long a = p.a;
+    const int kmin = d.kmin, kmax = d.kmax, g = d.g, k = p.k;
+    real o[3];
+    for (int q = 0; q < 3; ++q) {
+      o[q] = ob[q][a];
+      ob[q][a] = real(0.0);
+    }
+    if (k < kmin || k > kmax) return;
+    real xx[3];
+    point_pos(G, d, p, xx);
+    proj_t(xx, c.RADIUS, o);
+    for (int q = 0; q < 3; ++q) o[q] = c.ALPHA_D * o[q];
+    if (p.pole) {
+      real sb[5];
+      pole_ring_t(G, p.lp, o, sb);
+      for (int v = 1; v <= 5; ++v) {
+        const int vp = (v == 5) ? 1 : v + 1;
+        const real s = sb[v - 1];
+        for (int dd = 0; dd < 3; ++dd) {
+          const real* cI = G.cintp_pl + ((v * 2 + p.lp) * 3 + dd) * 3;
+          aadd(&ib[dd][a], -cI[0] * s);
+          aadd(&ib[dd][a + v], -cI[1] * s);
+          aadd(&ib[dd][a + vp], -cI[2] * s);
+        }
+      }
+      return;
+    }
+    const long nreg = d.nreg, ci = p.ci;
+    const long co[6] = {0, 0, -1, -1 - (long)g, -1 - (long)g, -(long)g};
+    const int ty[6] = {0, 1, 0, 1, 0, 1};
+    real Tb[6];
+    hex_sum_t(G, nreg, ci, o, Tb);
+    for (int n = 0; n < 6; ++n) tri_div_t(G, nreg, ci + co[n], ty[n], ib, a + co[n], g, -real(1.0), Tb[n]);
+  }
+};
+
+// ----------------------------------------------------------------------------
+// hyperdiffusion passes
+// ----------------------------------------------------------------------------
+// transpose of out = -lap(in) at a compute point: ib += -lap^T ob
+NH_HD inline void neg_lap_t(const Geo& G, const Dims& d, const Pt& p, real* ib, real ob) {
+  const long a = p.a;
+  if (p.pole) {
+    for (int v = 0; v < 6; ++v) aadd(&ib[a + v], -G.clap_pl[v * 2 + p.lp] * ob);
+    return;
+  }
+  for (int m = 0; m < 7; ++m) aadd(&ib[a + nb_off(m, d.g)], -rc(G.clap, d.nreg, m, p.ci) * ob);
+}
+
+// transpose of tri_grad: ob3[dd] -> sb at the triangle's corners
+NH_HD inline void tri_grad_t(const Geo& G, long nreg, long cj, int T, real* sb, long a, int g, const real ob3[3]) {
+  real b0 = real(0.0), b1 = real(0.0), b2 = real(0.0);
+  for (int dd = 0; dd < 3; ++dd) {
+    const real c1 = rc(G.cintp, nreg, (dd * 3 + 0) * 2 + T, cj);
+    const real c2 = rc(G.cintp, nreg, (dd * 3 + 1) * 2 + T, cj);
+    const real c3 = rc(G.cintp, nreg, (dd * 3 + 2) * 2 + T, cj);
+    const real o = ob3[dd] / real(3.0);
+    b0 += (real(2.0) * c1 - c2 - c3) * o;
+    b1 += (-c1 + real(2.0) * c2 - c3) * o;
+    b2 += (-c1 - c2 + real(2.0) * c3) * o;
+  }
+  aadd(&sb[a], b0);
+  aadd(&sb[(T == 0) ? a + 1 : a + 1 + g], b1);
+  aadd(&sb[(T == 0) ? a + 1 + g : a + g], b2);
+}
+
+// first pass: out[q] = -lap(in[q]) at the compute points
+struct HDiffPass1A {
+  static constexpr const char* kName = "hdiff_pass1_adj";
+  Dims d; Geo G;
+  real* ob[6];
+  real* ib[6];
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    for (int q = 0; q < 6; ++q) {
+      const real o = ob[q][p.a];
+      ob[q][p.a] = real(0.0);
+      neg_lap_t(G, d, p, ib[q], o);
+    }
+  }
+};
+
+// second pass: -lap of the four velocity fields, -Kh-weighted diffusion of the
+// temperature and density fields with Kh = rhog * fac * GAMMA_H
+struct HDiffPass2A {
+  static constexpr const char* kName = "hdiff_pass2_adj";
+  Dims d; Consts c; Geo G;
+  const real* in[6];   // forward inputs (the first pass, halo-filled)
+  const real* rhog;
+  real* ob[6];
+  real* ib[6];
+  real* rhog_b;
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    const long a = p.a;
+    for (int q = 0; q < 4; ++q) {
+      const real o = ob[q][a];
+      ob[q][a] = real(0.0);
+      neg_lap_t(G, d, p, ib[q], o);
+    }
+    for (int f = 0; f < 2; ++f) {
+      const real o = -ob[4 + f][a];
+      ob[4 + f][a] = real(0.0);
+      const real kfac = ((f == 0) ? c.CVdry : c.HDIFF_FACT_RHO) * c.GAMMA_H;
+      if (p.pole) diff_pl_t(p, in[4 + f], ib[4 + f], kfac, o);
+      else diff7_t(p, in[4 + f], ib[4 + f], kfac, o);
+    }
+  }
+  NH_HD void diff7_t(const Pt& p, const real* s, real* sb, real kfac, real o) const {
+    const int g = d.g;
+    const long nreg = d.nreg, ci = p.ci, a = p.a;
+    // the six triangles: TI(i,j) TJ(i,j) TI(i-1,j) TJ(i-1,j-1) TI(i-1,j-1) TJ(i,j-1)
+    const long tc[6] = {0, 0, -1, -1 - (long)g, -1 - (long)g, -(long)g};
+    const int tT[6] = {0, 1, 0, 1, 0, 1};
+    real tv[6][3];
+    for (int n = 0; n < 6; ++n) tri_grad(G, nreg, ci + tc[n], tT[n], s, a + tc[n], g, tv[n]);
+    const bool pent = (p.i == 1 && p.j == 1);
+    const real m0 = G.sgp[p.l] ? real(0.0) : real(1.0), m1 = G.sgp[p.l] ? real(1.0) : real(0.0);
+    if (pent)
+      for (int dd = 0; dd < 3; ++dd) tv[4][dd] = tv[4][dd] * m0 + tv[5][dd] * m1;
+    // kf of the six arcs: neighbours (+1,+1) (0,+1) (-1,0) (-1,-1) (0,-1) (+1,0)
+    const long kn[6] = {1 + (long)g, (long)g, -1, -1 - (long)g, -(long)g, 1};
+    const real k0 = rhog[a] * kfac;
+    real kf[6];
+    for (int e = 0; e < 6; ++e) kf[e] = real(0.5) * (k0 + rhog[a + kn[e]] * kfac);
+    real tb[6][3];
+    for (int n = 0; n < 6; ++n)
+      for (int dd = 0; dd < 3; ++dd) tb[n][dd] = real(0.0);
+    real kfb[6] = {real(0.0), real(0.0), real(0.0), real(0.0), real(0.0), real(0.0)};
+    for (int dd = 0; dd < 3; ++dd) {
+      const real* cd = G.cdiff + (long)(dd * 6) * nreg + ci;
+      for (int e = 0; e < 6; ++e) {
+        const int na = e, nb = (e + 1) % 6;   // arc e joins triangles e and e+1
+        const real cc = cd[e * nreg] * o;
+        kfb[e] += cc * (tv[na][dd] + tv[nb][dd]);
+        tb[na][dd] += cc * kf[e];
+        tb[nb][dd] += cc * kf[e];
+      }
+    }
+    real k0b = real(0.0);
+    for (int e = 0; e < 6; ++e) {
+      k0b += real(0.5) * kfb[e];
+      aadd(&rhog_b[a + kn[e]], real(0.5) * kfb[e] * kfac);
+    }
+    aadd(&rhog_b[a], k0b * kfac);
+    if (pent)
+      for (int dd = 0; dd < 3; ++dd) {
+        tb[5][dd] += tb[4][dd] * m1;
+        tb[4][dd] = tb[4][dd] * m0;
+      }
+    for (int n = 0; n < 6; ++n) tri_grad_t(G, nreg, ci + tc[n], tT[n], sb, a + tc[n], g, tb[n]);
+  }
+  NH_HD void diff_pl_t(const Pt& p, const real* s, real* sb, real kfac, real o) const {
+    const long a0 = p.a;
+    const int lp = p.lp;
+    real vt[5][3];
+    for (int v = 1; v <= 5; ++v) {
+      const int vp = (v == 5) ? 1 : v + 1;
+      for (int dd = 0; dd < 3; ++dd) {
+        const real* cI = G.cintp_pl + ((v * 2 + lp) * 3 + dd) * 3;
+        const real c0 = cI[0], c1 = cI[1], c2 = cI[2];
+        vt[v - 1][dd] = ((real(2.0) * c0 - c1 - c2) * s[a0] + (-c0 + real(2.0) * c1 - c2) * s[a0 + v]
+                         + (-c0 - c1 + real(2.0) * c2) * s[a0 + vp]) / real(3.0);
+      }
+    }
+    real vb[5][3];
+    for (int v = 0; v < 5; ++v)
+      for (int dd = 0; dd < 3; ++dd) vb[v][dd] = real(0.0);
+    for (int v = 1; v <= 5; ++v) {
+      const int vm = (v == 1) ? 5 : v - 1;
+      const real kavg = real(0.5) * (rhog[a0] * kfac + rhog[a0 + v] * kfac);
+      const real* cd = G.cdiff_pl + (v * 2 + lp) * 3;
+      const real sum = cd[0] * (vt[vm - 1][0] + vt[v - 1][0]) + cd[1] * (vt[vm - 1][1] + vt[v - 1][1])
+                       + cd[2] * (vt[vm - 1][2] + vt[v - 1][2]);
+      const real kb = o * sum;
+      aadd(&rhog_b[a0], real(0.5) * kb * kfac);
+      aadd(&rhog_b[a0 + v], real(0.5) * kb * kfac);
+      const real smb = o * kavg;
+      for (int dd = 0; dd < 3; ++dd) {
+        vb[vm - 1][dd] += smb * cd[dd];
+        vb[v - 1][dd] += smb * cd[dd];
+      }
+    }
+    for (int v = 1; v <= 5; ++v) {
+      const int vp = (v == 5) ? 1 : v + 1;
+      real b0 = real(0.0), b1 = real(0.0), b2 = real(0.0);
+      for (int dd = 0; dd < 3; ++dd) {
+        const real* cI = G.cintp_pl + ((v * 2 + lp) * 3 + dd) * 3;
+        const real c0 = cI[0], c1 = cI[1], c2 = cI[2];
+        const real w = vb[v - 1][dd] / real(3.0);
+        b0 += (real(2.0) * c0 - c1 - c2) * w;
+        b1 += (-c0 + real(2.0) * c1 - c2) * w;
+        b2 += (-c0 - c1 + real(2.0) * c2) * w;
+      }
+      aadd(&sb[a0], b0);
+      aadd(&sb[a0 + v], b1);
+      aadd(&sb[a0 + vp], b2);
+    }
+  }
+};
+
+// ----------------------------------------------------------------------------
+// kinetic energy at level k of a column (rhogkin_k): transpose for the
+// accessor's values; add(kk, q, v) receives the bar of get(kk, q)
+// ----------------------------------------------------------------------------
+template <class F, class ADD>
+NH_HD inline void rhogkin_k_t(const VProf<real>& V, int kmin, int kmax, int k, F get, ADD add, real kb) {
+  if (k < kmin || k > kmax) return;
+  const real rgk = get(k, 0);
+  const real vx = get(k, 1), vy = get(k, 2), vz = get(k, 3);
+  const real kh = real(0.5) * (vx * vx + vy * vy + vz * vz) / rgk;
+  real rgk_b = -kb * kh / rgk;
+  add(k, 1, kb * vx / rgk);
+  add(k, 2, kb * vy / rgk);
+  add(k, 3, kb * vz / rgk);
+  if (k + 1 <= kmax) {
+    const real w = get(k + 1, 4);
+    const real D = V.c2w0[k + 1] * get(k + 1, 0) + V.c2w1[k + 1] * rgk;
+    const real vup = real(0.5) * (w * w) / D;
+    const real ub = kb * V.w2c0[k];
+    add(k + 1, 4, ub * w / D);
+    const real Db = -ub * vup / D;
+    add(k + 1, 0, Db * V.c2w0[k + 1]);
+    rgk_b += Db * V.c2w1[k + 1];
+  }
+  if (k >= kmin + 1) {
+    const real w = get(k, 4);
+    const real D = V.c2w0[k] * rgk + V.c2w1[k] * get(k - 1, 0);
+    const real vdn = real(0.5) * (w * w) / D;
+    const real db = kb * V.w2c1[k];
+    add(k, 4, db * w / D);
+    const real Db = -db * vdn / D;
+    rgk_b += Db * V.c2w0[k];
+    add(k - 1, 0, Db * V.c2w1[k]);
+  }
+  add(k, 0, rgk_b);
+}
+
+// ----------------------------------------------------------------------------
+// momentum advection (AdvMomP): gadv bars -> bars of vx, vy, vz, w (diag),
+// rhog, rhogvx..vz, rhogw (prog)
+// ----------------------------------------------------------------------------
+struct AdvMomA {
+  static constexpr const char* kName = "adv_mom_adj";
+  Dims d; Consts c; Geo G;
+  const real* prog;
+  const real* diag;
+  real* ob;        // 4 fields: bars of out[0..3] (consumed)
+  real* prog_b;    // 6 fields
+  real* diag_b;    // 6 fields
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p)) return;
+    const long NF = d.NF, a = p.a, ks = p.ks;
+    const int kmin = d.kmin, kmax = d.kmax, k = p.k;
+    real o[4];
+    for (int q = 0; q < 4; ++q) {
+      o[q] = ob[q * NF + a];
+      ob[q * NF + a] = real(0.0);
+    }
+    if (!pt_compute(p) || k < kmin || k > kmax) return;
+    const real R = c.RADIUS;
+    const real* rw = prog + 4 * NF;
+    const real* rg = prog;
+    const real* wd = diag + I_W * NF;
+    real* wdb = diag_b + I_W * NF;
+    const int NN = p.pole ? 6 : 7;
+    const long g = d.g;
+    auto nbr = [&](int m) { return p.pole ? a + m : a + nb_off(m, g); };
+    auto pos = [&](int m, real* xx) {
+      if (p.pole) {
+        for (int q = 0; q < 3; ++q) xx[q] = G.gx_pl[(m * 2 + p.lp) * 3 + q];
+      } else {
+        for (int q = 0; q < 3; ++q) xx[q] = G.gx[q * d.nreg + p.ci + nb_off(m, g)];
+      }
+    };
+    // merged velocity at address b, level kk, position xx
+    auto vv3 = [&](long b, int kk, const real* xx, real* vo) {
+      const real wc = G.cfact[kk] * wd[b + ks] + G.dfact[kk] * wd[b];
+      for (int q = 0; q < 3; ++q) vo[q] = diag[(I_VX + q) * NF + b] + wc * xx[q] / R;
+    };
+    auto vv3_t = [&](long b, int kk, const real* xx, const real* vb) {
+      real wcb = real(0.0);
+      for (int q = 0; q < 3; ++q) {
+        aadd(&diag_b[(I_VX + q) * NF + b], vb[q]);
+        wcb += vb[q] * xx[q] / R;
+      }
+      aadd(&wdb[b + ks], wcb * G.cfact[kk]);
+      aadd(&wdb[b], wcb * G.dfact[kk]);
+    };
+    real x0[3];
+    pos(0, x0);
+    real vvm0[3];
+    vv3(a, k, x0, vvm0);
+    // forward tail
+    const real gxc = x0[0] / R, gyc = x0[1] / R, gzc = x0[2] / R;
+    const real rh = rg[a];
+    const real prd_b = o[3] * c.ALPHA - (o[0] * gxc + o[1] * gyc + o[2] * gzc);
+    const real dxb = o[0] + prd_b * gxc, dyb = o[1] + prd_b * gyc, dzb = o[2] + prd_b * gzc;
+    real vvm0_b[3] = {real(0.0), real(0.0), real(0.0)};
+    aadd(&prog_b[a], real(2.0) * c.OHM * (dxb * vvm0[1] - dyb * vvm0[0]));
+    vvm0_b[1] += real(2.0) * rh * c.OHM * dxb;
+    vvm0_b[0] -= real(2.0) * rh * c.OHM * dyb;
+    const real dvvb[3] = {dxb, dyb, dzb};
+    real divb[3], hib[3], lob[3];
+    for (int q = 0; q < 3; ++q) {
+      divb[q] = -dvvb[q];
+      hib[q] = -dvvb[q] * G.rdgz[k];
+      lob[q] = dvvb[q] * G.rdgz[k];
+    }
+    if (k + 1 <= kmax) {
+      real up[3], upb[3];
+      vv3(a + ks, k + 1, x0, up);
+      const real wup = rw[a + ks];
+      real wupb = real(0.0);
+      for (int q = 0; q < 3; ++q) {
+        wupb += hib[q] * (G.afact[k + 1] * up[q] + G.bfact[k + 1] * vvm0[q]);
+        upb[q] = hib[q] * wup * G.afact[k + 1];
+        vvm0_b[q] += hib[q] * wup * G.bfact[k + 1];
+      }
+      aadd(&prog_b[4 * NF + a + ks], wupb);
+      vv3_t(a + ks, k + 1, x0, upb);
+    }
+    if (k >= kmin + 1) {
+      real dn[3], dnb[3];
+      vv3(a - ks, k - 1, x0, dn);
+      const real w0 = rw[a];
+      real w0b = real(0.0);
+      for (int q = 0; q < 3; ++q) {
+        w0b += lob[q] * (G.afact[k] * vvm0[q] + G.bfact[k] * dn[q]);
+        vvm0_b[q] += lob[q] * w0 * G.afact[k];
+        dnb[q] = lob[q] * w0 * G.bfact[k];
+      }
+      aadd(&prog_b[4 * NF + a], w0b);
+      vv3_t(a - ks, k - 1, x0, dnb);
+    }
+    // divergence of rhogv * vv_q over the neighbours
+    for (int m = 0; m < NN; ++m) {
+      real xm[3], vv[3];
+      pos(m, xm);
+      const long bm = nbr(m);
+      vv3(bm, k, xm, vv);
+      real cdir[3];
+      if (p.pole) {
+        for (int dd = 0; dd < 3; ++dd) cdir[dd] = G.cdiv_pl[(m * 2 + p.lp) * 3 + dd];
+      } else {
+        for (int dd = 0; dd < 3; ++dd) cdir[dd] = rc(G.cdiv, d.nreg, dd * 7 + m, p.ci);
+      }
+      real fb[3] = {real(0.0), real(0.0), real(0.0)};
+      real vvb[3];
+      for (int q = 0; q < 3; ++q) {
+        vvb[q] = (m == 0) ? vvm0_b[q] : real(0.0);
+        for (int dd = 0; dd < 3; ++dd) {
+          const real t = cdir[dd] * divb[q];
+          fb[dd] += t * vv[q];
+          vvb[q] += t * prog[(1 + dd) * NF + bm];
+        }
+      }
+      for (int dd = 0; dd < 3; ++dd) aadd(&prog_b[(1 + dd) * NF + bm], fb[dd]);
+      vv3_t(bm, k, xm, vvb);
+    }
+  }
+};
+
+// ----------------------------------------------------------------------------
+// Pre_Post (PrePostP): in place on prog_b (bar of the boundary-conditioned
+// PROG on entry, bar of the PROG before the kernel on exit), with the bars of
+// DIAG and eth (consumed)
+// ----------------------------------------------------------------------------
+struct PrePostA {
+  static constexpr const char* kName = "prepost_adj";
+  Dims d; Consts c; Geo G;
+  const real* prog;    // boundary-conditioned PROG (the forward kernel's result)
+  const real* rhogq;
+  real* prog_b;
+  real* diag_b;
+  real* eth_b;
+  struct Th { real rho, ein, q, qd, cv, tem, Rm, pre; };
+  NH_HD Th thermo(long a) const {
+    const long NF = d.NF;
+    Th t;
+    const real rhog = prog[a];
+    t.rho = rhog / real(1.0);
+    t.ein = prog[5 * NF + a] / rhog;
+    t.q = rhogq[a] / rhog;
+    t.qd = real(1.0) - t.q;
+    t.cv = t.q * c.CVW_qv + t.qd * c.CVdry;
+    t.tem = t.ein / t.cv;
+    t.Rm = t.qd * c.Rdry + t.q * c.Rvap;
+    t.pre = t.rho * t.tem * t.Rm;
+    return t;
+  }
+  // bars of (rho, tem, pre) at an interior point -> prog_b (rhog, rhoge)
+  NH_HD void thermo_t(long a, const Th& t, real rhob, real temb, real preb) const {
+    const long NF = d.NF;
+    const real rhog = prog[a];
+    // pre = rho * tem * Rm
+    rhob += preb * t.tem * t.Rm;
+    temb += preb * t.rho * t.Rm;
+    const real Rmb = preb * t.rho * t.tem;
+    // tem = ein / cv
+    const real einb = temb / t.cv;
+    const real cvb = -temb * t.tem / t.cv;
+    // cv, Rm from q
+    const real qb = cvb * (c.CVW_qv - c.CVdry) + Rmb * (c.Rvap - c.Rdry);
+    // ein = rhoge / rhog, q = rhogq / rhog, rho = rhog
+    aadd(&prog_b[5 * NF + a], einb / rhog);
+    aadd(&prog_b[a], rhob - einb * t.ein / rhog - qb * t.q / rhog);
+  }
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p)) return;
+    const long NF = d.NF, a = p.a, ks = p.ks;
+    const int kmin = d.kmin, kmax = d.kmax, k = p.k;
+    real db[6];
+    for (int q = 0; q < 6; ++q) {
+      db[q] = diag_b[q * NF + a];
+      diag_b[q * NF + a] = real(0.0);
+    }
+    const real eb = eth_b[a];
+    eth_b[a] = real(0.0);
+    if (k >= kmin && k <= kmax) {
+      const Th t = thermo(a);
+      const real rhog = prog[a];
+      real rhob = real(0.0), temb = db[I_TEM], preb = db[I_PRE];
+      // eth = ein + pre / rho
+      real einb_x = eb;
+      preb += eb / t.rho;
+      rhob += -eb * t.pre / (t.rho * t.rho);
+      aadd(&prog_b[5 * NF + a], einb_x / rhog);
+      real rgb = -einb_x * t.ein / rhog;
+      // velocities
+      for (int q = 0; q < 3; ++q) {
+        const real v = prog[(1 + q) * NF + a] / rhog;
+        aadd(&prog_b[(1 + q) * NF + a], db[I_VX + q] / rhog);
+        rgb += -db[I_VX + q] * v / rhog;
+      }
+      aadd(&prog_b[a], rgb);
+      if (k > kmin) {
+        const real D = G.c2w0[k] * prog[a] + G.c2w1[k] * prog[a - ks];
+        const real rw = prog[4 * NF + a];
+        const real wb = db[I_W];
+        aadd(&prog_b[4 * NF + a], wb / D);
+        const real Db = -wb * rw / (D * D);
+        aadd(&prog_b[a], Db * G.c2w0[k]);
+        aadd(&prog_b[a - ks], Db * G.c2w1[k]);
+      } else {
+        // rhogw at kmin is set to zero: its incoming bar is dropped, w = 0
+        prog_b[4 * NF + a] = real(0.0);
+      }
+      thermo_t(a, t, rhob, temb, preb);
+      return;
+    }
+    // ghost rows: every PROG value written here is a function of the nearest
+    // two physical levels; the incoming bars of the ghost values are consumed
+    const bool top = (k == kmax + 1);
+    const long an = top ? a - ks : a + ks;
+    const long an2 = top ? a - 2 * ks : a + 2 * ks;
+    const Th tn = thermo(an);
+    const Th tn2 = thermo(an2);
+    const real dphi = top ? (G.phi[kmax + 1] - G.phi[kmax - 1]) : (G.phi[kmin - 1] - G.phi[kmin + 1]);
+    const real tem_g = tn.tem;
+    const real pre_g = tn2.pre - tn.rho * dphi;
+    const real rho_g = pre_g / (c.Rdry * tem_g);
+    const real rg_g = rho_g * real(1.0);
+    const real ein_g = c.CVdry * tem_g;
+    real pb[6];
+    for (int q = 0; q < 6; ++q) {
+      pb[q] = prog_b[q * NF + a];
+      prog_b[q * NF + a] = real(0.0);
+    }
+    const real sgn = top ? real(1.0) : -real(1.0);
+    real rggb = pb[0] + pb[5] * ein_g;
+    real eingb = pb[5] * rg_g;
+    // eth = ein_g + pre_g / rho_g
+    eingb += eb;
+    real pregb = db[I_PRE] + eb / rho_g;
+    real rhogb = -eb * pre_g / (rho_g * rho_g);
+    real temgb = db[I_TEM];
+    // velocities: v = sgn * (rhogv(an) / rhog(an)) * rg_g; diag = v / rg_g
+    real rgnb = real(0.0);
+    const real rgn = prog[an];
+    for (int q = 0; q < 3; ++q) {
+      const real u = prog[(1 + q) * NF + an] / rgn;
+      const real v = sgn * u * rg_g;
+      real vb = pb[1 + q] + db[I_VX + q] / rg_g;
+      rggb += -db[I_VX + q] * v / (rg_g * rg_g);
+      const real ub = vb * sgn * rg_g;
+      rggb += vb * sgn * u;
+      aadd(&prog_b[(1 + q) * NF + an], ub / rgn);
+      rgnb += -ub * u / rgn;
+    }
+    aadd(&prog_b[an], rgnb);
+    // rg_g = rho_g = pre_g / (Rdry tem_g)
+    rhogb += rggb;
+    pregb += rhogb / (c.Rdry * tem_g);
+    temgb += -rhogb * rho_g / tem_g;
+    temgb += eingb * c.CVdry;
+    // pre_g = tn2.pre - tn.rho * dphi; tem_g = tn.tem
+    thermo_t(an, tn, -pregb * dphi, temgb, real(0.0));
+    thermo_t(an2, tn2, real(0.0), real(0.0), pregb);
+  }
+};
+
+// ----------------------------------------------------------------------------
+// total specific energy (EthTotPMP): ethtot bar -> eth, PROG bars
+// ----------------------------------------------------------------------------
+struct EthTotA {
+  static constexpr const char* kName = "ethtot_adj";
+  Dims d; Consts c; Geo G;
+  const real* prog;
+  real* ethtot_b;
+  real* eth_b;
+  real* prog_b;
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p)) return;
+    const long NF = d.NF, a = p.a, ks = p.ks;
+    const long b = a - (long)p.k * ks;
+    const real eb = ethtot_b[a];
+    ethtot_b[a] = real(0.0);
+    aadd(&eth_b[a], eb);
+    auto get = [&](int kk, int q) { return prog[q * NF + b + (long)kk * ks]; };
+    auto add = [&](int kk, int q, real v) { aadd(&prog_b[q * NF + b + (long)kk * ks], v); };
+    const real kin0 = rhogkin_k(G, d.kmin, d.kmax, p.k, get);
+    const real rg = prog[a];
+    aadd(&prog_b[a], -eb * kin0 / (rg * rg));
+    rhogkin_k_t(G, d.kmin, d.kmax, p.k, get, add, eb / rg);
+  }
+};
+
+// ----------------------------------------------------------------------------
+// small step path1 (Path1P): dv bars -> split momentum and rhoge bars, gT,
+// damping force, and (ghost rows) rhog bars
+// ----------------------------------------------------------------------------
+struct Path1A {
+  static constexpr const char* kName = "vi_path1_adj";
+  Dims d; Consts c; Geo G;
+  real dt;
+  const real* prog;
+  const real* ps[6];
+  const real* gT;
+  const real* dd[3];
+  real* dv_b[3];     // consumed
+  real* prog_b;
+  real* ps_b[6];
+  real* gT_b;
+  real* dd_b[3];
+  NH_HD int stage_shift(int k) const {
+    const int kmin = d.kmin, kmax = d.kmax;
+    if (k == kmin || k == kmin - 1) return kmin + 1 - k;
+    if (k == kmax + 1) return -1;
+    return 0;
+  }
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    const long NF = d.NF, a = p.a, ks = p.ks;
+    const int kmin = d.kmin, kmax = d.kmax, k = p.k;
+    const real R = c.RADIUS;
+    const real RovCV = c.Rdry / c.CVdry;
+    const long sh = (long)stage_shift(k) * ks;
+    const long g = d.g;
+    real vb[3];
+    for (int q = 0; q < 3; ++q) {
+      vb[q] = dv_b[q][a];
+      dv_b[q][a] = real(0.0);
+    }
+    real xx[3];
+    point_pos(G, d, p, xx);
+    auto grad = [&](int dd2) {
+      real acc;
+      if (p.pole) {
+        acc = real(0.0);
+        for (int v = 0; v < 6; ++v) {
+          const real pv = G.cgrad_pl[(v * 2 + p.lp) * 3 + dd2] * ((ps[I_RHOGE][a + v + sh] * RovCV) * real(1.0) * real(1.0));
+          acc = (v == 0) ? pv : acc + pv;
+        }
+      } else {
+        acc = rc(G.cgrad, d.nreg, dd2 * 7, p.ci) * ((ps[I_RHOGE][a + sh] * RovCV) * real(1.0));
+        for (int m = 1; m < 7; ++m)
+          acc = acc + rc(G.cgrad, d.nreg, dd2 * 7 + m, p.ci) * ((ps[I_RHOGE][a + nb_off(m, g) + sh] * RovCV) * real(1.0));
+      }
+      return acc;
+    };
+    long at = a;
+    if (k < kmin || k > kmax) {
+      const bool top = (k == kmax + 1);
+      const long an = top ? a - ks : a + ks;
+      // forward momentum at the nearest physical level
+      real gr[3];
+      for (int dd2 = 0; dd2 < 3; ++dd2) gr[dd2] = grad(dd2) + real(0.0);
+      const real prd = (gr[0] * xx[0] + gr[1] * xx[1] + gr[2] * xx[2]) / R;
+      real vn[3];
+      for (int q = 0; q < 3; ++q) {
+        const real Pg = gr[q] - prd * xx[q] / R;
+        vn[q] = ps[1 + q][an] + (gT[(1 + q) * NF + an] - Pg + dd[q][an] + real(0.0)) * dt;
+      }
+      const real sgn = top ? real(1.0) : -real(1.0);
+      const real rgn = prog[an], rga = prog[a];
+      real rab = real(0.0), rnb = real(0.0);
+      for (int q = 0; q < 3; ++q) {
+        rab += sgn * vb[q] * vn[q] / rgn;
+        rnb -= sgn * vb[q] * vn[q] * rga / (rgn * rgn);
+        vb[q] = sgn * vb[q] * rga / rgn;
+      }
+      aadd(&prog_b[a], rab);
+      aadd(&prog_b[an], rnb);
+      at = an;
+    }
+    // momentum at level address `at`: out = ps + (gT - P(grad) + dd) dt
+    real gb[3];
+    for (int q = 0; q < 3; ++q) {
+      aadd(&ps_b[1 + q][at], vb[q]);
+      const real db = vb[q] * dt;
+      aadd(&gT_b[(1 + q) * NF + at], db);
+      aadd(&dd_b[q][at], db);
+      gb[q] = -db;
+    }
+    proj_t(xx, R, gb);
+    real* eb = ps_b[I_RHOGE];
+    for (int dd2 = 0; dd2 < 3; ++dd2) {
+      const real w = gb[dd2] * RovCV;
+      if (p.pole) {
+        for (int v = 0; v < 6; ++v) aadd(&eb[a + v + sh], G.cgrad_pl[(v * 2 + p.lp) * 3 + dd2] * w);
+      } else {
+        for (int m = 0; m < 7; ++m) aadd(&eb[a + nb_off(m, g) + sh], rc(G.cgrad, d.nreg, dd2 * 7 + m, p.ci) * w);
+      }
+    }
+  }
+};
+
+// ----------------------------------------------------------------------------
+// small step vi_main, horizontal part (ViMainH)
+// ----------------------------------------------------------------------------
+struct ViMainHA {
+  static constexpr const char* kName = "vi_main_h_adj";
+  Dims d; Consts c; Geo G;
+  const real* prog;
+  const real* dv[3];
+  const real* eth;
+  const real* ethtot;
+  real* hv_b[3];     // consumed
+  real* dv_b[3];
+  real* eth_b;
+  real* ethtot_b;
+  real* prog_b;
+  real* gT_b;
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    const long NF = d.NF, a = p.a;
+    const int k = p.k;
+    real hb[3];
+    for (int q = 0; q < 3; ++q) {
+      hb[q] = hv_b[q][a];
+      hv_b[q][a] = real(0.0);
+    }
+    if (k < d.kmin || k > d.kmax) return;
+    const real rc_ = c.Rdry / c.CVdry;
+    aadd(&gT_b[5 * NF + a], hb[1] * rc_);
+    const real heb = -hb[1] * rc_;
+    const int NN = p.pole ? 6 : 7;
+    for (int m = 0; m < NN; ++m) {
+      const long b = p.pole ? a + m : a + nb_off(m, d.g);
+      real cdir[3];
+      if (p.pole) {
+        for (int q = 0; q < 3; ++q) cdir[q] = G.cdiv_pl[(m * 2 + p.lp) * 3 + q];
+      } else {
+        for (int q = 0; q < 3; ++q) cdir[q] = rc(G.cdiv, d.nreg, q * 7 + m, p.ci);
+      }
+      const real e = eth[b], et = ethtot[b];
+      real eb = real(0.0), etb = real(0.0);
+      for (int q = 0; q < 3; ++q) {
+        const real w0 = cdir[q] * hb[0], w1 = cdir[q] * heb, w2 = cdir[q] * hb[2];
+        const real dq = dv[q][b];
+        aadd(&dv_b[q][b], w0 + w1 * e + w2 * et);
+        eb += w1 * dq;
+        aadd(&prog_b[(1 + q) * NF + b], w2 * et);
+        etb += w2 * (prog[(1 + q) * NF + b] + dq);
+      }
+      aadd(&eth_b[b], eb);
+      aadd(&ethtot_b[b], etb);
+    }
+  }
+};
+
+// ----------------------------------------------------------------------------
+// small step vi_main, vertical part (ViMainV): the transposed tridiagonal
+// solve, one thread per compute column, in FP64.  The forward solution x is
+// recomputed (the matrix bars need it); U, Gm, X are FP64 column scratch.
+// ----------------------------------------------------------------------------
+struct ViMainVA {
+  static constexpr const char* kName = "vi_main_v_adj";
+  Dims d; Consts c; Geo G;
+  double dt;
+  const real* ps[6];
+  const real* gT;
+  const double* Mc; const double* Mu; const double* Ml;
+  const real* hv[3];
+  double* U; double* Gm; double* X;
+  real* out_b;        // bar of the new rhogw split (consumed)
+  real* ps_b[6];
+  real* gT_b;
+  double* Mc_b; double* Mu_b; double* Ml_b;
+  real* hv_b[3];
+  NH_HD void operator()(long t) const {
+    Cell cl = cell_of(d, t);
+    if (cl.pole ? cl.v != 0 : !cl.inner) return;
+    const long NF = d.NF, ks = cl.ks, b = cl.base;
+    const int kmin = d.kmin, kmax = d.kmax;
+    const VProf<double>& V = G.v64;
+    const ConstsT<double>& C = c.d;
+    const double RovCV = C.Rdry / C.CVdry;
+    const double CVovRt2 = C.CVdry / C.Rdry / (dt * dt);
+    const int N = kmax - kmin;
+    auto lev = [&](int k) { return b + (long)k * ks; };
+    auto src = [&](int k) {
+      const long a = lev(k), am = lev(k - 1);
+      const double gr_k = (double)gT[a] + (-(double)hv[0][a] - (0.0 - 0.0) * V.rdgz[k]);
+      const double gr_m = (double)gT[am] + (-(double)hv[0][am] - (0.0 - 0.0) * V.rdgz[k - 1]);
+      const double sw = (double)gT[4 * NF + a] + 0.0 * C.ALPHA;
+      const double A = ((double)ps[I_RHOGW][a] * C.ALPHA + dt * sw) * 1.0;
+      const double B = ((double)ps[I_RHOGE][a] * RovCV + dt * (double)hv[1][a]) * 1.0 -
+                       ((double)ps[I_RHOGE][am] * RovCV + dt * (double)hv[1][am]) * 1.0;
+      const double Cc = ((double)ps[I_RHOG][a] + dt * gr_k) * 1.0 * V.afact[k] +
+                        ((double)ps[I_RHOG][am] + dt * gr_m) * 1.0 * V.bfact[k];
+      return (A - B * dt * V.rdgzh[k] - Cc * dt * C.GRAV) * CVovRt2;
+    };
+    // forward solve M x = S
+    double beta = 0.0, u_prev = 0.0;
+    for (int n = 0; n < N; ++n) {
+      const long a = lev(kmin + 1 + n);
+      const double S = src(kmin + 1 + n);
+      double u;
+      if (n == 0) {
+        beta = Mc[a];
+        u = S / beta;
+      } else {
+        const double gm = Mu[a - ks] / beta;
+        beta = Mc[a] - Ml[a] * gm;
+        u = (S - Ml[a] * u_prev) / beta;
+        Gm[a] = gm;
+      }
+      U[a] = u;
+      u_prev = u;
+    }
+    {
+      long a = lev(kmax);
+      double xv = u_prev;
+      X[a] = xv;
+      for (int n = N - 2; n >= 0; --n) {
+        const double gm_next = Gm[a];
+        a -= ks;
+        xv = U[a] - gm_next * xv;
+        X[a] = xv;
+      }
+    }
+    // transposed solve M^T y = xbar: row n of M^T is Mu[n-1] y[n-1] + Mc[n] y[n] + Ml[n+1] y[n+1]
+    out_b[lev(kmin)] = real(0.0);
+    out_b[lev(kmax + 1)] = real(0.0);
+    out_b[lev(kmin - 1)] = real(0.0);
+    u_prev = 0.0;
+    for (int n = 0; n < N; ++n) {
+      const long a = lev(kmin + 1 + n);
+      const double xb = (double)out_b[a];
+      out_b[a] = real(0.0);
+      double u;
+      if (n == 0) {
+        beta = Mc[a];
+        u = xb / beta;
+      } else {
+        const double gm = Ml[a] / beta;
+        beta = Mc[a] - Mu[a - ks] * gm;
+        u = (xb - Mu[a - ks] * u_prev) / beta;
+        Gm[a] = gm;
+      }
+      U[a] = u;
+      u_prev = u;
+    }
+    // back substitution, then the bars of S (y) and of the matrix (-y x^T on the band)
+    long a = lev(kmax);
+    double y = u_prev;
+    for (int n = N - 1; n >= 0; --n) {
+      if (n < N - 1) {
+        const double gm_next = Gm[a];
+        a -= ks;
+        y = U[a] - gm_next * y;
+      }
+      const int k = kmin + 1 + n;
+      const double xk = X[a];
+      Mc_b[a] += -y * xk;
+      if (n >= 1) Ml_b[a] += -y * X[a - ks];
+      if (n <= N - 2) Mu_b[a] += -y * X[a + ks];
+      const long am = a - ks;
+      const double s = y * CVovRt2;
+      const double Ab = s, Bb = -s * dt * V.rdgzh[k], Cb = -s * dt * C.GRAV;
+      ps_b[I_RHOGW][a] += (real)(Ab * C.ALPHA);
+      gT_b[4 * NF + a] += (real)(Ab * dt);
+      ps_b[I_RHOGE][a] += (real)(Bb * RovCV);
+      hv_b[1][a] += (real)(Bb * dt);
+      ps_b[I_RHOGE][am] += (real)(-Bb * RovCV);
+      hv_b[1][am] += (real)(-Bb * dt);
+      const double grk = Cb * V.afact[k] * dt, grm = Cb * V.bfact[k] * dt;
+      ps_b[I_RHOG][a] += (real)(Cb * V.afact[k]);
+      ps_b[I_RHOG][am] += (real)(Cb * V.bfact[k]);
+      gT_b[a] += (real)grk;
+      hv_b[0][a] += (real)(-grk);
+      gT_b[am] += (real)grm;
+      hv_b[0][am] += (real)(-grm);
+    }
+  }
+};
+
+// ----------------------------------------------------------------------------
+// small step mass and energy update (ViMainPost): bars of the new rhog and
+// rhoge splits (we[0], we[2], consumed) -> inputs; we_b[1] receives the bar
+// of the new rhogw split it read
+// ----------------------------------------------------------------------------
+struct ViMainPostA {
+  static constexpr const char* kName = "vi_main_post_adj";
+  Dims d; Consts c; Geo G;
+  real dt;
+  const real* prog;
+  const real* ps[6];
+  const real* dv[3];
+  const real* ethtot;
+  const real* gT;
+  const real* hv[3];
+  const real* we1;
+  real* we_b[3];
+  real* prog_b;
+  real* ps_b[6];
+  real* dv_b[3];
+  real* ethtot_b;
+  real* gT_b;
+  real* getot_b;
+  real* hv_b[3];
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    const long NF = d.NF, ks = p.ks, a = p.a;
+    const int kmin = d.kmin, kmax = d.kmax, k = p.k;
+    const long b = a - (long)k * ks;
+    auto at = [&](int kk) { return b + (long)kk * ks; };
+    const real b0 = we_b[0][a], b2 = we_b[2][a];
+    we_b[0][a] = real(0.0);
+    we_b[2][a] = real(0.0);
+    const real* pg = ps[I_RHOG];
+    const real* rw1 = we1;
+    auto rs1 = [&](int kk) {
+      real drhog2 = real(0.0);
+      if (kk >= kmin && kk <= kmax) {
+        const real hi = (kk + 1 <= kmax) ? real(0.0) * real(1.0) + real(1.0) * rw1[at(kk + 1)] * real(1.0) : real(0.0);
+        const real lo = (kk >= kmin + 1) ? real(0.0) * real(1.0) + real(1.0) * rw1[at(kk)] * real(1.0) : real(0.0);
+        drhog2 = -hv[0][at(kk)] - (hi - lo) * G.rdgz[kk];
+      }
+      return pg[at(kk)] + (gT[0 * NF + at(kk)] + drhog2) * dt;
+    };
+    auto rs1_t = [&](int kk, real rb) {
+      aadd(&ps_b[I_RHOG][at(kk)], rb);
+      aadd(&gT_b[at(kk)], rb * dt);
+      if (kk >= kmin && kk <= kmax) {
+        const real d2b = rb * dt;
+        aadd(&hv_b[0][at(kk)], -d2b);
+        if (kk + 1 <= kmax) aadd(&we_b[1][at(kk + 1)], -d2b * G.rdgz[kk]);
+        if (kk >= kmin + 1) aadd(&we_b[1][at(kk)], d2b * G.rdgz[kk]);
+      }
+    };
+    const int ke = (k < kmin) ? kmin : ((k > kmax) ? kmax : k);
+    aadd(&ps_b[I_RHOGE][at(ke)], b2);
+    aadd(&getot_b[a], b2 * dt);
+    const real dtotb = b2 * dt;
+    aadd(&ps_b[I_RHOG][a], b2 * G.phi[k]);
+    rs1_t(k, b0 - b2 * G.phi[k]);
+    if (k >= kmin && k <= kmax) {
+      aadd(&hv_b[2][a], -dtotb);
+      const real hib = -dtotb * G.rdgz[k], lob = dtotb * G.rdgz[k];
+      if (k + 1 <= kmax) {
+        const real W1 = prog[4 * NF + a + ks] + rw1[a + ks];
+        const real E1 = G.afact[k + 1] * ethtot[a + ks] + G.bfact[k + 1] * ethtot[a];
+        aadd(&prog_b[4 * NF + a + ks], hib * E1);
+        aadd(&we_b[1][a + ks], hib * E1);
+        aadd(&ethtot_b[a + ks], hib * W1 * G.afact[k + 1]);
+        aadd(&ethtot_b[a], hib * W1 * G.bfact[k + 1]);
+      }
+      if (k >= kmin + 1) {
+        const real W0 = prog[4 * NF + a] + rw1[a];
+        const real E0 = G.afact[k] * ethtot[a] + G.bfact[k] * ethtot[a - ks];
+        aadd(&prog_b[4 * NF + a], lob * E0);
+        aadd(&we_b[1][a], lob * E0);
+        aadd(&ethtot_b[a], lob * W0 * G.afact[k]);
+        aadd(&ethtot_b[a - ks], lob * W0 * G.bfact[k]);
+      }
+    }
+    // kinetic energies before (PROG + PROG_split) and after (new split) the step
+    auto g10 = [&](int kk, int q) { return prog[q * NF + at(kk)] + ps[q][at(kk)]; };
+    auto add10 = [&](int kk, int q, real v) {
+      aadd(&prog_b[q * NF + at(kk)], v);
+      aadd(&ps_b[q][at(kk)], v);
+    };
+    rhogkin_k_t(G, kmin, kmax, k, g10, add10, b2);
+    auto g11 = [&](int kk, int q) {
+      if (q == 0) return prog[at(kk)] + rs1(kk);
+      if (q == 4) return prog[4 * NF + at(kk)] + rw1[at(kk)];
+      return prog[q * NF + at(kk)] + dv[q - 1][at(kk)];
+    };
+    auto add11 = [&](int kk, int q, real v) {
+      aadd(&prog_b[q * NF + at(kk)], v);
+      if (q == 0) rs1_t(kk, v);
+      else if (q == 4) aadd(&we_b[1][at(kk)], v);
+      else aadd(&dv_b[q - 1][at(kk)], v);
+    };
+    rhogkin_k_t(G, kmin, kmax, k, g11, add11, -b2);
+  }
+};
+
+// ----------------------------------------------------------------------------
+// large-step sources and tridiagonal coefficients (ViPath0P)
+// ----------------------------------------------------------------------------
+// g~(kk) = GRAV - (dpgw(kk) - dbuo(kk)) / rhog_h(kk) of a column, in the
+// precision T of the profiles V; rg(kk), pre(kk) read the column's levels
+template <class T>
+struct GztCol {
+  const VProf<T>* V;
+  T GRAV;
+  int kmin, kmax;
+  template <class RG>
+  NH_HD T rhog_h(int kk, RG rg) const {
+    const int k2 = (kk < kmin) ? kmin : kk;
+    return V->c2w0[k2] * rg(k2) + V->c2w1[k2] * rg(k2 - 1);
+  }
+  NH_HD bool in(int kk) const { return kk >= kmin + 1 && kk <= kmax; }
+  template <class RG, class PR>
+  NH_HD T value(int kk, RG rg, PR pre) const {
+    const T dpgw = in(kk) ? T(1.0) * (pre(kk) * T(1.0) - pre(kk - 1) * T(1.0)) * V->rdgzh[kk] : T(0.0);
+    const T dbuo = in(kk) ? -GRAV * (V->c2w0[kk] * rg(kk) + V->c2w1[kk] * rg(kk - 1)) : T(0.0);
+    return GRAV - (dpgw - dbuo) / rhog_h(kk, rg);
+  }
+  // bar v of g~(kk) -> addrg(kk, bar), addpre(kk, bar)
+  template <class RG, class PR, class ARG, class APR>
+  NH_HD void adj(int kk, RG rg, PR pre, ARG addrg, APR addpre, T v) const {
+    const T dpgw = in(kk) ? T(1.0) * (pre(kk) * T(1.0) - pre(kk - 1) * T(1.0)) * V->rdgzh[kk] : T(0.0);
+    const T dbuo = in(kk) ? -GRAV * (V->c2w0[kk] * rg(kk) + V->c2w1[kk] * rg(kk - 1)) : T(0.0);
+    const T rh = rhog_h(kk, rg);
+    const T dpgwb = -v / rh, dbuob = v / rh, rhb = v * (dpgw - dbuo) / (rh * rh);
+    if (in(kk)) {
+      addpre(kk, dpgwb * V->rdgzh[kk]);
+      addpre(kk - 1, -dpgwb * V->rdgzh[kk]);
+      addrg(kk, -GRAV * V->c2w0[kk] * dbuob);
+      addrg(kk - 1, -GRAV * V->c2w1[kk] * dbuob);
+    }
+    const int k2 = (kk < kmin) ? kmin : kk;
+    addrg(k2, rhb * V->c2w0[k2]);
+    addrg(k2 - 1, rhb * V->c2w1[k2]);
+  }
+};
+
+struct ViPath0A {
+  static constexpr const char* kName = "vi_path0_adj";
+  Dims d; Consts c; Geo G;
+  double dt;
+  const real* prog;
+  const real* diag;
+  const real* eth;
+  const real* vt[6];
+  real* gT_b;        // consumed
+  double* Mc_b; double* Mu_b; double* Ml_b;   // consumed
+  real* getot_b;     // consumed
+  real* prog_b;
+  real* diag_b;
+  real* eth_b;
+  real* gadv_b;      // 4 fields
+  real* vt_b[6];
+  real* dd_b[3];
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    const long NF = d.NF, ks = p.ks, a = p.a;
+    const int kmin = d.kmin, kmax = d.kmax, k = p.k;
+    const real R = c.RADIUS;
+    const real* rg = prog;
+    const real* rw = prog + 4 * NF;
+    const real* pre = diag + I_PRE * NF;
+    const long b = a - (long)k * ks;
+    auto lev = [&](int kk) { return b + (long)kk * ks; };
+    real gb[6];
+    for (int q = 0; q < 6; ++q) {
+      gb[q] = gT_b[q * NF + a];
+      gT_b[q * NF + a] = real(0.0);
+    }
+    const real geb = getot_b[a];
+    getot_b[a] = real(0.0);
+    double mcb = 0.0, mub = 0.0, mlb = 0.0;
+    if (k >= kmin + 1 && k <= kmax) {
+      mcb = Mc_b[a]; mub = Mu_b[a]; mlb = Ml_b[a];
+      Mc_b[a] = 0.0; Mu_b[a] = 0.0; Ml_b[a] = 0.0;
+    }
+    auto add_rg = [&](int kk, real v) { aadd(&prog_b[lev(kk)], v); };
+    auto add_pre = [&](int kk, real v) { aadd(&diag_b[I_PRE * NF + lev(kk)], v); };
+    auto rgl = [&](int kk) { return rg[lev(kk)]; };
+    auto prl = [&](int kk) { return pre[lev(kk)]; };
+    GztCol<real> gz{&G, c.GRAV, kmin, kmax};
+    real xx[3];
+    point_pos(G, d, p, xx);
+    const real Kh = c.GAMMA_H;
+    const real rgk = rg[a];
+    real rgkb = real(0.0);
+    // gT0 = -vt5 + drhog
+    aadd(&vt_b[5][a], -gb[0]);
+    // gT1..3 = gadv + P(-(vt Kh) rhog) - dpg + dd
+    real fvb[3] = {gb[1], gb[2], gb[3]};
+    for (int q = 0; q < 3; ++q) {
+      aadd(&gadv_b[q * NF + a], gb[1 + q]);
+      aadd(&dd_b[q][a], gb[1 + q]);
+    }
+    proj_t(xx, R, fvb);
+    for (int q = 0; q < 3; ++q) {
+      aadd(&vt_b[q][a], -fvb[q] * Kh * rgk);
+      rgkb += -fvb[q] * vt[q][a] * Kh;
+    }
+    // gT4 = gadv_w - (vt3 Kh) rhog_h - dpgw(k) + dbuo(k)
+    if (k > kmin && k <= kmax) {
+      aadd(&gadv_b[3 * NF + a], gb[4] * G.c2w0[k]);
+      aadd(&gadv_b[3 * NF + a - ks], gb[4] * G.c2w1[k]);
+    }
+    if (k >= kmin) {
+      const real rgh = G.c2w0[k] * rgk + G.c2w1[k] * rg[a - ks];
+      aadd(&vt_b[3][a], -gb[4] * Kh * rgh);
+      const real rghb = -gb[4] * vt[3][a] * Kh;
+      rgkb += rghb * G.c2w0[k];
+      aadd(&prog_b[a - ks], rghb * G.c2w1[k]);
+    }
+    if (gz.in(k)) {
+      add_pre(k, -gb[4] * G.rdgzh[k]);
+      add_pre(k - 1, gb[4] * G.rdgzh[k]);
+      add_rg(k, -c.GRAV * G.c2w0[k] * gb[4]);
+      add_rg(k - 1, -c.GRAV * G.c2w1[k] * gb[4]);
+    }
+    // gT5 = -vt4 + drhoge + pw; getot = -vt4
+    aadd(&vt_b[4][a], -(gb[5] + geb));
+    if (k >= kmin && k <= kmax) {
+      // drhog = -div(rhogv) - (rw(k+1) - rw(k)) rdgz
+      const real hdb = -gb[0];
+      if (k + 1 <= kmax) aadd(&prog_b[4 * NF + a + ks], -gb[0] * G.rdgz[k]);
+      if (k >= kmin + 1) aadd(&prog_b[4 * NF + a], gb[0] * G.rdgz[k]);
+      // drhoge = -div(rhogv eth) - (ws_hi - ws_lo) rdgz
+      const real heb = -gb[5];
+      if (k + 1 <= kmax) {
+        const real wsb = -gb[5] * G.rdgz[k];
+        const real w = rw[a + ks];
+        aadd(&prog_b[4 * NF + a + ks], wsb * (G.afact[k + 1] * eth[a + ks] + G.bfact[k + 1] * eth[a]));
+        aadd(&eth_b[a + ks], wsb * w * G.afact[k + 1]);
+        aadd(&eth_b[a], wsb * w * G.bfact[k + 1]);
+      }
+      if (k >= kmin + 1) {
+        const real wsb = gb[5] * G.rdgz[k];
+        const real w = rw[a];
+        aadd(&prog_b[4 * NF + a], wsb * (G.afact[k] * eth[a] + G.bfact[k] * eth[a - ks]));
+        aadd(&eth_b[a], wsb * w * G.afact[k]);
+        aadd(&eth_b[a - ks], wsb * w * G.bfact[k]);
+      }
+      // the two divergences
+      const int NN = p.pole ? 6 : 7;
+      for (int m = 0; m < NN; ++m) {
+        const long bn = p.pole ? a + m : a + nb_off(m, d.g);
+        real cdir[3];
+        if (p.pole) {
+          for (int q = 0; q < 3; ++q) cdir[q] = G.cdiv_pl[(m * 2 + p.lp) * 3 + q];
+        } else {
+          for (int q = 0; q < 3; ++q) cdir[q] = rc(G.cdiv, d.nreg, q * 7 + m, p.ci);
+        }
+        const real e = eth[bn];
+        real eb = real(0.0);
+        for (int q = 0; q < 3; ++q) {
+          const real t0 = cdir[q] * hdb, t1 = cdir[q] * heb;
+          aadd(&prog_b[(1 + q) * NF + bn], t0 + t1 * e);
+          eb += t1 * prog[(1 + q) * NF + bn];
+        }
+        aadd(&eth_b[bn], eb);
+      }
+      // pressure gradient (level kmin -> kmin+1) and pw
+      const long sh = (k == kmin) ? ks : 0;
+      real gr[3];
+      for (int dd2 = 0; dd2 < 3; ++dd2) {
+        real acc = real(0.0);
+        if (p.pole) {
+          for (int v = 0; v < 6; ++v) {
+            const real pv = G.cgrad_pl[(v * 2 + p.lp) * 3 + dd2] * (pre[a + v + sh] * real(1.0) * real(1.0));
+            acc = (v == 0) ? pv : acc + pv;
+          }
+        } else {
+          acc = rc(G.cgrad, d.nreg, dd2 * 7, p.ci) * (pre[a + sh] * real(1.0));
+          for (int m = 1; m < 7; ++m)
+            acc = acc + rc(G.cgrad, d.nreg, dd2 * 7 + m, p.ci) * (pre[a + nb_off(m, d.g) + sh] * real(1.0));
+        }
+        gr[dd2] = acc + real(0.0);
+      }
+      real dpg[3];
+      {
+        const real prd = (gr[0] * xx[0] + gr[1] * xx[1] + gr[2] * xx[2]) / R;
+        for (int q = 0; q < 3; ++q) dpg[q] = gr[q] - prd * xx[q] / R;
+      }
+      const real pwb = gb[5];
+      real dpgb[3];
+      for (int q = 0; q < 3; ++q) {
+        dpgb[q] = -gb[1 + q] + pwb * diag[(I_VX + q) * NF + a];
+        aadd(&diag_b[(I_VX + q) * NF + a], pwb * dpg[q]);
+      }
+      proj_t(xx, R, dpgb);
+      for (int dd2 = 0; dd2 < 3; ++dd2) {
+        if (p.pole) {
+          for (int v = 0; v < 6; ++v) aadd(&diag_b[I_PRE * NF + a + v + sh], G.cgrad_pl[(v * 2 + p.lp) * 3 + dd2] * dpgb[dd2]);
+        } else {
+          for (int m = 0; m < 7; ++m)
+            aadd(&diag_b[I_PRE * NF + a + nb_off(m, d.g) + sh], rc(G.cgrad, d.nreg, dd2 * 7 + m, p.ci) * dpgb[dd2]);
+        }
+      }
+      // pw: w2c0(k) pwh(k+1) + w2c1(k) pwh(k), pwh(kk) = -g~(kk) rw(kk)
+      for (int s2 = 0; s2 < 2; ++s2) {
+        const int kk = (s2 == 0) ? k + 1 : k;
+        const real wgt = (s2 == 0) ? G.w2c0[k] : G.w2c1[k];
+        const real hb = pwb * wgt;
+        const real gzv = gz.value(kk, rgl, prl);
+        aadd(&prog_b[4 * NF + lev(kk)], -hb * gzv);
+        gz.adj(kk, rgl, prl, add_rg, add_pre, -hb * rw[lev(kk)]);
+      }
+    }
+    aadd(&prog_b[a], rgkb);
+    if (k >= kmin + 1 && k <= kmax) {
+      // tridiagonal coefficients (FP64)
+      const VProf<double>& V = G.v64;
+      const ConstsT<double>& C = c.d;
+      GztCol<double> gz64{&V, C.GRAV, kmin, kmax};
+      auto rg64 = [&](int kk) { return (double)rg[lev(kk)]; };
+      auto pr64 = [&](int kk) { return (double)pre[lev(kk)]; };
+      auto add_rg64 = [&](int kk, double v) { aadd(&prog_b[lev(kk)], (real)v); };
+      auto add_pre64 = [&](int kk, double v) { aadd(&diag_b[I_PRE * NF + lev(kk)], (real)v); };
+      auto eth_h_t = [&](int kk, double v) {
+        const int k2 = (kk < kmin) ? kmin : kk;
+        aadd(&eth_b[lev(k2)], (real)(v * V.afact[k2]));
+        aadd(&eth_b[lev(k2 - 1)], (real)(v * V.bfact[k2]));
+      };
+      const double Ca = (1.0 * V.rdgz[k] + 1.0 * V.rdgz[k - 1]) * 1.0;
+      const double Cb = V.dfact[k] - V.cfact[k - 1];
+      const double rz = V.rdgzh[k];
+      eth_h_t(k, mcb * rz * Ca);
+      gz64.adj(k, rg64, pr64, add_rg64, add_pre64, -mcb * rz * Cb);
+      eth_h_t(k + 1, -mub * rz * V.rdgz[k]);
+      gz64.adj(k + 1, rg64, pr64, add_rg64, add_pre64, -mub * rz * V.cfact[k]);
+      eth_h_t(k - 1, -mlb * rz * V.rdgz[k]);
+      gz64.adj(k - 1, rg64, pr64, add_rg64, add_pre64, mlb * rz * V.dfact[k - 1]);
+    }
+  }
+};
+
+// ----------------------------------------------------------------------------
+// surface pressure (mod_bndcnd.py:521-555, BNDCND_pre_sfc with a flat surface
+// z_srf = 0): density extrapolated by the quadratic Lagrange polynomial through
+// the lowest three full levels, pressure by hydrostatic balance from kmin.
+// One thread per column, FP64 from the stored fields.
+// ----------------------------------------------------------------------------
+struct SfcPresP {
+  static constexpr const char* kName = "sfc_pressure";
+  Dims d; Consts c; Geo G;
+  const real* prog;
+  const real* rhogq;
+  double* ps;          // [nreg + 12]
+  const double* ps_b;  // adjoint: bar of ps (null for the forward)
+  real* prog_b;
+  struct Th { double rho, ein, q, cv, tem, Rm, pre; };
+  NH_HD Th thermo(long a) const {
+    const ConstsT<double>& C = c.d;
+    Th t;
+    const double rhog = (double)prog[a];
+    t.rho = rhog;
+    t.ein = (double)prog[5 * d.NF + a] / rhog;
+    t.q = (double)rhogq[a] / rhog;
+    t.cv = t.q * C.CVW_qv + (1.0 - t.q) * C.CVdry;
+    t.tem = t.ein / t.cv;
+    t.Rm = (1.0 - t.q) * C.Rdry + t.q * C.Rvap;
+    t.pre = t.rho * t.tem * t.Rm;
+    return t;
+  }
+  NH_HD void operator()(long t) const {
+    Cell cl = cell_of(d, t);
+    const int k1 = d.kmin;
+    const VProf<double>& V = G.v64;
+    const double z1 = V.gz[k1], z2 = V.gz[k1 + 1], z3 = V.gz[k1 + 2], zs = 0.0;
+    const double L1 = ((zs - z2) * (zs - z3)) / ((z1 - z2) * (z1 - z3));
+    const double L2 = ((zs - z1) * (zs - z3)) / ((z2 - z1) * (z2 - z3));
+    const double L3 = ((zs - z1) * (zs - z2)) / ((z3 - z1) * (z3 - z2));
+    const long a1 = cl.base + (long)k1 * cl.ks, a2 = a1 + cl.ks, a3 = a2 + cl.ks;
+    const double GRAV = c.d.GRAV;
+    const Th t1 = thermo(a1);
+    if (!ps_b) {
+      const double rho_srf = L1 * (double)prog[a1] + L2 * (double)prog[a2] + L3 * (double)prog[a3];
+      ps[t] = t1.pre + 0.5 * (rho_srf + (double)prog[a1]) * GRAV * (z1 - zs);
+      return;
+    }
+    const double pb = ps_b[t];
+    const double rsb = 0.5 * GRAV * (z1 - zs) * pb;
+    double rg1b = rsb + rsb * L1;
+    prog_b[a2] += (real)(rsb * L2);
+    prog_b[a3] += (real)(rsb * L3);
+    // pre(kmin) = rho tem Rm, tem = ein / cv
+    const ConstsT<double>& C = c.d;
+    double rhob = pb * t1.tem * t1.Rm;
+    const double temb = pb * t1.rho * t1.Rm;
+    const double Rmb = pb * t1.rho * t1.tem;
+    const double einb = temb / t1.cv;
+    const double cvb = -temb * t1.tem / t1.cv;
+    const double qb = cvb * (C.CVW_qv - C.CVdry) + Rmb * (C.Rvap - C.Rdry);
+    const double rhog = (double)prog[a1];
+    prog_b[5 * d.NF + a1] += (real)(einb / rhog);
+    rg1b += rhob - einb * t1.ein / rhog - qb * t1.q / rhog;
+    prog_b[a1] += (real)rg1b;
+  }
+};
+
+NH_END

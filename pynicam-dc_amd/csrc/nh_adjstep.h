@@ -1,0 +1,349 @@
+// Adjoint of the dynamics step and the surface-pressure diagnostic.
+//
+// dyn_step_adj(prog_in, progq, prog_b): prog_b holds the bar of the state
+// after one dyn_step from prog_in on entry and the bar of prog_in on exit.
+// Checkpointing inside the step: a forward sweep records the input of each
+// RK stage; the reverse sweep then, per stage, recomputes the stage's
+// large-step fields and its small steps (each small step's split state kept
+// in a checkpoint slot), and runs the transposed kernels of nh_adjoint.h in
+// reverse order.  The tracer masses are held at progq (see nh_adjoint.h).
+#pragma once
+
+#include "nh_adjoint.h"
+#include "nh_engine.h"
+
+NH_BEGIN
+
+// adjoint workspace, fields of NF values (field precision)
+enum AwsField {
+  A_PROG = 0,      // 6  state of the recomputed stage
+  A_SC = 6,        // 18 stage inputs
+  A_P0B = 24,      // 6  bar of the step's initial state (PROG0)
+  A_DIAGB = 30,    // 6  stage bars from here to A_CK
+  A_ETHB = 36,     // 1
+  A_GADVB = 37,    // 4
+  A_HDAB = 41,     // 6
+  A_HDBB = 47,     // 6
+  A_DDB = 53,      // 3
+  A_DDTB = 56,     // 3
+  A_GTB = 59,      // 6
+  A_GETOTB = 65,   // 1
+  A_ETHTOTB = 66,  // 1
+  A_PSB = 67,      // 6  bar of the split state after the small step being reversed
+  A_PSB2 = 73,     // 6  bar of the split state before it
+  A_HVB = 79,      // 3
+  A_DDVB = 82,     // 3
+  A_CK = 85        // 6 per small-step checkpoint slot
+};
+// FP64: bars of the tridiagonal matrix, the recomputed solution
+enum AdwsField { AD_MCB = 0, AD_MUB = 1, AD_MLB = 2, AD_X = 3, AD_NFIELDS = 4 };
+
+inline long adj_fields(int nck) { return (long)A_CK + 6L * nck; }
+
+inline real* afield(const Context& ctx, int f) { return ctx.aws + (long)f * ctx.d.NF; }
+inline double* adfield(const Context& ctx, int f) { return ctx.adws + (long)f * ctx.d.NF; }
+
+template <class X>
+void zero_fields(X& x, real* p, long nfields, long NF) {
+  x.run(FillZero{p}, nfields * NF);
+}
+
+// transpose of halo(): destination bars onto their sources, up to 16 fields
+template <class X>
+void halo_adj(X& x, const Context& ctx, real* const* f, int nf) {
+  if (ctx.halo_n == 0) return;
+  for (int q0 = 0; q0 < nf; q0 += 16) {
+    HaloScatterA h;
+    h.d = ctx.d;
+    h.dst = ctx.halo_dst;
+    h.src = ctx.halo_src;
+    h.n = ctx.halo_n;
+    h.nf = (nf - q0 < 16) ? nf - q0 : 16;
+    for (int q = 0; q < h.nf; ++q) h.f[q] = f[q0 + q];
+    x.run(h, ctx.halo_n * ctx.d.kall);
+  }
+}
+
+template <class X>
+void halo_block_adj(X& x, const Context& ctx, real* base, int nf) {
+  real* f[16];
+  for (int q0 = 0; q0 < nf; q0 += 16) {
+    const int n = (nf - q0 < 16) ? nf - q0 : 16;
+    for (int q = 0; q < n; ++q) f[q] = base + (long)(q0 + q) * ctx.d.NF;
+    halo_adj(x, ctx, f, n);
+  }
+}
+
+// transpose of divdamp(): bars of the damping force ob -> bars of
+// rhogvx, rhogvy, rhogvz, rhogw (vb); A_DDTB is the intermediate bar
+template <class X>
+void divdamp_adj(X& x, const Context& ctx, real* const ob[3], real* const vb[4]) {
+  real* tb[3] = {afield(ctx, A_DDTB), afield(ctx, A_DDTB + 1), afield(ctx, A_DDTB + 2)};
+  zero_fields(x, tb[0], 3, ctx.d.NF);
+  DivDamp2A b;
+  b.d = ctx.d; b.c = ctx.c; b.G = ctx.G;
+  for (int q = 0; q < 3; ++q) { b.ob[q] = ob[q]; b.ib[q] = tb[q]; }
+  x.run_pts(b, ctx.d);
+  halo_adj(x, ctx, tb, 3);
+  DivDamp1A a;
+  a.d = ctx.d; a.c = ctx.c; a.G = ctx.G;
+  for (int q = 0; q < 3; ++q) a.ob[q] = tb[q];
+  for (int q = 0; q < 4; ++q) a.vb[q] = vb[q];
+  x.run_pts(a, ctx.d);
+}
+
+// reverse of one small step: ck = its input split state, nx = its output
+// (we0, dv0, dv1, dv2, we1, we2); psb (bar of nx, consumed) -> psb2 (bar of
+// ck, added) and the stage bars
+template <class X>
+void small_step_adj(X& x, const Context& ctx, const real* prog, real* const ck[6], real* const nx[6], double dts,
+                    real* prog_b, real* const psb[6], real* const psb2[6]) {
+  const Dims& d = ctx.d;
+  const long NF = d.NF;
+  real* gT = ctx.field(W_GT);
+  real* gTb = afield(ctx, A_GTB);
+  real* ddv = ctx.field(W_DDV);
+  // forward values of the step: damping force of ck, divergences of the new momentum
+  divdamp(x, ctx, ck[1], ck[2], ck[3], ck[4], ddv, ddv + NF, ddv + 2 * NF);
+  real* dvn[3] = {nx[1], nx[2], nx[3]};
+  small_hv(x, ctx, prog, dvn);
+  real* hv[3] = {ctx.field(W_HV), ctx.field(W_HV + 1), ctx.field(W_HV + 2)};
+  real* hvb[3] = {afield(ctx, A_HVB), afield(ctx, A_HVB + 1), afield(ctx, A_HVB + 2)};
+  real* web[3] = {psb[0], psb[4], psb[5]};
+  real* dvb[3] = {psb[1], psb[2], psb[3]};
+  halo_adj(x, ctx, web, 3);
+  {
+    ViMainPostA f;
+    f.d = d; f.c = ctx.c; f.G = ctx.G;
+    f.dt = (real)dts;
+    f.prog = prog;
+    for (int q = 0; q < 6; ++q) { f.ps[q] = ck[q]; f.ps_b[q] = psb2[q]; }
+    for (int q = 0; q < 3; ++q) { f.dv[q] = dvn[q]; f.dv_b[q] = dvb[q]; f.hv[q] = hv[q]; f.hv_b[q] = hvb[q]; }
+    f.ethtot = ctx.field(W_ETHTOT); f.gT = gT; f.hv[0] = hv[0];
+    f.we1 = nx[4];
+    for (int q = 0; q < 3; ++q) f.we_b[q] = web[q];
+    f.prog_b = prog_b; f.ethtot_b = afield(ctx, A_ETHTOTB); f.gT_b = gTb; f.getot_b = afield(ctx, A_GETOTB);
+    x.run_pts(f, d);
+  }
+  {
+    ViMainVA f;
+    f.d = d; f.c = ctx.c; f.G = ctx.G;
+    f.dt = dts;
+    for (int q = 0; q < 6; ++q) { f.ps[q] = ck[q]; f.ps_b[q] = psb2[q]; }
+    f.gT = gT;
+    f.Mc = ctx.dfield(D_MC); f.Mu = ctx.dfield(D_MU); f.Ml = ctx.dfield(D_ML);
+    for (int q = 0; q < 3; ++q) { f.hv[q] = hv[q]; f.hv_b[q] = hvb[q]; }
+    f.U = ctx.dfield(D_U); f.Gm = ctx.dfield(D_GM); f.X = adfield(ctx, AD_X);
+    f.out_b = web[1];
+    f.gT_b = gTb;
+    f.Mc_b = adfield(ctx, AD_MCB); f.Mu_b = adfield(ctx, AD_MUB); f.Ml_b = adfield(ctx, AD_MLB);
+    x.run(f, ncols(d));
+  }
+  {
+    ViMainHA f;
+    f.d = d; f.c = ctx.c; f.G = ctx.G;
+    f.prog = prog;
+    for (int q = 0; q < 3; ++q) { f.dv[q] = dvn[q]; f.hv_b[q] = hvb[q]; f.dv_b[q] = dvb[q]; }
+    f.eth = ctx.field(W_ETH); f.ethtot = ctx.field(W_ETHTOT);
+    f.eth_b = afield(ctx, A_ETHB); f.ethtot_b = afield(ctx, A_ETHTOTB);
+    f.prog_b = prog_b; f.gT_b = gTb;
+    x.run_pts(f, d);
+  }
+  halo_adj(x, ctx, dvb, 3);
+  real* ddvb[3] = {afield(ctx, A_DDVB), afield(ctx, A_DDVB + 1), afield(ctx, A_DDVB + 2)};
+  {
+    Path1A f;
+    f.d = d; f.c = ctx.c; f.G = ctx.G;
+    f.dt = (real)dts;
+    f.prog = prog;
+    for (int q = 0; q < 6; ++q) { f.ps[q] = ck[q]; f.ps_b[q] = psb2[q]; }
+    f.gT = gT;
+    for (int q = 0; q < 3; ++q) { f.dd[q] = ddv + q * NF; f.dv_b[q] = dvb[q]; f.dd_b[q] = ddvb[q]; }
+    f.prog_b = prog_b; f.gT_b = gTb;
+    x.run_pts(f, d);
+  }
+  real* vb[4] = {psb2[1], psb2[2], psb2[3], psb2[4]};
+  divdamp_adj(x, ctx, ddvb, vb);
+}
+
+// reverse of the large-step part of a stage (stage_large); prog is the
+// recomputed boundary-conditioned state
+template <class X>
+void stage_large_adj(X& x, const Context& ctx, const real* prog, const real* rhogq, double dts, real* prog_b) {
+  const Dims& d = ctx.d;
+  const long NF = d.NF;
+  real* diag = ctx.field(W_DIAG);
+  real* diagb = afield(ctx, A_DIAGB);
+  real* ethb = afield(ctx, A_ETHB);
+  {
+    EthTotA f;
+    f.d = d; f.c = ctx.c; f.G = ctx.G;
+    f.prog = prog;
+    f.ethtot_b = afield(ctx, A_ETHTOTB); f.eth_b = ethb; f.prog_b = prog_b;
+    x.run_pts(f, d);
+  }
+  {
+    ViPath0A f;
+    f.d = d; f.c = ctx.c; f.G = ctx.G;
+    f.dt = dts;
+    f.prog = prog; f.diag = diag; f.eth = ctx.field(W_ETH);
+    for (int q = 0; q < 6; ++q) { f.vt[q] = ctx.field(W_HDB + q); f.vt_b[q] = afield(ctx, A_HDBB + q); }
+    f.gT_b = afield(ctx, A_GTB);
+    f.Mc_b = adfield(ctx, AD_MCB); f.Mu_b = adfield(ctx, AD_MUB); f.Ml_b = adfield(ctx, AD_MLB);
+    f.getot_b = afield(ctx, A_GETOTB);
+    f.prog_b = prog_b; f.diag_b = diagb; f.eth_b = ethb;
+    f.gadv_b = afield(ctx, A_GADVB);
+    for (int q = 0; q < 3; ++q) f.dd_b[q] = afield(ctx, A_DDB + q);
+    x.run_pts(f, d);
+  }
+  {
+    real* ob[3] = {afield(ctx, A_DDB), afield(ctx, A_DDB + 1), afield(ctx, A_DDB + 2)};
+    real* vb[4] = {prog_b + NF, prog_b + 2 * NF, prog_b + 3 * NF, prog_b + 4 * NF};
+    divdamp_adj(x, ctx, ob, vb);
+  }
+  halo_block_adj(x, ctx, afield(ctx, A_HDBB), 6);
+  {
+    HDiffPass2A f;
+    f.d = d; f.c = ctx.c; f.G = ctx.G;
+    for (int q = 0; q < 6; ++q) {
+      f.in[q] = ctx.field(W_HDA + q);
+      f.ob[q] = afield(ctx, A_HDBB + q);
+      f.ib[q] = afield(ctx, A_HDAB + q);
+    }
+    f.rhog = prog;
+    f.rhog_b = prog_b;
+    x.run_pts(f, d);
+  }
+  halo_block_adj(x, ctx, afield(ctx, A_HDAB), 6);
+  {
+    HDiffPass1A f;
+    f.d = d; f.G = ctx.G;
+    for (int q = 0; q < 6; ++q) f.ob[q] = afield(ctx, A_HDAB + q);
+    f.ib[0] = diagb + I_VX * NF; f.ib[1] = diagb + I_VY * NF; f.ib[2] = diagb + I_VZ * NF;
+    f.ib[3] = diagb + I_W * NF; f.ib[4] = diagb + I_TEM * NF; f.ib[5] = prog_b;
+    x.run_pts(f, d);
+  }
+  {
+    AdvMomA f;
+    f.d = d; f.c = ctx.c; f.G = ctx.G;
+    f.prog = prog; f.diag = diag;
+    f.ob = afield(ctx, A_GADVB);
+    f.prog_b = prog_b; f.diag_b = diagb;
+    x.run_pts(f, d);
+  }
+  {
+    PrePostA f;
+    f.d = d; f.c = ctx.c; f.G = ctx.G;
+    f.prog = prog; f.rhogq = rhogq;
+    f.prog_b = prog_b; f.diag_b = diagb; f.eth_b = ethb;
+    x.run_pts(f, d);
+  }
+}
+
+template <class X>
+int dyn_step_adj(X& x, const Context& ctx, const real* prog_in, const real* progq, int nq, double dtl,
+                 int sstep_max, real* prog_b) {
+  const Dims& d = ctx.d;
+  const long NF = d.NF;
+  if (nq < 1 || !prog_in || !progq || !prog_b) return NH_ERR_ARG;
+  if (!ctx.aws || ctx.adj_ck < sstep_max + 1) return NH_ERR_STATE;
+  const double dts = dtl / (double)sstep_max;
+  const int nsub[3] = {sstep_max / 3, sstep_max / 2, sstep_max};
+  real* P = afield(ctx, A_PROG);
+  real* prog0 = ctx.field(W_PROG0);
+  auto ck_slot = [&](int i, real* f[6]) {
+    for (int q = 0; q < 6; ++q) f[q] = afield(ctx, A_CK + 6 * i + q);
+  };
+  // forward sweep: the inputs of stages 0..2
+  x.d2d(prog0, prog_in, 6 * NF * sizeof(real));
+  x.d2d(P, prog_in, 6 * NF * sizeof(real));
+  for (int nl = 0; nl < 3; ++nl) {
+    x.d2d(afield(ctx, A_SC + 6 * nl), P, 6 * NF * sizeof(real));
+    if (nl == 2) break;
+    stage_large(x, ctx, P, progq, dts);
+    SmallBufs s;
+    for (int q = 0; q < 6; ++q) s.ps[q] = ctx.field(W_SPLIT + q);
+    for (int q = 0; q < 3; ++q) s.dv[q] = ctx.field(W_SPLIT + 6 + q);
+    for (int q = 0; q < 3; ++q) s.we[q] = ctx.field(W_SPLIT + 9 + q);
+    x.run(SplitInit{NF, P, prog0, {s.ps[0], s.ps[1], s.ps[2], s.ps[3], s.ps[4], s.ps[5]}, nl == 0 ? 1 : 0}, NF);
+    for (int it = 0; it < nsub[nl]; ++it) {
+      small_step(x, ctx, P, s, dts, 1.0 / (double)nsub[nl]);
+      real* old[6] = {s.ps[0], s.ps[1], s.ps[2], s.ps[3], s.ps[4], s.ps[5]};
+      s.ps[0] = s.we[0]; s.ps[1] = s.dv[0]; s.ps[2] = s.dv[1]; s.ps[3] = s.dv[2]; s.ps[4] = s.we[1];
+      s.ps[5] = s.we[2];
+      s.we[0] = old[0]; s.dv[0] = old[1]; s.dv[1] = old[2]; s.dv[2] = old[3]; s.we[1] = old[4]; s.we[2] = old[5];
+    }
+    prog_update(x, ctx, P, s.ps);
+    halo_block(x, ctx, P, 6);
+  }
+  // reverse sweep
+  halo_block_adj(x, ctx, prog_b, 6);   // the final halo fill of PROG
+  zero_fields(x, afield(ctx, A_P0B), 6, NF);
+  for (int nl = 2; nl >= 0; --nl) {
+    if (nl < 2) halo_block_adj(x, ctx, prog_b, 6);
+    // recompute the stage, keeping every small step's split state
+    x.d2d(P, afield(ctx, A_SC + 6 * nl), 6 * NF * sizeof(real));
+    stage_large(x, ctx, P, progq, dts);
+    real* ck[6];
+    ck_slot(0, ck);
+    x.run(SplitInit{NF, P, prog0, {ck[0], ck[1], ck[2], ck[3], ck[4], ck[5]}, nl == 0 ? 1 : 0}, NF);
+    const double rweight = 1.0 / (double)nsub[nl];
+    for (int it = 0; it < nsub[nl]; ++it) {
+      real *c0[6], *c1[6];
+      ck_slot(it, c0);
+      ck_slot(it + 1, c1);
+      SmallBufs s;
+      for (int q = 0; q < 6; ++q) s.ps[q] = c0[q];
+      s.we[0] = c1[0]; s.dv[0] = c1[1]; s.dv[1] = c1[2]; s.dv[2] = c1[3]; s.we[1] = c1[4]; s.we[2] = c1[5];
+      small_step(x, ctx, P, s, dts, rweight);
+    }
+    // adjoint of the stage
+    zero_fields(x, afield(ctx, A_DIAGB), A_CK - A_DIAGB, NF);
+    x.run(FillZero64{adfield(ctx, AD_MCB)}, 3 * NF);
+    real *psb[6], *psb2[6];
+    for (int q = 0; q < 6; ++q) {
+      psb[q] = afield(ctx, A_PSB + q);
+      psb2[q] = afield(ctx, A_PSB2 + q);
+    }
+    {
+      ProgUpdateA f;
+      f.d = d; f.c = ctx.c; f.G = ctx.G;
+      f.prog_b = prog_b;
+      for (int q = 0; q < 6; ++q) f.ps_b[q] = psb[q];
+      x.run_pts(f, d);
+    }
+    for (int it = nsub[nl] - 1; it >= 0; --it) {
+      real *c0[6], *c1[6];
+      ck_slot(it, c0);
+      ck_slot(it + 1, c1);
+      zero_fields(x, psb2[0], 6, NF);
+      small_step_adj(x, ctx, P, c0, c1, dts, prog_b, psb, psb2);
+      for (int q = 0; q < 6; ++q) {
+        real* t = psb[q];
+        psb[q] = psb2[q];
+        psb2[q] = t;
+      }
+    }
+    x.run(SplitInitA{NF, prog_b, afield(ctx, A_P0B), {psb[0], psb[1], psb[2], psb[3], psb[4], psb[5]},
+                     nl == 0 ? 1 : 0},
+          NF);
+    stage_large_adj(x, ctx, P, progq, dts, prog_b);
+    int rc = x.check("dyn_step_adj stage");
+    if (rc) return rc;
+  }
+  x.run(AxpyP{prog_b, afield(ctx, A_P0B)}, 6 * NF);
+  return x.check("dyn_step_adj");
+}
+
+// surface pressure per column (forward) or its transpose added into prog_b
+template <class X>
+int sfc_pressure(X& x, const Context& ctx, const real* prog, const real* rhogq, double* ps, const double* ps_b,
+                 real* prog_b) {
+  if (!prog || !rhogq || (!ps_b && !ps) || (ps_b && !prog_b)) return NH_ERR_ARG;
+  if (ctx.d.kmin + 2 > ctx.d.kmax) return NH_ERR_ARG;
+  SfcPresP f{ctx.d, ctx.c, ctx.G, prog, rhogq, ps, ps_b, prog_b};
+  x.run(f, ncols(ctx.d));
+  return x.check(ps_b ? "nh_sfc_pressure_adj" : "nh_sfc_pressure");
+}
+
+NH_END

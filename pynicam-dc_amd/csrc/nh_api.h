@@ -131,6 +131,26 @@ inline int budget(nh_context* ctx, const void* prog, const void* progq, int32_t 
   return done(NH_DISPATCH(ctx, v_budget, prog, progq, lhv, out, stream, &err), where, err);
 }
 
+inline int dyn_step_adj(nh_context* ctx, const void* prog, const void* progq, int32_t nq, double dtl,
+                        int32_t sstep_max, void* prog_b, void* stream, int f32) {
+  const char* where = f32 ? "nh_dyn_step_adj_f32" : "nh_dyn_step_adj";
+  if (int rc = check_ready(ctx, where, f32, true)) return rc;
+  if (!prog || !progq || nq < 1 || !prog_b)
+    return fail(NH_ERR_ARG, where, "need prog, prog_b and at least one tracer field");
+  if (sstep_max < 3 || sstep_max % 6 != 0 || !(dtl > 0.0))
+    return fail(NH_ERR_ARG, where, "sstep_max must be a positive multiple of 6 and dtl > 0");
+  std::string err;
+  return done(NH_DISPATCH(ctx, v_dyn_step_adj, prog, progq, nq, dtl, sstep_max, prog_b, stream, &err), where, err);
+}
+
+inline int sfc_pressure(nh_context* ctx, const void* prog, const void* progq, double* ps, const double* ps_b,
+                        void* prog_b, void* stream, int f32, const char* where) {
+  if (int rc = check_ready(ctx, where, f32, false)) return rc;
+  if (!prog || !progq || (ps_b ? !prog_b : !ps)) return fail(NH_ERR_ARG, where, "null pointer");
+  std::string err;
+  return done(NH_DISPATCH(ctx, v_sfc_pressure, prog, progq, ps, ps_b, prog_b, stream, &err), where, err);
+}
+
 }  // namespace nh
 
 extern "C" {
@@ -276,6 +296,30 @@ int nh_prof_report(nh_context* ctx, int32_t max_kernels, char* names, double* to
   if (n < 0) return nh::fail(n, where, err);
   *n_kernels = n;
   return 0;
+}
+
+int nh_dyn_step_adj(nh_context* ctx, const double* prog, const double* progq, int32_t nq, double dtl,
+                    int32_t sstep_max, double* prog_b, void* stream) {
+  return nh::dyn_step_adj(ctx, prog, progq, nq, dtl, sstep_max, prog_b, stream, 0);
+}
+int nh_dyn_step_adj_f32(nh_context* ctx, const float* prog, const float* progq, int32_t nq, double dtl,
+                        int32_t sstep_max, float* prog_b, void* stream) {
+  return nh::dyn_step_adj(ctx, prog, progq, nq, dtl, sstep_max, prog_b, stream, 1);
+}
+
+int nh_sfc_pressure(nh_context* ctx, const double* prog, const double* progq, double* ps, void* stream) {
+  return nh::sfc_pressure(ctx, prog, progq, ps, nullptr, nullptr, stream, 0, "nh_sfc_pressure");
+}
+int nh_sfc_pressure_f32(nh_context* ctx, const float* prog, const float* progq, double* ps, void* stream) {
+  return nh::sfc_pressure(ctx, prog, progq, ps, nullptr, nullptr, stream, 1, "nh_sfc_pressure_f32");
+}
+int nh_sfc_pressure_adj(nh_context* ctx, const double* prog, const double* progq, const double* ps_b,
+                        double* prog_b, void* stream) {
+  return nh::sfc_pressure(ctx, prog, progq, nullptr, ps_b, prog_b, stream, 0, "nh_sfc_pressure_adj");
+}
+int nh_sfc_pressure_adj_f32(nh_context* ctx, const float* prog, const float* progq, const double* ps_b,
+                            float* prog_b, void* stream) {
+  return nh::sfc_pressure(ctx, prog, progq, nullptr, ps_b, prog_b, stream, 1, "nh_sfc_pressure_adj_f32");
 }
 
 }  // extern "C"

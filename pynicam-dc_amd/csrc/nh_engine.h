@@ -78,6 +78,11 @@ struct Context {
   long halo_n = 0;
   real* ws = nullptr;
   double* dws = nullptr;
+  // adjoint workspace (allocated by the first adjoint call)
+  real* aws = nullptr;
+  double* adws = nullptr;
+  int adj_ck = 0;              // small-step checkpoints it holds
+  bool halo_adj_ok = false;    // halo destinations unique and never sources
   bool geo_ready = false;
   int device = 0;
   void* prof = nullptr;   // executor-specific kernel timer state
@@ -234,6 +239,18 @@ int upload_halo(X& x, Context& ctx, const int32_t* dst, const int32_t* src, long
   if (ctx.halo_dst) { x.release(ctx.halo_dst); ctx.halo_dst = nullptr; }
   if (ctx.halo_src) { x.release(ctx.halo_src); ctx.halo_src = nullptr; }
   ctx.halo_n = n;
+  // the adjoint of the gather scatters destination bars onto sources in one
+  // pass: that needs every destination once and no destination as a source
+  {
+    std::vector<char> role((size_t)ncol, 0);
+    bool ok = true;
+    for (long h = 0; h < n && ok; ++h) {
+      if (role[dst[h]] & 1) ok = false;
+      role[dst[h]] |= 1;
+    }
+    for (long h = 0; h < n && ok; ++h) ok = !(role[src[h]] & 1);
+    ctx.halo_adj_ok = ok;
+  }
   if (n == 0) return 0;
   ctx.halo_dst = (int*)x.alloc(n * sizeof(int));
   ctx.halo_src = (int*)x.alloc(n * sizeof(int));
@@ -446,152 +463,181 @@ int budget(X& x, const Context& ctx, const real* prog, const real* progq, double
 // ----------------------------------------------------------------------------
 // one large step (mod_dynamics.py:1725-2741 with the RK3/HEVI schedule)
 // ----------------------------------------------------------------------------
+// the small-step buffers: split state ps (input), path1 momentum dv and
+// vi_main output we; the new split state is (we0, dv0, dv1, dv2, we1, we2)
+struct SmallBufs {
+  real* ps[6];
+  real* dv[3];
+  real* we[3];
+};
+
+// large-step part of RK stage nl: diagnostics and boundary values (in place
+// on prog's ghost rows), momentum advection, hyperdiffusion, divergence
+// damping of prog, the sources and matrix of the small steps, ethtot and the
+// start of the mean fluxes
+template <class X>
+void stage_large(X& x, const Context& ctx, real* prog, const real* rhogq, double dts) {
+  const Dims& d = ctx.d;
+  const long NF = d.NF;
+  real* diag = ctx.field(W_DIAG);
+  real* eth = ctx.field(W_ETH);
+  real* gadv = ctx.field(W_GADV);
+  {
+    PrePostP f{d, ctx.c, ctx.G, prog, rhogq, diag, eth};
+    x.run_pts(f, d);
+  }
+  run_staged(x, AdvMomP{d, ctx.c, ctx.G, prog, diag, gadv});
+  // hyperdiffusion: two passes with a halo fill after each
+  {
+    HDiffPassP<0> f;
+    f.d = d; f.c = ctx.c; f.G = ctx.G;
+    f.in[0] = diag + I_VX * NF; f.in[1] = diag + I_VY * NF; f.in[2] = diag + I_VZ * NF;
+    f.in[3] = diag + I_W * NF; f.in[4] = diag + I_TEM * NF; f.in[5] = prog;
+    for (int q = 0; q < 6; ++q) f.out[q] = ctx.field(W_HDA + q);
+    f.rhog = prog;
+    run_staged(x, f);
+    halo_block(x, ctx, ctx.field(W_HDA), 6);
+    HDiffPass2T f2;
+    f2.d = d; f2.c = ctx.c; f2.G = ctx.G; f2.tg = tile_geom(d, 12);
+    for (int q = 0; q < 6; ++q) {
+      f2.in[q] = ctx.field(W_HDA + q);
+      f2.out[q] = ctx.field(W_HDB + q);
+    }
+    f2.rhog = prog;
+    x.run_tiles(f2, f2.tg.nblocks, f2.tg.lds);
+    halo_block(x, ctx, ctx.field(W_HDB), 6);
+  }
+  // divergence damping of the large-step momentum
+  divdamp(x, ctx, prog + 1 * NF, prog + 2 * NF, prog + 3 * NF, prog + 4 * NF,
+          ctx.field(W_DD), ctx.field(W_DD + 1), ctx.field(W_DD + 2));
+  {
+    ViPath0P f;
+    f.d = d; f.c = ctx.c; f.G = ctx.G;
+    f.dt = dts;
+    f.prog = prog; f.diag = diag; f.eth = eth; f.gadv = gadv;
+    for (int q = 0; q < 6; ++q) f.vt[q] = ctx.field(W_HDB + q);
+    for (int q = 0; q < 3; ++q) f.dd[q] = ctx.field(W_DD + q);
+    f.gT = ctx.field(W_GT);
+    f.Mc = ctx.dfield(D_MC); f.Mu = ctx.dfield(D_MU); f.Ml = ctx.dfield(D_ML);
+    f.getot = ctx.field(W_GETOT); f.frhog = ctx.field(W_FRHOG);
+    run_staged(x, f);
+  }
+  {
+    EthTotPMP f{d, ctx.c, ctx.G, prog, eth, ctx.field(W_ETHTOT), ctx.field(W_PM)};
+    x.run_pts(f, d);
+  }
+}
+
+// the horizontal divergences of the small-step momentum dv (ViMainH) into W_HV
+template <class X>
+void small_hv(X& x, const Context& ctx, const real* prog, real* const dv[3]) {
+  ViMainH f;
+  f.d = ctx.d; f.c = ctx.c; f.G = ctx.G;
+  f.prog = prog;
+  for (int q = 0; q < 3; ++q) f.dv[q] = dv[q];
+  f.eth = ctx.field(W_ETH); f.ethtot = ctx.field(W_ETHTOT); f.gT = ctx.field(W_GT);
+  for (int q = 0; q < 3; ++q) f.hv[q] = ctx.field(W_HV + q);
+  run_staged(x, f);
+}
+
+// one small step (mod_vi.py:1283-1450): divergence damping of the split
+// momentum, path1, vi_main (horizontal, tridiagonal, mass/energy), halos
+template <class X>
+void small_step(X& x, const Context& ctx, const real* prog, const SmallBufs& s, double dts, double rweight) {
+  const Dims& d = ctx.d;
+  const long NF = d.NF;
+  real* gT = ctx.field(W_GT);
+  real* pm = ctx.field(W_PM);
+  real* ddv = ctx.field(W_DDV);
+  divdamp(x, ctx, s.ps[1], s.ps[2], s.ps[3], s.ps[4], ddv, ddv + NF, ddv + 2 * NF);
+  {
+    Path1P f;
+    f.d = d; f.c = ctx.c; f.G = ctx.G;
+    f.dt = (real)dts; f.rweight = (real)rweight;
+    f.prog = prog;
+    for (int q = 0; q < 6; ++q) f.ps[q] = s.ps[q];
+    f.gT = gT;
+    for (int q = 0; q < 3; ++q) f.dd[q] = ddv + q * NF;
+    for (int q = 0; q < 3; ++q) f.dv[q] = s.dv[q];
+    f.pm = pm;
+    run_staged(x, f);
+  }
+  halo(x, ctx, s.dv, 3);
+  small_hv(x, ctx, prog, s.dv);
+  real* hv[3] = {ctx.field(W_HV), ctx.field(W_HV + 1), ctx.field(W_HV + 2)};
+  {
+    // the z40 column (39 unknowns) keeps the sweep in registers
+    auto solve = [&](auto f) {
+      f.d = d; f.c = ctx.c; f.G = ctx.G;
+      f.dt = dts;
+      for (int q = 0; q < 6; ++q) f.ps[q] = s.ps[q];
+      f.gT = gT;
+      f.Mc = ctx.dfield(D_MC); f.Mu = ctx.dfield(D_MU); f.Ml = ctx.dfield(D_ML);
+      for (int q = 0; q < 3; ++q) f.hv[q] = hv[q];
+      f.U = ctx.dfield(D_U); f.Gm = ctx.dfield(D_GM);
+      f.out = s.we[1];
+      x.run(f, ncols(d));
+    };
+    if (d.kmax - d.kmin == 39) solve(ViMainV<39>{});
+    else solve(ViMainV<0>{});
+  }
+  {
+    ViMainPost f;
+    f.d = d; f.c = ctx.c; f.G = ctx.G;
+    f.dt = (real)dts; f.rweight = (real)rweight;
+    f.prog = prog;
+    for (int q = 0; q < 6; ++q) f.ps[q] = s.ps[q];
+    for (int q = 0; q < 3; ++q) f.dv[q] = s.dv[q];
+    f.ethtot = ctx.field(W_ETHTOT); f.gT = gT; f.getot = ctx.field(W_GETOT);
+    for (int q = 0; q < 3; ++q) f.hv[q] = hv[q];
+    for (int q = 0; q < 3; ++q) f.we[q] = s.we[q];
+    f.pm = pm;
+    x.run_pts(f, d);
+  }
+  halo(x, ctx, s.we, 3);
+}
+
+// PROG += PROG_split with the horizontal projection
+template <class X>
+void prog_update(X& x, const Context& ctx, real* prog, real* const ps[6]) {
+  ProgUpdateP f;
+  f.d = ctx.d; f.c = ctx.c; f.G = ctx.G;
+  f.prog = prog;
+  for (int q = 0; q < 6; ++q) f.ps[q] = ps[q];
+  x.run_pts(f, ctx.d);
+}
+
 template <class X>
 int dyn_step(X& x, const Context& ctx, real* prog, real* progq, int nq, double dtl, int sstep_max) {
   const Dims& d = ctx.d;
   const long NF = d.NF;
-  const long nc = ncols(d);
   const double dts = dtl / (double)sstep_max;
   const int nsub[3] = {sstep_max / 3, sstep_max / 2, sstep_max};
   if (nq < 1 || !prog || !progq) return NH_ERR_ARG;   // tracer 0 is qv (zero in a dry run)
-  const real* rhogq = progq;
   real* prog0 = ctx.field(W_PROG0);
-  real* diag = ctx.field(W_DIAG);
-  real* eth = ctx.field(W_ETH);
-  real* gadv = ctx.field(W_GADV);
-  real* gT = ctx.field(W_GT);
-  real* pm = ctx.field(W_PM);
-  real* ethtot = ctx.field(W_ETHTOT);
-  real* getot = ctx.field(W_GETOT);
-  real* frhog = ctx.field(W_FRHOG);
   x.d2d(prog0, prog, 6 * NF * sizeof(real));
   for (int nl = 0; nl < 3; ++nl) {
-    // diagnostics and boundary values
+    stage_large(x, ctx, prog, progq, dts);
+    // split state ps[0..5], path1 momentum dv[0..2], vi_main output we[0..2]
+    SmallBufs s;
+    for (int q = 0; q < 6; ++q) s.ps[q] = ctx.field(W_SPLIT + q);
+    for (int q = 0; q < 3; ++q) s.dv[q] = ctx.field(W_SPLIT + 6 + q);
+    for (int q = 0; q < 3; ++q) s.we[q] = ctx.field(W_SPLIT + 9 + q);
     {
-      PrePostP f{d, ctx.c, ctx.G, prog, rhogq, diag, eth};
-      x.run_pts(f, d);
-    }
-    run_staged(x, AdvMomP{d, ctx.c, ctx.G, prog, diag, gadv});
-    // hyperdiffusion: two passes with a halo fill after each
-    {
-      HDiffPassP<0> f;
-      f.d = d; f.c = ctx.c; f.G = ctx.G;
-      f.in[0] = diag + I_VX * NF; f.in[1] = diag + I_VY * NF; f.in[2] = diag + I_VZ * NF;
-      f.in[3] = diag + I_W * NF; f.in[4] = diag + I_TEM * NF; f.in[5] = prog;
-      for (int q = 0; q < 6; ++q) f.out[q] = ctx.field(W_HDA + q);
-      f.rhog = prog;
-      run_staged(x, f);
-      halo_block(x, ctx, ctx.field(W_HDA), 6);
-      HDiffPass2T f2;
-      f2.d = d; f2.c = ctx.c; f2.G = ctx.G; f2.tg = tile_geom(d, 12);
-      for (int q = 0; q < 6; ++q) {
-        f2.in[q] = ctx.field(W_HDA + q);
-        f2.out[q] = ctx.field(W_HDB + q);
-      }
-      f2.rhog = prog;
-      x.run_tiles(f2, f2.tg.nblocks, f2.tg.lds);
-      halo_block(x, ctx, ctx.field(W_HDB), 6);
-    }
-    // divergence damping of the large-step momentum
-    divdamp(x, ctx, prog + 1 * NF, prog + 2 * NF, prog + 3 * NF, prog + 4 * NF,
-            ctx.field(W_DD), ctx.field(W_DD + 1), ctx.field(W_DD + 2));
-    {
-      ViPath0P f;
-      f.d = d; f.c = ctx.c; f.G = ctx.G;
-      f.dt = dts;
-      f.prog = prog; f.diag = diag; f.eth = eth; f.gadv = gadv;
-      for (int q = 0; q < 6; ++q) f.vt[q] = ctx.field(W_HDB + q);
-      for (int q = 0; q < 3; ++q) f.dd[q] = ctx.field(W_DD + q);
-      f.gT = gT;
-      f.Mc = ctx.dfield(D_MC); f.Mu = ctx.dfield(D_MU); f.Ml = ctx.dfield(D_ML);
-      f.getot = getot; f.frhog = frhog;
-      run_staged(x, f);
-    }
-    {
-      EthTotPMP f{d, ctx.c, ctx.G, prog, eth, ethtot, pm};
-      x.run_pts(f, d);
-    }
-    // split state: ps[0..5], then path1 momentum dv[0..2] and vi_main output we[0..2]
-    real* ps[6];
-    real* dv[3];
-    real* we[3];
-    for (int q = 0; q < 6; ++q) ps[q] = ctx.field(W_SPLIT + q);
-    for (int q = 0; q < 3; ++q) dv[q] = ctx.field(W_SPLIT + 6 + q);
-    for (int q = 0; q < 3; ++q) we[q] = ctx.field(W_SPLIT + 9 + q);
-    {
-      SplitInit f{NF, prog, prog0, {ps[0], ps[1], ps[2], ps[3], ps[4], ps[5]}, nl == 0 ? 1 : 0};
+      SplitInit f{NF, prog, prog0, {s.ps[0], s.ps[1], s.ps[2], s.ps[3], s.ps[4], s.ps[5]}, nl == 0 ? 1 : 0};
       x.run(f, NF);
     }
     const double rweight = 1.0 / (double)nsub[nl];
     for (int it = 0; it < nsub[nl]; ++it) {
-      real* ddv = ctx.field(W_DDV);
-      divdamp(x, ctx, ps[1], ps[2], ps[3], ps[4], ddv, ddv + NF, ddv + 2 * NF);
-      {
-        Path1P f;
-        f.d = d; f.c = ctx.c; f.G = ctx.G;
-        f.dt = (real)dts; f.rweight = (real)rweight;
-        f.prog = prog;
-        for (int q = 0; q < 6; ++q) f.ps[q] = ps[q];
-        f.gT = gT;
-        for (int q = 0; q < 3; ++q) f.dd[q] = ddv + q * NF;
-        for (int q = 0; q < 3; ++q) f.dv[q] = dv[q];
-        f.pm = pm;
-        run_staged(x, f);
-      }
-      halo(x, ctx, dv, 3);
-      real* hv[3] = {ctx.field(W_HV), ctx.field(W_HV + 1), ctx.field(W_HV + 2)};
-      {
-        ViMainH f;
-        f.d = d; f.c = ctx.c; f.G = ctx.G;
-        f.prog = prog;
-        for (int q = 0; q < 3; ++q) f.dv[q] = dv[q];
-        f.eth = eth; f.ethtot = ethtot; f.gT = gT;
-        for (int q = 0; q < 3; ++q) f.hv[q] = hv[q];
-        run_staged(x, f);
-      }
-      {
-        // the z40 column (39 unknowns) keeps the sweep in registers
-        auto solve = [&](auto f) {
-          f.d = d; f.c = ctx.c; f.G = ctx.G;
-          f.dt = dts;
-          for (int q = 0; q < 6; ++q) f.ps[q] = ps[q];
-          f.gT = gT;
-          f.Mc = ctx.dfield(D_MC); f.Mu = ctx.dfield(D_MU); f.Ml = ctx.dfield(D_ML);
-          for (int q = 0; q < 3; ++q) f.hv[q] = hv[q];
-          f.U = ctx.dfield(D_U); f.Gm = ctx.dfield(D_GM);
-          f.out = we[1];
-          x.run(f, nc);
-        };
-        if (d.kmax - d.kmin == 39) solve(ViMainV<39>{});
-        else solve(ViMainV<0>{});
-      }
-      {
-        ViMainPost f;
-        f.d = d; f.c = ctx.c; f.G = ctx.G;
-        f.dt = (real)dts; f.rweight = (real)rweight;
-        f.prog = prog;
-        for (int q = 0; q < 6; ++q) f.ps[q] = ps[q];
-        for (int q = 0; q < 3; ++q) f.dv[q] = dv[q];
-        f.ethtot = ethtot; f.gT = gT; f.getot = getot;
-        for (int q = 0; q < 3; ++q) f.hv[q] = hv[q];
-        for (int q = 0; q < 3; ++q) f.we[q] = we[q];
-        f.pm = pm;
-        x.run_pts(f, d);
-      }
-      halo(x, ctx, we, 3);
+      small_step(x, ctx, prog, s, dts, rweight);
       // rotate buffers: the new split state is (we0, dv0, dv1, dv2, we1, we2)
-      real* old[6] = {ps[0], ps[1], ps[2], ps[3], ps[4], ps[5]};
-      ps[0] = we[0]; ps[1] = dv[0]; ps[2] = dv[1]; ps[3] = dv[2]; ps[4] = we[1]; ps[5] = we[2];
-      we[0] = old[0]; dv[0] = old[1]; dv[1] = old[2]; dv[2] = old[3]; we[1] = old[4]; we[2] = old[5];
+      real* old[6] = {s.ps[0], s.ps[1], s.ps[2], s.ps[3], s.ps[4], s.ps[5]};
+      s.ps[0] = s.we[0]; s.ps[1] = s.dv[0]; s.ps[2] = s.dv[1]; s.ps[3] = s.dv[2]; s.ps[4] = s.we[1];
+      s.ps[5] = s.we[2];
+      s.we[0] = old[0]; s.dv[0] = old[1]; s.dv[1] = old[2]; s.dv[2] = old[3]; s.we[1] = old[4]; s.we[2] = old[5];
     }
-    {
-      ProgUpdateP f;
-      f.d = d; f.c = ctx.c; f.G = ctx.G;
-      f.prog = prog;
-      for (int q = 0; q < 6; ++q) f.ps[q] = ps[q];
-      x.run_pts(f, d);
-    }
-    halo_block(x, ctx, pm, 5);
+    prog_update(x, ctx, prog, s.ps);
+    halo_block(x, ctx, ctx.field(W_PM), 5);
     if (nl < 2) halo_block(x, ctx, prog, 6);
     int rc = x.check("dyn_step stage");
     if (rc) return rc;

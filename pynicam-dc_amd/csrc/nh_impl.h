@@ -11,6 +11,7 @@
 
 #include "nh_variant.h"
 #include "nh_engine.h"
+#include "nh_adjstep.h"
 
 NH_BEGIN
 
@@ -55,7 +56,7 @@ void v_destroy(Context* c) {
   if (!c) return;
   NHX x(c->device, nullptr, &c->prof);
   x.sync();
-  void* bufs[7] = {c->ws, c->dws, c->geo_buf, c->geo64_buf, c->int_buf, c->halo_dst, c->halo_src};
+  void* bufs[9] = {c->ws, c->dws, c->geo_buf, c->geo64_buf, c->int_buf, c->halo_dst, c->halo_src, c->aws, c->adws};
   for (void* b : bufs)
     if (b) x.release(b);
   x.prof_release();
@@ -145,6 +146,46 @@ int v_budget(Context* c, const void* prog, const void* progq, double lhv, double
   NHX x(c->device, stream, &c->prof);
   int rc = budget(x, *c, (const real*)prog, (const real*)progq, lhv, out);
   if (rc == NH_ERR_ARG) return failv(err, rc, "grid too small for the partial-sum buffer");
+  if (rc) return failv(err, NH_ERR_DEVICE, x.message());
+  return 0;
+}
+
+int v_dyn_step_adj(Context* c, const void* prog, const void* progq, int nq, double dtl, int sstep_max, void* prog_b,
+                   void* stream, std::string* err) {
+  if (sstep_max < 3 || sstep_max > 64) return failv(err, NH_ERR_ARG, "sstep_max out of range for the adjoint");
+  if (!c->halo_adj_ok)
+    return failv(err, NH_ERR_STATE, "halo map has a destination listed twice or used as a source");
+  NHX x(c->device, stream, &c->prof);
+  if (!c->aws || c->adj_ck < sstep_max + 1) {
+    // first adjoint call (or more small steps than before): allocate the checkpoint workspace
+    x.sync();
+    if (c->aws) x.release(c->aws);
+    if (c->adws) x.release(c->adws);
+    c->aws = nullptr;
+    c->adws = nullptr;
+    c->adj_ck = 0;
+    const size_t bytes = (size_t)adj_fields(sstep_max + 1) * c->d.NF * sizeof(real);
+    const size_t dbytes = (size_t)AD_NFIELDS * c->d.NF * sizeof(double);
+    c->aws = (real*)x.alloc(bytes);
+    c->adws = (double*)x.alloc(dbytes);
+    if (!c->aws || !c->adws)
+      return failv(err, NH_ERR_ALLOC, "adjoint workspace allocation of " + std::to_string(bytes + dbytes) + " bytes");
+    x.zero(c->aws, bytes);
+    x.zero(c->adws, dbytes);
+    c->adj_ck = sstep_max + 1;
+  }
+  int rc = dyn_step_adj(x, *c, (const real*)prog, (const real*)progq, nq, dtl, sstep_max, (real*)prog_b);
+  if (rc == NH_ERR_ARG) return failv(err, rc, "invalid argument");
+  if (rc == NH_ERR_STATE) return failv(err, rc, "adjoint workspace missing");
+  if (rc) return failv(err, NH_ERR_DEVICE, x.message());
+  return 0;
+}
+
+int v_sfc_pressure(Context* c, const void* prog, const void* progq, double* ps, const double* ps_b, void* prog_b,
+                   void* stream, std::string* err) {
+  NHX x(c->device, stream, &c->prof);
+  int rc = sfc_pressure(x, *c, (const real*)prog, (const real*)progq, ps, ps_b, (real*)prog_b);
+  if (rc == NH_ERR_ARG) return failv(err, rc, "invalid argument (null pointer or fewer than three levels)");
   if (rc) return failv(err, NH_ERR_DEVICE, x.message());
   return 0;
 }

@@ -32,6 +32,10 @@
                      std::string* err);                                                                     \
   int v_budget(Context* c, const void* prog, const void* progq, double lhv, double* out, void* stream,      \
                std::string* err);                                                                           \
+  int v_dyn_step_adj(Context* c, const void* prog, const void* progq, int nq, double dtl, int sstep_max,      \
+                     void* prog_b, void* stream, std::string* err);                                         \
+  int v_sfc_pressure(Context* c, const void* prog, const void* progq, double* ps, const double* ps_b,       \
+                     void* prog_b, void* stream, std::string* err);                                         \
   int v_prof_enable(Context* c, bool on, std::string* err);                                                 \
   int v_prof_report(Context* c, int max_kernels, char* names, double* total_ms, int64_t* launches,          \
                     std::string* err);                                                                      \

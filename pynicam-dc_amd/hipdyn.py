@@ -55,6 +55,7 @@ class HipContext:
         lib = self.lib
         self.dims = native.NhDims(t.gall_1d, t.kall, t.kmin, t.kmax, t.lall)
         self.NF = int(lib.nh_field_size(C.byref(self.dims)))
+        self.ncol = t.lall * t.gall_1d * t.gall_1d + 12   # regional columns, then the pole columns
         if self.NF != layout.field_size(t.gall_1d, t.kall, t.lall):
             raise RuntimeError("field size mismatch between library and host layout")
         c = const
@@ -200,6 +201,39 @@ class HipDynamics:
         native.check(lib, self.ctx.fn("nh_dyn_step")(self.ctx.handle, C.c_void_p(self.prog.data_ptr()),
                                           C.c_void_p(self.progq.data_ptr()), self.progq.shape[0],
                                           self.dtl, self.sstep_max, self.ctx.stream()), "nh_dyn_step")
+
+    def dynamics_step_adj(self, prog, progq, prog_b):
+        """Transpose of one ``dynamics_step`` started from (prog, progq), applied in place to ``prog_b``.
+
+        On entry ``prog_b`` is the gradient of a scalar with respect to the state
+        after the step; on exit it is the gradient with respect to ``prog``.  The
+        tracer masses are held at ``progq`` (exact for zero tracers, the dry runs).
+        """
+        ctx = self.ctx
+        for a in (prog, progq, prog_b):
+            ctx._check_tensor(a)
+        native.check(ctx.lib, ctx.fn("nh_dyn_step_adj")(
+            ctx.handle, C.c_void_p(prog.data_ptr()), C.c_void_p(progq.data_ptr()), progq.shape[0], self.dtl,
+            self.sstep_max, C.c_void_p(prog_b.data_ptr()), ctx.stream()), "nh_dyn_step_adj")
+
+    def sfc_pressure(self, prog=None, progq=None):
+        """Surface pressure per column (BNDCND_pre_sfc, mod_bndcnd.py:521), FP64, shape (nreg + 12,)."""
+        ctx = self.ctx
+        prog = self.prog if prog is None else prog
+        progq = self.progq if progq is None else progq
+        ps = _torch().empty(ctx.ncol, dtype=_torch().float64, device=ctx.device)
+        native.check(ctx.lib, ctx.fn("nh_sfc_pressure")(ctx.handle, C.c_void_p(prog.data_ptr()),
+                                                        C.c_void_p(progq.data_ptr()), C.c_void_p(ps.data_ptr()),
+                                                        ctx.stream()), "nh_sfc_pressure")
+        return ps
+
+    def sfc_pressure_adj(self, prog, progq, ps_b, prog_b):
+        """Add the transpose of ``sfc_pressure`` at (prog, progq) applied to ``ps_b`` into ``prog_b``."""
+        ctx = self.ctx
+        ps_b = ps_b.to(dtype=_torch().float64).contiguous()
+        native.check(ctx.lib, ctx.fn("nh_sfc_pressure_adj")(
+            ctx.handle, C.c_void_p(prog.data_ptr()), C.c_void_p(progq.data_ptr()), C.c_void_p(ps_b.data_ptr()),
+            C.c_void_p(prog_b.data_ptr()), ctx.stream()), "nh_sfc_pressure_adj")
 
     def budget(self, lhv: float | None = None) -> dict:
         """Global energies and masses of the current state (Embudget._store_var, mod_embudget.py:82-139).

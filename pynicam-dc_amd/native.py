@@ -66,13 +66,17 @@ EXPORTS = {
     "nh_create_ex": (C.c_int, [C.POINTER(NhDims), C.POINTER(NhConsts), C.c_int, C.c_int32,
                                C.POINTER(C.c_void_p)]),
     "nh_precision": (C.c_int32, [C.c_void_p]),
+    "nh_dyn_step_adj": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_int32,
+                                  C.c_void_p, C.c_void_p]),
+    "nh_sfc_pressure": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "nh_sfc_pressure_adj": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "nh_prof_enable": (C.c_int, [C.c_void_p, C.c_int32]),
     "nh_prof_report": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.POINTER(C.c_double),
                                  C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
 }
 # the _f32 forms of the field entry points take the same arguments (float fields)
 for _name in ("nh_halo_fill", "nh_laplacian", "nh_divergence", "nh_gradient", "nh_dyn_step", "nh_tracer_step",
-              "nh_trcadv_winds", "nh_budget"):
+              "nh_trcadv_winds", "nh_budget", "nh_dyn_step_adj", "nh_sfc_pressure", "nh_sfc_pressure_adj"):
     EXPORTS[_name + "_f32"] = EXPORTS[_name]
 
 PREC_FP64, PREC_MIXED = 0, 1
