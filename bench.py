@@ -123,7 +123,7 @@ def hbm_traffic(glevel: int, precision: str, timeout: float = 300.0):
             cmd = ["rocprofv3", "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "pmc", "--",
                    sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0",
                    "--glevel", str(glevel), "--precision", precision, "--no-cpu-baseline", "--no-traffic",
-                   "--copy-probe"]
+                   "--no-adjoint", "--copy-probe"]
             print(f"bench: counter pass {ctr}", file=sys.stderr, flush=True)
             try:
                 r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), capture_output=True,
@@ -213,6 +213,87 @@ def run_form(hipdyn, case, precision: str, steps: int, warmup: int):
     return ms, prof, st, budget
 
 
+def adjoint_leg(hipdyn, precision: str, nsteps: int = 18, reps: int = 2) -> dict:
+    """Initial-state sensitivity at gl06 z40 (dt 600 s): J = sum over the 30-50N,
+    0-20E box of surface pressure squared after ``nsteps`` steps, its gradient
+    through the adjoint window (per-step checkpoints, recompute inside each
+    adjoint step).  Times the plain forward steps, the checkpointed forward
+    window and the backward pass (best of ``reps``), checks the gradient by a
+    central finite difference along one seeded direction, and reports the
+    device memory in use at the end of the backward pass (hipMemGetInfo)."""
+    import numpy as np
+    import torch
+    adj = pkg("adjoint")
+    case = pkg("setup").build_case(glevel=6, vlayer=40, dtl=600.0, seed=0)
+    ctx = hipdyn.HipContext(case.geo, case.const, device="cuda:0", precision=precision)
+    try:
+        dyn = hipdyn.HipDynamics(ctx, case.state)
+        m = adj.box_mask(dyn)
+        x0 = dyn.prog.clone()
+        q0 = dyn.progq.clone()
+
+        def plain():
+            dyn.prog.copy_(x0)
+            dyn.progq.copy_(q0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(nsteps):
+                dyn.dynamics_step()
+            torch.cuda.synchronize()
+            return time.perf_counter() - t0
+
+        def window():
+            x = x0.clone().requires_grad_(True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ps = adj.sfc_pressure(dyn, adj.forecast(dyn, x, nsteps))
+            J = (m * ps * ps).sum()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            J.backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            return float(J.detach()), x.grad.detach(), t1 - t0, t2 - t1
+
+        plain()
+        window()
+        t_plain = min(plain() for _ in range(reps))
+        runs = [window() for _ in range(reps)]
+        Jv, grad = runs[-1][0], runs[-1][1]
+        t_fwd = min(r[2] for r in runs)
+        t_bwd = min(r[3] for r in runs)
+        free, total = torch.cuda.mem_get_info()
+        # central difference along a seeded direction scaled per variable
+        rng = np.random.default_rng(7)
+        sc = x0.abs().amax(dim=1, keepdim=True).clone()
+        sc[3] = sc[1]
+        sc[4] = 0.01 * sc[1]
+        v = torch.from_numpy(rng.standard_normal(tuple(x0.shape))).to(x0.device, x0.dtype) * sc
+        eps = 1e-5 if precision == "float64" else 1e-3
+
+        def J_of(x):
+            dyn.prog.copy_(x)
+            dyn.progq.copy_(q0)
+            for _ in range(nsteps):
+                dyn.dynamics_step()
+            ps = dyn.sfc_pressure()
+            return float((m * ps * ps).sum())
+
+        fd = (J_of(x0 + eps * v) - J_of(x0 - eps * v)) / (2 * eps)
+        ad = float((grad.double() * v.double()).sum())
+    finally:
+        ctx.close()
+    res = {"workload": f"gl06 z40, dt 600 s, {nsteps} steps, J = sum over 30-50N 0-20E of ps^2",
+           "forward_ms_per_step": 1000.0 * t_plain / nsteps,
+           "window_forward_ms_per_step": 1000.0 * t_fwd / nsteps,
+           "adjoint_ms_per_step": 1000.0 * t_bwd / nsteps,
+           "adjoint_over_forward": t_bwd / t_plain,
+           "J": Jv, "fd_rel_err": abs(fd - ad) / abs(fd), "fd_eps": eps,
+           "device_mem_in_use_GB": (total - free) / 1e9}
+    print(f"bench: adjoint {precision}: " + json.dumps(res), file=sys.stderr, flush=True)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,6 +305,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure HBM traffic per kernel")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint window at gl06 z40")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the run, copy 512 MiB device-to-device once (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -305,6 +387,10 @@ def main():
         err = np.abs(a - b).reshape(-1, 6).max(axis=0) / scale
         out["mixed_vs_fp64_max_rel_err"] = dict(zip(("rhog", "rhogvx", "rhogvy", "rhogvz", "rhogw", "rhoge"),
                                                     [float(e) for e in err]))
+    if not args.no_adjoint:
+        adj = {FORMS[f][0]: adjoint_leg(hipdyn, f) for f in forms}
+        out["adjoint"] = adj
+        out["adjoint_over_forward"] = adj[FORMS[main_form][0]]["adjoint_over_forward"]
     if args.copy_probe:
         a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda:0")
         b = torch.empty_like(a)

@@ -51,3 +51,32 @@ def store_var(ctx, PROG, PROGq, lhv):
     mdry = gsum(ctx, rho * qd)
     mvap = gsum(ctx, rho * q)
     return {"pot": pot, "int": eint, "kin": ekin, "tot": pot + eint + ekin, "dry": mdry, "vap": mvap}
+
+
+def sfc_pressure(ctx, PROG, PROGq):
+    """Surface density and pressure per column: Bndc.BNDCND_pre_sfc
+    (pynicamdc/nhm/share/mod_bndcnd.py:521-555) with a flat surface z_srf = 0.
+
+    rho_srf is the quadratic Lagrange polynomial through (z, rho) at the lowest
+    three full levels, evaluated at z_srf; pre_srf = pre(kmin) +
+    0.5 (rho_srf + rho(kmin)) GRAV (z(kmin) - z_srf).  rho and pre are the
+    diagnostic density and pressure of ``compute_diagnostics``.  Returns
+    (rho_srf, pre_srf), each (i, j, l).
+    """
+    c, kmin = ctx.c, ctx.kmin
+    GSGAM2 = ctx.m["GSGAM2"]
+    rhog = PROG[..., od.I_RHOG]
+    rho = rhog / GSGAM2
+    ein = PROG[..., od.I_RHOGE] / rhog
+    q = PROGq[..., 0] / rhog
+    qd = 1.0 - q
+    tem = ein / (q * c.CVW[0] + qd * c.CVdry)
+    pre = rho * tem * (qd * c.Rdry + q * c.Rvap)
+    z = ctx.geo.vgrid.gz
+    z1, z2, z3, zs = z[kmin], z[kmin + 1], z[kmin + 2], 0.0
+    r1, r2, r3 = rho[:, :, kmin], rho[:, :, kmin + 1], rho[:, :, kmin + 2]
+    rho_srf = (((zs - z2) * (zs - z3)) / ((z1 - z2) * (z1 - z3)) * r1
+               + ((zs - z1) * (zs - z3)) / ((z2 - z1) * (z2 - z3)) * r2
+               + ((zs - z1) * (zs - z2)) / ((z3 - z1) * (z3 - z2)) * r3)
+    pre_srf = pre[:, :, kmin] + 0.5 * (rho_srf + r1) * c.GRAV * (z1 - zs)
+    return rho_srf, pre_srf

@@ -1,0 +1,138 @@
+"""Initial-state sensitivity of a forecast functional: the adjoint of the
+dynamics step behind ``torch.autograd.Function``.
+
+The reference has no adjoint; the forward operator differentiated here is
+``Dyn.dynamics_step`` (nhm/dynamics/mod_dynamics.py:1725), run through
+``HipDynamics.dynamics_step``.  ``forecast(dyn, prog, nsteps)`` advances a
+differentiable copy of the state; its backward pass calls the library's
+transposed step (``nh_dyn_step_adj``) once per step, newest first, from the
+checkpoint of that step's input state (per-step checkpointing; each adjoint
+call recomputes its step inside, with stage and small-step checkpoints).
+``sfc_pressure(dyn, prog)`` is the surface-pressure diagnostic
+(Bndc.BNDCND_pre_sfc, nhm/share/mod_bndcnd.py:521) with its transpose, and
+``box_mask`` selects the cells of a latitude/longitude box, so that
+
+    J = (box_mask(...) * sfc_pressure(dyn, forecast(dyn, prog0, 18)) ** 2).sum()
+    J.backward()
+
+gives dJ/dprog0.  Tracer masses are carried along the trajectory but held
+fixed in the transpose: the gradient is exact when they are zero (dry runs).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import layout
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _check_state(dyn, prog):
+    ctx = dyn.ctx
+    if prog.dtype != ctx.dtype or prog.device != ctx.device or tuple(prog.shape) != (6, ctx.NF):
+        raise ValueError(f"expected a ({6}, {ctx.NF}) {ctx.dtype} tensor on {ctx.device}")
+
+
+def _make_functions():
+    torch = _torch()
+
+    class Forecast(torch.autograd.Function):
+        """prog -> state after ``nsteps`` dynamics steps (tracers advance alongside)."""
+
+        @staticmethod
+        def forward(fctx, prog, dyn, nsteps):
+            _check_state(dyn, prog)
+            saved_prog, saved_q = dyn.prog, dyn.progq
+            progq = dyn.progq.clone()
+            checkpoints = []
+            work = prog.detach().clone().contiguous()
+            dyn.prog, dyn.progq = work, progq
+            try:
+                for _ in range(nsteps):
+                    checkpoints.append((work.clone(), progq.clone()))
+                    dyn.dynamics_step()
+            finally:
+                dyn.prog, dyn.progq = saved_prog, saved_q
+            fctx.dyn = dyn
+            fctx.checkpoints = checkpoints
+            fctx.progq_out = progq
+            return work
+
+        @staticmethod
+        def backward(fctx, grad):
+            dyn = fctx.dyn
+            gb = grad.detach().to(dtype=dyn.ctx.dtype).contiguous().clone()
+            for p, q in reversed(fctx.checkpoints):
+                dyn.dynamics_step_adj(p, q, gb)
+            fctx.checkpoints = None
+            return gb, None, None
+
+    class SfcPressure(torch.autograd.Function):
+        """prog -> surface pressure per column (FP64, nreg + 12)."""
+
+        @staticmethod
+        def forward(fctx, prog, dyn, progq):
+            _check_state(dyn, prog)
+            prog = prog.detach().contiguous()
+            fctx.dyn = dyn
+            fctx.save_for_backward(prog, progq)
+            return dyn.sfc_pressure(prog, progq)
+
+        @staticmethod
+        def backward(fctx, grad):
+            prog, progq = fctx.saved_tensors
+            gb = torch.zeros_like(prog)
+            fctx.dyn.sfc_pressure_adj(prog, progq, grad.detach(), gb)
+            return gb, None, None
+
+    return Forecast, SfcPressure
+
+
+_FN = {}
+
+
+def _fn(name):
+    if not _FN:
+        _FN["forecast"], _FN["sfc"] = _make_functions()
+    return _FN[name]
+
+
+def forecast(dyn, prog, nsteps: int):
+    """State after ``nsteps`` steps of ``dyn`` from ``prog`` (``dyn.progq`` as the initial tracers).
+
+    Differentiable with respect to ``prog``; ``dyn``'s own state is left unchanged.
+    """
+    return _fn("forecast").apply(prog, dyn, int(nsteps))
+
+
+def sfc_pressure(dyn, prog, progq=None):
+    """Surface pressure per column of ``prog`` (differentiable with respect to ``prog``)."""
+    return _fn("sfc").apply(prog, dyn, dyn.progq if progq is None else progq)
+
+
+def box_mask(dyn, lat_deg=(30.0, 50.0), lon_deg=(0.0, 20.0)):
+    """1.0 on the cells (interior regional points and pole centres) inside a
+    latitude/longitude box, 0.0 elsewhere; FP64 per column of ``sfc_pressure``."""
+    torch = _torch()
+    ctx = dyn.ctx
+    geo = ctx.geo
+    t = geo.topo
+    g = t.gall_1d
+    lat = np.degrees(layout._soa(geo.lat)[0])
+    lon = np.degrees(layout._soa(geo.lon)[0])
+    lon = np.where(lon < -180.0, lon + 360.0, np.where(lon >= 180.0, lon - 360.0, lon))
+    idx = np.arange(lat.size)
+    i = idx % g
+    j = (idx // g) % g
+    inner = (i >= 1) & (i <= g - 2) & (j >= 1) & (j <= g - 2)
+    inbox = (lat >= lat_deg[0]) & (lat <= lat_deg[1]) & (lon >= lon_deg[0]) & (lon <= lon_deg[1])
+    m = np.zeros(ctx.ncol)
+    m[:lat.size] = (inner & inbox).astype(np.float64)
+    lat_pl = np.degrees(np.asarray(geo.lat_pl).reshape(6, 2))
+    for lp in range(2):
+        if lat_deg[0] <= lat_pl[0, lp] <= lat_deg[1]:
+            m[lat.size + lp * 6] = 1.0
+    return torch.from_numpy(m).to(ctx.device)

@@ -1,0 +1,248 @@
+"""Adjoint of the dynamics step: finite-difference checks, the mixed form
+against FP64, the surface-pressure diagnostic, argument errors.
+
+The reference has no adjoint and no gradient test: what pins the transposed
+kernels is the derivative itself, i.e. central finite differences of the
+forward step (the kernel source of the product, host build on the CPU, the
+gfx950 build on the GPU) along seeded random directions.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import interior, mod
+
+hipdyn = mod("hipdyn")
+adjoint = mod("adjoint")
+native = mod("native")
+
+
+def _scales(prog):
+    """Per-variable direction scales: the field peak, the horizontal-momentum peak for the
+    (initially zero) vertical components."""
+    s = prog.abs().amax(dim=1, keepdim=True).clone()
+    s[3] = s[1]
+    s[4] = 0.01 * s[1]
+    return s
+
+
+def _fd_check(dyn, nsteps, directions, eps, seed=1, hold_q=False):
+    """(fd, adjoint) pairs of J = sum(w ps^2) after nsteps along seeded random directions."""
+    import torch
+    rng = np.random.default_rng(seed)
+    w = torch.from_numpy(rng.random(dyn.ctx.ncol)).to(dyn.ctx.device)
+    x0, q0 = dyn.prog.clone(), dyn.progq.clone()
+
+    def J(x):
+        dyn.prog.copy_(x)
+        dyn.progq.copy_(q0)
+        for _ in range(nsteps):
+            dyn.dynamics_step()
+        ps = dyn.sfc_pressure(progq=q0 if hold_q else None)
+        return float((w * ps * ps).sum())
+
+    dyn.prog.copy_(x0)
+    dyn.progq.copy_(q0)
+    traj = []
+    for _ in range(nsteps):
+        traj.append((dyn.prog.clone(), dyn.progq.clone()))
+        dyn.dynamics_step()
+    qJ = q0 if hold_q else dyn.progq
+    ps = dyn.sfc_pressure(progq=qJ)
+    gb = torch.zeros_like(dyn.prog)
+    dyn.sfc_pressure_adj(dyn.prog, qJ, 2 * w * ps, gb)
+    for p, q in reversed(traj):
+        dyn.dynamics_step_adj(p, q, gb)
+    sc = _scales(x0)
+    out = []
+    for mask in directions:
+        v = torch.from_numpy(rng.standard_normal(tuple(x0.shape))).to(x0.dtype).to(x0.device) * sc
+        if mask is not None:
+            m = torch.zeros((6, 1), dtype=x0.dtype, device=x0.device)
+            m[mask] = 1
+            v = v * m
+        fd = (J(x0 + eps * v) - J(x0 - eps * v)) / (2 * eps)
+        out.append((fd, float((gb.double() * v.double()).sum())))
+    dyn.prog.copy_(x0)
+    dyn.progq.copy_(q0)
+    return out
+
+
+# ----------------------------------------------------------------------------
+# host build (CPU)
+# ----------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def emu64(case_gl04, hostemu_lib):
+    ctx = hipdyn.HipContext(case_gl04.geo, case_gl04.const, device="cpu", lib_path=hostemu_lib)
+    yield ctx
+    ctx.close()
+
+
+def test_host_adjoint_matches_finite_differences_per_variable(case_gl04, emu64):
+    """One step, FP64: each prognostic variable alone, relative agreement 1e-6
+    (the finite differences themselves are good to about 1e-9 at this step size)."""
+    dyn = hipdyn.HipDynamics(emu64, case_gl04.state)
+    for q, (fd, ad) in enumerate(_fd_check(dyn, 1, list(range(6)), eps=1e-4)):
+        assert abs(fd) > 0 and abs(fd - ad) <= 1e-6 * abs(fd), (q, fd, ad)
+
+
+def test_host_adjoint_three_steps_with_strong_damping(case_gl04, hostemu_lib):
+    """Three steps with hyperdiffusion and divergence damping 300x stronger, so that their
+    transposes carry a visible share of the gradient."""
+    import copy
+    const = copy.copy(case_gl04.const)
+    const.GAMMA_H = 3.0e18
+    const.ALPHA_D = 3.0e18
+    ctx = hipdyn.HipContext(case_gl04.geo, const, device="cpu", lib_path=hostemu_lib)
+    try:
+        dyn = hipdyn.HipDynamics(ctx, case_gl04.state)
+        for fd, ad in _fd_check(dyn, 3, [None, None], eps=1e-5, seed=2):
+            assert abs(fd - ad) <= 1e-6 * abs(fd), (fd, ad)
+    finally:
+        ctx.close()
+
+
+def test_host_adjoint_moist_thermodynamics(case_gl04, emu64):
+    """Nonzero qv enters the step through q = rhogq / rhog (cv, gas constant); with the
+    tracer held at its input (as the transpose holds it) the derivative is exact."""
+    state = {k: v.copy() for k, v in case_gl04.state.items()}
+    rng = np.random.default_rng(5)
+    state["PROGq"][..., 0] = state["PROG"][..., 0] * 0.01 * rng.random(state["PROGq"].shape[:-1])
+    state["PROGq_pl"][..., 0] = state["PROG_pl"][..., 0] * 0.01
+    dyn = hipdyn.HipDynamics(emu64, state)
+    for fd, ad in _fd_check(dyn, 1, [None, 0, 5], eps=1e-5, seed=3, hold_q=True):
+        assert abs(fd - ad) <= 1e-6 * abs(fd), (fd, ad)
+
+
+def test_host_autograd_window_mixed_matches_fp64(case_gl04, hostemu_lib):
+    """J = sum over the 30-50N, 0-20E box of ps^2 after three steps, through
+    torch.autograd: the mixed form's gradient against the FP64 form's."""
+    import torch
+    grads = {}
+    for prec in ("float64", "mixed"):
+        ctx = hipdyn.HipContext(case_gl04.geo, case_gl04.const, device="cpu", lib_path=hostemu_lib,
+                                precision=prec)
+        try:
+            dyn = hipdyn.HipDynamics(ctx, case_gl04.state)
+            m = adjoint.box_mask(dyn)
+            assert m.sum() > 0
+            x = dyn.prog.clone().requires_grad_(True)
+            ps = adjoint.sfc_pressure(dyn, adjoint.forecast(dyn, x, 3))
+            (m * ps * ps).sum().backward()
+            grads[prec] = x.grad.double()
+        finally:
+            ctx.close()
+    g64, g32 = grads["float64"], grads["mixed"]
+    for q in range(6):
+        sc = g64[q].abs().max()
+        assert sc > 0 and (g64[q] - g32[q]).abs().max() <= 1e-4 * sc, q
+
+
+def test_host_autograd_window_equals_direct_adjoint(case_gl04, emu64):
+    import torch
+    dyn = hipdyn.HipDynamics(emu64, case_gl04.state)
+    m = adjoint.box_mask(dyn)
+    x = dyn.prog.clone().requires_grad_(True)
+    ps = adjoint.sfc_pressure(dyn, adjoint.forecast(dyn, x, 2))
+    (m * ps * ps).sum().backward()
+    # the same by hand
+    traj = []
+    for _ in range(2):
+        traj.append((dyn.prog.clone(), dyn.progq.clone()))
+        dyn.dynamics_step()
+    ps2 = dyn.sfc_pressure()
+    gb = torch.zeros_like(dyn.prog)
+    dyn.sfc_pressure_adj(dyn.prog, dyn.progq, 2 * m * ps2, gb)
+    for p, q in reversed(traj):
+        dyn.dynamics_step_adj(p, q, gb)
+    assert torch.equal(ps.detach(), ps2)
+    assert torch.allclose(x.grad, gb, rtol=0, atol=1e-12 * float(gb.abs().max()))
+
+
+def test_host_sfc_pressure_matches_oracle(case_gl04, state_gl04_tracers, emu64):
+    from oracle import budget as ob
+    from oracle import dynamics as od
+    c = case_gl04
+    t = c.topo
+    dyn = hipdyn.HipDynamics(emu64, state_gl04_tracers)
+    ps = dyn.sfc_pressure().numpy()
+    octx = od.OracleContext(c.geo, c.const, None)
+    _, ref = ob.sfc_pressure(octx, state_gl04_tracers["PROG"], state_gl04_tracers["PROGq"])
+    g = t.gall_1d
+    got = np.transpose(ps[:t.lall * g * g].reshape(t.lall, g, g), (2, 1, 0))
+    s = slice(1, g - 1)
+    assert np.max(np.abs(got[s, s] - ref[s, s])) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_adjoint_argument_errors(emu64):
+    lib = emu64.lib
+    prog = emu64.empty(6)
+    q = emu64.empty(1)
+    p = [C.c_void_p(a.data_ptr()) for a in (prog, q)]
+    rc = lib.nh_dyn_step_adj(emu64.handle, p[0], p[1], 1, 1200.0, 6, C.c_void_p(None), C.c_void_p(None))
+    assert rc == -1 and b"nh_dyn_step_adj" in lib.nh_last_error()
+    rc = lib.nh_dyn_step_adj(emu64.handle, p[0], p[1], 1, 1200.0, 4, p[0], C.c_void_p(None))
+    assert rc == -1
+    rc = lib.nh_dyn_step_adj_f32(emu64.handle, p[0], p[1], 1, 1200.0, 6, p[0], C.c_void_p(None))
+    assert rc == -2 and b"without _f32" in lib.nh_last_error()
+    rc = lib.nh_sfc_pressure_adj(emu64.handle, p[0], p[1], C.c_void_p(None), p[0], C.c_void_p(None))
+    assert rc == -1
+
+
+def test_adjoint_refuses_a_halo_map_it_cannot_transpose(case_gl04, hostemu_lib):
+    """A destination that is also a source would make the one-pass scatter order-dependent."""
+    ctx = hipdyn.HipContext(case_gl04.geo, case_gl04.const, device="cpu", lib_path=hostemu_lib)
+    try:
+        lib = ctx.lib
+        dst = np.array([0, 1], dtype=np.int32)
+        src = np.array([1, 2], dtype=np.int32)
+        ip = C.POINTER(C.c_int32)
+        native.check(lib, lib.nh_upload_halo_map(ctx.handle, dst.ctypes.data_as(ip), src.ctypes.data_as(ip), 2))
+        prog, q = ctx.empty(6).zero_(), ctx.empty(1).zero_()
+        rc = lib.nh_dyn_step_adj(ctx.handle, C.c_void_p(prog.data_ptr()), C.c_void_p(q.data_ptr()), 1, 1200.0,
+                                 6, C.c_void_p(prog.data_ptr()), C.c_void_p(None))
+        assert rc == -2 and b"halo map" in lib.nh_last_error()
+    finally:
+        ctx.close()
+
+
+# ----------------------------------------------------------------------------
+# gfx950 build
+# ----------------------------------------------------------------------------
+@pytest.mark.gpu
+def test_gpu_adjoint_matches_finite_differences(case_gl05):
+    """gl05 z40, two steps, FP64: J = sum(w ps^2) along three random directions over all
+    six prognostic variables, relative agreement 1e-6."""
+    print("gpu adjoint fd: gl05 z40 float64, 2 steps", flush=True)
+    ctx = hipdyn.HipContext(case_gl05.geo, case_gl05.const, device="cuda:0")
+    try:
+        dyn = hipdyn.HipDynamics(ctx, case_gl05.state)
+        for fd, ad in _fd_check(dyn, 2, [None, None, None], eps=1e-5, seed=4):
+            assert abs(fd - ad) <= 1e-6 * abs(fd), (fd, ad)
+    finally:
+        ctx.close()
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_mixed_matches_fp64(case_gl05):
+    """gl05 z40, three steps: the box functional's gradient, mixed form against FP64."""
+    import torch
+    print("gpu adjoint mixed vs fp64: gl05 z40, 3 steps", flush=True)
+    grads = {}
+    for prec in ("float64", "mixed"):
+        ctx = hipdyn.HipContext(case_gl05.geo, case_gl05.const, device="cuda:0", precision=prec)
+        try:
+            dyn = hipdyn.HipDynamics(ctx, case_gl05.state)
+            m = adjoint.box_mask(dyn)
+            x = dyn.prog.clone().requires_grad_(True)
+            ps = adjoint.sfc_pressure(dyn, adjoint.forecast(dyn, x, 3))
+            (m * ps * ps).sum().backward()
+            torch.cuda.synchronize()
+            grads[prec] = x.grad.double().cpu()
+        finally:
+            ctx.close()
+    g64, g32 = grads["float64"], grads["mixed"]
+    for q in range(6):
+        sc = g64[q].abs().max()
+        assert sc > 0 and (g64[q] - g32[q]).abs().max() <= 1e-3 * sc, q
