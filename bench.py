@@ -213,14 +213,15 @@ def run_form(hipdyn, case, precision: str, steps: int, warmup: int):
     return ms, prof, st, budget
 
 
-def adjoint_leg(hipdyn, precision: str, nsteps: int = 18, reps: int = 2) -> dict:
+def adjoint_leg(hipdyn, precision: str, nsteps: int = 18, reps: int = 2, ref=None) -> dict:
     """Initial-state sensitivity at gl06 z40 (dt 600 s): J = sum over the 30-50N,
     0-20E box of surface pressure squared after ``nsteps`` steps, its gradient
     through the adjoint window (per-step checkpoints, recompute inside each
     adjoint step).  Times the plain forward steps, the checkpointed forward
-    window and the backward pass (best of ``reps``), checks the gradient by a
-    central finite difference along one seeded direction, and reports the
-    device memory in use at the end of the backward pass (hipMemGetInfo)."""
+    window and the backward pass (best of ``reps``), checks the gradient of the
+    FP64 form by a central finite difference along one seeded direction (the
+    mixed form's against the FP64 gradient ``ref``), and reports the device
+    memory in use at the end of the backward pass (hipMemGetInfo)."""
     import numpy as np
     import torch
     adj = pkg("adjoint")
@@ -269,18 +270,30 @@ def adjoint_leg(hipdyn, precision: str, nsteps: int = 18, reps: int = 2) -> dict
         sc[3] = sc[1]
         sc[4] = 0.01 * sc[1]
         v = torch.from_numpy(rng.standard_normal(tuple(x0.shape))).to(x0.device, x0.dtype) * sc
-        eps = 1e-5 if precision == "float64" else 1e-3
-
-        def J_of(x):
-            dyn.prog.copy_(x)
-            dyn.progq.copy_(q0)
-            for _ in range(nsteps):
-                dyn.dynamics_step()
-            ps = dyn.sfc_pressure()
-            return float((m * ps * ps).sum())
-
-        fd = (J_of(x0 + eps * v) - J_of(x0 - eps * v)) / (2 * eps)
         ad = float((grad.double() * v.double()).sum())
+        check = {}
+        if ref is None:
+            # the step size balances truncation (eps^2) against round-off
+            eps = 1e-6
+
+            def J_of(x):
+                dyn.prog.copy_(x)
+                dyn.progq.copy_(q0)
+                for _ in range(nsteps):
+                    dyn.dynamics_step()
+                ps = dyn.sfc_pressure()
+                return float((m * ps * ps).sum())
+
+            fd = (J_of(x0 + eps * v) - J_of(x0 - eps * v)) / (2 * eps)
+            check = {"fd_rel_err": abs(fd - ad) / abs(fd), "fd_eps": eps}
+        else:
+            g64 = ref["grad"]
+            a64 = float((g64 * v.double()).sum())
+            gd = grad.double()
+            per = (gd - g64).abs().amax(dim=1) / g64.abs().amax(dim=1)
+            check = {"vs_fp64_dir_rel_err": abs(ad - a64) / abs(a64),
+                     "vs_fp64_max_rel_err": dict(zip(("rhog", "rhogvx", "rhogvy", "rhogvz", "rhogw", "rhoge"),
+                                                     [float(e) for e in per]))}
     finally:
         ctx.close()
     res = {"workload": f"gl06 z40, dt 600 s, {nsteps} steps, J = sum over 30-50N 0-20E of ps^2",
@@ -288,9 +301,10 @@ def adjoint_leg(hipdyn, precision: str, nsteps: int = 18, reps: int = 2) -> dict
            "window_forward_ms_per_step": 1000.0 * t_fwd / nsteps,
            "adjoint_ms_per_step": 1000.0 * t_bwd / nsteps,
            "adjoint_over_forward": t_bwd / t_plain,
-           "J": Jv, "fd_rel_err": abs(fd - ad) / abs(fd), "fd_eps": eps,
+           "J": Jv, **check,
            "device_mem_in_use_GB": (total - free) / 1e9}
     print(f"bench: adjoint {precision}: " + json.dumps(res), file=sys.stderr, flush=True)
+    res["grad"] = grad.double()
     return res
 
 
@@ -388,7 +402,15 @@ def main():
         out["mixed_vs_fp64_max_rel_err"] = dict(zip(("rhog", "rhogvx", "rhogvy", "rhogvz", "rhogw", "rhoge"),
                                                     [float(e) for e in err]))
     if not args.no_adjoint:
-        adj = {FORMS[f][0]: adjoint_leg(hipdyn, f) for f in forms}
+        adj = {}
+        ref = None
+        for f in forms:
+            r = adjoint_leg(hipdyn, f, ref=ref)
+            if f == "float64":
+                ref = r
+            adj[FORMS[f][0]] = r
+        for r in adj.values():
+            r.pop("grad", None)
         out["adjoint"] = adj
         out["adjoint_over_forward"] = adj[FORMS[main_form][0]]["adjoint_over_forward"]
     if args.copy_probe:

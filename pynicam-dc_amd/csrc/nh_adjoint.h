@@ -161,23 +161,8 @@ struct ProgUpdateA {
 };
 
 // ----------------------------------------------------------------------------
-// divergence damping (transposes of DivDamp1P / DivDamp2P and their tile forms)
+// divergence damping: helpers of the transposes (the kernels are below)
 // ----------------------------------------------------------------------------
-// transpose of tri_div: sbar -> vb[dd] at the triangle's three corners
-NH_HD inline void tri_div_t(const Geo& G, long nreg, long cj, int T, real* const* vb, long a, int g, real sgn,
-                            real sbar) {
-  const long o1 = (T == 0) ? 1 : 1 + g;
-  const long o2 = (T == 0) ? 1 + g : g;
-  for (int dd = 0; dd < 3; ++dd) {
-    const real c0 = rc(G.cintp, nreg, (dd * 3 + 0) * 2 + T, cj);
-    const real c1 = rc(G.cintp, nreg, (dd * 3 + 1) * 2 + T, cj);
-    const real c2 = rc(G.cintp, nreg, (dd * 3 + 2) * 2 + T, cj);
-    aadd(&vb[dd][a], c0 * sgn * sbar);
-    aadd(&vb[dd][a + o1], c1 * sgn * sbar);
-    aadd(&vb[dd][a + o2], c2 * sgn * sbar);
-  }
-}
-
 // transpose of hex_sum over the three directions: o[dd] -> Tb[6]
 NH_HD inline void hex_sum_t(const Geo& G, long nreg, long ci, const real o[3], real Tb[6]) {
   for (int n = 0; n < 6; ++n) Tb[n] = real(0.0);
@@ -205,109 +190,6 @@ NH_HD inline void pole_ring_t(const Geo& G, int lp, const real o[3], real sb[5])
     sb[v - 1] += s;
   }
 }
-
-struct DivDamp1A {
-  static constexpr const char* kName = "divdamp_3d_adj";
-  Dims d; Consts c; Geo G;
-  real* ob[3];    // bars of the operator's output (consumed)
-  real* vb[4];    // bars of rhogvx, rhogvy, rhogvz, rhogw
-  NH_HD void operator()(int plane, int x) const {
-    Pt p;
-    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
-    const int kmin = d.kmin, kmax = d.kmax, g = d.g, k = p.k;
-    const long ks = p.ks, a = p.a;
-    real o[3];
-    for (int q = 0; q < 3; ++q) {
-      o[q] = ob[q][a];
-      ob[q][a] = real(0.0);
-    }
-    if (k < kmin || k > kmax) return;
-    const real rz = G.rdgz[k] / real(3.0);
-    auto rwb = [&](long aa, int kk, real v) {
-      if (kk >= kmin + 1 && kk <= kmax) aadd(&vb[3][aa], v);
-    };
-    if (p.pole) {
-      real sb[5];
-      pole_ring_t(G, p.lp, o, sb);
-      for (int v = 1; v <= 5; ++v) {
-        const int vp = (v == 5) ? 1 : v + 1;
-        const real s = sb[v - 1];
-        for (int dd = 0; dd < 3; ++dd) {
-          const real* cI = G.cintp_pl + ((v * 2 + p.lp) * 3 + dd) * 3;
-          aadd(&vb[dd][a], cI[0] * s);
-          aadd(&vb[dd][a + v], cI[1] * s);
-          aadd(&vb[dd][a + vp], cI[2] * s);
-        }
-        rwb(a + ks, k + 1, s * rz); rwb(a + v + ks, k + 1, s * rz); rwb(a + vp + ks, k + 1, s * rz);
-        rwb(a, k, -s * rz); rwb(a + v, k, -s * rz); rwb(a + vp, k, -s * rz);
-      }
-      return;
-    }
-    const long nreg = d.nreg, ci = p.ci;
-    const long co[6] = {0, 0, -1, -1 - (long)g, -1 - (long)g, -(long)g};
-    const int ty[6] = {0, 1, 0, 1, 0, 1};
-    real Tb[6];
-    hex_sum_t(G, nreg, ci, o, Tb);
-    if (p.i == 1 && p.j == 1) {
-      const real m0 = G.sgp[p.l] ? real(0.0) : real(1.0), m1 = G.sgp[p.l] ? real(1.0) : real(0.0);
-      Tb[5] += Tb[4] * m1;
-      Tb[4] = Tb[4] * m0;
-    }
-    for (int n = 0; n < 6; ++n) {
-      const long aa = a + co[n];
-      const long o1 = (ty[n] == 0) ? 1 : 1 + g;
-      const long o2 = (ty[n] == 0) ? 1 + g : g;
-      tri_div_t(G, nreg, ci + co[n], ty[n], vb, aa, g, real(1.0), Tb[n]);
-      const real s = Tb[n] * rz;
-      rwb(aa + ks, k + 1, s); rwb(aa + o1 + ks, k + 1, s); rwb(aa + o2 + ks, k + 1, s);
-      rwb(aa, k, -s); rwb(aa + o1, k, -s); rwb(aa + o2, k, -s);
-    }
-  }
-};
-
-struct DivDamp2A {
-  static constexpr const char* kName = "divdamp_post_adj";
-  Dims d; Consts c; Geo G;
-  real* ob[3];    // bars of the damping force (consumed)
-  real* ib[3];    // bars of the first operator's (halo-filled) output
-  NH_HD void operator()(int plane, int x) const {
-    Pt p;
-    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
-    const long a = p.a;
-    const int kmin = d.kmin, kmax = d.kmax, g = d.g, k = p.k;
-    real o[3];
-    for (int q = 0; q < 3; ++q) {
-      o[q] = ob[q][a];
-      ob[q][a] = real(0.0);
-    }
-    if (k < kmin || k > kmax) return;
-    real xx[3];
-    point_pos(G, d, p, xx);
-    proj_t(xx, c.RADIUS, o);
-    for (int q = 0; q < 3; ++q) o[q] = c.ALPHA_D * o[q];
-    if (p.pole) {
-      real sb[5];
-      pole_ring_t(G, p.lp, o, sb);
-      for (int v = 1; v <= 5; ++v) {
-        const int vp = (v == 5) ? 1 : v + 1;
-        const real s = sb[v - 1];
-        for (int dd = 0; dd < 3; ++dd) {
-          const real* cI = G.cintp_pl + ((v * 2 + p.lp) * 3 + dd) * 3;
-          aadd(&ib[dd][a], -cI[0] * s);
-          aadd(&ib[dd][a + v], -cI[1] * s);
-          aadd(&ib[dd][a + vp], -cI[2] * s);
-        }
-      }
-      return;
-    }
-    const long nreg = d.nreg, ci = p.ci;
-    const long co[6] = {0, 0, -1, -1 - (long)g, -1 - (long)g, -(long)g};
-    const int ty[6] = {0, 1, 0, 1, 0, 1};
-    real Tb[6];
-    hex_sum_t(G, nreg, ci, o, Tb);
-    for (int n = 0; n < 6; ++n) tri_div_t(G, nreg, ci + co[n], ty[n], ib, a + co[n], g, -real(1.0), Tb[n]);
-  }
-};
 
 // ----------------------------------------------------------------------------
 // hyperdiffusion passes
@@ -807,108 +689,129 @@ struct EthTotA {
 };
 
 // ----------------------------------------------------------------------------
-// small step path1 (Path1P): dv bars -> split momentum and rhoge bars, gT,
-// damping force, and (ghost rows) rhog bars
+// gather-form helpers: the output points whose stencil reads point x
+// ----------------------------------------------------------------------------
+// regional point x = (i, j) of a region is neighbour m of the output point
+// (i - di_m, j - dj_m); that point computes when it is interior
+NH_HD inline bool src_of(const Dims& d, const Pt& x, int m, long* a, long* ci) {
+  const int di = ((4201 >> (2 * m)) & 3) - 1, dj = ((421 >> (2 * m)) & 3) - 1;
+  const int i = x.i - di, j = x.j - dj;
+  if (i < 1 || i > d.g - 2 || j < 1 || j > d.g - 2) return false;
+  const long o = (long)di + (long)dj * d.g;
+  *a = x.a - o;
+  *ci = x.ci - o;
+  return true;
+}
+
+// ----------------------------------------------------------------------------
+// small step path1 (Path1P), gather form: the thread of point x adds the
+// bars x receives.  The gradient term of an output at level k reads the split
+// rhoge at level k + shift(k) (kmin -> kmin+1, ghost rows from the nearest
+// physical level's gradient level); the pointwise terms of a ghost-row output
+// land on the nearest physical level (the thread there handles the ghost row,
+// whose momentum before scaling is the forward output at that level).
 // ----------------------------------------------------------------------------
 struct Path1A {
   static constexpr const char* kName = "vi_path1_adj";
   Dims d; Consts c; Geo G;
   real dt;
   const real* prog;
-  const real* ps[6];
-  const real* gT;
-  const real* dd[3];
-  real* dv_b[3];     // consumed
+  const real* dv[3];       // forward output (halo-filled)
+  const real* dv_b[3];     // bars of the output
   real* prog_b;
   real* ps_b[6];
   real* gT_b;
   real* dd_b[3];
-  NH_HD int stage_shift(int k) const {
+  NH_HD int shift(int k) const {
     const int kmin = d.kmin, kmax = d.kmax;
     if (k == kmin || k == kmin - 1) return kmin + 1 - k;
     if (k == kmax + 1) return -1;
     return 0;
   }
-  NH_HD void operator()(int plane, int x) const {
-    Pt p;
-    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
-    const long NF = d.NF, a = p.a, ks = p.ks;
-    const int kmin = d.kmin, kmax = d.kmax, k = p.k;
-    const real R = c.RADIUS;
-    const real RovCV = c.Rdry / c.CVdry;
-    const long sh = (long)stage_shift(k) * ks;
-    const long g = d.g;
+  // bar of the gradient (after the projection transpose) of the output at address a, level k
+  NH_HD void grad_bar(long a, long ks, int k, const real xx[3], real gb[3]) const {
+    const int kmin = d.kmin, kmax = d.kmax;
     real vb[3];
-    for (int q = 0; q < 3; ++q) {
-      vb[q] = dv_b[q][a];
-      dv_b[q][a] = real(0.0);
-    }
-    real xx[3];
-    point_pos(G, d, p, xx);
-    auto grad = [&](int dd2) {
-      real acc;
-      if (p.pole) {
-        acc = real(0.0);
-        for (int v = 0; v < 6; ++v) {
-          const real pv = G.cgrad_pl[(v * 2 + p.lp) * 3 + dd2] * ((ps[I_RHOGE][a + v + sh] * RovCV) * real(1.0) * real(1.0));
-          acc = (v == 0) ? pv : acc + pv;
-        }
-      } else {
-        acc = rc(G.cgrad, d.nreg, dd2 * 7, p.ci) * ((ps[I_RHOGE][a + sh] * RovCV) * real(1.0));
-        for (int m = 1; m < 7; ++m)
-          acc = acc + rc(G.cgrad, d.nreg, dd2 * 7 + m, p.ci) * ((ps[I_RHOGE][a + nb_off(m, g) + sh] * RovCV) * real(1.0));
-      }
-      return acc;
-    };
-    long at = a;
+    for (int q = 0; q < 3; ++q) vb[q] = dv_b[q][a];
     if (k < kmin || k > kmax) {
       const bool top = (k == kmax + 1);
       const long an = top ? a - ks : a + ks;
-      // forward momentum at the nearest physical level
-      real gr[3];
-      for (int dd2 = 0; dd2 < 3; ++dd2) gr[dd2] = grad(dd2) + real(0.0);
-      const real prd = (gr[0] * xx[0] + gr[1] * xx[1] + gr[2] * xx[2]) / R;
-      real vn[3];
-      for (int q = 0; q < 3; ++q) {
-        const real Pg = gr[q] - prd * xx[q] / R;
-        vn[q] = ps[1 + q][an] + (gT[(1 + q) * NF + an] - Pg + dd[q][an] + real(0.0)) * dt;
-      }
       const real sgn = top ? real(1.0) : -real(1.0);
-      const real rgn = prog[an], rga = prog[a];
-      real rab = real(0.0), rnb = real(0.0);
-      for (int q = 0; q < 3; ++q) {
-        rab += sgn * vb[q] * vn[q] / rgn;
-        rnb -= sgn * vb[q] * vn[q] * rga / (rgn * rgn);
-        vb[q] = sgn * vb[q] * rga / rgn;
-      }
-      aadd(&prog_b[a], rab);
-      aadd(&prog_b[an], rnb);
-      at = an;
+      for (int q = 0; q < 3; ++q) vb[q] = sgn * vb[q] * prog[a] / prog[an];
     }
-    // momentum at level address `at`: out = ps + (gT - P(grad) + dd) dt
-    real gb[3];
-    for (int q = 0; q < 3; ++q) {
-      aadd(&ps_b[1 + q][at], vb[q]);
-      const real db = vb[q] * dt;
-      aadd(&gT_b[(1 + q) * NF + at], db);
-      aadd(&dd_b[q][at], db);
-      gb[q] = -db;
-    }
-    proj_t(xx, R, gb);
-    real* eb = ps_b[I_RHOGE];
-    for (int dd2 = 0; dd2 < 3; ++dd2) {
-      const real w = gb[dd2] * RovCV;
+    for (int q = 0; q < 3; ++q) gb[q] = -vb[q] * dt;
+    proj_t(xx, c.RADIUS, gb);
+  }
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p)) return;
+    const long NF = d.NF, a = p.a, ks = p.ks;
+    const int kmin = d.kmin, kmax = d.kmax, k = p.k;
+    const real RovCV = c.Rdry / c.CVdry;
+    // outputs whose gradient level is k: their level offsets from k
+    int nsrc = 0;
+    int dk[3];
+    if (k >= kmin + 2 && k <= kmax) dk[nsrc++] = 0;
+    if (k == kmin + 1) { dk[nsrc++] = 0; dk[nsrc++] = -1; dk[nsrc++] = -2; }
+    if (k == kmax) dk[nsrc++] = 1;
+    real eb = real(0.0);
+    for (int s2 = 0; s2 < nsrc; ++s2) {
+      const int ko = k + dk[s2];
       if (p.pole) {
-        for (int v = 0; v < 6; ++v) aadd(&eb[a + v + sh], G.cgrad_pl[(v * 2 + p.lp) * 3 + dd2] * w);
+        // the pole centre reads the six pole points; x is its neighbour v
+        const long ac = a - p.v + (long)dk[s2] * ks;
+        real xx[3], gb[3];
+        for (int q = 0; q < 3; ++q) xx[q] = G.gx_pl[p.lp * 3 + q];
+        grad_bar(ac, ks, ko, xx, gb);
+        for (int dd2 = 0; dd2 < 3; ++dd2) eb += G.cgrad_pl[(p.v * 2 + p.lp) * 3 + dd2] * RovCV * gb[dd2];
       } else {
-        for (int m = 0; m < 7; ++m) aadd(&eb[a + nb_off(m, g) + sh], rc(G.cgrad, d.nreg, dd2 * 7 + m, p.ci) * w);
+        for (int m = 0; m < 7; ++m) {
+          long ao, co;
+          if (!src_of(d, p, m, &ao, &co)) continue;
+          ao += (long)dk[s2] * ks;
+          real xx[3], gb[3];
+          for (int q = 0; q < 3; ++q) xx[q] = G.gx[q * d.nreg + co];
+          grad_bar(ao, ks, ko, xx, gb);
+          for (int dd2 = 0; dd2 < 3; ++dd2) eb += rc(G.cgrad, d.nreg, dd2 * 7 + m, co) * RovCV * gb[dd2];
+        }
       }
+    }
+    if (nsrc > 0) ps_b[I_RHOGE][a] += eb;
+    // pointwise terms at the compute points (ghost rows folded into the nearest level)
+    if (!pt_compute(p) || k < kmin || k > kmax) return;
+    real vt[3];
+    for (int q = 0; q < 3; ++q) vt[q] = dv_b[q][a];
+    real rnb = real(0.0);
+    for (int s2 = 0; s2 < 2; ++s2) {
+      if (s2 == 0 ? k != kmin : k != kmax) continue;
+      const bool top = (s2 == 1);
+      const long ag = top ? a + ks : a - ks;     // the ghost row
+      const real sgn = top ? real(1.0) : -real(1.0);
+      const real rgn = prog[a], rgg = prog[ag];
+      real rgb = real(0.0);
+      for (int q = 0; q < 3; ++q) {
+        const real vb = dv_b[q][ag];
+        const real vn = dv[q][a];
+        rgb += sgn * vb * vn / rgn;
+        rnb -= sgn * vb * vn * rgg / (rgn * rgn);
+        vt[q] += sgn * vb * rgg / rgn;
+      }
+      prog_b[ag] += rgb;
+    }
+    prog_b[a] += rnb;
+    for (int q = 0; q < 3; ++q) {
+      ps_b[1 + q][a] += vt[q];
+      gT_b[(1 + q) * NF + a] += vt[q] * dt;
+      dd_b[q][a] += vt[q] * dt;
     }
   }
 };
 
 // ----------------------------------------------------------------------------
-// small step vi_main, horizontal part (ViMainH)
+// small step vi_main, horizontal part (ViMainH), gather form: the thread of
+// point x adds the bars of dv, eth, ethtot, PROG momentum at x from the
+// divergences of the compute points whose stencil holds x, and the bar of
+// gT's energy component at x
 // ----------------------------------------------------------------------------
 struct ViMainHA {
   static constexpr const char* kName = "vi_main_h_adj";
@@ -917,7 +820,7 @@ struct ViMainHA {
   const real* dv[3];
   const real* eth;
   const real* ethtot;
-  real* hv_b[3];     // consumed
+  const real* hv_b[3];
   real* dv_b[3];
   real* eth_b;
   real* ethtot_b;
@@ -925,58 +828,66 @@ struct ViMainHA {
   real* gT_b;
   NH_HD void operator()(int plane, int x) const {
     Pt p;
-    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    if (!pt_of(d, plane, x, p)) return;
     const long NF = d.NF, a = p.a;
     const int k = p.k;
-    real hb[3];
-    for (int q = 0; q < 3; ++q) {
-      hb[q] = hv_b[q][a];
-      hv_b[q][a] = real(0.0);
-    }
     if (k < d.kmin || k > d.kmax) return;
     const real rc_ = c.Rdry / c.CVdry;
-    aadd(&gT_b[5 * NF + a], hb[1] * rc_);
-    const real heb = -hb[1] * rc_;
-    const int NN = p.pole ? 6 : 7;
-    for (int m = 0; m < NN; ++m) {
-      const long b = p.pole ? a + m : a + nb_off(m, d.g);
-      real cdir[3];
-      if (p.pole) {
-        for (int q = 0; q < 3; ++q) cdir[q] = G.cdiv_pl[(m * 2 + p.lp) * 3 + q];
-      } else {
-        for (int q = 0; q < 3; ++q) cdir[q] = rc(G.cdiv, d.nreg, q * 7 + m, p.ci);
-      }
-      const real e = eth[b], et = ethtot[b];
-      real eb = real(0.0), etb = real(0.0);
+    real w0[3] = {real(0.0), real(0.0), real(0.0)}, w1[3] = {real(0.0), real(0.0), real(0.0)},
+         w2[3] = {real(0.0), real(0.0), real(0.0)};
+    auto take = [&](long ao, const real cdir[3]) {
+      const real h0 = hv_b[0][ao], h1 = -hv_b[1][ao] * rc_, h2 = hv_b[2][ao];
       for (int q = 0; q < 3; ++q) {
-        const real w0 = cdir[q] * hb[0], w1 = cdir[q] * heb, w2 = cdir[q] * hb[2];
-        const real dq = dv[q][b];
-        aadd(&dv_b[q][b], w0 + w1 * e + w2 * et);
-        eb += w1 * dq;
-        aadd(&prog_b[(1 + q) * NF + b], w2 * et);
-        etb += w2 * (prog[(1 + q) * NF + b] + dq);
+        w0[q] += cdir[q] * h0;
+        w1[q] += cdir[q] * h1;
+        w2[q] += cdir[q] * h2;
       }
-      aadd(&eth_b[b], eb);
-      aadd(&ethtot_b[b], etb);
+    };
+    if (p.pole) {
+      real cdir[3];
+      for (int q = 0; q < 3; ++q) cdir[q] = G.cdiv_pl[(p.v * 2 + p.lp) * 3 + q];
+      take(a - p.v, cdir);
+    } else {
+      for (int m = 0; m < 7; ++m) {
+        long ao, co;
+        if (!src_of(d, p, m, &ao, &co)) continue;
+        real cdir[3];
+        for (int q = 0; q < 3; ++q) cdir[q] = rc(G.cdiv, d.nreg, q * 7 + m, co);
+        take(ao, cdir);
+      }
     }
+    const real e = eth[a], et = ethtot[a];
+    real eb = real(0.0), etb = real(0.0);
+    for (int q = 0; q < 3; ++q) {
+      const real dq = dv[q][a];
+      dv_b[q][a] += w0[q] + w1[q] * e + w2[q] * et;
+      eb += w1[q] * dq;
+      prog_b[(1 + q) * NF + a] += w2[q] * et;
+      etb += w2[q] * (prog[(1 + q) * NF + a] + dq);
+    }
+    eth_b[a] += eb;
+    ethtot_b[a] += etb;
+    if (pt_compute(p)) gT_b[5 * NF + a] += hv_b[1][a] * rc_;
   }
 };
 
 // ----------------------------------------------------------------------------
 // small step vi_main, vertical part (ViMainV): the transposed tridiagonal
-// solve, one thread per compute column, in FP64.  The forward solution x is
-// recomputed (the matrix bars need it); U, Gm, X are FP64 column scratch.
+// solve, one thread per compute column, in FP64.  x is the forward solution
+// (the new rhogw split as stored; the matrix bars are -y x^T on the band).
+// NK > 0 (the number of unknowns) keeps the elimination in registers; NK = 0
+// uses the FP64 column scratch U, Gm.
 // ----------------------------------------------------------------------------
+template <int NK>
 struct ViMainVA {
   static constexpr const char* kName = "vi_main_v_adj";
+  static constexpr int kUnroll = NK > 0 ? NK : 4;
   Dims d; Consts c; Geo G;
   double dt;
-  const real* ps[6];
-  const real* gT;
+  const real* x;       // forward solution (rhogw split after the step)
   const double* Mc; const double* Mu; const double* Ml;
-  const real* hv[3];
-  double* U; double* Gm; double* X;
-  real* out_b;        // bar of the new rhogw split (consumed)
+  double* U; double* Gm;
+  real* out_b;         // bar of the new rhogw split
   real* ps_b[6];
   real* gT_b;
   double* Mc_b; double* Mu_b; double* Ml_b;
@@ -990,86 +901,47 @@ struct ViMainVA {
     const ConstsT<double>& C = c.d;
     const double RovCV = C.Rdry / C.CVdry;
     const double CVovRt2 = C.CVdry / C.Rdry / (dt * dt);
-    const int N = kmax - kmin;
-    auto lev = [&](int k) { return b + (long)k * ks; };
-    auto src = [&](int k) {
-      const long a = lev(k), am = lev(k - 1);
-      const double gr_k = (double)gT[a] + (-(double)hv[0][a] - (0.0 - 0.0) * V.rdgz[k]);
-      const double gr_m = (double)gT[am] + (-(double)hv[0][am] - (0.0 - 0.0) * V.rdgz[k - 1]);
-      const double sw = (double)gT[4 * NF + a] + 0.0 * C.ALPHA;
-      const double A = ((double)ps[I_RHOGW][a] * C.ALPHA + dt * sw) * 1.0;
-      const double B = ((double)ps[I_RHOGE][a] * RovCV + dt * (double)hv[1][a]) * 1.0 -
-                       ((double)ps[I_RHOGE][am] * RovCV + dt * (double)hv[1][am]) * 1.0;
-      const double Cc = ((double)ps[I_RHOG][a] + dt * gr_k) * 1.0 * V.afact[k] +
-                        ((double)ps[I_RHOG][am] + dt * gr_m) * 1.0 * V.bfact[k];
-      return (A - B * dt * V.rdgzh[k] - Cc * dt * C.GRAV) * CVovRt2;
-    };
-    // forward solve M x = S
-    double beta = 0.0, u_prev = 0.0;
+    const int N = (NK > 0) ? NK : kmax - kmin;
+    double ub[NK > 0 ? NK : 1], gb[NK > 0 ? NK : 1];
+    // elimination of M^T y = xbar; row n: Mu[n-1] y[n-1] + Mc[n] y[n] + Ml[n+1] y[n+1]
+    long a = b + (long)kmin * ks;
+    double beta = 0.0, u_prev = 0.0, mu_m = 0.0;
+#pragma unroll kUnroll
     for (int n = 0; n < N; ++n) {
-      const long a = lev(kmin + 1 + n);
-      const double S = src(kmin + 1 + n);
-      double u;
-      if (n == 0) {
-        beta = Mc[a];
-        u = S / beta;
-      } else {
-        const double gm = Mu[a - ks] / beta;
-        beta = Mc[a] - Ml[a] * gm;
-        u = (S - Ml[a] * u_prev) / beta;
-        Gm[a] = gm;
-      }
-      U[a] = u;
-      u_prev = u;
-    }
-    {
-      long a = lev(kmax);
-      double xv = u_prev;
-      X[a] = xv;
-      for (int n = N - 2; n >= 0; --n) {
-        const double gm_next = Gm[a];
-        a -= ks;
-        xv = U[a] - gm_next * xv;
-        X[a] = xv;
-      }
-    }
-    // transposed solve M^T y = xbar: row n of M^T is Mu[n-1] y[n-1] + Mc[n] y[n] + Ml[n+1] y[n+1]
-    out_b[lev(kmin)] = real(0.0);
-    out_b[lev(kmax + 1)] = real(0.0);
-    out_b[lev(kmin - 1)] = real(0.0);
-    u_prev = 0.0;
-    for (int n = 0; n < N; ++n) {
-      const long a = lev(kmin + 1 + n);
+      a += ks;
       const double xb = (double)out_b[a];
-      out_b[a] = real(0.0);
+      const double mc = Mc[a], ml = Ml[a];
       double u;
       if (n == 0) {
-        beta = Mc[a];
+        beta = mc;
         u = xb / beta;
       } else {
-        const double gm = Ml[a] / beta;
-        beta = Mc[a] - Mu[a - ks] * gm;
-        u = (xb - Mu[a - ks] * u_prev) / beta;
-        Gm[a] = gm;
+        const double gm = ml / beta;
+        beta = mc - mu_m * gm;
+        u = (xb - mu_m * u_prev) / beta;
+        if (NK > 0) gb[NK > 0 ? n : 0] = gm;
+        else Gm[a] = gm;
       }
-      U[a] = u;
+      if (NK > 0) ub[NK > 0 ? n : 0] = u;
+      else U[a] = u;
       u_prev = u;
+      mu_m = Mu[a];
     }
-    // back substitution, then the bars of S (y) and of the matrix (-y x^T on the band)
-    long a = lev(kmax);
-    double y = u_prev;
+    // back substitution from the top; per level the bars of S = y and of the band
+    double y = u_prev, x_up = 0.0;
+    double xk = (double)x[a];
+    double gm_next = 0.0;
+#pragma unroll kUnroll
     for (int n = N - 1; n >= 0; --n) {
-      if (n < N - 1) {
-        const double gm_next = Gm[a];
-        a -= ks;
-        y = U[a] - gm_next * y;
-      }
+      if (n < N - 1) y = ((NK > 0) ? ub[NK > 0 ? n : 0] : U[a]) - gm_next * y;
+      if (NK == 0 && n > 0) gm_next = Gm[a];
+      if (NK > 0 && n > 0) gm_next = gb[NK > 0 ? n : 0];
       const int k = kmin + 1 + n;
-      const double xk = X[a];
-      Mc_b[a] += -y * xk;
-      if (n >= 1) Ml_b[a] += -y * X[a - ks];
-      if (n <= N - 2) Mu_b[a] += -y * X[a + ks];
       const long am = a - ks;
+      const double x_dn = (n >= 1) ? (double)x[am] : 0.0;
+      Mc_b[a] += -y * xk;
+      if (n >= 1) Ml_b[a] += -y * x_dn;
+      if (n <= N - 2) Mu_b[a] += -y * x_up;
       const double s = y * CVovRt2;
       const double Ab = s, Bb = -s * dt * V.rdgzh[k], Cb = -s * dt * C.GRAV;
       ps_b[I_RHOGW][a] += (real)(Ab * C.ALPHA);
@@ -1085,6 +957,9 @@ struct ViMainVA {
       hv_b[0][a] += (real)(-grk);
       gT_b[am] += (real)grm;
       hv_b[0][am] += (real)(-grm);
+      x_up = xk;
+      xk = x_dn;
+      a = am;
     }
   }
 };
@@ -1116,6 +991,11 @@ struct ViMainPostA {
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    point(p, [](real* q, real v) { aadd(q, v); });
+  }
+  // the transpose at one compute point; add(address, value) accumulates a bar
+  template <class ADD>
+  NH_HD void point(const Pt& p, ADD add) const {
     const long NF = d.NF, ks = p.ks, a = p.a;
     const int kmin = d.kmin, kmax = d.kmax, k = p.k;
     const long b = a - (long)k * ks;
@@ -1135,46 +1015,46 @@ struct ViMainPostA {
       return pg[at(kk)] + (gT[0 * NF + at(kk)] + drhog2) * dt;
     };
     auto rs1_t = [&](int kk, real rb) {
-      aadd(&ps_b[I_RHOG][at(kk)], rb);
-      aadd(&gT_b[at(kk)], rb * dt);
+      add(&ps_b[I_RHOG][at(kk)], rb);
+      add(&gT_b[at(kk)], rb * dt);
       if (kk >= kmin && kk <= kmax) {
         const real d2b = rb * dt;
-        aadd(&hv_b[0][at(kk)], -d2b);
-        if (kk + 1 <= kmax) aadd(&we_b[1][at(kk + 1)], -d2b * G.rdgz[kk]);
-        if (kk >= kmin + 1) aadd(&we_b[1][at(kk)], d2b * G.rdgz[kk]);
+        add(&hv_b[0][at(kk)], -d2b);
+        if (kk + 1 <= kmax) add(&we_b[1][at(kk + 1)], -d2b * G.rdgz[kk]);
+        if (kk >= kmin + 1) add(&we_b[1][at(kk)], d2b * G.rdgz[kk]);
       }
     };
     const int ke = (k < kmin) ? kmin : ((k > kmax) ? kmax : k);
-    aadd(&ps_b[I_RHOGE][at(ke)], b2);
-    aadd(&getot_b[a], b2 * dt);
+    add(&ps_b[I_RHOGE][at(ke)], b2);
+    add(&getot_b[a], b2 * dt);
     const real dtotb = b2 * dt;
-    aadd(&ps_b[I_RHOG][a], b2 * G.phi[k]);
+    add(&ps_b[I_RHOG][a], b2 * G.phi[k]);
     rs1_t(k, b0 - b2 * G.phi[k]);
     if (k >= kmin && k <= kmax) {
-      aadd(&hv_b[2][a], -dtotb);
+      add(&hv_b[2][a], -dtotb);
       const real hib = -dtotb * G.rdgz[k], lob = dtotb * G.rdgz[k];
       if (k + 1 <= kmax) {
         const real W1 = prog[4 * NF + a + ks] + rw1[a + ks];
         const real E1 = G.afact[k + 1] * ethtot[a + ks] + G.bfact[k + 1] * ethtot[a];
-        aadd(&prog_b[4 * NF + a + ks], hib * E1);
-        aadd(&we_b[1][a + ks], hib * E1);
-        aadd(&ethtot_b[a + ks], hib * W1 * G.afact[k + 1]);
-        aadd(&ethtot_b[a], hib * W1 * G.bfact[k + 1]);
+        add(&prog_b[4 * NF + a + ks], hib * E1);
+        add(&we_b[1][a + ks], hib * E1);
+        add(&ethtot_b[a + ks], hib * W1 * G.afact[k + 1]);
+        add(&ethtot_b[a], hib * W1 * G.bfact[k + 1]);
       }
       if (k >= kmin + 1) {
         const real W0 = prog[4 * NF + a] + rw1[a];
         const real E0 = G.afact[k] * ethtot[a] + G.bfact[k] * ethtot[a - ks];
-        aadd(&prog_b[4 * NF + a], lob * E0);
-        aadd(&we_b[1][a], lob * E0);
-        aadd(&ethtot_b[a], lob * W0 * G.afact[k]);
-        aadd(&ethtot_b[a - ks], lob * W0 * G.bfact[k]);
+        add(&prog_b[4 * NF + a], lob * E0);
+        add(&we_b[1][a], lob * E0);
+        add(&ethtot_b[a], lob * W0 * G.afact[k]);
+        add(&ethtot_b[a - ks], lob * W0 * G.bfact[k]);
       }
     }
     // kinetic energies before (PROG + PROG_split) and after (new split) the step
     auto g10 = [&](int kk, int q) { return prog[q * NF + at(kk)] + ps[q][at(kk)]; };
     auto add10 = [&](int kk, int q, real v) {
-      aadd(&prog_b[q * NF + at(kk)], v);
-      aadd(&ps_b[q][at(kk)], v);
+      add(&prog_b[q * NF + at(kk)], v);
+      add(&ps_b[q][at(kk)], v);
     };
     rhogkin_k_t(G, kmin, kmax, k, g10, add10, b2);
     auto g11 = [&](int kk, int q) {
@@ -1183,10 +1063,10 @@ struct ViMainPostA {
       return prog[q * NF + at(kk)] + dv[q - 1][at(kk)];
     };
     auto add11 = [&](int kk, int q, real v) {
-      aadd(&prog_b[q * NF + at(kk)], v);
+      add(&prog_b[q * NF + at(kk)], v);
       if (q == 0) rs1_t(kk, v);
-      else if (q == 4) aadd(&we_b[1][at(kk)], v);
-      else aadd(&dv_b[q - 1][at(kk)], v);
+      else if (q == 4) add(&we_b[1][at(kk)], v);
+      else add(&dv_b[q - 1][at(kk)], v);
     };
     rhogkin_k_t(G, kmin, kmax, k, g11, add11, -b2);
   }
@@ -1492,6 +1372,135 @@ struct SfcPresP {
     prog_b[5 * d.NF + a1] += (real)(einb / rhog);
     rg1b += rhob - einb * t1.ein / rhog - qb * t1.q / rhog;
     prog_b[a1] += (real)rg1b;
+  }
+};
+
+// the point of a compute column at level k
+NH_HD inline Pt pt_at(const Dims& d, const Cell& cl, int k) {
+  Pt p;
+  p.pole = cl.pole; p.inner = cl.inner;
+  p.i = cl.i; p.j = cl.j; p.l = cl.l; p.v = cl.v; p.lp = cl.lp; p.k = k;
+  p.ks = cl.ks;
+  p.a = cl.base + (long)k * cl.ks;
+  p.ci = cl.ci;
+  (void)d;
+  return p;
+}
+
+
+// ----------------------------------------------------------------------------
+// divergence damping, gather form.  Both operators are out(cell) =
+// hex_sum(cell, T) over the six triangles around the cell of triangle values
+// T = tri_div (+ the vertical rhogw term for the first operator).
+// Kernel W: per compute cell the six weights w_n = sum_dd H_n(dd) o_dd that
+// its output bar o sends to its triangles n (H_n: the hex_sum coefficient of
+// triangle n; in the first operator the pentagon corner (1,1) sends triangle
+// 4's share to triangle 5, as the forward substitutes T5 for T4 there).
+// Kernel G: per point x the triangle bars S(T) = sum of the weights of the
+// cells around T, then the tri_div transposes at x's corner of each triangle.
+// The pole centre keeps its five ring-triangle bars in w_0..w_4.
+// ----------------------------------------------------------------------------
+template <int OP>
+struct DivDampWA {
+  static constexpr const char* kName = OP == 1 ? "divdamp_3d_adj" : "divdamp_post_adj";
+  Dims d; Consts c; Geo G;
+  const real* ob[3];
+  real* w;            // 6 fields
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p)) return;
+    const long NF = d.NF, a = p.a;
+    real wn[6] = {real(0.0), real(0.0), real(0.0), real(0.0), real(0.0), real(0.0)};
+    if (pt_compute(p) && p.k >= d.kmin && p.k <= d.kmax) {
+      real o[3];
+      for (int q = 0; q < 3; ++q) o[q] = ob[q][a];
+      if (OP == 2) {
+        real xx[3];
+        point_pos(G, d, p, xx);
+        proj_t(xx, c.RADIUS, o);
+        for (int q = 0; q < 3; ++q) o[q] = c.ALPHA_D * o[q];
+      }
+      if (p.pole) {
+        real sb[5];
+        pole_ring_t(G, p.lp, o, sb);
+        for (int v = 0; v < 5; ++v) wn[v] = sb[v];
+      } else {
+        hex_sum_t(G, d.nreg, p.ci, o, wn);
+        if (OP == 1 && p.i == 1 && p.j == 1 && G.sgp[p.l]) {
+          wn[5] += wn[4];
+          wn[4] = real(0.0);
+        }
+      }
+    }
+    for (int n = 0; n < 6; ++n) w[n * NF + a] = wn[n];
+  }
+};
+
+template <int OP>
+struct DivDampGA {
+  static constexpr const char* kName = OP == 1 ? "divdamp_3d_adj" : "divdamp_post_adj";
+  Dims d; Consts c; Geo G;
+  const real* w;      // the weights of DivDampWA
+  real* vb[4];        // OP 1: bars of rhogvx..vz, rhogw (added); OP 2: bars of the first result (set)
+  // sum of the six triangles' bars at level offset (0 or -1) and their tri_div transposes at x
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p)) return;
+    const long NF = d.NF, a = p.a, ks = p.ks;
+    const int kmin = d.kmin, kmax = d.kmax, k = p.k;
+    const real sgn = (OP == 1) ? real(1.0) : -real(1.0);
+    real acc[3] = {real(0.0), real(0.0), real(0.0)};
+    real ssum = real(0.0), ssum_dn = real(0.0);
+    const bool vert = (OP == 1) && k >= kmin + 1 && k <= kmax;
+    if (p.pole) {
+      const long ac = a - p.v;   // pole centre
+      const int lp = p.lp;
+      auto tri = [&](int v, int corner) {
+        const real S = w[(v - 1) * NF + ac];
+        for (int dd = 0; dd < 3; ++dd) acc[dd] += G.cintp_pl[((v * 2 + lp) * 3 + dd) * 3 + corner] * sgn * S;
+        ssum += S;
+        if (vert) ssum_dn += w[(v - 1) * NF + ac - ks];
+      };
+      if (p.v == 0) {
+        for (int v = 1; v <= 5; ++v) tri(v, 0);
+      } else {
+        tri(p.v, 1);
+        tri(p.v == 1 ? 5 : p.v - 1, 2);
+      }
+    } else {
+      const int g = d.g, i = p.i, j = p.j;
+      const long nreg = d.nreg;
+      // weight n of cell (ii, jj) at level offset dk (0 outside the region's array)
+      auto wc = [&](int n, int ii, int jj, int dk) -> real {
+        if (ii < 0 || jj < 0 || ii > g - 1 || jj > g - 1) return real(0.0);
+        return w[n * NF + a + (long)(ii - i) + (long)(jj - j) * g + (long)dk * ks];
+      };
+      // the six triangles with x as a corner: (type, base (bi, bj), corner of x, cells n)
+      const int tT[6] = {0, 0, 0, 1, 1, 1};
+      const int tbi[6] = {0, -1, -1, 0, -1, 0}, tbj[6] = {0, 0, -1, 0, -1, -1};
+      const int tcor[6] = {0, 1, 2, 0, 1, 2};
+      for (int t = 0; t < 6; ++t) {
+        const int bi = i + tbi[t], bj = j + tbj[t];
+        if (bi < 0 || bj < 0 || bi > g - 1 || bj > g - 1) continue;
+        auto S_at = [&](int dk) {
+          // TI(b): cells b (n 0), b+(1,0) (n 2), b+(1,1) (n 4); TJ(b): b (n 1), b+(1,1) (n 3), b+(0,1) (n 5)
+          return (tT[t] == 0) ? wc(0, bi, bj, dk) + wc(2, bi + 1, bj, dk) + wc(4, bi + 1, bj + 1, dk)
+                              : wc(1, bi, bj, dk) + wc(3, bi + 1, bj + 1, dk) + wc(5, bi, bj + 1, dk);
+        };
+        const real S = S_at(0);
+        const long cj = p.ci + (long)tbi[t] + (long)tbj[t] * g;
+        for (int dd = 0; dd < 3; ++dd)
+          acc[dd] += rc(G.cintp, nreg, (dd * 3 + tcor[t]) * 2 + tT[t], cj) * sgn * S;
+        ssum += S;
+        if (vert) ssum_dn += S_at(-1);
+      }
+    }
+    if (OP == 1) {
+      for (int dd = 0; dd < 3; ++dd) vb[dd][a] += acc[dd];
+      if (vert) vb[3][a] += ssum_dn * G.rdgz[k - 1] / real(3.0) - ssum * G.rdgz[k] / real(3.0);
+    } else {
+      for (int dd = 0; dd < 3; ++dd) vb[dd][a] = acc[dd];
+    }
   }
 };
 

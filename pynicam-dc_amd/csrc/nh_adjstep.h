@@ -33,12 +33,15 @@ enum AwsField {
   A_PSB2 = 73,     // 6  bar of the split state before it
   A_HVB = 79,      // 3
   A_DDVB = 82,     // 3
-  A_CK = 85        // 6 per small-step checkpoint slot
+  A_TRIW = 85,     // 6  triangle weights of the damping transposes
+  A_CK = 91        // per small-step checkpoint slot: its split state (6), then the
+                   // divergences (3) of the step from it
 };
+constexpr int kSlot = 9;
 // FP64: bars of the tridiagonal matrix, the recomputed solution
 enum AdwsField { AD_MCB = 0, AD_MUB = 1, AD_MLB = 2, AD_X = 3, AD_NFIELDS = 4 };
 
-inline long adj_fields(int nck) { return (long)A_CK + 6L * nck; }
+inline long adj_fields(int nck) { return (long)A_CK + (long)kSlot * nck; }
 
 inline real* afield(const Context& ctx, int f) { return ctx.aws + (long)f * ctx.d.NF; }
 inline double* adfield(const Context& ctx, int f) { return ctx.adws + (long)f * ctx.d.NF; }
@@ -79,39 +82,36 @@ void halo_block_adj(X& x, const Context& ctx, real* base, int nf) {
 template <class X>
 void divdamp_adj(X& x, const Context& ctx, real* const ob[3], real* const vb[4]) {
   real* tb[3] = {afield(ctx, A_DDTB), afield(ctx, A_DDTB + 1), afield(ctx, A_DDTB + 2)};
-  zero_fields(x, tb[0], 3, ctx.d.NF);
-  DivDamp2A b;
-  b.d = ctx.d; b.c = ctx.c; b.G = ctx.G;
-  for (int q = 0; q < 3; ++q) { b.ob[q] = ob[q]; b.ib[q] = tb[q]; }
-  x.run_pts(b, ctx.d);
+  real* w = afield(ctx, A_TRIW);
+  DivDampWA<2> w2{ctx.d, ctx.c, ctx.G, {ob[0], ob[1], ob[2]}, w};
+  x.run_pts(w2, ctx.d);
+  DivDampGA<2> g2{ctx.d, ctx.c, ctx.G, w, {tb[0], tb[1], tb[2], nullptr}};
+  x.run_pts(g2, ctx.d);
   halo_adj(x, ctx, tb, 3);
-  DivDamp1A a;
-  a.d = ctx.d; a.c = ctx.c; a.G = ctx.G;
-  for (int q = 0; q < 3; ++q) a.ob[q] = tb[q];
-  for (int q = 0; q < 4; ++q) a.vb[q] = vb[q];
-  x.run_pts(a, ctx.d);
+  DivDampWA<1> w1{ctx.d, ctx.c, ctx.G, {tb[0], tb[1], tb[2]}, w};
+  x.run_pts(w1, ctx.d);
+  DivDampGA<1> g1{ctx.d, ctx.c, ctx.G, w, {vb[0], vb[1], vb[2], vb[3]}};
+  x.run_pts(g1, ctx.d);
 }
 
 // reverse of one small step: ck = its input split state, nx = its output
 // (we0, dv0, dv1, dv2, we1, we2); psb (bar of nx, consumed) -> psb2 (bar of
 // ck, added) and the stage bars
+// (hvf: the divergences the recomputed step kept)
 template <class X>
-void small_step_adj(X& x, const Context& ctx, const real* prog, real* const ck[6], real* const nx[6], double dts,
-                    real* prog_b, real* const psb[6], real* const psb2[6]) {
+void small_step_adj(X& x, const Context& ctx, const real* prog, real* const ck[6], real* const nx[6],
+                    real* hvf, double dts, real* prog_b, real* const psb[6], real* const psb2[6]) {
   const Dims& d = ctx.d;
   const long NF = d.NF;
   real* gT = ctx.field(W_GT);
   real* gTb = afield(ctx, A_GTB);
-  real* ddv = ctx.field(W_DDV);
-  // forward values of the step: damping force of ck, divergences of the new momentum
-  divdamp(x, ctx, ck[1], ck[2], ck[3], ck[4], ddv, ddv + NF, ddv + 2 * NF);
   real* dvn[3] = {nx[1], nx[2], nx[3]};
-  small_hv(x, ctx, prog, dvn);
-  real* hv[3] = {ctx.field(W_HV), ctx.field(W_HV + 1), ctx.field(W_HV + 2)};
+  real* hv[3] = {hvf, hvf + NF, hvf + 2 * NF};
   real* hvb[3] = {afield(ctx, A_HVB), afield(ctx, A_HVB + 1), afield(ctx, A_HVB + 2)};
   real* web[3] = {psb[0], psb[4], psb[5]};
   real* dvb[3] = {psb[1], psb[2], psb[3]};
   halo_adj(x, ctx, web, 3);
+  zero_fields(x, hvb[0], 3, NF);
   {
     ViMainPostA f;
     f.d = d; f.c = ctx.c; f.G = ctx.G;
@@ -119,25 +119,28 @@ void small_step_adj(X& x, const Context& ctx, const real* prog, real* const ck[6
     f.prog = prog;
     for (int q = 0; q < 6; ++q) { f.ps[q] = ck[q]; f.ps_b[q] = psb2[q]; }
     for (int q = 0; q < 3; ++q) { f.dv[q] = dvn[q]; f.dv_b[q] = dvb[q]; f.hv[q] = hv[q]; f.hv_b[q] = hvb[q]; }
-    f.ethtot = ctx.field(W_ETHTOT); f.gT = gT; f.hv[0] = hv[0];
+    f.ethtot = ctx.field(W_ETHTOT); f.gT = gT;
     f.we1 = nx[4];
     for (int q = 0; q < 3; ++q) f.we_b[q] = web[q];
     f.prog_b = prog_b; f.ethtot_b = afield(ctx, A_ETHTOTB); f.gT_b = gTb; f.getot_b = afield(ctx, A_GETOTB);
     x.run_pts(f, d);
   }
   {
-    ViMainVA f;
-    f.d = d; f.c = ctx.c; f.G = ctx.G;
-    f.dt = dts;
-    for (int q = 0; q < 6; ++q) { f.ps[q] = ck[q]; f.ps_b[q] = psb2[q]; }
-    f.gT = gT;
-    f.Mc = ctx.dfield(D_MC); f.Mu = ctx.dfield(D_MU); f.Ml = ctx.dfield(D_ML);
-    for (int q = 0; q < 3; ++q) { f.hv[q] = hv[q]; f.hv_b[q] = hvb[q]; }
-    f.U = ctx.dfield(D_U); f.Gm = ctx.dfield(D_GM); f.X = adfield(ctx, AD_X);
-    f.out_b = web[1];
-    f.gT_b = gTb;
-    f.Mc_b = adfield(ctx, AD_MCB); f.Mu_b = adfield(ctx, AD_MUB); f.Ml_b = adfield(ctx, AD_MLB);
-    x.run(f, ncols(d));
+    auto vert = [&](auto f) {
+      f.d = d; f.c = ctx.c; f.G = ctx.G;
+      f.dt = dts;
+      f.x = nx[4];
+      f.Mc = ctx.dfield(D_MC); f.Mu = ctx.dfield(D_MU); f.Ml = ctx.dfield(D_ML);
+      f.U = ctx.dfield(D_U); f.Gm = ctx.dfield(D_GM);
+      f.out_b = web[1];
+      for (int q = 0; q < 6; ++q) f.ps_b[q] = psb2[q];
+      f.gT_b = gTb;
+      f.Mc_b = adfield(ctx, AD_MCB); f.Mu_b = adfield(ctx, AD_MUB); f.Ml_b = adfield(ctx, AD_MLB);
+      for (int q = 0; q < 3; ++q) f.hv_b[q] = hvb[q];
+      x.run(f, ncols(d));
+    };
+    if (d.kmax - d.kmin == 39) vert(ViMainVA<39>{});
+    else vert(ViMainVA<0>{});
   }
   {
     ViMainHA f;
@@ -151,14 +154,14 @@ void small_step_adj(X& x, const Context& ctx, const real* prog, real* const ck[6
   }
   halo_adj(x, ctx, dvb, 3);
   real* ddvb[3] = {afield(ctx, A_DDVB), afield(ctx, A_DDVB + 1), afield(ctx, A_DDVB + 2)};
+  zero_fields(x, ddvb[0], 3, NF);
   {
     Path1A f;
     f.d = d; f.c = ctx.c; f.G = ctx.G;
     f.dt = (real)dts;
     f.prog = prog;
-    for (int q = 0; q < 6; ++q) { f.ps[q] = ck[q]; f.ps_b[q] = psb2[q]; }
-    f.gT = gT;
-    for (int q = 0; q < 3; ++q) { f.dd[q] = ddv + q * NF; f.dv_b[q] = dvb[q]; f.dd_b[q] = ddvb[q]; }
+    for (int q = 0; q < 6; ++q) f.ps_b[q] = psb2[q];
+    for (int q = 0; q < 3; ++q) { f.dv[q] = dvn[q]; f.dv_b[q] = dvb[q]; f.dd_b[q] = ddvb[q]; }
     f.prog_b = prog_b; f.gT_b = gTb;
     x.run_pts(f, d);
   }
@@ -252,8 +255,9 @@ int dyn_step_adj(X& x, const Context& ctx, const real* prog_in, const real* prog
   real* P = afield(ctx, A_PROG);
   real* prog0 = ctx.field(W_PROG0);
   auto ck_slot = [&](int i, real* f[6]) {
-    for (int q = 0; q < 6; ++q) f[q] = afield(ctx, A_CK + 6 * i + q);
+    for (int q = 0; q < 6; ++q) f[q] = afield(ctx, A_CK + kSlot * i + q);
   };
+  auto slot_hv = [&](int i) { return afield(ctx, A_CK + kSlot * i + 6); };
   // forward sweep: the inputs of stages 0..2
   x.d2d(prog0, prog_in, 6 * NF * sizeof(real));
   x.d2d(P, prog_in, 6 * NF * sizeof(real));
@@ -295,10 +299,11 @@ int dyn_step_adj(X& x, const Context& ctx, const real* prog_in, const real* prog
       SmallBufs s;
       for (int q = 0; q < 6; ++q) s.ps[q] = c0[q];
       s.we[0] = c1[0]; s.dv[0] = c1[1]; s.dv[1] = c1[2]; s.dv[2] = c1[3]; s.we[1] = c1[4]; s.we[2] = c1[5];
+      s.hv = slot_hv(it);
       small_step(x, ctx, P, s, dts, rweight);
     }
     // adjoint of the stage
-    zero_fields(x, afield(ctx, A_DIAGB), A_CK - A_DIAGB, NF);
+    zero_fields(x, afield(ctx, A_DIAGB), A_TRIW - A_DIAGB, NF);
     x.run(FillZero64{adfield(ctx, AD_MCB)}, 3 * NF);
     real *psb[6], *psb2[6];
     for (int q = 0; q < 6; ++q) {
@@ -317,7 +322,7 @@ int dyn_step_adj(X& x, const Context& ctx, const real* prog_in, const real* prog
       ck_slot(it, c0);
       ck_slot(it + 1, c1);
       zero_fields(x, psb2[0], 6, NF);
-      small_step_adj(x, ctx, P, c0, c1, dts, prog_b, psb, psb2);
+      small_step_adj(x, ctx, P, c0, c1, slot_hv(it), dts, prog_b, psb, psb2);
       for (int q = 0; q < 6; ++q) {
         real* t = psb[q];
         psb[q] = psb2[q];

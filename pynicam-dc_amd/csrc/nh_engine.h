@@ -469,6 +469,8 @@ struct SmallBufs {
   real* ps[6];
   real* dv[3];
   real* we[3];
+  real* ddv = nullptr;   // damping force of ps (3 fields); default the step workspace
+  real* hv = nullptr;    // divergences of dv (3 fields); default the step workspace
 };
 
 // large-step part of RK stage nl: diagnostics and boundary values (in place
@@ -528,15 +530,15 @@ void stage_large(X& x, const Context& ctx, real* prog, const real* rhogq, double
   }
 }
 
-// the horizontal divergences of the small-step momentum dv (ViMainH) into W_HV
+// the horizontal divergences of the small-step momentum dv (ViMainH) into hv (3 fields)
 template <class X>
-void small_hv(X& x, const Context& ctx, const real* prog, real* const dv[3]) {
+void small_hv(X& x, const Context& ctx, const real* prog, real* const dv[3], real* hv) {
   ViMainH f;
   f.d = ctx.d; f.c = ctx.c; f.G = ctx.G;
   f.prog = prog;
   for (int q = 0; q < 3; ++q) f.dv[q] = dv[q];
   f.eth = ctx.field(W_ETH); f.ethtot = ctx.field(W_ETHTOT); f.gT = ctx.field(W_GT);
-  for (int q = 0; q < 3; ++q) f.hv[q] = ctx.field(W_HV + q);
+  for (int q = 0; q < 3; ++q) f.hv[q] = hv + q * ctx.d.NF;
   run_staged(x, f);
 }
 
@@ -548,7 +550,8 @@ void small_step(X& x, const Context& ctx, const real* prog, const SmallBufs& s, 
   const long NF = d.NF;
   real* gT = ctx.field(W_GT);
   real* pm = ctx.field(W_PM);
-  real* ddv = ctx.field(W_DDV);
+  real* ddv = s.ddv ? s.ddv : ctx.field(W_DDV);
+  real* hvb = s.hv ? s.hv : ctx.field(W_HV);
   divdamp(x, ctx, s.ps[1], s.ps[2], s.ps[3], s.ps[4], ddv, ddv + NF, ddv + 2 * NF);
   {
     Path1P f;
@@ -563,8 +566,8 @@ void small_step(X& x, const Context& ctx, const real* prog, const SmallBufs& s, 
     run_staged(x, f);
   }
   halo(x, ctx, s.dv, 3);
-  small_hv(x, ctx, prog, s.dv);
-  real* hv[3] = {ctx.field(W_HV), ctx.field(W_HV + 1), ctx.field(W_HV + 2)};
+  small_hv(x, ctx, prog, s.dv, hvb);
+  real* hv[3] = {hvb, hvb + NF, hvb + 2 * NF};
   {
     // the z40 column (39 unknowns) keeps the sweep in registers
     auto solve = [&](auto f) {
