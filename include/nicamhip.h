@@ -232,6 +232,21 @@ int nh_dyn_step_adj(nh_context* ctx, const double* prog, const double* progq, in
 int nh_dyn_step_adj_f32(nh_context* ctx, const float* prog, const float* progq, int32_t nq, double dtl,
                         int32_t sstep_max, float* prog_b, void* stream);
 
+/* The same pair with the stage inputs recorded (no counterpart in the
+ * reference): nh_dyn_step_record is nh_dyn_step that also writes the input
+ * state of each of the three RK stages to stage_in (3 x 6 fields of NF
+ * values); nh_dyn_step_adj_rec is nh_dyn_step_adj from those records (the
+ * first six fields being the step's input state), which saves it the
+ * recomputation of stages 0 and 1. */
+int nh_dyn_step_record(nh_context* ctx, double* prog, double* progq, int32_t nq, double dtl, int32_t sstep_max,
+                       double* stage_in, void* stream);
+int nh_dyn_step_record_f32(nh_context* ctx, float* prog, float* progq, int32_t nq, double dtl, int32_t sstep_max,
+                           float* stage_in, void* stream);
+int nh_dyn_step_adj_rec(nh_context* ctx, const double* stage_in, const double* progq, int32_t nq, double dtl,
+                        int32_t sstep_max, double* prog_b, void* stream);
+int nh_dyn_step_adj_rec_f32(nh_context* ctx, const float* stage_in, const float* progq, int32_t nq, double dtl,
+                            int32_t sstep_max, float* prog_b, void* stream);
+
 /* Surface pressure per column (Bndc.BNDCND_pre_sfc, nhm/share/mod_bndcnd.py:521-555,
  * with a flat surface z_srf = 0): density extrapolated quadratically from the
  * lowest three levels, hydrostatic pressure from kmin.  ps: nreg + 12 doubles

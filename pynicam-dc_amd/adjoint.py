@@ -4,10 +4,11 @@ dynamics step behind ``torch.autograd.Function``.
 The reference has no adjoint; the forward operator differentiated here is
 ``Dyn.dynamics_step`` (nhm/dynamics/mod_dynamics.py:1725), run through
 ``HipDynamics.dynamics_step``.  ``forecast(dyn, prog, nsteps)`` advances a
-differentiable copy of the state; its backward pass calls the library's
-transposed step (``nh_dyn_step_adj``) once per step, newest first, from the
-checkpoint of that step's input state (per-step checkpointing; each adjoint
-call recomputes its step inside, with stage and small-step checkpoints).
+differentiable copy of the state, recording the input of every RK stage
+(``nh_dyn_step_record``); its backward pass calls the library's transposed
+step (``nh_dyn_step_adj_rec``) once per step, newest first, from those
+records (per-step checkpointing; each adjoint call recomputes its stages'
+small steps inside, keeping each small step's split state).
 ``sfc_pressure(dyn, prog)`` is the surface-pressure diagnostic
 (Bndc.BNDCND_pre_sfc, nhm/share/mod_bndcnd.py:521) with its transpose, and
 ``box_mask`` selects the cells of a latitude/longitude box, so that
@@ -52,8 +53,10 @@ def _make_functions():
             dyn.prog, dyn.progq = work, progq
             try:
                 for _ in range(nsteps):
-                    checkpoints.append((work.clone(), progq.clone()))
-                    dyn.dynamics_step()
+                    stages = dyn.ctx.empty(18)
+                    q_in = progq.clone()
+                    dyn.dynamics_step_record(stages)
+                    checkpoints.append((stages, q_in))
             finally:
                 dyn.prog, dyn.progq = saved_prog, saved_q
             fctx.dyn = dyn
@@ -65,8 +68,8 @@ def _make_functions():
         def backward(fctx, grad):
             dyn = fctx.dyn
             gb = grad.detach().to(dtype=dyn.ctx.dtype).contiguous().clone()
-            for p, q in reversed(fctx.checkpoints):
-                dyn.dynamics_step_adj(p, q, gb)
+            for stages, q in reversed(fctx.checkpoints):
+                dyn.dynamics_step_adj_rec(stages, q, gb)
             fctx.checkpoints = None
             return gb, None, None
 

@@ -194,15 +194,6 @@ NH_HD inline void pole_ring_t(const Geo& G, int lp, const real o[3], real sb[5])
 // ----------------------------------------------------------------------------
 // hyperdiffusion passes
 // ----------------------------------------------------------------------------
-// transpose of out = -lap(in) at a compute point: ib += -lap^T ob
-NH_HD inline void neg_lap_t(const Geo& G, const Dims& d, const Pt& p, real* ib, real ob) {
-  const long a = p.a;
-  if (p.pole) {
-    for (int v = 0; v < 6; ++v) aadd(&ib[a + v], -G.clap_pl[v * 2 + p.lp] * ob);
-    return;
-  }
-  for (int m = 0; m < 7; ++m) aadd(&ib[a + nb_off(m, d.g)], -rc(G.clap, d.nreg, m, p.ci) * ob);
-}
 
 // transpose of tri_grad: ob3[dd] -> sb at the triangle's corners
 NH_HD inline void tri_grad_t(const Geo& G, long nreg, long cj, int T, real* sb, long a, int g, const real ob3[3]) {
@@ -221,25 +212,52 @@ NH_HD inline void tri_grad_t(const Geo& G, long nreg, long cj, int T, real* sb, 
   aadd(&sb[(T == 0) ? a + 1 + g : a + g], b2);
 }
 
-// first pass: out[q] = -lap(in[q]) at the compute points
-struct HDiffPass1A {
-  static constexpr const char* kName = "hdiff_pass1_adj";
+// ----------------------------------------------------------------------------
+// gather-form helpers: the output points whose stencil reads point x
+// ----------------------------------------------------------------------------
+// regional point x = (i, j) of a region is neighbour m of the output point
+// (i - di_m, j - dj_m); that point computes when it is interior
+NH_HD inline bool src_of(const Dims& d, const Pt& x, int m, long* a, long* ci) {
+  const int di = ((4201 >> (2 * m)) & 3) - 1, dj = ((421 >> (2 * m)) & 3) - 1;
+  const int i = x.i - di, j = x.j - dj;
+  if (i < 1 || i > d.g - 2 || j < 1 || j > d.g - 2) return false;
+  const long o = (long)di + (long)dj * d.g;
+  *a = x.a - o;
+  *ci = x.ci - o;
+  return true;
+}
+
+// -lap transposed, gather form: ib[q](x) += -sum over the compute points a
+// whose stencil holds x of clap(a, m) ob[q](a) (both hyperdiffusion passes)
+struct NegLapGA {
+  static constexpr const char* kName = "hdiff_lap_adj";
   Dims d; Geo G;
-  real* ob[6];
+  int nq;
+  const real* ob[6];
   real* ib[6];
   NH_HD void operator()(int plane, int x) const {
     Pt p;
-    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
-    for (int q = 0; q < 6; ++q) {
-      const real o = ob[q][p.a];
-      ob[q][p.a] = real(0.0);
-      neg_lap_t(G, d, p, ib[q], o);
+    if (!pt_of(d, plane, x, p)) return;
+    const long a = p.a;
+    real acc[6] = {real(0.0), real(0.0), real(0.0), real(0.0), real(0.0), real(0.0)};
+    if (p.pole) {
+      const real cl = G.clap_pl[p.v * 2 + p.lp];
+      for (int q = 0; q < nq; ++q) acc[q] = cl * ob[q][a - p.v];
+    } else {
+      for (int m = 0; m < 7; ++m) {
+        long ao, co;
+        if (!src_of(d, p, m, &ao, &co)) continue;
+        const real cl = rc(G.clap, d.nreg, m, co);
+        for (int q = 0; q < nq; ++q) acc[q] += cl * ob[q][ao];
+      }
     }
+    for (int q = 0; q < nq; ++q) ib[q][a] += -acc[q];
   }
 };
 
-// second pass: -lap of the four velocity fields, -Kh-weighted diffusion of the
-// temperature and density fields with Kh = rhog * fac * GAMMA_H
+// second pass, diffused fields: -Kh-weighted diffusion of the temperature and
+// density fields with Kh = rhog * fac * GAMMA_H (the four Laplacian fields
+// go through NegLapGA)
 struct HDiffPass2A {
   static constexpr const char* kName = "hdiff_pass2_adj";
   Dims d; Consts c; Geo G;
@@ -252,11 +270,6 @@ struct HDiffPass2A {
     Pt p;
     if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
     const long a = p.a;
-    for (int q = 0; q < 4; ++q) {
-      const real o = ob[q][a];
-      ob[q][a] = real(0.0);
-      neg_lap_t(G, d, p, ib[q], o);
-    }
     for (int f = 0; f < 2; ++f) {
       const real o = -ob[4 + f][a];
       ob[4 + f][a] = real(0.0);
@@ -688,20 +701,6 @@ struct EthTotA {
   }
 };
 
-// ----------------------------------------------------------------------------
-// gather-form helpers: the output points whose stencil reads point x
-// ----------------------------------------------------------------------------
-// regional point x = (i, j) of a region is neighbour m of the output point
-// (i - di_m, j - dj_m); that point computes when it is interior
-NH_HD inline bool src_of(const Dims& d, const Pt& x, int m, long* a, long* ci) {
-  const int di = ((4201 >> (2 * m)) & 3) - 1, dj = ((421 >> (2 * m)) & 3) - 1;
-  const int i = x.i - di, j = x.j - dj;
-  if (i < 1 || i > d.g - 2 || j < 1 || j > d.g - 2) return false;
-  const long o = (long)di + (long)dj * d.g;
-  *a = x.a - o;
-  *ci = x.ci - o;
-  return true;
-}
 
 // ----------------------------------------------------------------------------
 // small step path1 (Path1P), gather form: the thread of point x adds the
@@ -927,10 +926,14 @@ struct ViMainVA {
       u_prev = u;
       mu_m = Mu[a];
     }
-    // back substitution from the top; per level the bars of S = y and of the band
+    // back substitution from the top; per level the bars of S = y and of the
+    // band.  The source at level k also reads level k-1: those bars are
+    // carried to the next (lower) level in registers, so each level's bars
+    // are written once.
     double y = u_prev, x_up = 0.0;
     double xk = (double)x[a];
     double gm_next = 0.0;
+    double cE = 0.0, cH1 = 0.0, cG = 0.0, cT0 = 0.0, cH0 = 0.0;
 #pragma unroll kUnroll
     for (int n = N - 1; n >= 0; --n) {
       if (n < N - 1) y = ((NK > 0) ? ub[NK > 0 ? n : 0] : U[a]) - gm_next * y;
@@ -944,23 +947,29 @@ struct ViMainVA {
       if (n <= N - 2) Mu_b[a] += -y * x_up;
       const double s = y * CVovRt2;
       const double Ab = s, Bb = -s * dt * V.rdgzh[k], Cb = -s * dt * C.GRAV;
+      const double grk = Cb * V.afact[k] * dt, grm = Cb * V.bfact[k] * dt;
       ps_b[I_RHOGW][a] += (real)(Ab * C.ALPHA);
       gT_b[4 * NF + a] += (real)(Ab * dt);
-      ps_b[I_RHOGE][a] += (real)(Bb * RovCV);
-      hv_b[1][a] += (real)(Bb * dt);
-      ps_b[I_RHOGE][am] += (real)(-Bb * RovCV);
-      hv_b[1][am] += (real)(-Bb * dt);
-      const double grk = Cb * V.afact[k] * dt, grm = Cb * V.bfact[k] * dt;
-      ps_b[I_RHOG][a] += (real)(Cb * V.afact[k]);
-      ps_b[I_RHOG][am] += (real)(Cb * V.bfact[k]);
-      gT_b[a] += (real)grk;
-      hv_b[0][a] += (real)(-grk);
-      gT_b[am] += (real)grm;
-      hv_b[0][am] += (real)(-grm);
+      ps_b[I_RHOGE][a] += (real)(Bb * RovCV + cE);
+      hv_b[1][a] += (real)(Bb * dt + cH1);
+      ps_b[I_RHOG][a] += (real)(Cb * V.afact[k] + cG);
+      gT_b[a] += (real)(grk + cT0);
+      hv_b[0][a] += (real)(-grk + cH0);
+      cE = -Bb * RovCV;
+      cH1 = -Bb * dt;
+      cG = Cb * V.bfact[k];
+      cT0 = grm;
+      cH0 = -grm;
       x_up = xk;
       xk = x_dn;
       a = am;
     }
+    // level kmin: the bars the lowest source sent down
+    ps_b[I_RHOGE][a] += (real)cE;
+    hv_b[1][a] += (real)cH1;
+    ps_b[I_RHOG][a] += (real)cG;
+    gT_b[a] += (real)cT0;
+    hv_b[0][a] += (real)cH0;
   }
 };
 
@@ -969,8 +978,23 @@ struct ViMainVA {
 // rhoge splits (we[0], we[2], consumed) -> inputs; we_b[1] receives the bar
 // of the new rhogw split it read
 // ----------------------------------------------------------------------------
+// the point of a compute column at level k
+NH_HD inline Pt pt_at(const Dims& d, const Cell& cl, int k) {
+  Pt p;
+  p.pole = cl.pole; p.inner = cl.inner;
+  p.i = cl.i; p.j = cl.j; p.l = cl.l; p.v = cl.v; p.lp = cl.lp; p.k = k;
+  p.ks = cl.ks;
+  p.a = cl.base + (long)k * cl.ks;
+  p.ci = cl.ci;
+  (void)d;
+  return p;
+}
+
 struct ViMainPostA {
   static constexpr const char* kName = "vi_main_post_adj";
+  // the bar fields it adds to: PROG 0..4, split 0..5, dv 0..2, the new rhogw
+  // split, hv 0 and 2, gT 0, ethtot, getot
+  enum { F_P = 0, F_S = 5, F_D = 11, F_W1 = 14, F_H0 = 15, F_H2 = 16, F_G0 = 17, F_ET = 18, F_GE = 19, F_N = 20 };
   Dims d; Consts c; Geo G;
   real dt;
   const real* prog;
@@ -988,12 +1012,22 @@ struct ViMainPostA {
   real* gT_b;
   real* getot_b;
   real* hv_b[3];
-  NH_HD void operator()(int plane, int x) const {
-    Pt p;
-    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
-    point(p, [](real* q, real v) { aadd(q, v); });
+  NH_HD real* fptr(int f) const {
+    if (f < F_S) return prog_b + (long)f * d.NF;
+    if (f < F_D) return ps_b[f - F_S];
+    if (f < F_W1) return dv_b[f - F_D];
+    switch (f) {
+      case F_W1: return we_b[1];
+      case F_H0: return hv_b[0];
+      case F_H2: return hv_b[2];
+      case F_G0: return gT_b;
+      case F_ET: return ethtot_b;
+      default: return getot_b;
+    }
   }
-  // the transpose at one compute point; add(address, value) accumulates a bar
+  // the transpose at one compute point; add(field, dk, value) accumulates the
+  // bar of field at level k + dk of the point's column (dk = -1 .. 2); the
+  // kernel adds them atomically (neighbouring levels' points share targets)
   template <class ADD>
   NH_HD void point(const Pt& p, ADD add) const {
     const long NF = d.NF, ks = p.ks, a = p.a;
@@ -1001,8 +1035,6 @@ struct ViMainPostA {
     const long b = a - (long)k * ks;
     auto at = [&](int kk) { return b + (long)kk * ks; };
     const real b0 = we_b[0][a], b2 = we_b[2][a];
-    we_b[0][a] = real(0.0);
-    we_b[2][a] = real(0.0);
     const real* pg = ps[I_RHOG];
     const real* rw1 = we1;
     auto rs1 = [&](int kk) {
@@ -1014,47 +1046,51 @@ struct ViMainPostA {
       }
       return pg[at(kk)] + (gT[0 * NF + at(kk)] + drhog2) * dt;
     };
-    auto rs1_t = [&](int kk, real rb) {
-      add(&ps_b[I_RHOG][at(kk)], rb);
-      add(&gT_b[at(kk)], rb * dt);
+    // dk: level of the rs1 argument relative to k
+    auto rs1_t = [&](int dk, real rb) {
+      const int kk = k + dk;
+      add(F_S + I_RHOG, dk, rb);
+      add(F_G0, dk, rb * dt);
       if (kk >= kmin && kk <= kmax) {
         const real d2b = rb * dt;
-        add(&hv_b[0][at(kk)], -d2b);
-        if (kk + 1 <= kmax) add(&we_b[1][at(kk + 1)], -d2b * G.rdgz[kk]);
-        if (kk >= kmin + 1) add(&we_b[1][at(kk)], d2b * G.rdgz[kk]);
+        add(F_H0, dk, -d2b);
+        if (kk + 1 <= kmax) add(F_W1, dk + 1, -d2b * G.rdgz[kk]);
+        if (kk >= kmin + 1) add(F_W1, dk, d2b * G.rdgz[kk]);
       }
     };
-    const int ke = (k < kmin) ? kmin : ((k > kmax) ? kmax : k);
-    add(&ps_b[I_RHOGE][at(ke)], b2);
-    add(&getot_b[a], b2 * dt);
+    // the energy split of the nearest physical level
+    if (k < kmin) add(F_S + I_RHOGE, 1, b2);
+    else if (k > kmax) add(F_S + I_RHOGE, -1, b2);
+    else add(F_S + I_RHOGE, 0, b2);
+    add(F_GE, 0, b2 * dt);
     const real dtotb = b2 * dt;
-    add(&ps_b[I_RHOG][a], b2 * G.phi[k]);
-    rs1_t(k, b0 - b2 * G.phi[k]);
+    add(F_S + I_RHOG, 0, b2 * G.phi[k]);
+    rs1_t(0, b0 - b2 * G.phi[k]);
     if (k >= kmin && k <= kmax) {
-      add(&hv_b[2][a], -dtotb);
+      add(F_H2, 0, -dtotb);
       const real hib = -dtotb * G.rdgz[k], lob = dtotb * G.rdgz[k];
       if (k + 1 <= kmax) {
         const real W1 = prog[4 * NF + a + ks] + rw1[a + ks];
         const real E1 = G.afact[k + 1] * ethtot[a + ks] + G.bfact[k + 1] * ethtot[a];
-        add(&prog_b[4 * NF + a + ks], hib * E1);
-        add(&we_b[1][a + ks], hib * E1);
-        add(&ethtot_b[a + ks], hib * W1 * G.afact[k + 1]);
-        add(&ethtot_b[a], hib * W1 * G.bfact[k + 1]);
+        add(F_P + 4, 1, hib * E1);
+        add(F_W1, 1, hib * E1);
+        add(F_ET, 1, hib * W1 * G.afact[k + 1]);
+        add(F_ET, 0, hib * W1 * G.bfact[k + 1]);
       }
       if (k >= kmin + 1) {
         const real W0 = prog[4 * NF + a] + rw1[a];
         const real E0 = G.afact[k] * ethtot[a] + G.bfact[k] * ethtot[a - ks];
-        add(&prog_b[4 * NF + a], lob * E0);
-        add(&we_b[1][a], lob * E0);
-        add(&ethtot_b[a], lob * W0 * G.afact[k]);
-        add(&ethtot_b[a - ks], lob * W0 * G.bfact[k]);
+        add(F_P + 4, 0, lob * E0);
+        add(F_W1, 0, lob * E0);
+        add(F_ET, 0, lob * W0 * G.afact[k]);
+        add(F_ET, -1, lob * W0 * G.bfact[k]);
       }
     }
     // kinetic energies before (PROG + PROG_split) and after (new split) the step
     auto g10 = [&](int kk, int q) { return prog[q * NF + at(kk)] + ps[q][at(kk)]; };
     auto add10 = [&](int kk, int q, real v) {
-      add(&prog_b[q * NF + at(kk)], v);
-      add(&ps_b[q][at(kk)], v);
+      add(F_P + q, kk - k, v);
+      add(F_S + q, kk - k, v);
     };
     rhogkin_k_t(G, kmin, kmax, k, g10, add10, b2);
     auto g11 = [&](int kk, int q) {
@@ -1063,12 +1099,18 @@ struct ViMainPostA {
       return prog[q * NF + at(kk)] + dv[q - 1][at(kk)];
     };
     auto add11 = [&](int kk, int q, real v) {
-      add(&prog_b[q * NF + at(kk)], v);
-      if (q == 0) rs1_t(kk, v);
-      else if (q == 4) add(&we_b[1][at(kk)], v);
-      else add(&dv_b[q - 1][at(kk)], v);
+      add(F_P + q, kk - k, v);
+      if (q == 0) rs1_t(kk - k, v);
+      else if (q == 4) add(F_W1, kk - k, v);
+      else add(F_D + q - 1, kk - k, v);
     };
     rhogkin_k_t(G, kmin, kmax, k, g11, add11, -b2);
+  }
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    const long a = p.a, ks = p.ks;
+    point(p, [&](int f, int dk, real v) { aadd(&fptr(f)[a + (long)dk * ks], v); });
   }
 };
 
@@ -1375,17 +1417,6 @@ struct SfcPresP {
   }
 };
 
-// the point of a compute column at level k
-NH_HD inline Pt pt_at(const Dims& d, const Cell& cl, int k) {
-  Pt p;
-  p.pole = cl.pole; p.inner = cl.inner;
-  p.i = cl.i; p.j = cl.j; p.l = cl.l; p.v = cl.v; p.lp = cl.lp; p.k = k;
-  p.ks = cl.ks;
-  p.a = cl.base + (long)k * cl.ks;
-  p.ci = cl.ci;
-  (void)d;
-  return p;
-}
 
 
 // ----------------------------------------------------------------------------

@@ -216,15 +216,22 @@ void stage_large_adj(X& x, const Context& ctx, const real* prog, const real* rho
     f.rhog = prog;
     f.rhog_b = prog_b;
     x.run_pts(f, d);
+    NegLapGA l;
+    l.d = d; l.G = ctx.G; l.nq = 4;
+    for (int q = 0; q < 4; ++q) {
+      l.ob[q] = afield(ctx, A_HDBB + q);
+      l.ib[q] = afield(ctx, A_HDAB + q);
+    }
+    x.run_pts(l, d);
   }
   halo_block_adj(x, ctx, afield(ctx, A_HDAB), 6);
   {
-    HDiffPass1A f;
-    f.d = d; f.G = ctx.G;
-    for (int q = 0; q < 6; ++q) f.ob[q] = afield(ctx, A_HDAB + q);
-    f.ib[0] = diagb + I_VX * NF; f.ib[1] = diagb + I_VY * NF; f.ib[2] = diagb + I_VZ * NF;
-    f.ib[3] = diagb + I_W * NF; f.ib[4] = diagb + I_TEM * NF; f.ib[5] = prog_b;
-    x.run_pts(f, d);
+    NegLapGA l;
+    l.d = d; l.G = ctx.G; l.nq = 6;
+    for (int q = 0; q < 6; ++q) l.ob[q] = afield(ctx, A_HDAB + q);
+    l.ib[0] = diagb + I_VX * NF; l.ib[1] = diagb + I_VY * NF; l.ib[2] = diagb + I_VZ * NF;
+    l.ib[3] = diagb + I_W * NF; l.ib[4] = diagb + I_TEM * NF; l.ib[5] = prog_b;
+    x.run_pts(l, d);
   }
   {
     AdvMomA f;
@@ -243,11 +250,14 @@ void stage_large_adj(X& x, const Context& ctx, const real* prog, const real* rho
   }
 }
 
+// stage_in (optional): the stage inputs recorded by the forward step (3 x 6
+// fields, the first being prog_in); the forward sweep is then skipped
 template <class X>
 int dyn_step_adj(X& x, const Context& ctx, const real* prog_in, const real* progq, int nq, double dtl,
-                 int sstep_max, real* prog_b) {
+                 int sstep_max, real* prog_b, const real* stage_in = nullptr) {
   const Dims& d = ctx.d;
   const long NF = d.NF;
+  if (stage_in) prog_in = stage_in;
   if (nq < 1 || !prog_in || !progq || !prog_b) return NH_ERR_ARG;
   if (!ctx.aws || ctx.adj_ck < sstep_max + 1) return NH_ERR_STATE;
   const double dts = dtl / (double)sstep_max;
@@ -258,10 +268,13 @@ int dyn_step_adj(X& x, const Context& ctx, const real* prog_in, const real* prog
     for (int q = 0; q < 6; ++q) f[q] = afield(ctx, A_CK + kSlot * i + q);
   };
   auto slot_hv = [&](int i) { return afield(ctx, A_CK + kSlot * i + 6); };
-  // forward sweep: the inputs of stages 0..2
+  auto stage_input = [&](int nl) -> const real* {
+    return stage_in ? stage_in + (long)nl * 6 * NF : afield(ctx, A_SC + 6 * nl);
+  };
+  // forward sweep: the inputs of stages 0..2 (unless recorded)
   x.d2d(prog0, prog_in, 6 * NF * sizeof(real));
   x.d2d(P, prog_in, 6 * NF * sizeof(real));
-  for (int nl = 0; nl < 3; ++nl) {
+  for (int nl = 0; nl < 3 && !stage_in; ++nl) {
     x.d2d(afield(ctx, A_SC + 6 * nl), P, 6 * NF * sizeof(real));
     if (nl == 2) break;
     stage_large(x, ctx, P, progq, dts);
@@ -286,7 +299,7 @@ int dyn_step_adj(X& x, const Context& ctx, const real* prog_in, const real* prog
   for (int nl = 2; nl >= 0; --nl) {
     if (nl < 2) halo_block_adj(x, ctx, prog_b, 6);
     // recompute the stage, keeping every small step's split state
-    x.d2d(P, afield(ctx, A_SC + 6 * nl), 6 * NF * sizeof(real));
+    x.d2d(P, stage_input(nl), 6 * NF * sizeof(real));
     stage_large(x, ctx, P, progq, dts);
     real* ck[6];
     ck_slot(0, ck);

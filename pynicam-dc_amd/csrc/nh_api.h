@@ -90,14 +90,14 @@ inline int gradient(nh_context* ctx, const void* scl, void* gx, void* gy, void* 
 }
 
 inline int dyn_step(nh_context* ctx, void* prog, void* progq, int32_t nq, double dtl, int32_t sstep_max,
-                    void* stream, int f32) {
-  const char* where = f32 ? "nh_dyn_step_f32" : "nh_dyn_step";
+                    void* stream, int f32, void* record = nullptr, const char* name = nullptr) {
+  const char* where = name ? name : (f32 ? "nh_dyn_step_f32" : "nh_dyn_step");
   if (int rc = check_ready(ctx, where, f32, true)) return rc;
   if (!prog || !progq || nq < 1) return fail(NH_ERR_ARG, where, "need prog and at least one tracer field");
   if (sstep_max < 3 || sstep_max % 6 != 0 || !(dtl > 0.0))
     return fail(NH_ERR_ARG, where, "sstep_max must be a positive multiple of 6 and dtl > 0");
   std::string err;
-  return done(NH_DISPATCH(ctx, v_dyn_step, prog, progq, nq, dtl, sstep_max, stream, &err), where, err);
+  return done(NH_DISPATCH(ctx, v_dyn_step, prog, progq, nq, dtl, sstep_max, record, stream, &err), where, err);
 }
 
 inline int tracer_step(nh_context* ctx, const void* prog, void* progq, int32_t nq, double dtl, void* stream,
@@ -132,15 +132,17 @@ inline int budget(nh_context* ctx, const void* prog, const void* progq, int32_t 
 }
 
 inline int dyn_step_adj(nh_context* ctx, const void* prog, const void* progq, int32_t nq, double dtl,
-                        int32_t sstep_max, void* prog_b, void* stream, int f32) {
-  const char* where = f32 ? "nh_dyn_step_adj_f32" : "nh_dyn_step_adj";
+                        int32_t sstep_max, void* prog_b, void* stream, int f32, const void* stage_in = nullptr,
+                        const char* name = nullptr) {
+  const char* where = name ? name : (f32 ? "nh_dyn_step_adj_f32" : "nh_dyn_step_adj");
   if (int rc = check_ready(ctx, where, f32, true)) return rc;
   if (!prog || !progq || nq < 1 || !prog_b)
     return fail(NH_ERR_ARG, where, "need prog, prog_b and at least one tracer field");
   if (sstep_max < 3 || sstep_max % 6 != 0 || !(dtl > 0.0))
     return fail(NH_ERR_ARG, where, "sstep_max must be a positive multiple of 6 and dtl > 0");
   std::string err;
-  return done(NH_DISPATCH(ctx, v_dyn_step_adj, prog, progq, nq, dtl, sstep_max, prog_b, stream, &err), where, err);
+  return done(NH_DISPATCH(ctx, v_dyn_step_adj, prog, progq, nq, dtl, sstep_max, prog_b, stage_in, stream, &err),
+              where, err);
 }
 
 inline int sfc_pressure(nh_context* ctx, const void* prog, const void* progq, double* ps, const double* ps_b,
@@ -305,6 +307,27 @@ int nh_dyn_step_adj(nh_context* ctx, const double* prog, const double* progq, in
 int nh_dyn_step_adj_f32(nh_context* ctx, const float* prog, const float* progq, int32_t nq, double dtl,
                         int32_t sstep_max, float* prog_b, void* stream) {
   return nh::dyn_step_adj(ctx, prog, progq, nq, dtl, sstep_max, prog_b, stream, 1);
+}
+
+int nh_dyn_step_record(nh_context* ctx, double* prog, double* progq, int32_t nq, double dtl, int32_t sstep_max,
+                       double* stage_in, void* stream) {
+  if (!stage_in) return nh::fail(NH_ERR_ARG, "nh_dyn_step_record", "null stage_in");
+  return nh::dyn_step(ctx, prog, progq, nq, dtl, sstep_max, stream, 0, stage_in, "nh_dyn_step_record");
+}
+int nh_dyn_step_record_f32(nh_context* ctx, float* prog, float* progq, int32_t nq, double dtl, int32_t sstep_max,
+                           float* stage_in, void* stream) {
+  if (!stage_in) return nh::fail(NH_ERR_ARG, "nh_dyn_step_record_f32", "null stage_in");
+  return nh::dyn_step(ctx, prog, progq, nq, dtl, sstep_max, stream, 1, stage_in, "nh_dyn_step_record_f32");
+}
+int nh_dyn_step_adj_rec(nh_context* ctx, const double* stage_in, const double* progq, int32_t nq, double dtl,
+                        int32_t sstep_max, double* prog_b, void* stream) {
+  return nh::dyn_step_adj(ctx, stage_in, progq, nq, dtl, sstep_max, prog_b, stream, 0, stage_in,
+                          "nh_dyn_step_adj_rec");
+}
+int nh_dyn_step_adj_rec_f32(nh_context* ctx, const float* stage_in, const float* progq, int32_t nq, double dtl,
+                            int32_t sstep_max, float* prog_b, void* stream) {
+  return nh::dyn_step_adj(ctx, stage_in, progq, nq, dtl, sstep_max, prog_b, stream, 1, stage_in,
+                          "nh_dyn_step_adj_rec_f32");
 }
 
 int nh_sfc_pressure(nh_context* ctx, const double* prog, const double* progq, double* ps, void* stream) {

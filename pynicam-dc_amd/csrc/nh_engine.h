@@ -610,8 +610,10 @@ void prog_update(X& x, const Context& ctx, real* prog, real* const ps[6]) {
   x.run_pts(f, ctx.d);
 }
 
+// record (optional): the input state of each RK stage, 3 x 6 fields, for the adjoint
 template <class X>
-int dyn_step(X& x, const Context& ctx, real* prog, real* progq, int nq, double dtl, int sstep_max) {
+int dyn_step(X& x, const Context& ctx, real* prog, real* progq, int nq, double dtl, int sstep_max,
+             real* record = nullptr) {
   const Dims& d = ctx.d;
   const long NF = d.NF;
   const double dts = dtl / (double)sstep_max;
@@ -620,6 +622,7 @@ int dyn_step(X& x, const Context& ctx, real* prog, real* progq, int nq, double d
   real* prog0 = ctx.field(W_PROG0);
   x.d2d(prog0, prog, 6 * NF * sizeof(real));
   for (int nl = 0; nl < 3; ++nl) {
+    if (record) x.d2d(record + (long)nl * 6 * NF, prog, 6 * NF * sizeof(real));
     stage_large(x, ctx, prog, progq, dts);
     // split state ps[0..5], path1 momentum dv[0..2], vi_main output we[0..2]
     SmallBufs s;

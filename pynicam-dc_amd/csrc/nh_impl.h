@@ -117,10 +117,10 @@ int v_gradient(Context* c, const void* scl, void* gx, void* gy, void* gz, void* 
   return 0;
 }
 
-int v_dyn_step(Context* c, void* prog, void* progq, int nq, double dtl, int sstep_max, void* stream,
+int v_dyn_step(Context* c, void* prog, void* progq, int nq, double dtl, int sstep_max, void* record, void* stream,
                std::string* err) {
   NHX x(c->device, stream, &c->prof);
-  int rc = dyn_step(x, *c, (real*)prog, (real*)progq, nq, dtl, sstep_max);
+  int rc = dyn_step(x, *c, (real*)prog, (real*)progq, nq, dtl, sstep_max, (real*)record);
   if (rc == NH_ERR_ARG) return failv(err, rc, "invalid argument");
   if (rc) return failv(err, NH_ERR_DEVICE, x.message());
   return 0;
@@ -151,7 +151,7 @@ int v_budget(Context* c, const void* prog, const void* progq, double lhv, double
 }
 
 int v_dyn_step_adj(Context* c, const void* prog, const void* progq, int nq, double dtl, int sstep_max, void* prog_b,
-                   void* stream, std::string* err) {
+                   const void* stage_in, void* stream, std::string* err) {
   if (sstep_max < 3 || sstep_max > 64) return failv(err, NH_ERR_ARG, "sstep_max out of range for the adjoint");
   if (!c->halo_adj_ok)
     return failv(err, NH_ERR_STATE, "halo map has a destination listed twice or used as a source");
@@ -174,7 +174,8 @@ int v_dyn_step_adj(Context* c, const void* prog, const void* progq, int nq, doub
     x.zero(c->adws, dbytes);
     c->adj_ck = sstep_max + 1;
   }
-  int rc = dyn_step_adj(x, *c, (const real*)prog, (const real*)progq, nq, dtl, sstep_max, (real*)prog_b);
+  int rc = dyn_step_adj(x, *c, (const real*)prog, (const real*)progq, nq, dtl, sstep_max, (real*)prog_b,
+                        (const real*)stage_in);
   if (rc == NH_ERR_ARG) return failv(err, rc, "invalid argument");
   if (rc == NH_ERR_STATE) return failv(err, rc, "adjoint workspace missing");
   if (rc) return failv(err, NH_ERR_DEVICE, x.message());

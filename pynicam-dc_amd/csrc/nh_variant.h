@@ -24,8 +24,8 @@
   int v_divergence(Context* c, const void* vx, const void* vy, const void* vz, void* out, void* stream,     \
                    std::string* err);                                                                       \
   int v_gradient(Context* c, const void* scl, void* gx, void* gy, void* gz, void* stream, std::string* err); \
-  int v_dyn_step(Context* c, void* prog, void* progq, int nq, double dtl, int sstep_max, void* stream,      \
-                 std::string* err);                                                                         \
+  int v_dyn_step(Context* c, void* prog, void* progq, int nq, double dtl, int sstep_max, void* record,      \
+                 void* stream, std::string* err);                                                           \
   int v_tracer_step(Context* c, const void* prog, void* progq, int nq, double dtl, void* stream,            \
                     std::string* err);                                                                      \
   int v_trcadv_winds(Context* c, void* prog, double time, int kind, const double* params, void* stream,     \
@@ -33,7 +33,7 @@
   int v_budget(Context* c, const void* prog, const void* progq, double lhv, double* out, void* stream,      \
                std::string* err);                                                                           \
   int v_dyn_step_adj(Context* c, const void* prog, const void* progq, int nq, double dtl, int sstep_max,      \
-                     void* prog_b, void* stream, std::string* err);                                         \
+                     void* prog_b, const void* stage_in, void* stream, std::string* err);                   \
   int v_sfc_pressure(Context* c, const void* prog, const void* progq, double* ps, const double* ps_b,       \
                      void* prog_b, void* stream, std::string* err);                                         \
   int v_prof_enable(Context* c, bool on, std::string* err);                                                 \

@@ -202,6 +202,25 @@ class HipDynamics:
                                           C.c_void_p(self.progq.data_ptr()), self.progq.shape[0],
                                           self.dtl, self.sstep_max, self.ctx.stream()), "nh_dyn_step")
 
+    def dynamics_step_record(self, stage_in):
+        """``dynamics_step`` that also writes the input of each RK stage to ``stage_in`` (18 fields)."""
+        ctx = self.ctx
+        ctx._check_tensor(stage_in)
+        if stage_in.shape[0] != 18:
+            raise ValueError("stage_in needs 3 x 6 fields")
+        native.check(ctx.lib, ctx.fn("nh_dyn_step_record")(
+            ctx.handle, C.c_void_p(self.prog.data_ptr()), C.c_void_p(self.progq.data_ptr()), self.progq.shape[0],
+            self.dtl, self.sstep_max, C.c_void_p(stage_in.data_ptr()), ctx.stream()), "nh_dyn_step_record")
+
+    def dynamics_step_adj_rec(self, stage_in, progq, prog_b):
+        """``dynamics_step_adj`` from the stage inputs ``dynamics_step_record`` wrote."""
+        ctx = self.ctx
+        for a in (stage_in, progq, prog_b):
+            ctx._check_tensor(a)
+        native.check(ctx.lib, ctx.fn("nh_dyn_step_adj_rec")(
+            ctx.handle, C.c_void_p(stage_in.data_ptr()), C.c_void_p(progq.data_ptr()), progq.shape[0], self.dtl,
+            self.sstep_max, C.c_void_p(prog_b.data_ptr()), ctx.stream()), "nh_dyn_step_adj_rec")
+
     def dynamics_step_adj(self, prog, progq, prog_b):
         """Transpose of one ``dynamics_step`` started from (prog, progq), applied in place to ``prog_b``.
 

@@ -68,6 +68,10 @@ EXPORTS = {
     "nh_precision": (C.c_int32, [C.c_void_p]),
     "nh_dyn_step_adj": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_int32,
                                   C.c_void_p, C.c_void_p]),
+    "nh_dyn_step_record": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_int32,
+                                     C.c_void_p, C.c_void_p]),
+    "nh_dyn_step_adj_rec": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_int32,
+                                      C.c_void_p, C.c_void_p]),
     "nh_sfc_pressure": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "nh_sfc_pressure_adj": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "nh_prof_enable": (C.c_int, [C.c_void_p, C.c_int32]),
@@ -76,7 +80,8 @@ EXPORTS = {
 }
 # the _f32 forms of the field entry points take the same arguments (float fields)
 for _name in ("nh_halo_fill", "nh_laplacian", "nh_divergence", "nh_gradient", "nh_dyn_step", "nh_tracer_step",
-              "nh_trcadv_winds", "nh_budget", "nh_dyn_step_adj", "nh_sfc_pressure", "nh_sfc_pressure_adj"):
+              "nh_trcadv_winds", "nh_budget", "nh_dyn_step_adj", "nh_sfc_pressure", "nh_sfc_pressure_adj",
+              "nh_dyn_step_record", "nh_dyn_step_adj_rec"):
     EXPORTS[_name + "_f32"] = EXPORTS[_name]
 
 PREC_FP64, PREC_MIXED = 0, 1
