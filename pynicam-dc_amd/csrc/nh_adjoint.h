@@ -1026,8 +1026,7 @@ struct ViMainPostA {
     }
   }
   // the transpose at one compute point; add(field, dk, value) accumulates the
-  // bar of field at level k + dk of the point's column (dk = -1 .. 2); the
-  // kernel adds them atomically (neighbouring levels' points share targets)
+  // bar of field at level k + dk of the point's column (dk = -1 .. 2)
   template <class ADD>
   NH_HD void point(const Pt& p, ADD add) const {
     const long NF = d.NF, ks = p.ks, a = p.a;
@@ -1106,11 +1105,15 @@ struct ViMainPostA {
     };
     rhogkin_k_t(G, kmin, kmax, k, g11, add11, -b2);
   }
+  // one launch per level class k mod 4 (color): the points of one launch
+  // write disjoint bars (their targets k-1 .. k+2 of their own column), so
+  // plain adds suffice
+  int color;
   NH_HD void operator()(int plane, int x) const {
     Pt p;
-    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    if (!pt_of(d, plane, x, p) || !pt_compute(p) || (p.k & 3) != color) return;
     const long a = p.a, ks = p.ks;
-    point(p, [&](int f, int dk, real v) { aadd(&fptr(f)[a + (long)dk * ks], v); });
+    point(p, [&](int f, int dk, real v) { fptr(f)[a + (long)dk * ks] += v; });
   }
 };
 
