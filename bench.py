@@ -320,6 +320,8 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure HBM traffic per kernel")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint window at gl06 z40")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the adjoint window at gl06 z40 (both forms) and print its JSON")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the run, copy 512 MiB device-to-device once (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -329,6 +331,23 @@ def main():
         sys.exit(2)
     gl = args.glevel
     forms = ["float64", "mixed"] if args.precision == "both" else [args.precision]
+    if args.adjoint_only:
+        import torch
+        if not torch.cuda.is_available():
+            print("bench.py: no GPU visible", file=sys.stderr)
+            sys.exit(1)
+        hipdyn = pkg("hipdyn")
+        adj, ref = {}, None
+        for f in forms:
+            r = adjoint_leg(hipdyn, f, ref=ref)
+            if f == "float64":
+                ref = r
+            adj[FORMS[f][0]] = r
+        for r in adj.values():
+            r.pop("grad", None)
+        print(json.dumps({"metric": "adjoint/forward step cost at gl06 z40 over 18 steps", "adjoint": adj}),
+              flush=True)
+        return
     main_form = forms[-1]
     traffic, factors = (None, "not measured")
     if not args.no_traffic:
