@@ -216,8 +216,8 @@ def run_form(hipdyn, case, precision: str, steps: int, warmup: int):
 def adjoint_leg(hipdyn, precision: str, nsteps: int = 18, reps: int = 2, ref=None) -> dict:
     """Initial-state sensitivity at gl06 z40 (dt 600 s): J = sum over the 30-50N,
     0-20E box of surface pressure squared after ``nsteps`` steps, its gradient
-    through the adjoint window (per-step checkpoints, recompute inside each
-    adjoint step).  Times the plain forward steps, the checkpointed forward
+    through the adjoint window (per-step tapes of the forward steps; the
+    adjoint steps recompute only the diagnostics).  Times the plain forward steps, the checkpointed forward
     window and the backward pass (best of ``reps``), checks the gradient of the
     FP64 form by a central finite difference along one seeded direction (the
     mixed form's against the FP64 gradient ``ref``), and reports the device
@@ -243,6 +243,8 @@ def adjoint_leg(hipdyn, precision: str, nsteps: int = 18, reps: int = 2, ref=Non
             torch.cuda.synchronize()
             return time.perf_counter() - t0
 
+        mem = {}
+
         def window():
             x = x0.clone().requires_grad_(True)
             torch.cuda.synchronize()
@@ -251,6 +253,11 @@ def adjoint_leg(hipdyn, precision: str, nsteps: int = 18, reps: int = 2, ref=Non
             J = (m * ps * ps).sum()
             torch.cuda.synchronize()
             t1 = time.perf_counter()
+            if "peak" not in mem:
+                # device memory in use with every step's tape alive (cached free blocks released)
+                torch.cuda.empty_cache()
+                free, total = torch.cuda.mem_get_info()
+                mem["peak"] = (total - free) / 1e9
             J.backward()
             torch.cuda.synchronize()
             t2 = time.perf_counter()
@@ -263,7 +270,7 @@ def adjoint_leg(hipdyn, precision: str, nsteps: int = 18, reps: int = 2, ref=Non
         Jv, grad = runs[-1][0], runs[-1][1]
         t_fwd = min(r[2] for r in runs)
         t_bwd = min(r[3] for r in runs)
-        free, total = torch.cuda.mem_get_info()
+        tape_gb = dyn.tape_bytes() / 1e9
         # central difference along a seeded direction scaled per variable
         rng = np.random.default_rng(7)
         sc = x0.abs().amax(dim=1, keepdim=True).clone()
@@ -302,7 +309,7 @@ def adjoint_leg(hipdyn, precision: str, nsteps: int = 18, reps: int = 2, ref=Non
            "adjoint_ms_per_step": 1000.0 * t_bwd / nsteps,
            "adjoint_over_forward": t_bwd / t_plain,
            "J": Jv, **check,
-           "device_mem_in_use_GB": (total - free) / 1e9}
+           "tape_GB_per_step": tape_gb, "peak_device_mem_GB": mem["peak"]}
     print(f"bench: adjoint {precision}: " + json.dumps(res), file=sys.stderr, flush=True)
     res["grad"] = grad.double()
     return res

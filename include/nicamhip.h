@@ -247,6 +247,24 @@ int nh_dyn_step_adj_rec(nh_context* ctx, const double* stage_in, const double* p
 int nh_dyn_step_adj_rec_f32(nh_context* ctx, const float* stage_in, const float* progq, int32_t nq, double dtl,
                             int32_t sstep_max, float* prog_b, void* stream);
 
+/* Taped step and its adjoint (no counterpart in the reference): the tape
+ * holds, per RK stage, its input state and the fields its small steps read
+ * (hyperdiffusion passes, large-step tendencies, total energy, the FP64
+ * tridiagonal matrix) and every small step's split state and divergences;
+ * nh_tape_bytes gives its size for one step (device memory the caller owns).
+ * nh_dyn_step_tape is nh_dyn_step writing its intermediate fields to the tape
+ * instead of the workspace; nh_dyn_step_adj_tape is nh_dyn_step_adj from that
+ * tape, recomputing only the diagnostics. */
+int64_t nh_tape_bytes(const nh_context* ctx, int32_t sstep_max);
+int nh_dyn_step_tape(nh_context* ctx, double* prog, double* progq, int32_t nq, double dtl, int32_t sstep_max,
+                     void* tape, void* stream);
+int nh_dyn_step_tape_f32(nh_context* ctx, float* prog, float* progq, int32_t nq, double dtl, int32_t sstep_max,
+                         void* tape, void* stream);
+int nh_dyn_step_adj_tape(nh_context* ctx, const void* tape, const double* progq, int32_t nq, double dtl,
+                         int32_t sstep_max, double* prog_b, void* stream);
+int nh_dyn_step_adj_tape_f32(nh_context* ctx, const void* tape, const float* progq, int32_t nq, double dtl,
+                             int32_t sstep_max, float* prog_b, void* stream);
+
 /* Surface pressure per column (Bndc.BNDCND_pre_sfc, nhm/share/mod_bndcnd.py:521-555,
  * with a flat surface z_srf = 0): density extrapolated quadratically from the
  * lowest three levels, hydrostatic pressure from kmin.  ps: nreg + 12 doubles

@@ -4,11 +4,12 @@ dynamics step behind ``torch.autograd.Function``.
 The reference has no adjoint; the forward operator differentiated here is
 ``Dyn.dynamics_step`` (nhm/dynamics/mod_dynamics.py:1725), run through
 ``HipDynamics.dynamics_step``.  ``forecast(dyn, prog, nsteps)`` advances a
-differentiable copy of the state, recording the input of every RK stage
-(``nh_dyn_step_record``); its backward pass calls the library's transposed
-step (``nh_dyn_step_adj_rec``) once per step, newest first, from those
-records (per-step checkpointing; each adjoint call recomputes its stages'
-small steps inside, keeping each small step's split state).
+differentiable copy of the state, keeping per step either its tape (every
+intermediate field the transposes read: ``nh_dyn_step_tape``; the default)
+or only its RK-stage inputs (``nh_dyn_step_record``, less memory); its
+backward pass calls the library's transposed step once per step, newest
+first (``nh_dyn_step_adj_tape`` or ``nh_dyn_step_adj_rec``, which then
+recomputes the stages and small steps inside).
 ``sfc_pressure(dyn, prog)`` is the surface-pressure diagnostic
 (Bndc.BNDCND_pre_sfc, nhm/share/mod_bndcnd.py:521) with its transpose, and
 ``box_mask`` selects the cells of a latitude/longitude box, so that
@@ -44,7 +45,7 @@ def _make_functions():
         """prog -> state after ``nsteps`` dynamics steps (tracers advance alongside)."""
 
         @staticmethod
-        def forward(fctx, prog, dyn, nsteps):
+        def forward(fctx, prog, dyn, nsteps, store):
             _check_state(dyn, prog)
             saved_prog, saved_q = dyn.prog, dyn.progq
             progq = dyn.progq.clone()
@@ -53,13 +54,18 @@ def _make_functions():
             dyn.prog, dyn.progq = work, progq
             try:
                 for _ in range(nsteps):
-                    stages = dyn.ctx.empty(18)
                     q_in = progq.clone()
-                    dyn.dynamics_step_record(stages)
-                    checkpoints.append((stages, q_in))
+                    if store == "tape":
+                        rec = dyn.new_tape()
+                        dyn.dynamics_step_tape(rec)
+                    else:
+                        rec = dyn.ctx.empty(18)
+                        dyn.dynamics_step_record(rec)
+                    checkpoints.append((rec, q_in))
             finally:
                 dyn.prog, dyn.progq = saved_prog, saved_q
             fctx.dyn = dyn
+            fctx.store = store
             fctx.checkpoints = checkpoints
             fctx.progq_out = progq
             return work
@@ -68,10 +74,13 @@ def _make_functions():
         def backward(fctx, grad):
             dyn = fctx.dyn
             gb = grad.detach().to(dtype=dyn.ctx.dtype).contiguous().clone()
-            for stages, q in reversed(fctx.checkpoints):
-                dyn.dynamics_step_adj_rec(stages, q, gb)
+            for rec, q in reversed(fctx.checkpoints):
+                if fctx.store == "tape":
+                    dyn.dynamics_step_adj_tape(rec, q, gb)
+                else:
+                    dyn.dynamics_step_adj_rec(rec, q, gb)
             fctx.checkpoints = None
-            return gb, None, None
+            return gb, None, None, None
 
     class SfcPressure(torch.autograd.Function):
         """prog -> surface pressure per column (FP64, nreg + 12)."""
@@ -103,12 +112,17 @@ def _fn(name):
     return _FN[name]
 
 
-def forecast(dyn, prog, nsteps: int):
+def forecast(dyn, prog, nsteps: int, store: str = "tape"):
     """State after ``nsteps`` steps of ``dyn`` from ``prog`` (``dyn.progq`` as the initial tracers).
 
     Differentiable with respect to ``prog``; ``dyn``'s own state is left unchanged.
+    ``store``: "tape" keeps every step's full tape (``nh_tape_bytes`` per step;
+    the backward pass recomputes only diagnostics), "stages" only the RK-stage
+    inputs (18 fields per step; the backward pass recomputes each step's stages).
     """
-    return _fn("forecast").apply(prog, dyn, int(nsteps))
+    if store not in ("tape", "stages"):
+        raise ValueError("store must be 'tape' or 'stages'")
+    return _fn("forecast").apply(prog, dyn, int(nsteps), store)
 
 
 def sfc_pressure(dyn, prog, progq=None):

@@ -100,10 +100,11 @@ void divdamp_adj(X& x, const Context& ctx, real* const ob[3], real* const vb[4])
 // (hvf: the divergences the recomputed step kept)
 template <class X>
 void small_step_adj(X& x, const Context& ctx, const real* prog, real* const ck[6], real* const nx[6],
-                    real* hvf, double dts, real* prog_b, real* const psb[6], real* const psb2[6]) {
+                    real* hvf, double dts, real* prog_b, real* const psb[6], real* const psb2[6],
+                    const StageFields& sf) {
   const Dims& d = ctx.d;
   const long NF = d.NF;
-  real* gT = ctx.field(W_GT);
+  real* gT = sf.gT;
   real* gTb = afield(ctx, A_GTB);
   real* dvn[3] = {nx[1], nx[2], nx[3]};
   real* hv[3] = {hvf, hvf + NF, hvf + 2 * NF};
@@ -119,7 +120,7 @@ void small_step_adj(X& x, const Context& ctx, const real* prog, real* const ck[6
     f.prog = prog;
     for (int q = 0; q < 6; ++q) { f.ps[q] = ck[q]; f.ps_b[q] = psb2[q]; }
     for (int q = 0; q < 3; ++q) { f.dv[q] = dvn[q]; f.dv_b[q] = dvb[q]; f.hv[q] = hv[q]; f.hv_b[q] = hvb[q]; }
-    f.ethtot = ctx.field(W_ETHTOT); f.gT = gT;
+    f.ethtot = sf.ethtot; f.gT = gT;
     f.we1 = nx[4];
     for (int q = 0; q < 3; ++q) f.we_b[q] = web[q];
     f.prog_b = prog_b; f.ethtot_b = afield(ctx, A_ETHTOTB); f.gT_b = gTb; f.getot_b = afield(ctx, A_GETOTB);
@@ -130,7 +131,7 @@ void small_step_adj(X& x, const Context& ctx, const real* prog, real* const ck[6
       f.d = d; f.c = ctx.c; f.G = ctx.G;
       f.dt = dts;
       f.x = nx[4];
-      f.Mc = ctx.dfield(D_MC); f.Mu = ctx.dfield(D_MU); f.Ml = ctx.dfield(D_ML);
+      f.Mc = sf.M; f.Mu = sf.M + NF; f.Ml = sf.M + 2 * NF;
       f.U = ctx.dfield(D_U); f.Gm = ctx.dfield(D_GM);
       f.out_b = web[1];
       for (int q = 0; q < 6; ++q) f.ps_b[q] = psb2[q];
@@ -147,7 +148,7 @@ void small_step_adj(X& x, const Context& ctx, const real* prog, real* const ck[6
     f.d = d; f.c = ctx.c; f.G = ctx.G;
     f.prog = prog;
     for (int q = 0; q < 3; ++q) { f.dv[q] = dvn[q]; f.hv_b[q] = hvb[q]; f.dv_b[q] = dvb[q]; }
-    f.eth = ctx.field(W_ETH); f.ethtot = ctx.field(W_ETHTOT);
+    f.eth = ctx.field(W_ETH); f.ethtot = sf.ethtot;
     f.eth_b = afield(ctx, A_ETHB); f.ethtot_b = afield(ctx, A_ETHTOTB);
     f.prog_b = prog_b; f.gT_b = gTb;
     x.run_pts(f, d);
@@ -172,7 +173,8 @@ void small_step_adj(X& x, const Context& ctx, const real* prog, real* const ck[6
 // reverse of the large-step part of a stage (stage_large); prog is the
 // recomputed boundary-conditioned state
 template <class X>
-void stage_large_adj(X& x, const Context& ctx, const real* prog, const real* rhogq, double dts, real* prog_b) {
+void stage_large_adj(X& x, const Context& ctx, const real* prog, const real* rhogq, double dts, real* prog_b,
+                     const StageFields& sf) {
   const Dims& d = ctx.d;
   const long NF = d.NF;
   real* diag = ctx.field(W_DIAG);
@@ -190,7 +192,7 @@ void stage_large_adj(X& x, const Context& ctx, const real* prog, const real* rho
     f.d = d; f.c = ctx.c; f.G = ctx.G;
     f.dt = dts;
     f.prog = prog; f.diag = diag; f.eth = ctx.field(W_ETH);
-    for (int q = 0; q < 6; ++q) { f.vt[q] = ctx.field(W_HDB + q); f.vt_b[q] = afield(ctx, A_HDBB + q); }
+    for (int q = 0; q < 6; ++q) { f.vt[q] = sf.hdb + q * NF; f.vt_b[q] = afield(ctx, A_HDBB + q); }
     f.gT_b = afield(ctx, A_GTB);
     f.Mc_b = adfield(ctx, AD_MCB); f.Mu_b = adfield(ctx, AD_MUB); f.Ml_b = adfield(ctx, AD_MLB);
     f.getot_b = afield(ctx, A_GETOTB);
@@ -209,7 +211,7 @@ void stage_large_adj(X& x, const Context& ctx, const real* prog, const real* rho
     HDiffPass2A f;
     f.d = d; f.c = ctx.c; f.G = ctx.G;
     for (int q = 0; q < 6; ++q) {
-      f.in[q] = ctx.field(W_HDA + q);
+      f.in[q] = sf.hda + q * NF;
       f.ob[q] = afield(ctx, A_HDBB + q);
       f.ib[q] = afield(ctx, A_HDAB + q);
     }
@@ -277,14 +279,14 @@ int dyn_step_adj(X& x, const Context& ctx, const real* prog_in, const real* prog
   for (int nl = 0; nl < 3 && !stage_in; ++nl) {
     x.d2d(afield(ctx, A_SC + 6 * nl), P, 6 * NF * sizeof(real));
     if (nl == 2) break;
-    stage_large(x, ctx, P, progq, dts);
+    stage_large(x, ctx, P, progq, dts, ws_stage(ctx));
     SmallBufs s;
     for (int q = 0; q < 6; ++q) s.ps[q] = ctx.field(W_SPLIT + q);
     for (int q = 0; q < 3; ++q) s.dv[q] = ctx.field(W_SPLIT + 6 + q);
     for (int q = 0; q < 3; ++q) s.we[q] = ctx.field(W_SPLIT + 9 + q);
     x.run(SplitInit{NF, P, prog0, {s.ps[0], s.ps[1], s.ps[2], s.ps[3], s.ps[4], s.ps[5]}, nl == 0 ? 1 : 0}, NF);
     for (int it = 0; it < nsub[nl]; ++it) {
-      small_step(x, ctx, P, s, dts, 1.0 / (double)nsub[nl]);
+      small_step(x, ctx, P, s, dts, 1.0 / (double)nsub[nl], ws_stage(ctx));
       real* old[6] = {s.ps[0], s.ps[1], s.ps[2], s.ps[3], s.ps[4], s.ps[5]};
       s.ps[0] = s.we[0]; s.ps[1] = s.dv[0]; s.ps[2] = s.dv[1]; s.ps[3] = s.dv[2]; s.ps[4] = s.we[1];
       s.ps[5] = s.we[2];
@@ -300,7 +302,7 @@ int dyn_step_adj(X& x, const Context& ctx, const real* prog_in, const real* prog
     if (nl < 2) halo_block_adj(x, ctx, prog_b, 6);
     // recompute the stage, keeping every small step's split state
     x.d2d(P, stage_input(nl), 6 * NF * sizeof(real));
-    stage_large(x, ctx, P, progq, dts);
+    stage_large(x, ctx, P, progq, dts, ws_stage(ctx));
     real* ck[6];
     ck_slot(0, ck);
     x.run(SplitInit{NF, P, prog0, {ck[0], ck[1], ck[2], ck[3], ck[4], ck[5]}, nl == 0 ? 1 : 0}, NF);
@@ -313,7 +315,7 @@ int dyn_step_adj(X& x, const Context& ctx, const real* prog_in, const real* prog
       for (int q = 0; q < 6; ++q) s.ps[q] = c0[q];
       s.we[0] = c1[0]; s.dv[0] = c1[1]; s.dv[1] = c1[2]; s.dv[2] = c1[3]; s.we[1] = c1[4]; s.we[2] = c1[5];
       s.hv = slot_hv(it);
-      small_step(x, ctx, P, s, dts, rweight);
+      small_step(x, ctx, P, s, dts, rweight, ws_stage(ctx));
     }
     // adjoint of the stage
     zero_fields(x, afield(ctx, A_DIAGB), A_TRIW - A_DIAGB, NF);
@@ -335,7 +337,7 @@ int dyn_step_adj(X& x, const Context& ctx, const real* prog_in, const real* prog
       ck_slot(it, c0);
       ck_slot(it + 1, c1);
       zero_fields(x, psb2[0], 6, NF);
-      small_step_adj(x, ctx, P, c0, c1, slot_hv(it), dts, prog_b, psb, psb2);
+      small_step_adj(x, ctx, P, c0, c1, slot_hv(it), dts, prog_b, psb, psb2, ws_stage(ctx));
       for (int q = 0; q < 6; ++q) {
         real* t = psb[q];
         psb[q] = psb2[q];
@@ -345,12 +347,75 @@ int dyn_step_adj(X& x, const Context& ctx, const real* prog_in, const real* prog
     x.run(SplitInitA{NF, prog_b, afield(ctx, A_P0B), {psb[0], psb[1], psb[2], psb[3], psb[4], psb[5]},
                      nl == 0 ? 1 : 0},
           NF);
-    stage_large_adj(x, ctx, P, progq, dts, prog_b);
+    stage_large_adj(x, ctx, P, progq, dts, prog_b, ws_stage(ctx));
     int rc = x.check("dyn_step_adj stage");
     if (rc) return rc;
   }
   x.run(AxpyP{prog_b, afield(ctx, A_P0B)}, 6 * NF);
   return x.check("dyn_step_adj");
+}
+
+// the adjoint from the tape of the forward step (dyn_step with a tape): only
+// the diagnostics of each stage are recomputed (Pre_Post, which also sets
+// the ghost rows of the stage state), then the transposes run
+template <class X>
+int dyn_step_adj_tape(X& x, const Context& ctx, const void* tape, const real* progq, int nq, double dtl,
+                      int sstep_max, real* prog_b) {
+  const Dims& d = ctx.d;
+  const long NF = d.NF;
+  if (nq < 1 || !tape || !progq || !prog_b) return NH_ERR_ARG;
+  if (!ctx.aws) return NH_ERR_STATE;
+  const double dts = dtl / (double)sstep_max;
+  const int nsub[3] = {sstep_max / 3, sstep_max / 2, sstep_max};
+  const Tape tp(const_cast<void*>(tape), NF, sstep_max);
+  real* P = afield(ctx, A_PROG);
+  halo_block_adj(x, ctx, prog_b, 6);   // the final halo fill of PROG
+  zero_fields(x, afield(ctx, A_P0B), 6, NF);
+  for (int nl = 2; nl >= 0; --nl) {
+    if (nl < 2) halo_block_adj(x, ctx, prog_b, 6);
+    const StageFields sf = tp.stage(nl);
+    x.d2d(P, tp.input(nl), 6 * NF * sizeof(real));
+    {
+      PrePostP f{d, ctx.c, ctx.G, P, progq, ctx.field(W_DIAG), ctx.field(W_ETH)};
+      x.run_pts(f, d);
+    }
+    zero_fields(x, afield(ctx, A_DIAGB), A_TRIW - A_DIAGB, NF);
+    x.run(FillZero64{adfield(ctx, AD_MCB)}, 3 * NF);
+    real *psb[6], *psb2[6];
+    for (int q = 0; q < 6; ++q) {
+      psb[q] = afield(ctx, A_PSB + q);
+      psb2[q] = afield(ctx, A_PSB2 + q);
+    }
+    {
+      ProgUpdateA f;
+      f.d = d; f.c = ctx.c; f.G = ctx.G;
+      f.prog_b = prog_b;
+      for (int q = 0; q < 6; ++q) f.ps_b[q] = psb[q];
+      x.run_pts(f, d);
+    }
+    for (int it = nsub[nl] - 1; it >= 0; --it) {
+      real *c0[6], *c1[6];
+      for (int q = 0; q < 6; ++q) {
+        c0[q] = tp.slot(nl, it) + q * NF;
+        c1[q] = tp.slot(nl, it + 1) + q * NF;
+      }
+      zero_fields(x, psb2[0], 6, NF);
+      small_step_adj(x, ctx, P, c0, c1, tp.slot(nl, it) + 6 * NF, dts, prog_b, psb, psb2, sf);
+      for (int q = 0; q < 6; ++q) {
+        real* t = psb[q];
+        psb[q] = psb2[q];
+        psb2[q] = t;
+      }
+    }
+    x.run(SplitInitA{NF, prog_b, afield(ctx, A_P0B), {psb[0], psb[1], psb[2], psb[3], psb[4], psb[5]},
+                     nl == 0 ? 1 : 0},
+          NF);
+    stage_large_adj(x, ctx, P, progq, dts, prog_b, sf);
+    int rc = x.check("dyn_step_adj_tape stage");
+    if (rc) return rc;
+  }
+  x.run(AxpyP{prog_b, afield(ctx, A_P0B)}, 6 * NF);
+  return x.check("dyn_step_adj_tape");
 }
 
 // surface pressure per column (forward) or its transpose added into prog_b

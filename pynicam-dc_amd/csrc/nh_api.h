@@ -90,14 +90,15 @@ inline int gradient(nh_context* ctx, const void* scl, void* gx, void* gy, void* 
 }
 
 inline int dyn_step(nh_context* ctx, void* prog, void* progq, int32_t nq, double dtl, int32_t sstep_max,
-                    void* stream, int f32, void* record = nullptr, const char* name = nullptr) {
+                    void* stream, int f32, void* record = nullptr, const char* name = nullptr,
+                    void* tape = nullptr) {
   const char* where = name ? name : (f32 ? "nh_dyn_step_f32" : "nh_dyn_step");
   if (int rc = check_ready(ctx, where, f32, true)) return rc;
   if (!prog || !progq || nq < 1) return fail(NH_ERR_ARG, where, "need prog and at least one tracer field");
   if (sstep_max < 3 || sstep_max % 6 != 0 || !(dtl > 0.0))
     return fail(NH_ERR_ARG, where, "sstep_max must be a positive multiple of 6 and dtl > 0");
   std::string err;
-  return done(NH_DISPATCH(ctx, v_dyn_step, prog, progq, nq, dtl, sstep_max, record, stream, &err), where, err);
+  return done(NH_DISPATCH(ctx, v_dyn_step, prog, progq, nq, dtl, sstep_max, record, tape, stream, &err), where, err);
 }
 
 inline int tracer_step(nh_context* ctx, const void* prog, void* progq, int32_t nq, double dtl, void* stream,
@@ -143,6 +144,19 @@ inline int dyn_step_adj(nh_context* ctx, const void* prog, const void* progq, in
   std::string err;
   return done(NH_DISPATCH(ctx, v_dyn_step_adj, prog, progq, nq, dtl, sstep_max, prog_b, stage_in, stream, &err),
               where, err);
+}
+
+inline int dyn_step_adj_tape(nh_context* ctx, const void* tape, const void* progq, int32_t nq, double dtl,
+                             int32_t sstep_max, void* prog_b, void* stream, int f32) {
+  const char* where = f32 ? "nh_dyn_step_adj_tape_f32" : "nh_dyn_step_adj_tape";
+  if (int rc = check_ready(ctx, where, f32, true)) return rc;
+  if (!tape || !progq || nq < 1 || !prog_b)
+    return fail(NH_ERR_ARG, where, "need the tape, prog_b and at least one tracer field");
+  if (sstep_max < 3 || sstep_max % 6 != 0 || !(dtl > 0.0))
+    return fail(NH_ERR_ARG, where, "sstep_max must be a positive multiple of 6 and dtl > 0");
+  std::string err;
+  return done(NH_DISPATCH(ctx, v_dyn_step_adj_tape, tape, progq, nq, dtl, sstep_max, prog_b, stream, &err), where,
+              err);
 }
 
 inline int sfc_pressure(nh_context* ctx, const void* prog, const void* progq, double* ps, const double* ps_b,
@@ -328,6 +342,30 @@ int nh_dyn_step_adj_rec_f32(nh_context* ctx, const float* stage_in, const float*
                             int32_t sstep_max, float* prog_b, void* stream) {
   return nh::dyn_step_adj(ctx, stage_in, progq, nq, dtl, sstep_max, prog_b, stream, 1, stage_in,
                           "nh_dyn_step_adj_rec_f32");
+}
+
+int64_t nh_tape_bytes(const nh_context* ctx, int32_t sstep_max) {
+  if (int rc = nh::check_ctx(ctx, "nh_tape_bytes", -1)) return rc;
+  if (sstep_max < 3 || sstep_max % 6 != 0) return nh::fail(NH_ERR_ARG, "nh_tape_bytes", "sstep_max must be a positive multiple of 6");
+  return ctx->c64 ? nh::f64::v_tape_bytes(ctx->c64, sstep_max) : nh::f32::v_tape_bytes(ctx->c32, sstep_max);
+}
+int nh_dyn_step_tape(nh_context* ctx, double* prog, double* progq, int32_t nq, double dtl, int32_t sstep_max,
+                     void* tape, void* stream) {
+  if (!tape) return nh::fail(NH_ERR_ARG, "nh_dyn_step_tape", "null tape");
+  return nh::dyn_step(ctx, prog, progq, nq, dtl, sstep_max, stream, 0, nullptr, "nh_dyn_step_tape", tape);
+}
+int nh_dyn_step_tape_f32(nh_context* ctx, float* prog, float* progq, int32_t nq, double dtl, int32_t sstep_max,
+                         void* tape, void* stream) {
+  if (!tape) return nh::fail(NH_ERR_ARG, "nh_dyn_step_tape_f32", "null tape");
+  return nh::dyn_step(ctx, prog, progq, nq, dtl, sstep_max, stream, 1, nullptr, "nh_dyn_step_tape_f32", tape);
+}
+int nh_dyn_step_adj_tape(nh_context* ctx, const void* tape, const double* progq, int32_t nq, double dtl,
+                         int32_t sstep_max, double* prog_b, void* stream) {
+  return nh::dyn_step_adj_tape(ctx, tape, progq, nq, dtl, sstep_max, prog_b, stream, 0);
+}
+int nh_dyn_step_adj_tape_f32(nh_context* ctx, const void* tape, const float* progq, int32_t nq, double dtl,
+                             int32_t sstep_max, float* prog_b, void* stream) {
+  return nh::dyn_step_adj_tape(ctx, tape, progq, nq, dtl, sstep_max, prog_b, stream, 1);
 }
 
 int nh_sfc_pressure(nh_context* ctx, const double* prog, const double* progq, double* ps, void* stream) {

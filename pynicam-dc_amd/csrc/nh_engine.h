@@ -473,12 +473,26 @@ struct SmallBufs {
   real* hv = nullptr;    // divergences of dv (3 fields); default the step workspace
 };
 
+// the fields a stage computes that its small steps (and the adjoint) read:
+// the step workspace, or a tape slot when the step is taped
+struct StageFields {
+  real* hda;       // 6  hyperdiffusion pass 1
+  real* hdb;       // 6  hyperdiffusion pass 2
+  real* gT;        // 6  large-step tendencies
+  real* ethtot;    // 1
+  double* M;       // 3  FP64 tridiagonal matrix Mc, Mu, Ml
+};
+
+inline StageFields ws_stage(const Context& ctx) {
+  return {ctx.field(W_HDA), ctx.field(W_HDB), ctx.field(W_GT), ctx.field(W_ETHTOT), ctx.dfield(D_MC)};
+}
+
 // large-step part of RK stage nl: diagnostics and boundary values (in place
 // on prog's ghost rows), momentum advection, hyperdiffusion, divergence
 // damping of prog, the sources and matrix of the small steps, ethtot and the
 // start of the mean fluxes
 template <class X>
-void stage_large(X& x, const Context& ctx, real* prog, const real* rhogq, double dts) {
+void stage_large(X& x, const Context& ctx, real* prog, const real* rhogq, double dts, const StageFields& sf) {
   const Dims& d = ctx.d;
   const long NF = d.NF;
   real* diag = ctx.field(W_DIAG);
@@ -495,19 +509,19 @@ void stage_large(X& x, const Context& ctx, real* prog, const real* rhogq, double
     f.d = d; f.c = ctx.c; f.G = ctx.G;
     f.in[0] = diag + I_VX * NF; f.in[1] = diag + I_VY * NF; f.in[2] = diag + I_VZ * NF;
     f.in[3] = diag + I_W * NF; f.in[4] = diag + I_TEM * NF; f.in[5] = prog;
-    for (int q = 0; q < 6; ++q) f.out[q] = ctx.field(W_HDA + q);
+    for (int q = 0; q < 6; ++q) f.out[q] = sf.hda + q * NF;
     f.rhog = prog;
     run_staged(x, f);
-    halo_block(x, ctx, ctx.field(W_HDA), 6);
+    halo_block(x, ctx, sf.hda, 6);
     HDiffPass2T f2;
     f2.d = d; f2.c = ctx.c; f2.G = ctx.G; f2.tg = tile_geom(d, 12);
     for (int q = 0; q < 6; ++q) {
-      f2.in[q] = ctx.field(W_HDA + q);
-      f2.out[q] = ctx.field(W_HDB + q);
+      f2.in[q] = sf.hda + q * NF;
+      f2.out[q] = sf.hdb + q * NF;
     }
     f2.rhog = prog;
     x.run_tiles(f2, f2.tg.nblocks, f2.tg.lds);
-    halo_block(x, ctx, ctx.field(W_HDB), 6);
+    halo_block(x, ctx, sf.hdb, 6);
   }
   // divergence damping of the large-step momentum
   divdamp(x, ctx, prog + 1 * NF, prog + 2 * NF, prog + 3 * NF, prog + 4 * NF,
@@ -517,27 +531,27 @@ void stage_large(X& x, const Context& ctx, real* prog, const real* rhogq, double
     f.d = d; f.c = ctx.c; f.G = ctx.G;
     f.dt = dts;
     f.prog = prog; f.diag = diag; f.eth = eth; f.gadv = gadv;
-    for (int q = 0; q < 6; ++q) f.vt[q] = ctx.field(W_HDB + q);
+    for (int q = 0; q < 6; ++q) f.vt[q] = sf.hdb + q * NF;
     for (int q = 0; q < 3; ++q) f.dd[q] = ctx.field(W_DD + q);
-    f.gT = ctx.field(W_GT);
-    f.Mc = ctx.dfield(D_MC); f.Mu = ctx.dfield(D_MU); f.Ml = ctx.dfield(D_ML);
+    f.gT = sf.gT;
+    f.Mc = sf.M; f.Mu = sf.M + NF; f.Ml = sf.M + 2 * NF;
     f.getot = ctx.field(W_GETOT); f.frhog = ctx.field(W_FRHOG);
     run_staged(x, f);
   }
   {
-    EthTotPMP f{d, ctx.c, ctx.G, prog, eth, ctx.field(W_ETHTOT), ctx.field(W_PM)};
+    EthTotPMP f{d, ctx.c, ctx.G, prog, eth, sf.ethtot, ctx.field(W_PM)};
     x.run_pts(f, d);
   }
 }
 
 // the horizontal divergences of the small-step momentum dv (ViMainH) into hv (3 fields)
 template <class X>
-void small_hv(X& x, const Context& ctx, const real* prog, real* const dv[3], real* hv) {
+void small_hv(X& x, const Context& ctx, const real* prog, real* const dv[3], real* hv, const StageFields& sf) {
   ViMainH f;
   f.d = ctx.d; f.c = ctx.c; f.G = ctx.G;
   f.prog = prog;
   for (int q = 0; q < 3; ++q) f.dv[q] = dv[q];
-  f.eth = ctx.field(W_ETH); f.ethtot = ctx.field(W_ETHTOT); f.gT = ctx.field(W_GT);
+  f.eth = ctx.field(W_ETH); f.ethtot = sf.ethtot; f.gT = sf.gT;
   for (int q = 0; q < 3; ++q) f.hv[q] = hv + q * ctx.d.NF;
   run_staged(x, f);
 }
@@ -545,10 +559,11 @@ void small_hv(X& x, const Context& ctx, const real* prog, real* const dv[3], rea
 // one small step (mod_vi.py:1283-1450): divergence damping of the split
 // momentum, path1, vi_main (horizontal, tridiagonal, mass/energy), halos
 template <class X>
-void small_step(X& x, const Context& ctx, const real* prog, const SmallBufs& s, double dts, double rweight) {
+void small_step(X& x, const Context& ctx, const real* prog, const SmallBufs& s, double dts, double rweight,
+                const StageFields& sf) {
   const Dims& d = ctx.d;
   const long NF = d.NF;
-  real* gT = ctx.field(W_GT);
+  real* gT = sf.gT;
   real* pm = ctx.field(W_PM);
   real* ddv = s.ddv ? s.ddv : ctx.field(W_DDV);
   real* hvb = s.hv ? s.hv : ctx.field(W_HV);
@@ -566,7 +581,7 @@ void small_step(X& x, const Context& ctx, const real* prog, const SmallBufs& s, 
     run_staged(x, f);
   }
   halo(x, ctx, s.dv, 3);
-  small_hv(x, ctx, prog, s.dv, hvb);
+  small_hv(x, ctx, prog, s.dv, hvb, sf);
   real* hv[3] = {hvb, hvb + NF, hvb + 2 * NF};
   {
     // the z40 column (39 unknowns) keeps the sweep in registers
@@ -575,7 +590,7 @@ void small_step(X& x, const Context& ctx, const real* prog, const SmallBufs& s, 
       f.dt = dts;
       for (int q = 0; q < 6; ++q) f.ps[q] = s.ps[q];
       f.gT = gT;
-      f.Mc = ctx.dfield(D_MC); f.Mu = ctx.dfield(D_MU); f.Ml = ctx.dfield(D_ML);
+      f.Mc = sf.M; f.Mu = sf.M + NF; f.Ml = sf.M + 2 * NF;
       for (int q = 0; q < 3; ++q) f.hv[q] = hv[q];
       f.U = ctx.dfield(D_U); f.Gm = ctx.dfield(D_GM);
       f.out = s.we[1];
@@ -591,7 +606,7 @@ void small_step(X& x, const Context& ctx, const real* prog, const SmallBufs& s, 
     f.prog = prog;
     for (int q = 0; q < 6; ++q) f.ps[q] = s.ps[q];
     for (int q = 0; q < 3; ++q) f.dv[q] = s.dv[q];
-    f.ethtot = ctx.field(W_ETHTOT); f.gT = gT; f.getot = ctx.field(W_GETOT);
+    f.ethtot = sf.ethtot; f.gT = gT; f.getot = ctx.field(W_GETOT);
     for (int q = 0; q < 3; ++q) f.hv[q] = hv[q];
     for (int q = 0; q < 3; ++q) f.we[q] = s.we[q];
     f.pm = pm;
@@ -610,10 +625,49 @@ void prog_update(X& x, const Context& ctx, real* prog, real* const ps[6]) {
   x.run_pts(f, ctx.d);
 }
 
-// record (optional): the input state of each RK stage, 3 x 6 fields, for the adjoint
+// The tape of one step (optional, for the adjoint): per RK stage its input
+// state and the stage fields, per small step its split state and vi_main
+// divergences, in field-precision fields of NF values, then the three
+// stages' FP64 matrices.  The step writes them in place of its workspace
+// fields, so taping adds no traffic beyond the stage-input copies.
+struct TapeLayout {
+  static constexpr int kStage = 25;   // input 6, hda 6, hdb 6, gT 6, ethtot 1
+  static constexpr int kSlot = 9;     // split state 6, divergences 3
+  int nsub[3];
+  long real_fields() const { return 3L * kStage + (long)kSlot * (nsub[0] + nsub[1] + nsub[2] + 3); }
+  long slot0(int nl) const {
+    long s = 3L * kStage;
+    for (int m = 0; m < nl; ++m) s += (long)kSlot * (nsub[m] + 1);
+    return s;
+  }
+  static size_t bytes(long NF, int sstep_max) {
+    TapeLayout t{{sstep_max / 3, sstep_max / 2, sstep_max}};
+    return (size_t)t.real_fields() * NF * sizeof(real) + (size_t)9 * NF * sizeof(double);
+  }
+};
+
+struct Tape {
+  TapeLayout L;
+  real* r;       // real part
+  double* m;     // FP64 matrices
+  long NF;
+  Tape(void* base, long NF_, int sstep_max)
+      : L{{sstep_max / 3, sstep_max / 2, sstep_max}}, r((real*)base), m(nullptr), NF(NF_) {
+    m = (double*)(r + L.real_fields() * NF);
+  }
+  real* input(int nl) const { return r + (long)nl * TapeLayout::kStage * NF; }
+  StageFields stage(int nl) const {
+    real* b = input(nl);
+    return {b + 6 * NF, b + 12 * NF, b + 18 * NF, b + 24 * NF, m + (long)nl * 3 * NF};
+  }
+  real* slot(int nl, int it) const { return r + (L.slot0(nl) + (long)TapeLayout::kSlot * it) * NF; }
+};
+
+// record (optional): the input state of each RK stage, 3 x 6 fields;
+// tape (optional): the full tape of the step (record is then unused)
 template <class X>
 int dyn_step(X& x, const Context& ctx, real* prog, real* progq, int nq, double dtl, int sstep_max,
-             real* record = nullptr) {
+             real* record = nullptr, void* tape = nullptr) {
   const Dims& d = ctx.d;
   const long NF = d.NF;
   const double dts = dtl / (double)sstep_max;
@@ -621,21 +675,39 @@ int dyn_step(X& x, const Context& ctx, real* prog, real* progq, int nq, double d
   if (nq < 1 || !prog || !progq) return NH_ERR_ARG;   // tracer 0 is qv (zero in a dry run)
   real* prog0 = ctx.field(W_PROG0);
   x.d2d(prog0, prog, 6 * NF * sizeof(real));
+  const Tape tp(tape, NF, sstep_max);
   for (int nl = 0; nl < 3; ++nl) {
-    if (record) x.d2d(record + (long)nl * 6 * NF, prog, 6 * NF * sizeof(real));
-    stage_large(x, ctx, prog, progq, dts);
+    if (tape) x.d2d(tp.input(nl), prog, 6 * NF * sizeof(real));
+    else if (record) x.d2d(record + (long)nl * 6 * NF, prog, 6 * NF * sizeof(real));
+    const StageFields sf = tape ? tp.stage(nl) : ws_stage(ctx);
+    stage_large(x, ctx, prog, progq, dts, sf);
     // split state ps[0..5], path1 momentum dv[0..2], vi_main output we[0..2]
     SmallBufs s;
-    for (int q = 0; q < 6; ++q) s.ps[q] = ctx.field(W_SPLIT + q);
-    for (int q = 0; q < 3; ++q) s.dv[q] = ctx.field(W_SPLIT + 6 + q);
-    for (int q = 0; q < 3; ++q) s.we[q] = ctx.field(W_SPLIT + 9 + q);
+    if (tape) {
+      real* c0 = tp.slot(nl, 0);
+      for (int q = 0; q < 6; ++q) s.ps[q] = c0 + q * NF;
+    } else {
+      for (int q = 0; q < 6; ++q) s.ps[q] = ctx.field(W_SPLIT + q);
+      for (int q = 0; q < 3; ++q) s.dv[q] = ctx.field(W_SPLIT + 6 + q);
+      for (int q = 0; q < 3; ++q) s.we[q] = ctx.field(W_SPLIT + 9 + q);
+    }
     {
       SplitInit f{NF, prog, prog0, {s.ps[0], s.ps[1], s.ps[2], s.ps[3], s.ps[4], s.ps[5]}, nl == 0 ? 1 : 0};
       x.run(f, NF);
     }
     const double rweight = 1.0 / (double)nsub[nl];
     for (int it = 0; it < nsub[nl]; ++it) {
-      small_step(x, ctx, prog, s, dts, rweight);
+      if (tape) {
+        // outputs straight into the next slot: (we0, dv0, dv1, dv2, we1, we2)
+        real* c1 = tp.slot(nl, it + 1);
+        s.we[0] = c1; s.dv[0] = c1 + NF; s.dv[1] = c1 + 2 * NF; s.dv[2] = c1 + 3 * NF;
+        s.we[1] = c1 + 4 * NF; s.we[2] = c1 + 5 * NF;
+        s.hv = tp.slot(nl, it) + 6 * NF;
+        small_step(x, ctx, prog, s, dts, rweight, sf);
+        for (int q = 0; q < 6; ++q) s.ps[q] = c1 + q * NF;
+        continue;
+      }
+      small_step(x, ctx, prog, s, dts, rweight, sf);
       // rotate buffers: the new split state is (we0, dv0, dv1, dv2, we1, we2)
       real* old[6] = {s.ps[0], s.ps[1], s.ps[2], s.ps[3], s.ps[4], s.ps[5]};
       s.ps[0] = s.we[0]; s.ps[1] = s.dv[0]; s.ps[2] = s.dv[1]; s.ps[3] = s.dv[2]; s.ps[4] = s.we[1];

@@ -117,10 +117,10 @@ int v_gradient(Context* c, const void* scl, void* gx, void* gy, void* gz, void* 
   return 0;
 }
 
-int v_dyn_step(Context* c, void* prog, void* progq, int nq, double dtl, int sstep_max, void* record, void* stream,
-               std::string* err) {
+int v_dyn_step(Context* c, void* prog, void* progq, int nq, double dtl, int sstep_max, void* record, void* tape,
+               void* stream, std::string* err) {
   NHX x(c->device, stream, &c->prof);
-  int rc = dyn_step(x, *c, (real*)prog, (real*)progq, nq, dtl, sstep_max, (real*)record);
+  int rc = dyn_step(x, *c, (real*)prog, (real*)progq, nq, dtl, sstep_max, (real*)record, tape);
   if (rc == NH_ERR_ARG) return failv(err, rc, "invalid argument");
   if (rc) return failv(err, NH_ERR_DEVICE, x.message());
   return 0;
@@ -150,30 +150,52 @@ int v_budget(Context* c, const void* prog, const void* progq, double lhv, double
   return 0;
 }
 
+namespace {
+// the adjoint workspace with nck small-step checkpoint slots (grown on demand)
+template <class X>
+int adj_workspace(X& x, Context* c, int nck, std::string* err) {
+  if (c->aws && c->adj_ck >= nck) return 0;
+  x.sync();
+  if (c->aws) x.release(c->aws);
+  if (c->adws) x.release(c->adws);
+  c->aws = nullptr;
+  c->adws = nullptr;
+  c->adj_ck = 0;
+  const size_t bytes = (size_t)adj_fields(nck) * c->d.NF * sizeof(real);
+  const size_t dbytes = (size_t)AD_NFIELDS * c->d.NF * sizeof(double);
+  c->aws = (real*)x.alloc(bytes);
+  c->adws = (double*)x.alloc(dbytes);
+  if (!c->aws || !c->adws)
+    return failv(err, NH_ERR_ALLOC, "adjoint workspace allocation of " + std::to_string(bytes + dbytes) + " bytes");
+  x.zero(c->aws, bytes);
+  x.zero(c->adws, dbytes);
+  c->adj_ck = nck;
+  return 0;
+}
+}  // namespace
+
+long v_tape_bytes(const Context* c, int sstep_max) { return (long)TapeLayout::bytes(c->d.NF, sstep_max); }
+
+int v_dyn_step_adj_tape(Context* c, const void* tape, const void* progq, int nq, double dtl, int sstep_max,
+                        void* prog_b, void* stream, std::string* err) {
+  if (sstep_max < 3 || sstep_max > 64) return failv(err, NH_ERR_ARG, "sstep_max out of range for the adjoint");
+  if (!c->halo_adj_ok)
+    return failv(err, NH_ERR_STATE, "halo map has a destination listed twice or used as a source");
+  NHX x(c->device, stream, &c->prof);
+  if (int rc = adj_workspace(x, c, 0, err)) return rc;
+  int rc = dyn_step_adj_tape(x, *c, tape, (const real*)progq, nq, dtl, sstep_max, (real*)prog_b);
+  if (rc == NH_ERR_ARG) return failv(err, rc, "invalid argument");
+  if (rc) return failv(err, NH_ERR_DEVICE, x.message());
+  return 0;
+}
+
 int v_dyn_step_adj(Context* c, const void* prog, const void* progq, int nq, double dtl, int sstep_max, void* prog_b,
                    const void* stage_in, void* stream, std::string* err) {
   if (sstep_max < 3 || sstep_max > 64) return failv(err, NH_ERR_ARG, "sstep_max out of range for the adjoint");
   if (!c->halo_adj_ok)
     return failv(err, NH_ERR_STATE, "halo map has a destination listed twice or used as a source");
   NHX x(c->device, stream, &c->prof);
-  if (!c->aws || c->adj_ck < sstep_max + 1) {
-    // first adjoint call (or more small steps than before): allocate the checkpoint workspace
-    x.sync();
-    if (c->aws) x.release(c->aws);
-    if (c->adws) x.release(c->adws);
-    c->aws = nullptr;
-    c->adws = nullptr;
-    c->adj_ck = 0;
-    const size_t bytes = (size_t)adj_fields(sstep_max + 1) * c->d.NF * sizeof(real);
-    const size_t dbytes = (size_t)AD_NFIELDS * c->d.NF * sizeof(double);
-    c->aws = (real*)x.alloc(bytes);
-    c->adws = (double*)x.alloc(dbytes);
-    if (!c->aws || !c->adws)
-      return failv(err, NH_ERR_ALLOC, "adjoint workspace allocation of " + std::to_string(bytes + dbytes) + " bytes");
-    x.zero(c->aws, bytes);
-    x.zero(c->adws, dbytes);
-    c->adj_ck = sstep_max + 1;
-  }
+  if (int rc = adj_workspace(x, c, sstep_max + 1, err)) return rc;
   int rc = dyn_step_adj(x, *c, (const real*)prog, (const real*)progq, nq, dtl, sstep_max, (real*)prog_b,
                         (const real*)stage_in);
   if (rc == NH_ERR_ARG) return failv(err, rc, "invalid argument");

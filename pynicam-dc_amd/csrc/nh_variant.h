@@ -25,7 +25,10 @@
                    std::string* err);                                                                       \
   int v_gradient(Context* c, const void* scl, void* gx, void* gy, void* gz, void* stream, std::string* err); \
   int v_dyn_step(Context* c, void* prog, void* progq, int nq, double dtl, int sstep_max, void* record,      \
-                 void* stream, std::string* err);                                                           \
+                 void* tape, void* stream, std::string* err);                                               \
+  long v_tape_bytes(const Context* c, int sstep_max);                                                      \
+  int v_dyn_step_adj_tape(Context* c, const void* tape, const void* progq, int nq, double dtl,              \
+                          int sstep_max, void* prog_b, void* stream, std::string* err);                      \
   int v_tracer_step(Context* c, const void* prog, void* progq, int nq, double dtl, void* stream,            \
                     std::string* err);                                                                      \
   int v_trcadv_winds(Context* c, void* prog, double time, int kind, const double* params, void* stream,     \

@@ -221,6 +221,35 @@ class HipDynamics:
             ctx.handle, C.c_void_p(stage_in.data_ptr()), C.c_void_p(progq.data_ptr()), progq.shape[0], self.dtl,
             self.sstep_max, C.c_void_p(prog_b.data_ptr()), ctx.stream()), "nh_dyn_step_adj_rec")
 
+    def tape_bytes(self) -> int:
+        """Bytes of one step's tape (``dynamics_step_tape``)."""
+        n = int(self.ctx.lib.nh_tape_bytes(self.ctx.handle, self.sstep_max))
+        if n < 0:
+            native.check(self.ctx.lib, n, "nh_tape_bytes")
+        return n
+
+    def new_tape(self):
+        """A device buffer for one step's tape."""
+        return _torch().empty(self.tape_bytes(), dtype=_torch().uint8, device=self.ctx.device)
+
+    def dynamics_step_tape(self, tape):
+        """``dynamics_step`` writing its intermediate fields to ``tape`` (for ``dynamics_step_adj_tape``)."""
+        ctx = self.ctx
+        if tape.numel() < self.tape_bytes() or tape.device != ctx.device:
+            raise ValueError("tape too small or on another device")
+        native.check(ctx.lib, ctx.fn("nh_dyn_step_tape")(
+            ctx.handle, C.c_void_p(self.prog.data_ptr()), C.c_void_p(self.progq.data_ptr()), self.progq.shape[0],
+            self.dtl, self.sstep_max, C.c_void_p(tape.data_ptr()), ctx.stream()), "nh_dyn_step_tape")
+
+    def dynamics_step_adj_tape(self, tape, progq, prog_b):
+        """``dynamics_step_adj`` from the tape of the step (``progq``: the step's input tracers)."""
+        ctx = self.ctx
+        for a in (progq, prog_b):
+            ctx._check_tensor(a)
+        native.check(ctx.lib, ctx.fn("nh_dyn_step_adj_tape")(
+            ctx.handle, C.c_void_p(tape.data_ptr()), C.c_void_p(progq.data_ptr()), progq.shape[0], self.dtl,
+            self.sstep_max, C.c_void_p(prog_b.data_ptr()), ctx.stream()), "nh_dyn_step_adj_tape")
+
     def dynamics_step_adj(self, prog, progq, prog_b):
         """Transpose of one ``dynamics_step`` started from (prog, progq), applied in place to ``prog_b``.
 

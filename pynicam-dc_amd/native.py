@@ -72,6 +72,11 @@ EXPORTS = {
                                      C.c_void_p, C.c_void_p]),
     "nh_dyn_step_adj_rec": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_int32,
                                       C.c_void_p, C.c_void_p]),
+    "nh_tape_bytes": (C.c_int64, [C.c_void_p, C.c_int32]),
+    "nh_dyn_step_tape": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_int32,
+                                   C.c_void_p, C.c_void_p]),
+    "nh_dyn_step_adj_tape": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_int32,
+                                       C.c_void_p, C.c_void_p]),
     "nh_sfc_pressure": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "nh_sfc_pressure_adj": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "nh_prof_enable": (C.c_int, [C.c_void_p, C.c_int32]),
@@ -81,7 +86,7 @@ EXPORTS = {
 # the _f32 forms of the field entry points take the same arguments (float fields)
 for _name in ("nh_halo_fill", "nh_laplacian", "nh_divergence", "nh_gradient", "nh_dyn_step", "nh_tracer_step",
               "nh_trcadv_winds", "nh_budget", "nh_dyn_step_adj", "nh_sfc_pressure", "nh_sfc_pressure_adj",
-              "nh_dyn_step_record", "nh_dyn_step_adj_rec"):
+              "nh_dyn_step_record", "nh_dyn_step_adj_rec", "nh_dyn_step_tape", "nh_dyn_step_adj_tape"):
     EXPORTS[_name + "_f32"] = EXPORTS[_name]
 
 PREC_FP64, PREC_MIXED = 0, 1

@@ -139,12 +139,26 @@ def test_host_autograd_window_mixed_matches_fp64(case_gl04, hostemu_lib):
         assert sc > 0 and (g64[q] - g32[q]).abs().max() <= 1e-4 * sc, q
 
 
-def test_host_autograd_window_equals_direct_adjoint(case_gl04, emu64):
+def test_host_taped_step_equals_plain_step(case_gl04, emu64):
+    """The taped forward step writes its intermediates to the tape instead of the
+    workspace: the state it produces is bit-identical."""
+    import torch
+    a = hipdyn.HipDynamics(emu64, case_gl04.state)
+    b = hipdyn.HipDynamics(emu64, case_gl04.state)
+    tape = b.new_tape()
+    for _ in range(2):
+        a.dynamics_step()
+        b.dynamics_step_tape(tape)
+    assert torch.equal(a.prog, b.prog) and torch.equal(a.progq, b.progq)
+
+
+@pytest.mark.parametrize("store", ["tape", "stages"])
+def test_host_autograd_window_equals_direct_adjoint(case_gl04, emu64, store):
     import torch
     dyn = hipdyn.HipDynamics(emu64, case_gl04.state)
     m = adjoint.box_mask(dyn)
     x = dyn.prog.clone().requires_grad_(True)
-    ps = adjoint.sfc_pressure(dyn, adjoint.forecast(dyn, x, 2))
+    ps = adjoint.sfc_pressure(dyn, adjoint.forecast(dyn, x, 2, store=store))
     (m * ps * ps).sum().backward()
     # the same by hand
     traj = []
