@@ -284,6 +284,10 @@ int nh_sfc_pressure_adj_f32(nh_context* ctx, const float* prog, const float* pro
  * max_kernels entries (names: 32 bytes each, NUL-terminated; summed duration
  * in ms; launch count), sets *n_kernels and clears the record. */
 int nh_prof_enable(nh_context* ctx, int32_t on);
+/* Restrict the timers to the launches of one kernel (its report name; NULL or
+ * "" = every kernel), so that a timed region carries one event pair per launch
+ * of that kernel only. */
+int nh_prof_only(nh_context* ctx, const char* kernel);
 int nh_prof_report(nh_context* ctx, int32_t max_kernels, char* names, double* total_ms, int64_t* launches,
                    int32_t* n_kernels);
 

@@ -296,6 +296,12 @@ int nh_budget_f32(nh_context* ctx, const float* prog, const float* progq, int32_
   return nh::budget(ctx, prog, progq, nq, lhv, out, stream, 1);
 }
 
+int nh_prof_only(nh_context* ctx, const char* kernel) {
+  if (int rc = nh::check_ctx(ctx, "nh_prof_only", -1)) return rc;
+  std::string err;
+  return nh::done(NH_DISPATCH(ctx, v_prof_only, kernel, &err), "nh_prof_only", err);
+}
+
 int nh_prof_enable(nh_context* ctx, int32_t on) {
   if (!ctx) return nh::fail(NH_ERR_ARG, "nh_prof_enable", "null context");
   std::string err;

@@ -53,6 +53,7 @@ __global__ __launch_bounds__(256) void k_pts(F f, int nx, int nblk) {
 // launch stream around the kernel; aggregated per kernel name on report.
 struct HipProf {
   bool on = false;
+  std::string only;                    // time only the launches of this kernel ("" = all)
   std::vector<hipEvent_t> ev;          // pairs (start, stop)
   size_t used = 0;                     // pairs in use
   std::vector<const char*> name;       // per used pair
@@ -82,6 +83,11 @@ class HipExec {
  public:
   HipExec(int device, void* stream, void** prof_slot);
   int prof_enable(bool on);
+  int prof_only(const char* name) {
+    if (!prof_) return NH_ERR_STATE;
+    prof_->only = name ? name : "";
+    return 0;
+  }
   int prof_report(int max_kernels, char* names, double* total_ms, int64_t* launches);
   void prof_release();
   void* alloc(size_t bytes) {
@@ -113,7 +119,7 @@ class HipExec {
     if (n <= 0) return;
     const int bs = 256;
     long nb = (n + bs - 1) / bs;
-    hipEvent_t* pair = prof_ && prof_->on ? next_pair(F::kName) : nullptr;
+    hipEvent_t* pair = timed(F::kName) ? next_pair(F::kName) : nullptr;
     if (pair) note(hipEventRecord(pair[0], s_), "hipEventRecord");
     hipLaunchKernelGGL(k_run<F>, dim3((unsigned)nb), dim3(bs), 0, s_, f, n);
     note(hipGetLastError(), "kernel launch");
@@ -126,7 +132,7 @@ class HipExec {
     const int nx = (d.g * d.g + bs - 1) / bs;
     const int ny = d.lall * d.kall + 2 * d.kall;
     const int nblk = nx * ny;
-    hipEvent_t* pair = prof_ && prof_->on ? next_pair(F::kName) : nullptr;
+    hipEvent_t* pair = timed(F::kName) ? next_pair(F::kName) : nullptr;
     if (pair) note(hipEventRecord(pair[0], s_), "hipEventRecord");
     hipLaunchKernelGGL(k_pts<F>, dim3(nblk), dim3(bs), 0, s_, f, nx, nblk);
     note(hipGetLastError(), "kernel launch");
@@ -135,7 +141,7 @@ class HipExec {
   }
   template <class F>
   void run_tiles(const F& f, int nblocks, int lds_elems) {
-    hipEvent_t* pair = prof_ && prof_->on ? next_pair(F::kName) : nullptr;
+    hipEvent_t* pair = timed(F::kName) ? next_pair(F::kName) : nullptr;
     if (pair) note(hipEventRecord(pair[0], s_), "hipEventRecord");
     hipLaunchKernelGGL(k_tiles<F>, dim3(nblocks), dim3(256), (size_t)lds_elems * sizeof(typename F::LdsT), s_, f,
                        nblocks);
@@ -159,6 +165,7 @@ class HipExec {
       where_ = where;
     }
   }
+  bool timed(const char* name) const { return prof_ && prof_->on && (prof_->only.empty() || prof_->only == name); }
   hipEvent_t* next_pair(const char* name) {
     HipProf* p = prof_;
     if (p->used * 2 == p->ev.size()) {

@@ -219,6 +219,12 @@ int v_prof_enable(Context* c, bool on, std::string* err) {
   return 0;
 }
 
+int v_prof_only(Context* c, const char* name, std::string* err) {
+  NHX x(c->device, nullptr, &c->prof);
+  if (x.prof_only(name)) return failv(err, NH_ERR_DEVICE, x.message());
+  return 0;
+}
+
 int v_prof_report(Context* c, int max_kernels, char* names, double* total_ms, int64_t* launches,
                   std::string* err) {
   NHX x(c->device, nullptr, &c->prof);

@@ -40,6 +40,7 @@
   int v_sfc_pressure(Context* c, const void* prog, const void* progq, double* ps, const double* ps_b,       \
                      void* prog_b, void* stream, std::string* err);                                         \
   int v_prof_enable(Context* c, bool on, std::string* err);                                                 \
+  int v_prof_only(Context* c, const char* name, std::string* err);                                          \
   int v_prof_report(Context* c, int max_kernels, char* names, double* total_ms, int64_t* launches,          \
                     std::string* err);                                                                      \
   }                                                                                                         \

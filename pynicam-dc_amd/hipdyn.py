@@ -115,6 +115,10 @@ class HipContext:
         """Start/stop per-launch kernel timers (events on the launch stream)."""
         native.check(self.lib, self.lib.nh_prof_enable(self.handle, 1 if on else 0), "nh_prof_enable")
 
+    def prof_only(self, kernel: str | None):
+        """Time only the launches of ``kernel`` (its report name); None = every kernel."""
+        native.check(self.lib, self.lib.nh_prof_only(self.handle, (kernel or "").encode()), "nh_prof_only")
+
     def prof_report(self, max_kernels: int = 64) -> dict:
         """{kernel name: (total ms, launches)} of the launches timed since the last report."""
         names = C.create_string_buffer(32 * max_kernels)

@@ -80,6 +80,7 @@ EXPORTS = {
     "nh_sfc_pressure": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "nh_sfc_pressure_adj": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "nh_prof_enable": (C.c_int, [C.c_void_p, C.c_int32]),
+    "nh_prof_only": (C.c_int, [C.c_void_p, C.c_char_p]),
     "nh_prof_report": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.POINTER(C.c_double),
                                  C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
 }

@@ -17,6 +17,7 @@ namespace nh {
 
 struct HostProf {
   bool on = false;
+  std::string only;
   std::vector<std::pair<const char*, double>> rec;
 };
 
@@ -27,6 +28,11 @@ class HostExec {
       if (!*prof_slot) *prof_slot = new HostProf();
       prof_ = static_cast<HostProf*>(*prof_slot);
     }
+  }
+  int prof_only(const char* name) {
+    if (!prof_) return -2;
+    prof_->only = name ? name : "";
+    return 0;
   }
   int prof_enable(bool on) {
     if (!prof_) return -2;
@@ -69,7 +75,7 @@ class HostExec {
     auto t0 = std::chrono::steady_clock::now();
 #pragma omp parallel for schedule(static)
     for (long t = 0; t < n; ++t) f(t);
-    if (prof_ && prof_->on)
+    if (prof_ && prof_->on && (prof_->only.empty() || prof_->only == F::kName))
       prof_->rec.emplace_back(F::kName,
                               std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
   }
@@ -81,7 +87,7 @@ class HostExec {
 #pragma omp parallel for schedule(static)
     for (int y = 0; y < ny; ++y)
       for (int x = 0; x < nx; ++x) f(y, x);
-    if (prof_ && prof_->on)
+    if (prof_ && prof_->on && (prof_->only.empty() || prof_->only == F::kName))
       prof_->rec.emplace_back(F::kName,
                               std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
   }
@@ -99,7 +105,7 @@ class HostExec {
         for (int t = 0; t < 256; ++t) f.phase2(b, t, lds.data());
       }
     }
-    if (prof_ && prof_->on)
+    if (prof_ && prof_->on && (prof_->only.empty() || prof_->only == F::kName))
       prof_->rec.emplace_back(F::kName,
                               std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
   }
