@@ -192,6 +192,23 @@ int nh_dyn_step_f32(nh_context* ctx, float* prog, float* progq, int32_t nq, doub
 int nh_tracer_step(nh_context* ctx, const double* prog, double* progq, int32_t nq, double dtl, void* stream);
 int nh_tracer_step_f32(nh_context* ctx, const float* prog, float* progq, int32_t nq, double dtl, void* stream);
 
+/* Adjoint of one nh_tracer_step with respect to the tracer masses (no
+ * counterpart in the reference, which has no adjoint).  prog (the mass
+ * fluxes) and progq (the tracers the forward step starts from) are unchanged;
+ * progq_b holds on entry the gradient of a scalar with respect to the tracers
+ * after the step, on exit its gradient with respect to progq, every point
+ * including the halo points.  The fluxes are held fixed and the limiters'
+ * min/max/clip selections and upwind switches are those of the forward pass
+ * from progq, which makes the step linear in progq: the transpose is the
+ * exact derivative wherever no selection is tied.  Uses the adjoint
+ * workspace of nh_dyn_step_adj_tape (allocated by the first call); needs the
+ * halo map with unique destinations that are never sources (else
+ * NH_ERR_STATE). */
+int nh_tracer_step_adj(nh_context* ctx, const double* prog, const double* progq, int32_t nq, double dtl,
+                       double* progq_b, void* stream);
+int nh_tracer_step_adj_f32(nh_context* ctx, const float* prog, const float* progq, int32_t nq, double dtl,
+                           float* progq_b, void* stream);
+
 /* Prescribed winds of the offline tracer runs (Forcing.forcing_update,
  * mod_forcing.py:113-206): rhogv = v(t) rhog, rhogw = w(t) rhog at every
  * point, prog's density unchanged.

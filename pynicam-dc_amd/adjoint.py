@@ -19,6 +19,11 @@ recomputes the stages and small steps inside).
 
 gives dJ/dprog0.  Tracer masses are carried along the trajectory but held
 fixed in the transpose: the gradient is exact when they are zero (dry runs).
+
+``tracer_forecast(run, progq, nsteps)`` is the same for the offline tracer
+transport (TIME_integ_type 'TRCADV', mod_dynamics.py:2030): the tracers after
+``nsteps`` steps of prescribed winds, differentiable with respect to the
+initial tracers (``nh_tracer_step_adj`` per step).
 """
 from __future__ import annotations
 
@@ -100,7 +105,40 @@ def _make_functions():
             fctx.dyn.sfc_pressure_adj(prog, progq, grad.detach(), gb)
             return gb, None, None
 
-    return Forecast, SfcPressure
+    class TracerForecast(torch.autograd.Function):
+        """progq -> tracers after ``nsteps`` offline tracer steps (winds prescribed)."""
+
+        @staticmethod
+        def forward(fctx, progq, run, nsteps):
+            ctx = run.ctx
+            if progq.dtype != ctx.dtype or progq.device != ctx.device or tuple(progq.shape) != tuple(run.progq.shape):
+                raise ValueError(f"expected a {tuple(run.progq.shape)} {ctx.dtype} tensor on {ctx.device}")
+            saved_q, saved_prog, saved_time = run.progq, run.prog.clone(), run.time
+            work = progq.detach().clone().contiguous()
+            checkpoints = []
+            run.progq = work
+            try:
+                for _ in range(nsteps):
+                    checkpoints.append((run.prog.clone(), work.clone()))
+                    run.dynamics_step()
+            finally:
+                run.progq = saved_q
+                run.prog.copy_(saved_prog)
+                run.time = saved_time
+            fctx.run = run
+            fctx.checkpoints = checkpoints
+            return work
+
+        @staticmethod
+        def backward(fctx, grad):
+            run = fctx.run
+            gb = grad.detach().to(dtype=run.ctx.dtype).contiguous().clone()
+            for prog, q in reversed(fctx.checkpoints):
+                run.tracer_step_adj(prog, q, gb)
+            fctx.checkpoints = None
+            return gb, None, None
+
+    return Forecast, SfcPressure, TracerForecast
 
 
 _FN = {}
@@ -108,7 +146,7 @@ _FN = {}
 
 def _fn(name):
     if not _FN:
-        _FN["forecast"], _FN["sfc"] = _make_functions()
+        _FN["forecast"], _FN["sfc"], _FN["tracer"] = _make_functions()
     return _FN[name]
 
 
@@ -123,6 +161,18 @@ def forecast(dyn, prog, nsteps: int, store: str = "tape"):
     if store not in ("tape", "stages"):
         raise ValueError("store must be 'tape' or 'stages'")
     return _fn("forecast").apply(prog, dyn, int(nsteps), store)
+
+
+def tracer_forecast(run, progq, nsteps: int):
+    """Tracers after ``nsteps`` steps of the offline tracer run ``run`` (a
+    ``HipOfflineTracer``) from ``progq``, differentiable with respect to ``progq``.
+
+    The winds are prescribed (they do not depend on the tracers); ``run``'s own
+    tracers, winds and time are left unchanged.  Each step keeps its winds and
+    input tracers; the backward pass calls ``nh_tracer_step_adj`` per step,
+    newest first, with the limiters' selections of the forward pass.
+    """
+    return _fn("tracer").apply(progq, run, int(nsteps))
 
 
 def sfc_pressure(dyn, prog, progq=None):

@@ -11,6 +11,7 @@
 
 #include "nh_adjoint.h"
 #include "nh_engine.h"
+#include "nh_tracer_adj.h"
 
 NH_BEGIN
 
@@ -427,6 +428,181 @@ int sfc_pressure(X& x, const Context& ctx, const real* prog, const real* rhogq, 
   SfcPresP f{ctx.d, ctx.c, ctx.G, prog, rhogq, ps, ps_b, prog_b};
   x.run(f, ncols(ctx.d));
   return x.check(ps_b ? "nh_sfc_pressure_adj" : "nh_sfc_pressure");
+}
+
+// ----------------------------------------------------------------------------
+// adjoint of the offline tracer step (tracer_step) with respect to the tracer
+// masses; the mass fluxes of prog are held fixed, the limiters' switches are
+// those of the forward pass from progq.  progq_b: bar of the tracers after
+// the step on entry, bar of progq on exit (every point, halo points
+// included: the caller's halo values are inputs of the step too).
+// Each tracer is independent: its transport is recomputed up to the
+// flux-form update, then the transposes run in reverse order.
+// ----------------------------------------------------------------------------
+enum TrcAdjField {
+  TA_RQ1 = 0,      // 1  tracer after the first vertical half step
+  TA_RQHB = 1,     // 1  bar of the horizontal step's output
+  TA_QAB = 2,      // 3  bars of the upwind face values
+  TA_QOB = 5,      // 2  bars of the outflow bounds
+  TA_GQB = 7,      // 3  bars of the gradient
+  TA_IB = 10,      // 1  bar of the tracer before the first half step
+  TA_QHS = 11,     // 5  face values of the vertical half step as linear forms
+  TA_NFIELDS = 16
+};
+
+template <class X>
+int tracer_step_adj(X& x, const Context& ctx, const real* prog, const real* progq, int nq, double dtl,
+                    real* progq_b) {
+  const Dims& d = ctx.d;
+  const long NF = d.NF, nc = ncols(d);
+  if (nq < 1 || !prog || !progq || !progq_b) return NH_ERR_ARG;
+  if (!ctx.aws || adj_fields(ctx.adj_ck) < TA_NFIELDS) return NH_ERR_STATE;
+  const real dt = (real)dtl;
+  const int lim = ctx.c.THUBURN_LIM;
+  real* frhog = ctx.field(W_FRHOG);
+  real* rhog1 = ctx.field(W_RHOG1);
+  real* rhog2 = ctx.field(W_RHOG2);
+  real* flxv = ctx.field(W_FLXV);
+  real* fa[3] = {ctx.field(W_FA), ctx.field(W_FA + 1), ctx.field(W_FA + 2)};
+  real* xc[9];
+  for (int q = 0; q < 9; ++q) xc[q] = ctx.field(W_XC + q);
+  real* gq[3] = {ctx.field(W_GQ), ctx.field(W_GQ + 1), ctx.field(W_GQ + 2)};
+  real* qa[3] = {ctx.field(W_QA), ctx.field(W_QA + 1), ctx.field(W_QA + 2)};
+  real* qout[2] = {ctx.field(W_QOUT), ctx.field(W_QOUT + 1)};
+  real* rqh = ctx.field(W_RQ);
+  real* rq1 = afield(ctx, TA_RQ1);
+  real* rqh_b = afield(ctx, TA_RQHB);
+  real* qa_b[3] = {afield(ctx, TA_QAB), afield(ctx, TA_QAB + 1), afield(ctx, TA_QAB + 2)};
+  real* qo_b[2] = {afield(ctx, TA_QOB), afield(ctx, TA_QOB + 1)};
+  real* gq_b[3] = {afield(ctx, TA_GQB), afield(ctx, TA_GQB + 1), afield(ctx, TA_GQB + 2)};
+  real* ib = afield(ctx, TA_IB);
+  real* qhs = afield(ctx, TA_QHS);
+  x.run(FillZero{frhog}, NF);
+  {
+    TrcHFlux f;
+    f.d = d; f.c = ctx.c; f.G = ctx.G;
+    f.dt = dt; f.pm = prog;
+    for (int q = 0; q < 3; ++q) f.fa[q] = fa[q];
+    for (int q = 0; q < 9; ++q) f.xc[q] = xc[q];
+    x.run(f, nc);
+  }
+  halo_block_adj(x, ctx, progq_b, nq);   // the final halo fill of progq
+  for (int iq = 0; iq < nq; ++iq) {
+    const real* q0 = progq + (long)iq * NF;
+    real* rq_b = progq_b + (long)iq * NF;
+    // forward: first half step (also the vertical flux and rhog1), then the
+    // horizontal step up to its output rqh
+    x.d2d(rq1, q0, NF * sizeof(real));
+    {
+      TrcVert1 f;
+      f.d = d; f.c = ctx.c; f.G = ctx.G;
+      f.dt = dt; f.rw_mean = prog + 4 * NF; f.rhog_in = prog; f.frhog = frhog;
+      f.flxv = flxv; f.rhog1 = rhog1; f.rhogq = rq1; f.nq = 1; f.limiter = lim;
+      x.run(f, nc);
+    }
+    if (iq == 0) {
+      TrcRhog2 f;
+      f.d = d; f.c = ctx.c; f.G = ctx.G;
+      f.dt = dt;
+      for (int q = 0; q < 3; ++q) f.fa[q] = fa[q];
+      f.rhog1 = rhog1; f.frhog = frhog; f.rhog2 = rhog2;
+      x.run(f, nc);
+    }
+    {
+      TrcGradQ f;
+      f.d = d; f.G = ctx.G; f.rq = rq1; f.rhog1 = rhog1;
+      for (int q = 0; q < 3; ++q) f.gq[q] = gq[q];
+      x.run(f, nc);
+    }
+    halo(x, ctx, gq, 3);
+    {
+      TrcRemap f;
+      f.d = d; f.c = ctx.c; f.G = ctx.G;
+      f.dt = dt; f.rq = rq1; f.rhog1 = rhog1;
+      for (int q = 0; q < 3; ++q) { f.gq[q] = gq[q]; f.fa[q] = fa[q]; f.qa[q] = qa[q]; }
+      for (int q = 0; q < 9; ++q) f.xc[q] = xc[q];
+      x.run(f, nc);
+    }
+    if (lim) {
+      TrcQout f;
+      f.d = d; f.c = ctx.c; f.G = ctx.G;
+      f.dt = dt; f.rq = rq1; f.rhog1 = rhog1; f.frhog = frhog;
+      for (int q = 0; q < 3; ++q) f.fa[q] = fa[q];
+      f.qout[0] = qout[0]; f.qout[1] = qout[1];
+      x.run(f, nc);
+      halo(x, ctx, qout, 2);
+    }
+    {
+      TrcApply f;
+      f.d = d; f.c = ctx.c; f.G = ctx.G;
+      f.dt = dt; f.rq = rq1; f.rhog1 = rhog1;
+      for (int q = 0; q < 3; ++q) { f.fa[q] = fa[q]; f.qa[q] = qa[q]; }
+      f.qout[0] = qout[0]; f.qout[1] = qout[1];
+      f.out = rqh; f.limiter = lim;
+      x.run(f, nc);
+    }
+    // reverse; rq_b becomes the bar of rq1 once the compute cells' bars have
+    // gone through the second half step (elsewhere rq1 passes unchanged)
+    zero_fields(x, rqh_b, TA_QHS - TA_RQHB, NF);
+    {
+      TrcVert2A f;
+      f.d = d; f.c = ctx.c; f.G = ctx.G;
+      f.dt = dt; f.flxv = flxv; f.rhog2 = rhog2; f.frhog = frhog;
+      f.in = rqh; f.ob = rq_b; f.ib = rqh_b; f.qhs = qhs; f.limiter = lim;
+      x.run(f, nc);
+    }
+    TgSink sk;
+    sk.rhog1 = rhog1; sk.rq_b = rq_b;
+    for (int q = 0; q < 3; ++q) sk.qa_b[q] = qa_b[q];
+    for (int q = 0; q < 2; ++q) sk.qo_b[q] = qo_b[q];
+    {
+      TrcApplyA f;
+      f.d = d; f.c = ctx.c; f.G = ctx.G;
+      f.dt = dt; f.rq = rq1; f.rhog1 = rhog1;
+      for (int q = 0; q < 3; ++q) { f.fa[q] = fa[q]; f.qa[q] = qa[q]; }
+      f.qout[0] = qout[0]; f.qout[1] = qout[1];
+      f.ob = rqh_b; f.sk = sk; f.limiter = lim;
+      x.run(f, nc);
+    }
+    if (lim) {
+      halo_adj(x, ctx, qo_b, 2);
+      TrcQoutA f;
+      f.d = d; f.c = ctx.c; f.G = ctx.G;
+      f.dt = dt; f.rq = rq1; f.rhog1 = rhog1; f.frhog = frhog;
+      for (int q = 0; q < 3; ++q) f.fa[q] = fa[q];
+      f.qo_b[0] = qo_b[0]; f.qo_b[1] = qo_b[1];
+      f.sk = sk;
+      x.run(f, nc);
+    }
+    {
+      TrcRemapA f;
+      f.d = d; f.c = ctx.c; f.G = ctx.G;
+      f.dt = dt; f.rhog1 = rhog1;
+      for (int q = 0; q < 3; ++q) { f.fa[q] = fa[q]; f.qa_b[q] = qa_b[q]; f.gq_b[q] = gq_b[q]; }
+      for (int q = 0; q < 9; ++q) f.xc[q] = xc[q];
+      f.rq_b = rq_b;
+      x.run(f, nc);
+    }
+    halo_adj(x, ctx, gq_b, 3);
+    {
+      TrcGradQA f;
+      f.d = d; f.G = ctx.G; f.rhog1 = rhog1;
+      for (int q = 0; q < 3; ++q) f.gq_b[q] = gq_b[q];
+      f.rq_b = rq_b;
+      x.run(f, nc);
+    }
+    {
+      TrcVert1A f;
+      f.d = d; f.c = ctx.c; f.G = ctx.G;
+      f.dt = dt; f.rw_mean = prog + 4 * NF; f.rhog_in = prog; f.frhog = frhog;
+      f.in = q0; f.ob = rq_b; f.ib = ib; f.qhs = qhs; f.limiter = lim;
+      x.run(f, nc);
+    }
+    x.d2d(rq_b, ib, NF * sizeof(real));
+    int rc = x.check("tracer_step_adj");
+    if (rc) return rc;
+  }
+  return 0;
 }
 
 NH_END

@@ -111,6 +111,17 @@ inline int tracer_step(nh_context* ctx, const void* prog, void* progq, int32_t n
   return done(NH_DISPATCH(ctx, v_tracer_step, prog, progq, nq, dtl, stream, &err), where, err);
 }
 
+inline int tracer_step_adj(nh_context* ctx, const void* prog, const void* progq, int32_t nq, double dtl,
+                           void* progq_b, void* stream, int f32) {
+  const char* where = f32 ? "nh_tracer_step_adj_f32" : "nh_tracer_step_adj";
+  if (int rc = check_ready(ctx, where, f32, true)) return rc;
+  if (!prog || !progq || nq < 1 || !progq_b)
+    return fail(NH_ERR_ARG, where, "need prog, progq_b and at least one tracer field");
+  if (!(dtl > 0.0)) return fail(NH_ERR_ARG, where, "dtl must be > 0");
+  std::string err;
+  return done(NH_DISPATCH(ctx, v_tracer_step_adj, prog, progq, nq, dtl, progq_b, stream, &err), where, err);
+}
+
 inline int trcadv_winds(nh_context* ctx, void* prog, double time, int32_t kind, const double* params, void* stream,
                         int f32) {
   const char* where = f32 ? "nh_trcadv_winds_f32" : "nh_trcadv_winds";
@@ -318,6 +329,15 @@ int nh_prof_report(nh_context* ctx, int32_t max_kernels, char* names, double* to
   if (n < 0) return nh::fail(n, where, err);
   *n_kernels = n;
   return 0;
+}
+
+int nh_tracer_step_adj(nh_context* ctx, const double* prog, const double* progq, int32_t nq, double dtl,
+                       double* progq_b, void* stream) {
+  return nh::tracer_step_adj(ctx, prog, progq, nq, dtl, progq_b, stream, 0);
+}
+int nh_tracer_step_adj_f32(nh_context* ctx, const float* prog, const float* progq, int32_t nq, double dtl,
+                           float* progq_b, void* stream) {
+  return nh::tracer_step_adj(ctx, prog, progq, nq, dtl, progq_b, stream, 1);
 }
 
 int nh_dyn_step_adj(nh_context* ctx, const double* prog, const double* progq, int32_t nq, double dtl,

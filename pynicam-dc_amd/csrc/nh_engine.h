@@ -391,7 +391,7 @@ void tracer_transport(X& x, const Context& ctx, real* progq, int nq, double dt, 
       f.dt = (real)dt; f.rq = rq; f.rhog1 = rhog1;
       for (int q = 0; q < 3; ++q) { f.fa[q] = fa[q]; f.qa[q] = qa[q]; }
       f.qout[0] = qout[0]; f.qout[1] = qout[1];
-      f.out = rqh;
+      f.out = rqh; f.limiter = ctx.c.THUBURN_LIM;
       x.run(f, nc);
     }
     {

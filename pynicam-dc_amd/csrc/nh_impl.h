@@ -204,6 +204,19 @@ int v_dyn_step_adj(Context* c, const void* prog, const void* progq, int nq, doub
   return 0;
 }
 
+int v_tracer_step_adj(Context* c, const void* prog, const void* progq, int nq, double dtl, void* progq_b,
+                      void* stream, std::string* err) {
+  if (!c->halo_adj_ok)
+    return failv(err, NH_ERR_STATE, "halo map has a destination listed twice or used as a source");
+  NHX x(c->device, stream, &c->prof);
+  if (int rc = adj_workspace(x, c, 0, err)) return rc;
+  int rc = tracer_step_adj(x, *c, (const real*)prog, (const real*)progq, nq, dtl, (real*)progq_b);
+  if (rc == NH_ERR_ARG) return failv(err, rc, "invalid argument");
+  if (rc == NH_ERR_STATE) return failv(err, rc, "adjoint workspace missing");
+  if (rc) return failv(err, NH_ERR_DEVICE, x.message());
+  return 0;
+}
+
 int v_sfc_pressure(Context* c, const void* prog, const void* progq, double* ps, const double* ps_b, void* prog_b,
                    void* stream, std::string* err) {
   NHX x(c->device, stream, &c->prof);

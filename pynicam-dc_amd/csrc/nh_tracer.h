@@ -560,12 +560,26 @@ struct TrcApply {
   const real* qa[3];
   const real* qout[2];
   real* out;
+  int limiter;      // 0: the face values go in unlimited (mod_src_tracer.py:789)
   NH_HD void operator()(long t) const {
     Cell cl = cell_of(d, t);
     if (cl.pole ? cl.v != 0 : !cl.inner) return;
     const int g = d.g;
     for (int k = 0; k < d.kall; ++k) {
       const long a = cl.base + k * cl.ks;
+      if (!limiter) {
+        if (cl.pole) {
+          real r = rq[a];
+          for (int v = 1; v <= 5; ++v) r -= fa[0][a + v] * G.rarea_pl[cl.lp] * dt * qa[0][a + v];
+          out[a] = r;
+        } else {
+          real f[6];
+          for (int m = 0; m < 6; ++m) f[m] = face_flux(G, d, fa, a, cl.ci, cl.i, cl.j, cl.l, m, dt);
+          out[a] = rq[a] - (f[0] * qa[0][a] + f[1] * qa[1][a] + f[2] * qa[2][a] + f[3] * qa[0][a - 1]
+                            + f[4] * qa[1][a - 1 - g] + f[5] * qa[2][a - g]);
+        }
+        continue;
+      }
       if (cl.pole) {
         const real ra = G.rarea_pl[cl.lp];
         const real q0 = rq[a] / rhog1[a];

@@ -37,6 +37,8 @@
                std::string* err);                                                                           \
   int v_dyn_step_adj(Context* c, const void* prog, const void* progq, int nq, double dtl, int sstep_max,      \
                      void* prog_b, const void* stage_in, void* stream, std::string* err);                   \
+  int v_tracer_step_adj(Context* c, const void* prog, const void* progq, int nq, double dtl, void* progq_b,    \
+                        void* stream, std::string* err);                                                      \
   int v_sfc_pressure(Context* c, const void* prog, const void* progq, double* ps, const double* ps_b,       \
                      void* prog_b, void* stream, std::string* err);                                         \
   int v_prof_enable(Context* c, bool on, std::string* err);                                                 \

@@ -350,3 +350,21 @@ class HipOfflineTracer(HipDynamics):
                                              self.ctx.stream()), "nh_tracer_step")
         self.time += self.dtl
         self.set_winds(self.time)
+
+    def tracer_step_adj(self, prog, progq, progq_b):
+        """Transpose of one tracer step with respect to the tracers, in place on ``progq_b``.
+
+        ``prog``: the winds (mass fluxes) the step ran with, ``progq``: the tracers
+        it started from.  On entry ``progq_b`` is the gradient of a scalar with
+        respect to the tracers after the step, on exit with respect to ``progq``.
+        The fluxes are held fixed and the limiters' selections are those of the
+        forward pass (the offline-tracer sensitivity).
+        """
+        ctx = self.ctx
+        for a in (prog, progq, progq_b):
+            ctx._check_tensor(a)
+        if progq_b.shape != progq.shape:
+            raise ValueError("progq_b must have the shape of progq")
+        native.check(ctx.lib, ctx.fn("nh_tracer_step_adj")(
+            ctx.handle, C.c_void_p(prog.data_ptr()), C.c_void_p(progq.data_ptr()), progq.shape[0], self.dtl,
+            C.c_void_p(progq_b.data_ptr()), ctx.stream()), "nh_tracer_step_adj")

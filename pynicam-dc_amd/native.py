@@ -59,6 +59,8 @@ EXPORTS = {
     "nh_dyn_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_int32,
                               C.c_void_p]),
     "nh_tracer_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_void_p]),
+    "nh_tracer_step_adj": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_void_p,
+                                     C.c_void_p]),
     "nh_trcadv_winds": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int32, C.POINTER(C.c_double),
                                   C.c_void_p]),
     "nh_budget": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double,
@@ -87,7 +89,8 @@ EXPORTS = {
 # the _f32 forms of the field entry points take the same arguments (float fields)
 for _name in ("nh_halo_fill", "nh_laplacian", "nh_divergence", "nh_gradient", "nh_dyn_step", "nh_tracer_step",
               "nh_trcadv_winds", "nh_budget", "nh_dyn_step_adj", "nh_sfc_pressure", "nh_sfc_pressure_adj",
-              "nh_dyn_step_record", "nh_dyn_step_adj_rec", "nh_dyn_step_tape", "nh_dyn_step_adj_tape"):
+              "nh_dyn_step_record", "nh_dyn_step_adj_rec", "nh_dyn_step_tape", "nh_dyn_step_adj_tape",
+              "nh_tracer_step_adj"):
     EXPORTS[_name + "_f32"] = EXPORTS[_name]
 
 PREC_FP64, PREC_MIXED = 0, 1

@@ -222,6 +222,145 @@ def test_adjoint_refuses_a_halo_map_it_cannot_transpose(case_gl04, hostemu_lib):
 
 
 # ----------------------------------------------------------------------------
+# tracer transport (offline tracer step), host build
+# ----------------------------------------------------------------------------
+def _rough_tracers(state, nq=2, amp=0.3, seed=0):
+    """Tracers with mixing ratios 1 + amp * U(0, 1) per point: no two neighbours tie, so
+    every min/max/clip of the limiters has a unique winner."""
+    st = {k: v.copy() for k, v in state.items()}
+    rng = np.random.default_rng(seed)
+    shp, shp_pl = st["PROG"].shape[:-1] + (nq,), st["PROG_pl"].shape[:-1] + (nq,)
+    st["PROGq"] = st["PROG"][..., :1] * (1.0 + amp * rng.random(shp))
+    st["PROGq_pl"] = st["PROG_pl"][..., :1] * (1.0 + amp * rng.random(shp_pl))
+    return st
+
+
+def _trc_run(ctx, state, w0=3.0):
+    """Deformational flow with vertical motion w0 [m/s] at its initial time."""
+    return hipdyn.HipOfflineTracer(ctx, _rough_tracers(state), "deform", (12 * 86400.0, 40.0, w0, 10000.0))
+
+
+def _trc_step(run, prog, q):
+    q = q.clone()
+    ctx = run.ctx
+    native.check(ctx.lib, ctx.fn("nh_tracer_step")(ctx.handle, C.c_void_p(prog.data_ptr()), C.c_void_p(q.data_ptr()),
+                                                  q.shape[0], run.dtl, ctx.stream()), "nh_tracer_step")
+    return q
+
+
+def _column_directions(run, ncols, rng):
+    """Random perturbations of all levels of ``ncols`` random columns, per tracer at its peak."""
+    ctx = run.ctx
+    q0 = run.progq
+    g, ka = ctx.geo.topo.gall_1d, ctx.geo.topo.kall
+    nreg = ctx.ncol - 12
+    sc = q0.abs().amax(dim=1, keepdim=True)
+    for c in rng.integers(0, ctx.ncol, size=ncols):
+        if c < nreg:
+            l, r = divmod(int(c), g * g)
+            idx = [(l * ka + k) * g * g + r for k in range(ka)]
+        else:
+            lp, v = divmod(int(c) - nreg, 6)
+            idx = [nreg * ka + (lp * ka + k) * 6 + v for k in range(ka)]
+        d = q0.new_zeros(q0.shape)
+        d[:, idx] = q0.new_tensor(rng.standard_normal((q0.shape[0], ka))) * sc
+        yield d
+
+
+def test_host_tracer_adjoint_exact_without_limiter(case_gl04, hostemu_lib):
+    """Limiters off: the step is linear in the tracers and J = sum(w q^2) quadratic, so a
+    central difference of any size is exact; agreement to rounding (1e-10)."""
+    import copy
+    import torch
+    const = copy.copy(case_gl04.const)
+    const.THUBURN_LIM = False
+    ctx = hipdyn.HipContext(case_gl04.geo, const, device="cpu", lib_path=hostemu_lib)
+    try:
+        run = _trc_run(ctx, case_gl04.state, w0=0.1)
+        prog, q0 = run.prog.clone(), run.progq.clone()
+        rng = np.random.default_rng(11)
+        w = torch.from_numpy(rng.random(tuple(q0.shape)))
+        out = _trc_step(run, prog, q0)
+        gb = 2 * w * out
+        run.tracer_step_adj(prog, q0, gb)
+        sc = q0.abs().amax(dim=1, keepdim=True)
+        J = lambda q: float((w * _trc_step(run, prog, q) ** 2).sum())
+        for _ in range(2):
+            v = torch.from_numpy(rng.standard_normal(tuple(q0.shape))) * sc
+            fd = (J(q0 + 1e-2 * v) - J(q0 - 1e-2 * v)) / 2e-2
+            ad = float((gb * v).sum())
+            assert abs(fd - ad) <= 1e-10 * abs(fd), (fd, ad)
+    finally:
+        ctx.close()
+
+
+def test_host_tracer_adjoint_with_limiters(case_gl04, hostemu_lib):
+    """Thuburn limiters on, horizontal and vertical: the step is piecewise linear, so the
+    check perturbs one column at a time (all levels, both tracers) and dots the field
+    difference with w, which keeps rounding local; agreement 1e-5 on 24 columns."""
+    import torch
+    ctx = hipdyn.HipContext(case_gl04.geo, case_gl04.const, device="cpu", lib_path=hostemu_lib)
+    try:
+        run = _trc_run(ctx, case_gl04.state)
+        prog, q0 = run.prog.clone(), run.progq.clone()
+        rng = np.random.default_rng(12)
+        w = torch.from_numpy(rng.random(tuple(q0.shape)))
+        gb = w.clone()
+        run.tracer_step_adj(prog, q0, gb)
+        eps = 1e-6
+        for v in _column_directions(run, 24, rng):
+            fd = float((w * (_trc_step(run, prog, q0 + eps * v) - _trc_step(run, prog, q0 - eps * v))).sum()) / (2 * eps)
+            ad = float((gb * v).sum())
+            assert abs(fd - ad) <= 1e-5 * abs(fd), (fd, ad)
+    finally:
+        ctx.close()
+
+
+def test_host_tracer_forecast_autograd_and_mixed(case_gl04, hostemu_lib):
+    """Three steps through torch.autograd equal three direct adjoint calls; limiters off,
+    the mixed form's gradient agrees with FP64's to 1e-5 of its peak."""
+    import copy
+    import torch
+    const = copy.copy(case_gl04.const)
+    const.THUBURN_LIM = False
+    grads = {}
+    for prec in ("float64", "mixed"):
+        ctx = hipdyn.HipContext(case_gl04.geo, const, device="cpu", lib_path=hostemu_lib, precision=prec)
+        try:
+            run = _trc_run(ctx, case_gl04.state, w0=0.1)
+            w = torch.from_numpy(np.random.default_rng(13).random(tuple(run.progq.shape))).to(ctx.dtype)
+            q = run.progq.clone().requires_grad_(True)
+            t0, p0 = run.time, run.prog.clone()
+            (w * adjoint.tracer_forecast(run, q, 3)).sum().backward()
+            assert run.time == t0 and torch.equal(run.prog, p0)
+            grads[prec] = q.grad.double()
+            if prec == "float64":
+                traj = []
+                for _ in range(3):
+                    traj.append((run.prog.clone(), run.progq.clone()))
+                    run.dynamics_step()
+                gb = w.clone()
+                for p, qq in reversed(traj):
+                    run.tracer_step_adj(p, qq, gb)
+                assert torch.equal(gb, q.grad)
+        finally:
+            ctx.close()
+    sc = grads["float64"].abs().max()
+    assert (grads["float64"] - grads["mixed"]).abs().max() <= 1e-5 * sc
+
+
+def test_tracer_adjoint_argument_errors(emu64):
+    lib = emu64.lib
+    prog, q = emu64.empty(6), emu64.empty(1)
+    p = [C.c_void_p(a.data_ptr()) for a in (prog, q)]
+    assert lib.nh_tracer_step_adj(emu64.handle, p[0], p[1], 1, 1200.0, C.c_void_p(None), C.c_void_p(None)) == -1
+    assert b"nh_tracer_step_adj" in lib.nh_last_error()
+    assert lib.nh_tracer_step_adj(emu64.handle, p[0], p[1], 0, 1200.0, p[1], C.c_void_p(None)) == -1
+    assert lib.nh_tracer_step_adj(emu64.handle, p[0], p[1], 1, 0.0, p[1], C.c_void_p(None)) == -1
+    assert lib.nh_tracer_step_adj_f32(emu64.handle, p[0], p[1], 1, 1200.0, p[1], C.c_void_p(None)) == -2
+
+
+# ----------------------------------------------------------------------------
 # gfx950 build
 # ----------------------------------------------------------------------------
 @pytest.mark.gpu
@@ -260,3 +399,43 @@ def test_gpu_adjoint_mixed_matches_fp64(case_gl05):
     for q in range(6):
         sc = g64[q].abs().max()
         assert sc > 0 and (g64[q] - g32[q]).abs().max() <= 1e-3 * sc, q
+
+
+@pytest.mark.gpu
+def test_gpu_tracer_adjoint_matches_finite_differences(case_gl05):
+    """gl05 z40, the offline tracer step on gfx950: limiters off, global directions (exact
+    central differences, 1e-10); limiters on, 12 single-column directions (1e-5)."""
+    import copy
+    import torch
+    print("gpu tracer adjoint fd: gl05 z40 float64", flush=True)
+    for lim in (False, True):
+        const = copy.copy(case_gl05.const)
+        const.THUBURN_LIM = lim
+        ctx = hipdyn.HipContext(case_gl05.geo, const, device="cuda:0")
+        try:
+            run = _trc_run(ctx, case_gl05.state, w0=3.0 if lim else 0.1)
+            prog, q0 = run.prog.clone(), run.progq.clone()
+            rng = np.random.default_rng(21)
+            w = torch.from_numpy(rng.random(tuple(q0.shape))).to(q0.device)
+            if not lim:
+                gb = 2 * w * _trc_step(run, prog, q0)
+            else:
+                gb = w.clone()
+            run.tracer_step_adj(prog, q0, gb)
+            torch.cuda.synchronize()
+            if not lim:
+                sc = q0.abs().amax(dim=1, keepdim=True)
+                J = lambda q: float((w * _trc_step(run, prog, q) ** 2).sum())
+                v = torch.from_numpy(rng.standard_normal(tuple(q0.shape))).to(q0.device) * sc
+                fd = (J(q0 + 1e-2 * v) - J(q0 - 1e-2 * v)) / 2e-2
+                ad = float((gb * v).sum())
+                assert abs(fd - ad) <= 1e-10 * abs(fd), (fd, ad)
+            else:
+                eps = 1e-6
+                for v in _column_directions(run, 12, rng):
+                    d = _trc_step(run, prog, q0 + eps * v) - _trc_step(run, prog, q0 - eps * v)
+                    fd = float((w * d).sum()) / (2 * eps)
+                    ad = float((gb * v).sum())
+                    assert abs(fd - ad) <= 1e-5 * abs(fd), (fd, ad)
+        finally:
+            ctx.close()
