@@ -539,6 +539,8 @@ int tracer_step_adj(X& x, const Context& ctx, const real* prog, const real* prog
       for (int q = 0; q < 3; ++q) { f.fa[q] = fa[q]; f.qa[q] = qa[q]; }
       f.qout[0] = qout[0]; f.qout[1] = qout[1];
       f.out = rqh; f.limiter = lim;
+      for (int q = 0; q < 3; ++q) f.gq[q] = gq[q];
+      for (int q = 0; q < 9; ++q) f.xc[q] = xc[q];
       x.run(f, nc);
     }
     // reverse; rq_b becomes the bar of rq1 once the compute cells' bars have
@@ -562,6 +564,8 @@ int tracer_step_adj(X& x, const Context& ctx, const real* prog, const real* prog
       for (int q = 0; q < 3; ++q) { f.fa[q] = fa[q]; f.qa[q] = qa[q]; }
       f.qout[0] = qout[0]; f.qout[1] = qout[1];
       f.ob = rqh_b; f.sk = sk; f.limiter = lim;
+      for (int q = 0; q < 3; ++q) f.gq_b[q] = gq_b[q];
+      for (int q = 0; q < 9; ++q) f.xc[q] = xc[q];
       x.run(f, nc);
     }
     if (lim) {

@@ -392,6 +392,8 @@ void tracer_transport(X& x, const Context& ctx, real* progq, int nq, double dt, 
       for (int q = 0; q < 3; ++q) { f.fa[q] = fa[q]; f.qa[q] = qa[q]; }
       f.qout[0] = qout[0]; f.qout[1] = qout[1];
       f.out = rqh; f.limiter = ctx.c.THUBURN_LIM;
+      for (int q = 0; q < 3; ++q) f.gq[q] = gq[q];
+      for (int q = 0; q < 9; ++q) f.xc[q] = xc[q];
       x.run(f, nc);
     }
     {

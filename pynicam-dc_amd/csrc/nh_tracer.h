@@ -560,7 +560,28 @@ struct TrcApply {
   const real* qa[3];
   const real* qout[2];
   real* out;
-  int limiter;      // 0: the face values go in unlimited (mod_src_tracer.py:789)
+  // limiter off (mod_src_tracer.py:789): the face values go in unlimited; faces
+  // 3..5 then take the remap's own values of this cell (:1785), upwind by this
+  // cell's mask between the extrapolations from here and from the face's owner
+  int limiter;
+  const real* gq[3];
+  const real* xc[9];
+  NH_HD real face_unlimited(const Cell& cl, long a, int m, real fm) const {
+    if (m < 3) return qa[m][a];
+    const int g = d.g, arc = m - 3;
+    const long o = (m == 3) ? a - 1 : (m == 4) ? a - 1 - g : a - g;
+    const long co = (m == 3) ? cl.ci - 1 : (m == 4) ? cl.ci - 1 - g : cl.ci - g;
+    const long nreg = d.nreg;
+    const real* x0 = xc[arc * 3];
+    const real* x1 = xc[arc * 3 + 1];
+    const real* x2 = xc[arc * 3 + 2];
+    const real am = rq[a] / rhog1[a] + gq[0][a] * (x0[o] - rc(G.gx, nreg, 0, cl.ci))
+                    + gq[1][a] * (x1[o] - rc(G.gx, nreg, 1, cl.ci)) + gq[2][a] * (x2[o] - rc(G.gx, nreg, 2, cl.ci));
+    const real ap = rq[o] / rhog1[o] + gq[0][o] * (x0[o] - rc(G.gx, nreg, 0, co))
+                    + gq[1][o] * (x1[o] - rc(G.gx, nreg, 1, co)) + gq[2][o] * (x2[o] - rc(G.gx, nreg, 2, co));
+    const real cm = upwind_mask(fm / rhog1[a]);
+    return cm * am + (real(1.0) - cm) * ap;
+  }
   NH_HD void operator()(long t) const {
     Cell cl = cell_of(d, t);
     if (cl.pole ? cl.v != 0 : !cl.inner) return;
@@ -573,10 +594,10 @@ struct TrcApply {
           for (int v = 1; v <= 5; ++v) r -= fa[0][a + v] * G.rarea_pl[cl.lp] * dt * qa[0][a + v];
           out[a] = r;
         } else {
-          real f[6];
+          real f[6], s[6];
           for (int m = 0; m < 6; ++m) f[m] = face_flux(G, d, fa, a, cl.ci, cl.i, cl.j, cl.l, m, dt);
-          out[a] = rq[a] - (f[0] * qa[0][a] + f[1] * qa[1][a] + f[2] * qa[2][a] + f[3] * qa[0][a - 1]
-                            + f[4] * qa[1][a - 1 - g] + f[5] * qa[2][a - g]);
+          for (int m = 0; m < 6; ++m) s[m] = face_unlimited(cl, a, m, f[m]);
+          out[a] = rq[a] - (f[0] * s[0] + f[1] * s[1] + f[2] * s[2] + f[3] * s[3] + f[4] * s[4] + f[5] * s[5]);
         }
         continue;
       }

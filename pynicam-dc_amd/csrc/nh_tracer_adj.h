@@ -107,6 +107,8 @@ struct TrcApplyA {
   real* ob;
   TgSink sk;
   int limiter;
+  real* gq_b[3];       // limiter off: bars of the gradient (faces 3..5)
+  const real* xc[9];
   NH_HD void operator()(long t) const {
     Cell cl = cell_of(d, t);
     if (cl.pole ? cl.v != 0 : !cl.inner) return;
@@ -123,8 +125,19 @@ struct TrcApplyA {
         } else {
           real f[6];
           for (int m = 0; m < 6; ++m) f[m] = face_flux(G, d, fa, a, cl.ci, cl.i, cl.j, cl.l, m, dt);
-          const long own[6] = {a, a, a, a - 1, a - 1 - g, a - g};
-          for (int m = 0; m < 6; ++m) aadd(&sk.qa_b[m % 3][own[m]], -f[m] * w);
+          for (int m = 0; m < 3; ++m) aadd(&sk.qa_b[m][a], -f[m] * w);
+          // faces 3..5: cm * (extrapolation from here) + (1 - cm) * (from the owner)
+          for (int m = 3; m < 6; ++m) {
+            const int arc = m - 3;
+            const long o = (m == 3) ? a - 1 : (m == 4) ? a - 1 - g : a - g;
+            const long co = (m == 3) ? cl.ci - 1 : (m == 4) ? cl.ci - 1 - g : cl.ci - g;
+            const real cm = upwind_mask(f[m] / rhog1[a]);
+            const long src = (cm == real(1.0)) ? a : o;
+            const long cs = (cm == real(1.0)) ? cl.ci : co;
+            const real wb = -f[m] * w;
+            aadd(&sk.rq_b[src], wb / rhog1[src]);
+            for (int dd = 0; dd < 3; ++dd) aadd(&gq_b[dd][src], wb * (xc[arc * 3 + dd][o] - rc(G.gx, d.nreg, dd, cs)));
+          }
         }
         continue;
       }

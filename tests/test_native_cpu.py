@@ -198,6 +198,32 @@ def test_host_build_offline_tracer_matches_oracle(case_gl04, state_gl04_tracers,
         assert rel_err(a, b, ap, bp) < 1e-12, q
 
 
+def test_host_build_offline_tracer_without_limiters_matches_oracle(case_gl04, state_gl04_tracers, hostemu_lib):
+    """THUBURN_LIM off: unlimited face values in both directions (mod_src_tracer.py:789,
+    the vertical switch likewise), two TRCADV steps against the oracle."""
+    import copy
+    from oracle import dynamics as od
+    from oracle import trcadv as ot
+    c = case_gl04
+    t = c.topo
+    const = copy.copy(c.const)
+    const.THUBURN_LIM = False
+    ref = ot.OfflineTracer(od.OracleContext(c.geo, const, None), state_gl04_tracers, "deform", DEFORM)
+    ctx = hipdyn.HipContext(c.geo, const, device="cpu", lib_path=hostemu_lib)
+    try:
+        run = hipdyn.HipOfflineTracer(ctx, state_gl04_tracers, "deform", DEFORM)
+        for _ in range(2):
+            ref.step()
+            run.dynamics_step()
+        got = run.state()
+    finally:
+        ctx.close()
+    for q in range(1, got["PROGq"].shape[-1]):
+        a, b = interior(ref.PROGq[..., q], t), interior(got["PROGq"][..., q], t)
+        ap, bp = pole_centre(ref.PROGq_pl[..., q], t), pole_centre(got["PROGq_pl"][..., q], t)
+        assert rel_err(a, b, ap, bp) < 1e-12, q
+
+
 def test_trcadv_winds_argument_errors(emu_ctx):
     lib = emu_ctx.lib
     prog = emu_ctx.empty(6)
