@@ -90,14 +90,14 @@ KERNEL_TYPES = {
     "ViMainPost": "vi_main_post",
     "ProgUpdateP": "prog_update", "HaloGather": "halo_gather", "TrcVert1": "trc_vert1", "TrcVert2": "trc_vert2",
     "TrcHFlux": "trc_hflux", "TrcRhog2": "trc_rhog2", "TrcGradQ": "trc_gradq", "TrcRemap": "trc_remap",
-    "TrcQout": "trc_qout", "TrcApply": "trc_apply", "DivDamp1T": "divdamp_3d", "DivDamp2T": "divdamp_post",
+    "TrcQout": "trc_qout", "TrcApply": "trc_apply", "DivDampK<1>": "divdamp_3d", "DivDampK<2>": "divdamp_post",
     "HDiffPass2T": "hdiff_pass2",
 }
 
 
 def _kernel_key(name: str):
     import re
-    m = re.search(r"k_(?:pts|run|tiles)<nh::f(?:64|32)::(?:StagedPointT<nh::f(?:64|32)::)?(\w+(?:<\d+>)?)\s*>",
+    m = re.search(r"k_(?:pts|run|tiles|cols)<nh::f(?:64|32)::(?:StagedPointT<nh::f(?:64|32)::)?(\w+(?:<\d+>)?)\s*>",
                   name)
     return KERNEL_TYPES.get(m.group(1)) if m else None
 

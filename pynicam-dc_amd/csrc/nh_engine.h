@@ -288,25 +288,29 @@ void halo_block(X& x, const Context& ctx, real* base, int nf) {
 
 inline long ncols(const Dims& d) { return d.nreg + 12; }
 
+// level chunks of the divergence-damping column tiles
+constexpr int kDivDampChunks = 4;
+
 // ----------------------------------------------------------------------------
 // 3-D divergence damping of (vx, vy, vz, w) into out (3 fields)
 // ----------------------------------------------------------------------------
 template <class X>
 void divdamp(X& x, const Context& ctx, const real* vx, const real* vy, const real* vz, const real* w,
              real* out0, real* out1, real* out2) {
-  const TileGeom tg = tile_geom(ctx.d, 2);
-  DivDamp1T a;
-  a.d = ctx.d; a.c = ctx.c; a.G = ctx.G; a.tg = tg;
-  a.rvx = vx; a.rvy = vy; a.rvz = vz; a.rw = w;
+  const ColGeom cg = col_geom(ctx.d, kDivDampChunks);
+  DivDampK<1> a;
+  a.d = ctx.d; a.c = ctx.c; a.G = ctx.G; a.cg = cg;
+  a.v[0] = vx; a.v[1] = vy; a.v[2] = vz; a.rw = w;
   for (int q = 0; q < 3; ++q) a.out[q] = ctx.field(W_DDT + q);
-  x.run_tiles(a, tg.nblocks, DivDamp1T::lds_elems(tg, ctx.d.g));
+  x.run_cols(a, cg.nblocks, DivDampK<1>::lds_elems(cg));
   real* tmp[3] = {ctx.field(W_DDT), ctx.field(W_DDT + 1), ctx.field(W_DDT + 2)};
   halo(x, ctx, tmp, 3);
-  DivDamp2T b;
-  b.d = ctx.d; b.c = ctx.c; b.G = ctx.G; b.tg = tg;
-  for (int q = 0; q < 3; ++q) b.in[q] = tmp[q];
+  DivDampK<2> b;
+  b.d = ctx.d; b.c = ctx.c; b.G = ctx.G; b.cg = cg;
+  for (int q = 0; q < 3; ++q) b.v[q] = tmp[q];
+  b.rw = nullptr;
   b.out[0] = out0; b.out[1] = out1; b.out[2] = out2;
-  x.run_tiles(b, tg.nblocks, DivDamp2T::lds_elems(tg, ctx.d.g));
+  x.run_cols(b, cg.nblocks, DivDampK<2>::lds_elems(cg));
 }
 
 // ----------------------------------------------------------------------------

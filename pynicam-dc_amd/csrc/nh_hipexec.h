@@ -49,6 +49,26 @@ __global__ __launch_bounds__(256) void k_pts(F f, int nx, int nblk) {
   f(plane, xb * (int)blockDim.x + (int)threadIdx.x);
 }
 
+// column tiles: per block init(), then per level level_a, barrier, level_b;
+// the number of levels is uniform over the block
+template <class F>
+__global__ __launch_bounds__(320) void k_cols(F f, int nblk) {
+  extern __shared__ __align__(16) unsigned char nh_lds_bytes[];
+  typename F::LdsT* lds = reinterpret_cast<typename F::LdsT*>(nh_lds_bytes);
+  const int L = blockIdx.x;
+  const int per = (nblk + kXcds - 1) / kXcds;
+  const int xcd = L % kXcds, idx = L / kXcds;
+  const int rem = nblk % kXcds;
+  const int blk = (rem == 0 || xcd < rem) ? xcd * per + idx : rem * per + (xcd - rem) * (per - 1) + idx;
+  typename F::State st;
+  const int nl = f.init(blk, (int)threadIdx.x, st);
+  for (int s = 0; s < nl; ++s) {
+    f.level_a(s, st, lds);
+    __syncthreads();
+    f.level_b(s, st, lds);
+  }
+}
+
 // Kernel timers: one event pair per launch while enabled, recorded on the
 // launch stream around the kernel; aggregated per kernel name on report.
 struct HipProf {
@@ -144,6 +164,16 @@ class HipExec {
     hipEvent_t* pair = timed(F::kName) ? next_pair(F::kName) : nullptr;
     if (pair) note(hipEventRecord(pair[0], s_), "hipEventRecord");
     hipLaunchKernelGGL(k_tiles<F>, dim3(nblocks), dim3(256), (size_t)lds_elems * sizeof(typename F::LdsT), s_, f,
+                       nblocks);
+    note(hipGetLastError(), "kernel launch");
+    if (pair) note(hipEventRecord(pair[1], s_), "hipEventRecord");
+    if (sync_each_) note(hipStreamSynchronize(s_), "kernel execution");
+  }
+  template <class F>
+  void run_cols(const F& f, int nblocks, int lds_elems) {
+    hipEvent_t* pair = timed(F::kName) ? next_pair(F::kName) : nullptr;
+    if (pair) note(hipEventRecord(pair[0], s_), "hipEventRecord");
+    hipLaunchKernelGGL(k_cols<F>, dim3(nblocks), dim3(320), (size_t)lds_elems * sizeof(typename F::LdsT), s_, f,
                        nblocks);
     note(hipGetLastError(), "kernel launch");
     if (pair) note(hipEventRecord(pair[1], s_), "hipEventRecord");

@@ -1,5 +1,6 @@
-// LDS-tiled kernels for the triangle-based operators (3-D divergence damping,
-// the Kh-weighted diffusion of the second hyperdiffusion pass).
+// LDS-tiled kernels for the triangle-based operators: the Kh-weighted
+// diffusion of the second hyperdiffusion pass as level tiles (below), the 3-D
+// divergence damping as column tiles (DivDampK, at the end).
 //
 // A block owns a tile of one region level: all interior i (W = g-2 points,
 // one or more wavefronts) times TH rows.  Phase 1 computes the two triangle
@@ -124,138 +125,6 @@ void run_staged(X& x, const P& pk) {
 }
 
 // ----------------------------------------------------------------------------
-// first operator: triangle-mean 3-D divergence, gradient back to the cells
-// (kernels/oprt3ddivdamp.py:50-173)
-// ----------------------------------------------------------------------------
-struct DivDamp1T {
-  using LdsT = real;
-  static constexpr bool kStage = false;
-  static constexpr const char* kName = "divdamp_3d";
-  static int lds_elems(const TileGeom& tg, int) { return tg.lds; }
-  Dims d; Consts c; Geo G; TileGeom tg;
-  const real *rvx, *rvy, *rvz, *rw;
-  real* out[3];
-  NH_HD real rwv(long aa, int kk) const {
-    return (kk >= d.kmin + 1 && kk <= d.kmax) ? (real(0.0) * real(1.0) + rw[aa] * real(1.0)) : real(0.0);
-  }
-  NH_HD bool active(int k) const { return k >= d.kmin && k <= d.kmax; }
-  NH_HD void phase1(int blk, int tid, real* lds) const {
-    if (blk >= tg.ntiles) return;
-    const TileAt t = tile_at(d, tg, blk);
-    if (!active(t.k)) return;
-    const int g = d.g, W1 = tg.W + 1;
-    const long gg = (long)g * g, ks = gg;
-    const int nc = W1 * (t.rows + 1);
-    for (int cidx = tid; cidx < nc; cidx += 256) {
-      const int ci_ = cidx % W1, cj_ = cidx / W1;
-      const int i = ci_, j = t.j0 - 1 + cj_;
-      const long cj = ((long)t.l * g + j) * g + i;
-      const long a = ((long)(t.l * d.kall + t.k) * g + j) * g + i;
-      for (int T = 0; T < 2; ++T) {
-        const long o1 = (T == 0) ? 1 : 1 + g;
-        const long o2 = (T == 0) ? 1 + g : g;
-        const real s = tri_div(G, d.nreg, cj, T, rvx, rvy, rvz, a, g, real(1.0));
-        const real up = (rwv(a + ks, t.k + 1) + rwv(a + o1 + ks, t.k + 1) + rwv(a + o2 + ks, t.k + 1));
-        const real dn = (rwv(a, t.k) + rwv(a + o1, t.k) + rwv(a + o2, t.k));
-        lds[(T * (t.rows + 1) + cj_) * W1 + ci_] = s + (up - dn) / real(3.0) * G.rdgz[t.k];
-      }
-    }
-  }
-  NH_HD void phase2(int blk, int tid, const real* lds) const {
-    const int g = d.g;
-    if (blk >= tg.ntiles) {
-      pole(blk - tg.ntiles, tid);
-      return;
-    }
-    const TileAt t = tile_at(d, tg, blk);
-    const int W1 = tg.W + 1;
-    const int ti = tid % tg.W, tj = tid / tg.W;
-    if (tj >= t.rows) return;
-    const int i = ti + 1, j = t.j0 + tj;
-    const long ci = ((long)t.l * g + j) * g + i;
-    const long a = ((long)(t.l * d.kall + t.k) * g + j) * g + i;
-    if (!active(t.k)) {
-      for (int q = 0; q < 3; ++q) out[q][a] = real(0.0);
-      return;
-    }
-    const int R = t.rows + 1;
-    auto L = [&](int T, int ii, int jj) { return lds[(T * R + (jj - t.j0 + 1)) * W1 + ii]; };
-    real T6[6] = {L(0, i, j), L(1, i, j), L(0, i - 1, j), L(1, i - 1, j - 1), L(0, i - 1, j - 1), L(1, i, j - 1)};
-    if (i == 1 && j == 1) {
-      const real m0 = G.sgp[t.l] ? real(0.0) : real(1.0), m1 = G.sgp[t.l] ? real(1.0) : real(0.0);
-      T6[4] = T6[4] * m0 + T6[5] * m1;
-    }
-    for (int dd = 0; dd < 3; ++dd) out[dd][a] = hex_sum(G, d.nreg, ci, dd, T6);
-  }
-  NH_HD void pole(int pb, int tid) const {
-    if (tid != 0) return;
-    DivDamp1P p{d, c, G, rvx, rvy, rvz, rw, {out[0], out[1], out[2]}};
-    p(d.lall * d.kall + pb, 0);
-  }
-};
-
-// second operator on -(first result), coefficient, projection
-// (kernels/oprtdivdamp.py:54-173, kernels/divdamppostcomm.py)
-struct DivDamp2T {
-  using LdsT = real;
-  static constexpr bool kStage = false;
-  static constexpr const char* kName = "divdamp_post";
-  static int lds_elems(const TileGeom& tg, int) { return tg.lds; }
-  Dims d; Consts c; Geo G; TileGeom tg;
-  const real* in[3];
-  real* out[3];
-  NH_HD bool active(int k) const { return k >= d.kmin && k <= d.kmax; }
-  NH_HD void phase1(int blk, int tid, real* lds) const {
-    if (blk >= tg.ntiles) return;
-    const TileAt t = tile_at(d, tg, blk);
-    if (!active(t.k)) return;
-    const int g = d.g, W1 = tg.W + 1;
-    const int nc = W1 * (t.rows + 1);
-    for (int cidx = tid; cidx < nc; cidx += 256) {
-      const int ci_ = cidx % W1, cj_ = cidx / W1;
-      const int i = ci_, j = t.j0 - 1 + cj_;
-      const long cj = ((long)t.l * g + j) * g + i;
-      const long a = ((long)(t.l * d.kall + t.k) * g + j) * g + i;
-      for (int T = 0; T < 2; ++T)
-        lds[(T * (t.rows + 1) + cj_) * W1 + ci_] = tri_div(G, d.nreg, cj, T, in[0], in[1], in[2], a, g, -real(1.0));
-    }
-  }
-  NH_HD void phase2(int blk, int tid, const real* lds) const {
-    const int g = d.g;
-    if (blk >= tg.ntiles) {
-      if (tid == 0) {
-        DivDamp2P p{d, c, G, {in[0], in[1], in[2]}, {out[0], out[1], out[2]}};
-        p(d.lall * d.kall + (blk - tg.ntiles), 0);
-      }
-      return;
-    }
-    const TileAt t = tile_at(d, tg, blk);
-    const int W1 = tg.W + 1;
-    const int ti = tid % tg.W, tj = tid / tg.W;
-    if (tj >= t.rows) return;
-    const int i = ti + 1, j = t.j0 + tj;
-    const long ci = ((long)t.l * g + j) * g + i;
-    const long a = ((long)(t.l * d.kall + t.k) * g + j) * g + i;
-    real o[3] = {real(0.0), real(0.0), real(0.0)};
-    if (active(t.k)) {
-      const int R = t.rows + 1;
-      auto L = [&](int T, int ii, int jj) { return lds[(T * R + (jj - t.j0 + 1)) * W1 + ii]; };
-      const real T6[6] = {L(0, i, j), L(1, i, j), L(0, i - 1, j), L(1, i - 1, j - 1), L(0, i - 1, j - 1),
-                            L(1, i, j - 1)};
-      for (int dd = 0; dd < 3; ++dd) o[dd] = hex_sum(G, d.nreg, ci, dd, T6);
-    }
-    const real Rr = c.RADIUS;
-    const real gd[3] = {c.ALPHA_D * o[0], c.ALPHA_D * o[1], c.ALPHA_D * o[2]};
-    const real gx = G.gx[ci], gy = G.gx[d.nreg + ci], gz = G.gx[2 * d.nreg + ci];
-    const real prd = (gd[0] * gx + gd[1] * gy + gd[2] * gz) / Rr;
-    out[0][a] = gd[0] - prd * gx / Rr;
-    out[1][a] = gd[1] - prd * gy / Rr;
-    out[2][a] = gd[2] - prd * gz / Rr;
-  }
-};
-
-
-// ----------------------------------------------------------------------------
 // second hyperdiffusion pass (mod_numfilter.py:1125-1264): Laplacians of the
 // velocity components and w, Kh-weighted diffusion (kernels/oprtdiffusion.py:59-177)
 // of temperature and density; the triangle gradients of the two diffused
@@ -342,6 +211,212 @@ struct HDiffPass2T {
       }
       out[4 + f][a] = -inner;
     }
+  }
+};
+
+// ----------------------------------------------------------------------------
+// divergence damping as column tiles.  A block owns a TW x TW tile of one
+// region's points over a chunk of levels; each thread owns one corner column
+// of the tile (the (TW+1)^2 corners whose two triangles the tile's hexagons
+// use), keeps that column's 2 x 9 triangle interpolation coefficients in
+// registers and, for the TW^2 corners that are also tile points, the 18
+// hexagon coefficients.  Per level: the two triangle values -> LDS (double
+// buffered by level parity, so one barrier per level), then the hexagon sums
+// -> out.  Per point and level this reads the three velocity components at
+// three corners and the new level of w, instead of re-reading 36 coefficients
+// per point and level as the tile kernels above do.  (Issuing the next
+// level's loads before the barrier measured slower: the registers they hold
+// cost more occupancy than the latency they hide.)
+// OP 1: first operator, triangle-mean 3-D divergence and its gradient back to
+// the cells (kernels/oprt3ddivdamp.py:50-173); OP 2: second operator on the
+// negated first result with ALPHA_D and the projection onto the sphere
+// (kernels/oprtdivdamp.py:54-173, kernels/divdamppostcomm.py).  The point
+// forms DivDamp1P / DivDamp2P do the pole cells.
+// ----------------------------------------------------------------------------
+struct ColGeom {
+  int TW;         // tile width in points; divides g - 2 (a power of two)
+  int ntx;        // tiles per region row
+  int nchunk;     // level chunks
+  int KC;         // levels per chunk
+  int ntiles;     // lall * nchunk * ntx * ntx
+  int nblocks;    // + 2 * kall pole blocks
+};
+constexpr int kColThreads = 320;
+
+inline ColGeom col_geom(const Dims& d, int nchunk) {
+  ColGeom c;
+  const int W = d.g - 2;
+  c.TW = (W < 16) ? W : 16;
+  c.ntx = W / c.TW;
+  c.nchunk = nchunk;
+  c.KC = (d.kall + nchunk - 1) / nchunk;
+  c.ntiles = d.lall * nchunk * c.ntx * c.ntx;
+  c.nblocks = c.ntiles + 2 * d.kall;
+  return c;
+}
+
+template <int OP>
+struct DivDampK {
+  using LdsT = real;
+  static constexpr const char* kName = (OP == 1) ? "divdamp_3d" : "divdamp_post";
+  Dims d; Consts c; Geo G; ColGeom cg;
+  const real* v[3];     // OP 1: the momentum; OP 2: the first result (negated inside)
+  const real* rw;       // OP 1: rhogw
+  real* out[3];
+  static int lds_elems(const ColGeom& cg) { return 4 * (cg.TW + 1) * (cg.TW + 1); }
+  struct State {
+    real ci[2][9];      // cintp of the corner column, [T][dd * 3 + vertex]
+    real cd[18];        // cdiff of the tile point, [dd * 6 + m]
+    real gx[3];         // OP 2: the point's position
+    real dn[2];         // OP 1: the lower w sums of the two triangles at this level
+    real pv[12];        // v at the corners a, a+1, a+1+g, a+g of this level, [dd * 4 + n]
+    real pw[4];         // OP 1: rw at the same corners one level up
+    long a;             // the corner's address at this level
+    int k, lidx;
+    bool corner, outp, pent, sgp, have_dn;
+  };
+  NH_HD bool active(int k) const { return k >= d.kmin && k <= d.kmax; }
+  NH_HD void pole(int pb) const {
+    if (OP == 1) {
+      DivDamp1P p{d, c, G, v[0], v[1], v[2], rw, {out[0], out[1], out[2]}};
+      p(d.lall * d.kall + pb, 0);
+    } else {
+      DivDamp2P p{d, c, G, {v[0], v[1], v[2]}, {out[0], out[1], out[2]}};
+      p(d.lall * d.kall + pb, 0);
+    }
+  }
+  // loads of an active level k at the corner address a (k + 1 <= kall - 1)
+  NH_HD void fetch(State& s, long a) const {
+    const long g = d.g;
+    const long off[4] = {0, 1, 1 + g, g};
+    for (int dd = 0; dd < 3; ++dd)
+      for (int n = 0; n < 4; ++n) s.pv[dd * 4 + n] = v[dd][a + off[n]];
+    if (OP == 1)
+      for (int n = 0; n < 4; ++n) s.pw[n] = rw[a + g * g + off[n]];
+  }
+  // the block's number of levels (uniform over the block)
+  NH_HD int init(int blk, int tid, State& s) const {
+    s.corner = s.outp = s.pent = s.sgp = s.have_dn = false;
+    if (blk >= cg.ntiles) {
+      if (tid == 0) pole(blk - cg.ntiles);
+      return 0;
+    }
+    const int g = d.g, W1 = cg.TW + 1, tpr = cg.ntx * cg.ntx;
+    const int tile = blk % tpr, rest = blk / tpr;
+    const int chunk = rest % cg.nchunk, l = rest / cg.nchunk;
+    const int k0 = chunk * cg.KC;
+    const int k1 = (k0 + cg.KC < d.kall) ? k0 + cg.KC : d.kall;
+    if (k0 >= k1) return 0;
+    const int i0 = 1 + (tile % cg.ntx) * cg.TW, j0 = 1 + (tile / cg.ntx) * cg.TW;
+    const int tx = tid % W1, ty = tid / W1;
+    s.k = k0;
+    if (tid >= W1 * W1) return k1 - k0;
+    s.corner = true;
+    s.lidx = ty * W1 + tx;
+    const int i = i0 - 1 + tx, j = j0 - 1 + ty;
+    const long nreg = d.nreg, col = ((long)l * g + j) * g + i;
+    s.a = ((long)(l * d.kall + k0) * g + j) * g + i;
+    for (int T = 0; T < 2; ++T)
+      for (int n = 0; n < 9; ++n) s.ci[T][n] = rc(G.cintp, nreg, n * 2 + T, col);
+    if (tx >= 1 && ty >= 1) {
+      s.outp = true;
+      for (int n = 0; n < 18; ++n) s.cd[n] = G.cdiff[(long)n * nreg + col];
+      if (OP == 1) {
+        s.pent = (i == 1 && j == 1);
+        s.sgp = G.sgp[l] != 0;
+      }
+      if (OP == 2)
+        for (int dd = 0; dd < 3; ++dd) s.gx[dd] = G.gx[(long)dd * nreg + col];
+    }
+    return k1 - k0;
+  }
+  // triangle T of the corner from the level's loads (tri_div's arithmetic)
+  NH_HD real tri(const State& s, int T) const {
+    const int n1 = (T == 0) ? 1 : 2, n2 = (T == 0) ? 2 : 3;
+    const real sgn = (OP == 1) ? real(1.0) : -real(1.0);
+    real r = real(0.0);
+    for (int dd = 0; dd < 3; ++dd) {
+      const real t0 = s.ci[T][dd * 3 + 0] * (sgn * s.pv[dd * 4 + 0]);
+      const real t1 = s.ci[T][dd * 3 + 1] * (sgn * s.pv[dd * 4 + n1]);
+      const real t2 = s.ci[T][dd * 3 + 2] * (sgn * s.pv[dd * 4 + n2]);
+      if (dd == 0) r = t0 + t1 + t2;
+      else r = r + t0 + t1 + t2;
+    }
+    return r;
+  }
+  // the w sums of both triangles (rwv summed over each triangle's corners) from
+  // the four corner values r at level kk.  Outside kmin+1..kmax, r * 0 + 0 * 1
+  // is +0 as rwv's 0 is, so the level test is a factor.
+  NH_HD void wsums(const real r[4], int kk, real w[2]) const {
+    const real m = (kk >= d.kmin + 1 && kk <= d.kmax) ? real(1.0) : real(0.0);
+    const real r00 = real(0.0) * real(1.0) + r[0] * m;
+    const real r10 = real(0.0) * real(1.0) + r[1] * m;
+    const real r11 = real(0.0) * real(1.0) + r[2] * m;
+    const real r01 = real(0.0) * real(1.0) + r[3] * m;
+    w[0] = (r00 + r10 + r11);
+    w[1] = (r00 + r11 + r01);
+  }
+  NH_HD void level_a(int step, State& s, real* lds) const {
+    if (!s.corner) return;
+    const long g = d.g;
+    if (active(s.k)) {
+      const int W1 = cg.TW + 1;
+      real* buf = lds + (step & 1) * 2 * W1 * W1;
+      fetch(s, s.a);
+      real up[2], dn[2];
+      if (OP == 1) {
+        wsums(s.pw, s.k + 1, up);
+        if (s.have_dn) {
+          dn[0] = s.dn[0];
+          dn[1] = s.dn[1];
+        } else {
+          const real r[4] = {rw[s.a], rw[s.a + 1], rw[s.a + 1 + g], rw[s.a + g]};
+          wsums(r, s.k, dn);
+        }
+      }
+      for (int T = 0; T < 2; ++T) {
+        real val = tri(s, T);
+        if (OP == 1) {
+          val = val + (up[T] - dn[T]) / real(3.0) * G.rdgz[s.k];
+          s.dn[T] = up[T];
+        }
+        buf[T * W1 * W1 + s.lidx] = val;
+      }
+      s.have_dn = true;
+    }
+  }
+  NH_HD void level_b(int step, State& s, const real* lds) const {
+    if (s.outp) {
+      const int W1 = cg.TW + 1;
+      real o[3] = {real(0.0), real(0.0), real(0.0)};
+      const bool act = active(s.k);
+      if (act) {
+        const real* b0 = lds + (step & 1) * 2 * W1 * W1;
+        const real* b1 = b0 + W1 * W1;
+        const int x = s.lidx;
+        // {TI(i,j), TJ(i,j), TI(i-1,j), TJ(i-1,j-1), TI(i-1,j-1), TJ(i,j-1)}
+        real T6[6] = {b0[x], b1[x], b0[x - 1], b1[x - 1 - W1], b0[x - 1 - W1], b1[x - W1]};
+        if (OP == 1 && s.pent) {
+          const real m0 = s.sgp ? real(0.0) : real(1.0), m1 = s.sgp ? real(1.0) : real(0.0);
+          T6[4] = T6[4] * m0 + T6[5] * m1;
+        }
+        for (int dd = 0; dd < 3; ++dd) {
+          const real* cd = s.cd + dd * 6;
+          o[dd] = cd[0] * (T6[0] + T6[1]) + cd[1] * (T6[1] + T6[2]) + cd[2] * (T6[2] + T6[3])
+                + cd[3] * (T6[3] + T6[4]) + cd[4] * (T6[4] + T6[5]) + cd[5] * (T6[5] + T6[0]);
+        }
+      }
+      if (OP == 1) {
+        for (int dd = 0; dd < 3; ++dd) out[dd][s.a] = act ? o[dd] : real(0.0);
+      } else {
+        const real Rr = c.RADIUS;
+        const real gd[3] = {c.ALPHA_D * o[0], c.ALPHA_D * o[1], c.ALPHA_D * o[2]};
+        const real prd = (gd[0] * s.gx[0] + gd[1] * s.gx[1] + gd[2] * s.gx[2]) / Rr;
+        for (int dd = 0; dd < 3; ++dd) out[dd][s.a] = gd[dd] - prd * s.gx[dd] / Rr;
+      }
+    }
+    s.a += (long)d.g * d.g;
+    s.k += 1;
   }
 };
 
