@@ -1,0 +1,163 @@
+// Column tiles for 7-point stencil kernels.
+//
+// A block owns a TW x TW tile of one region's points (TW = 16, or g - 2 on
+// smaller grids) over a chunk of levels.  Each of the first TW^2 threads owns
+// one tile column: the per-column coefficients of its operator are loaded once
+// into registers, and then, level by level, the thread stages its own values
+// of the operator's stencil fields into LDS (the first 4 TW + 2 threads also
+// stage one point of the ring around the tile), a barrier, and the thread
+// forms its point's result from its neighbours' staged values.  The staging
+// planes are double-buffered by level parity, so one barrier per level
+// suffices.  The point kernels re-read the 21 stencil coefficients (and the
+// neighbours' values) per point and level; here each value is loaded from
+// memory once per level and each coefficient once per chunk.
+//
+// An operator Op supplies
+//   kName, kNS (stencil fields), Col (per-column registers), d (Dims),
+//   stage_level(k), stage(k, a, v[kNS]), init_col(col index, Col&),
+//   point(k, a, Col, nb) with nb(m, f) = field f at stencil neighbour m
+//   (0 (0,0) 1 (+1,0) 2 (+1,+1) 3 (0,+1) 4 (-1,0) 5 (-1,-1) 6 (0,-1)),
+//   pole(pb): the pole centre of pole level pb (its point kernel).
+#pragma once
+
+#include "nh_tile.h"
+
+NH_BEGIN
+
+template <class Op>
+struct ColStencil {
+  using LdsT = real;
+  static constexpr const char* kName = Op::kName;
+  static constexpr int kThreads = 256;
+  static constexpr int NS = Op::kNS;
+  Op op;
+  ColGeom cg;
+  static int lds_elems(const ColGeom& cg) { return 2 * NS * (cg.TW + 2) * (cg.TW + 2); }
+  struct State {
+    typename Op::Col col;
+    long a, ra;        // own and ring addresses at the current level
+    int k, sidx, ridx;
+    bool outp, ring;
+  };
+  NH_HD int init(int blk, int tid, State& s) const {
+    const Dims& d = op.d;
+    s.outp = s.ring = false;
+    if (blk >= cg.ntiles) {
+      if (tid == 0) op.pole(blk - cg.ntiles);
+      return 0;
+    }
+    const int g = d.g, TW = cg.TW, SW = TW + 2, tpr = cg.ntx * cg.ntx;
+    const int tile = blk % tpr, rest = blk / tpr;
+    const int chunk = rest % cg.nchunk, l = rest / cg.nchunk;
+    const int k0 = chunk * cg.KC;
+    const int k1 = (k0 + cg.KC < d.kall) ? k0 + cg.KC : d.kall;
+    if (k0 >= k1) return 0;
+    const int i0 = 1 + (tile % cg.ntx) * TW, j0 = 1 + (tile / cg.ntx) * TW;
+    s.k = k0;
+    if (tid < TW * TW) {
+      const int tx = tid % TW, ty = tid / TW;
+      const int i = i0 + tx, j = j0 + ty;
+      s.outp = true;
+      s.sidx = (ty + 1) * SW + tx + 1;
+      s.a = ((long)(l * d.kall + k0) * g + j) * g + i;
+      op.init_col(((long)l * g + j) * g + i, s.col);
+    }
+    // the ring: row -1 (x = -1..TW-1), row TW (x = 0..TW), columns -1 and TW (y = 0..TW-1)
+    if (tid < 4 * TW + 2) {
+      int x, y;
+      if (tid < TW + 1) { x = tid - 1; y = -1; }
+      else if (tid < 2 * TW + 2) { x = tid - (TW + 1); y = TW; }
+      else if (tid < 3 * TW + 2) { x = -1; y = tid - (2 * TW + 2); }
+      else { x = TW; y = tid - (3 * TW + 2); }
+      s.ring = true;
+      s.ridx = (y + 1) * SW + x + 1;
+      s.ra = ((long)(l * d.kall + k0) * g + (j0 + y)) * g + (i0 + x);
+    }
+    return k1 - k0;
+  }
+  NH_HD void level_a(int step, State& s, real* lds) const {
+    if (!op.stage_level(s.k)) return;
+    const int SW = cg.TW + 2, P = SW * SW;
+    real* buf = lds + (step & 1) * NS * P;
+    real v[NS];
+    if (s.outp) {
+      op.stage(s.k, s.a, v);
+      for (int f = 0; f < NS; ++f) buf[f * P + s.sidx] = v[f];
+    }
+    if (s.ring) {
+      op.stage(s.k, s.ra, v);
+      for (int f = 0; f < NS; ++f) buf[f * P + s.ridx] = v[f];
+    }
+  }
+  NH_HD void level_b(int step, State& s, const real* lds) const {
+    if (s.outp) {
+      const int SW = cg.TW + 2, P = SW * SW;
+      const real* buf = lds + (step & 1) * NS * P;
+      const int x = s.sidx;
+      op.point(s.k, s.a, s.col, [&](int m, int f) {
+        const int off = (m == 0) ? 0 : (m == 1) ? 1 : (m == 2) ? 1 + SW : (m == 3) ? SW : (m == 4) ? -1
+                      : (m == 5) ? -1 - SW : -SW;
+        return buf[f * P + x + off];
+      });
+    }
+    const long gg = (long)op.d.g * op.d.g;
+    s.a += gg;
+    s.ra += gg;
+    s.k += 1;
+  }
+};
+
+// vi_main, horizontal part (ViMainH) as a column operator: the 21 divergence
+// coefficients in registers, the 8 stencil fields staged per level
+struct ViMainHC {
+  static constexpr const char* kName = "vi_main_h";
+  static constexpr int kNS = 8;
+  ViMainH pk;
+  Dims d;
+  struct Col {
+    real c[3][7];
+  };
+  NH_HD bool stage_level(int k) const { return k >= d.kmin && k <= d.kmax; }
+  NH_HD void stage(int, long a, real* v) const {
+    for (int f = 0; f < kNS; ++f) v[f] = pk.stage_ptr(f, 0)[a];
+  }
+  NH_HD void init_col(long ci, Col& col) const {
+    for (int dd = 0; dd < 3; ++dd)
+      for (int m = 0; m < 7; ++m) col.c[dd][m] = rc(pk.G.cdiv, d.nreg, dd * 7 + m, ci);
+  }
+  NH_HD void pole(int pb) const { pk(d.lall * d.kall + pb, 0); }
+  // ViMainH::at with hdiv_multi<3>, term for term
+  template <class NB>
+  NH_HD void point(int k, long a, const Col& col, NB nb) const {
+    if (k < d.kmin || k > d.kmax) return;
+    const long NF = d.NF;
+    real s[3][3];
+#pragma unroll
+    for (int m = 0; m < 7; ++m) {
+      real v[3][3];
+      const real e = nb(m, 3), et = nb(m, 4);
+      for (int q = 0; q < 3; ++q) {
+        const real dq = nb(m, q);
+        v[0][q] = dq * real(1.0);
+        v[1][q] = dq * e * real(1.0);
+        v[2][q] = (nb(m, 5 + q) + dq) * et * real(1.0);
+      }
+      for (int i = 0; i < 3; ++i) {
+        const real tx = col.c[0][m] * v[i][0], ty = col.c[1][m] * v[i][1], tz = col.c[2][m] * v[i][2];
+        s[i][0] = (m == 0) ? tx : s[i][0] + tx;
+        s[i][1] = (m == 0) ? ty : s[i][1] + ty;
+        s[i][2] = (m == 0) ? tz : s[i][2] + tz;
+      }
+    }
+    const real divs1 = s[0][0] + s[0][1] + s[0][2];
+    const real he = s[1][0] + s[1][1] + s[1][2];
+    const real hd = s[2][0] + s[2][1] + s[2][2];
+    const real drhoge = -he - (real(0.0) - real(0.0)) * pk.G.rdgz[k];
+    const real grhoge1 = pk.gT[5 * NF + a] + drhoge;
+    pk.hv[0][a] = divs1;
+    pk.hv[1][a] = grhoge1 * pk.c.Rdry / pk.c.CVdry;
+    pk.hv[2][a] = hd;
+  }
+};
+
+NH_END

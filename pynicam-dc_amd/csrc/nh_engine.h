@@ -24,6 +24,7 @@
 #include "nh_tracer.h"
 #include "nh_point.h"
 #include "nh_tile.h"
+#include "nh_colstencil.h"
 #include "nh_budget.h"
 #include "nh_trcadv.h"
 #include "../../include/nicamhip.h"
@@ -288,8 +289,9 @@ void halo_block(X& x, const Context& ctx, real* base, int nf) {
 
 inline long ncols(const Dims& d) { return d.nreg + 12; }
 
-// level chunks of the divergence-damping column tiles
+// level chunks of the column tiles (divergence damping, stencil operators)
 constexpr int kDivDampChunks = 4;
+constexpr int kColChunks = 4;
 
 // ----------------------------------------------------------------------------
 // 3-D divergence damping of (vx, vy, vz, w) into out (3 fields)
@@ -559,7 +561,11 @@ void small_hv(X& x, const Context& ctx, const real* prog, real* const dv[3], rea
   for (int q = 0; q < 3; ++q) f.dv[q] = dv[q];
   f.eth = ctx.field(W_ETH); f.ethtot = sf.ethtot; f.gT = sf.gT;
   for (int q = 0; q < 3; ++q) f.hv[q] = hv + q * ctx.d.NF;
-  run_staged(x, f);
+  ColStencil<ViMainHC> cs;
+  cs.op.pk = f;
+  cs.op.d = ctx.d;
+  cs.cg = col_geom(ctx.d, kColChunks);
+  x.run_cols(cs, cs.cg.nblocks, ColStencil<ViMainHC>::lds_elems(cs.cg));
 }
 
 // one small step (mod_vi.py:1283-1450): divergence damping of the split

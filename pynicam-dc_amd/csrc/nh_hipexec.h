@@ -52,7 +52,7 @@ __global__ __launch_bounds__(256) void k_pts(F f, int nx, int nblk) {
 // column tiles: per block init(), then per level level_a, barrier, level_b;
 // the number of levels is uniform over the block
 template <class F>
-__global__ __launch_bounds__(320) void k_cols(F f, int nblk) {
+__global__ __launch_bounds__(F::kThreads) void k_cols(F f, int nblk) {
   extern __shared__ __align__(16) unsigned char nh_lds_bytes[];
   typename F::LdsT* lds = reinterpret_cast<typename F::LdsT*>(nh_lds_bytes);
   const int L = blockIdx.x;
@@ -173,7 +173,7 @@ class HipExec {
   void run_cols(const F& f, int nblocks, int lds_elems) {
     hipEvent_t* pair = timed(F::kName) ? next_pair(F::kName) : nullptr;
     if (pair) note(hipEventRecord(pair[0], s_), "hipEventRecord");
-    hipLaunchKernelGGL(k_cols<F>, dim3(nblocks), dim3(320), (size_t)lds_elems * sizeof(typename F::LdsT), s_, f,
+    hipLaunchKernelGGL(k_cols<F>, dim3(nblocks), dim3(F::kThreads), (size_t)lds_elems * sizeof(typename F::LdsT), s_, f,
                        nblocks);
     note(hipGetLastError(), "kernel launch");
     if (pair) note(hipEventRecord(pair[1], s_), "hipEventRecord");

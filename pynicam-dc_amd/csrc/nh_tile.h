@@ -258,6 +258,7 @@ inline ColGeom col_geom(const Dims& d, int nchunk) {
 template <int OP>
 struct DivDampK {
   using LdsT = real;
+  static constexpr int kThreads = kColThreads;
   static constexpr const char* kName = (OP == 1) ? "divdamp_3d" : "divdamp_post";
   Dims d; Consts c; Geo G; ColGeom cg;
   const real* v[3];     // OP 1: the momentum; OP 2: the first result (negated inside)

@@ -109,6 +109,28 @@ class HostExec {
       prof_->rec.emplace_back(F::kName,
                               std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
   }
+  template <class F>
+  void run_cols(const F& f, int nblocks, int lds_elems) {
+    auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel
+    {
+      std::vector<typename F::LdsT> lds((size_t)lds_elems + 1);
+      constexpr int nt = F::kThreads;
+      std::vector<typename F::State> st(nt);
+#pragma omp for schedule(static)
+      for (int b = 0; b < nblocks; ++b) {
+        int nl = 0;
+        for (int t = 0; t < nt; ++t) nl = f.init(b, t, st[t]);
+        for (int s = 0; s < nl; ++s) {
+          for (int t = 0; t < nt; ++t) f.level_a(s, st[t], lds.data());
+          for (int t = 0; t < nt; ++t) f.level_b(s, st[t], lds.data());
+        }
+      }
+    }
+    if (prof_ && prof_->on && (prof_->only.empty() || prof_->only == F::kName))
+      prof_->rec.emplace_back(F::kName,
+                              std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+  }
   void sync() {}
   int check(const char*) { return 0; }
   std::string message() const { return "host"; }

@@ -317,7 +317,8 @@ def test_host_tracer_adjoint_with_limiters(case_gl04, hostemu_lib):
 
 
 def test_host_tracer_forecast_autograd_and_mixed(case_gl04, hostemu_lib):
-    """Three steps through torch.autograd equal three direct adjoint calls; limiters off,
+    """Three steps through torch.autograd equal three direct adjoint calls (to summation
+    order); limiters off,
     the mixed form's gradient agrees with FP64's to 1e-5 of its peak."""
     import copy
     import torch
@@ -342,7 +343,8 @@ def test_host_tracer_forecast_autograd_and_mixed(case_gl04, hostemu_lib):
                 gb = w.clone()
                 for p, qq in reversed(traj):
                     run.tracer_step_adj(p, qq, gb)
-                assert torch.equal(gb, q.grad)
+                # the scatter transposes add atomically: equal up to summation order
+                assert torch.allclose(gb, q.grad, rtol=0, atol=1e-12 * float(gb.abs().max()))
         finally:
             ctx.close()
     sc = grads["float64"].abs().max()
