@@ -1026,30 +1026,68 @@ struct ViMainPost {
   const real* hv[3];
   real* we[3];       // we[1] (rhogw split) is read
   real* pm;
+  // Every value the point needs is loaded first, at levels clamped into
+  // 0..kall-1 (a clamped level's value is never used: the level tests below
+  // are the reference's), so that the loads issue together instead of one
+  // memory round trip per level test.
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
     const long NF = d.NF, ks = p.ks, a = p.a;
     const int kmin = d.kmin, kmax = d.kmax, k = p.k;
     const long b = a - (long)k * ks;
-    auto at = [&](int kk) { return b + (long)kk * ks; };
+    // window levels k-1 .. k+2 -> index 0..3
+    long A[4];
+    for (int j = 0; j < 4; ++j) {
+      int kk = k - 1 + j;
+      kk = (kk < 0) ? 0 : ((kk > d.kall - 1) ? d.kall - 1 : kk);
+      A[j] = b + (long)kk * ks;
+    }
     const real* pg = ps[I_RHOG];
-    const real* pe = ps[I_RHOGE];
     const real* rw1 = we[1];
-    auto rs1 = [&](int kk) {
+    real P0[3], S0[3], H0[3], T0[3], ET[3], RW[4], P4[2], S4[2], PQ[3], SQ[3], DQ[3];
+    for (int j = 0; j < 3; ++j) {
+      P0[j] = prog[A[j]];
+      S0[j] = pg[A[j]];
+      H0[j] = hv[0][A[j]];
+      T0[j] = gT[A[j]];
+      ET[j] = ethtot[A[j]];
+    }
+    for (int j = 0; j < 4; ++j) RW[j] = rw1[A[j]];
+    for (int j = 0; j < 2; ++j) {
+      P4[j] = prog[4 * NF + A[1 + j]];
+      S4[j] = ps[4][A[1 + j]];
+    }
+    for (int q = 0; q < 3; ++q) {
+      PQ[q] = prog[(1 + q) * NF + a];
+      SQ[q] = ps[1 + q][a];
+      DQ[q] = dv[q][a];
+    }
+    const real h2 = hv[2][a], ge = getot[a];
+    const int ke = (k < kmin) ? kmin : ((k > kmax) ? kmax : k);
+    const real pe_k = ps[I_RHOGE][b + (long)ke * ks];
+    const real pm0 = pm[0 * NF + a], pm4 = pm[4 * NF + a];
+    // j: window index of level kk (k - 1 + j)
+    auto rs1 = [&](int kk, int j) {
       real drhog2 = real(0.0);
       if (kk >= kmin && kk <= kmax) {
-        const real hi = (kk + 1 <= kmax) ? real(0.0) * real(1.0) + real(1.0) * rw1[at(kk + 1)] * real(1.0) : real(0.0);
-        const real lo = (kk >= kmin + 1) ? real(0.0) * real(1.0) + real(1.0) * rw1[at(kk)] * real(1.0) : real(0.0);
-        drhog2 = -hv[0][at(kk)] - (hi - lo) * G.rdgz[kk];
+        const real hi = (kk + 1 <= kmax) ? real(0.0) * real(1.0) + real(1.0) * RW[j + 1] * real(1.0) : real(0.0);
+        const real lo = (kk >= kmin + 1) ? real(0.0) * real(1.0) + real(1.0) * RW[j] * real(1.0) : real(0.0);
+        drhog2 = -H0[j] - (hi - lo) * G.rdgz[kk];
       }
-      return pg[at(kk)] + (gT[0 * NF + at(kk)] + drhog2) * dt;
+      return S0[j] + (T0[j] + drhog2) * dt;
     };
-    auto g10 = [&](int kk, int q) { return prog[q * NF + at(kk)] + ps[q][at(kk)]; };
+    auto g10 = [&](int kk, int q) {
+      const int j = kk - k + 1;
+      if (q == 0) return P0[j] + S0[j];
+      if (q == 4) return P4[j - 1] + S4[j - 1];
+      return PQ[q - 1] + SQ[q - 1];
+    };
     auto g11 = [&](int kk, int q) {
-      if (q == 0) return prog[at(kk)] + rs1(kk);
-      if (q == 4) return prog[4 * NF + at(kk)] + rw1[at(kk)];
-      return prog[q * NF + at(kk)] + dv[q - 1][at(kk)];
+      const int j = kk - k + 1;
+      if (q == 0) return P0[j] + rs1(kk, j);
+      if (q == 4) return P4[j - 1] + RW[j];
+      return PQ[q - 1] + DQ[q - 1];
     };
     const real kin10 = rhogkin_k(G, kmin, kmax, k, g10);
     const real kin11 = rhogkin_k(G, kmin, kmax, k, g11);
@@ -1057,21 +1095,19 @@ struct ViMainPost {
     if (k >= kmin && k <= kmax) {
       real hi = real(0.0), lo = real(0.0);
       if (k + 1 <= kmax)
-        hi = real(0.0) * real(1.0) + real(1.0) * ((prog[4 * NF + a + ks] + rw1[a + ks]) *
-                                (G.afact[k + 1] * ethtot[a + ks] + G.bfact[k + 1] * ethtot[a])) * real(1.0);
+        hi = real(0.0) * real(1.0) + real(1.0) * ((P4[1] + RW[2]) * (G.afact[k + 1] * ET[2] + G.bfact[k + 1] * ET[1]))
+             * real(1.0);
       if (k >= kmin + 1)
-        lo = real(0.0) * real(1.0) + real(1.0) * ((prog[4 * NF + a] + rw1[a]) *
-                                (G.afact[k] * ethtot[a] + G.bfact[k] * ethtot[a - ks])) * real(1.0);
-      dtot = -hv[2][a] - (hi - lo) * G.rdgz[k];
+        lo = real(0.0) * real(1.0) + real(1.0) * ((P4[0] + RW[1]) * (G.afact[k] * ET[1] + G.bfact[k] * ET[0]))
+             * real(1.0);
+      dtot = -h2 - (hi - lo) * G.rdgz[k];
     }
-    const real r1 = rs1(k);
-    const int ke = (k < kmin) ? kmin : ((k > kmax) ? kmax : k);
-    const real pe_k = pe[at(ke)];
-    const real re1 = pe_k + (getot[a] + dtot) * dt + (kin10 - kin11) + (pg[a] - r1) * G.phi[k];
+    const real r1 = rs1(k, 1);
+    const real re1 = pe_k + (ge + dtot) * dt + (kin10 - kin11) + (S0[1] - r1) * G.phi[k];
     we[0][a] = r1;
     we[2][a] = re1;
-    pm[0 * NF + a] = pm[0 * NF + a] + r1 * rweight;
-    pm[4 * NF + a] = pm[4 * NF + a] + rw1[a] * rweight;
+    pm[0 * NF + a] = pm0 + r1 * rweight;
+    pm[4 * NF + a] = pm4 + RW[1] * rweight;
   }
 };
 
