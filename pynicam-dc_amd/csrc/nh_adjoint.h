@@ -375,8 +375,11 @@ struct HDiffPass2A {
 // kinetic energy at level k of a column (rhogkin_k): transpose for the
 // accessor's values; add(kk, q, v) receives the bar of get(kk, q)
 // ----------------------------------------------------------------------------
+// (always inlined: callers that sum the bars in registers need the level
+// offsets of add() to be constants)
 template <class F, class ADD>
-NH_HD inline void rhogkin_k_t(const VProf<real>& V, int kmin, int kmax, int k, F get, ADD add, real kb) {
+NH_HD inline __attribute__((always_inline)) void rhogkin_k_t(const VProf<real>& V, int kmin, int kmax, int k, F get,
+                                                             ADD add, real kb) {
   if (k < kmin || k > kmax) return;
   const real rgk = get(k, 0);
   const real vx = get(k, 1), vy = get(k, 2), vz = get(k, 3);
@@ -1026,24 +1029,49 @@ struct ViMainPostA {
     }
   }
   // the transpose at one compute point; add(field, dk, value) accumulates the
-  // bar of field at level k + dk of the point's column (dk = -1 .. 2)
+  // bar of field at level k + dk of the point's column (dk = -1 .. 2).  The
+  // forward values are loaded first, at levels clamped into 0..kall-1 (as in
+  // ViMainPost), so that no load waits behind a level test.
   template <class ADD>
   NH_HD void point(const Pt& p, ADD add) const {
     const long NF = d.NF, ks = p.ks, a = p.a;
     const int kmin = d.kmin, kmax = d.kmax, k = p.k;
     const long b = a - (long)k * ks;
-    auto at = [&](int kk) { return b + (long)kk * ks; };
-    const real b0 = we_b[0][a], b2 = we_b[2][a];
+    long A[4];
+    for (int j = 0; j < 4; ++j) {
+      int kk = k - 1 + j;
+      kk = (kk < 0) ? 0 : ((kk > d.kall - 1) ? d.kall - 1 : kk);
+      A[j] = b + (long)kk * ks;
+    }
     const real* pg = ps[I_RHOG];
-    const real* rw1 = we1;
-    auto rs1 = [&](int kk) {
+    real P0[3], S0[3], H0[3], T0[3], ET[3], RW[4], P4[2], PQ[3], SQ[3], DQ[3], S4[2];
+    for (int j = 0; j < 3; ++j) {
+      P0[j] = prog[A[j]];
+      S0[j] = pg[A[j]];
+      H0[j] = hv[0][A[j]];
+      T0[j] = gT[A[j]];
+      ET[j] = ethtot[A[j]];
+    }
+    for (int j = 0; j < 4; ++j) RW[j] = we1[A[j]];
+    for (int j = 0; j < 2; ++j) {
+      P4[j] = prog[4 * NF + A[1 + j]];
+      S4[j] = ps[4][A[1 + j]];
+    }
+    for (int q = 0; q < 3; ++q) {
+      PQ[q] = prog[(1 + q) * NF + a];
+      SQ[q] = ps[1 + q][a];
+      DQ[q] = dv[q][a];
+    }
+    const real b0 = we_b[0][a], b2 = we_b[2][a];
+    // j: window index of level kk (k - 1 + j)
+    auto rs1 = [&](int kk, int j) {
       real drhog2 = real(0.0);
       if (kk >= kmin && kk <= kmax) {
-        const real hi = (kk + 1 <= kmax) ? real(0.0) * real(1.0) + real(1.0) * rw1[at(kk + 1)] * real(1.0) : real(0.0);
-        const real lo = (kk >= kmin + 1) ? real(0.0) * real(1.0) + real(1.0) * rw1[at(kk)] * real(1.0) : real(0.0);
-        drhog2 = -hv[0][at(kk)] - (hi - lo) * G.rdgz[kk];
+        const real hi = (kk + 1 <= kmax) ? real(0.0) * real(1.0) + real(1.0) * RW[j + 1] * real(1.0) : real(0.0);
+        const real lo = (kk >= kmin + 1) ? real(0.0) * real(1.0) + real(1.0) * RW[j] * real(1.0) : real(0.0);
+        drhog2 = -H0[j] - (hi - lo) * G.rdgz[kk];
       }
-      return pg[at(kk)] + (gT[0 * NF + at(kk)] + drhog2) * dt;
+      return S0[j] + (T0[j] + drhog2) * dt;
     };
     // dk: level of the rs1 argument relative to k
     auto rs1_t = [&](int dk, real rb) {
@@ -1057,10 +1085,12 @@ struct ViMainPostA {
         if (kk >= kmin + 1) add(F_W1, dk, d2b * G.rdgz[kk]);
       }
     };
-    // the energy split of the nearest physical level
-    if (k < kmin) add(F_S + I_RHOGE, 1, b2);
-    else if (k > kmax) add(F_S + I_RHOGE, -1, b2);
-    else add(F_S + I_RHOGE, 0, b2);
+    // the energy split of the nearest physical level (three adds of selected
+    // values: a single add at a selected level would index the caller's
+    // register sums dynamically)
+    add(F_S + I_RHOGE, 1, (k < kmin) ? b2 : real(0.0));
+    add(F_S + I_RHOGE, -1, (k > kmax) ? b2 : real(0.0));
+    add(F_S + I_RHOGE, 0, (k >= kmin && k <= kmax) ? b2 : real(0.0));
     add(F_GE, 0, b2 * dt);
     const real dtotb = b2 * dt;
     add(F_S + I_RHOG, 0, b2 * G.phi[k]);
@@ -1069,16 +1099,16 @@ struct ViMainPostA {
       add(F_H2, 0, -dtotb);
       const real hib = -dtotb * G.rdgz[k], lob = dtotb * G.rdgz[k];
       if (k + 1 <= kmax) {
-        const real W1 = prog[4 * NF + a + ks] + rw1[a + ks];
-        const real E1 = G.afact[k + 1] * ethtot[a + ks] + G.bfact[k + 1] * ethtot[a];
+        const real W1 = P4[1] + RW[2];
+        const real E1 = G.afact[k + 1] * ET[2] + G.bfact[k + 1] * ET[1];
         add(F_P + 4, 1, hib * E1);
         add(F_W1, 1, hib * E1);
         add(F_ET, 1, hib * W1 * G.afact[k + 1]);
         add(F_ET, 0, hib * W1 * G.bfact[k + 1]);
       }
       if (k >= kmin + 1) {
-        const real W0 = prog[4 * NF + a] + rw1[a];
-        const real E0 = G.afact[k] * ethtot[a] + G.bfact[k] * ethtot[a - ks];
+        const real W0 = P4[0] + RW[1];
+        const real E0 = G.afact[k] * ET[1] + G.bfact[k] * ET[0];
         add(F_P + 4, 0, lob * E0);
         add(F_W1, 0, lob * E0);
         add(F_ET, 0, lob * W0 * G.afact[k]);
@@ -1086,16 +1116,22 @@ struct ViMainPostA {
       }
     }
     // kinetic energies before (PROG + PROG_split) and after (new split) the step
-    auto g10 = [&](int kk, int q) { return prog[q * NF + at(kk)] + ps[q][at(kk)]; };
+    auto g10 = [&](int kk, int q) {
+      const int j = kk - k + 1;
+      if (q == 0) return P0[j] + S0[j];
+      if (q == 4) return P4[j - 1] + S4[j - 1];
+      return PQ[q - 1] + SQ[q - 1];
+    };
     auto add10 = [&](int kk, int q, real v) {
       add(F_P + q, kk - k, v);
       add(F_S + q, kk - k, v);
     };
     rhogkin_k_t(G, kmin, kmax, k, g10, add10, b2);
     auto g11 = [&](int kk, int q) {
-      if (q == 0) return prog[at(kk)] + rs1(kk);
-      if (q == 4) return prog[4 * NF + at(kk)] + rw1[at(kk)];
-      return prog[q * NF + at(kk)] + dv[q - 1][at(kk)];
+      const int j = kk - k + 1;
+      if (q == 0) return P0[j] + rs1(kk, j);
+      if (q == 4) return P4[j - 1] + RW[j];
+      return PQ[q - 1] + DQ[q - 1];
     };
     auto add11 = [&](int kk, int q, real v) {
       add(F_P + q, kk - k, v);
@@ -1107,13 +1143,42 @@ struct ViMainPostA {
   }
   // one launch per level class k mod 4 (color): the points of one launch
   // write disjoint bars (their targets k-1 .. k+2 of their own column), so
-  // plain adds suffice
+  // plain adds suffice.  The bars are summed in registers first; the
+  // (field, level) pairs the transpose can reach are then read together and
+  // written back (a pair the point did not reach gets + 0).
   int color;
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p) || !pt_compute(p) || (p.k & 3) != color) return;
     const long a = p.a, ks = p.ks;
-    point(p, [&](int f, int dk, real v) { fptr(f)[a + (long)dk * ks] += v; });
+    real acc[F_N][4];
+    for (int f = 0; f < F_N; ++f)
+      for (int j = 0; j < 4; ++j) acc[f][j] = real(0.0);
+    point(p, [&](int f, int dk, real v) { acc[f][dk + 1] += v; });
+    // reachable pairs: (field, dk + 1)
+    constexpr int np = 37;
+    constexpr signed char pf[np] = {F_P, F_P, F_P, F_P + 1, F_P + 2, F_P + 3, F_P + 4, F_P + 4,
+                                    F_S, F_S, F_S, F_S + 1, F_S + 2, F_S + 3, F_S + 4, F_S + 4,
+                                    F_S + 5, F_S + 5, F_S + 5, F_D, F_D + 1, F_D + 2,
+                                    F_W1, F_W1, F_W1, F_W1, F_H0, F_H0, F_H0, F_H2,
+                                    F_G0, F_G0, F_G0, F_ET, F_ET, F_ET, F_GE};
+    constexpr signed char pj[np] = {0, 1, 2, 1, 1, 1, 1, 2,
+                                    0, 1, 2, 1, 1, 1, 1, 2,
+                                    0, 1, 2, 1, 1, 1,
+                                    0, 1, 2, 3, 0, 1, 2, 1,
+                                    0, 1, 2, 0, 1, 2, 1};
+    const int k = p.k;
+    real cur[np];
+#pragma unroll
+    for (int n = 0; n < np; ++n) {
+      const int kk = k + pj[n] - 1;
+      if (kk >= 0 && kk < d.kall) cur[n] = fptr(pf[n])[a + (long)(pj[n] - 1) * ks];
+    }
+#pragma unroll
+    for (int n = 0; n < np; ++n) {
+      const int kk = k + pj[n] - 1;
+      if (kk >= 0 && kk < d.kall) fptr(pf[n])[a + (long)(pj[n] - 1) * ks] = cur[n] + acc[pf[n]][pj[n]];
+    }
   }
 };
 
