@@ -91,14 +91,14 @@ KERNEL_TYPES = {
     "ProgUpdateP": "prog_update", "HaloGather": "halo_gather", "TrcVert1": "trc_vert1", "TrcVert2": "trc_vert2",
     "TrcHFlux": "trc_hflux", "TrcRhog2": "trc_rhog2", "TrcGradQ": "trc_gradq", "TrcRemap": "trc_remap",
     "TrcQout": "trc_qout", "TrcApply": "trc_apply", "DivDampK<1>": "divdamp_3d", "DivDampK<2>": "divdamp_post",
-    "HDiffPass2T": "hdiff_pass2",
+    "HDiffPass2T": "hdiff_pass2", "ViMainHC": "vi_main_h",
 }
 
 
 def _kernel_key(name: str):
     import re
-    m = re.search(r"k_(?:pts|run|tiles|cols)<nh::f(?:64|32)::(?:StagedPointT<nh::f(?:64|32)::)?(\w+(?:<\d+>)?)\s*>",
-                  name)
+    m = re.search(r"k_(?:pts|run|tiles|cols)<nh::f(?:64|32)::(?:(?:StagedPointT|ColStencil)<nh::f(?:64|32)::)?"
+                  r"(\w+(?:<\d+>)?)\s*>", name)
     return KERNEL_TYPES.get(m.group(1)) if m else None
 
 
