@@ -89,19 +89,18 @@ struct PrePostP {
   real* diag;
   real* eth;
   struct Th { real rho, ein, tem, pre; };
-  NH_HD Th thermo(long a) const {
-    const long NF = d.NF;
+  NH_HD Th thermo_v(real rhog, real rhoge, real rhogq0) const {
     Th t;
-    const real rhog = prog[a];
     t.rho = rhog / real(1.0);
-    t.ein = prog[5 * NF + a] / rhog;
-    const real q = rhogq[a] / rhog;
+    t.ein = rhoge / rhog;
+    const real q = rhogq0 / rhog;
     const real qd = real(1.0) - q;
     const real cv = q * c.CVW_qv + qd * c.CVdry;
     t.tem = t.ein / cv;
     t.pre = t.rho * t.tem * (qd * c.Rdry + q * c.Rvap);
     return t;
   }
+  NH_HD Th thermo(long a) const { return thermo_v(prog[a], prog[5 * d.NF + a], rhogq[a]); }
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p)) return;
@@ -113,22 +112,27 @@ struct PrePostP {
     real* tem = diag + I_TEM * NF;
     real* w = diag + I_W * NF;
     if (k >= kmin && k <= kmax) {
-      Th t = thermo(a);
-      const real rhog = rg[a];
-      diag[I_VX * NF + a] = prog[1 * NF + a] / rhog;
-      diag[I_VY * NF + a] = prog[2 * NF + a] / rhog;
-      diag[I_VZ * NF + a] = prog[3 * NF + a] / rhog;
+      // every value is read before the first store (the stores could alias
+      // the reads for all the compiler knows, which would serialize them)
+      const long al = (k > kmin) ? a - ks : a + ks;   // level below, or above at kmin
+      const real rhog = rg[a], rhoge = prog[5 * NF + a], q0 = rhogq[a];
+      const real v1 = prog[1 * NF + a], v2 = prog[2 * NF + a], v3 = prog[3 * NF + a];
+      const real rwa = rw[a], rgl = rg[al], rgel = prog[5 * NF + al], ql = rhogq[al];
+      Th t = thermo_v(rhog, rhoge, q0);
+      diag[I_VX * NF + a] = v1 / rhog;
+      diag[I_VY * NF + a] = v2 / rhog;
+      diag[I_VZ * NF + a] = v3 / rhog;
       tem[a] = t.tem;
       pre[a] = t.pre;
       if (k > kmin) {
-        w[a] = rw[a] / (G.c2w0[k] * rg[a] + G.c2w1[k] * rg[a - ks]);
+        w[a] = rwa / (G.c2w0[k] * rhog + G.c2w1[k] * rgl);
       } else {
         // bottom: rigid lid, with the density of the ghost row after the boundary condition
-        const Th t1 = thermo(a + ks);
+        const Th t1 = thermo_v(rgl, rgel, ql);
         const real pre_b = t1.pre - t.rho * (G.phi[kmin - 1] - G.phi[kmin + 1]);
         const real rho_b = pre_b / (c.Rdry * t.tem);
         rw[a] = real(0.0);
-        w[a] = rw[a] / (G.c2w0[kmin] * rg[a] + G.c2w1[kmin] * (rho_b * real(1.0)));
+        w[a] = real(0.0) / (G.c2w0[kmin] * rhog + G.c2w1[kmin] * (rho_b * real(1.0)));
       }
       eth[a] = t.ein + t.pre / t.rho;
       return;
@@ -738,15 +742,28 @@ struct EthTotPMP {
   const real* eth;
   real* ethtot;
   real* pm;
+  // the column values are read first (levels k-1, k+1 clamped into range;
+  // rhogkin_k uses them only where they exist), then the stores
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p)) return;
     const long NF = d.NF, a = p.a, ks = p.ks;
-    const long b = a - (long)p.k * ks;
-    auto get = [&](int kk, int q) { return prog[q * NF + b + (long)kk * ks]; };
-    const real kin0 = rhogkin_k(G, d.kmin, d.kmax, p.k, get);
-    ethtot[a] = eth[a] + kin0 / prog[a] + G.phi[p.k];
-    for (int q = 0; q < 5; ++q) pm[q * NF + a] = prog[q * NF + a];
+    const int k = p.k;
+    const long am = (k > 0) ? a - ks : a, ap = (k < d.kall - 1) ? a + ks : a;
+    const real rg[3] = {prog[am], prog[a], prog[ap]};
+    const real rv[3] = {prog[NF + a], prog[2 * NF + a], prog[3 * NF + a]};
+    const real rw[2] = {prog[4 * NF + a], prog[4 * NF + ap]};
+    const real e = eth[a];
+    auto get = [&](int kk, int q) {
+      if (q == 0) return rg[kk - k + 1];
+      if (q == 4) return rw[kk - k];
+      return rv[q - 1];
+    };
+    const real kin0 = rhogkin_k(G, d.kmin, d.kmax, k, get);
+    ethtot[a] = e + kin0 / rg[1] + G.phi[k];
+    pm[a] = rg[1];
+    for (int q = 0; q < 3; ++q) pm[(1 + q) * NF + a] = rv[q];
+    pm[4 * NF + a] = rw[0];
   }
 };
 
@@ -833,6 +850,10 @@ struct Path1P {
       }
       return acc;
     };
+    // the mass-flux sums are read before any store (a store to dv could alias
+    // them for all the compiler knows, which would put each read behind it)
+    real pmq[3];
+    for (int q = 0; q < 3; ++q) pmq[q] = pm[(1 + q) * NF + a];
     real v[3];
     if (k >= kmin && k <= kmax) {
       level(p, a, grad, v);
@@ -847,7 +868,7 @@ struct Path1P {
     }
     for (int q = 0; q < 3; ++q) {
       dv[q][a] = v[q];
-      pm[(1 + q) * NF + a] = pm[(1 + q) * NF + a] + v[q] * rweight;
+      pm[(1 + q) * NF + a] = pmq[q] + v[q] * rweight;
     }
   }
 };
