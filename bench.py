@@ -58,7 +58,8 @@ def kernel_bytes(t, nq: int = 1, elem: int = 8) -> dict:
         "prepost": 14 * F,                      # prog 6 + qv in; diag 6 + eth out
         "adv_mom": 13 * F + 24 * Cc,            # prog 5 + diag 4 in; 4 out; div + xyz coefs
         "hdiff_pass1": 12 * F + 7 * Cc,         # 6 in, 6 out; Laplacian coefs
-        "hdiff_pass2": 13 * F + 43 * Cc,        # 6 + rhog in, 6 out; lap + intp + diff coefs
+        "hdiff_pass2_lap": 8 * F + 7 * Cc,      # fields 0..3 in, 4 out; Laplacian coefs
+        "hdiff_pass2_kh": 5 * F + 36 * Cc,      # fields 4, 5 + rhog in, 2 out; intp + diff coefs
         "divdamp_3d": 7 * F + 36 * Cc,          # 4 in, 3 out; intp + diff coefs
         "divdamp_post": 6 * F + 39 * Cc,        # 3 in, 3 out; intp + diff + xyz
         "vi_path0": 31 * F + 3 * D + 45 * Cc,   # 23 in, 8 out + FP64 matrix 3 out; div + grad + xyz
@@ -91,12 +92,15 @@ KERNEL_TYPES = {
     "ProgUpdateP": "prog_update", "HaloGather": "halo_gather", "TrcVert1": "trc_vert1", "TrcVert2": "trc_vert2",
     "TrcHFlux": "trc_hflux", "TrcRhog2": "trc_rhog2", "TrcGradQ": "trc_gradq", "TrcRemap": "trc_remap",
     "TrcQout": "trc_qout", "TrcApply": "trc_apply", "DivDampK<1>": "divdamp_3d", "DivDampK<2>": "divdamp_post",
-    "HDiffPass2T": "hdiff_pass2", "ViMainHC": "vi_main_h",
+    "ViMainHC": "vi_main_h", "DiffKhK": "hdiff_pass2_kh",
 }
 
 
 def _kernel_key(name: str):
     import re
+    lap = re.search(r"ColStencil<nh::f(?:64|32)::LapC<(\d+),", name)
+    if lap:
+        return "hdiff_pass1" if lap.group(1) == "6" else "hdiff_pass2_lap"
     m = re.search(r"k_(?:pts|run|tiles|cols)<nh::f(?:64|32)::(?:(?:StagedPointT|ColStencil)<nh::f(?:64|32)::)?"
                   r"(\w+(?:<\d+>)?)\s*>", name)
     return KERNEL_TYPES.get(m.group(1)) if m else None

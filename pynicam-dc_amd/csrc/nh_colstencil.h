@@ -160,4 +160,40 @@ struct ViMainHC {
   }
 };
 
+// minus the 7-point Laplacian of NFLD fields (HDiffPassP's Laplacians) as a
+// column operator: the 7 Laplacian coefficients in registers.  P is the point
+// kernel that does the pole centres (all of its fields) unless pole_off.
+template <int NFLD, class P>
+struct LapC {
+  static constexpr const char* kName = (NFLD == 6) ? "hdiff_pass1" : "hdiff_pass2_lap";
+  static constexpr int kNS = NFLD;
+  P pk;
+  Dims d;
+  const real* in[NFLD];
+  real* out[NFLD];
+  bool pole_off;
+  struct Col {
+    real c[7];
+  };
+  NH_HD bool stage_level(int) const { return true; }
+  NH_HD void stage(int, long a, real* v) const {
+    for (int f = 0; f < NFLD; ++f) v[f] = in[f][a];
+  }
+  NH_HD void init_col(long ci, Col& col) const {
+    for (int m = 0; m < 7; ++m) col.c[m] = rc(pk.G.clap, d.nreg, m, ci);
+  }
+  NH_HD void pole(int pb) const {
+    if (!pole_off) pk(d.lall * d.kall + pb, 0);
+  }
+  template <class NB>
+  NH_HD void point(int, long a, const Col& col, NB nb) const {
+    for (int f = 0; f < NFLD; ++f) {
+      real r = col.c[0] * nb(0, f);
+#pragma unroll
+      for (int m = 1; m < 7; ++m) r = r + col.c[m] * nb(m, f);
+      out[f][a] = -r;
+    }
+  }
+};
+
 NH_END

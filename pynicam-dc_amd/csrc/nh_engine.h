@@ -519,16 +519,42 @@ void stage_large(X& x, const Context& ctx, real* prog, const real* rhogq, double
     f.in[3] = diag + I_W * NF; f.in[4] = diag + I_TEM * NF; f.in[5] = prog;
     for (int q = 0; q < 6; ++q) f.out[q] = sf.hda + q * NF;
     f.rhog = prog;
-    run_staged(x, f);
+    const ColGeom cg = col_geom(d, kColChunks);
+    {
+      ColStencil<LapC<6, HDiffPassP<0>>> cs;
+      cs.op.pk = f;
+      cs.op.d = d;
+      for (int q = 0; q < 6; ++q) { cs.op.in[q] = f.in[q]; cs.op.out[q] = f.out[q]; }
+      cs.op.pole_off = false;
+      cs.cg = cg;
+      x.run_cols(cs, cg.nblocks, decltype(cs)::lds_elems(cg));
+    }
     halo_block(x, ctx, sf.hda, 6);
-    HDiffPass2T f2;
-    f2.d = d; f2.c = ctx.c; f2.G = ctx.G; f2.tg = tile_geom(d, 12);
+    // second pass: the Laplacians of fields 0..3, then the Kh-weighted
+    // diffusion of fields 4, 5 (whose pole blocks do all six fields)
+    HDiffPassP<1> f2;
+    f2.d = d; f2.c = ctx.c; f2.G = ctx.G;
     for (int q = 0; q < 6; ++q) {
       f2.in[q] = sf.hda + q * NF;
       f2.out[q] = sf.hdb + q * NF;
     }
     f2.rhog = prog;
-    x.run_tiles(f2, f2.tg.nblocks, f2.tg.lds);
+    {
+      ColStencil<LapC<4, HDiffPassP<1>>> cs;
+      cs.op.pk = f2;
+      cs.op.d = d;
+      for (int q = 0; q < 4; ++q) { cs.op.in[q] = f2.in[q]; cs.op.out[q] = f2.out[q]; }
+      cs.op.pole_off = true;
+      cs.cg = cg;
+      x.run_cols(cs, cg.nblocks, decltype(cs)::lds_elems(cg));
+    }
+    {
+      DiffKhK k2;
+      k2.d = d; k2.c = ctx.c; k2.G = ctx.G; k2.cg = cg;
+      for (int q = 0; q < 6; ++q) { k2.in[q] = f2.in[q]; k2.out[q] = f2.out[q]; }
+      k2.rhog = prog;
+      x.run_cols(k2, cg.nblocks, DiffKhK::lds_elems(cg));
+    }
     halo_block(x, ctx, sf.hdb, 6);
   }
   // divergence damping of the large-step momentum

@@ -1,6 +1,6 @@
-// LDS-tiled kernels for the triangle-based operators: the Kh-weighted
-// diffusion of the second hyperdiffusion pass as level tiles (below), the 3-D
-// divergence damping as column tiles (DivDampK, at the end).
+// LDS-tiled kernels: the level tiles of the staged stencil point kernels
+// (StagedPointT, below), and the column tiles of the triangle-based operators
+// (the 3-D divergence damping DivDampK, the Kh-weighted diffusion DiffKhK).
 //
 // A block owns a tile of one region level: all interior i (W = g-2 points,
 // one or more wavefronts) times TH rows.  Phase 1 computes the two triangle
@@ -123,96 +123,6 @@ void run_staged(X& x, const P& pk) {
   f.tg = tile_geom(pk.d, 0);
   x.run_tiles(f, f.tg.nblocks, P::kNStage * (f.tg.TH + 2) * pk.d.g);
 }
-
-// ----------------------------------------------------------------------------
-// second hyperdiffusion pass (mod_numfilter.py:1125-1264): Laplacians of the
-// velocity components and w, Kh-weighted diffusion (kernels/oprtdiffusion.py:59-177)
-// of temperature and density; the triangle gradients of the two diffused
-// fields (3 components x TI/TJ) go through LDS
-// ----------------------------------------------------------------------------
-struct HDiffPass2T {
-  using LdsT = real;
-  static constexpr bool kStage = false;
-  static constexpr const char* kName = "hdiff_pass2";
-  Dims d; Consts c; Geo G; TileGeom tg;
-  const real* in[6];
-  real* out[6];
-  const real* rhog;
-  // LDS layout: [field f (0,1)][triangle T][component dd][row][col]
-  NH_HD int lidx(const TileAt& t, int f, int T, int dd, int cj_, int ci_) const {
-    const int W1 = tg.W + 1, R = t.rows + 1;
-    return (((f * 2 + T) * 3 + dd) * R + cj_) * W1 + ci_;
-  }
-  NH_HD void phase1(int blk, int tid, real* lds) const {
-    if (blk >= tg.ntiles) return;
-    const TileAt t = tile_at(d, tg, blk);
-    const int g = d.g, W1 = tg.W + 1;
-    const int nc = W1 * (t.rows + 1);
-    for (int cidx = tid; cidx < nc; cidx += 256) {
-      const int ci_ = cidx % W1, cj_ = cidx / W1;
-      const int i = ci_, j = t.j0 - 1 + cj_;
-      const long cj = ((long)t.l * g + j) * g + i;
-      const long a = ((long)(t.l * d.kall + t.k) * g + j) * g + i;
-      for (int f = 0; f < 2; ++f)
-        for (int T = 0; T < 2; ++T) {
-          real o3[3];
-          tri_grad(G, d.nreg, cj, T, in[4 + f], a, g, o3);
-          for (int dd = 0; dd < 3; ++dd) lds[lidx(t, f, T, dd, cj_, ci_)] = o3[dd];
-        }
-    }
-  }
-  NH_HD void phase2(int blk, int tid, const real* lds) const {
-    const int g = d.g;
-    if (blk >= tg.ntiles) {
-      if (tid == 0) {
-        HDiffPassP<1> p;
-        p.d = d; p.c = c; p.G = G; p.rhog = rhog;
-        for (int q = 0; q < 6; ++q) { p.in[q] = in[q]; p.out[q] = out[q]; }
-        p(d.lall * d.kall + (blk - tg.ntiles), 0);
-      }
-      return;
-    }
-    const TileAt t = tile_at(d, tg, blk);
-    const int ti = tid % tg.W, tj = tid / tg.W;
-    if (tj >= t.rows) return;
-    const int i = ti + 1, j = t.j0 + tj;
-    const long nreg = d.nreg;
-    const long ci = ((long)t.l * g + j) * g + i;
-    const long a = ((long)(t.l * d.kall + t.k) * g + j) * g + i;
-    for (int q = 0; q < 4; ++q) out[q][a] = -lap7(G, nreg, ci, in[q], a, g);
-    const bool pent = (i == 1 && j == 1);
-    const real m0 = G.sgp[t.l] ? real(0.0) : real(1.0), m1 = G.sgp[t.l] ? real(1.0) : real(0.0);
-    // corner offsets of the six triangles: TI(i,j) TJ(i,j) TI(i-1,j) TJ(i-1,j-1) TI(i-1,j-1) TJ(i,j-1)
-    const int cdi[6] = {0, 0, -1, -1, -1, 0}, cdj[6] = {0, 0, 0, -1, -1, -1}, cT[6] = {0, 1, 0, 1, 0, 1};
-    for (int f = 0; f < 2; ++f) {
-      const real fac = (f == 0) ? c.CVdry : c.HDIFF_FACT_RHO;
-      const real gam = c.GAMMA_H;
-      auto kh = [&](long x) { return rhog[x] * fac * gam; };
-      real tv[6][3];
-      for (int n = 0; n < 6; ++n)
-        for (int dd = 0; dd < 3; ++dd)
-          tv[n][dd] = lds[lidx(t, f, cT[n], dd, j + cdj[n] - t.j0 + 1, i + cdi[n])];
-      if (pent)
-        for (int dd = 0; dd < 3; ++dd) tv[4][dd] = tv[4][dd] * m0 + tv[5][dd] * m1;
-      const real k0 = kh(a);
-      const real kf1 = real(0.5) * (k0 + kh(a + 1 + g));
-      const real kf2 = real(0.5) * (k0 + kh(a + g));
-      const real kf3 = real(0.5) * (k0 + kh(a - 1));
-      const real kf4 = real(0.5) * (k0 + kh(a - 1 - g));
-      const real kf5 = real(0.5) * (k0 + kh(a - g));
-      const real kf6 = real(0.5) * (k0 + kh(a + 1));
-      real inner = real(0.0);
-      for (int dd = 0; dd < 3; ++dd) {
-        const real* cd = G.cdiff + (long)(dd * 6) * nreg + ci;
-        const real blk2 = kf1 * cd[0] * (tv[0][dd] + tv[1][dd]) + kf2 * cd[nreg] * (tv[1][dd] + tv[2][dd])
-                          + kf3 * cd[2 * nreg] * (tv[2][dd] + tv[3][dd]) + kf4 * cd[3 * nreg] * (tv[3][dd] + tv[4][dd])
-                          + kf5 * cd[4 * nreg] * (tv[4][dd] + tv[5][dd]) + kf6 * cd[5 * nreg] * (tv[5][dd] + tv[0][dd]);
-        inner = (dd == 0) ? blk2 : inner + blk2;
-      }
-      out[4 + f][a] = -inner;
-    }
-  }
-};
 
 // ----------------------------------------------------------------------------
 // divergence damping as column tiles.  A block owns a TW x TW tile of one
@@ -414,6 +324,130 @@ struct DivDampK {
         const real gd[3] = {c.ALPHA_D * o[0], c.ALPHA_D * o[1], c.ALPHA_D * o[2]};
         const real prd = (gd[0] * s.gx[0] + gd[1] * s.gx[1] + gd[2] * s.gx[2]) / Rr;
         for (int dd = 0; dd < 3; ++dd) out[dd][s.a] = gd[dd] - prd * s.gx[dd] / Rr;
+      }
+    }
+    s.a += (long)d.g * d.g;
+    s.k += 1;
+  }
+};
+
+// ----------------------------------------------------------------------------
+// the Kh-weighted diffusion of the second hyperdiffusion pass (fields 4 and 5
+// of HDiffPass2T: temperature with CVdry, density with HDIFF_FACT_RHO) as
+// column tiles, the same layout as DivDampK: corner threads form the triangle
+// gradients of both fields from their 18 held cintp coefficients, tile points
+// the Kh-weighted hexagon sums from their 18 held cdiff coefficients.
+// Results equal HDiffPass2T's (diffusion7, kernels/oprtdiffusion.py:59-177).
+// ----------------------------------------------------------------------------
+struct DiffKhK {
+  using LdsT = real;
+  static constexpr int kThreads = kColThreads;
+  static constexpr const char* kName = "hdiff_pass2_kh";
+  Dims d; Consts c; Geo G; ColGeom cg;
+  const real* in[6];        // the pass's six inputs (fields 4, 5 diffused here)
+  real* out[6];
+  const real* rhog;
+  // LDS per buffer: [field f][triangle T][component dd][corner]
+  static int lds_elems(const ColGeom& cg) { return 2 * 12 * (cg.TW + 1) * (cg.TW + 1); }
+  struct State {
+    real ci[2][9];
+    real cd[18];
+    long a;
+    int k, lidx;
+    bool corner, outp, pent, sgp;
+  };
+  NH_HD int init(int blk, int tid, State& s) const {
+    s.corner = s.outp = s.pent = s.sgp = false;
+    if (blk >= cg.ntiles) {
+      if (tid == 0) {
+        HDiffPassP<1> p;
+        p.d = d; p.c = c; p.G = G; p.rhog = rhog;
+        for (int q = 0; q < 6; ++q) { p.in[q] = in[q]; p.out[q] = out[q]; }
+        p(d.lall * d.kall + (blk - cg.ntiles), 0);
+      }
+      return 0;
+    }
+    const int g = d.g, W1 = cg.TW + 1, tpr = cg.ntx * cg.ntx;
+    const int tile = blk % tpr, rest = blk / tpr;
+    const int chunk = rest % cg.nchunk, l = rest / cg.nchunk;
+    const int k0 = chunk * cg.KC;
+    const int k1 = (k0 + cg.KC < d.kall) ? k0 + cg.KC : d.kall;
+    if (k0 >= k1) return 0;
+    const int i0 = 1 + (tile % cg.ntx) * cg.TW, j0 = 1 + (tile / cg.ntx) * cg.TW;
+    const int tx = tid % W1, ty = tid / W1;
+    s.k = k0;
+    if (tid >= W1 * W1) return k1 - k0;
+    s.corner = true;
+    s.lidx = ty * W1 + tx;
+    const int i = i0 - 1 + tx, j = j0 - 1 + ty;
+    const long nreg = d.nreg, col = ((long)l * g + j) * g + i;
+    s.a = ((long)(l * d.kall + k0) * g + j) * g + i;
+    for (int T = 0; T < 2; ++T)
+      for (int n = 0; n < 9; ++n) s.ci[T][n] = rc(G.cintp, nreg, n * 2 + T, col);
+    if (tx >= 1 && ty >= 1) {
+      s.outp = true;
+      for (int n = 0; n < 18; ++n) s.cd[n] = G.cdiff[(long)n * nreg + col];
+      s.pent = (i == 1 && j == 1);
+      s.sgp = G.sgp[l] != 0;
+    }
+    return k1 - k0;
+  }
+  NH_HD void level_a(int step, State& s, real* lds) const {
+    if (!s.corner) return;
+    const long g = d.g;
+    const int P = (cg.TW + 1) * (cg.TW + 1);
+    real* buf = lds + (step & 1) * 12 * P;
+    for (int f = 0; f < 2; ++f) {
+      const real* sf = in[4 + f];
+      const real v00 = sf[s.a], v10 = sf[s.a + 1], v11 = sf[s.a + 1 + g], v01 = sf[s.a + g];
+      for (int T = 0; T < 2; ++T) {
+        // tri_grad with the held coefficients
+        const real s0 = v00, s1 = (T == 0) ? v10 : v11, s2 = (T == 0) ? v11 : v01;
+        for (int dd = 0; dd < 3; ++dd) {
+          const real c1 = s.ci[T][dd * 3 + 0], c2 = s.ci[T][dd * 3 + 1], c3 = s.ci[T][dd * 3 + 2];
+          buf[((f * 2 + T) * 3 + dd) * P + s.lidx] =
+              ((real(2.0) * c1 - c2 - c3) * s0 + (-c1 + real(2.0) * c2 - c3) * s1 + (-c1 - c2 + real(2.0) * c3) * s2)
+              / real(3.0);
+        }
+      }
+    }
+  }
+  NH_HD void level_b(int step, State& s, const real* lds) const {
+    if (s.outp) {
+      const long g = d.g;
+      const int W1 = cg.TW + 1, P = W1 * W1;
+      const real* buf = lds + (step & 1) * 12 * P;
+      const long a = s.a;
+      const int x = s.lidx;
+      // corners of TI(i,j) TJ(i,j) TI(i-1,j) TJ(i-1,j-1) TI(i-1,j-1) TJ(i,j-1)
+      const int cx[6] = {x, x, x - 1, x - 1 - W1, x - 1 - W1, x - W1}, cT[6] = {0, 1, 0, 1, 0, 1};
+      const real r0 = rhog[a], r1 = rhog[a + 1 + g], r2 = rhog[a + g], r3 = rhog[a - 1], r4 = rhog[a - 1 - g],
+                 r5 = rhog[a - g], r6 = rhog[a + 1];
+      const real m0 = s.sgp ? real(0.0) : real(1.0), m1 = s.sgp ? real(1.0) : real(0.0);
+      for (int f = 0; f < 2; ++f) {
+        const real fac = (f == 0) ? c.CVdry : c.HDIFF_FACT_RHO;
+        const real gam = c.GAMMA_H;
+        real tv[6][3];
+        for (int n = 0; n < 6; ++n)
+          for (int dd = 0; dd < 3; ++dd) tv[n][dd] = buf[((f * 2 + cT[n]) * 3 + dd) * P + cx[n]];
+        if (s.pent)
+          for (int dd = 0; dd < 3; ++dd) tv[4][dd] = tv[4][dd] * m0 + tv[5][dd] * m1;
+        const real k0 = r0 * fac * gam;
+        const real kf1 = real(0.5) * (k0 + r1 * fac * gam);
+        const real kf2 = real(0.5) * (k0 + r2 * fac * gam);
+        const real kf3 = real(0.5) * (k0 + r3 * fac * gam);
+        const real kf4 = real(0.5) * (k0 + r4 * fac * gam);
+        const real kf5 = real(0.5) * (k0 + r5 * fac * gam);
+        const real kf6 = real(0.5) * (k0 + r6 * fac * gam);
+        real inner = real(0.0);
+        for (int dd = 0; dd < 3; ++dd) {
+          const real* cd = s.cd + dd * 6;
+          const real blk2 = kf1 * cd[0] * (tv[0][dd] + tv[1][dd]) + kf2 * cd[1] * (tv[1][dd] + tv[2][dd])
+                            + kf3 * cd[2] * (tv[2][dd] + tv[3][dd]) + kf4 * cd[3] * (tv[3][dd] + tv[4][dd])
+                            + kf5 * cd[4] * (tv[4][dd] + tv[5][dd]) + kf6 * cd[5] * (tv[5][dd] + tv[0][dd]);
+          inner = (dd == 0) ? blk2 : inner + blk2;
+        }
+        out[4 + f][a] = -inner;
       }
     }
     s.a += (long)d.g * d.g;
