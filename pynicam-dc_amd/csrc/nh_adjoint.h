@@ -1146,10 +1146,17 @@ struct ViMainPostA {
   // plain adds suffice.  The bars are summed in registers first; the
   // (field, level) pairs the transpose can reach are then read together and
   // written back (a pair the point did not reach gets + 0).
+  // The launch covers only the levels of its colour: plane index
+  // g * nkc + i (g: region 0..lall-1, then the two pole columns) is level
+  // color + 4 i of that region or pole column.
   int color;
-  NH_HD void operator()(int plane, int x) const {
+  NH_HD static int ncolor_levels(const Dims& d, int color) { return (d.kall - color + 3) / 4; }
+  NH_HD void operator()(int cplane, int x) const {
+    const int nkc = ncolor_levels(d, color);
+    const int grp = cplane / nkc, kk = color + 4 * (cplane - grp * nkc);
+    const int plane = (grp < d.lall) ? grp * d.kall + kk : d.lall * d.kall + (grp - d.lall) * d.kall + kk;
     Pt p;
-    if (!pt_of(d, plane, x, p) || !pt_compute(p) || (p.k & 3) != color) return;
+    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
     const long a = p.a, ks = p.ks;
     real acc[F_N][4];
     for (int f = 0; f < F_N; ++f)

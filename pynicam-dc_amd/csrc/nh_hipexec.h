@@ -148,9 +148,13 @@ class HipExec {
   }
   template <class F, class D>
   void run_pts(const F& f, const D& d) {
+    run_pts_n(f, d, d.lall * d.kall + 2 * d.kall);
+  }
+  // ny planes of the point grid (the functor maps its plane index)
+  template <class F, class D>
+  void run_pts_n(const F& f, const D& d, int ny) {
     const int bs = 256;
     const int nx = (d.g * d.g + bs - 1) / bs;
-    const int ny = d.lall * d.kall + 2 * d.kall;
     const int nblk = nx * ny;
     hipEvent_t* pair = timed(F::kName) ? next_pair(F::kName) : nullptr;
     if (pair) note(hipEventRecord(pair[0], s_), "hipEventRecord");

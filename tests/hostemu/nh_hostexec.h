@@ -81,8 +81,11 @@ class HostExec {
   }
   template <class F, class D>
   void run_pts(const F& f, const D& d) {
+    run_pts_n(f, d, d.lall * d.kall + 2 * d.kall);
+  }
+  template <class F, class D>
+  void run_pts_n(const F& f, const D& d, int ny) {
     auto t0 = std::chrono::steady_clock::now();
-    const int ny = d.lall * d.kall + 2 * d.kall;
     const int nx = d.g * d.g;
 #pragma omp parallel for schedule(static)
     for (int y = 0; y < ny; ++y)
