@@ -1272,13 +1272,13 @@ struct ViPath0A {
     }
     auto add_rg = [&](int kk, real v) { aadd(&prog_b[lev(kk)], v); };
     auto add_pre = [&](int kk, real v) { aadd(&diag_b[I_PRE * NF + lev(kk)], v); };
-    auto rgl = [&](int kk) { return rg[lev(kk)]; };
-    auto prl = [&](int kk) { return pre[lev(kk)]; };
+    auto rgl = [&](int kk) { return ldc(rg + (lev(kk))); };
+    auto prl = [&](int kk) { return ldc(pre + (lev(kk))); };
     GztCol<real> gz{&G, c.GRAV, kmin, kmax};
     real xx[3];
     point_pos(G, d, p, xx);
     const real Kh = c.GAMMA_H;
-    const real rgk = rg[a];
+    const real rgk = ldc(rg + (a));
     real rgkb = real(0.0);
     // gT0 = -vt5 + drhog
     aadd(&vt_b[5][a], -gb[0]);
@@ -1291,7 +1291,7 @@ struct ViPath0A {
     proj_t(xx, R, fvb);
     for (int q = 0; q < 3; ++q) {
       aadd(&vt_b[q][a], -fvb[q] * Kh * rgk);
-      rgkb += -fvb[q] * vt[q][a] * Kh;
+      rgkb += -fvb[q] * ldc(vt[q] + (a)) * Kh;
     }
     // gT4 = gadv_w - (vt3 Kh) rhog_h - dpgw(k) + dbuo(k)
     if (k > kmin && k <= kmax) {
@@ -1299,9 +1299,9 @@ struct ViPath0A {
       aadd(&gadv_b[3 * NF + a - ks], gb[4] * G.c2w1[k]);
     }
     if (k >= kmin) {
-      const real rgh = G.c2w0[k] * rgk + G.c2w1[k] * rg[a - ks];
+      const real rgh = G.c2w0[k] * rgk + G.c2w1[k] * ldc(rg + (a - ks));
       aadd(&vt_b[3][a], -gb[4] * Kh * rgh);
-      const real rghb = -gb[4] * vt[3][a] * Kh;
+      const real rghb = -gb[4] * ldc(vt[3] + (a)) * Kh;
       rgkb += rghb * G.c2w0[k];
       aadd(&prog_b[a - ks], rghb * G.c2w1[k]);
     }
@@ -1322,15 +1322,15 @@ struct ViPath0A {
       const real heb = -gb[5];
       if (k + 1 <= kmax) {
         const real wsb = -gb[5] * G.rdgz[k];
-        const real w = rw[a + ks];
-        aadd(&prog_b[4 * NF + a + ks], wsb * (G.afact[k + 1] * eth[a + ks] + G.bfact[k + 1] * eth[a]));
+        const real w = ldc(rw + (a + ks));
+        aadd(&prog_b[4 * NF + a + ks], wsb * (G.afact[k + 1] * ldc(eth + (a + ks)) + G.bfact[k + 1] * ldc(eth + (a))));
         aadd(&eth_b[a + ks], wsb * w * G.afact[k + 1]);
         aadd(&eth_b[a], wsb * w * G.bfact[k + 1]);
       }
       if (k >= kmin + 1) {
         const real wsb = gb[5] * G.rdgz[k];
-        const real w = rw[a];
-        aadd(&prog_b[4 * NF + a], wsb * (G.afact[k] * eth[a] + G.bfact[k] * eth[a - ks]));
+        const real w = ldc(rw + (a));
+        aadd(&prog_b[4 * NF + a], wsb * (G.afact[k] * ldc(eth + (a)) + G.bfact[k] * ldc(eth + (a - ks))));
         aadd(&eth_b[a], wsb * w * G.afact[k]);
         aadd(&eth_b[a - ks], wsb * w * G.bfact[k]);
       }
@@ -1344,12 +1344,12 @@ struct ViPath0A {
         } else {
           for (int q = 0; q < 3; ++q) cdir[q] = rc(G.cdiv, d.nreg, q * 7 + m, p.ci);
         }
-        const real e = eth[bn];
+        const real e = ldc(eth + (bn));
         real eb = real(0.0);
         for (int q = 0; q < 3; ++q) {
           const real t0 = cdir[q] * hdb, t1 = cdir[q] * heb;
           aadd(&prog_b[(1 + q) * NF + bn], t0 + t1 * e);
-          eb += t1 * prog[(1 + q) * NF + bn];
+          eb += t1 * ldc(prog + ((1 + q) * NF + bn));
         }
         aadd(&eth_b[bn], eb);
       }
@@ -1360,13 +1360,13 @@ struct ViPath0A {
         real acc = real(0.0);
         if (p.pole) {
           for (int v = 0; v < 6; ++v) {
-            const real pv = G.cgrad_pl[(v * 2 + p.lp) * 3 + dd2] * (pre[a + v + sh] * real(1.0) * real(1.0));
+            const real pv = G.cgrad_pl[(v * 2 + p.lp) * 3 + dd2] * (ldc(pre + (a + v + sh)) * real(1.0) * real(1.0));
             acc = (v == 0) ? pv : acc + pv;
           }
         } else {
-          acc = rc(G.cgrad, d.nreg, dd2 * 7, p.ci) * (pre[a + sh] * real(1.0));
+          acc = rc(G.cgrad, d.nreg, dd2 * 7, p.ci) * (ldc(pre + (a + sh)) * real(1.0));
           for (int m = 1; m < 7; ++m)
-            acc = acc + rc(G.cgrad, d.nreg, dd2 * 7 + m, p.ci) * (pre[a + nb_off(m, d.g) + sh] * real(1.0));
+            acc = acc + rc(G.cgrad, d.nreg, dd2 * 7 + m, p.ci) * (ldc(pre + (a + nb_off(m, d.g) + sh)) * real(1.0));
         }
         gr[dd2] = acc + real(0.0);
       }
@@ -1378,7 +1378,7 @@ struct ViPath0A {
       const real pwb = gb[5];
       real dpgb[3];
       for (int q = 0; q < 3; ++q) {
-        dpgb[q] = -gb[1 + q] + pwb * diag[(I_VX + q) * NF + a];
+        dpgb[q] = -gb[1 + q] + pwb * ldc(diag + ((I_VX + q) * NF + a));
         aadd(&diag_b[(I_VX + q) * NF + a], pwb * dpg[q]);
       }
       proj_t(xx, R, dpgb);
@@ -1397,7 +1397,7 @@ struct ViPath0A {
         const real hb = pwb * wgt;
         const real gzv = gz.value(kk, rgl, prl);
         aadd(&prog_b[4 * NF + lev(kk)], -hb * gzv);
-        gz.adj(kk, rgl, prl, add_rg, add_pre, -hb * rw[lev(kk)]);
+        gz.adj(kk, rgl, prl, add_rg, add_pre, -hb * ldc(rw + (lev(kk))));
       }
     }
     aadd(&prog_b[a], rgkb);
@@ -1406,8 +1406,8 @@ struct ViPath0A {
       const VProf<double>& V = G.v64;
       const ConstsT<double>& C = c.d;
       GztCol<double> gz64{&V, C.GRAV, kmin, kmax};
-      auto rg64 = [&](int kk) { return (double)rg[lev(kk)]; };
-      auto pr64 = [&](int kk) { return (double)pre[lev(kk)]; };
+      auto rg64 = [&](int kk) { return (double)ldc(rg + (lev(kk))); };
+      auto pr64 = [&](int kk) { return (double)ldc(pre + (lev(kk))); };
       auto add_rg64 = [&](int kk, double v) { aadd(&prog_b[lev(kk)], (real)v); };
       auto add_pre64 = [&](int kk, double v) { aadd(&diag_b[I_PRE * NF + lev(kk)], (real)v); };
       auto eth_h_t = [&](int kk, double v) {

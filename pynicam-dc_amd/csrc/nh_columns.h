@@ -42,6 +42,21 @@ NH_BEGIN
 
 using real = NH_REAL;
 
+// Load of a value that no thread writes during the kernel (coefficients, and
+// input fields a kernel only reads).  On the device it reads through the
+// constant address space, which the compiler knows is never stored to, so
+// such loads may be issued ahead of the kernel's stores and atomics instead
+// of waiting behind each one.
+template <class T>
+NH_HD inline T ldc(const T* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef const __attribute__((address_space(4))) T* cptr;
+  return *(cptr)p;
+#else
+  return *p;
+#endif
+}
+
 // type-exact absolute value and sign transfer (no promotion of float arguments)
 NH_HD inline float nabs(float x) { return fabsf(x); }
 NH_HD inline double nabs(double x) { return fabs(x); }
@@ -215,7 +230,14 @@ NH_HD inline void stencil_offsets(int g, long off[7]) {
 }
 
 template <class T>
-NH_HD inline T rc(const T* a, long nreg, int comp, long ci) { return a[comp * nreg + ci]; }
+NH_HD inline T rc(const T* a, long nreg, int comp, long ci) { return ldc(a + comp * nreg + ci); }
+// the same as a plain load for 8-byte values (the tracer transport's flux
+// and remap kernels measured slower with ldc in the FP64 form, faster in FP32)
+template <class T>
+NH_HD inline T rcp(const T* a, long nreg, int comp, long ci) {
+  if constexpr (sizeof(T) == 4) return ldc(a + comp * nreg + ci);
+  else return a[comp * nreg + ci];
+}
 
 // 7-point divergence of (ax, ay, az)(point) at one level for a regional cell
 template <class FX, class FY, class FZ>

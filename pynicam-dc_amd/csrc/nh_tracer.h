@@ -216,8 +216,8 @@ struct TrcHFlux {
         const real* f = pm + q * NF;
         const long o = a + di + (long)dj * g;
         const long cc = ci + di + (long)dj * g;
-        const real w1 = rc(G.twt, nreg, T * 3 + 0, cc), w2 = rc(G.twt, nreg, T * 3 + 1, cc),
-                     w3 = rc(G.twt, nreg, T * 3 + 2, cc);
+        const real w1 = rcp(G.twt, nreg, T * 3 + 0, cc), w2 = rcp(G.twt, nreg, T * 3 + 1, cc),
+                     w3 = rcp(G.twt, nreg, T * 3 + 2, cc);
         if (T == 0) {
           real s = f[o] * w1 + f[o + 1] * w2;
           return s + f[o + 1 + g] * w3;
@@ -247,11 +247,11 @@ struct TrcHFlux {
         const real rr = real(1.0) / dmax(rs, NH_EPS);
         real v2[3];
         for (int q = 0; q < 3; ++q) v2[q] = triv(Ta, dia, dja, q + 1) + triv(Tb, dib, djb, q + 1);
-        const real hx = rc(G.ahn, nreg, arc * 3 + 0, ci), hy = rc(G.ahn, nreg, arc * 3 + 1, ci),
-                     hz = rc(G.ahn, nreg, arc * 3 + 2, ci);
+        const real hx = rcp(G.ahn, nreg, arc * 3 + 0, ci), hy = rcp(G.ahn, nreg, arc * 3 + 1, ci),
+                     hz = rcp(G.ahn, nreg, arc * 3 + 2, ci);
         fa[arc][a] = real(0.5) * (v2[0] * hx + v2[1] * hy + v2[2] * hz);
         for (int dd = 0; dd < 3; ++dd)
-          xc[arc * 3 + dd][a] = rc(G.axr, nreg, arc * 3 + dd, ci) - v2[dd] * rr * dt * real(0.5);
+          xc[arc * 3 + dd][a] = rcp(G.axr, nreg, arc * 3 + dd, ci) - v2[dd] * rr * dt * real(0.5);
       }
     }
   }
@@ -261,7 +261,7 @@ struct TrcHFlux {
 NH_HD inline real face_flux(const Geo& G, const Dims& d, const real* const* fa, long a, long ci, int i, int j,
                               int l, int m, real dt) {
   const int g = d.g;
-  const real ra = rc(G.rarea, d.nreg, 0, ci);
+  const real ra = rcp(G.rarea, d.nreg, 0, ci);
   switch (m) {
     case 0: return fa[0][a] * ra * dt;
     case 1: return fa[1][a] * ra * dt;
@@ -329,8 +329,8 @@ struct TrcGradQ {
             acc = (v == 0) ? pv : acc + pv;
           }
         } else {
-          acc = rc(G.cgrad, d.nreg, dd * 7, cl.ci) * qv[0];
-          for (int m = 1; m < 7; ++m) acc = acc + rc(G.cgrad, d.nreg, dd * 7 + m, cl.ci) * qv[m];
+          acc = rcp(G.cgrad, d.nreg, dd * 7, cl.ci) * qv[0];
+          for (int m = 1; m < 7; ++m) acc = acc + rcp(G.cgrad, d.nreg, dd * 7 + m, cl.ci) * qv[m];
         }
         gq[dd][cl.base + kk] = acc;
       }
@@ -386,11 +386,11 @@ struct TrcRemap {
         const long cn = ci + noff[arc];
         const real xcx = xc[arc * 3 + 0][a], xcy = xc[arc * 3 + 1][a], xcz = xc[arc * 3 + 2][a];
         const real qc = rq[a] / rhog1[a], qn = rq[an] / rhog1[an];
-        const real qap = qc + gq[0][a] * (xcx - rc(G.gx, nreg, 0, ci)) + gq[1][a] * (xcy - rc(G.gx, nreg, 1, ci))
-                         + gq[2][a] * (xcz - rc(G.gx, nreg, 2, ci));
-        const real qam = qn + gq[0][an] * (xcx - rc(G.gx, nreg, 0, cn)) + gq[1][an] * (xcy - rc(G.gx, nreg, 1, cn))
-                         + gq[2][an] * (xcz - rc(G.gx, nreg, 2, cn));
-        const real ch = fa[arc][a] * rc(G.rarea, nreg, 0, ci) * dt / rhog1[a];
+        const real qap = qc + gq[0][a] * (xcx - rcp(G.gx, nreg, 0, ci)) + gq[1][a] * (xcy - rcp(G.gx, nreg, 1, ci))
+                         + gq[2][a] * (xcz - rcp(G.gx, nreg, 2, ci));
+        const real qam = qn + gq[0][an] * (xcx - rcp(G.gx, nreg, 0, cn)) + gq[1][an] * (xcy - rcp(G.gx, nreg, 1, cn))
+                         + gq[2][an] * (xcz - rcp(G.gx, nreg, 2, cn));
+        const real ch = fa[arc][a] * rcp(G.rarea, nreg, 0, ci) * dt / rhog1[a];
         const real cm = upwind_mask(ch);
         qa[arc][a] = cm * qam + (real(1.0) - cm) * qap;
       }
@@ -474,9 +474,9 @@ NH_HD inline void trc_cell(const Geo& G, const Dims& d, const Cell& cl, long a, 
   const long nreg = d.nreg;
   const long o3 = a - 1, o4 = a - 1 - d.g, o5 = a - d.g;
   const long c3 = cl.ci - 1, c4 = cl.ci - 1 - d.g, c5 = cl.ci - d.g;
-  x.cmo[3] = upwind_mask(fa[0][o3] * rc(G.rarea, nreg, 0, c3) * dt / rhog1[o3]);
-  x.cmo[4] = upwind_mask(fa[1][o4] * rc(G.rarea, nreg, 0, c4) * dt / rhog1[o4]);
-  x.cmo[5] = upwind_mask(fa[2][o5] * rc(G.rarea, nreg, 0, c5) * dt / rhog1[o5]);
+  x.cmo[3] = upwind_mask(fa[0][o3] * rcp(G.rarea, nreg, 0, c3) * dt / rhog1[o3]);
+  x.cmo[4] = upwind_mask(fa[1][o4] * rcp(G.rarea, nreg, 0, c4) * dt / rhog1[o4]);
+  x.cmo[5] = upwind_mask(fa[2][o5] * rcp(G.rarea, nreg, 0, c5) * dt / rhog1[o5]);
 }
 
 struct TrcQout {
@@ -575,10 +575,10 @@ struct TrcApply {
     const real* x0 = xc[arc * 3];
     const real* x1 = xc[arc * 3 + 1];
     const real* x2 = xc[arc * 3 + 2];
-    const real am = rq[a] / rhog1[a] + gq[0][a] * (x0[o] - rc(G.gx, nreg, 0, cl.ci))
-                    + gq[1][a] * (x1[o] - rc(G.gx, nreg, 1, cl.ci)) + gq[2][a] * (x2[o] - rc(G.gx, nreg, 2, cl.ci));
-    const real ap = rq[o] / rhog1[o] + gq[0][o] * (x0[o] - rc(G.gx, nreg, 0, co))
-                    + gq[1][o] * (x1[o] - rc(G.gx, nreg, 1, co)) + gq[2][o] * (x2[o] - rc(G.gx, nreg, 2, co));
+    const real am = rq[a] / rhog1[a] + gq[0][a] * (x0[o] - rcp(G.gx, nreg, 0, cl.ci))
+                    + gq[1][a] * (x1[o] - rcp(G.gx, nreg, 1, cl.ci)) + gq[2][a] * (x2[o] - rcp(G.gx, nreg, 2, cl.ci));
+    const real ap = rq[o] / rhog1[o] + gq[0][o] * (x0[o] - rcp(G.gx, nreg, 0, co))
+                    + gq[1][o] * (x1[o] - rcp(G.gx, nreg, 1, co)) + gq[2][o] * (x2[o] - rcp(G.gx, nreg, 2, co));
     const real cm = upwind_mask(fm / rhog1[a]);
     return cm * am + (real(1.0) - cm) * ap;
   }
