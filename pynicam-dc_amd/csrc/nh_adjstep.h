@@ -484,7 +484,7 @@ int tracer_step_adj(X& x, const Context& ctx, const real* prog, const real* prog
     f.dt = dt; f.pm = prog;
     for (int q = 0; q < 3; ++q) f.fa[q] = fa[q];
     for (int q = 0; q < 9; ++q) f.xc[q] = xc[q];
-    x.run(f, nc);
+    x.run_pts(f, d);
   }
   halo_block_adj(x, ctx, progq_b, nq);   // the final halo fill of progq
   for (int iq = 0; iq < nq; ++iq) {
@@ -506,13 +506,13 @@ int tracer_step_adj(X& x, const Context& ctx, const real* prog, const real* prog
       f.dt = dt;
       for (int q = 0; q < 3; ++q) f.fa[q] = fa[q];
       f.rhog1 = rhog1; f.frhog = frhog; f.rhog2 = rhog2;
-      x.run(f, nc);
+      x.run_pts(f, d);
     }
     {
       TrcGradQ f;
       f.d = d; f.G = ctx.G; f.rq = rq1; f.rhog1 = rhog1;
       for (int q = 0; q < 3; ++q) f.gq[q] = gq[q];
-      x.run(f, nc);
+      x.run_pts(f, d);
     }
     halo(x, ctx, gq, 3);
     {
@@ -521,7 +521,7 @@ int tracer_step_adj(X& x, const Context& ctx, const real* prog, const real* prog
       f.dt = dt; f.rq = rq1; f.rhog1 = rhog1;
       for (int q = 0; q < 3; ++q) { f.gq[q] = gq[q]; f.fa[q] = fa[q]; f.qa[q] = qa[q]; }
       for (int q = 0; q < 9; ++q) f.xc[q] = xc[q];
-      x.run(f, nc);
+      x.run_pts(f, d);
     }
     if (lim) {
       TrcQout f;
@@ -529,7 +529,7 @@ int tracer_step_adj(X& x, const Context& ctx, const real* prog, const real* prog
       f.dt = dt; f.rq = rq1; f.rhog1 = rhog1; f.frhog = frhog;
       for (int q = 0; q < 3; ++q) f.fa[q] = fa[q];
       f.qout[0] = qout[0]; f.qout[1] = qout[1];
-      x.run(f, nc);
+      x.run_pts(f, d);
       halo(x, ctx, qout, 2);
     }
     {
@@ -541,7 +541,7 @@ int tracer_step_adj(X& x, const Context& ctx, const real* prog, const real* prog
       f.out = rqh; f.limiter = lim;
       for (int q = 0; q < 3; ++q) f.gq[q] = gq[q];
       for (int q = 0; q < 9; ++q) f.xc[q] = xc[q];
-      x.run(f, nc);
+      x.run_pts(f, d);
     }
     // reverse; rq_b becomes the bar of rq1 once the compute cells' bars have
     // gone through the second half step (elsewhere rq1 passes unchanged)

@@ -220,6 +220,18 @@ NH_HD inline bool pt_of(const Dims& d, int plane, int x, Pt& p) {
   return true;
 }
 
+// the column of a point
+NH_HD inline Cell cell_of_pt(const Pt& p) {
+  Cell c;
+  c.pole = p.pole;
+  c.inner = p.inner;
+  c.i = p.i; c.j = p.j; c.l = p.l; c.v = p.v; c.lp = p.lp;
+  c.ks = p.ks;
+  c.base = p.a - (long)p.k * p.ks;
+  c.ci = p.ci;
+  return c;
+}
+
 // the compute points of the horizontal operators: regional interior and pole centres
 NH_HD inline bool pt_compute(const Pt& p) { return p.pole ? p.v == 0 : p.inner; }
 

@@ -350,7 +350,7 @@ void tracer_transport(X& x, const Context& ctx, real* progq, int nq, double dt, 
     f.dt = (real)dt; f.pm = pm;
     for (int q = 0; q < 3; ++q) f.fa[q] = fa[q];
     for (int q = 0; q < 9; ++q) f.xc[q] = xc[q];
-    x.run(f, nc);
+    x.run_pts(f, d);
   }
   {
     TrcRhog2 f;
@@ -358,7 +358,7 @@ void tracer_transport(X& x, const Context& ctx, real* progq, int nq, double dt, 
     f.dt = (real)dt;
     for (int q = 0; q < 3; ++q) f.fa[q] = fa[q];
     f.rhog1 = rhog1; f.frhog = frhog; f.rhog2 = rhog2;
-    x.run(f, nc);
+    x.run_pts(f, d);
   }
   real* gq[3] = {ctx.field(W_GQ), ctx.field(W_GQ + 1), ctx.field(W_GQ + 2)};
   real* qa[3] = {ctx.field(W_QA), ctx.field(W_QA + 1), ctx.field(W_QA + 2)};
@@ -371,7 +371,7 @@ void tracer_transport(X& x, const Context& ctx, real* progq, int nq, double dt, 
       f.d = d; f.G = ctx.G;
       f.rq = rq; f.rhog1 = rhog1;
       for (int q = 0; q < 3; ++q) f.gq[q] = gq[q];
-      x.run(f, nc);
+      x.run_pts(f, d);
     }
     halo(x, ctx, gq, 3);
     {
@@ -380,7 +380,7 @@ void tracer_transport(X& x, const Context& ctx, real* progq, int nq, double dt, 
       f.dt = (real)dt; f.rq = rq; f.rhog1 = rhog1;
       for (int q = 0; q < 3; ++q) { f.gq[q] = gq[q]; f.fa[q] = fa[q]; f.qa[q] = qa[q]; }
       for (int q = 0; q < 9; ++q) f.xc[q] = xc[q];
-      x.run(f, nc);
+      x.run_pts(f, d);
     }
     if (ctx.c.THUBURN_LIM) {
       TrcQout f;
@@ -388,7 +388,7 @@ void tracer_transport(X& x, const Context& ctx, real* progq, int nq, double dt, 
       f.dt = (real)dt; f.rq = rq; f.rhog1 = rhog1; f.frhog = frhog;
       for (int q = 0; q < 3; ++q) f.fa[q] = fa[q];
       f.qout[0] = qout[0]; f.qout[1] = qout[1];
-      x.run(f, nc);
+      x.run_pts(f, d);
       halo(x, ctx, qout, 2);
     }
     {
@@ -400,7 +400,7 @@ void tracer_transport(X& x, const Context& ctx, real* progq, int nq, double dt, 
       f.out = rqh; f.limiter = ctx.c.THUBURN_LIM;
       for (int q = 0; q < 3; ++q) f.gq[q] = gq[q];
       for (int q = 0; q < 9; ++q) f.xc[q] = xc[q];
-      x.run(f, nc);
+      x.run_pts(f, d);
     }
     {
       TrcVert2 f;

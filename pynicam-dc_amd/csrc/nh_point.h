@@ -14,17 +14,6 @@
 
 NH_BEGIN
 
-NH_HD inline Cell cell_of_pt(const Pt& p) {
-  Cell c;
-  c.pole = p.pole;
-  c.inner = p.inner;
-  c.i = p.i; c.j = p.j; c.l = p.l; c.v = p.v; c.lp = p.lp;
-  c.ks = p.ks;
-  c.base = p.a - (long)p.k * p.ks;
-  c.ci = p.ci;
-  return c;
-}
-
 // ----------------------------------------------------------------------------
 // single operators (tests and callers of the C-ABI)
 // ----------------------------------------------------------------------------

@@ -174,13 +174,19 @@ struct TrcHFlux {
   const real* pm;     // 5 mean-flux fields: rhog, rhogvx, rhogvy, rhogvz, rhogw
   real* fa[3];
   real* xc[9];
-  NH_HD void operator()(long t) const {
-    Cell cl = cell_of(d, t);
+  // the column launch walks every level; the point launch (more threads, the
+  // levels being independent) does one
+  NH_HD void operator()(long t) const { levels(cell_of(d, t), 0, d.kall); }
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (pt_of(d, plane, x, p)) levels(cell_of_pt(p), p.k, p.k + 1);
+  }
+  NH_HD void levels(const Cell& cl, int k0, int k1) const {
     const long NF = d.NF;
     if (cl.pole) {
       if (cl.v != 0) return;
       const int lp = cl.lp;
-      for (int k = 0; k < d.kall; ++k) {
+      for (int k = k0; k < k1; ++k) {
         const long a0 = cl.base + k * 6;
         real rt[6][4];
         for (int v = 1; v <= 5; ++v) {
@@ -209,7 +215,7 @@ struct TrcHFlux {
     if (i > g - 2 || j > g - 2) return;
     const long nreg = d.nreg, ci = cl.ci, ks = cl.ks;
     const real m0 = G.sgp[cl.l] ? real(0.0) : real(1.0), m1 = G.sgp[cl.l] ? real(1.0) : real(0.0);
-    for (int k = 0; k < d.kall; ++k) {
+    for (int k = k0; k < k1; ++k) {
       const long a = cl.base + k * ks;
       // triangle value of type T with corner at offset (di, dj) from this cell
       auto tri = [&](int T, int di, int dj, int q) -> real {
@@ -285,10 +291,16 @@ struct TrcRhog2 {
   const real* rhog1;
   const real* frhog;
   real* rhog2;
-  NH_HD void operator()(long t) const {
-    Cell cl = cell_of(d, t);
+  // the column launch walks every level; the point launch (more threads, the
+  // levels being independent) does one
+  NH_HD void operator()(long t) const { levels(cell_of(d, t), 0, d.kall); }
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (pt_of(d, plane, x, p)) levels(cell_of_pt(p), p.k, p.k + 1);
+  }
+  NH_HD void levels(const Cell& cl, int k0, int k1) const {
     if (cl.pole ? cl.v != 0 : !cl.inner) return;
-    for (int k = 0; k < d.kall; ++k) {
+    for (int k = k0; k < k1; ++k) {
       const long a = cl.base + k * cl.ks;
       if (cl.pole) {
         real r = rhog1[a];
@@ -311,12 +323,18 @@ struct TrcGradQ {
   const real* rq;
   const real* rhog1;
   real* gq[3];
-  NH_HD void operator()(long t) const {
-    Cell cl = cell_of(d, t);
+  // the column launch walks every level; the point launch (more threads, the
+  // levels being independent) does one
+  NH_HD void operator()(long t) const { levels(cell_of(d, t), 0, d.kall); }
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (pt_of(d, plane, x, p)) levels(cell_of_pt(p), p.k, p.k + 1);
+  }
+  NH_HD void levels(const Cell& cl, int k0, int k1) const {
     if (cl.pole ? cl.v != 0 : !cl.inner) return;
     long nb[7];
     int nn = neighbours(d, cl, nb);
-    for (int k = 0; k < d.kall; ++k) {
+    for (int k = k0; k < k1; ++k) {
       const long kk = k * cl.ks;
       real qv[7];
       for (int m = 0; m < nn; ++m) qv[m] = rq[nb[m] + kk] / rhog1[nb[m] + kk];
@@ -351,8 +369,14 @@ struct TrcRemap {
   const real* fa[3];
   const real* xc[9];
   real* qa[3];
-  NH_HD void operator()(long t) const {
-    Cell cl = cell_of(d, t);
+  // the column launch walks every level; the point launch (more threads, the
+  // levels being independent) does one
+  NH_HD void operator()(long t) const { levels(cell_of(d, t), 0, d.kall); }
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (pt_of(d, plane, x, p)) levels(cell_of_pt(p), p.k, p.k + 1);
+  }
+  NH_HD void levels(const Cell& cl, int k0, int k1) const {
     if (cl.pole) {
       if (cl.v == 0) return;
       // pole arc v: upwind value from the pole centre or spoke v
@@ -360,7 +384,7 @@ struct TrcRemap {
       const long R0 = d.R + (long)lp * d.kall * 6;
       const real* xp0 = G.gx_pl + (0 * 2 + lp) * 3;
       const real* xpv = G.gx_pl + (v * 2 + lp) * 3;
-      for (int k = 0; k < d.kall; ++k) {
+      for (int k = k0; k < k1; ++k) {
         const long a0 = R0 + k * 6, av = a0 + v;
         const real q0 = rq[a0] / rhog1[a0], qv = rq[av] / rhog1[av];
         const real xcx = xc[0][av], xcy = xc[1][av], xcz = xc[2][av];
@@ -378,7 +402,7 @@ struct TrcRemap {
     // neighbour across each arc: AI (i+1, j), AIJ (i+1, j+1), AJ (i, j+1)
     const long noff[3] = {1, 1 + (long)g, (long)g};
     const bool valid[3] = {j >= 1, true, i >= 1};
-    for (int k = 0; k < d.kall; ++k) {
+    for (int k = k0; k < k1; ++k) {
       const long a = cl.base + k * ks;
       for (int arc = 0; arc < 3; ++arc) {
         if (!valid[arc]) { qa[arc][a] = real(0.0); continue; }
@@ -488,10 +512,16 @@ struct TrcQout {
   const real* frhog;
   const real* fa[3];
   real* qout[2];
-  NH_HD void operator()(long t) const {
-    Cell cl = cell_of(d, t);
+  // the column launch walks every level; the point launch (more threads, the
+  // levels being independent) does one
+  NH_HD void operator()(long t) const { levels(cell_of(d, t), 0, d.kall); }
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (pt_of(d, plane, x, p)) levels(cell_of_pt(p), p.k, p.k + 1);
+  }
+  NH_HD void levels(const Cell& cl, int k0, int k1) const {
     if (cl.pole ? cl.v != 0 : !cl.inner) return;
-    for (int k = 0; k < d.kall; ++k) {
+    for (int k = k0; k < k1; ++k) {
       const long a = cl.base + k * cl.ks;
       const real dk = real(1.0) * frhog[a] / rhog1[a] * dt;
       if (cl.pole) {
@@ -582,11 +612,17 @@ struct TrcApply {
     const real cm = upwind_mask(fm / rhog1[a]);
     return cm * am + (real(1.0) - cm) * ap;
   }
-  NH_HD void operator()(long t) const {
-    Cell cl = cell_of(d, t);
+  // the column launch walks every level; the point launch (more threads, the
+  // levels being independent) does one
+  NH_HD void operator()(long t) const { levels(cell_of(d, t), 0, d.kall); }
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (pt_of(d, plane, x, p)) levels(cell_of_pt(p), p.k, p.k + 1);
+  }
+  NH_HD void levels(const Cell& cl, int k0, int k1) const {
     if (cl.pole ? cl.v != 0 : !cl.inner) return;
     const int g = d.g;
-    for (int k = 0; k < d.kall; ++k) {
+    for (int k = k0; k < k1; ++k) {
       const long a = cl.base + k * cl.ks;
       if (!limiter) {
         if (cl.pole) {
