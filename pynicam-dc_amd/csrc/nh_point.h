@@ -560,20 +560,41 @@ struct ViPath0P {
     const real* pre = diag + I_PRE * NF;
     const long b = a - (long)k * ks;     // column base
     auto lev = [&](int kk) { return b + (long)kk * ks; };
+    // the column window k-2 .. k+1 of rhog, pressure, eth and rhogw, loaded
+    // first at levels clamped into range (a clamped value is never used: the
+    // level tests below are the reference's), read through selects
+    real Wrg[4], Wpr[4], Wet[4], Wrw[4];
+    for (int jw = 0; jw < 4; ++jw) {
+      int kw = k - 2 + jw;
+      kw = (kw < 0) ? 0 : ((kw > d.kall - 1) ? d.kall - 1 : kw);
+      const long aw = lev(kw);
+      Wrg[jw] = rg[aw];
+      Wpr[jw] = pre[aw];
+      Wet[jw] = eth[aw];
+      Wrw[jw] = rw[aw];
+    }
+    auto win = [&](const real* W, int kk) {
+      const int jw = kk - k + 2;
+      return (jw <= 0) ? W[0] : (jw == 1) ? W[1] : (jw == 2) ? W[2] : W[3];
+    };
+    auto RG = [&](int kk) { return win(Wrg, kk); };
+    auto PR = [&](int kk) { return win(Wpr, kk); };
+    auto ET = [&](int kk) { return win(Wet, kk); };
+    auto RW = [&](int kk) { return win(Wrw, kk); };
     auto rhog_h = [&](int kk) {
       const int k2 = (kk < kmin) ? kmin : kk;
-      return G.c2w0[k2] * rg[lev(k2)] + G.c2w1[k2] * rg[lev(k2 - 1)];
+      return G.c2w0[k2] * RG(k2) + G.c2w1[k2] * RG(k2 - 1);
     };
     auto dpgw = [&](int kk) {
-      return (kk >= kmin + 1 && kk <= kmax) ? real(1.0) * (pre[lev(kk)] * real(1.0) - pre[lev(kk - 1)] * real(1.0)) * G.rdgzh[kk] : real(0.0);
+      return (kk >= kmin + 1 && kk <= kmax) ? real(1.0) * (PR(kk) * real(1.0) - PR(kk - 1) * real(1.0)) * G.rdgzh[kk] : real(0.0);
     };
     auto dbuo = [&](int kk) {
       return (kk >= kmin + 1 && kk <= kmax)
-                 ? -c.GRAV * (G.c2w0[kk] * (rg[lev(kk)] / real(1.0) * real(1.0)) + G.c2w1[kk] * (rg[lev(kk - 1)] / real(1.0) * real(1.0)))
+                 ? -c.GRAV * (G.c2w0[kk] * (RG(kk) / real(1.0) * real(1.0)) + G.c2w1[kk] * (RG(kk - 1) / real(1.0) * real(1.0)))
                  : real(0.0);
     };
     auto gzt = [&](int kk) { return c.GRAV - (dpgw(kk) - dbuo(kk)) / rhog_h(kk); };
-    auto pwh = [&](int kk) { return -gzt(kk) * rw[lev(kk)]; };
+    auto pwh = [&](int kk) { return -gzt(kk) * RW(kk); };
     real gx, gy, gz;
     if (p.pole) {
       gx = G.gx_pl[p.lp * 3 + 0]; gy = G.gx_pl[p.lp * 3 + 1]; gz = G.gx_pl[p.lp * 3 + 2];
@@ -582,11 +603,11 @@ struct ViPath0P {
     }
     const real Kh = c.GAMMA_H;
     // hyperdiffusion tendency + momentum advection = large-step tendency g0
-    const real rgk = rg[a];
+    const real rgk = RG(k);
     real fvx = -(vt[0][a] * Kh + real(0.0)) * rgk;
     real fvy = -(vt[1][a] * Kh + real(0.0)) * rgk;
     real fvz = -(vt[2][a] * Kh + real(0.0)) * rgk;
-    const real rgh = (k >= kmin) ? G.c2w0[k] * rgk + G.c2w1[k] * rg[a - ks] : real(0.0);
+    const real rgh = (k >= kmin) ? G.c2w0[k] * rgk + G.c2w1[k] * RG(k - 1) : real(0.0);
     const real fw = -(vt[3][a] * Kh + real(0.0)) * rgh;
     const real fe = -(vt[4][a] + real(0.0));
     const real fr = -(vt[5][a] + real(0.0));
@@ -621,14 +642,14 @@ struct ViPath0P {
         }
       }, dvs);
       const real hd = dvs[0];
-      const real vmh_hi = (k + 1 <= kmax) ? real(0.0) * real(1.0) + real(1.0) * rw[a + ks] * real(1.0) : real(0.0);
-      const real vmh_lo = (k >= kmin + 1) ? real(0.0) * real(1.0) + real(1.0) * rw[a] * real(1.0) : real(0.0);
+      const real vmh_hi = (k + 1 <= kmax) ? real(0.0) * real(1.0) + real(1.0) * RW(k + 1) * real(1.0) : real(0.0);
+      const real vmh_lo = (k >= kmin + 1) ? real(0.0) * real(1.0) + real(1.0) * RW(k) * real(1.0) : real(0.0);
       drhog = -hd - (vmh_hi - vmh_lo) * G.rdgz[k];
       const real he = dvs[1];
       const real ws_hi = (k + 1 <= kmax)
-          ? real(0.0) * real(1.0) + real(1.0) * (rw[a + ks] * (G.afact[k + 1] * eth[a + ks] + G.bfact[k + 1] * eth[a])) * real(1.0) : real(0.0);
+          ? real(0.0) * real(1.0) + real(1.0) * (RW(k + 1) * (G.afact[k + 1] * ET(k + 1) + G.bfact[k + 1] * ET(k))) * real(1.0) : real(0.0);
       const real ws_lo = (k >= kmin + 1)
-          ? real(0.0) * real(1.0) + real(1.0) * (rw[a] * (G.afact[k] * eth[a] + G.bfact[k] * eth[a - ks])) * real(1.0) : real(0.0);
+          ? real(0.0) * real(1.0) + real(1.0) * (RW(k) * (G.afact[k] * ET(k) + G.bfact[k] * ET(k - 1))) * real(1.0) : real(0.0);
       drhoge = -he - (ws_hi - ws_lo) * G.rdgz[k];
       presgrad_ld(p, ld, R, dpg);
       pw = diag[I_VX * NF + a] * dpg[0] + diag[I_VY * NF + a] * dpg[1] + diag[I_VZ * NF + a] * dpg[2]
@@ -647,20 +668,20 @@ struct ViPath0P {
       const ConstsT<double>& C = c.d;
       auto rhog_h64 = [&](int kk) {
         const int k2 = (kk < kmin) ? kmin : kk;
-        return V.c2w0[k2] * (double)rg[lev(k2)] + V.c2w1[k2] * (double)rg[lev(k2 - 1)];
+        return V.c2w0[k2] * (double)RG(k2) + V.c2w1[k2] * (double)RG(k2 - 1);
       };
       auto eth_h64 = [&](int kk) {
         const int k2 = (kk < kmin) ? kmin : kk;
-        return V.afact[k2] * (double)eth[lev(k2)] + V.bfact[k2] * (double)eth[lev(k2 - 1)];
+        return V.afact[k2] * (double)ET(k2) + V.bfact[k2] * (double)ET(k2 - 1);
       };
       auto dpgw64 = [&](int kk) {
         return (kk >= kmin + 1 && kk <= kmax)
-                   ? 1.0 * ((double)pre[lev(kk)] * 1.0 - (double)pre[lev(kk - 1)] * 1.0) * V.rdgzh[kk] : 0.0;
+                   ? 1.0 * ((double)PR(kk) * 1.0 - (double)PR(kk - 1) * 1.0) * V.rdgzh[kk] : 0.0;
       };
       auto dbuo64 = [&](int kk) {
         return (kk >= kmin + 1 && kk <= kmax)
-                   ? -C.GRAV * (V.c2w0[kk] * ((double)rg[lev(kk)] / 1.0 * 1.0) +
-                                V.c2w1[kk] * ((double)rg[lev(kk - 1)] / 1.0 * 1.0))
+                   ? -C.GRAV * (V.c2w0[kk] * ((double)RG(kk) / 1.0 * 1.0) +
+                                V.c2w1[kk] * ((double)RG(kk - 1) / 1.0 * 1.0))
                    : 0.0;
       };
       auto gzt64 = [&](int kk) { return C.GRAV - (dpgw64(kk) - dbuo64(kk)) / rhog_h64(kk); };
