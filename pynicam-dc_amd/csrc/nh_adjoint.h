@@ -270,15 +270,34 @@ struct HDiffPass2A {
     Pt p;
     if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
     const long a = p.a;
+    // regional points: the bars of rhog and of each field summed per stencil
+    // point first, then one atomic add per stencil point
+    real rb[7] = {real(0.0), real(0.0), real(0.0), real(0.0), real(0.0), real(0.0), real(0.0)};
     for (int f = 0; f < 2; ++f) {
       const real o = -ob[4 + f][a];
       ob[4 + f][a] = real(0.0);
       const real kfac = ((f == 0) ? c.CVdry : c.HDIFF_FACT_RHO) * c.GAMMA_H;
       if (p.pole) diff_pl_t(p, in[4 + f], ib[4 + f], kfac, o);
-      else diff7_t(p, in[4 + f], ib[4 + f], kfac, o);
+      else diff7_t(p, in[4 + f], ib[4 + f], kfac, o, rb);
+    }
+    if (!p.pole)
+      for (int m = 0; m < 7; ++m) aadd(&rhog_b[a + nb_off(m, d.g)], rb[m]);
+  }
+  // tri_grad_t's three vertex bars without the adds
+  NH_HD void tri_grad_b(long cj, int T, const real ob3[3], real& b0, real& b1, real& b2) const {
+    const long nreg = d.nreg;
+    b0 = real(0.0); b1 = real(0.0); b2 = real(0.0);
+    for (int dd = 0; dd < 3; ++dd) {
+      const real c1 = rc(G.cintp, nreg, (dd * 3 + 0) * 2 + T, cj);
+      const real c2 = rc(G.cintp, nreg, (dd * 3 + 1) * 2 + T, cj);
+      const real c3 = rc(G.cintp, nreg, (dd * 3 + 2) * 2 + T, cj);
+      const real o = ob3[dd] / real(3.0);
+      b0 += (real(2.0) * c1 - c2 - c3) * o;
+      b1 += (-c1 + real(2.0) * c2 - c3) * o;
+      b2 += (-c1 - c2 + real(2.0) * c3) * o;
     }
   }
-  NH_HD void diff7_t(const Pt& p, const real* s, real* sb, real kfac, real o) const {
+  NH_HD void diff7_t(const Pt& p, const real* s, real* sb, real kfac, real o, real rb[7]) const {
     const int g = d.g;
     const long nreg = d.nreg, ci = p.ci, a = p.a;
     // the six triangles: TI(i,j) TJ(i,j) TI(i-1,j) TJ(i-1,j-1) TI(i-1,j-1) TJ(i,j-1)
@@ -309,18 +328,30 @@ struct HDiffPass2A {
         tb[nb][dd] += cc * kf[e];
       }
     }
+    // stencil index of the arcs' neighbours kn[e]: (+1,+1) 2, (0,+1) 3, (-1,0) 4, (-1,-1) 5, (0,-1) 6, (+1,0) 1
+    const int em[6] = {2, 3, 4, 5, 6, 1};
     real k0b = real(0.0);
     for (int e = 0; e < 6; ++e) {
       k0b += real(0.5) * kfb[e];
-      aadd(&rhog_b[a + kn[e]], real(0.5) * kfb[e] * kfac);
+      rb[em[e]] += real(0.5) * kfb[e] * kfac;
     }
-    aadd(&rhog_b[a], k0b * kfac);
+    rb[0] += k0b * kfac;
     if (pent)
       for (int dd = 0; dd < 3; ++dd) {
         tb[5][dd] += tb[4][dd] * m1;
         tb[4][dd] = tb[4][dd] * m0;
       }
-    for (int n = 0; n < 6; ++n) tri_grad_t(G, nreg, ci + tc[n], tT[n], sb, a + tc[n], g, tb[n]);
+    // the triangles' vertices as stencil indices
+    const int vm[6][3] = {{0, 1, 2}, {0, 2, 3}, {4, 0, 3}, {5, 0, 4}, {5, 6, 0}, {6, 1, 0}};
+    real acc[7] = {real(0.0), real(0.0), real(0.0), real(0.0), real(0.0), real(0.0), real(0.0)};
+    for (int n = 0; n < 6; ++n) {
+      real b0, b1, b2;
+      tri_grad_b(ci + tc[n], tT[n], tb[n], b0, b1, b2);
+      acc[vm[n][0]] += b0;
+      acc[vm[n][1]] += b1;
+      acc[vm[n][2]] += b2;
+    }
+    for (int m = 0; m < 7; ++m) aadd(&sb[a + nb_off(m, g)], acc[m]);
   }
   NH_HD void diff_pl_t(const Pt& p, const real* s, real* sb, real kfac, real o) const {
     const long a0 = p.a;
@@ -1382,12 +1413,18 @@ struct ViPath0A {
         aadd(&diag_b[(I_VX + q) * NF + a], pwb * dpg[q]);
       }
       proj_t(xx, R, dpgb);
-      for (int dd2 = 0; dd2 < 3; ++dd2) {
-        if (p.pole) {
-          for (int v = 0; v < 6; ++v) aadd(&diag_b[I_PRE * NF + a + v + sh], G.cgrad_pl[(v * 2 + p.lp) * 3 + dd2] * dpgb[dd2]);
-        } else {
-          for (int m = 0; m < 7; ++m)
-            aadd(&diag_b[I_PRE * NF + a + nb_off(m, d.g) + sh], rc(G.cgrad, d.nreg, dd2 * 7 + m, p.ci) * dpgb[dd2]);
+      // one atomic add per stencil point (the three directions summed first)
+      if (p.pole) {
+        for (int v = 0; v < 6; ++v) {
+          real sb = real(0.0);
+          for (int dd2 = 0; dd2 < 3; ++dd2) sb += G.cgrad_pl[(v * 2 + p.lp) * 3 + dd2] * dpgb[dd2];
+          aadd(&diag_b[I_PRE * NF + a + v + sh], sb);
+        }
+      } else {
+        for (int m = 0; m < 7; ++m) {
+          real sb = real(0.0);
+          for (int dd2 = 0; dd2 < 3; ++dd2) sb += rc(G.cgrad, d.nreg, dd2 * 7 + m, p.ci) * dpgb[dd2];
+          aadd(&diag_b[I_PRE * NF + a + nb_off(m, d.g) + sh], sb);
         }
       }
       // pw: w2c0(k) pwh(k+1) + w2c1(k) pwh(k), pwh(kk) = -g~(kk) rw(kk)
