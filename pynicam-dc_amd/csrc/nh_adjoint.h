@@ -1301,8 +1301,20 @@ struct ViPath0A {
       mcb = Mc_b[a]; mub = Mu_b[a]; mlb = Ml_b[a];
       Mc_b[a] = 0.0; Mu_b[a] = 0.0; Ml_b[a] = 0.0;
     }
-    auto add_rg = [&](int kk, real v) { aadd(&prog_b[lev(kk)], v); };
-    auto add_pre = [&](int kk, real v) { aadd(&diag_b[I_PRE * NF + lev(kk)], v); };
+    // bars of rhog, pressure, eth and rhogw at the column levels k-2 .. k+1,
+    // summed here and added once per level at the end (select-based adds keep
+    // the indices static)
+    real Arg[4] = {real(0.0), real(0.0), real(0.0), real(0.0)}, Apr[4] = {real(0.0), real(0.0), real(0.0), real(0.0)};
+    real Aet[4] = {real(0.0), real(0.0), real(0.0), real(0.0)}, Arw[4] = {real(0.0), real(0.0), real(0.0), real(0.0)};
+    auto accw = [&](real* A, int kk, real v) {
+      const int jw = kk - k + 2;
+      A[0] += (jw <= 0) ? v : real(0.0);
+      A[1] += (jw == 1) ? v : real(0.0);
+      A[2] += (jw == 2) ? v : real(0.0);
+      A[3] += (jw >= 3) ? v : real(0.0);
+    };
+    auto add_rg = [&](int kk, real v) { accw(Arg, kk, v); };
+    auto add_pre = [&](int kk, real v) { accw(Apr, kk, v); };
     auto rgl = [&](int kk) { return ldc(rg + (lev(kk))); };
     auto prl = [&](int kk) { return ldc(pre + (lev(kk))); };
     GztCol<real> gz{&G, c.GRAV, kmin, kmax};
@@ -1334,7 +1346,7 @@ struct ViPath0A {
       aadd(&vt_b[3][a], -gb[4] * Kh * rgh);
       const real rghb = -gb[4] * ldc(vt[3] + (a)) * Kh;
       rgkb += rghb * G.c2w0[k];
-      aadd(&prog_b[a - ks], rghb * G.c2w1[k]);
+      Arg[1] += rghb * G.c2w1[k];
     }
     if (gz.in(k)) {
       add_pre(k, -gb[4] * G.rdgzh[k]);
@@ -1347,23 +1359,23 @@ struct ViPath0A {
     if (k >= kmin && k <= kmax) {
       // drhog = -div(rhogv) - (rw(k+1) - rw(k)) rdgz
       const real hdb = -gb[0];
-      if (k + 1 <= kmax) aadd(&prog_b[4 * NF + a + ks], -gb[0] * G.rdgz[k]);
-      if (k >= kmin + 1) aadd(&prog_b[4 * NF + a], gb[0] * G.rdgz[k]);
+      if (k + 1 <= kmax) Arw[3] += -gb[0] * G.rdgz[k];
+      if (k >= kmin + 1) Arw[2] += gb[0] * G.rdgz[k];
       // drhoge = -div(rhogv eth) - (ws_hi - ws_lo) rdgz
       const real heb = -gb[5];
       if (k + 1 <= kmax) {
         const real wsb = -gb[5] * G.rdgz[k];
         const real w = ldc(rw + (a + ks));
-        aadd(&prog_b[4 * NF + a + ks], wsb * (G.afact[k + 1] * ldc(eth + (a + ks)) + G.bfact[k + 1] * ldc(eth + (a))));
-        aadd(&eth_b[a + ks], wsb * w * G.afact[k + 1]);
-        aadd(&eth_b[a], wsb * w * G.bfact[k + 1]);
+        Arw[3] += wsb * (G.afact[k + 1] * ldc(eth + (a + ks)) + G.bfact[k + 1] * ldc(eth + (a)));
+        Aet[3] += wsb * w * G.afact[k + 1];
+        Aet[2] += wsb * w * G.bfact[k + 1];
       }
       if (k >= kmin + 1) {
         const real wsb = gb[5] * G.rdgz[k];
         const real w = ldc(rw + (a));
-        aadd(&prog_b[4 * NF + a], wsb * (G.afact[k] * ldc(eth + (a)) + G.bfact[k] * ldc(eth + (a - ks))));
-        aadd(&eth_b[a], wsb * w * G.afact[k]);
-        aadd(&eth_b[a - ks], wsb * w * G.bfact[k]);
+        Arw[2] += wsb * (G.afact[k] * ldc(eth + (a)) + G.bfact[k] * ldc(eth + (a - ks)));
+        Aet[2] += wsb * w * G.afact[k];
+        Aet[1] += wsb * w * G.bfact[k];
       }
       // the two divergences
       const int NN = p.pole ? 6 : 7;
@@ -1433,11 +1445,11 @@ struct ViPath0A {
         const real wgt = (s2 == 0) ? G.w2c0[k] : G.w2c1[k];
         const real hb = pwb * wgt;
         const real gzv = gz.value(kk, rgl, prl);
-        aadd(&prog_b[4 * NF + lev(kk)], -hb * gzv);
+        accw(Arw, kk, -hb * gzv);
         gz.adj(kk, rgl, prl, add_rg, add_pre, -hb * ldc(rw + (lev(kk))));
       }
     }
-    aadd(&prog_b[a], rgkb);
+    Arg[2] += rgkb;
     if (k >= kmin + 1 && k <= kmax) {
       // tridiagonal coefficients (FP64)
       const VProf<double>& V = G.v64;
@@ -1445,12 +1457,12 @@ struct ViPath0A {
       GztCol<double> gz64{&V, C.GRAV, kmin, kmax};
       auto rg64 = [&](int kk) { return (double)ldc(rg + (lev(kk))); };
       auto pr64 = [&](int kk) { return (double)ldc(pre + (lev(kk))); };
-      auto add_rg64 = [&](int kk, double v) { aadd(&prog_b[lev(kk)], (real)v); };
-      auto add_pre64 = [&](int kk, double v) { aadd(&diag_b[I_PRE * NF + lev(kk)], (real)v); };
+      auto add_rg64 = [&](int kk, double v) { accw(Arg, kk, (real)v); };
+      auto add_pre64 = [&](int kk, double v) { accw(Apr, kk, (real)v); };
       auto eth_h_t = [&](int kk, double v) {
         const int k2 = (kk < kmin) ? kmin : kk;
-        aadd(&eth_b[lev(k2)], (real)(v * V.afact[k2]));
-        aadd(&eth_b[lev(k2 - 1)], (real)(v * V.bfact[k2]));
+        accw(Aet, k2, (real)(v * V.afact[k2]));
+        accw(Aet, k2 - 1, (real)(v * V.bfact[k2]));
       };
       const double Ca = (1.0 * V.rdgz[k] + 1.0 * V.rdgz[k - 1]) * 1.0;
       const double Cb = V.dfact[k] - V.cfact[k - 1];
@@ -1461,6 +1473,15 @@ struct ViPath0A {
       gz64.adj(k + 1, rg64, pr64, add_rg64, add_pre64, -mub * rz * V.cfact[k]);
       eth_h_t(k - 1, -mlb * rz * V.rdgz[k]);
       gz64.adj(k - 1, rg64, pr64, add_rg64, add_pre64, mlb * rz * V.dfact[k - 1]);
+    }
+    for (int jw = 0; jw < 4; ++jw) {
+      const int kw = k - 2 + jw;
+      if (kw < 0 || kw > d.kall - 1) continue;
+      const long aw = lev(kw);
+      if (Arg[jw] != real(0.0)) aadd(&prog_b[aw], Arg[jw]);
+      if (Apr[jw] != real(0.0)) aadd(&diag_b[I_PRE * NF + aw], Apr[jw]);
+      if (Aet[jw] != real(0.0)) aadd(&eth_b[aw], Aet[jw]);
+      if (Arw[jw] != real(0.0)) aadd(&prog_b[4 * NF + aw], Arw[jw]);
     }
   }
 };
