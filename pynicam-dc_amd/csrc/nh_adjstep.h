@@ -566,7 +566,7 @@ int tracer_step_adj(X& x, const Context& ctx, const real* prog, const real* prog
       f.ob = rqh_b; f.sk = sk; f.limiter = lim;
       for (int q = 0; q < 3; ++q) f.gq_b[q] = gq_b[q];
       for (int q = 0; q < 9; ++q) f.xc[q] = xc[q];
-      x.run(f, nc);
+      x.run_pts(f, d);
     }
     if (lim) {
       halo_adj(x, ctx, qo_b, 2);
@@ -576,7 +576,7 @@ int tracer_step_adj(X& x, const Context& ctx, const real* prog, const real* prog
       for (int q = 0; q < 3; ++q) f.fa[q] = fa[q];
       f.qo_b[0] = qo_b[0]; f.qo_b[1] = qo_b[1];
       f.sk = sk;
-      x.run(f, nc);
+      x.run_pts(f, d);
     }
     {
       TrcRemapA f;
@@ -585,7 +585,7 @@ int tracer_step_adj(X& x, const Context& ctx, const real* prog, const real* prog
       for (int q = 0; q < 3; ++q) { f.fa[q] = fa[q]; f.qa_b[q] = qa_b[q]; f.gq_b[q] = gq_b[q]; }
       for (int q = 0; q < 9; ++q) f.xc[q] = xc[q];
       f.rq_b = rq_b;
-      x.run(f, nc);
+      x.run_pts(f, d);
     }
     halo_adj(x, ctx, gq_b, 3);
     {
@@ -593,7 +593,7 @@ int tracer_step_adj(X& x, const Context& ctx, const real* prog, const real* prog
       f.d = d; f.G = ctx.G; f.rhog1 = rhog1;
       for (int q = 0; q < 3; ++q) f.gq_b[q] = gq_b[q];
       f.rq_b = rq_b;
-      x.run(f, nc);
+      x.run_pts(f, d);
     }
     {
       TrcVert1A f;

@@ -1,14 +1,17 @@
-// LDS-tiled kernels: the level tiles of the staged stencil point kernels
-// (StagedPointT, below), and the column tiles of the triangle-based operators
-// (the 3-D divergence damping DivDampK, the Kh-weighted diffusion DiffKhK).
+// LDS-tiled kernels.
 //
-// A block owns a tile of one region level: all interior i (W = g-2 points,
-// one or more wavefronts) times TH rows.  Phase 1 computes the two triangle
-// values of every tile corner (i', j'), i' in 0..W, j' in j0-1..j0+TH-1, once
-// into LDS; phase 2 forms each cell's hexagon sum from the six triangles
-// around it.  Each triangle value is shared by three cells, so this does a
-// third of the work and loads of the per-point form.  Blocks past the
-// regional tiles handle the pole levels (the pole centre's five triangles).
+// Level tiles (StagedPointT): a block owns all interior i (W = g-2 points) of
+// TH rows of one region level; phase 0 stages the fields a point kernel reads
+// on its stencil (the tile's rows and one halo row on each side) into LDS, and
+// the point kernel then reads its stencil from there.  Blocks past the
+// regional tiles do the pole levels with the point kernel itself.
+//
+// Column tiles (DivDampK, DiffKhK; the stencil engine of nh_colstencil.h): a
+// block owns a 16 x 16 tile of one region over a chunk of levels, each
+// thread one corner column with its coefficients in registers; per level the
+// corner threads form their two triangle values into double-buffered LDS, one
+// barrier, then the tile points form their hexagon sums.
+//
 // On the GPU the phases are separated by a workgroup barrier; the host test
 // build runs them one after the other.
 #pragma once

@@ -109,11 +109,17 @@ struct TrcApplyA {
   int limiter;
   real* gq_b[3];       // limiter off: bars of the gradient (faces 3..5)
   const real* xc[9];
-  NH_HD void operator()(long t) const {
-    Cell cl = cell_of(d, t);
+  // per level on the point grid (levels independent; the adds to other
+  // points are atomic), or every level of a column
+  NH_HD void operator()(long t) const { levels(cell_of(d, t), 0, d.kall); }
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (pt_of(d, plane, x, p)) levels(cell_of_pt(p), p.k, p.k + 1);
+  }
+  NH_HD void levels(const Cell& cl, int k0, int k1) const {
     if (cl.pole ? cl.v != 0 : !cl.inner) return;
     const int g = d.g;
-    for (int k = 0; k < d.kall; ++k) {
+    for (int k = k0; k < k1; ++k) {
       const long a = cl.base + k * cl.ks;
       const real w = ob[a];
       ob[a] = real(0.0);
@@ -210,10 +216,16 @@ struct TrcQoutA {
   const real* fa[3];
   real* qo_b[2];
   TgSink sk;
-  NH_HD void operator()(long t) const {
-    Cell cl = cell_of(d, t);
+  // per level on the point grid (levels independent; the adds to other
+  // points are atomic), or every level of a column
+  NH_HD void operator()(long t) const { levels(cell_of(d, t), 0, d.kall); }
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (pt_of(d, plane, x, p)) levels(cell_of_pt(p), p.k, p.k + 1);
+  }
+  NH_HD void levels(const Cell& cl, int k0, int k1) const {
     if (cl.pole ? cl.v != 0 : !cl.inner) return;
-    for (int k = 0; k < d.kall; ++k) {
+    for (int k = k0; k < k1; ++k) {
       const long a = cl.base + k * cl.ks;
       const real b0 = qo_b[0][a], b1 = qo_b[1][a];
       qo_b[0][a] = real(0.0);
@@ -298,15 +310,21 @@ struct TrcRemapA {
   real* qa_b[3];
   real* rq_b;
   real* gq_b[3];
-  NH_HD void operator()(long t) const {
-    Cell cl = cell_of(d, t);
+  // per level on the point grid (levels independent; the adds to other
+  // points are atomic), or every level of a column
+  NH_HD void operator()(long t) const { levels(cell_of(d, t), 0, d.kall); }
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (pt_of(d, plane, x, p)) levels(cell_of_pt(p), p.k, p.k + 1);
+  }
+  NH_HD void levels(const Cell& cl, int k0, int k1) const {
     if (cl.pole) {
       if (cl.v == 0) return;
       const int lp = cl.lp, v = cl.v;
       const long R0 = d.R + (long)lp * d.kall * 6;
       const real* xp0 = G.gx_pl + (0 * 2 + lp) * 3;
       const real* xpv = G.gx_pl + (v * 2 + lp) * 3;
-      for (int k = 0; k < d.kall; ++k) {
+      for (int k = k0; k < k1; ++k) {
         const long a0 = R0 + k * 6, av = a0 + v;
         const real w = qa_b[0][av];
         qa_b[0][av] = real(0.0);
@@ -324,7 +342,7 @@ struct TrcRemapA {
     const long nreg = d.nreg, ci = cl.ci, ks = cl.ks;
     const long noff[3] = {1, 1 + (long)g, (long)g};
     const bool valid[3] = {j >= 1, true, i >= 1};
-    for (int k = 0; k < d.kall; ++k) {
+    for (int k = k0; k < k1; ++k) {
       const long a = cl.base + k * ks;
       for (int arc = 0; arc < 3; ++arc) {
         const real w = qa_b[arc][a];
@@ -352,12 +370,18 @@ struct TrcGradQA {
   const real* rhog1;
   real* gq_b[3];
   real* rq_b;
-  NH_HD void operator()(long t) const {
-    Cell cl = cell_of(d, t);
+  // per level on the point grid (levels independent; the adds to other
+  // points are atomic), or every level of a column
+  NH_HD void operator()(long t) const { levels(cell_of(d, t), 0, d.kall); }
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (pt_of(d, plane, x, p)) levels(cell_of_pt(p), p.k, p.k + 1);
+  }
+  NH_HD void levels(const Cell& cl, int k0, int k1) const {
     if (cl.pole ? cl.v != 0 : !cl.inner) return;
     long nb[7];
     const int nn = neighbours(d, cl, nb);
-    for (int k = 0; k < d.kall; ++k) {
+    for (int k = k0; k < k1; ++k) {
       const long kk = k * cl.ks;
       real gb[3];
       for (int dd = 0; dd < 3; ++dd) {
