@@ -336,11 +336,11 @@ struct DivDampK {
 
 // ----------------------------------------------------------------------------
 // the Kh-weighted diffusion of the second hyperdiffusion pass (fields 4 and 5
-// of HDiffPass2T: temperature with CVdry, density with HDIFF_FACT_RHO) as
+// of the pass: temperature with CVdry, density with HDIFF_FACT_RHO) as
 // column tiles, the same layout as DivDampK: corner threads form the triangle
 // gradients of both fields from their 18 held cintp coefficients, tile points
 // the Kh-weighted hexagon sums from their 18 held cdiff coefficients.
-// Results equal HDiffPass2T's (diffusion7, kernels/oprtdiffusion.py:59-177).
+// Results equal the point form's (diffusion7, kernels/oprtdiffusion.py:59-177).
 // ----------------------------------------------------------------------------
 struct DiffKhK {
   using LdsT = real;
