@@ -1,11 +1,11 @@
 // Column tiles for 7-point stencil kernels.
 //
-// A block owns a TW x TW tile of one region's points (TW = 16, or g - 2 on
-// smaller grids) over a chunk of levels.  Each of the first TW^2 threads owns
-// one tile column: the per-column coefficients of its operator are loaded once
-// into registers, and then, level by level, the thread stages its own values
-// of the operator's stencil fields into LDS (the first 4 TW + 2 threads also
-// stage one point of the ring around the tile), a barrier, and the thread
+// A block owns a TX x TY tile of one region's points (st_geom) over a chunk of
+// levels.  Each of the first TX TY threads owns one tile column: the
+// per-column coefficients of its operator are loaded once into registers, and
+// then, level by level, the thread stages its own values of the operator's
+// stencil fields into LDS (the first 2 TX + 2 TY + 2 threads also stage one
+// point of the ring around the tile), a barrier, and the thread
 // forms its point's result from its neighbours' staged values.  The staging
 // planes are double-buffered by level parity, so one barrier per level
 // suffices.  The point kernels re-read the 21 stencil coefficients (and the
@@ -24,6 +24,31 @@
 
 NH_BEGIN
 
+// tile shape of the stencil engine: TX x TY = 256 points, 64 x 4 where the
+// region is wide enough (a wavefront then reads one 64-point row segment;
+// measured faster than 32 x 8 and 16 x 16 at gl07)
+struct StGeom {
+  int TX, TY;       // tile width and height (points)
+  int nx, ny;       // tiles per region row and column
+  int nchunk, KC;   // level chunks, levels per chunk
+  int ntiles;       // lall * nchunk * nx * ny
+  int nblocks;      // + 2 * kall pole blocks
+};
+
+inline StGeom st_geom(const Dims& d, int nchunk) {
+  StGeom c;
+  const int W = d.g - 2;
+  c.TX = (W < 64) ? W : 64;
+  c.TY = (256 / c.TX < W) ? 256 / c.TX : W;
+  c.nx = W / c.TX;
+  c.ny = W / c.TY;
+  c.nchunk = nchunk;
+  c.KC = (d.kall + nchunk - 1) / nchunk;
+  c.ntiles = d.lall * nchunk * c.nx * c.ny;
+  c.nblocks = c.ntiles + 2 * d.kall;
+  return c;
+}
+
 template <class Op>
 struct ColStencil {
   using LdsT = real;
@@ -31,8 +56,8 @@ struct ColStencil {
   static constexpr int kThreads = 256;
   static constexpr int NS = Op::kNS;
   Op op;
-  ColGeom cg;
-  static int lds_elems(const ColGeom& cg) { return 2 * NS * (cg.TW + 2) * (cg.TW + 2); }
+  StGeom cg;
+  static int lds_elems(const StGeom& cg) { return 2 * NS * (cg.TX + 2) * (cg.TY + 2); }
   struct State {
     typename Op::Col col;
     long a, ra;        // own and ring addresses at the current level
@@ -46,29 +71,29 @@ struct ColStencil {
       if (tid == 0) op.pole(blk - cg.ntiles);
       return 0;
     }
-    const int g = d.g, TW = cg.TW, SW = TW + 2, tpr = cg.ntx * cg.ntx;
+    const int g = d.g, TX = cg.TX, TY = cg.TY, SW = TX + 2, tpr = cg.nx * cg.ny;
     const int tile = blk % tpr, rest = blk / tpr;
     const int chunk = rest % cg.nchunk, l = rest / cg.nchunk;
     const int k0 = chunk * cg.KC;
     const int k1 = (k0 + cg.KC < d.kall) ? k0 + cg.KC : d.kall;
     if (k0 >= k1) return 0;
-    const int i0 = 1 + (tile % cg.ntx) * TW, j0 = 1 + (tile / cg.ntx) * TW;
+    const int i0 = 1 + (tile % cg.nx) * TX, j0 = 1 + (tile / cg.nx) * TY;
     s.k = k0;
-    if (tid < TW * TW) {
-      const int tx = tid % TW, ty = tid / TW;
+    if (tid < TX * TY) {
+      const int tx = tid % TX, ty = tid / TX;
       const int i = i0 + tx, j = j0 + ty;
       s.outp = true;
       s.sidx = (ty + 1) * SW + tx + 1;
       s.a = ((long)(l * d.kall + k0) * g + j) * g + i;
       op.init_col(((long)l * g + j) * g + i, s.col);
     }
-    // the ring: row -1 (x = -1..TW-1), row TW (x = 0..TW), columns -1 and TW (y = 0..TW-1)
-    if (tid < 4 * TW + 2) {
+    // the ring: row -1 (x = -1..TX-1), row TY (x = 0..TX), columns -1 and TX (y = 0..TY-1)
+    if (tid < 2 * TX + 2 * TY + 2) {
       int x, y;
-      if (tid < TW + 1) { x = tid - 1; y = -1; }
-      else if (tid < 2 * TW + 2) { x = tid - (TW + 1); y = TW; }
-      else if (tid < 3 * TW + 2) { x = -1; y = tid - (2 * TW + 2); }
-      else { x = TW; y = tid - (3 * TW + 2); }
+      if (tid < TX + 1) { x = tid - 1; y = -1; }
+      else if (tid < 2 * TX + 2) { x = tid - (TX + 1); y = TY; }
+      else if (tid < 2 * TX + TY + 2) { x = -1; y = tid - (2 * TX + 2); }
+      else { x = TX; y = tid - (2 * TX + TY + 2); }
       s.ring = true;
       s.ridx = (y + 1) * SW + x + 1;
       s.ra = ((long)(l * d.kall + k0) * g + (j0 + y)) * g + (i0 + x);
@@ -77,7 +102,7 @@ struct ColStencil {
   }
   NH_HD void level_a(int step, State& s, real* lds) const {
     if (!op.stage_level(s.k)) return;
-    const int SW = cg.TW + 2, P = SW * SW;
+    const int SW = cg.TX + 2, P = SW * (cg.TY + 2);
     real* buf = lds + (step & 1) * NS * P;
     real v[NS];
     if (s.outp) {
@@ -91,7 +116,7 @@ struct ColStencil {
   }
   NH_HD void level_b(int step, State& s, const real* lds) const {
     if (s.outp) {
-      const int SW = cg.TW + 2, P = SW * SW;
+      const int SW = cg.TX + 2, P = SW * (cg.TY + 2);
       const real* buf = lds + (step & 1) * NS * P;
       const int x = s.sidx;
       op.point(s.k, s.a, s.col, [&](int m, int f) {

@@ -520,14 +520,15 @@ void stage_large(X& x, const Context& ctx, real* prog, const real* rhogq, double
     for (int q = 0; q < 6; ++q) f.out[q] = sf.hda + q * NF;
     f.rhog = prog;
     const ColGeom cg = col_geom(d, kColChunks);
+    const StGeom sg = st_geom(d, kColChunks);
     {
       ColStencil<LapC<6, HDiffPassP<0>>> cs;
       cs.op.pk = f;
       cs.op.d = d;
       for (int q = 0; q < 6; ++q) { cs.op.in[q] = f.in[q]; cs.op.out[q] = f.out[q]; }
       cs.op.pole_off = false;
-      cs.cg = cg;
-      x.run_cols(cs, cg.nblocks, decltype(cs)::lds_elems(cg));
+      cs.cg = sg;
+      x.run_cols(cs, sg.nblocks, decltype(cs)::lds_elems(sg));
     }
     halo_block(x, ctx, sf.hda, 6);
     // second pass: the Laplacians of fields 0..3, then the Kh-weighted
@@ -545,8 +546,8 @@ void stage_large(X& x, const Context& ctx, real* prog, const real* rhogq, double
       cs.op.d = d;
       for (int q = 0; q < 4; ++q) { cs.op.in[q] = f2.in[q]; cs.op.out[q] = f2.out[q]; }
       cs.op.pole_off = true;
-      cs.cg = cg;
-      x.run_cols(cs, cg.nblocks, decltype(cs)::lds_elems(cg));
+      cs.cg = sg;
+      x.run_cols(cs, sg.nblocks, decltype(cs)::lds_elems(sg));
     }
     {
       DiffKhK k2;
@@ -590,7 +591,7 @@ void small_hv(X& x, const Context& ctx, const real* prog, real* const dv[3], rea
   ColStencil<ViMainHC> cs;
   cs.op.pk = f;
   cs.op.d = ctx.d;
-  cs.cg = col_geom(ctx.d, kColChunks);
+  cs.cg = st_geom(ctx.d, kColChunks);
   x.run_cols(cs, cs.cg.nblocks, ColStencil<ViMainHC>::lds_elems(cs.cg));
 }
 

@@ -128,11 +128,11 @@ void run_staged(X& x, const P& pk) {
 }
 
 // ----------------------------------------------------------------------------
-// divergence damping as column tiles.  A block owns a TW x TW tile of one
+// divergence damping as column tiles.  A block owns a TX x TY tile of one
 // region's points over a chunk of levels; each thread owns one corner column
-// of the tile (the (TW+1)^2 corners whose two triangles the tile's hexagons
+// of the tile (the (TX+1)(TY+1) corners whose two triangles the tile's hexagons
 // use), keeps that column's 2 x 9 triangle interpolation coefficients in
-// registers and, for the TW^2 corners that are also tile points, the 18
+// registers and, for the TX TY corners that are also tile points, the 18
 // hexagon coefficients.  Per level: the two triangle values -> LDS (double
 // buffered by level parity, so one barrier per level), then the hexagon sums
 // -> out.  Per point and level this reads the three velocity components at
@@ -147,23 +147,25 @@ void run_staged(X& x, const P& pk) {
 // forms DivDamp1P / DivDamp2P do the pole cells.
 // ----------------------------------------------------------------------------
 struct ColGeom {
-  int TW;         // tile width in points; divides g - 2 (a power of two)
-  int ntx;        // tiles per region row
+  int TX, TY;     // tile width and height in points (divide g - 2, a power of two)
+  int nx, ny;     // tiles per region row and column
   int nchunk;     // level chunks
   int KC;         // levels per chunk
-  int ntiles;     // lall * nchunk * ntx * ntx
+  int ntiles;     // lall * nchunk * nx * ny
   int nblocks;    // + 2 * kall pole blocks
 };
-constexpr int kColThreads = 320;
+constexpr int kColThreads = 384;   // (64 + 1) x (4 + 1) corners
 
 inline ColGeom col_geom(const Dims& d, int nchunk) {
   ColGeom c;
   const int W = d.g - 2;
-  c.TW = (W < 16) ? W : 16;
-  c.ntx = W / c.TW;
+  c.TX = (W < 64) ? W : 64;
+  c.TY = (256 / c.TX < W) ? 256 / c.TX : W;
+  c.nx = W / c.TX;
+  c.ny = W / c.TY;
   c.nchunk = nchunk;
   c.KC = (d.kall + nchunk - 1) / nchunk;
-  c.ntiles = d.lall * nchunk * c.ntx * c.ntx;
+  c.ntiles = d.lall * nchunk * c.nx * c.ny;
   c.nblocks = c.ntiles + 2 * d.kall;
   return c;
 }
@@ -177,7 +179,7 @@ struct DivDampK {
   const real* v[3];     // OP 1: the momentum; OP 2: the first result (negated inside)
   const real* rw;       // OP 1: rhogw
   real* out[3];
-  static int lds_elems(const ColGeom& cg) { return 4 * (cg.TW + 1) * (cg.TW + 1); }
+  static int lds_elems(const ColGeom& cg) { return 4 * (cg.TX + 1) * (cg.TY + 1); }
   struct State {
     real ci[2][9];      // cintp of the corner column, [T][dd * 3 + vertex]
     real cd[18];        // cdiff of the tile point, [dd * 6 + m]
@@ -215,16 +217,16 @@ struct DivDampK {
       if (tid == 0) pole(blk - cg.ntiles);
       return 0;
     }
-    const int g = d.g, W1 = cg.TW + 1, tpr = cg.ntx * cg.ntx;
+    const int g = d.g, W1 = cg.TX + 1, tpr = cg.nx * cg.ny;
     const int tile = blk % tpr, rest = blk / tpr;
     const int chunk = rest % cg.nchunk, l = rest / cg.nchunk;
     const int k0 = chunk * cg.KC;
     const int k1 = (k0 + cg.KC < d.kall) ? k0 + cg.KC : d.kall;
     if (k0 >= k1) return 0;
-    const int i0 = 1 + (tile % cg.ntx) * cg.TW, j0 = 1 + (tile / cg.ntx) * cg.TW;
+    const int i0 = 1 + (tile % cg.nx) * cg.TX, j0 = 1 + (tile / cg.nx) * cg.TY;
     const int tx = tid % W1, ty = tid / W1;
     s.k = k0;
-    if (tid >= W1 * W1) return k1 - k0;
+    if (tid >= W1 * (cg.TY + 1)) return k1 - k0;
     s.corner = true;
     s.lidx = ty * W1 + tx;
     const int i = i0 - 1 + tx, j = j0 - 1 + ty;
@@ -274,8 +276,8 @@ struct DivDampK {
     if (!s.corner) return;
     const long g = d.g;
     if (active(s.k)) {
-      const int W1 = cg.TW + 1;
-      real* buf = lds + (step & 1) * 2 * W1 * W1;
+      const int P = (cg.TX + 1) * (cg.TY + 1);
+      real* buf = lds + (step & 1) * 2 * P;
       fetch(s, s.a);
       real up[2], dn[2];
       if (OP == 1) {
@@ -294,19 +296,19 @@ struct DivDampK {
           val = val + (up[T] - dn[T]) / real(3.0) * G.rdgz[s.k];
           s.dn[T] = up[T];
         }
-        buf[T * W1 * W1 + s.lidx] = val;
+        buf[T * P + s.lidx] = val;
       }
       s.have_dn = true;
     }
   }
   NH_HD void level_b(int step, State& s, const real* lds) const {
     if (s.outp) {
-      const int W1 = cg.TW + 1;
+      const int W1 = cg.TX + 1, P = W1 * (cg.TY + 1);
       real o[3] = {real(0.0), real(0.0), real(0.0)};
       const bool act = active(s.k);
       if (act) {
-        const real* b0 = lds + (step & 1) * 2 * W1 * W1;
-        const real* b1 = b0 + W1 * W1;
+        const real* b0 = lds + (step & 1) * 2 * P;
+        const real* b1 = b0 + P;
         const int x = s.lidx;
         // {TI(i,j), TJ(i,j), TI(i-1,j), TJ(i-1,j-1), TI(i-1,j-1), TJ(i,j-1)}
         real T6[6] = {b0[x], b1[x], b0[x - 1], b1[x - 1 - W1], b0[x - 1 - W1], b1[x - W1]};
@@ -351,7 +353,7 @@ struct DiffKhK {
   real* out[6];
   const real* rhog;
   // LDS per buffer: [field f][triangle T][component dd][corner]
-  static int lds_elems(const ColGeom& cg) { return 2 * 12 * (cg.TW + 1) * (cg.TW + 1); }
+  static int lds_elems(const ColGeom& cg) { return 2 * 12 * (cg.TX + 1) * (cg.TY + 1); }
   struct State {
     real ci[2][9];
     real cd[18];
@@ -370,16 +372,16 @@ struct DiffKhK {
       }
       return 0;
     }
-    const int g = d.g, W1 = cg.TW + 1, tpr = cg.ntx * cg.ntx;
+    const int g = d.g, W1 = cg.TX + 1, tpr = cg.nx * cg.ny;
     const int tile = blk % tpr, rest = blk / tpr;
     const int chunk = rest % cg.nchunk, l = rest / cg.nchunk;
     const int k0 = chunk * cg.KC;
     const int k1 = (k0 + cg.KC < d.kall) ? k0 + cg.KC : d.kall;
     if (k0 >= k1) return 0;
-    const int i0 = 1 + (tile % cg.ntx) * cg.TW, j0 = 1 + (tile / cg.ntx) * cg.TW;
+    const int i0 = 1 + (tile % cg.nx) * cg.TX, j0 = 1 + (tile / cg.nx) * cg.TY;
     const int tx = tid % W1, ty = tid / W1;
     s.k = k0;
-    if (tid >= W1 * W1) return k1 - k0;
+    if (tid >= W1 * (cg.TY + 1)) return k1 - k0;
     s.corner = true;
     s.lidx = ty * W1 + tx;
     const int i = i0 - 1 + tx, j = j0 - 1 + ty;
@@ -398,7 +400,7 @@ struct DiffKhK {
   NH_HD void level_a(int step, State& s, real* lds) const {
     if (!s.corner) return;
     const long g = d.g;
-    const int P = (cg.TW + 1) * (cg.TW + 1);
+    const int P = (cg.TX + 1) * (cg.TY + 1);
     real* buf = lds + (step & 1) * 12 * P;
     for (int f = 0; f < 2; ++f) {
       const real* sf = in[4 + f];
@@ -418,7 +420,7 @@ struct DiffKhK {
   NH_HD void level_b(int step, State& s, const real* lds) const {
     if (s.outp) {
       const long g = d.g;
-      const int W1 = cg.TW + 1, P = W1 * W1;
+      const int W1 = cg.TX + 1, P = W1 * (cg.TY + 1);
       const real* buf = lds + (step & 1) * 12 * P;
       const long a = s.a;
       const int x = s.lidx;
