@@ -319,7 +319,9 @@ int dyn_step_adj(X& x, const Context& ctx, const real* prog_in, const real* prog
       small_step(x, ctx, P, s, dts, rweight, ws_stage(ctx));
     }
     // adjoint of the stage
-    zero_fields(x, afield(ctx, A_DIAGB), A_TRIW - A_DIAGB, NF);
+    // the stage bars the small steps accumulate into; A_PSB is stored whole by
+    // ProgUpdateA, A_PSB2 .. A_DDVB are cleared inside each small step
+    zero_fields(x, afield(ctx, A_DIAGB), A_PSB - A_DIAGB, NF);
     x.run(FillZero64{adfield(ctx, AD_MCB)}, 3 * NF);
     real *psb[6], *psb2[6];
     for (int q = 0; q < 6; ++q) {
@@ -380,7 +382,9 @@ int dyn_step_adj_tape(X& x, const Context& ctx, const void* tape, const real* pr
       PrePostP f{d, ctx.c, ctx.G, P, progq, ctx.field(W_DIAG), ctx.field(W_ETH)};
       x.run_pts(f, d);
     }
-    zero_fields(x, afield(ctx, A_DIAGB), A_TRIW - A_DIAGB, NF);
+    // the stage bars the small steps accumulate into; A_PSB is stored whole by
+    // ProgUpdateA, A_PSB2 .. A_DDVB are cleared inside each small step
+    zero_fields(x, afield(ctx, A_DIAGB), A_PSB - A_DIAGB, NF);
     x.run(FillZero64{adfield(ctx, AD_MCB)}, 3 * NF);
     real *psb[6], *psb2[6];
     for (int q = 0; q < 6; ++q) {
