@@ -83,11 +83,18 @@ struct HaloScatterA {
     long ks_d, ks_s;
     const long bd = h.colbase(dst[hh], &ks_d) + k * ks_d;
     const long bs = h.colbase(src[hh], &ks_s) + k * ks_s;
-    for (int q = 0; q < nf; ++q) {
-      const real v = f[q][bd];
-      f[q][bd] = real(0.0);
-      if (v != real(0.0)) aadd(&f[q][bs], v);
-    }
+    // every destination bar read before any store or add (one memory round
+    // trip rather than one per field)
+    real v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      if (q < nf) v[q] = f[q][bd];
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      if (q < nf) {
+        f[q][bd] = real(0.0);
+        if (v[q] != real(0.0)) aadd(&f[q][bs], v[q]);
+      }
   }
 };
 
