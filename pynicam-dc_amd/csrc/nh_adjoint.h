@@ -492,14 +492,23 @@ struct AdvMomA {
       const real wc = G.cfact[kk] * wd[b + ks] + G.dfact[kk] * wd[b];
       for (int q = 0; q < 3; ++q) vo[q] = diag[(I_VX + q) * NF + b] + wc * xx[q] / R;
     };
+    // w bars of this column at levels k-1 .. k+2, summed before the adds
+    real wcol[4] = {real(0.0), real(0.0), real(0.0), real(0.0)};
     auto vv3_t = [&](long b, int kk, const real* xx, const real* vb) {
       real wcb = real(0.0);
       for (int q = 0; q < 3; ++q) {
         aadd(&diag_b[(I_VX + q) * NF + b], vb[q]);
         wcb += vb[q] * xx[q] / R;
       }
-      aadd(&wdb[b + ks], wcb * G.cfact[kk]);
-      aadd(&wdb[b], wcb * G.dfact[kk]);
+      if (b == a - ks || b == a || b == a + ks) {
+        // the column's own levels kk, kk + 1 (kk = k - 1, k, k + 1)
+        const int jw = kk - k + 1;
+        wcol[jw + 1] += wcb * G.cfact[kk];
+        wcol[jw] += wcb * G.dfact[kk];
+      } else {
+        aadd(&wdb[b + ks], wcb * G.cfact[kk]);
+        aadd(&wdb[b], wcb * G.dfact[kk]);
+      }
     };
     real x0[3];
     pos(0, x0);
@@ -571,6 +580,10 @@ struct AdvMomA {
       }
       for (int dd = 0; dd < 3; ++dd) aadd(&prog_b[(1 + dd) * NF + bm], fb[dd]);
       vv3_t(bm, k, xm, vvb);
+    }
+    for (int jw = 0; jw < 4; ++jw) {
+      const int kw = k - 1 + jw;
+      if (kw >= 0 && kw <= d.kall - 1 && wcol[jw] != real(0.0)) aadd(&wdb[a + (long)(jw - 1) * ks], wcol[jw]);
     }
   }
 };
