@@ -98,12 +98,41 @@ struct HaloScatterA {
   }
 };
 
-// zero an FP64 range
-struct FillZero64 {
-  static constexpr const char* kName = "fill_zero";
-  double* p;
-  NH_HD void operator()(long t) const { p[t] = 0.0; }
+// zero a range of 4-byte words: thread t < nvec clears the 16-byte chunk t
+// (from the first 16-byte boundary, one dwordx4 store), the next threads the
+// head words before that boundary and the tail words after the last chunk
+struct alignas(16) Zero4 {
+  unsigned v[4];
 };
+struct ZeroWords {
+  static constexpr const char* kName = "fill_zero";
+  unsigned* w;   // first word
+  long head;     // words before the first 16-byte boundary (0..3)
+  long nvec;     // 16-byte chunks
+  long nw;       // words in all
+  NH_HD void operator()(long t) const {
+    if (t < nvec) {
+      const Zero4 z = {{0u, 0u, 0u, 0u}};
+      reinterpret_cast<Zero4*>(w + head)[t] = z;
+      return;
+    }
+    t -= nvec;
+    const long i = t < head ? t : head + 4 * nvec + (t - head);
+    if (i < nw) w[i] = 0u;
+  }
+  // threads to launch
+  long threads() const { return nvec + (nw - 4 * nvec); }
+};
+inline ZeroWords zero_words(void* p, long nbytes) {
+  ZeroWords z;
+  z.w = static_cast<unsigned*>(p);
+  z.nw = nbytes / 4;
+  const long mis = (long)((reinterpret_cast<unsigned long>(p) >> 2) & 3);
+  z.head = mis ? 4 - mis : 0;
+  if (z.head > z.nw) z.head = z.nw;
+  z.nvec = (z.nw - z.head) / 4;
+  return z;
+}
 
 // y += x over a flat range
 struct AxpyP {

@@ -48,8 +48,13 @@ inline real* afield(const Context& ctx, int f) { return ctx.aws + (long)f * ctx.
 inline double* adfield(const Context& ctx, int f) { return ctx.adws + (long)f * ctx.d.NF; }
 
 template <class X>
+void zero_bytes(X& x, void* p, long nbytes) {
+  const ZeroWords z = zero_words(p, nbytes);
+  x.run(z, z.threads());
+}
+template <class X>
 void zero_fields(X& x, real* p, long nfields, long NF) {
-  x.run(FillZero{p}, nfields * NF);
+  zero_bytes(x, p, nfields * NF * (long)sizeof(real));
 }
 
 // transpose of halo(): destination bars onto their sources, up to 16 fields
@@ -113,7 +118,10 @@ void small_step_adj(X& x, const Context& ctx, const real* prog, real* const ck[6
   real* web[3] = {psb[0], psb[4], psb[5]};
   real* dvb[3] = {psb[1], psb[2], psb[3]};
   halo_adj(x, ctx, web, 3);
-  zero_fields(x, hvb[0], 3, NF);
+  // the divergence bars and the damping-force bars (adjacent fields) start at
+  // zero; nothing before Path1A touches the damping-force bars
+  static_assert(A_DDVB == A_HVB + 3, "bars cleared together");
+  zero_fields(x, hvb[0], 6, NF);
   {
     ViMainPostA f;
     f.d = d; f.c = ctx.c; f.G = ctx.G;
@@ -156,7 +164,6 @@ void small_step_adj(X& x, const Context& ctx, const real* prog, real* const ck[6
   }
   halo_adj(x, ctx, dvb, 3);
   real* ddvb[3] = {afield(ctx, A_DDVB), afield(ctx, A_DDVB + 1), afield(ctx, A_DDVB + 2)};
-  zero_fields(x, ddvb[0], 3, NF);
   {
     Path1A f;
     f.d = d; f.c = ctx.c; f.G = ctx.G;
@@ -322,7 +329,7 @@ int dyn_step_adj(X& x, const Context& ctx, const real* prog_in, const real* prog
     // the stage bars the small steps accumulate into; A_PSB is stored whole by
     // ProgUpdateA, A_PSB2 .. A_DDVB are cleared inside each small step
     zero_fields(x, afield(ctx, A_DIAGB), A_PSB - A_DIAGB, NF);
-    x.run(FillZero64{adfield(ctx, AD_MCB)}, 3 * NF);
+    zero_bytes(x, adfield(ctx, AD_MCB), 3 * NF * (long)sizeof(double));
     real *psb[6], *psb2[6];
     for (int q = 0; q < 6; ++q) {
       psb[q] = afield(ctx, A_PSB + q);
@@ -385,7 +392,7 @@ int dyn_step_adj_tape(X& x, const Context& ctx, const void* tape, const real* pr
     // the stage bars the small steps accumulate into; A_PSB is stored whole by
     // ProgUpdateA, A_PSB2 .. A_DDVB are cleared inside each small step
     zero_fields(x, afield(ctx, A_DIAGB), A_PSB - A_DIAGB, NF);
-    x.run(FillZero64{adfield(ctx, AD_MCB)}, 3 * NF);
+    zero_bytes(x, adfield(ctx, AD_MCB), 3 * NF * (long)sizeof(double));
     real *psb[6], *psb2[6];
     for (int q = 0; q < 6; ++q) {
       psb[q] = afield(ctx, A_PSB + q);
