@@ -1024,20 +1024,20 @@ struct ViMainVA {
       if (NK > 0 && n > 0) gm_next = gb[NK > 0 ? n : 0];
       const int k = kmin + 1 + n;
       const long am = a - ks;
-      const double x_dn = (n >= 1) ? (double)x[am] : 0.0;
-      Mc_b[a] += -y * xk;
-      if (n >= 1) Ml_b[a] += -y * x_dn;
-      if (n <= N - 2) Mu_b[a] += -y * x_up;
+      const double x_dn = (n >= 1) ? (double)ldc(&x[am]) : 0.0;
+      radd(&Mc_b[a], -y * xk);
+      if (n >= 1) radd(&Ml_b[a], -y * x_dn);
+      if (n <= N - 2) radd(&Mu_b[a], -y * x_up);
       const double s = y * CVovRt2;
       const double Ab = s, Bb = -s * dt * V.rdgzh[k], Cb = -s * dt * C.GRAV;
       const double grk = Cb * V.afact[k] * dt, grm = Cb * V.bfact[k] * dt;
-      ps_b[I_RHOGW][a] += (real)(Ab * C.ALPHA);
-      gT_b[4 * NF + a] += (real)(Ab * dt);
-      ps_b[I_RHOGE][a] += (real)(Bb * RovCV + cE);
-      hv_b[1][a] += (real)(Bb * dt + cH1);
-      ps_b[I_RHOG][a] += (real)(Cb * V.afact[k] + cG);
-      gT_b[a] += (real)(grk + cT0);
-      hv_b[0][a] += (real)(-grk + cH0);
+      radd(&ps_b[I_RHOGW][a], (real)(Ab * C.ALPHA));
+      radd(&gT_b[4 * NF + a], (real)(Ab * dt));
+      radd(&ps_b[I_RHOGE][a], (real)(Bb * RovCV + cE));
+      radd(&hv_b[1][a], (real)(Bb * dt + cH1));
+      radd(&ps_b[I_RHOG][a], (real)(Cb * V.afact[k] + cG));
+      radd(&gT_b[a], (real)(grk + cT0));
+      radd(&hv_b[0][a], (real)(-grk + cH0));
       cE = -Bb * RovCV;
       cH1 = -Bb * dt;
       cG = Cb * V.bfact[k];
@@ -1048,12 +1048,17 @@ struct ViMainVA {
       a = am;
     }
     // level kmin: the bars the lowest source sent down
-    ps_b[I_RHOGE][a] += (real)cE;
-    hv_b[1][a] += (real)cH1;
-    ps_b[I_RHOG][a] += (real)cG;
-    gT_b[a] += (real)cT0;
-    hv_b[0][a] += (real)cH0;
+    radd(&ps_b[I_RHOGE][a], (real)cE);
+    radd(&hv_b[1][a], (real)cH1);
+    radd(&ps_b[I_RHOG][a], (real)cG);
+    radd(&gT_b[a], (real)cT0);
+    radd(&hv_b[0][a], (real)cH0);
   }
+  // *p += v with the read through ldc: every bar element of a column is read
+  // and written once, by that column's thread, so the reads of a level need
+  // not wait behind the stores of the level above
+  template <class T>
+  NH_HD static void radd(T* p, T v) { *p = ldc(p) + v; }
 };
 
 // ----------------------------------------------------------------------------
