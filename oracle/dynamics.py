@@ -392,6 +392,23 @@ def rhogkin(ctx, rhog, rvx, rvy, rvz, rw, tag, kax):
     return out
 
 
+def hdiff_tendency(vt, vtp, rhog, rhog_p, rhog_h, rhog_h_p, kh, kh_p, kh_h, kh_h_p):
+    """Hyperdiffusion tendency from the iterated Laplacians vt (6 components) with the
+    Laplacian-1 filter off (its terms are the literal 0.0): Numf._hdiff_tendency_host
+    (mod_numfilter.py:1519-1578)."""
+    out = []
+    for v_, rg, rgh, k, kh_ in ((vt, rhog, rhog_h, kh, kh_h), (vtp, rhog_p, rhog_h_p, kh_p, kh_h_p)):
+        t = np.zeros(rg.shape + (6,))
+        t[..., I_RHOGVX] = -(v_[0] * k + 0.0) * rg
+        t[..., I_RHOGVY] = -(v_[1] * k + 0.0) * rg
+        t[..., I_RHOGVZ] = -(v_[2] * k + 0.0) * rg
+        t[..., I_RHOGW] = -(v_[3] * kh_ + 0.0) * rgh
+        t[..., I_RHOGE] = -(v_[4] + 0.0)
+        t[..., I_RHOG] = -(v_[5] + 0.0)
+        out.append(t)
+    return out[0], out[1]
+
+
 def thomas(S, Mc, Mu, Ml):
     """Tridiagonal solve along axis 0 (kernels/virhowsolver.py:42-64)."""
     N = len(S)
@@ -606,27 +623,18 @@ class OracleDynamics:
             ctx.halo(V, Vp)
             vt = [V[..., i] for i in range(6)]
             vtp = [Vp[..., i] for i in range(6)]
-        tend, tendp = [], []
+        rgh = []
         for reg in (True, False):
             kax = 2 if reg else 1
-            tag = "" if reg else "_pl"
-            v_ = vt if reg else vtp
             rg = rhog if reg else rhog_p
-            kh = khf if reg else khf_p
-            C2W = ctx.m["C2Wfact" + tag]
-            rgh = np.zeros_like(rg)
+            C2W = ctx.m["C2Wfact" + ("" if reg else "_pl")]
+            r = np.zeros_like(rg)
             K, Km = _ks(kax, kmin, kmax + 2), _ks(kax, kmin - 1, kmax + 1)
-            rgh[K] = C2W[K + (..., 0)] * rg[K] + C2W[K + (..., 1)] * rg[Km]
-            rgh[_ks(kax, kmin - 1, kmin)] = 0.0
-            t = np.zeros(rg.shape + (6,))
-            t[..., I_RHOGVX] = -(v_[0] * kh + 0.0) * rg
-            t[..., I_RHOGVY] = -(v_[1] * kh + 0.0) * rg
-            t[..., I_RHOGVZ] = -(v_[2] * kh + 0.0) * rg
-            t[..., I_RHOGW] = -(v_[3] * kh + 0.0) * rgh
-            t[..., I_RHOGE] = -(v_[4] + 0.0)
-            t[..., I_RHOG] = -(v_[5] + 0.0)
-            (tend if reg else tendp).append(t)
-        T, Tp = tend[0], tendp[0]
+            r[K] = C2W[K + (..., 0)] * rg[K] + C2W[K + (..., 1)] * rg[Km]
+            r[_ks(kax, kmin - 1, kmin)] = 0.0
+            rgh.append(r)
+        # the linear filter: Kh at half levels is Kh itself (mod_numfilter.py:1208-1211)
+        T, Tp = hdiff_tendency(vt, vtp, rhog, rhog_p, rgh[0], rgh[1], khf, khf_p, khf, khf_p)
         horizontalize_inplace(ctx, T[..., 1], T[..., 2], T[..., 3], Tp[..., 1], Tp[..., 2], Tp[..., 3])
         return T, Tp
 
@@ -782,48 +790,56 @@ class OracleDynamics:
             PSp[:, kmax + 1, :, I_RHOGE] = PSp[:, kmax, :, I_RHOGE]
             ddv, ddvp = self.divdamp(PS[..., I_RHOGVX], PS[..., I_RHOGVY], PS[..., I_RHOGVZ], PS[..., I_RHOGW],
                                      PSp[..., I_RHOGVX], PSp[..., I_RHOGVY], PSp[..., I_RHOGVZ], PSp[..., I_RHOGW])
-            # path1: horizontal explicit momentum (kernels/vipath1.py:68-160)
-            Pg, _, Pg_p, _ = pres_gradient(ctx, preg, preg_p, False)
-            dvh, dvh_p = [], []
-            for reg in (True, False):
-                kax = 2 if reg else 1
-                PS_ = PS if reg else PSp
-                gT_ = gTr if reg else gTp
-                Pg_ = Pg if reg else Pg_p
-                dd_ = ddv if reg else ddvp
-                rg = (PR0 if reg else PR0p)[..., I_RHOG]
-                comps = []
-                for d, iv in enumerate((I_RHOGVX, I_RHOGVY, I_RHOGVZ)):
-                    drv = gT_[..., iv] - Pg_[..., d] + dd_[d] + 0.0
-                    comps.append(PS_[..., iv] + drv * dt)
-                for a in comps:
-                    kt = _ks(kax, kmax + 1, kmax + 2)
-                    kb = _ks(kax, kmin - 1, kmin)
-                    top = 1.0 * (a[_ks(kax, kmax, kmax + 1)] / rg[_ks(kax, kmax, kmax + 1)]) * rg[kt]
-                    btm = -1.0 * (a[_ks(kax, kmin, kmin + 1)] / rg[_ks(kax, kmin, kmin + 1)]) * rg[kb]
-                    a[kt] = top
-                    a[kb] = btm
-                (dvh if reg else dvh_p).extend(comps)
-            drhogw, drhogw_p = gTr[..., I_RHOGW] + 0.0 * alpha, gTp[..., I_RHOGW] + 0.0 * alpha
-            DV = np.stack(dvh, axis=-1)
-            DVp = np.stack(dvh_p, axis=-1)
+            DV, DVp, drhogw, drhogw_p = self._vi_path1(preg, preg_p, gTr, gTp, ddv, ddvp, PS, PSp, PR0, PR0p, dt)
             ctx.halo(DV, DVp)
             we = self._vi_main(DV, DVp, PS, PSp, preg, preg_p, PR0, PR0p, eth, eth_p,
                                gTr, gTp, drhogw, drhogw_p, grhogetot0, grhogetot0_p, Mat, dt)
             WE = np.stack(we[0], axis=-1)
             WEp = np.stack(we[1], axis=-1)
             ctx.halo(WE, WEp)
-            PS[..., I_RHOGVX], PS[..., I_RHOGVY], PS[..., I_RHOGVZ] = DV[..., 0], DV[..., 1], DV[..., 2]
-            PS[..., I_RHOG], PS[..., I_RHOGW], PS[..., I_RHOGE] = WE[..., 0], WE[..., 1], WE[..., 2]
-            PSp[..., I_RHOGVX], PSp[..., I_RHOGVY], PSp[..., I_RHOGVZ] = DVp[..., 0], DVp[..., 1], DVp[..., 2]
-            PSp[..., I_RHOG], PSp[..., I_RHOGW], PSp[..., I_RHOGE] = WEp[..., 0], WEp[..., 1], WEp[..., 2]
-            PROG_mean += PS[..., 0:5] * rw_itr
-            PROG_mean_p += PSp[..., 0:5] * rw_itr
+            self._vi_path2(PS, DV, WE, PROG_mean, rw_itr)
+            self._vi_path2(PSp, DVp, WEp, PROG_mean_p, rw_itr)
         P += PS
         Pp += PSp
         horizontalize_inplace(ctx, P[..., 1], P[..., 2], P[..., 3], Pp[..., 1], Pp[..., 2], Pp[..., 3])
         ctx.halo(PROG_mean, PROG_mean_p)
         return PROG_mean, PROG_mean_p
+
+    def _vi_path1(self, preg, preg_p, gTr, gTp, ddv, ddvp, PS, PSp, PR0, PR0p, dt):
+        """Horizontal explicit momentum update of one small step with its boundary rows
+        (kernels/vipath1.py:68-160; the vertical and 2-D divergence damping are off)."""
+        ctx = self.ctx
+        kmin, kmax, alpha = ctx.kmin, ctx.kmax, ctx.c.ALPHA
+        Pg, _, Pg_p, _ = pres_gradient(ctx, preg, preg_p, False)
+        dvh, dvh_p = [], []
+        for reg in (True, False):
+            kax = 2 if reg else 1
+            PS_ = PS if reg else PSp
+            gT_ = gTr if reg else gTp
+            Pg_ = Pg if reg else Pg_p
+            dd_ = ddv if reg else ddvp
+            rg = (PR0 if reg else PR0p)[..., I_RHOG]
+            comps = []
+            for d, iv in enumerate((I_RHOGVX, I_RHOGVY, I_RHOGVZ)):
+                drv = gT_[..., iv] - Pg_[..., d] + dd_[d] + 0.0
+                comps.append(PS_[..., iv] + drv * dt)
+            for a in comps:
+                kt = _ks(kax, kmax + 1, kmax + 2)
+                kb = _ks(kax, kmin - 1, kmin)
+                top = 1.0 * (a[_ks(kax, kmax, kmax + 1)] / rg[_ks(kax, kmax, kmax + 1)]) * rg[kt]
+                btm = -1.0 * (a[_ks(kax, kmin, kmin + 1)] / rg[_ks(kax, kmin, kmin + 1)]) * rg[kb]
+                a[kt] = top
+                a[kb] = btm
+            (dvh if reg else dvh_p).extend(comps)
+        drhogw, drhogw_p = gTr[..., I_RHOGW] + 0.0 * alpha, gTp[..., I_RHOGW] + 0.0 * alpha
+        return np.stack(dvh, axis=-1), np.stack(dvh_p, axis=-1), drhogw, drhogw_p
+
+    @staticmethod
+    def _vi_path2(PS, DV, WE, PROG_mean, rw_itr):
+        """Split-state write-back and running mean of one small step (kernels/vipath2.py:60-87)."""
+        PS[..., I_RHOGVX], PS[..., I_RHOGVY], PS[..., I_RHOGVZ] = DV[..., 0], DV[..., 1], DV[..., 2]
+        PS[..., I_RHOG], PS[..., I_RHOGW], PS[..., I_RHOGE] = WE[..., 0], WE[..., 1], WE[..., 2]
+        PROG_mean += PS[..., 0:5] * rw_itr
 
     def _rhow_matrix(self, eth_h, g_tilde, dt, tag, kax):
         """Tridiagonal coefficients (kernels/vimatrix.py:67-136)."""

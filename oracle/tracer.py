@@ -217,11 +217,16 @@ def horizontal_flux(ctx, rho, rvx, rvy, rvz, rho_pl, rvx_pl, rvy_pl, rvz_pl, dt)
 
 def horizontal_remap(ctx, q, q_pl, cmask, cmask_pl, xc, xc_pl):
     """Upwind face values from the centre values and gradients (mod_src_tracer.py:1568-1843)."""
-    geo = ctx.geo
-    t = geo.topo
-    g = t.gall_1d
     gq, gq_pl = gradient(ctx, q, q_pl)
     ctx.halo(gq, gq_pl)
+    return remap_faces(ctx, q, q_pl, gq, gq_pl, cmask, cmask_pl, xc, xc_pl)
+
+
+def remap_faces(ctx, q, q_pl, gq, gq_pl, cmask, cmask_pl, xc, xc_pl):
+    """Face values from the centre values and their (halo-filled) gradients: the
+    reconstruction part of horizontal_remap (mod_src_tracer.py:1640-1843)."""
+    geo = ctx.geo
+    g = geo.topo.gall_1d
     x = geo.GRD_x[:, :, 0][:, :, None]                    # (g, g, 1, l, 3)
     x_pl = geo.GRD_x_pl[:, 0]                              # (6, 2, 3)
     q_a = np.zeros(q.shape + (6,))

@@ -1,0 +1,89 @@
+"""Write the long-run oracle fixtures for the dynamics step.
+
+Runs the numpy FP64 oracle (``oracle/dynamics.py``) from the seeded
+benchmark-family state for a full simulated window and keeps a compact
+fingerprint of the end state, so that the device path can be compared with the
+oracle over the same number of steps on a machine where the oracle would be
+too slow to rerun:
+
+  gl05 z40, dt 1200 s, 72 steps (1 simulated day; the CPU-runnable case)
+  gl06 z40, dt 600 s, 72 steps (12 simulated hours)
+
+Per prognostic (rhog, rhogvx, rhogvy, rhogvz, rhogw, rhoge) the fixture holds
+the values at 4096 seeded interior points on physical levels, the pole-centre
+columns, and the per-level RMS over the interior; plus the oracle's global
+budgets (mass and energies) at the start and the end.  The state needs no
+storing: ``setup.build_case`` regenerates it bit for bit from the seed.
+
+Usage (CPU, about 5 min for gl05 and 20 min for gl06):
+    python tests/golden/make_step_fixtures.py [gl05|gl06 ...]
+"""
+from __future__ import annotations
+
+import importlib
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+CASES = {
+    "gl05": dict(glevel=5, vlayer=40, dtl=1200.0, seed=0, nsteps=72),
+    "gl06": dict(glevel=6, vlayer=40, dtl=600.0, seed=0, nsteps=72),
+}
+NSAMPLE = 4096
+
+
+def sample_points(topo, n, seed=1234):
+    """Seeded interior points (i, j, k, l) on physical levels."""
+    rng = np.random.default_rng(seed)
+    g = topo.gall_1d
+    i = rng.integers(1, g - 1, n)
+    j = rng.integers(1, g - 1, n)
+    k = rng.integers(topo.kmin, topo.kmax + 1, n)
+    l = rng.integers(0, topo.lall, n)
+    return np.stack([i, j, k, l]).astype(np.int32)
+
+
+def fingerprint(topo, PROG, PROG_pl, idx):
+    g = topo.gall_1d
+    ks = slice(topo.kmin, topo.kmax + 1)
+    inner = PROG[1:g - 1, 1:g - 1, ks]
+    return {
+        "sample": np.stack([PROG[idx[0], idx[1], idx[2], idx[3], v] for v in range(6)]),
+        "pole": np.ascontiguousarray(PROG_pl[0, ks, :, :6]),
+        "level_rms": np.sqrt(np.mean(inner[..., :6] ** 2, axis=(0, 1, 3))),
+    }
+
+
+def run(name):
+    setup = importlib.import_module("pynicam-dc_amd.setup")
+    from oracle import budget as ob
+    from oracle import dynamics as od
+    p = CASES[name]
+    case = setup.build_case(glevel=p["glevel"], vlayer=p["vlayer"], dtl=p["dtl"], seed=p["seed"])
+    octx = od.OracleContext(case.geo, case.const, None)
+    dyn = od.OracleDynamics(octx, case.state)
+    idx = sample_points(case.topo, NSAMPLE)
+    b0 = ob.store_var(octx, dyn.PROG, dyn.PROGq, case.const.LHV)
+    t0 = time.time()
+    for n in range(p["nsteps"]):
+        dyn.step()
+        print(f"{name}: step {n + 1}/{p['nsteps']} ({time.time() - t0:.0f} s)", flush=True)
+    b1 = ob.store_var(octx, dyn.PROG, dyn.PROGq, case.const.LHV)
+    fp = fingerprint(case.topo, dyn.PROG, dyn.PROG_pl, idx)
+    keys = sorted(b0)
+    np.savez_compressed(
+        os.path.join(HERE, f"oracle_{name}_z40_{p['nsteps']}steps.npz"),
+        glevel=p["glevel"], vlayer=p["vlayer"], dtl=p["dtl"], seed=p["seed"], nsteps=p["nsteps"],
+        idx=idx, budget_keys=np.array(keys), budget0=np.array([b0[k] for k in keys]),
+        budget1=np.array([b1[k] for k in keys]), **fp)
+
+
+if __name__ == "__main__":
+    for name in (sys.argv[1:] or list(CASES)):
+        run(name)
