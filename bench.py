@@ -14,9 +14,12 @@ is its algorithmic bytes per launch divided by its average launch time,
 measured with events on the launch stream during the timed steps; its HBM
 traffic per launch comes from two one-step rocprofv3 counter passes
 (FETCH_SIZE, WRITE_SIZE) run as child processes first (--no-traffic skips them).  The CPU
-baseline times the numpy FP64 oracle on one gl05 z40 step on the host and
-scales it to gl07 by the cell ratio (the step's work is proportional to the
-number of cells).
+baseline times the numpy FP64 oracle on the host: one warm-up and three
+steady gl06 z40 steps, scaled to gl07 by the cell ratio (the step's work is
+proportional to the number of cells).  The accuracy leg runs both forms for
+one simulated day from the same state and reports the mixed form's
+surface-pressure difference from the FP64 form and both forms' mass and
+energy drift.
 """
 from __future__ import annotations
 
@@ -159,27 +162,125 @@ def hbm_traffic(glevel: int, precision: str, timeout: float = 300.0):
 RESOURCES: dict = {}
 
 
-def cpu_baseline(seconds_cap: float = 60.0) -> dict:
-    """One gl05 z40 large step of the numpy oracle on the host, scaled to gl07 SDPD."""
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(glevel: int = 6, steady: int = 3) -> dict:
+    """The numpy FP64 oracle on one host core: one warm-up large step, then
+    ``steady`` timed steps at gl06 z40 (the step's work is proportional to the
+    number of cells, so the gl07 rate is the gl06 rate over the cell ratio)."""
     from oracle.dynamics import OracleContext, OracleDynamics
-    case = pkg("setup").build_case(glevel=5, vlayer=40, dtl=1200.0, seed=1)
+    dts = {5: 1200.0, 6: 600.0}
+    case = pkg("setup").build_case(glevel=glevel, vlayer=40, dtl=dts[glevel], seed=1)
     dyn = OracleDynamics(OracleContext(case.geo, case.const, None), case.state)
-    t0 = time.perf_counter()
     dyn.step()
-    t_gl05 = time.perf_counter() - t0
-    cells5 = 10 * 4 ** 5 + 2
+    t0 = time.perf_counter()
+    for _ in range(steady):
+        dyn.step()
+    t_step = (time.perf_counter() - t0) / steady
+    cells = 10 * 4 ** glevel + 2
     cells7 = 10 * 4 ** 7 + 2
-    t_gl07 = t_gl05 * cells7 / cells5
-    threads = os.environ.get("OMP_NUM_THREADS", "unset")
+    t_gl07 = t_step * cells7 / cells
+    env = {k: os.environ.get(k, "unset") for k in ("OMP_NUM_THREADS", "MKL_NUM_THREADS", "OPENBLAS_NUM_THREADS")}
     return {
         "value": 300.0 / t_gl07,
         "unit": "SDPD",
         "cores": 1,
         "kind": "port",
-        "sample": (f"numpy FP64 oracle, one gl05 z40 large step (dt 1200 s) = {t_gl05:.2f} s on 1 host core "
-                   f"(OMP_NUM_THREADS={threads}, os.cpu_count()={os.cpu_count()}), scaled by the cell ratio "
-                   f"{cells7}/{cells5} to a gl07 step of {t_gl07:.1f} s at dt 300 s"),
+        "sample": (f"numpy FP64 oracle, gl{glevel:02d} z40: one warm-up large step, then {steady} steady steps "
+                   f"of {t_step:.2f} s each on one host core ({_cpu_model()}, os.cpu_count()={os.cpu_count()}, "
+                   f"{', '.join(f'{k}={v}' for k, v in env.items())}; numpy's elementwise kernels run on one "
+                   f"thread); gl07 step = {t_gl07:.1f} s by the cell ratio {cells7}/{cells}, SDPD at dt 300 s"),
+        "gl_measured": glevel,
+        "s_per_step_measured": t_step,
     }
+
+
+def sizing_leg(hipdyn, glevel: int, steps: int = 10, warmup: int = 2) -> dict:
+    """The mixed form at a larger grid: ms per step and the device memory it holds."""
+    import torch
+    dt = 300.0 * 2.0 ** (7 - glevel)
+    print(f"bench: sizing: setting up gl{glevel:02d} z40", file=sys.stderr, flush=True)
+    case = pkg("setup").build_case(glevel=glevel, vlayer=40, dtl=dt, seed=1)
+    torch.cuda.empty_cache()
+    free0, total = torch.cuda.mem_get_info()
+    ctx = hipdyn.HipContext(case.geo, case.const, device="cuda:0", precision="mixed")
+    try:
+        dyn = hipdyn.HipDynamics(ctx, case.state)
+        for _ in range(warmup):
+            dyn.dynamics_step()
+        torch.cuda.synchronize()
+        free1, _ = torch.cuda.mem_get_info()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            dyn.dynamics_step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1000.0 / steps
+        b = dyn.budget()
+        ok = all(map(lambda v: v == v, b.values()))
+    finally:
+        ctx.close()
+    res = {"workload": f"gl{glevel:02d} z40 ({10 * 4 ** glevel + 2} cells), dt {dt:g} s, mixed form",
+           "ms_per_step": ms, "sdpd": dt / (ms / 1000.0), "device_mem_GB": (free0 - free1) / 1e9,
+           "device_total_GB": total / 1e9, "budget_finite": ok}
+    print("bench: sizing " + json.dumps(res), file=sys.stderr, flush=True)
+    return res
+
+
+def accuracy_leg(hipdyn, case, nsteps: int) -> dict:
+    """Both forms over ``nsteps`` steps from the same state: surface-pressure
+    difference of the mixed form from the FP64 form (BNDCND_pre_sfc,
+    mod_bndcnd.py:521, over every regional interior column and the poles), the
+    largest and RMS difference of each prognostic relative to its largest
+    value, and each form's drift of dry mass and total energy (compensated
+    global sums, Embudget._store_var, mod_embudget.py:82-139)."""
+    import numpy as np
+    t = case.topo
+    g = t.gall_1d
+    out, ps, st = {}, {}, {}
+    for f in ("float64", "mixed"):
+        ctx = hipdyn.HipContext(case.geo, case.const, device="cuda:0", precision=f)
+        try:
+            dyn = hipdyn.HipDynamics(ctx, case.state)
+            b0 = dyn.budget()
+            for n in range(nsteps):
+                dyn.dynamics_step()
+                if (n + 1) % 96 == 0:
+                    ctx.sync()
+                    print(f"bench: accuracy {f}: step {n + 1}/{nsteps}", file=sys.stderr, flush=True)
+            b1 = dyn.budget()
+            ps[f] = dyn.sfc_pressure().cpu().numpy()
+            st[f] = dyn.state()
+        finally:
+            ctx.close()
+        out[FORMS[f][0]] = {"dry_mass_drift_rel": (b1["dry"] - b0["dry"]) / b0["dry"],
+                            "total_energy_drift_rel": (b1["tot"] - b0["tot"]) / b0["tot"],
+                            "budget_start": b0, "budget_end": b1}
+    nreg = t.lall * g * g
+    inner = np.zeros((t.lall, g, g), dtype=bool)
+    inner[:, 1:g - 1, 1:g - 1] = True
+    cols = np.concatenate([inner.reshape(-1), np.array([True] + [False] * 5 + [True] + [False] * 5)])
+    assert cols.size == nreg + 12
+    d = (ps["mixed"] - ps["float64"])[cols]
+    out["ps_rms_diff_hPa"] = float(np.sqrt(np.mean(d * d)) / 100.0)
+    out["ps_max_abs_diff_hPa"] = float(np.abs(d).max() / 100.0)
+    out["ps_fp64_range_hPa"] = [float(ps["float64"][cols].min() / 100.0), float(ps["float64"][cols].max() / 100.0)]
+    a = st["float64"]["PROG"][1:g - 1, 1:g - 1, t.kmin:t.kmax + 1].reshape(-1, 6)
+    b = st["mixed"]["PROG"][1:g - 1, 1:g - 1, t.kmin:t.kmax + 1].reshape(-1, 6).astype(np.float64)
+    scale = np.abs(a).max(axis=0)
+    names = ("rhog", "rhogvx", "rhogvy", "rhogvz", "rhogw", "rhoge")
+    out["prog_max_rel_diff"] = dict(zip(names, [float(e) for e in np.abs(a - b).max(axis=0) / scale]))
+    out["prog_rms_rel_diff"] = dict(zip(names, [float(e) for e in np.sqrt(np.mean((a - b) ** 2, axis=0)) / scale]))
+    out["workload"] = (f"gl{t.glevel:02d} z{t.kmax - t.kmin + 1}, dt {case.const.DTL:g} s, {nsteps} steps "
+                       f"({nsteps * case.const.DTL / 86400.0:g} simulated days), mixed form against the FP64 form")
+    return out
 
 
 FORMS = {"float64": ("fp64", "f64", 8), "mixed": ("mixed", "f32+f64-solve", 4)}
@@ -348,6 +449,12 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint window at gl06 z40")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the adjoint window at gl06 z40 (both forms) and print its JSON")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the one-simulated-day run of both forms (surface-pressure difference, drift)")
+    ap.add_argument("--accuracy-steps", type=int, default=None,
+                    help="steps of the accuracy leg (default: one simulated day)")
+    ap.add_argument("--sizing-glevel", type=int, default=None,
+                    help="also time the mixed form at this glevel (e.g. 8) and report its device memory")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the run, copy 512 MiB device-to-device once (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -444,6 +551,12 @@ def main():
         err = np.abs(a - b).reshape(-1, 6).max(axis=0) / scale
         out["mixed_vs_fp64_max_rel_err"] = dict(zip(("rhog", "rhogvx", "rhogvy", "rhogvz", "rhogw", "rhoge"),
                                                     [float(e) for e in err]))
+    if "fp64" in form_info and "mixed" in form_info and not args.no_accuracy:
+        nacc = args.accuracy_steps or int(round(86400.0 / dt))
+        out["accuracy"] = accuracy_leg(hipdyn, case, nacc)
+        print("bench: accuracy " + json.dumps(out["accuracy"]), file=sys.stderr, flush=True)
+    if args.sizing_glevel:
+        out["sizing"] = sizing_leg(hipdyn, args.sizing_glevel)
     if not args.no_adjoint:
         adj = {}
         ref = None

@@ -1054,11 +1054,15 @@ struct ViMainVA {
     radd(&gT_b[a], (real)cT0);
     radd(&hv_b[0][a], (real)cH0);
   }
-  // *p += v with the read through ldc: every bar element of a column is read
-  // and written once, by that column's thread, so the reads of a level need
-  // not wait behind the stores of the level above
+  // *p += v for a bar element this column's thread owns.  Level k (n = k -
+  // kmin - 1) adds once each to Mc_b, Ml_b (n >= 1), Mu_b (n <= N - 2),
+  // ps_b[RHOGW], ps_b[RHOGE], ps_b[RHOG], hv_b[0], hv_b[1], gT_b[0] and
+  // gT_b[4] at its own level k; the contributions of the source row at k to
+  // level k - 1 are carried in registers (cE .. cH0) and added at k - 1 with
+  // that level's own, and level kmin takes the last carry.  So every element
+  // is read once, before its one store, by this thread only: ld_own.
   template <class T>
-  NH_HD static void radd(T* p, T v) { *p = ldc(p) + v; }
+  NH_HD static void radd(T* p, T v) { *p = ld_own(p) + v; }
 };
 
 // ----------------------------------------------------------------------------

@@ -199,6 +199,11 @@ int nh_create_ex(const nh_dims* dims, const nh_consts* consts, int device, int32
   if (dims->gall_1d < 4 || dims->lall < 1 || dims->kmin != 1 || dims->kmax != dims->kall - 2 ||
       dims->kall > NH_KMAX || dims->kmax - dims->kmin < 2)
     return nh::fail(NH_ERR_ARG, where, "unsupported dimensions (need gall_1d>=4, kmin=1, kmax=kall-2, kall<=80)");
+  // a region is 2^(glevel - rlevel) points wide; the column tiles (st_geom, col_geom:
+  // sides min(W, 64) and min(256 / TX, W)) then cover it exactly
+  const int W = dims->gall_1d - 2;
+  if ((W & (W - 1)) != 0)
+    return nh::fail(NH_ERR_ARG, where, "unsupported dimensions (gall_1d - 2 must be a power of two)");
   nh_context* ctx = new (std::nothrow) nh_context{precision, nullptr, nullptr};
   if (!ctx) return nh::fail(NH_ERR_ALLOC, where, "host allocation");
   std::string err;

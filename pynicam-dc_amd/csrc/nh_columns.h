@@ -57,6 +57,18 @@ NH_HD inline T ldc(const T* p) {
 #endif
 }
 
+// Load of an element that the calling thread alone reads and writes during
+// the kernel, once each and the load first (a read-modify-write of a value
+// the thread owns, such as a column's adjoint bar).  No other thread stores
+// to it, so like ldc it may read through the constant address space and need
+// not wait behind the thread's stores to other elements.  A second load of
+// the element after its store would break this: the caller keeps one load
+// and one store per element.
+template <class T>
+NH_HD inline T ld_own(const T* p) {
+  return ldc(p);
+}
+
 // type-exact absolute value and sign transfer (no promotion of float arguments)
 NH_HD inline float nabs(float x) { return fabsf(x); }
 NH_HD inline double nabs(double x) { return fabs(x); }

@@ -200,6 +200,15 @@ class HipDynamics:
         self.dtl = float(c.DTL)
         self.sstep_max = int(c.SSTEP_MAX)
 
+    @staticmethod
+    def _check_state_pair(progq, *progs):
+        """PROG-shaped arguments hold the 6 prognostics; PROGq at least qv."""
+        for p in progs:
+            if p.dim() != 2 or p.shape[0] != 6:
+                raise ValueError("a PROG argument needs 6 fields")
+        if progq.dim() != 2 or progq.shape[0] < 1:
+            raise ValueError("PROGq needs at least one tracer field (qv)")
+
     def dynamics_step(self):
         lib = self.ctx.lib
         native.check(lib, self.ctx.fn("nh_dyn_step")(self.ctx.handle, C.c_void_p(self.prog.data_ptr()),
@@ -221,6 +230,9 @@ class HipDynamics:
         ctx = self.ctx
         for a in (stage_in, progq, prog_b):
             ctx._check_tensor(a)
+        self._check_state_pair(progq, prog_b)
+        if stage_in.shape[0] != 18:
+            raise ValueError("stage_in needs 3 x 6 fields")
         native.check(ctx.lib, ctx.fn("nh_dyn_step_adj_rec")(
             ctx.handle, C.c_void_p(stage_in.data_ptr()), C.c_void_p(progq.data_ptr()), progq.shape[0], self.dtl,
             self.sstep_max, C.c_void_p(prog_b.data_ptr()), ctx.stream()), "nh_dyn_step_adj_rec")
@@ -239,8 +251,10 @@ class HipDynamics:
     def dynamics_step_tape(self, tape):
         """``dynamics_step`` writing its intermediate fields to ``tape`` (for ``dynamics_step_adj_tape``)."""
         ctx = self.ctx
-        if tape.numel() < self.tape_bytes() or tape.device != ctx.device:
-            raise ValueError("tape too small or on another device")
+        if tape.dtype != _torch().uint8 or tape.numel() < self.tape_bytes() or tape.device != ctx.device \
+                or not tape.is_contiguous():
+            raise ValueError(f"tape must be a contiguous uint8 tensor of at least {self.tape_bytes()} bytes "
+                             f"on {ctx.device}")
         native.check(ctx.lib, ctx.fn("nh_dyn_step_tape")(
             ctx.handle, C.c_void_p(self.prog.data_ptr()), C.c_void_p(self.progq.data_ptr()), self.progq.shape[0],
             self.dtl, self.sstep_max, C.c_void_p(tape.data_ptr()), ctx.stream()), "nh_dyn_step_tape")
@@ -250,6 +264,11 @@ class HipDynamics:
         ctx = self.ctx
         for a in (progq, prog_b):
             ctx._check_tensor(a)
+        self._check_state_pair(progq, prog_b)
+        if tape.dtype != _torch().uint8 or tape.numel() < self.tape_bytes() or tape.device != ctx.device \
+                or not tape.is_contiguous():
+            raise ValueError(f"tape must be a contiguous uint8 tensor of at least {self.tape_bytes()} bytes "
+                             f"on {ctx.device}")
         native.check(ctx.lib, ctx.fn("nh_dyn_step_adj_tape")(
             ctx.handle, C.c_void_p(tape.data_ptr()), C.c_void_p(progq.data_ptr()), progq.shape[0], self.dtl,
             self.sstep_max, C.c_void_p(prog_b.data_ptr()), ctx.stream()), "nh_dyn_step_adj_tape")
@@ -264,6 +283,7 @@ class HipDynamics:
         ctx = self.ctx
         for a in (prog, progq, prog_b):
             ctx._check_tensor(a)
+        self._check_state_pair(progq, prog_b, prog)
         native.check(ctx.lib, ctx.fn("nh_dyn_step_adj")(
             ctx.handle, C.c_void_p(prog.data_ptr()), C.c_void_p(progq.data_ptr()), progq.shape[0], self.dtl,
             self.sstep_max, C.c_void_p(prog_b.data_ptr()), ctx.stream()), "nh_dyn_step_adj")
@@ -273,6 +293,9 @@ class HipDynamics:
         ctx = self.ctx
         prog = self.prog if prog is None else prog
         progq = self.progq if progq is None else progq
+        for a in (prog, progq):
+            ctx._check_tensor(a)
+        self._check_state_pair(progq, prog)
         ps = _torch().empty(ctx.ncol, dtype=_torch().float64, device=ctx.device)
         native.check(ctx.lib, ctx.fn("nh_sfc_pressure")(ctx.handle, C.c_void_p(prog.data_ptr()),
                                                         C.c_void_p(progq.data_ptr()), C.c_void_p(ps.data_ptr()),
@@ -282,6 +305,11 @@ class HipDynamics:
     def sfc_pressure_adj(self, prog, progq, ps_b, prog_b):
         """Add the transpose of ``sfc_pressure`` at (prog, progq) applied to ``ps_b`` into ``prog_b``."""
         ctx = self.ctx
+        for a in (prog, progq, prog_b):
+            ctx._check_tensor(a)
+        self._check_state_pair(progq, prog_b, prog)
+        if ps_b.numel() != ctx.ncol or ps_b.device != ctx.device:
+            raise ValueError(f"ps_b needs {ctx.ncol} values (one per column) on {ctx.device}")
         ps_b = ps_b.to(dtype=_torch().float64).contiguous()
         native.check(ctx.lib, ctx.fn("nh_sfc_pressure_adj")(
             ctx.handle, C.c_void_p(prog.data_ptr()), C.c_void_p(progq.data_ptr()), C.c_void_p(ps_b.data_ptr()),
@@ -342,6 +370,15 @@ class HipOfflineTracer(HipDynamics):
         lib = self.ctx.lib
         native.check(lib, self.ctx.fn("nh_trcadv_winds")(self.ctx.handle, C.c_void_p(self.prog.data_ptr()), float(time),
                                               self.kind, self._params, self.ctx.stream()), "nh_trcadv_winds")
+
+    @staticmethod
+    def _check_state_pair(progq, *progs):
+        """PROG-shaped arguments hold the 6 prognostics; PROGq at least qv."""
+        for p in progs:
+            if p.dim() != 2 or p.shape[0] != 6:
+                raise ValueError("a PROG argument needs 6 fields")
+        if progq.dim() != 2 or progq.shape[0] < 1:
+            raise ValueError("PROGq needs at least one tracer field (qv)")
 
     def dynamics_step(self):
         lib = self.ctx.lib

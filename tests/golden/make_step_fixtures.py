@@ -7,7 +7,7 @@ oracle over the same number of steps on a machine where the oracle would be
 too slow to rerun:
 
   gl05 z40, dt 1200 s, 72 steps (1 simulated day; the CPU-runnable case)
-  gl06 z40, dt 600 s, 72 steps (12 simulated hours)
+  gl06 z40, dt 600 s, 72 steps (12 simulated hours), and the same case after one step
 
 Per prognostic (rhog, rhogvx, rhogvy, rhogvz, rhogw, rhoge) the fixture holds
 the values at 4096 seeded interior points on physical levels, the pole-centre
@@ -16,7 +16,7 @@ budgets (mass and energies) at the start and the end.  The state needs no
 storing: ``setup.build_case`` regenerates it bit for bit from the seed.
 
 Usage (CPU, about 5 min for gl05 and 20 min for gl06):
-    python tests/golden/make_step_fixtures.py [gl05|gl06 ...]
+    python tests/golden/make_step_fixtures.py [gl05|gl06|gl06_1 ...]
 """
 from __future__ import annotations
 
@@ -34,6 +34,8 @@ sys.path.insert(0, ROOT)
 CASES = {
     "gl05": dict(glevel=5, vlayer=40, dtl=1200.0, seed=0, nsteps=72),
     "gl06": dict(glevel=6, vlayer=40, dtl=600.0, seed=0, nsteps=72),
+    # one step at gl06: the host build of the kernels runs its multi-tile paths against this
+    "gl06_1": dict(glevel=6, vlayer=40, dtl=600.0, seed=0, nsteps=1),
 }
 NSAMPLE = 4096
 
@@ -78,7 +80,7 @@ def run(name):
     fp = fingerprint(case.topo, dyn.PROG, dyn.PROG_pl, idx)
     keys = sorted(b0)
     np.savez_compressed(
-        os.path.join(HERE, f"oracle_{name}_z40_{p['nsteps']}steps.npz"),
+        os.path.join(HERE, f"oracle_gl{p['glevel']:02d}_z40_{p['nsteps']}steps.npz"),
         glevel=p["glevel"], vlayer=p["vlayer"], dtl=p["dtl"], seed=p["seed"], nsteps=p["nsteps"],
         idx=idx, budget_keys=np.array(keys), budget0=np.array([b0[k] for k in keys]),
         budget1=np.array([b1[k] for k in keys]), **fp)

@@ -204,6 +204,33 @@ def test_adjoint_argument_errors(emu64):
     assert rc == -1
 
 
+def test_adjoint_wrapper_argument_errors(case_gl04, emu64):
+    """The Python entry points refuse tensors of the wrong shape, dtype or size before any
+    pointer reaches the library."""
+    import torch
+    dyn = hipdyn.HipDynamics(emu64, case_gl04.state)
+    prog, q = dyn.prog.clone(), dyn.progq.clone()
+    five = emu64.empty(5)
+    with pytest.raises(ValueError, match="stage_in"):
+        dyn.dynamics_step_adj_rec(emu64.empty(17), q, prog.clone())
+    with pytest.raises(ValueError, match="6 fields"):
+        dyn.dynamics_step_adj_rec(emu64.empty(18), q, five)
+    with pytest.raises(ValueError, match="tape"):
+        dyn.dynamics_step_adj_tape(torch.empty(16, dtype=torch.uint8), q, prog.clone())
+    with pytest.raises(ValueError, match="tape"):
+        dyn.dynamics_step_adj_tape(torch.empty(dyn.tape_bytes(), dtype=torch.float64), q, prog.clone())
+    with pytest.raises(ValueError, match="6 fields"):
+        dyn.dynamics_step_adj(five, q, prog.clone())
+    with pytest.raises(ValueError):
+        dyn.sfc_pressure(prog.float())
+    with pytest.raises(ValueError, match="6 fields"):
+        dyn.sfc_pressure(five)
+    with pytest.raises(ValueError, match="ps_b"):
+        dyn.sfc_pressure_adj(prog, q, torch.zeros(emu64.ncol - 1, dtype=torch.float64), prog.clone())
+    with pytest.raises(ValueError):
+        dyn.sfc_pressure_adj(prog, q, torch.zeros(emu64.ncol, dtype=torch.float64), prog[:, :-1].clone())
+
+
 def test_adjoint_refuses_a_halo_map_it_cannot_transpose(case_gl04, hostemu_lib):
     """A destination that is also a source would make the one-pass scatter order-dependent."""
     ctx = hipdyn.HipContext(case_gl04.geo, case_gl04.const, device="cpu", lib_path=hostemu_lib)

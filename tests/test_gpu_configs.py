@@ -1,0 +1,215 @@
+"""The benchmark configurations at their stated sizes on the MI355X (gfx950).
+
+config 0/1  the FP64 HIP form against the numpy FP64 oracle after 72 steps at
+            gl05 (dt 1200 s) and gl06 (dt 600 s), through the oracle fingerprints
+            committed in tests/golden/ (make_step_fixtures.py wrote them);
+config 2    the offline tracer run (gl06, 60 levels to 12 km, 4 blob tracers,
+            reversing deformational flow) over its full 12-day period at
+            dt 900 s, FP64 and mixed: the error against the initial field (the
+            exact answer of a reversing flow), bounds and mass;
+config 3    gl07 z40, dt 300 s, one simulated day: the mixed form's surface
+            pressure against the FP64 form, dry-mass and energy drift;
+config 4    the adjoint at gl06 z40 over 18 steps: the gradient of
+            J = sum over a 20x20-degree box of ps^2 by central finite
+            differences along 5 seeded directions.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, interior, mod, pole_centre
+
+pytestmark = pytest.mark.gpu
+
+hipdyn = mod("hipdyn")
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+# ----------------------------------------------------------------------------
+# configs 0 and 1: 72 steps against the oracle
+# ----------------------------------------------------------------------------
+# Largest difference of the sampled points from the oracle, relative to the
+# prognostic's largest sampled value, after 72 steps; and of the per-level RMS.
+# One step agrees to 1e-12; the differences (the terrain-gradient metric is
+# round-off in the oracle and exactly zero on the device) grow with the flow.
+STEP72_TOL = 1e-8
+LEVEL_RMS_TOL = 1e-10
+
+
+@pytest.mark.parametrize("name", ["gl05", "gl06"])
+def test_72_steps_against_oracle_fixture(gpu, name):
+    path = os.path.join(GOLDEN, f"oracle_{name}_z40_72steps.npz")
+    if not os.path.exists(path):
+        pytest.skip(f"{os.path.basename(path)} not written yet (tests/golden/make_step_fixtures.py {name})")
+    fx = np.load(path)
+    gl, nsteps = int(fx["glevel"]), int(fx["nsteps"])
+    print(f"  {name}: {nsteps} FP64 steps at dt {float(fx['dtl']):g} s", flush=True)
+    case = mod("setup").build_case(glevel=gl, vlayer=int(fx["vlayer"]), dtl=float(fx["dtl"]), seed=int(fx["seed"]))
+    t = case.topo
+    ctx = hipdyn.HipContext(case.geo, case.const, device="cuda:0")
+    try:
+        dyn = hipdyn.HipDynamics(ctx, case.state)
+        for _ in range(nsteps):
+            dyn.dynamics_step()
+        st = dyn.state()
+        budget = dyn.budget()
+    finally:
+        ctx.close()
+    P, Pp = st["PROG"], st["PROG_pl"]
+    i, j, k, l = fx["idx"]
+    names = ("rhog", "rhogvx", "rhogvy", "rhogvz", "rhogw", "rhoge")
+    g = t.gall_1d
+    ks = slice(t.kmin, t.kmax + 1)
+    rms = np.sqrt(np.mean(P[1:g - 1, 1:g - 1, ks][..., :6] ** 2, axis=(0, 1, 3)))
+    errs, rerrs = {}, {}
+    for v, n in enumerate(names):
+        a, b = fx["sample"][v], P[i, j, k, l, v]
+        errs[n] = float(np.abs(a - b).max() / np.abs(a).max())
+        ap, bp = fx["pole"][..., v], Pp[0, ks, :, v]
+        errs[n] = max(errs[n], float(np.abs(ap - bp).max() / np.abs(a).max()))
+        rerrs[n] = float(np.abs(rms[:, v] - fx["level_rms"][:, v]).max() / fx["level_rms"][:, v].max())
+    print(f"  {name}: sample max rel diff " + " ".join(f"{n} {e:.2e}" for n, e in errs.items()), flush=True)
+    print(f"  {name}: level-RMS rel diff " + " ".join(f"{n} {e:.2e}" for n, e in rerrs.items()), flush=True)
+    assert max(errs.values()) < STEP72_TOL, errs
+    assert max(rerrs.values()) < LEVEL_RMS_TOL, rerrs
+    keys = [str(x) for x in fx["budget_keys"]]
+    for key, v0, v1 in zip(keys, fx["budget0"], fx["budget1"]):
+        if key in ("dry", "tot", "pot", "int"):
+            assert abs(budget[key] - v1) <= 1e-10 * abs(v1), (key, budget[key], v1)
+    # dry mass of the oracle run itself is conserved to round-off over the window
+    d0, d1 = fx["budget0"][keys.index("dry")], fx["budget1"][keys.index("dry")]
+    assert abs(d1 / d0 - 1.0) < 1e-12
+
+
+# ----------------------------------------------------------------------------
+# config 3: one simulated day at gl07, mixed against FP64
+# ----------------------------------------------------------------------------
+def test_gl07_one_day_mixed_surface_pressure_and_drift(gpu):
+    """288 steps of 300 s from the seeded state (seed 1): the mixed form's surface pressure
+    stays within 0.01 hPa (RMS over every column) of the FP64 form's, and both forms keep
+    their dry mass to round-off."""
+    bench = pytest.importorskip("bench")
+    case = mod("setup").build_case(glevel=7, vlayer=40, dtl=300.0, seed=1)
+    print("  gl07 z40: 288 steps of each form", flush=True)
+    acc = bench.accuracy_leg(hipdyn, case, 288)
+    print("  ps RMS diff %.3e hPa, max %.3e hPa; drift fp64 dry %.2e tot %.2e, mixed dry %.2e tot %.2e" % (
+        acc["ps_rms_diff_hPa"], acc["ps_max_abs_diff_hPa"], acc["fp64"]["dry_mass_drift_rel"],
+        acc["fp64"]["total_energy_drift_rel"], acc["mixed"]["dry_mass_drift_rel"],
+        acc["mixed"]["total_energy_drift_rel"]), flush=True)
+    assert acc["ps_rms_diff_hPa"] <= 0.01
+    assert abs(acc["fp64"]["dry_mass_drift_rel"]) < 1e-12
+    assert abs(acc["mixed"]["dry_mass_drift_rel"]) < 1e-5
+
+
+# ----------------------------------------------------------------------------
+# config 4: adjoint at gl06 z40, 18 steps, 5 directions
+# ----------------------------------------------------------------------------
+def test_gl06_adjoint_five_directions(gpu):
+    """J = sum over 30-50N, 0-20E of ps^2 after 18 steps; the gradient with respect to the
+    initial rhogvx/vy/vz and rhoge (and the other prognostics) against central differences
+    of step 1e-6 relative along 5 seeded directions, agreement 1e-6."""
+    torch = gpu
+    adj = mod("adjoint")
+    case = mod("setup").build_case(glevel=6, vlayer=40, dtl=600.0, seed=0)
+    ctx = hipdyn.HipContext(case.geo, case.const, device="cuda:0")
+    try:
+        dyn = hipdyn.HipDynamics(ctx, case.state)
+        m = adj.box_mask(dyn)
+        x0, q0 = dyn.prog.clone(), dyn.progq.clone()
+        x = x0.clone().requires_grad_(True)
+        ps = adj.sfc_pressure(dyn, adj.forecast(dyn, x, 18))
+        (m * ps * ps).sum().backward()
+        grad = x.grad.detach().double()
+
+        def J_of(xx):
+            dyn.prog.copy_(xx)
+            dyn.progq.copy_(q0)
+            for _ in range(18):
+                dyn.dynamics_step()
+            p = dyn.sfc_pressure()
+            return float((m * p * p).sum())
+
+        rng = np.random.default_rng(2024)
+        sc = x0.abs().amax(dim=1, keepdim=True).clone()
+        sc[3] = sc[1]
+        sc[4] = 0.01 * sc[1]
+        eps = 1e-6
+        for n in range(5):
+            v = torch.from_numpy(rng.standard_normal(tuple(x0.shape))).to(x0.device, x0.dtype) * sc
+            fd = (J_of(x0 + eps * v) - J_of(x0 - eps * v)) / (2 * eps)
+            ad = float((grad * v.double()).sum())
+            print(f"  direction {n}: fd {fd:.9e} adjoint {ad:.9e} rel {abs(fd - ad) / abs(fd):.2e}", flush=True)
+            assert abs(fd - ad) <= 1e-6 * abs(fd), (n, fd, ad)
+    finally:
+        ctx.close()
+
+
+# ----------------------------------------------------------------------------
+# config 2: full-period offline tracer run, FP64 and mixed
+# ----------------------------------------------------------------------------
+def _tracer_period(precision):
+    case = mod("setup").build_case(glevel=6, vlayer=60, ztop=12000.0, dz_bottom=200.0, dtl=900.0, seed=0)
+    geo, t = case.geo, case.topo
+    st = mod("initial").blob_tracers(geo, case.state, ntracer=4, seed=7, nblob=6, radius_m=1.0e6,
+                                     with_constant=True)
+    period = 12 * 86400.0
+    nsteps = int(round(period / 900.0))
+    ctx = hipdyn.HipContext(geo, case.const, device="cuda:0", precision=precision)
+    try:
+        run = hipdyn.HipOfflineTracer(ctx, st, "deform", (period, 40.0, 0.1, 12000.0))
+        for n in range(nsteps):
+            run.dynamics_step()
+            if (n + 1) % 288 == 0:
+                ctx.sync()
+                print(f"    {precision}: step {n + 1}/{nsteps}", flush=True)
+        got = run.state()
+    finally:
+        ctx.close()
+    return case, st, got
+
+
+def test_tracer_full_period_fp64_and_mixed(gpu):
+    """1152 steps of 900 s (one 12-day period of the reversing flow): per tracer the
+    normalised L2 and L-inf error of q against the initial field, which the flow returns to,
+    stay at the level of the scheme's dissipation (< 0.1 and < 0.5); the mixed form's q
+    agrees with the FP64 form's to 1e-3 of the range; tracer mass is kept to 1e-10 (FP64)
+    and 1e-5 (mixed) of its start; q stays in [0, 1] within 1e-3 next to the icosahedron's
+    vertices and exactly (1e-12) elsewhere."""
+    res = {}
+    for prec in ("float64", "mixed"):
+        print(f"  tracer period: {prec}", flush=True)
+        res[prec] = _tracer_period(prec)
+    case, st0, _ = res["float64"]
+    geo, t = case.geo, case.topo
+    vol = interior(geo.VOLUME, t)
+    q0 = interior(st0["PROGq"][..., 1:5], t) / interior(st0["PROGq"][..., 5:6], t)
+    m0 = (interior(st0["PROGq"][..., 1:5], t) * vol[..., None]).sum(axis=(0, 1, 2, 3))
+    qs = {}
+    for prec, (_, _, got) in res.items():
+        Q = interior(got["PROGq"], t).astype(np.float64)
+        q = Q[..., 1:5] / Q[..., 5:6]
+        qs[prec] = q
+        m1 = (Q[..., 1:5] * vol[..., None]).sum(axis=(0, 1, 2, 3))
+        d = q - q0
+        w = vol[..., None]
+        l2 = np.sqrt((d * d * w).sum(axis=(0, 1, 2, 3)) / (q0 * q0 * w).sum(axis=(0, 1, 2, 3)))
+        linf = np.abs(d).max(axis=(0, 1, 2, 3)) / np.abs(q0).max(axis=(0, 1, 2, 3))
+        print(f"  {prec}: L2 {l2}, Linf {linf}, mass {m1 / m0 - 1.0}, q in [{q.min():.3e}, {q.max():.3e}]",
+              flush=True)
+        assert np.all(l2 < 0.1) and np.all(linf < 0.5), (l2, linf)
+        assert np.all(np.abs(m1 / m0 - 1.0) < (1e-10 if prec == "float64" else 1e-5)), m1 / m0
+        assert q.min() >= -1e-3 and q.max() <= 1.0 + 1e-3, (q.min(), q.max())
+        far = slice(3, t.gall_1d - 5)
+        qf = q[far, far]
+        assert qf.min() >= -1e-12 and qf.max() <= 1.0 + 1e-12, (qf.min(), qf.max())
+    dm = np.abs(qs["mixed"] - qs["float64"]).max()
+    print(f"  mixed vs fp64 q max diff {dm:.3e}", flush=True)
+    assert dm < 1e-3

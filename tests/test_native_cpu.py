@@ -67,6 +67,9 @@ def test_create_rejects_bad_dims(hostemu_lib):
     assert rc == -1 and b"dimensions" in lib.nh_last_error()
     big = native.NhDims(10, 100, 1, 98, 10)     # kall above the column buffers
     assert lib.nh_create(C.byref(big), C.byref(cst), 0, C.byref(h)) == -1
+    odd = native.NhDims(8, 12, 1, 10, 10)       # a region 6 points wide: no icosahedral grid
+    rc = lib.nh_create(C.byref(odd), C.byref(cst), 0, C.byref(h))
+    assert rc == -1 and b"power of two" in lib.nh_last_error()
 
 
 @pytest.fixture(scope="module")
@@ -320,3 +323,32 @@ def test_backend_selection(hostemu_lib):
         backend.Backend().configure("cuda", "float64")
     with pytest.raises(ValueError):
         backend.Backend().configure("hip", "float16", lib_path=hostemu_lib)
+
+
+def test_host_build_gl06_step_multitile_matches_oracle(hostemu_lib):
+    """One gl06 z40 FP64 step of the kernel source (host build) against the oracle's step
+    (the fingerprint tests/golden/make_step_fixtures.py wrote).  At gl06 a region is 32
+    points wide: the column tiles run 32 x 8 tiles, four rows of them per region, so their
+    multi-tile paths are exercised (gl04 and gl05 regions fit one tile)."""
+    from conftest import GOLDEN
+    fx = np.load(os.path.join(GOLDEN, "oracle_gl06_z40_1steps.npz"))
+    case = mod("setup").build_case(glevel=int(fx["glevel"]), vlayer=int(fx["vlayer"]), dtl=float(fx["dtl"]),
+                                   seed=int(fx["seed"]))
+    t = case.topo
+    ctx = hipdyn.HipContext(case.geo, case.const, device="cpu", lib_path=hostemu_lib)
+    try:
+        dyn = hipdyn.HipDynamics(ctx, case.state)
+        dyn.dynamics_step()
+        st = dyn.state()
+    finally:
+        ctx.close()
+    P, Pp = st["PROG"], st["PROG_pl"]
+    i, j, k, l = fx["idx"]
+    ks = slice(t.kmin, t.kmax + 1)
+    for v in range(6):
+        a = fx["sample"][v]
+        e = max(np.abs(P[i, j, k, l, v] - a).max(), np.abs(Pp[0, ks, :, v] - fx["pole"][..., v]).max())
+        assert e <= 1e-10 * np.abs(a).max(), (v, e / np.abs(a).max())
+    g = t.gall_1d
+    rms = np.sqrt(np.mean(P[1:g - 1, 1:g - 1, ks][..., :6] ** 2, axis=(0, 1, 3)))
+    assert np.abs(rms - fx["level_rms"]).max() <= 1e-12 * fx["level_rms"].max()
