@@ -120,6 +120,9 @@ def make_inputs(seed: int = 0) -> dict:
         d["qh" + tag] = u(0.0, 1.0, shp)
         d["dq" + tag] = 0.05 * n(shp)
         d["gradq" + tag] = 1.0e-6 * n(shp + (3,))
+    # latitude per column and temperature for the Held-Suarez forcing
+    d["lat"] = u(-0.5 * np.pi, 0.5 * np.pi, (G, G, LALL))
+    d["tem"] = u(200.0, 310.0, R3)
     # tracer Courant numbers with both signs and a few exact zeros
     ck = 0.3 * n(R3 + (2,))
     ck[0, 0, 2, 0, 0] = 0.0

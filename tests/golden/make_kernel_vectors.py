@@ -542,6 +542,22 @@ def case_horizontal_limiter(d):
     return {"Qout": np.asarray(Qout), "q_a": np.asarray(qa), "Qout_pl": np.asarray(Qoutp), "q_a_pl": np.asarray(qap)}
 
 
+def case_held_suarez(d):
+    """AfHeldsuarez.AF_heldsuarez (nhm/forcing/mod_af_heldsuarez.py:43-95), dry and moist."""
+    from pynicamdc.nhm.forcing.mod_af_heldsuarez import AfHeldsuarez
+    cnst = types.SimpleNamespace(CONST_Rdry=kc.RDRY, CONST_CPdry=kc.CPDRY, CONST_CVdry=kc.CVDRY,
+                                 CONST_PRE00=1.0e5)
+    out = {}
+    for name, moist in (("dry", False), ("moist", True)):
+        hs = AfHeldsuarez()
+        hs.AF_heldsuarez_init(moist_case=moist)
+        r = hs.AF_heldsuarez(d["lat"], d["pres"], d["tem"], d["v1"], d["v2"], d["v3"], KMIN, KMAX, cnst, np.float64,
+                             xp=np)
+        for n_, a in zip(("fvx", "fvy", "fvz", "fe"), r):
+            out[f"{name}_{n_}"] = a
+    return out
+
+
 CASES = {n[5:]: f for n, f in sorted(globals().items()) if n.startswith("case_")}
 
 

@@ -347,3 +347,14 @@ def test_tracer_horizontal_limiter():
     same(qa[f, f], ref("horizontal_limiter", "q_a")[f, f])
     same(qa, ref("horizontal_limiter", "q_a"))
     same(qap[1:6], ref("horizontal_limiter", "q_a_pl")[1:6])
+
+
+@pytest.mark.parametrize("moist", [False, True])
+def test_held_suarez_tendencies(moist):
+    from oracle import forcing as of
+    c = CTX.c
+    r = of.heldsuarez_tendencies(I["lat"], I["pres"], I["tem"], I["v1"], I["v2"], I["v3"], kc.KMIN, kc.KMAX, c,
+                                 moist_case=moist)
+    name = "moist" if moist else "dry"
+    for n, a in zip(("fvx", "fvy", "fvz", "fe"), r):
+        same(a, ref("held_suarez", f"{name}_{n}"))
