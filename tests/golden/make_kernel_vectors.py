@@ -436,8 +436,9 @@ def case_divdamp3d(d):
     return out
 
 
-def _ref_function(path, cls, name):
-    """Compile one method of a reference module from its own source text."""
+def _ref_function(path, cls, name, **globs):
+    """Compile one method of a reference module from its own source text (``globs``:
+    the module-level names it reads)."""
     src = open(os.path.join(REF, path)).read()
     tree = ast.parse(src)
     for node in ast.walk(tree):
@@ -446,6 +447,7 @@ def _ref_function(path, cls, name):
                 if isinstance(f, ast.FunctionDef) and f.name == name:
                     code = textwrap.dedent(ast.get_source_segment(src, f))
                     ns = {"np": np, "adm": types.SimpleNamespace(ADM_have_pl=True)}
+                    ns.update(globs)
                     exec(compile(code, f"{path}:{f.lineno}", "exec"), ns)
                     return ns[name]
     raise LookupError(name)
@@ -556,6 +558,14 @@ def case_held_suarez(d):
         for n_, a in zip(("fvx", "fvy", "fvz", "fe"), r):
             out[f"{name}_{n_}"] = a
     return out
+
+
+def case_restart_pack(d):
+    """Prgv._pack5d (nhm/share/mod_prgvar.py:150-154): the (ij, k, l) layout of a restart
+    variable, ij = j * gall_1d + i."""
+    adm = types.SimpleNamespace(ADM_gall_1d=kc.G, ADM_kall=kc.KALL, ADM_lall=kc.LALL)
+    pack = _ref_function("pynicamdc/nhm/share/mod_prgvar.py", "Prgv", "_pack5d", adm=adm)
+    return {"prog_rhogvx": pack(None, d["PROG"], 1), "prog_rhoge": pack(None, d["PROG"], 5)}
 
 
 CASES = {n[5:]: f for n, f in sorted(globals().items()) if n.startswith("case_")}
