@@ -1,7 +1,8 @@
 """One-call construction of a benchmark-family case (host, FP64).
 
-Grid (icosahedral, rlevel 1, flat topography), stretched vertical grid,
-constants, geometry/operator coefficients and the seeded initial state.
+Grid (icosahedral, rlevel 1, flat topography; standard, spring-relaxed or read
+from the reference's grid files), stretched vertical grid, constants,
+geometry/operator coefficients and the seeded initial state.
 """
 from __future__ import annotations
 
@@ -19,10 +20,29 @@ class Case:
     state: dict
 
 
+def make_hgrid(topo: topology.Topology, kind="standard"):
+    """The horizontal grid: "standard" (recursive icosahedron subdivision), "spring"
+    (then spring dynamics with beta 1.15 and the gravity-centre move, the reference's
+    MKGRD_standard -> MKGRD_spring -> MKGRD_gravcenter), a grid-file prefix
+    ("json:<prefix>" or "npz:<prefix>", Grd.GRD_input_hgrid) or an HGrid."""
+    if isinstance(kind, grid.HGrid):
+        return kind
+    if kind == "standard":
+        return grid.generate_icosahedral(topo)
+    if kind == "spring":
+        hs, _ = grid.spring_relax(topo, grid.generate_icosahedral(topo))
+        return grid.gravity_centre(topo, hs)
+    if isinstance(kind, str) and ":" in kind:
+        mode, prefix = kind.split(":", 1)
+        return grid.read_hgrid(prefix, topo, io_mode=mode)
+    raise ValueError(f"unknown horizontal grid {kind!r}")
+
+
 def build_case(glevel: int, vlayer: int = 40, ztop: float = 30000.0, dz_bottom: float = 100.0,
-               dtl: float = 1200.0, seed: int = 0, rlevel: int = 1, with_state: bool = True) -> Case:
+               dtl: float = 1200.0, seed: int = 0, rlevel: int = 1, with_state: bool = True,
+               hgrid="standard") -> Case:
     topo = topology.Topology(glevel, rlevel, vlayer)
-    hg = grid.generate_icosahedral(topo)
+    hg = make_hgrid(topo, hgrid)
     zh = grid.stretched_interfaces(vlayer, ztop, dz_bottom)
     gz, gzh = grid.vgrid_from_interfaces(zh)
     vg = grid.VGrid(gz, gzh)
