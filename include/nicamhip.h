@@ -223,6 +223,20 @@ int nh_trcadv_winds(nh_context* ctx, double* prog, double time, int32_t kind, co
 int nh_trcadv_winds_f32(nh_context* ctx, float* prog, double time, int32_t kind, const double* params,
                         void* stream);
 
+/* Held-Suarez forcing step (AF_TYPE 'HELD-SUAREZ'; Dyn.forcing_step,
+ * nhm/dynamics/mod_dynamics.py:597-720 -> Frc.forcing_step_hs / hs_apply_core,
+ * nhm/forcing/mod_forcing.py:12-29 -> AfHeldsuarez.AF_heldsuarez,
+ * nhm/forcing/mod_af_heldsuarez.py:43-95): diagnostics of prog, Rayleigh
+ * friction of rhogvx/vy/vz and Newtonian relaxation of rhoge on the regional
+ * physical levels (the pole points are not forced, as in the reference),
+ * PROG += dt * f * rho, then the halo fill of the six prognostics.  Tracer 0 of
+ * progq is qv.  moist_case 0: Held and Suarez (1994), 1: Thatcher and
+ * Jablonowski (2016) equilibrium temperature. */
+int nh_forcing_hs(nh_context* ctx, double* prog, const double* progq, double dt, int32_t moist_case,
+                  void* stream);
+int nh_forcing_hs_f32(nh_context* ctx, float* prog, const float* progq, double dt, int32_t moist_case,
+                      void* stream);
+
 /* Global mass and energy sums (Embudget._store_var, mod_embudget.py:82-139):
  * sum of var * VOLUME over the regional interior points of the physical
  * levels (pole cells omitted, as in _gsum, mod_embudget.py:74-80), with

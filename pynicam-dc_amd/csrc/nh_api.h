@@ -143,6 +143,17 @@ inline int budget(nh_context* ctx, const void* prog, const void* progq, int32_t 
   return done(NH_DISPATCH(ctx, v_budget, prog, progq, lhv, out, stream, &err), where, err);
 }
 
+inline int forcing_hs(nh_context* ctx, void* prog, const void* progq, double dt, int32_t moist_case, void* stream,
+                      int f32) {
+  const char* where = f32 ? "nh_forcing_hs_f32" : "nh_forcing_hs";
+  if (int rc = check_ready(ctx, where, f32, true)) return rc;
+  if (!prog || !progq) return fail(NH_ERR_ARG, where, "need prog and the qv field");
+  if (!(dt > 0.0) || (moist_case != 0 && moist_case != 1))
+    return fail(NH_ERR_ARG, where, "need dt > 0 and moist_case 0 or 1");
+  std::string err;
+  return done(NH_DISPATCH(ctx, v_forcing_hs, prog, progq, dt, moist_case, stream, &err), where, err);
+}
+
 inline int dyn_step_adj(nh_context* ctx, const void* prog, const void* progq, int32_t nq, double dtl,
                         int32_t sstep_max, void* prog_b, void* stream, int f32, const void* stage_in = nullptr,
                         const char* name = nullptr) {
@@ -310,6 +321,15 @@ int nh_budget(nh_context* ctx, const double* prog, const double* progq, int32_t 
 int nh_budget_f32(nh_context* ctx, const float* prog, const float* progq, int32_t nq, double lhv, double* out,
                   void* stream) {
   return nh::budget(ctx, prog, progq, nq, lhv, out, stream, 1);
+}
+
+int nh_forcing_hs(nh_context* ctx, double* prog, const double* progq, double dt, int32_t moist_case,
+                  void* stream) {
+  return nh::forcing_hs(ctx, prog, progq, dt, moist_case, stream, 0);
+}
+int nh_forcing_hs_f32(nh_context* ctx, float* prog, const float* progq, double dt, int32_t moist_case,
+                      void* stream) {
+  return nh::forcing_hs(ctx, prog, progq, dt, moist_case, stream, 1);
 }
 
 int nh_prof_only(nh_context* ctx, const char* kernel) {

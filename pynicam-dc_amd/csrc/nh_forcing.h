@@ -14,7 +14,7 @@
 // tracer 0 the only water tracer, flat topography (GSGAM2 = 1).
 #pragma once
 
-#include "nh_step.h"
+#include "nh_engine.h"
 
 NH_BEGIN
 
@@ -22,6 +22,20 @@ NH_BEGIN
 struct HsParams {
   double sigma_b, Kf, Dth_z, Ka, Ks, T_eq0, DT_y, PRE00;
 };
+
+// dry (Held and Suarez 1994) or moist (Thatcher and Jablonowski 2016) case
+inline HsParams hs_params(int moist_case) {
+  HsParams h;
+  h.sigma_b = 0.7;
+  h.Kf = 1.0 / (1.0 * 86400.0);
+  h.Dth_z = 10.0;
+  h.Ka = 1.0 / (40.0 * 86400.0);
+  h.Ks = 1.0 / (4.0 * 86400.0);
+  h.T_eq0 = moist_case ? 294.0 : 315.0;
+  h.DT_y = moist_case ? 65.0 : 60.0;
+  h.PRE00 = 1.0e5;
+  return h;
+}
 
 template <class T>
 NH_HD inline T hs_pre(const ConstsT<T>& c, T rhog, T rhoge, T rhogq) {
@@ -99,5 +113,28 @@ struct HeldSuarezP {
     prog[5 * NF + a] = prog[5 * NF + a] + dt * fe * rho * real(1.0);
   }
 };
+
+// one forcing step of dt on prog (tracer 0 of rhogq is qv), then the halo
+// fill of the six prognostics; the surface reference pressures go to a
+// workspace field (the forcing runs between steps, when it is free)
+template <class X>
+int forcing_hs(X& x, const Context& ctx, real* prog, const real* rhogq, double dt, int moist_case) {
+  const Dims& d = ctx.d;
+  real* psref = ctx.field(W_DIAG);
+  {
+    HsSfcRef f{d, ctx.c, prog, rhogq, psref};
+    x.run(f, d.nreg);
+  }
+  {
+    HeldSuarezP f;
+    f.d = d; f.c = ctx.c; f.G = ctx.G;
+    f.hp = hs_params(moist_case);
+    f.dt = (real)dt;
+    f.prog = prog; f.rhogq = rhogq; f.psref = psref;
+    x.run_pts(f, d);
+  }
+  halo_block(x, ctx, prog, 6);
+  return x.check("forcing_hs");
+}
 
 NH_END

@@ -12,6 +12,7 @@
 #include "nh_variant.h"
 #include "nh_engine.h"
 #include "nh_adjstep.h"
+#include "nh_forcing.h"
 
 NH_BEGIN
 
@@ -137,6 +138,14 @@ int v_trcadv_winds(Context* c, void* prog, double time, int kind, const double* 
                    std::string* err) {
   NHX x(c->device, stream, &c->prof);
   int rc = trcadv_winds(x, *c, (real*)prog, time, kind, params);
+  if (rc) return failv(err, NH_ERR_DEVICE, x.message());
+  return 0;
+}
+
+int v_forcing_hs(Context* c, void* prog, const void* progq, double dt, int moist_case, void* stream,
+                 std::string* err) {
+  NHX x(c->device, stream, &c->prof);
+  int rc = forcing_hs(x, *c, (real*)prog, (const real*)progq, dt, moist_case);
   if (rc) return failv(err, NH_ERR_DEVICE, x.message());
   return 0;
 }

@@ -33,6 +33,8 @@
                     std::string* err);                                                                      \
   int v_trcadv_winds(Context* c, void* prog, double time, int kind, const double* params, void* stream,     \
                      std::string* err);                                                                     \
+  int v_forcing_hs(Context* c, void* prog, const void* progq, double dt, int moist_case, void* stream,      \
+                   std::string* err);                                                                       \
   int v_budget(Context* c, const void* prog, const void* progq, double lhv, double* out, void* stream,      \
                std::string* err);                                                                           \
   int v_dyn_step_adj(Context* c, const void* prog, const void* progq, int nq, double dtl, int sstep_max,      \

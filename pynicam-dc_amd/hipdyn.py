@@ -315,6 +315,19 @@ class HipDynamics:
             ctx.handle, C.c_void_p(prog.data_ptr()), C.c_void_p(progq.data_ptr()), C.c_void_p(ps_b.data_ptr()),
             C.c_void_p(prog_b.data_ptr()), ctx.stream()), "nh_sfc_pressure_adj")
 
+    def forcing_step(self, af_type: str = "HELD-SUAREZ", moist_case: bool = False, dt: float | None = None):
+        """The forcing after a dynamics step (``Dyn.forcing_step``, mod_dynamics.py:597-720), in place:
+        Held-Suarez Rayleigh friction and temperature relaxation (AF_heldsuarez,
+        mod_af_heldsuarez.py:43-95) over ``dt`` (default the large step), then the halo fill.
+        The regional points are forced, the pole points are not, as in the reference."""
+        if af_type != "HELD-SUAREZ":
+            raise NotImplementedError(f"backend='hip' implements AF_TYPE 'HELD-SUAREZ', not {af_type!r}")
+        ctx = self.ctx
+        dt = self.dtl if dt is None else float(dt)
+        native.check(ctx.lib, ctx.fn("nh_forcing_hs")(
+            ctx.handle, C.c_void_p(self.prog.data_ptr()), C.c_void_p(self.progq.data_ptr()), dt,
+            1 if moist_case else 0, ctx.stream()), "nh_forcing_hs")
+
     def budget(self, lhv: float | None = None) -> dict:
         """Global energies and masses of the current state (Embudget._store_var, mod_embudget.py:82-139).
 
