@@ -171,6 +171,84 @@ int nh_divergence_f32(nh_context* ctx, const float* vx, const float* vy, const f
 int nh_gradient(nh_context* ctx, const double* scl, double* gx, double* gy, double* gz, void* stream);
 int nh_gradient_f32(nh_context* ctx, const float* scl, float* gx, float* gy, float* gz, void* stream);
 
+/* ---- Standalone operators of the reference's kernel API.  Each runs one
+ * operator of the step on caller fields (device layout, NF values per field;
+ * multi-field arguments are NF-strided blocks) on the given stream.  Vertical
+ * metrics are those of flat topography (the context's k-profiles), where
+ * every metric factor of the reference expression is exactly 1. */
+
+/* Tridiagonal matrix of the vertical implicit rhogw solve, rows kmin+1..kmax of
+ * Mc, Mu, Ml (FP64 in both forms; other rows untouched), from eth and g_tilde at
+ * half levels.  Replaces compute_rhow_matrix_reg/_pl with precombine_matrix_reg/_pl
+ * (nhm/dynamics/kernels/vimatrix.py:67-165; called from mod_vi.py:1776-1922
+ * vi_rhow_update_matrix). */
+int nh_vi_matrix(nh_context* ctx, const double* eth_h, const double* g_tilde, double dt, double* Mc, double* Mu,
+                 double* Ml, void* stream);
+int nh_vi_matrix_f32(nh_context* ctx, const float* eth_h, const float* g_tilde, double dt, double* Mc, double* Mu,
+                     double* Ml, void* stream);
+
+/* rhogw column solve (FP64 arithmetic in both forms): source assembly, the two
+ * boundary rows and the Thomas sweeps on rows kmin+1..kmax; out rows below kmin
+ * and above kmax+1 are copies of rhogw; out may be rhogw.  Replaces
+ * compute_rhow_solver_reg/_pl (nhm/dynamics/kernels/virhowsolver.py:67-172;
+ * called from mod_vi.py:2433-2504 vi_rhow_solver). */
+int nh_vi_solve(nh_context* ctx, const double* rhogw, const double* rhogw0, const double* preg0, const double* rhog0,
+                const double* Srho, const double* Sw, const double* Spre, const double* Mc, const double* Mu,
+                const double* Ml, double dt, double* out, void* stream);
+int nh_vi_solve_f32(nh_context* ctx, const float* rhogw, const float* rhogw0, const float* preg0, const float* rhog0,
+                    const float* Srho, const float* Sw, const float* Spre, const double* Mc, const double* Mu,
+                    const double* Ml, double dt, float* out, void* stream);
+
+/* Diagnostics, boundary conditions and eth of the large step: diag (6 fields:
+ * pre, tem, vx, vy, vz, w) and eth from prog and qv (progq's first field); the
+ * ghost levels of prog (rhog, rhogvx..vz, rhogw, rhoge) are set in place.
+ * Replaces compute_diagnostics (kernels/diag.py:53), Bndc.BNDCND_all
+ * (mod_bndcnd.py:182-266, kernels/bndcnd.py:54-296) and THRMDYN_eth
+ * (kernels/thrmdyn.py:56) as called at mod_dynamics.py:965-1283. */
+int nh_prepost(nh_context* ctx, double* prog, const double* progq, double* diag, double* eth, void* stream);
+int nh_prepost_f32(nh_context* ctx, float* prog, const float* progq, float* diag, float* eth, void* stream);
+
+/* Fourth-order horizontal hyperdiffusion (lap order 2, DIRECT coefficient):
+ * the two iterated-Laplacian passes with a halo fill after each, then the
+ * tendency, six fields in PROG order (rhog, rhogvx, rhogvy, rhogvz, rhogw,
+ * rhoge), momentum horizontalised; the tracer tendency of the reference is zero
+ * under MIURA2004 transport and is not produced.  prog and diag as left by
+ * nh_prepost.  Uses the step workspace.  Replaces Numf.numfilter_hdiffusion
+ * (nhm/dynamics/mod_numfilter.py:845-1469, tendency :1519-1578). */
+int nh_hdiffusion(nh_context* ctx, const double* prog, const double* diag, double* tend, void* stream);
+int nh_hdiffusion_f32(nh_context* ctx, const float* prog, const float* diag, float* tend, void* stream);
+
+/* 3-D divergence damping (lap order 2): gdx, gdy, gdz on the interior points
+ * and pole centres; the vertical component gdvz of the reference is zero when
+ * its vertical damping is off (every reference configuration) and is not
+ * produced.  Uses the step workspace.  Replaces Numf.numfilter_divdamp
+ * (nhm/dynamics/mod_numfilter.py:2032-2268: OPRT3D_divdamp, kernels/oprt3ddivdamp.py:240,
+ * halo, kernels/divdamppostcomm.py:29). */
+int nh_divdamp3d(nh_context* ctx, const double* rhogvx, const double* rhogvy, const double* rhogvz,
+                 const double* rhogw, double* gdx, double* gdy, double* gdz, void* stream);
+int nh_divdamp3d_f32(nh_context* ctx, const float* rhogvx, const float* rhogvy, const float* rhogvz,
+                     const float* rhogw, float* gdx, float* gdy, float* gdz, void* stream);
+
+/* Vertical Thuburn limiter of the face values q_h (rows kmin+1..kmax limited in
+ * place) from the cell values q, d and the two Courant numbers ck (2 fields).
+ * Replaces compute_vertical_limiter (nhm/dynamics/kernels/verticallimiter.py:41-92;
+ * mod_src_tracer.py:1849-2113 vertical_limiter_thuburn). */
+int nh_limiter_v(nh_context* ctx, double* q_h, const double* q, const double* d, const double* ck, void* stream);
+int nh_limiter_v_f32(nh_context* ctx, float* q_h, const float* q, const float* d, const float* ck, void* stream);
+
+/* Horizontal Thuburn limiter of the face values q_a (6 fields, in place): the
+ * outflow bounds Qout from q, d, the Courant numbers ch (6 fields) and the
+ * upwind masks cmask (6 fields; faces 0..2 are read), their halo fill, then the
+ * limited faces.  Pole points use the first field of ch, cmask and q_a.  Uses the
+ * step workspace.  Replaces compute_horizontal_limiter_qout/_apply and their _pl
+ * forms (nhm/dynamics/kernels/horizontallimiter.py:51-255) with the Qout
+ * exchange between them (mod_src_tracer.py:2115-2662
+ * horizontal_limiter_thuburn). */
+int nh_limiter_h(nh_context* ctx, double* q_a, const double* q, const double* d, const double* ch,
+                 const double* cmask, void* stream);
+int nh_limiter_h_f32(nh_context* ctx, float* q_a, const float* q, const float* d, const float* ch,
+                     const float* cmask, void* stream);
+
 /* One dynamical-core large step, in place: prog holds the 6 prognostic
  * fields (rhog, rhogvx, rhogvy, rhogvz, rhogw, rhoge), progq the nq >= 1
  * tracer mass fields (tracer 0 = qv).  RK3 with sstep_max/3, sstep_max/2 and

@@ -12,7 +12,11 @@ struct nh_context {
   int precision;            // NH_PREC_FP64 or NH_PREC_MIXED
   nh::f64::Context* c64;
   nh::f32::Context* c32;
+  long NF;                  // values per field
 };
+
+// values per field of a context whose pointer has been checked
+inline long nh_field_size_of(const nh_context* ctx) { return ctx->NF; }
 
 namespace nh {
 
@@ -189,6 +193,20 @@ inline int sfc_pressure(nh_context* ctx, const void* prog, const void* progq, do
   return done(NH_DISPATCH(ctx, v_sfc_pressure, prog, progq, ps, ps_b, prog_b, stream, &err), where, err);
 }
 
+inline int op(nh_context* ctx, int which, const void* const* in, int nin, void* const* out, int nout, double dt,
+              void* stream, int f32, const char* name, bool need_halo) {
+  std::string where = std::string(name) + (f32 ? "_f32" : "");
+  if (int rc = check_ready(ctx, where.c_str(), f32, need_halo)) return rc;
+  for (int q = 0; q < nin; ++q)
+    if (!in[q]) return fail(NH_ERR_ARG, where.c_str(), "null field pointer");
+  for (int q = 0; q < nout; ++q)
+    if (!out[q]) return fail(NH_ERR_ARG, where.c_str(), "null field pointer");
+  if ((which == OP_VI_MATRIX || which == OP_VI_SOLVE) && !(dt > 0.0))
+    return fail(NH_ERR_ARG, where.c_str(), "dt must be > 0");
+  std::string err;
+  return done(NH_DISPATCH(ctx, v_op, which, in, out, dt, stream, &err), where.c_str(), err);
+}
+
 }  // namespace nh
 
 extern "C" {
@@ -215,7 +233,8 @@ int nh_create_ex(const nh_dims* dims, const nh_consts* consts, int device, int32
   const int W = dims->gall_1d - 2;
   if ((W & (W - 1)) != 0)
     return nh::fail(NH_ERR_ARG, where, "unsupported dimensions (gall_1d - 2 must be a power of two)");
-  nh_context* ctx = new (std::nothrow) nh_context{precision, nullptr, nullptr};
+  const long nreg = (long)dims->lall * dims->gall_1d * dims->gall_1d;
+  nh_context* ctx = new (std::nothrow) nh_context{precision, nullptr, nullptr, nreg * dims->kall + 2L * dims->kall * 6};
   if (!ctx) return nh::fail(NH_ERR_ALLOC, where, "host allocation");
   std::string err;
   const int rc = (precision == NH_PREC_FP64) ? nh::f64::v_create(dims, consts, device, &ctx->c64, &err)
@@ -432,6 +451,104 @@ int nh_sfc_pressure_adj(nh_context* ctx, const double* prog, const double* progq
 int nh_sfc_pressure_adj_f32(nh_context* ctx, const float* prog, const float* progq, const double* ps_b,
                             float* prog_b, void* stream) {
   return nh::sfc_pressure(ctx, prog, progq, nullptr, ps_b, prog_b, stream, 1, "nh_sfc_pressure_adj_f32");
+}
+
+/* standalone operators of the reference's kernel API */
+int nh_vi_matrix(nh_context* ctx, const double* eth_h, const double* g_tilde, double dt, double* Mc, double* Mu,
+                 double* Ml, void* stream) {
+  const void* in[2] = {eth_h, g_tilde};
+  void* out[3] = {Mc, Mu, Ml};
+  return nh::op(ctx, OP_VI_MATRIX, in, 2, out, 3, dt, stream, 0, "nh_vi_matrix", false);
+}
+int nh_vi_matrix_f32(nh_context* ctx, const float* eth_h, const float* g_tilde, double dt, double* Mc, double* Mu,
+                     double* Ml, void* stream) {
+  const void* in[2] = {eth_h, g_tilde};
+  void* out[3] = {Mc, Mu, Ml};
+  return nh::op(ctx, OP_VI_MATRIX, in, 2, out, 3, dt, stream, 1, "nh_vi_matrix", false);
+}
+
+int nh_vi_solve(nh_context* ctx, const double* rhogw, const double* rhogw0, const double* preg0, const double* rhog0,
+                const double* Srho, const double* Sw, const double* Spre, const double* Mc, const double* Mu,
+                const double* Ml, double dt, double* out, void* stream) {
+  const void* in[10] = {rhogw, rhogw0, preg0, rhog0, Srho, Sw, Spre, Mc, Mu, Ml};
+  void* o[1] = {out};
+  return nh::op(ctx, OP_VI_SOLVE, in, 10, o, 1, dt, stream, 0, "nh_vi_solve", false);
+}
+int nh_vi_solve_f32(nh_context* ctx, const float* rhogw, const float* rhogw0, const float* preg0, const float* rhog0,
+                    const float* Srho, const float* Sw, const float* Spre, const double* Mc, const double* Mu,
+                    const double* Ml, double dt, float* out, void* stream) {
+  const void* in[10] = {rhogw, rhogw0, preg0, rhog0, Srho, Sw, Spre, Mc, Mu, Ml};
+  void* o[1] = {out};
+  return nh::op(ctx, OP_VI_SOLVE, in, 10, o, 1, dt, stream, 1, "nh_vi_solve", false);
+}
+
+int nh_prepost(nh_context* ctx, double* prog, const double* progq, double* diag, double* eth, void* stream) {
+  const void* in[2] = {prog, progq};
+  void* out[3] = {prog, diag, eth};
+  return nh::op(ctx, OP_PREPOST, in, 2, out, 3, 0.0, stream, 0, "nh_prepost", false);
+}
+int nh_prepost_f32(nh_context* ctx, float* prog, const float* progq, float* diag, float* eth, void* stream) {
+  const void* in[2] = {prog, progq};
+  void* out[3] = {prog, diag, eth};
+  return nh::op(ctx, OP_PREPOST, in, 2, out, 3, 0.0, stream, 1, "nh_prepost", false);
+}
+
+int nh_hdiffusion(nh_context* ctx, const double* prog, const double* diag, double* tend, void* stream) {
+  if (int rc = nh::check_ctx(ctx, "nh_hdiffusion", 0)) return rc;
+  const long NF = nh_field_size_of(ctx);
+  const void* in[2] = {prog, diag};
+  void* out[6];
+  for (int q = 0; q < 6; ++q) out[q] = tend ? tend + q * NF : nullptr;
+  return nh::op(ctx, OP_HDIFF, in, 2, out, 6, 0.0, stream, 0, "nh_hdiffusion", true);
+}
+int nh_hdiffusion_f32(nh_context* ctx, const float* prog, const float* diag, float* tend, void* stream) {
+  if (int rc = nh::check_ctx(ctx, "nh_hdiffusion_f32", 1)) return rc;
+  const long NF = nh_field_size_of(ctx);
+  const void* in[2] = {prog, diag};
+  void* out[6];
+  for (int q = 0; q < 6; ++q) out[q] = tend ? tend + q * NF : nullptr;
+  return nh::op(ctx, OP_HDIFF, in, 2, out, 6, 0.0, stream, 1, "nh_hdiffusion", true);
+}
+
+int nh_divdamp3d(nh_context* ctx, const double* rhogvx, const double* rhogvy, const double* rhogvz,
+                 const double* rhogw, double* gdx, double* gdy, double* gdz, void* stream) {
+  const void* in[4] = {rhogvx, rhogvy, rhogvz, rhogw};
+  void* out[3] = {gdx, gdy, gdz};
+  return nh::op(ctx, OP_DIVDAMP, in, 4, out, 3, 0.0, stream, 0, "nh_divdamp3d", true);
+}
+int nh_divdamp3d_f32(nh_context* ctx, const float* rhogvx, const float* rhogvy, const float* rhogvz,
+                     const float* rhogw, float* gdx, float* gdy, float* gdz, void* stream) {
+  const void* in[4] = {rhogvx, rhogvy, rhogvz, rhogw};
+  void* out[3] = {gdx, gdy, gdz};
+  return nh::op(ctx, OP_DIVDAMP, in, 4, out, 3, 0.0, stream, 1, "nh_divdamp3d", true);
+}
+
+int nh_limiter_v(nh_context* ctx, double* q_h, const double* q, const double* d, const double* ck, void* stream) {
+  if (int rc = nh::check_ctx(ctx, "nh_limiter_v", 0)) return rc;
+  const long NF = nh_field_size_of(ctx);
+  const void* in[4] = {q, d, ck, ck ? ck + NF : nullptr};
+  void* out[1] = {q_h};
+  return nh::op(ctx, OP_LIMITER_V, in, 4, out, 1, 0.0, stream, 0, "nh_limiter_v", false);
+}
+int nh_limiter_v_f32(nh_context* ctx, float* q_h, const float* q, const float* d, const float* ck, void* stream) {
+  if (int rc = nh::check_ctx(ctx, "nh_limiter_v_f32", 1)) return rc;
+  const long NF = nh_field_size_of(ctx);
+  const void* in[4] = {q, d, ck, ck ? ck + NF : nullptr};
+  void* out[1] = {q_h};
+  return nh::op(ctx, OP_LIMITER_V, in, 4, out, 1, 0.0, stream, 1, "nh_limiter_v", false);
+}
+
+int nh_limiter_h(nh_context* ctx, double* q_a, const double* q, const double* d, const double* ch,
+                 const double* cmask, void* stream) {
+  const void* in[4] = {q, d, ch, cmask};
+  void* out[1] = {q_a};
+  return nh::op(ctx, OP_LIMITER_H, in, 4, out, 1, 0.0, stream, 0, "nh_limiter_h", true);
+}
+int nh_limiter_h_f32(nh_context* ctx, float* q_a, const float* q, const float* d, const float* ch,
+                     const float* cmask, void* stream) {
+  const void* in[4] = {q, d, ch, cmask};
+  void* out[1] = {q_a};
+  return nh::op(ctx, OP_LIMITER_H, in, 4, out, 1, 0.0, stream, 1, "nh_limiter_h", true);
 }
 
 }  // extern "C"

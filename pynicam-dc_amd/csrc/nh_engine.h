@@ -495,6 +495,59 @@ inline StageFields ws_stage(const Context& ctx) {
   return {ctx.field(W_HDA), ctx.field(W_HDB), ctx.field(W_GT), ctx.field(W_ETHTOT), ctx.dfield(D_MC)};
 }
 
+// hyperdiffusion (numfilter_hdiffusion, mod_numfilter.py:845-1469, lap order 2):
+// the iterated Laplacians of (vx, vy, vz, w, T, rho) in two passes, a halo fill
+// after each; pass 1 into sf.hda, pass 2 into sf.hdb (6 fields each)
+template <class X>
+void hdiff_passes(X& x, const Context& ctx, const real* prog, const real* diag, const StageFields& sf) {
+  const Dims& d = ctx.d;
+  const long NF = d.NF;
+  HDiffPassP<0> f;
+  f.d = d; f.c = ctx.c; f.G = ctx.G;
+  f.in[0] = diag + I_VX * NF; f.in[1] = diag + I_VY * NF; f.in[2] = diag + I_VZ * NF;
+  f.in[3] = diag + I_W * NF; f.in[4] = diag + I_TEM * NF; f.in[5] = prog;
+  for (int q = 0; q < 6; ++q) f.out[q] = sf.hda + q * NF;
+  f.rhog = prog;
+  const ColGeom cg = col_geom(d, kColChunks);
+  const StGeom sg = st_geom(d, kColChunks);
+  {
+    ColStencil<LapC<6, HDiffPassP<0>>> cs;
+    cs.op.pk = f;
+    cs.op.d = d;
+    for (int q = 0; q < 6; ++q) { cs.op.in[q] = f.in[q]; cs.op.out[q] = f.out[q]; }
+    cs.op.pole_off = false;
+    cs.cg = sg;
+    x.run_cols(cs, sg.nblocks, decltype(cs)::lds_elems(sg));
+  }
+  halo_block(x, ctx, sf.hda, 6);
+  // second pass: the Laplacians of fields 0..3, then the Kh-weighted
+  // diffusion of fields 4, 5 (whose pole blocks do all six fields)
+  HDiffPassP<1> f2;
+  f2.d = d; f2.c = ctx.c; f2.G = ctx.G;
+  for (int q = 0; q < 6; ++q) {
+    f2.in[q] = sf.hda + q * NF;
+    f2.out[q] = sf.hdb + q * NF;
+  }
+  f2.rhog = prog;
+  {
+    ColStencil<LapC<4, HDiffPassP<1>>> cs;
+    cs.op.pk = f2;
+    cs.op.d = d;
+    for (int q = 0; q < 4; ++q) { cs.op.in[q] = f2.in[q]; cs.op.out[q] = f2.out[q]; }
+    cs.op.pole_off = true;
+    cs.cg = sg;
+    x.run_cols(cs, sg.nblocks, decltype(cs)::lds_elems(sg));
+  }
+  {
+    DiffKhK k2;
+    k2.d = d; k2.c = ctx.c; k2.G = ctx.G; k2.cg = cg;
+    for (int q = 0; q < 6; ++q) { k2.in[q] = f2.in[q]; k2.out[q] = f2.out[q]; }
+    k2.rhog = prog;
+    x.run_cols(k2, cg.nblocks, DiffKhK::lds_elems(cg));
+  }
+  halo_block(x, ctx, sf.hdb, 6);
+}
+
 // large-step part of RK stage nl: diagnostics and boundary values (in place
 // on prog's ghost rows), momentum advection, hyperdiffusion, divergence
 // damping of prog, the sources and matrix of the small steps, ethtot and the
@@ -511,53 +564,7 @@ void stage_large(X& x, const Context& ctx, real* prog, const real* rhogq, double
     x.run_pts(f, d);
   }
   run_staged(x, AdvMomP{d, ctx.c, ctx.G, prog, diag, gadv});
-  // hyperdiffusion: two passes with a halo fill after each
-  {
-    HDiffPassP<0> f;
-    f.d = d; f.c = ctx.c; f.G = ctx.G;
-    f.in[0] = diag + I_VX * NF; f.in[1] = diag + I_VY * NF; f.in[2] = diag + I_VZ * NF;
-    f.in[3] = diag + I_W * NF; f.in[4] = diag + I_TEM * NF; f.in[5] = prog;
-    for (int q = 0; q < 6; ++q) f.out[q] = sf.hda + q * NF;
-    f.rhog = prog;
-    const ColGeom cg = col_geom(d, kColChunks);
-    const StGeom sg = st_geom(d, kColChunks);
-    {
-      ColStencil<LapC<6, HDiffPassP<0>>> cs;
-      cs.op.pk = f;
-      cs.op.d = d;
-      for (int q = 0; q < 6; ++q) { cs.op.in[q] = f.in[q]; cs.op.out[q] = f.out[q]; }
-      cs.op.pole_off = false;
-      cs.cg = sg;
-      x.run_cols(cs, sg.nblocks, decltype(cs)::lds_elems(sg));
-    }
-    halo_block(x, ctx, sf.hda, 6);
-    // second pass: the Laplacians of fields 0..3, then the Kh-weighted
-    // diffusion of fields 4, 5 (whose pole blocks do all six fields)
-    HDiffPassP<1> f2;
-    f2.d = d; f2.c = ctx.c; f2.G = ctx.G;
-    for (int q = 0; q < 6; ++q) {
-      f2.in[q] = sf.hda + q * NF;
-      f2.out[q] = sf.hdb + q * NF;
-    }
-    f2.rhog = prog;
-    {
-      ColStencil<LapC<4, HDiffPassP<1>>> cs;
-      cs.op.pk = f2;
-      cs.op.d = d;
-      for (int q = 0; q < 4; ++q) { cs.op.in[q] = f2.in[q]; cs.op.out[q] = f2.out[q]; }
-      cs.op.pole_off = true;
-      cs.cg = sg;
-      x.run_cols(cs, sg.nblocks, decltype(cs)::lds_elems(sg));
-    }
-    {
-      DiffKhK k2;
-      k2.d = d; k2.c = ctx.c; k2.G = ctx.G; k2.cg = cg;
-      for (int q = 0; q < 6; ++q) { k2.in[q] = f2.in[q]; k2.out[q] = f2.out[q]; }
-      k2.rhog = prog;
-      x.run_cols(k2, cg.nblocks, DiffKhK::lds_elems(cg));
-    }
-    halo_block(x, ctx, sf.hdb, 6);
-  }
+  hdiff_passes(x, ctx, prog, diag, sf);
   // divergence damping of the large-step momentum
   divdamp(x, ctx, prog + 1 * NF, prog + 2 * NF, prog + 3 * NF, prog + 4 * NF,
           ctx.field(W_DD), ctx.field(W_DD + 1), ctx.field(W_DD + 2));

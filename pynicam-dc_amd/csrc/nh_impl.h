@@ -13,6 +13,7 @@
 #include "nh_engine.h"
 #include "nh_adjstep.h"
 #include "nh_forcing.h"
+#include "nh_ops.h"
 
 NH_BEGIN
 
@@ -115,6 +116,82 @@ int v_gradient(Context* c, const void* scl, void* gx, void* gy, void* gz, void* 
   OpGradP f{c->d, c->G, (const real*)scl, (real*)gx, (real*)gy, (real*)gz};
   x.run_pts(f, c->d);
   if (x.check("nh_gradient")) return failv(err, NH_ERR_DEVICE, x.message());
+  return 0;
+}
+
+// standalone operators (nh_ops.h); in[] and out[] as listed per op in nh_variant.h
+int v_op(Context* c, int op, const void* const* in, void* const* out, double dt, void* stream, std::string* err) {
+  NHX x(c->device, stream, &c->prof);
+  const Dims& d = c->d;
+  const long NF = d.NF;
+  auto R = [&](int q) { return (const real*)in[q]; };
+  switch (op) {
+    case OP_VI_MATRIX: {
+      ViMatrixOp f{d, c->c, c->G, dt, R(0), R(1), (double*)out[0], (double*)out[1], (double*)out[2]};
+      x.run(f, ncols(d));
+      break;
+    }
+    case OP_VI_SOLVE: {
+      auto solve = [&](auto f) {
+        f.d = d; f.c = c->c; f.G = c->G; f.dt = dt;
+        f.rhogw = R(0); f.rhogw0 = R(1); f.preg0 = R(2); f.rhog0 = R(3);
+        f.Srho = R(4); f.Sw = R(5); f.Spre = R(6);
+        f.Mc = (const double*)in[7]; f.Mu = (const double*)in[8]; f.Ml = (const double*)in[9];
+        f.out = (real*)out[0];
+        x.run(f, ncols(d));
+      };
+      if (d.kmax - d.kmin == 39) solve(ViSolveOp<39>{});
+      else solve(ViSolveOp<0>{});
+      break;
+    }
+    case OP_PREPOST: {
+      PrePostP f{d, c->c, c->G, (real*)out[0], R(1), (real*)out[1], (real*)out[2]};
+      x.run_pts(f, d);
+      break;
+    }
+    case OP_HDIFF: {
+      // the two passes into the step workspace (cleared, so that columns no
+      // halo fill reaches hold zeros), a halo fill after each, then the tendency
+      const real* prog = R(0);
+      const real* diag = R(1);
+      real* hda = c->field(W_HDA);
+      real* hdb = c->field(W_HDB);
+      x.run(FillZero{hda}, 12 * NF);
+      const StageFields sf{hda, hdb, c->field(W_GT), c->field(W_ETHTOT), c->dfield(D_MC)};
+      hdiff_passes(x, *c, prog, diag, sf);
+      HDiffTendOp f;
+      f.d = d; f.c = c->c; f.G = c->G;
+      f.rhog = prog;
+      for (int q = 0; q < 6; ++q) { f.vt[q] = hdb + q * NF; f.out[q] = (real*)out[q]; }
+      x.run_pts(f, d);
+      break;
+    }
+    case OP_DIVDAMP: {
+      divdamp(x, *c, R(0), R(1), R(2), R(3), (real*)out[0], (real*)out[1], (real*)out[2]);
+      break;
+    }
+    case OP_LIMITER_V: {
+      LimVOp f{d, (real*)out[0], R(0), R(1), R(2), R(3)};
+      x.run(f, ncols(d));
+      break;
+    }
+    case OP_LIMITER_H: {
+      // q_a is read by the neighbours of each cell: limit from a copy of it
+      real* qa_in = c->field(W_HDA);
+      real* qo = c->field(W_QOUT);
+      x.d2d(qa_in, out[0], 6 * NF * sizeof(real));
+      const LimHGeom H{d, c->G.sgp, R(0), R(3)};
+      LimHQoutOp a{H, R(1), R(2), {qo, qo + NF}};
+      x.run_pts(a, d);
+      halo(x, *c, a.qout, 2);
+      LimHApplyOp b{H, qa_in, {qo, qo + NF}, (real*)out[0]};
+      x.run_pts(b, d);
+      break;
+    }
+    default:
+      return failv(err, NH_ERR_ARG, "unknown operator");
+  }
+  if (x.check("operator")) return failv(err, NH_ERR_DEVICE, x.message());
   return 0;
 }
 

@@ -9,6 +9,17 @@
 
 #include "../../include/nicamhip.h"
 
+// standalone operators of v_op: inputs in[], outputs out[] (field pointers)
+enum NhOp {
+  OP_VI_MATRIX = 0,   // in: eth_h, g_tilde; out: Mc, Mu, Ml (double)
+  OP_VI_SOLVE = 1,    // in: rhogw, rhogw0, preg0, rhog0, Srho, Sw, Spre, Mc, Mu, Ml (double); out: rhogw
+  OP_PREPOST = 2,     // in: -, progq; out: prog (ghost rows, in place), diag (6), eth
+  OP_HDIFF = 3,       // in: prog, diag; out: 6 tendencies
+  OP_DIVDAMP = 4,     // in: rhogvx, rhogvy, rhogvz, rhogw; out: gdx, gdy, gdz
+  OP_LIMITER_V = 5,   // in: q, d, ck0, ck1; out: q_h (in place)
+  OP_LIMITER_H = 6    // in: q, d, ch (6), cmask (6); out: q_a (6, in place)
+};
+
 #define NH_VARIANT_API(NS)                                                                                  \
   namespace nh {                                                                                            \
   namespace NS {                                                                                            \
@@ -24,6 +35,8 @@
   int v_divergence(Context* c, const void* vx, const void* vy, const void* vz, void* out, void* stream,     \
                    std::string* err);                                                                       \
   int v_gradient(Context* c, const void* scl, void* gx, void* gy, void* gz, void* stream, std::string* err); \
+  int v_op(Context* c, int op, const void* const* in, void* const* out, double dt, void* stream,              \
+           std::string* err);                                                                               \
   int v_dyn_step(Context* c, void* prog, void* progq, int nq, double dtl, int sstep_max, void* record,      \
                  void* tape, void* stream, std::string* err);                                               \
   long v_tape_bytes(const Context* c, int sstep_max);                                                      \

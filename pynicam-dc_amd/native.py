@@ -82,6 +82,13 @@ EXPORTS = {
     "nh_sfc_pressure": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "nh_sfc_pressure_adj": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "nh_forcing_hs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int32, C.c_void_p]),
+    "nh_vi_matrix": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double] + [C.c_void_p] * 4),
+    "nh_vi_solve": (C.c_int, [C.c_void_p] + [C.c_void_p] * 10 + [C.c_double, C.c_void_p, C.c_void_p]),
+    "nh_prepost": (C.c_int, [C.c_void_p] + [C.c_void_p] * 5),
+    "nh_hdiffusion": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4),
+    "nh_divdamp3d": (C.c_int, [C.c_void_p] + [C.c_void_p] * 8),
+    "nh_limiter_v": (C.c_int, [C.c_void_p] + [C.c_void_p] * 5),
+    "nh_limiter_h": (C.c_int, [C.c_void_p] + [C.c_void_p] * 6),
     "nh_prof_enable": (C.c_int, [C.c_void_p, C.c_int32]),
     "nh_prof_only": (C.c_int, [C.c_void_p, C.c_char_p]),
     "nh_prof_report": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.POINTER(C.c_double),
@@ -91,7 +98,8 @@ EXPORTS = {
 for _name in ("nh_halo_fill", "nh_laplacian", "nh_divergence", "nh_gradient", "nh_dyn_step", "nh_tracer_step",
               "nh_trcadv_winds", "nh_budget", "nh_dyn_step_adj", "nh_sfc_pressure", "nh_sfc_pressure_adj",
               "nh_dyn_step_record", "nh_dyn_step_adj_rec", "nh_dyn_step_tape", "nh_dyn_step_adj_tape",
-              "nh_tracer_step_adj", "nh_forcing_hs"):
+              "nh_tracer_step_adj", "nh_forcing_hs", "nh_vi_matrix", "nh_vi_solve", "nh_prepost", "nh_hdiffusion",
+              "nh_divdamp3d", "nh_limiter_v", "nh_limiter_h"):
     EXPORTS[_name + "_f32"] = EXPORTS[_name]
 
 PREC_FP64, PREC_MIXED = 0, 1
