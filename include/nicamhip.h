@@ -326,6 +326,20 @@ int nh_budget(nh_context* ctx, const double* prog, const double* progq, int32_t 
 int nh_budget_f32(nh_context* ctx, const float* prog, const float* progq, int32_t nq, double lhv, double* out,
                   void* stream);
 
+/* nh_budget without any device allocation (no counterpart in the reference;
+ * for callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops).  The same sums, in either form (prog and progq
+ * are double or float as the context's fields), with both reduction stages on
+ * the device in the library's fixed 8 MiB FP64 scratch array (2^20 values,
+ * held by each GPU the library runs on until the process ends): at most
+ * (2^20 - 10) / 10 blocks of 256 interior points, i.e. 26,843,136 interior
+ * points of the physical levels (gl08 z40 has 26,214,400); a larger grid is
+ * refused with NH_ERR_ARG.  Every partial is written before it is read, so no
+ * state carries over between calls; calls that overlap on one GPU (two streams
+ * or threads) share the array and race.  Synchronises the stream. */
+int nh_budget_noalloc(nh_context* ctx, const void* prog, const void* progq, int32_t nq, double lhv, double* out,
+                      void* stream);
+
 /* Adjoint of one nh_dyn_step (no counterpart in the reference, which has no
  * adjoint; the forward call it transposes is Dyn.dynamics_step,
  * nhm/dynamics/mod_dynamics.py:1725).  prog, progq: the state the forward step

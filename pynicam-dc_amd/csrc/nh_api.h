@@ -13,6 +13,7 @@ struct nh_context {
   nh::f64::Context* c64;
   nh::f32::Context* c32;
   long NF;                  // values per field
+  int device;               // the device the context lives on
 };
 
 // values per field of a context whose pointer has been checked
@@ -234,7 +235,7 @@ int nh_create_ex(const nh_dims* dims, const nh_consts* consts, int device, int32
   if ((W & (W - 1)) != 0)
     return nh::fail(NH_ERR_ARG, where, "unsupported dimensions (gall_1d - 2 must be a power of two)");
   const long nreg = (long)dims->lall * dims->gall_1d * dims->gall_1d;
-  nh_context* ctx = new (std::nothrow) nh_context{precision, nullptr, nullptr, nreg * dims->kall + 2L * dims->kall * 6};
+  nh_context* ctx = new (std::nothrow) nh_context{precision, nullptr, nullptr, nreg * dims->kall + 2L * dims->kall * 6, device};
   if (!ctx) return nh::fail(NH_ERR_ALLOC, where, "host allocation");
   std::string err;
   const int rc = (precision == NH_PREC_FP64) ? nh::f64::v_create(dims, consts, device, &ctx->c64, &err)
@@ -243,6 +244,12 @@ int nh_create_ex(const nh_dims* dims, const nh_consts* consts, int device, int32
     delete ctx;
     return nh::fail(rc, where, err);
   }
+  ctx->device = device;
+  // set-up of the allocation-free entry point: the scratch array's address on
+  // this device (looked up once per device; a failure is reported by that entry)
+  double* sp = nullptr;
+  long sn = 0;
+  (void)nh::scratch_lookup(device, &sp, &sn, nullptr);
   *out = ctx;
   return 0;
 }
@@ -451,6 +458,18 @@ int nh_sfc_pressure_adj(nh_context* ctx, const double* prog, const double* progq
 int nh_sfc_pressure_adj_f32(nh_context* ctx, const float* prog, const float* progq, const double* ps_b,
                             float* prog_b, void* stream) {
   return nh::sfc_pressure(ctx, prog, progq, nullptr, ps_b, prog_b, stream, 1, "nh_sfc_pressure_adj_f32");
+}
+
+int nh_budget_noalloc(nh_context* ctx, const void* prog, const void* progq, int32_t nq, double lhv, double* out,
+                      void* stream) {
+  const char* where = "nh_budget_noalloc";
+  if (int rc = nh::check_ready(ctx, where, -1, false)) return rc;
+  if (!prog || !progq || nq < 1 || !out) return nh::fail(NH_ERR_ARG, where, "need prog, qv and an output array");
+  std::string err;
+  double* ws = nullptr;
+  long ws_n = 0;
+  if (int rc = nh::scratch_lookup(ctx->device, &ws, &ws_n, &err)) return nh::fail(rc, where, err);
+  return nh::done(NH_DISPATCH(ctx, v_budget_ws, prog, progq, lhv, out, ws, ws_n, stream, &err), where, err);
 }
 
 /* standalone operators of the reference's kernel API */

@@ -82,4 +82,37 @@ struct BudgetT {
   }
 };
 
+// Second stage on the device (the allocation-free budget): one block sums the
+// nblocks partials of every term, each thread a strided share compensated, then
+// six threads the 256 thread sums; fin[term] = (sum, correction).
+struct BudgetSumT {
+  using LdsT = double;
+  static constexpr bool kStage = false;
+  static constexpr const char* kName = "budget_sum";
+  long nblocks;
+  const double* part;    // [nblocks][B_NTERMS][2]
+  double* fin;           // [B_NTERMS][2]
+  NH_HD void phase1(int, int t, double* lds) const {
+    for (int m = 0; m < B_NTERMS; ++m) {
+      double s = 0.0, c = 0.0;
+      for (long b = t; b < nblocks; b += 256) {
+        neu_add(s, c, part[(b * B_NTERMS + m) * 2]);
+        neu_add(s, c, part[(b * B_NTERMS + m) * 2 + 1]);
+      }
+      lds[(m * 256 + t) * 2] = s;
+      lds[(m * 256 + t) * 2 + 1] = c;
+    }
+  }
+  NH_HD void phase2(int, int t, double* lds) const {
+    if (t >= B_NTERMS) return;
+    double s = 0.0, c = 0.0;
+    for (int u = 0; u < 256; ++u) {
+      neu_add(s, c, lds[(t * 256 + u) * 2]);
+      neu_add(s, c, lds[(t * 256 + u) * 2 + 1]);
+    }
+    fin[t * 2] = s;
+    fin[t * 2 + 1] = c;
+  }
+};
+
 NH_END

@@ -468,6 +468,38 @@ int budget(X& x, const Context& ctx, const real* prog, const real* progq, double
   return 0;
 }
 
+// the same sums with both reduction stages on the device and a caller-provided
+// FP64 scratch of ws_n values (block partials, then the six final pairs); no
+// allocation.  NH_ERR_ARG when the grid needs more partials than ws_n holds.
+template <class X>
+int budget_ws(X& x, const Context& ctx, const real* prog, const real* progq, double lhv, double out[6],
+              double* ws, long ws_n) {
+  const Dims& d = ctx.d;
+  const int W = d.g - 2;
+  const long npts = (long)d.lall * (d.kmax - d.kmin + 1) * W * W;
+  const int nblocks = (int)((npts + 255) / 256);
+  const long npart = (long)nblocks * B_NTERMS * 2;
+  if (npart + 2 * B_NTERMS > ws_n) return NH_ERR_ARG;
+  double* part = ws;
+  double* fin = ws + npart;
+  BudgetT f{d, ctx.c, ctx.G, prog, progq, lhv, npts, part};
+  x.run_tiles(f, nblocks, B_NTERMS * 256);
+  BudgetSumT r{nblocks, part, fin};
+  x.run_tiles(r, 1, 2 * B_NTERMS * 256);
+  double h[2 * B_NTERMS];
+  x.d2h(h, fin, sizeof(h));
+  if (x.check("budget")) return NH_ERR_DEVICE;
+  double sum[B_NTERMS];
+  for (int m = 0; m < B_NTERMS; ++m) sum[m] = h[2 * m] + h[2 * m + 1];
+  out[0] = sum[B_POT];
+  out[1] = sum[B_INT];
+  out[2] = sum[B_KIN];
+  out[3] = sum[B_POT] + sum[B_INT] + sum[B_KIN];
+  out[4] = sum[B_DRY];
+  out[5] = sum[B_VAP];
+  return 0;
+}
+
 // ----------------------------------------------------------------------------
 // one large step (mod_dynamics.py:1725-2741 with the RK3/HEVI schedule)
 // ----------------------------------------------------------------------------

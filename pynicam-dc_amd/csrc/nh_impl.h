@@ -236,6 +236,16 @@ int v_budget(Context* c, const void* prog, const void* progq, double lhv, double
   return 0;
 }
 
+int v_budget_ws(Context* c, const void* prog, const void* progq, double lhv, double* out, double* ws, long ws_n,
+                void* stream, std::string* err) {
+  NHX x(c->device, stream, &c->prof);
+  int rc = budget_ws(x, *c, (const real*)prog, (const real*)progq, lhv, out, ws, ws_n);
+  if (rc == NH_ERR_ARG)
+    return failv(err, rc, "grid needs more than the " + std::to_string(ws_n) + " FP64 values of the budget scratch");
+  if (rc) return failv(err, NH_ERR_DEVICE, x.message());
+  return 0;
+}
+
 namespace {
 // the adjoint workspace with nck small-step checkpoint slots (grown on demand)
 template <class X>

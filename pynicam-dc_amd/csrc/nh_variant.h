@@ -50,6 +50,8 @@ enum NhOp {
                    std::string* err);                                                                       \
   int v_budget(Context* c, const void* prog, const void* progq, double lhv, double* out, void* stream,      \
                std::string* err);                                                                           \
+  int v_budget_ws(Context* c, const void* prog, const void* progq, double lhv, double* out, double* ws,       \
+                  long ws_n, void* stream, std::string* err);                                               \
   int v_dyn_step_adj(Context* c, const void* prog, const void* progq, int nq, double dtl, int sstep_max,      \
                      void* prog_b, const void* stage_in, void* stream, std::string* err);                   \
   int v_tracer_step_adj(Context* c, const void* prog, const void* progq, int nq, double dtl, void* progq_b,    \
@@ -65,3 +67,11 @@ enum NhOp {
 
 NH_VARIANT_API(f64)
 NH_VARIANT_API(f32)
+
+namespace nh {
+// the library's fixed FP64 scratch on a device (the allocation-free entry
+// points): *p its device address, *n its size in values.  Defined beside the
+// scratch array by each build (gfx950: a file-scope device array whose address
+// is looked up once per device; host test build: a static array).
+int scratch_lookup(int device, double** p, long* n, std::string* err);
+}

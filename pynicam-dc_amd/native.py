@@ -65,6 +65,8 @@ EXPORTS = {
                                   C.c_void_p]),
     "nh_budget": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double,
                             C.POINTER(C.c_double), C.c_void_p]),
+    "nh_budget_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double,
+                                    C.POINTER(C.c_double), C.c_void_p]),
     "nh_create_ex": (C.c_int, [C.POINTER(NhDims), C.POINTER(NhConsts), C.c_int, C.c_int32,
                                C.POINTER(C.c_void_p)]),
     "nh_precision": (C.c_int32, [C.c_void_p]),
