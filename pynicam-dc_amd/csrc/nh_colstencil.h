@@ -14,7 +14,9 @@
 //
 // An operator Op supplies
 //   kName, kNS (stencil fields), Col (per-column registers), d (Dims),
-//   stage_level(k), stage(k, a, v[kNS]), init_col(col index, Col&),
+//   Stg (registers of the column a thread stages: its own and its ring column),
+//   init_stage(col index, Stg&), stage_level(k), stage(k, a, Stg, v[kNS]),
+//   init_col(col index, Col&),
 //   point(k, a, Col, nb) with nb(m, f) = field f at stencil neighbour m
 //   (0 (0,0) 1 (+1,0) 2 (+1,+1) 3 (0,+1) 4 (-1,0) 5 (-1,-1) 6 (0,-1)),
 //   pole(pb): the pole centre of pole level pb (its point kernel).
@@ -60,6 +62,7 @@ struct ColStencil {
   static int lds_elems(const StGeom& cg) { return 2 * NS * (cg.TX + 2) * (cg.TY + 2); }
   struct State {
     typename Op::Col col;
+    typename Op::Stg sg, rsg;   // staging registers of the own and the ring column
     long a, ra;        // own and ring addresses at the current level
     int k, sidx, ridx;
     bool outp, ring;
@@ -86,6 +89,7 @@ struct ColStencil {
       s.sidx = (ty + 1) * SW + tx + 1;
       s.a = ((long)(l * d.kall + k0) * g + j) * g + i;
       op.init_col(((long)l * g + j) * g + i, s.col);
+      op.init_stage(((long)l * g + j) * g + i, s.sg);
     }
     // the ring: row -1 (x = -1..TX-1), row TY (x = 0..TX), columns -1 and TX (y = 0..TY-1)
     if (tid < 2 * TX + 2 * TY + 2) {
@@ -97,6 +101,7 @@ struct ColStencil {
       s.ring = true;
       s.ridx = (y + 1) * SW + x + 1;
       s.ra = ((long)(l * d.kall + k0) * g + (j0 + y)) * g + (i0 + x);
+      op.init_stage(((long)l * g + (j0 + y)) * g + (i0 + x), s.rsg);
     }
     return k1 - k0;
   }
@@ -106,11 +111,11 @@ struct ColStencil {
     real* buf = lds + (step & 1) * NS * P;
     real v[NS];
     if (s.outp) {
-      op.stage(s.k, s.a, v);
+      op.stage(s.k, s.a, s.sg, v);
       for (int f = 0; f < NS; ++f) buf[f * P + s.sidx] = v[f];
     }
     if (s.ring) {
-      op.stage(s.k, s.ra, v);
+      op.stage(s.k, s.ra, s.rsg, v);
       for (int f = 0; f < NS; ++f) buf[f * P + s.ridx] = v[f];
     }
   }
@@ -142,8 +147,10 @@ struct ViMainHC {
   struct Col {
     real c[3][7];
   };
+  struct Stg {};
+  NH_HD void init_stage(long, Stg&) const {}
   NH_HD bool stage_level(int k) const { return k >= d.kmin && k <= d.kmax; }
-  NH_HD void stage(int, long a, real* v) const {
+  NH_HD void stage(int, long a, const Stg&, real* v) const {
     for (int f = 0; f < kNS; ++f) v[f] = pk.stage_ptr(f, 0)[a];
   }
   NH_HD void init_col(long ci, Col& col) const {
@@ -200,8 +207,10 @@ struct LapC {
   struct Col {
     real c[7];
   };
+  struct Stg {};
+  NH_HD void init_stage(long, Stg&) const {}
   NH_HD bool stage_level(int) const { return true; }
-  NH_HD void stage(int, long a, real* v) const {
+  NH_HD void stage(int, long a, const Stg&, real* v) const {
     for (int f = 0; f < NFLD; ++f) v[f] = in[f][a];
   }
   NH_HD void init_col(long ci, Col& col) const {
@@ -218,6 +227,118 @@ struct LapC {
       for (int m = 1; m < 7; ++m) r = r + col.c[m] * nb(m, f);
       out[f][a] = -r;
     }
+  }
+};
+
+// momentum advection and Coriolis (AdvMomP) as a column operator: the 21
+// divergence coefficients and the column's position in registers; staged per
+// level are the merged velocity v + w_full r/|r| (formed once per point, where
+// the point kernel forms it once per neighbour) and the three momentum
+// components.  Term for term the point kernel's arithmetic; pole centres run
+// the point kernel.
+struct AdvMomC {
+  static constexpr const char* kName = "adv_mom";
+  static constexpr int kNS = 6;
+  AdvMomP pk;
+  Dims d;
+  struct Col {
+    real c[3][7];
+    real x[3];
+  };
+  struct Stg {
+    real x[3];
+  };
+  NH_HD void init_stage(long ci, Stg& sg) const {
+    for (int q = 0; q < 3; ++q) sg.x[q] = pk.G.gx[q * d.nreg + ci];
+  }
+  NH_HD bool stage_level(int k) const { return k >= d.kmin && k <= d.kmax; }
+  NH_HD void stage(int k, long a, const Stg& sg, real* v) const {
+    const long NF = d.NF, gg = (long)d.g * d.g;
+    const real* wd = pk.diag + I_W * NF;
+    const real R = pk.c.RADIUS;
+    const real wc = pk.G.cfact[k] * wd[a + gg] + pk.G.dfact[k] * wd[a];
+    v[0] = pk.diag[I_VX * NF + a] + wc * sg.x[0] / R;
+    v[1] = pk.diag[I_VY * NF + a] + wc * sg.x[1] / R;
+    v[2] = pk.diag[I_VZ * NF + a] + wc * sg.x[2] / R;
+    v[3] = pk.prog[1 * NF + a];
+    v[4] = pk.prog[2 * NF + a];
+    v[5] = pk.prog[3 * NF + a];
+  }
+  NH_HD void init_col(long ci, Col& col) const {
+    for (int dd = 0; dd < 3; ++dd)
+      for (int m = 0; m < 7; ++m) col.c[dd][m] = rc(pk.G.cdiv, d.nreg, dd * 7 + m, ci);
+    for (int q = 0; q < 3; ++q) col.x[q] = pk.G.gx[q * d.nreg + ci];
+  }
+  NH_HD void pole(int pb) const { pk(d.lall * d.kall + pb, 0); }
+  template <class NB>
+  NH_HD void point(int k, long a, const Col& col, NB nb) const {
+    const long NF = d.NF, ks = (long)d.g * d.g;
+    const int kmin = d.kmin, kmax = d.kmax;
+    real* out = pk.out;
+    if (k < kmin || k > kmax) {
+      for (int q = 0; q < 4; ++q) out[q * NF + a] = real(0.0);
+      return;
+    }
+    const Geo& G = pk.G;
+    const real R = pk.c.RADIUS;
+    const real* rw = pk.prog + 4 * NF;
+    const real* rg = pk.prog;
+    const real* vxd = pk.diag + I_VX * NF;
+    const real* vyd = pk.diag + I_VY * NF;
+    const real* vzd = pk.diag + I_VZ * NF;
+    const real* wd = pk.diag + I_W * NF;
+    const real* x0 = col.x;
+    auto vv3 = [&](long b, int kk, real* o) {
+      const real wc = G.cfact[kk] * wd[b + ks] + G.dfact[kk] * wd[b];
+      o[0] = vxd[b] + wc * x0[0] / R;
+      o[1] = vyd[b] + wc * x0[1] / R;
+      o[2] = vzd[b] + wc * x0[2] / R;
+    };
+    real sd[3][3];
+#pragma unroll
+    for (int m = 0; m < 7; ++m) {
+      const real vv[3] = {nb(m, 0), nb(m, 1), nb(m, 2)};
+      const real fx0 = nb(m, 3), fy0 = nb(m, 4), fz0 = nb(m, 5);
+      const real cx = col.c[0][m], cy = col.c[1][m], cz = col.c[2][m];
+      for (int q = 0; q < 3; ++q) {
+        const real tx = cx * (fx0 * vv[q] * real(1.0));
+        const real ty = cy * (fy0 * vv[q] * real(1.0));
+        const real tz = cz * (fz0 * vv[q] * real(1.0));
+        sd[0][q] = (m == 0) ? tx : sd[0][q] + tx;
+        sd[1][q] = (m == 0) ? ty : sd[1][q] + ty;
+        sd[2][q] = (m == 0) ? tz : sd[2][q] + tz;
+      }
+    }
+    real div[3];
+    for (int q = 0; q < 3; ++q) div[q] = sd[0][q] + sd[1][q] + sd[2][q];
+    const real vvm0[3] = {nb(0, 0), nb(0, 1), nb(0, 2)};
+    real hi[3] = {real(0.0), real(0.0), real(0.0)}, lo[3] = {real(0.0), real(0.0), real(0.0)};
+    if (k + 1 <= kmax) {
+      real up[3];
+      vv3(a + ks, k + 1, up);
+      const real wup = rw[a + ks];
+      for (int q = 0; q < 3; ++q)
+        hi[q] = real(0.0) * real(1.0) + wup * (G.afact[k + 1] * up[q] + G.bfact[k + 1] * vvm0[q]) * real(1.0);
+    }
+    if (k >= kmin + 1) {
+      real dn[3];
+      vv3(a - ks, k - 1, dn);
+      const real w0 = rw[a];
+      for (int q = 0; q < 3; ++q)
+        lo[q] = real(0.0) * real(1.0) + w0 * (G.afact[k] * vvm0[q] + G.bfact[k] * dn[q]) * real(1.0);
+    }
+    real dvv[3];
+    for (int q = 0; q < 3; ++q) dvv[q] = -div[q] - (hi[q] - lo[q]) * G.rdgz[k];
+    const real gxc = x0[0] / R, gyc = x0[1] / R, gzc = x0[2] / R;
+    const real rh = rg[a];
+    const real dx = dvv[0] - (-real(2.0) * rh * (pk.c.OHM * vvm0[1]));
+    const real dy = dvv[1] - (real(2.0) * rh * (pk.c.OHM * vvm0[0]));
+    const real dz = dvv[2];
+    const real prd = dx * gxc + dy * gyc + dz * gzc;
+    out[0 * NF + a] = dx - prd * gxc;
+    out[1 * NF + a] = dy - prd * gyc;
+    out[2 * NF + a] = dz - prd * gzc;
+    out[3 * NF + a] = prd * pk.c.ALPHA;
   }
 };
 

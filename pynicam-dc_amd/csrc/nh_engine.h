@@ -595,7 +595,18 @@ void stage_large(X& x, const Context& ctx, real* prog, const real* rhogq, double
     PrePostP f{d, ctx.c, ctx.G, prog, rhogq, diag, eth};
     x.run_pts(f, d);
   }
-  run_staged(x, AdvMomP{d, ctx.c, ctx.G, prog, diag, gadv});
+  // momentum advection: column tiles with 4-byte fields (gl07: 292 -> 189 us);
+  // with 8-byte fields their registers (236) cost more than the loads they save
+  // (466 -> 596 us), so the FP64 form keeps the level-staged point kernel
+  if (sizeof(real) == sizeof(float)) {
+    ColStencil<AdvMomC> cs;
+    cs.op.pk = AdvMomP{d, ctx.c, ctx.G, prog, diag, gadv};
+    cs.op.d = d;
+    cs.cg = st_geom(d, kColChunks);
+    x.run_cols(cs, cs.cg.nblocks, decltype(cs)::lds_elems(cs.cg));
+  } else {
+    run_staged(x, AdvMomP{d, ctx.c, ctx.G, prog, diag, gadv});
+  }
   hdiff_passes(x, ctx, prog, diag, sf);
   // divergence damping of the large-step momentum
   divdamp(x, ctx, prog + 1 * NF, prog + 2 * NF, prog + 3 * NF, prog + 4 * NF,
