@@ -240,15 +240,30 @@ def accuracy_leg(hipdyn, case, nsteps: int) -> dict:
     mod_bndcnd.py:521, over every regional interior column and the poles), the
     largest and RMS difference of each prognostic relative to its largest
     value, and each form's drift of dry mass and total energy (compensated
-    global sums, Embudget._store_var, mod_embudget.py:82-139)."""
+    global sums, Embudget._store_var, mod_embudget.py:82-139, which leave the
+    pole cells out) and of the dry mass over every cell, poles included."""
     import numpy as np
     t = case.topo
     g = t.gall_1d
+    geo = case.geo
+    ks = slice(t.kmin, t.kmax + 1)
+
+    def dry_mass_all_cells(s):
+        """Dry mass over every prognostic cell: the regional interior and the two
+        pole cells (the budget's _gsum leaves the pole cells out, so its dry mass
+        moves with the exchange across their faces)."""
+        P, Pp, Q, Qp = s["PROG"], s["PROG_pl"], s["PROGq"], s["PROGq_pl"]
+        reg = P[1:g - 1, 1:g - 1, ks, :, 0] - Q[1:g - 1, 1:g - 1, ks, :, 0]
+        pl = Pp[0, ks, :, 0] - Qp[0, ks, :, 0]
+        return float((reg.astype(np.float64) * geo.VOLUME[1:g - 1, 1:g - 1, ks]).sum()
+                     + (pl.astype(np.float64) * geo.VOLUME_pl[0, ks]).sum())
+
     out, ps, st = {}, {}, {}
     for f in ("float64", "mixed"):
         ctx = hipdyn.HipContext(case.geo, case.const, device="cuda:0", precision=f)
         try:
             dyn = hipdyn.HipDynamics(ctx, case.state)
+            m0 = dry_mass_all_cells(dyn.state())
             b0 = dyn.budget()
             for n in range(nsteps):
                 dyn.dynamics_step()
@@ -260,7 +275,8 @@ def accuracy_leg(hipdyn, case, nsteps: int) -> dict:
             st[f] = dyn.state()
         finally:
             ctx.close()
-        out[FORMS[f][0]] = {"dry_mass_drift_rel": (b1["dry"] - b0["dry"]) / b0["dry"],
+        out[FORMS[f][0]] = {"dry_mass_drift_rel_all_cells": (dry_mass_all_cells(st[f]) - m0) / m0,
+                            "dry_mass_drift_rel": (b1["dry"] - b0["dry"]) / b0["dry"],
                             "total_energy_drift_rel": (b1["tot"] - b0["tot"]) / b0["tot"],
                             "budget_start": b0, "budget_end": b1}
     nreg = t.lall * g * g
@@ -333,15 +349,19 @@ def run_form(hipdyn, case, precision: str, steps: int, warmup: int, candidates, 
     return ms, prof, dom, tot / n, st, budget
 
 
-def adjoint_leg(hipdyn, precision: str, nsteps: int = 18, reps: int = 2, ref=None) -> dict:
+def adjoint_leg(hipdyn, precision: str, nsteps: int = 18, reps: int = 2, ref=None, store: str = "tape",
+                same=None) -> dict:
     """Initial-state sensitivity at gl06 z40 (dt 600 s): J = sum over the 30-50N,
     0-20E box of surface pressure squared after ``nsteps`` steps, its gradient
-    through the adjoint window (per-step tapes of the forward steps; the
-    adjoint steps recompute only the diagnostics).  Times the plain forward steps, the checkpointed forward
+    through the adjoint window.  ``store="tape"``: per-step tapes of the forward
+    steps, the adjoint steps recompute only the diagnostics; ``"stages"``: only
+    the three RK-stage inputs of every step are kept and the adjoint recomputes
+    each stage.  Times the plain forward steps, the checkpointed forward
     window and the backward pass (best of ``reps``), checks the gradient of the
     FP64 form by a central finite difference along one seeded direction (the
-    mixed form's against the FP64 gradient ``ref``), and reports the device
-    memory in use at the end of the backward pass (hipMemGetInfo)."""
+    mixed form's against the FP64 gradient ``ref``; a "stages" run's against the
+    same form's "tape" gradient ``same``), and reports the device memory in use
+    with the window's checkpoints alive (hipMemGetInfo)."""
     import numpy as np
     import torch
     adj = pkg("adjoint")
@@ -369,7 +389,7 @@ def adjoint_leg(hipdyn, precision: str, nsteps: int = 18, reps: int = 2, ref=Non
             x = x0.clone().requires_grad_(True)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            ps = adj.sfc_pressure(dyn, adj.forecast(dyn, x, nsteps))
+            ps = adj.sfc_pressure(dyn, adj.forecast(dyn, x, nsteps, store=store))
             J = (m * ps * ps).sum()
             torch.cuda.synchronize()
             t1 = time.perf_counter()
@@ -399,7 +419,11 @@ def adjoint_leg(hipdyn, precision: str, nsteps: int = 18, reps: int = 2, ref=Non
         v = torch.from_numpy(rng.standard_normal(tuple(x0.shape))).to(x0.device, x0.dtype) * sc
         ad = float((grad.double() * v.double()).sum())
         check = {}
-        if ref is None:
+        if same is not None:
+            gs = same["grad"]
+            per = (grad.double() - gs).abs().amax(dim=1) / gs.abs().amax(dim=1)
+            check = {"vs_tape_max_rel_err": float(per.max())}
+        elif ref is None:
             # the step size balances truncation (eps^2) against round-off
             eps = 1e-6
 
@@ -424,12 +448,13 @@ def adjoint_leg(hipdyn, precision: str, nsteps: int = 18, reps: int = 2, ref=Non
     finally:
         ctx.close()
     res = {"workload": f"gl06 z40, dt 600 s, {nsteps} steps, J = sum over 30-50N 0-20E of ps^2",
+           "store": store,
            "forward_ms_per_step": 1000.0 * t_plain / nsteps,
            "window_forward_ms_per_step": 1000.0 * t_fwd / nsteps,
            "adjoint_ms_per_step": 1000.0 * t_bwd / nsteps,
            "adjoint_over_forward": t_bwd / t_plain,
            "J": Jv, **check,
-           "tape_GB_per_step": tape_gb, "peak_device_mem_GB": mem["peak"]}
+           "tape_GB_per_step": tape_gb if store == "tape" else None, "peak_device_mem_GB": mem["peak"]}
     print(f"bench: adjoint {precision}: " + json.dumps(res), file=sys.stderr, flush=True)
     res["grad"] = grad.double()
     return res
@@ -565,6 +590,8 @@ def main():
             if f == "float64":
                 ref = r
             adj[FORMS[f][0]] = r
+            # per-step checkpoints of the stage inputs only, each stage recomputed
+            adj[FORMS[f][0] + "_stages"] = adjoint_leg(hipdyn, f, store="stages", same=r)
         for r in adj.values():
             r.pop("grad", None)
         out["adjoint"] = adj

@@ -119,12 +119,29 @@ def load(path: str | None = None):
             f"nicamhip library not found at {path}; build it first "
             "(python -c 'import __graft_entry__ as g; g.build()')")
     lib = C.CDLL(path)
+    missing = []
     for name, (res, args) in EXPORTS.items():
-        fn = getattr(lib, name)
+        try:
+            fn = getattr(lib, name)
+        except AttributeError:
+            missing.append(name)
+            continue
         fn.restype = res
         fn.argtypes = args
+    if "nh_create_ex" in missing or "nh_last_error" in missing:
+        raise RuntimeError(f"{path} is not a nicamhip library (no nh_create_ex / nh_last_error)")
+    for name in missing:
+        # a library built from older sources: the entry point fails when called
+        setattr(lib, name, _missing_entry(path, name))
     _cache[path] = lib
     return lib
+
+
+def _missing_entry(path: str, name: str):
+    def call(*_args):
+        raise RuntimeError(f"{path} has no entry point {name}; rebuild it "
+                           "(python -c 'import __graft_entry__ as g; g.build()')")
+    return call
 
 
 def check(lib, rc: int, what: str = ""):
