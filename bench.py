@@ -580,6 +580,12 @@ def main():
         nacc = args.accuracy_steps or int(round(86400.0 / dt))
         out["accuracy"] = accuracy_leg(hipdyn, case, nacc)
         print("bench: accuracy " + json.dumps(out["accuracy"]), file=sys.stderr, flush=True)
+        # the same day with the FP64 hydrostatic basic state (ref_type 'INPUT': the
+        # resting atmosphere), which the mixed form subtracts in FP64
+        case_bs = pkg("setup").build_case(glevel=gl, vlayer=40, dtl=dt, seed=1, ref_type="INPUT")
+        out["accuracy_basic_state"] = accuracy_leg(hipdyn, case_bs, nacc)
+        print("bench: accuracy with the basic state " + json.dumps(out["accuracy_basic_state"]), file=sys.stderr,
+              flush=True)
     if args.sizing_glevel:
         out["sizing"] = sizing_leg(hipdyn, args.sizing_glevel)
     if not args.no_adjoint:

@@ -108,6 +108,13 @@ typedef struct {
    * (GRD_LAT/GRD_LON) and [v][lpl] at the poles, full- and half-level
    * heights gz, gzh [kall] (GRD_vz Z/ZH under flat topography) */
   const double *lat, *lon, *lat_pl, *lon_pl, *gz, *gzh;
+  /* hydrostatic basic state [kall] (Bsst.bsstate_setup with ref_type 'INPUT',
+   * mod_bsstate.py:20-90, 159-200, flat topography: k-profiles): pressure,
+   * temperature and density, kept in FP64 in both forms; the step forms
+   * pre - pre_bs and rho - rho_bs for the pressure gradient and buoyancy and
+   * tem - tem_bs, rho - rho_bs for the hyperdiffusion.  NULL = ref_type 'NOBASE'
+   * (zeros). */
+  const double *pre_bs, *tem_bs, *rho_bs;
 } nh_geometry;
 
 typedef struct nh_context nh_context;

@@ -41,6 +41,15 @@ class OracleContext:
         self.pnt_m0 = np.where(t.have_sgp, 0.0, 1.0)
         self.pnt_m1 = np.where(t.have_sgp, 1.0, 0.0)
 
+    def bs(self, name, reg):
+        """Basic-state array (Bsst.pre_bs / tem_bs / rho_bs, mod_bsstate.py:159-200)
+        of the regional or pole layout; 0.0 for ref_type NOBASE."""
+        b = getattr(self.geo, "bsstate", None)
+        if b is None:
+            return 0.0
+        full = b.full(name)
+        return full[0] if reg else full[1]
+
     # ------------------------------------------------------------------
     def halo(self, var, var_pl):
         """COMM_data_transfer (share/mod_comm.py:1563-1644) on one device."""
@@ -485,8 +494,9 @@ class OracleDynamics:
             DIAG[...] = np.stack([pre, tem, vx, vy, vz, w], axis=-1)
             self._bndcnd(DIAG, PROG, rho, ein, GSGAM2, PHI, C2W, GZ, kax)
             eth = ein + DIAG[..., I_pre] / rho
-            pregd = (DIAG[..., I_pre] - 0.0) * GSGAM2
-            rhogd = (rho - 0.0) * GSGAM2
+            pbs, rbs = ctx.bs("pre", reg), ctx.bs("rho", reg)
+            pregd = (DIAG[..., I_pre] - pbs) * GSGAM2
+            rhogd = (rho - rbs) * GSGAM2
             out["rho" + tag], out["ein" + tag], out["eth" + tag] = rho, ein, eth
             out["pregd" + tag], out["rhogd" + tag], out["q" + tag] = pregd, rhogd, q
         return out
@@ -600,9 +610,9 @@ class OracleDynamics:
         rhog, rhog_p = self.PROG[..., I_RHOG], self.PROG_pl[..., I_RHOG]
         Kh, Kh_p = c.GAMMA_H, c.GAMMA_H
         vt = [D[..., I_vx].copy(), D[..., I_vy].copy(), D[..., I_vz].copy(), D[..., I_w].copy(),
-              D[..., I_tem] - 0.0, pp["rho"] - 0.0]
+              D[..., I_tem] - ctx.bs("tem", True), pp["rho"] - ctx.bs("rho", True)]
         vtp = [Dp[..., I_vx].copy(), Dp[..., I_vy].copy(), Dp[..., I_vz].copy(), Dp[..., I_w].copy(),
-               Dp[..., I_tem] - 0.0, pp["rho_pl"] - 0.0]
+               Dp[..., I_tem] - ctx.bs("tem", False), pp["rho_pl"] - ctx.bs("rho", False)]
         khf = np.full_like(rhog, Kh)
         khf_p = np.full_like(rhog_p, Kh_p)
         nlap = c.LAP_ORDER_HDIFF

@@ -210,8 +210,8 @@ struct LapC {
   struct Stg {};
   NH_HD void init_stage(long, Stg&) const {}
   NH_HD bool stage_level(int) const { return true; }
-  NH_HD void stage(int, long a, const Stg&, real* v) const {
-    for (int f = 0; f < NFLD; ++f) v[f] = in[f][a];
+  NH_HD void stage(int k, long a, const Stg&, real* v) const {
+    for (int f = 0; f < NFLD; ++f) v[f] = pk.input(f, a, k);
   }
   NH_HD void init_col(long ci, Col& col) const {
     for (int m = 0; m < 7; ++m) col.c[m] = rc(pk.G.clap, d.nreg, m, ci);

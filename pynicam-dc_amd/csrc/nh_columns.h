@@ -139,6 +139,12 @@ struct Geo : VProf<real> {
   const double* lon_pl;   // v*2+lp
   // the vertical profiles in FP64
   VProf<double> v64;
+  // hydrostatic basic state (Bsst.pre_bs / tem_bs / rho_bs, mod_bsstate.py:159),
+  // FP64 k-profiles in every form; all zero for ref_type NOBASE
+  const double* pre_bs;
+  const double* tem_bs;
+  const double* rho_bs;
+  bool bs_on;
 };
 
 enum { I_RHOG = 0, I_RHOGVX, I_RHOGVY, I_RHOGVZ, I_RHOGW, I_RHOGE };

@@ -137,7 +137,7 @@ inline long geo_total(const Dims& d) {
   return t;
 }
 
-inline long geo64_total(const Dims& d) { return (long)kNVert * d.kall + kNReg64 * d.nreg + kNPl64 * 12L; }
+inline long geo64_total(const Dims& d) { return (long)kNVert * d.kall + kNReg64 * d.nreg + kNPl64 * 12L + 3L * d.kall; }
 
 // copy n doubles to the device in the field precision
 template <class X>
@@ -224,6 +224,20 @@ int upload_geometry(X& x, Context& ctx, const nh_geometry* g) {
     x.h2d(q, pl64[i], 12 * sizeof(double));
     *pl64dst[i] = q;
     q += 12;
+  }
+  // basic state (zeros when absent)
+  {
+    const double* bsrc[3] = {g->pre_bs, g->tem_bs, g->rho_bs};
+    const double** bdst[3] = {&G.pre_bs, &G.tem_bs, &G.rho_bs};
+    std::vector<double> zero((size_t)d.kall, 0.0);
+    G.bs_on = false;
+    for (int i = 0; i < 3; ++i) {
+      const double* src = bsrc[i] ? bsrc[i] : zero.data();
+      for (int k = 0; k < d.kall; ++k) G.bs_on = G.bs_on || src[k] != 0.0;
+      x.h2d(q, src, d.kall * sizeof(double));
+      *bdst[i] = q;
+      q += d.kall;
+    }
   }
   x.h2d(ctx.int_buf, g->sgp, d.lall * sizeof(int));
   G.sgp = ctx.int_buf;
@@ -540,6 +554,10 @@ void hdiff_passes(X& x, const Context& ctx, const real* prog, const real* diag, 
   f.in[3] = diag + I_W * NF; f.in[4] = diag + I_TEM * NF; f.in[5] = prog;
   for (int q = 0; q < 6; ++q) f.out[q] = sf.hda + q * NF;
   f.rhog = prog;
+  if (ctx.G.bs_on) {
+    f.sub[4] = ctx.G.tem_bs;
+    f.sub[5] = ctx.G.rho_bs;
+  }
   const ColGeom cg = col_geom(d, kColChunks);
   const StGeom sg = st_geom(d, kColChunks);
   {

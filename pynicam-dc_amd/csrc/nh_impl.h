@@ -277,6 +277,7 @@ int v_dyn_step_adj_tape(Context* c, const void* tape, const void* progq, int nq,
   if (sstep_max < 3 || sstep_max > 64) return failv(err, NH_ERR_ARG, "sstep_max out of range for the adjoint");
   if (!c->halo_adj_ok)
     return failv(err, NH_ERR_STATE, "halo map has a destination listed twice or used as a source");
+  if (c->G.bs_on) return failv(err, NH_ERR_STATE, "the adjoint supports the NOBASE basic state only");
   NHX x(c->device, stream, &c->prof);
   if (int rc = adj_workspace(x, c, 0, err)) return rc;
   int rc = dyn_step_adj_tape(x, *c, tape, (const real*)progq, nq, dtl, sstep_max, (real*)prog_b);
@@ -290,6 +291,7 @@ int v_dyn_step_adj(Context* c, const void* prog, const void* progq, int nq, doub
   if (sstep_max < 3 || sstep_max > 64) return failv(err, NH_ERR_ARG, "sstep_max out of range for the adjoint");
   if (!c->halo_adj_ok)
     return failv(err, NH_ERR_STATE, "halo map has a destination listed twice or used as a source");
+  if (c->G.bs_on) return failv(err, NH_ERR_STATE, "the adjoint supports the NOBASE basic state only");
   NHX x(c->device, stream, &c->prof);
   if (int rc = adj_workspace(x, c, sstep_max + 1, err)) return rc;
   int rc = dyn_step_adj(x, *c, (const real*)prog, (const real*)progq, nq, dtl, sstep_max, (real*)prog_b,

@@ -333,13 +333,19 @@ struct HDiffPassP {
   const real* in[6];
   real* out[6];
   const real* rhog;
+  // first pass: FP64 k-profile subtracted from input f (tem - tem_bs,
+  // rho - rho_bs, mod_numfilter.py:1068-1076), or none
+  const double* sub[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   NH_HD const real* stage_ptr(int f, int) const { return in[f]; }
   NH_HD bool stage_level(int) const { return true; }
+  NH_HD real input(int f, long b, int k) const {
+    return sub[f] ? (real)((double)in[f][b] - sub[f][k]) : in[f][b];
+  }
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
     if (!LAST) {
-      at(p, [&](int f, long b) { return in[f][b]; });
+      at(p, [&](int f, long b) { return input(f, b, p.k); });
       return;
     }
     const long a = p.a;
@@ -579,6 +585,10 @@ struct ViPath0P {
     };
     auto RG = [&](int kk) { return win(Wrg, kk); };
     auto PR = [&](int kk) { return win(Wpr, kk); };
+    // pregd = (pre - pre_bs) GSGAM2, rhogd = (rho - rho_bs) GSGAM2 with the basic
+    // state in FP64 (mod_dynamics.py:1056-1057); GSGAM2 = 1
+    auto PRD = [&](int kk) { return (real)((double)PR(kk) - G.pre_bs[kk]) * real(1.0); };
+    auto RGD = [&](int kk) { return (real)((double)(RG(kk) / real(1.0)) - G.rho_bs[kk]) * real(1.0); };
     auto ET = [&](int kk) { return win(Wet, kk); };
     auto RW = [&](int kk) { return win(Wrw, kk); };
     auto rhog_h = [&](int kk) {
@@ -586,11 +596,11 @@ struct ViPath0P {
       return G.c2w0[k2] * RG(k2) + G.c2w1[k2] * RG(k2 - 1);
     };
     auto dpgw = [&](int kk) {
-      return (kk >= kmin + 1 && kk <= kmax) ? real(1.0) * (PR(kk) * real(1.0) - PR(kk - 1) * real(1.0)) * G.rdgzh[kk] : real(0.0);
+      return (kk >= kmin + 1 && kk <= kmax) ? real(1.0) * (PRD(kk) * real(1.0) - PRD(kk - 1) * real(1.0)) * G.rdgzh[kk] : real(0.0);
     };
     auto dbuo = [&](int kk) {
       return (kk >= kmin + 1 && kk <= kmax)
-                 ? -c.GRAV * (G.c2w0[kk] * (RG(kk) / real(1.0) * real(1.0)) + G.c2w1[kk] * (RG(kk - 1) / real(1.0) * real(1.0)))
+                 ? -c.GRAV * (G.c2w0[kk] * RGD(kk) + G.c2w1[kk] * RGD(kk - 1))
                  : real(0.0);
     };
     auto gzt = [&](int kk) { return c.GRAV - (dpgw(kk) - dbuo(kk)) / rhog_h(kk); };
@@ -651,7 +661,11 @@ struct ViPath0P {
       const real ws_lo = (k >= kmin + 1)
           ? real(0.0) * real(1.0) + real(1.0) * (RW(k) * (G.afact[k] * ET(k) + G.bfact[k] * ET(k - 1))) * real(1.0) : real(0.0);
       drhoge = -he - (ws_hi - ws_lo) * G.rdgz[k];
-      presgrad_ld(p, ld, R, dpg);
+      // the gradient of pregd at the level the pressure is read from (kmin -> kmin+1)
+      const int kg = (k == kmin) ? kmin + 1 : k;
+      presgrad_ld(p, [&](int f, long b) {
+        return f == 4 ? (real)((double)ld(4, b) - G.pre_bs[kg]) * real(1.0) : ld(f, b);
+      }, R, dpg);
       pw = diag[I_VX * NF + a] * dpg[0] + diag[I_VY * NF + a] * dpg[1] + diag[I_VZ * NF + a] * dpg[2]
          + G.w2c0[k] * pwh(k + 1) + G.w2c1[k] * pwh(k);
     }
@@ -676,12 +690,14 @@ struct ViPath0P {
       };
       auto dpgw64 = [&](int kk) {
         return (kk >= kmin + 1 && kk <= kmax)
-                   ? 1.0 * ((double)PR(kk) * 1.0 - (double)PR(kk - 1) * 1.0) * V.rdgzh[kk] : 0.0;
+                   ? 1.0 * (((double)PR(kk) - G.pre_bs[kk]) * 1.0 - ((double)PR(kk - 1) - G.pre_bs[kk - 1]) * 1.0)
+                         * V.rdgzh[kk]
+                   : 0.0;
       };
       auto dbuo64 = [&](int kk) {
         return (kk >= kmin + 1 && kk <= kmax)
-                   ? -C.GRAV * (V.c2w0[kk] * ((double)RG(kk) / 1.0 * 1.0) +
-                                V.c2w1[kk] * ((double)RG(kk - 1) / 1.0 * 1.0))
+                   ? -C.GRAV * (V.c2w0[kk] * (((double)RG(kk) / 1.0 - G.rho_bs[kk]) * 1.0) +
+                                V.c2w1[kk] * (((double)RG(kk - 1) / 1.0 - G.rho_bs[kk - 1]) * 1.0))
                    : 0.0;
       };
       auto gzt64 = [&](int kk) { return C.GRAV - (dpgw64(kk) - dbuo64(kk)) / rhog_h64(kk); };

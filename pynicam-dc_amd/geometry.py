@@ -140,6 +140,7 @@ class Geometry:
 
     def __init__(self, topo: Topology, hgrid: HGrid, vgrid: VGrid, radius: float, grav: float):
         self.topo, self.vgrid = topo, vgrid
+        self.bsstate = None     # basic state (bsstate.BasicState) or None for NOBASE
         self.radius = radius
         self.grav = grav
         g = topo.gall_1d

@@ -141,4 +141,9 @@ def geometry_arrays(geo) -> dict:
               "lat_pl", "lon_pl"):
         out[k] = np.ascontiguousarray(out[k], dtype=np.float64)
     out.update({k: np.ascontiguousarray(v, dtype=np.float64) for k, v in vertical_profiles(geo).items()})
+    # basic state k-profiles (zeros for NOBASE)
+    bs = getattr(geo, "bsstate", None)
+    kall = geo.topo.kall
+    for k, name in (("pre_bs", "pre"), ("tem_bs", "tem"), ("rho_bs", "rho")):
+        out[k] = np.ascontiguousarray(getattr(bs, name) if bs is not None else np.zeros(kall), dtype=np.float64)
     return out

@@ -35,7 +35,7 @@ _VERT_FIELDS = ["afact", "bfact", "cfact", "dfact", "rdgz", "rdgzh", "c2w0", "c2
 
 
 _TRC_FIELDS = ["tri_w", "arc_hn", "rarea", "arc_xr", "tri_w_pl", "arc_hn_pl", "rarea_pl", "arc_xr_pl",
-               "area", "dgz", "lat", "lon", "lat_pl", "lon_pl", "gz", "gzh"]
+               "area", "dgz", "lat", "lon", "lat_pl", "lon_pl", "gz", "gzh", "pre_bs", "tem_bs", "rho_bs"]
 
 
 class NhGeometry(C.Structure):

@@ -41,6 +41,7 @@ class ReferenceGeometry:
         if (t.gall_1d, t.kall, t.lall) != (int(adm.ADM_gall_1d), int(adm.ADM_kall), int(adm.ADM_lall)):
             raise ValueError("the reference's Adm dimensions differ from the halo topology's")
         self.vgrid = _VGrid(grd)
+        self.bsstate = None     # set a bsstate.BasicState for ref_type 'INPUT'
         self.radius = float(cnst.CONST_RADIUS)
         self.grav = float(cnst.CONST_GRAV)
         k0 = 0

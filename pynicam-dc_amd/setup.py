@@ -8,7 +8,7 @@ from __future__ import annotations
 
 from dataclasses import dataclass
 
-from . import config, geometry, grid, initial, topology
+from . import bsstate, config, geometry, grid, initial, topology
 
 
 @dataclass
@@ -40,7 +40,9 @@ def make_hgrid(topo: topology.Topology, kind="standard"):
 
 def build_case(glevel: int, vlayer: int = 40, ztop: float = 30000.0, dz_bottom: float = 100.0,
                dtl: float = 1200.0, seed: int = 0, rlevel: int = 1, with_state: bool = True,
-               hgrid="standard") -> Case:
+               hgrid="standard", ref_type: str = "NOBASE", ref_fname: str | None = None) -> Case:
+    """``ref_type`` / ``ref_fname``: the basic state ([bsstateparam], mod_bsstate.py:20-90);
+    with 'INPUT' and no file the benchmark's resting atmosphere (``bsstate.rest_profile``)."""
     topo = topology.Topology(glevel, rlevel, vlayer)
     hg = make_hgrid(topo, hgrid)
     zh = grid.stretched_interfaces(vlayer, ztop, dz_bottom)
@@ -48,5 +50,9 @@ def build_case(glevel: int, vlayer: int = 40, ztop: float = 30000.0, dz_bottom: 
     vg = grid.VGrid(gz, gzh)
     c = config.benchmark_const(glevel, dtl)
     geo = geometry.Geometry(topo, hg, vg, c.RADIUS, c.GRAV)
+    if ref_type == "INPUT" and not ref_fname:
+        geo.bsstate = bsstate.BasicState(geo, c, *bsstate.rest_profile(geo, c))
+    else:
+        geo.bsstate = bsstate.bsstate_setup(geo, c, ref_type, ref_fname)
     state = initial.baroclinic_bumps_state(geo, c, seed=seed) if with_state else None
     return Case(topo, vg, c, geo, state)
