@@ -214,7 +214,8 @@ void stage_large_adj(X& x, const Context& ctx, const real* prog, const real* rho
     real* vb[4] = {prog_b + NF, prog_b + 2 * NF, prog_b + 3 * NF, prog_b + 4 * NF};
     divdamp_adj(x, ctx, ob, vb);
   }
-  halo_block_adj(x, ctx, afield(ctx, A_HDBB), 6);
+  // the forward step fills no halo of pass 2's output (path 0 reads it at its
+  // compute points only), so its bars stay zero off the compute points
   {
     HDiffPass2A f;
     f.d = d; f.c = ctx.c; f.G = ctx.G;

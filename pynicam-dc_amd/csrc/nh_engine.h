@@ -542,10 +542,13 @@ inline StageFields ws_stage(const Context& ctx) {
 }
 
 // hyperdiffusion (numfilter_hdiffusion, mod_numfilter.py:845-1469, lap order 2):
-// the iterated Laplacians of (vx, vy, vz, w, T, rho) in two passes, a halo fill
-// after each; pass 1 into sf.hda, pass 2 into sf.hdb (6 fields each)
+// the iterated Laplacians of (vx, vy, vz, w, T, rho) in two passes, pass 1
+// into sf.hda with a halo fill, pass 2 into sf.hdb (6 fields each).  The halo
+// fill of pass 2 (fill_hdb) is for callers that read hdb off the compute
+// points; the step reads it only at its own compute points (path 0).
 template <class X>
-void hdiff_passes(X& x, const Context& ctx, const real* prog, const real* diag, const StageFields& sf) {
+void hdiff_passes(X& x, const Context& ctx, const real* prog, const real* diag, const StageFields& sf,
+                  bool fill_hdb = true) {
   const Dims& d = ctx.d;
   const long NF = d.NF;
   HDiffPassP<0> f;
@@ -595,7 +598,7 @@ void hdiff_passes(X& x, const Context& ctx, const real* prog, const real* diag, 
     k2.rhog = prog;
     x.run_cols(k2, cg.nblocks, DiffKhK::lds_elems(cg));
   }
-  halo_block(x, ctx, sf.hdb, 6);
+  if (fill_hdb) halo_block(x, ctx, sf.hdb, 6);
 }
 
 // large-step part of RK stage nl: diagnostics and boundary values (in place
@@ -625,7 +628,7 @@ void stage_large(X& x, const Context& ctx, real* prog, const real* rhogq, double
   } else {
     run_staged(x, AdvMomP{d, ctx.c, ctx.G, prog, diag, gadv});
   }
-  hdiff_passes(x, ctx, prog, diag, sf);
+  hdiff_passes(x, ctx, prog, diag, sf, false);
   // divergence damping of the large-step momentum
   divdamp(x, ctx, prog + 1 * NF, prog + 2 * NF, prog + 3 * NF, prog + 4 * NF,
           ctx.field(W_DD), ctx.field(W_DD + 1), ctx.field(W_DD + 2));
@@ -822,7 +825,9 @@ int dyn_step(X& x, const Context& ctx, real* prog, real* progq, int nq, double d
       s.we[0] = old[0]; s.dv[0] = old[1]; s.dv[1] = old[2]; s.dv[2] = old[3]; s.we[1] = old[4]; s.we[2] = old[5];
     }
     prog_update(x, ctx, prog, s.ps);
-    halo_block(x, ctx, ctx.field(W_PM), 5);
+    // the mean fluxes feed the tracer transport after the last stage; each
+    // stage restarts them at every point (EthTotPMP)
+    if (nl == 2) halo_block(x, ctx, ctx.field(W_PM), 5);
     if (nl < 2) halo_block(x, ctx, prog, 6);
     int rc = x.check("dyn_step stage");
     if (rc) return rc;
