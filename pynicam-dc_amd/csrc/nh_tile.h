@@ -357,6 +357,7 @@ struct DiffKhK {
   struct State {
     real ci[2][9];
     real cd[18];
+    real rg[7];         // rhog at the tile point and its neighbours, loaded before the barrier
     long a;
     int k, lidx;
     bool corner, outp, pent, sgp;
@@ -402,6 +403,13 @@ struct DiffKhK {
     const long g = d.g;
     const int P = (cg.TX + 1) * (cg.TY + 1);
     real* buf = lds + (step & 1) * 12 * P;
+    if (s.outp) {
+      // the point's density loads go out with the triangle loads, so that
+      // level_b does not wait for memory after the barrier
+      const long a = s.a;
+      s.rg[0] = rhog[a]; s.rg[1] = rhog[a + 1 + g]; s.rg[2] = rhog[a + g]; s.rg[3] = rhog[a - 1];
+      s.rg[4] = rhog[a - 1 - g]; s.rg[5] = rhog[a - g]; s.rg[6] = rhog[a + 1];
+    }
     for (int f = 0; f < 2; ++f) {
       const real* sf = in[4 + f];
       const real v00 = sf[s.a], v10 = sf[s.a + 1], v11 = sf[s.a + 1 + g], v01 = sf[s.a + g];
@@ -426,8 +434,7 @@ struct DiffKhK {
       const int x = s.lidx;
       // corners of TI(i,j) TJ(i,j) TI(i-1,j) TJ(i-1,j-1) TI(i-1,j-1) TJ(i,j-1)
       const int cx[6] = {x, x, x - 1, x - 1 - W1, x - 1 - W1, x - W1}, cT[6] = {0, 1, 0, 1, 0, 1};
-      const real r0 = rhog[a], r1 = rhog[a + 1 + g], r2 = rhog[a + g], r3 = rhog[a - 1], r4 = rhog[a - 1 - g],
-                 r5 = rhog[a - g], r6 = rhog[a + 1];
+      const real r0 = s.rg[0], r1 = s.rg[1], r2 = s.rg[2], r3 = s.rg[3], r4 = s.rg[4], r5 = s.rg[5], r6 = s.rg[6];
       const real m0 = s.sgp ? real(0.0) : real(1.0), m1 = s.sgp ? real(1.0) : real(0.0);
       for (int f = 0; f < 2; ++f) {
         const real fac = (f == 0) ? c.CVdry : c.HDIFF_FACT_RHO;
