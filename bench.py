@@ -95,7 +95,7 @@ KERNEL_TYPES = {
     "ProgUpdateP": "prog_update", "HaloGather": "halo_gather", "TrcVert1": "trc_vert1", "TrcVert2": "trc_vert2",
     "TrcHFlux": "trc_hflux", "TrcRhog2": "trc_rhog2", "TrcGradQ": "trc_gradq", "TrcRemap": "trc_remap",
     "TrcQout": "trc_qout", "TrcApply": "trc_apply", "DivDampK<1>": "divdamp_3d", "DivDampK<2>": "divdamp_post",
-    "ViMainHC": "vi_main_h", "DiffKhK": "hdiff_pass2_kh",
+    "ViMainHC": "vi_main_h", "DiffKhK": "hdiff_pass2_kh", "AdvMomC": "adv_mom",
 }
 
 
@@ -130,7 +130,7 @@ def hbm_traffic(glevel: int, precision: str, timeout: float = 300.0):
             cmd = ["rocprofv3", "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "pmc", "--",
                    sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0",
                    "--glevel", str(glevel), "--precision", precision, "--no-cpu-baseline", "--no-traffic",
-                   "--no-adjoint", "--copy-probe"]
+                   "--no-adjoint", "--no-accuracy", "--copy-probe"]
             print(f"bench: counter pass {ctr}", file=sys.stderr, flush=True)
             try:
                 r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), capture_output=True,
