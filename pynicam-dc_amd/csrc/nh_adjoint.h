@@ -955,37 +955,33 @@ struct ViMainHA {
 
 // ----------------------------------------------------------------------------
 // small step vi_main, vertical part (ViMainV): the transposed tridiagonal
-// solve, one thread per compute column, in FP64.  x is the forward solution
-// (the new rhogw split as stored; the matrix bars are -y x^T on the band).
-// NK > 0 (the number of unknowns) keeps the elimination in registers; NK = 0
-// uses the FP64 column scratch U, Gm.
+// solve in FP64, x being the forward solution (the new rhogw split as stored;
+// the matrix bars are -y x^T on the band).  Two launches.  ViMainVAy: the
+// transposed Thomas sweeps, one thread per compute column, storing the
+// solution y of M^T y = xbar into Y at levels kmin+1..kmax (NK > 0, the
+// number of unknowns, keeps the elimination in registers; NK = 0 uses the
+// FP64 column scratch Y, Gm).  ViMainVAb: one thread per compute point and
+// level kmin..kmax adds its own level's bars, from y and x there and from
+// level k+1's y (the source row at k+1 also reads level k), so every bar
+// element is read and written once, by one thread, and the levels run in
+// parallel.
 // ----------------------------------------------------------------------------
 template <int NK>
-struct ViMainVA {
+struct ViMainVAy {
   static constexpr const char* kName = "vi_main_v_adj";
   static constexpr int kUnroll = NK > 0 ? NK : 4;
-  Dims d; Consts c; Geo G;
-  double dt;
-  const real* x;       // forward solution (rhogw split after the step)
+  Dims d;
+  const real* out_b;   // bar of the new rhogw split
   const double* Mc; const double* Mu; const double* Ml;
-  double* U; double* Gm;
-  real* out_b;         // bar of the new rhogw split
-  real* ps_b[6];
-  real* gT_b;
-  double* Mc_b; double* Mu_b; double* Ml_b;
-  real* hv_b[3];
+  double* Y;           // y (and, for NK = 0, the elimination's u first)
+  double* Gm;          // NK = 0: the elimination's multipliers
   NH_HD void operator()(long t) const {
     Cell cl = cell_of(d, t);
     if (cl.pole ? cl.v != 0 : !cl.inner) return;
-    const long NF = d.NF, ks = cl.ks, b = cl.base;
+    const long ks = cl.ks, b = cl.base;
     const int kmin = d.kmin, kmax = d.kmax;
-    const VProf<double>& V = G.v64;
-    const ConstsT<double>& C = c.d;
-    const double RovCV = C.Rdry / C.CVdry;
-    const double CVovRt2 = C.CVdry / C.Rdry / (dt * dt);
     const int N = (NK > 0) ? NK : kmax - kmin;
     double ub[NK > 0 ? NK : 1], gb[NK > 0 ? NK : 1];
-    // elimination of M^T y = xbar; row n: Mu[n-1] y[n-1] + Mc[n] y[n] + Ml[n+1] y[n+1]
     long a = b + (long)kmin * ks;
     double beta = 0.0, u_prev = 0.0, mu_m = 0.0;
 #pragma unroll kUnroll
@@ -1005,62 +1001,85 @@ struct ViMainVA {
         else Gm[a] = gm;
       }
       if (NK > 0) ub[NK > 0 ? n : 0] = u;
-      else U[a] = u;
+      else Y[a] = u;
       u_prev = u;
       mu_m = Mu[a];
     }
-    // back substitution from the top; per level the bars of S = y and of the
-    // band.  The source at level k also reads level k-1: those bars are
-    // carried to the next (lower) level in registers, so each level's bars
-    // are written once.
-    double y = u_prev, x_up = 0.0;
-    double xk = (double)x[a];
-    double gm_next = 0.0;
-    double cE = 0.0, cH1 = 0.0, cG = 0.0, cT0 = 0.0, cH0 = 0.0;
+    double y = u_prev, gm_next = 0.0;
 #pragma unroll kUnroll
     for (int n = N - 1; n >= 0; --n) {
-      if (n < N - 1) y = ((NK > 0) ? ub[NK > 0 ? n : 0] : U[a]) - gm_next * y;
+      if (n < N - 1) y = ((NK > 0) ? ub[NK > 0 ? n : 0] : Y[a]) - gm_next * y;
       if (NK == 0 && n > 0) gm_next = Gm[a];
       if (NK > 0 && n > 0) gm_next = gb[NK > 0 ? n : 0];
-      const int k = kmin + 1 + n;
-      const long am = a - ks;
-      const double x_dn = (n >= 1) ? (double)ldc(&x[am]) : 0.0;
-      radd(&Mc_b[a], -y * xk);
-      if (n >= 1) radd(&Ml_b[a], -y * x_dn);
-      if (n <= N - 2) radd(&Mu_b[a], -y * x_up);
-      const double s = y * CVovRt2;
-      const double Ab = s, Bb = -s * dt * V.rdgzh[k], Cb = -s * dt * C.GRAV;
-      const double grk = Cb * V.afact[k] * dt, grm = Cb * V.bfact[k] * dt;
-      radd(&ps_b[I_RHOGW][a], (real)(Ab * C.ALPHA));
-      radd(&gT_b[4 * NF + a], (real)(Ab * dt));
-      radd(&ps_b[I_RHOGE][a], (real)(Bb * RovCV + cE));
-      radd(&hv_b[1][a], (real)(Bb * dt + cH1));
-      radd(&ps_b[I_RHOG][a], (real)(Cb * V.afact[k] + cG));
-      radd(&gT_b[a], (real)(grk + cT0));
-      radd(&hv_b[0][a], (real)(-grk + cH0));
-      cE = -Bb * RovCV;
-      cH1 = -Bb * dt;
-      cG = Cb * V.bfact[k];
-      cT0 = grm;
-      cH0 = -grm;
-      x_up = xk;
-      xk = x_dn;
-      a = am;
+      Y[a] = y;
+      a -= ks;
     }
-    // level kmin: the bars the lowest source sent down
-    radd(&ps_b[I_RHOGE][a], (real)cE);
-    radd(&hv_b[1][a], (real)cH1);
-    radd(&ps_b[I_RHOG][a], (real)cG);
-    radd(&gT_b[a], (real)cT0);
-    radd(&hv_b[0][a], (real)cH0);
   }
-  // *p += v for a bar element this column's thread owns.  Level k (n = k -
-  // kmin - 1) adds once each to Mc_b, Ml_b (n >= 1), Mu_b (n <= N - 2),
-  // ps_b[RHOGW], ps_b[RHOGE], ps_b[RHOG], hv_b[0], hv_b[1], gT_b[0] and
-  // gT_b[4] at its own level k; the contributions of the source row at k to
-  // level k - 1 are carried in registers (cE .. cH0) and added at k - 1 with
-  // that level's own, and level kmin takes the last carry.  So every element
-  // is read once, before its one store, by this thread only: ld_own.
+};
+
+struct ViMainVAb {
+  static constexpr const char* kName = "vi_main_v_adj_bars";
+  Dims d; Consts c; Geo G;
+  double dt;
+  const real* x;       // forward solution (rhogw split after the step)
+  const double* Y;
+  real* ps_b[6];
+  real* gT_b;
+  double* Mc_b; double* Mu_b; double* Ml_b;
+  real* hv_b[3];
+  NH_HD void operator()(int plane, int xi) const {
+    Pt p;
+    if (!pt_of(d, plane, xi, p) || !pt_compute(p)) return;
+    const int kmin = d.kmin, kmax = d.kmax, k = p.k;
+    if (k < kmin || k > kmax) return;
+    const long NF = d.NF, ks = p.ks, a = p.a;
+    const VProf<double>& V = G.v64;
+    const ConstsT<double>& C = c.d;
+    const double RovCV = C.Rdry / C.CVdry;
+    const double CVovRt2 = C.CVdry / C.Rdry / (dt * dt);
+    const int N = kmax - kmin;
+    // the terms the source row at level k + 1 adds at level k (zero above kmax)
+    double cE = 0.0, cH1 = 0.0, cG = 0.0, cT0 = 0.0, cH0 = 0.0;
+    if (k + 1 <= kmax) {
+      const double s1 = ldc(&Y[a + ks]) * CVovRt2;
+      const double Bb1 = -s1 * dt * V.rdgzh[k + 1], Cb1 = -s1 * dt * C.GRAV;
+      const double grm1 = Cb1 * V.bfact[k + 1] * dt;
+      cE = -Bb1 * RovCV;
+      cH1 = -Bb1 * dt;
+      cG = Cb1 * V.bfact[k + 1];
+      cT0 = grm1;
+      cH0 = -grm1;
+    }
+    if (k == kmin) {
+      // level kmin: the bars the lowest source sent down
+      radd(&ps_b[I_RHOGE][a], (real)cE);
+      radd(&hv_b[1][a], (real)cH1);
+      radd(&ps_b[I_RHOG][a], (real)cG);
+      radd(&gT_b[a], (real)cT0);
+      radd(&hv_b[0][a], (real)cH0);
+      return;
+    }
+    const int n = k - kmin - 1;
+    const double y = ldc(&Y[a]);
+    const double xk = (double)ldc(&x[a]);
+    const double x_dn = (n >= 1) ? (double)ldc(&x[a - ks]) : 0.0;
+    const double x_up = (n <= N - 2) ? (double)ldc(&x[a + ks]) : 0.0;
+    radd(&Mc_b[a], -y * xk);
+    if (n >= 1) radd(&Ml_b[a], -y * x_dn);
+    if (n <= N - 2) radd(&Mu_b[a], -y * x_up);
+    const double s = y * CVovRt2;
+    const double Ab = s, Bb = -s * dt * V.rdgzh[k], Cb = -s * dt * C.GRAV;
+    const double grk = Cb * V.afact[k] * dt;
+    radd(&ps_b[I_RHOGW][a], (real)(Ab * C.ALPHA));
+    radd(&gT_b[4 * NF + a], (real)(Ab * dt));
+    radd(&ps_b[I_RHOGE][a], (real)(Bb * RovCV + cE));
+    radd(&hv_b[1][a], (real)(Bb * dt + cH1));
+    radd(&ps_b[I_RHOG][a], (real)(Cb * V.afact[k] + cG));
+    radd(&gT_b[a], (real)(grk + cT0));
+    radd(&hv_b[0][a], (real)(-grk + cH0));
+  }
+  // every bar element of level k is added to once, by the thread of that
+  // point and level, which reads it once before its one store: ld_own
   template <class T>
   NH_HD static void radd(T* p, T v) { *p = ld_own(p) + v; }
 };

@@ -136,21 +136,26 @@ void small_step_adj(X& x, const Context& ctx, const real* prog, real* const ck[6
     for (f.color = 0; f.color < 4; ++f.color) x.run_pts_n(f, d, (d.lall + 2) * ViMainPostA::ncolor_levels(d, f.color));
   }
   {
-    auto vert = [&](auto f) {
-      f.d = d; f.c = ctx.c; f.G = ctx.G;
-      f.dt = dts;
-      f.x = nx[4];
-      f.Mc = sf.M; f.Mu = sf.M + NF; f.Ml = sf.M + 2 * NF;
-      f.U = ctx.dfield(D_U); f.Gm = ctx.dfield(D_GM);
+    // transposed solve: y per column, then the bars per point and level
+    auto sweep = [&](auto f) {
+      f.d = d;
       f.out_b = web[1];
-      for (int q = 0; q < 6; ++q) f.ps_b[q] = psb2[q];
-      f.gT_b = gTb;
-      f.Mc_b = adfield(ctx, AD_MCB); f.Mu_b = adfield(ctx, AD_MUB); f.Ml_b = adfield(ctx, AD_MLB);
-      for (int q = 0; q < 3; ++q) f.hv_b[q] = hvb[q];
+      f.Mc = sf.M; f.Mu = sf.M + NF; f.Ml = sf.M + 2 * NF;
+      f.Y = ctx.dfield(D_U); f.Gm = ctx.dfield(D_GM);
       x.run(f, ncols(d));
     };
-    if (d.kmax - d.kmin == 39) vert(ViMainVA<39>{});
-    else vert(ViMainVA<0>{});
+    if (d.kmax - d.kmin == 39) sweep(ViMainVAy<39>{});
+    else sweep(ViMainVAy<0>{});
+    ViMainVAb f;
+    f.d = d; f.c = ctx.c; f.G = ctx.G;
+    f.dt = dts;
+    f.x = nx[4];
+    f.Y = ctx.dfield(D_U);
+    for (int q = 0; q < 6; ++q) f.ps_b[q] = psb2[q];
+    f.gT_b = gTb;
+    f.Mc_b = adfield(ctx, AD_MCB); f.Mu_b = adfield(ctx, AD_MUB); f.Ml_b = adfield(ctx, AD_MLB);
+    for (int q = 0; q < 3; ++q) f.hv_b[q] = hvb[q];
+    x.run_pts(f, d);
   }
   {
     ViMainHA f;
