@@ -480,140 +480,189 @@ NH_HD inline __attribute__((always_inline)) void rhogkin_k_t(const VProf<real>& 
 
 // ----------------------------------------------------------------------------
 // momentum advection (AdvMomP): gadv bars -> bars of vx, vy, vz, w (diag),
-// rhog, rhogvx..vz, rhogw (prog)
+// rhog, rhogvx..vz, rhogw (prog), as three gathers, without atomics:
+//   AdvMomG1, every point: the bars ob are consumed; at a compute point of a
+//     physical level the bar of the advection tendency dvv (the tail's
+//     projection and ALPHA folded in) -> S, the Coriolis term's rhog bar into
+//     prog_b; S = 0 elsewhere;
+//   AdvMomG2, every point and level: the bars the point receives — from the
+//     divergence terms of the compute points whose stencil holds it, from its
+//     own tail (Coriolis, the vertical terms) and from the vertical terms of
+//     its column's levels k - 1 and k + 1 — into prog_b (rhogv, rhogw) and
+//     diag_b (v), and the w part of its merged-velocity bar -> W;
+//   AdvMomG3, every point and level: the w bars of the merged velocity's
+//     interpolation, from W at the level and the one below.
+// Each sum is a forward point's transposed terms, grouped by the point they
+// land on (a compute point's stencil neighbour m is the point it adds to with
+// that point's coefficient m).
 // ----------------------------------------------------------------------------
-struct AdvMomA {
-  static constexpr const char* kName = "adv_mom_adj";
+struct AdvMomG1 {
+  static constexpr const char* kName = "adv_mom_adj1";
   Dims d; Consts c; Geo G;
   const real* prog;
   const real* diag;
   real* ob;        // 4 fields: bars of out[0..3] (consumed)
-  real* prog_b;    // 6 fields
-  real* diag_b;    // 6 fields
+  real* prog_b;    // rhog bar (Coriolis)
+  real* S[3];      // out: the dvv bars
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p)) return;
     const long NF = d.NF, a = p.a, ks = p.ks;
-    const int kmin = d.kmin, kmax = d.kmax, k = p.k;
+    const int k = p.k;
     real o[4];
     for (int q = 0; q < 4; ++q) {
       o[q] = ob[q * NF + a];
       ob[q * NF + a] = real(0.0);
     }
-    if (!pt_compute(p) || k < kmin || k > kmax) return;
+    real dv[3] = {real(0.0), real(0.0), real(0.0)};
+    if (pt_compute(p) && k >= d.kmin && k <= d.kmax) {
+      const real R = c.RADIUS;
+      real x0[3];
+      point_pos(G, d, p, x0);
+      const real* wd = diag + I_W * NF;
+      const real wc = G.cfact[k] * wd[a + ks] + G.dfact[k] * wd[a];
+      real vvm0[3];
+      for (int q = 0; q < 3; ++q) vvm0[q] = diag[(I_VX + q) * NF + a] + wc * x0[q] / R;
+      const real gxc = x0[0] / R, gyc = x0[1] / R, gzc = x0[2] / R;
+      const real prd_b = o[3] * c.ALPHA - (o[0] * gxc + o[1] * gyc + o[2] * gzc);
+      dv[0] = o[0] + prd_b * gxc;
+      dv[1] = o[1] + prd_b * gyc;
+      dv[2] = o[2] + prd_b * gzc;
+      prog_b[a] = prog_b[a] + real(2.0) * c.OHM * (dv[0] * vvm0[1] - dv[1] * vvm0[0]);
+    }
+    for (int q = 0; q < 3; ++q) S[q][a] = dv[q];
+  }
+};
+
+struct AdvMomG2 {
+  static constexpr const char* kName = "adv_mom_adj2";
+  Dims d; Consts c; Geo G;
+  const real* prog;
+  const real* diag;
+  const real* S[3];
+  real* prog_b;
+  real* diag_b;
+  real* W;
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p)) return;
+    const long NF = d.NF, a = p.a, ks = p.ks;
+    const int kmin = d.kmin, kmax = d.kmax, k = p.k;
+    if (k < kmin || k > kmax) {
+      W[a] = real(0.0);
+      return;
+    }
     const real R = c.RADIUS;
     const real* rw = prog + 4 * NF;
     const real* rg = prog;
     const real* wd = diag + I_W * NF;
-    real* wdb = diag_b + I_W * NF;
-    const int NN = p.pole ? 6 : 7;
-    const long g = d.g;
-    auto nbr = [&](int m) { return p.pole ? a + m : a + nb_off(m, g); };
-    auto pos = [&](int m, real* xx) {
-      if (p.pole) {
-        for (int q = 0; q < 3; ++q) xx[q] = G.gx_pl[(m * 2 + p.lp) * 3 + q];
-      } else {
-        for (int q = 0; q < 3; ++q) xx[q] = G.gx[q * d.nreg + p.ci + nb_off(m, g)];
-      }
-    };
-    // merged velocity at address b, level kk, position xx
-    auto vv3 = [&](long b, int kk, const real* xx, real* vo) {
+    // the point's position (a pole point's own) and merged velocity
+    real xp[3];
+    if (p.pole) {
+      for (int q = 0; q < 3; ++q) xp[q] = G.gx_pl[(p.v * 2 + p.lp) * 3 + q];
+    } else {
+      for (int q = 0; q < 3; ++q) xp[q] = G.gx[q * d.nreg + p.ci];
+    }
+    auto vv3 = [&](long b, int kk, real* vo) {
       const real wc = G.cfact[kk] * wd[b + ks] + G.dfact[kk] * wd[b];
-      for (int q = 0; q < 3; ++q) vo[q] = diag[(I_VX + q) * NF + b] + wc * xx[q] / R;
+      for (int q = 0; q < 3; ++q) vo[q] = diag[(I_VX + q) * NF + b] + wc * xp[q] / R;
     };
-    // w bars of this column at levels k-1 .. k+2, summed before the adds
-    real wcol[4] = {real(0.0), real(0.0), real(0.0), real(0.0)};
-    auto vv3_t = [&](long b, int kk, const real* xx, const real* vb) {
-      real wcb = real(0.0);
-      for (int q = 0; q < 3; ++q) {
-        aadd(&diag_b[(I_VX + q) * NF + b], vb[q]);
-        wcb += vb[q] * xx[q] / R;
-      }
-      if (b == a - ks || b == a || b == a + ks) {
-        // the column's own levels kk, kk + 1 (kk = k - 1, k, k + 1)
-        const int jw = kk - k + 1;
-        wcol[jw + 1] += wcb * G.cfact[kk];
-        wcol[jw] += wcb * G.dfact[kk];
-      } else {
-        aadd(&wdb[b + ks], wcb * G.cfact[kk]);
-        aadd(&wdb[b], wcb * G.dfact[kk]);
-      }
-    };
-    real x0[3];
-    pos(0, x0);
-    real vvm0[3];
-    vv3(a, k, x0, vvm0);
-    // forward tail
-    const real gxc = x0[0] / R, gyc = x0[1] / R, gzc = x0[2] / R;
-    const real rh = rg[a];
-    const real prd_b = o[3] * c.ALPHA - (o[0] * gxc + o[1] * gyc + o[2] * gzc);
-    const real dxb = o[0] + prd_b * gxc, dyb = o[1] + prd_b * gyc, dzb = o[2] + prd_b * gzc;
-    real vvm0_b[3] = {real(0.0), real(0.0), real(0.0)};
-    aadd(&prog_b[a], real(2.0) * c.OHM * (dxb * vvm0[1] - dyb * vvm0[0]));
-    vvm0_b[1] += real(2.0) * rh * c.OHM * dxb;
-    vvm0_b[0] -= real(2.0) * rh * c.OHM * dyb;
-    const real dvvb[3] = {dxb, dyb, dzb};
-    real divb[3], hib[3], lob[3];
-    for (int q = 0; q < 3; ++q) {
-      divb[q] = -dvvb[q];
-      hib[q] = -dvvb[q] * G.rdgz[k];
-      lob[q] = dvvb[q] * G.rdgz[k];
-    }
-    if (k + 1 <= kmax) {
-      real up[3], upb[3];
-      vv3(a + ks, k + 1, x0, up);
-      const real wup = rw[a + ks];
-      real wupb = real(0.0);
-      for (int q = 0; q < 3; ++q) {
-        wupb += hib[q] * (G.afact[k + 1] * up[q] + G.bfact[k + 1] * vvm0[q]);
-        upb[q] = hib[q] * wup * G.afact[k + 1];
-        vvm0_b[q] += hib[q] * wup * G.bfact[k + 1];
-      }
-      aadd(&prog_b[4 * NF + a + ks], wupb);
-      vv3_t(a + ks, k + 1, x0, upb);
-    }
-    if (k >= kmin + 1) {
-      real dn[3], dnb[3];
-      vv3(a - ks, k - 1, x0, dn);
-      const real w0 = rw[a];
-      real w0b = real(0.0);
-      for (int q = 0; q < 3; ++q) {
-        w0b += lob[q] * (G.afact[k] * vvm0[q] + G.bfact[k] * dn[q]);
-        vvm0_b[q] += lob[q] * w0 * G.afact[k];
-        dnb[q] = lob[q] * w0 * G.bfact[k];
-      }
-      aadd(&prog_b[4 * NF + a], w0b);
-      vv3_t(a - ks, k - 1, x0, dnb);
-    }
-    // divergence of rhogv * vv_q over the neighbours
-    for (int m = 0; m < NN; ++m) {
-      real xm[3], vv[3];
-      pos(m, xm);
-      const long bm = nbr(m);
-      vv3(bm, k, xm, vv);
-      real cdir[3];
-      if (p.pole) {
-        for (int dd = 0; dd < 3; ++dd) cdir[dd] = G.cdiv_pl[(m * 2 + p.lp) * 3 + dd];
-      } else {
-        for (int dd = 0; dd < 3; ++dd) cdir[dd] = rc(G.cdiv, d.nreg, dd * 7 + m, p.ci);
-      }
-      real fb[3] = {real(0.0), real(0.0), real(0.0)};
-      real vvb[3];
-      for (int q = 0; q < 3; ++q) {
-        vvb[q] = (m == 0) ? vvm0_b[q] : real(0.0);
+    real vvP[3];
+    vv3(a, k, vvP);
+    real vvb[3] = {real(0.0), real(0.0), real(0.0)};
+    real fb[3] = {real(0.0), real(0.0), real(0.0)};
+    // divergence terms of the compute points Q whose stencil holds this point
+    auto from = [&](long aQ, const real cdir[3]) {
+      real divb[3];
+      for (int q = 0; q < 3; ++q) divb[q] = -S[q][aQ];
+      for (int q = 0; q < 3; ++q)
         for (int dd = 0; dd < 3; ++dd) {
           const real t = cdir[dd] * divb[q];
-          fb[dd] += t * vv[q];
-          vvb[q] += t * prog[(1 + dd) * NF + bm];
+          fb[dd] += t * vvP[q];
+          vvb[q] += t * prog[(1 + dd) * NF + a];
         }
+    };
+    if (p.pole) {
+      real cdir[3];
+      for (int dd = 0; dd < 3; ++dd) cdir[dd] = G.cdiv_pl[(p.v * 2 + p.lp) * 3 + dd];
+      from(a - p.v, cdir);
+    } else {
+      const long g = d.g;
+      for (int m = 0; m < 7; ++m) {
+        const int iq = p.i - (((4201 >> (2 * m)) & 3) - 1), jq = p.j - (((421 >> (2 * m)) & 3) - 1);
+        if (iq < 1 || iq > d.g - 2 || jq < 1 || jq > d.g - 2) continue;
+        const long off = nb_off(m, g);
+        real cdir[3];
+        for (int dd = 0; dd < 3; ++dd) cdir[dd] = rc(G.cdiv, d.nreg, dd * 7 + m, p.ci - off);
+        from(a - off, cdir);
       }
-      for (int dd = 0; dd < 3; ++dd) aadd(&prog_b[(1 + dd) * NF + bm], fb[dd]);
-      vv3_t(bm, k, xm, vvb);
     }
-    for (int jw = 0; jw < 4; ++jw) {
-      const int kw = k - 1 + jw;
-      if (kw >= 0 && kw <= d.kall - 1 && wcol[jw] != real(0.0)) aadd(&wdb[a + (long)(jw - 1) * ks], wcol[jw]);
+    real rwb = real(0.0);
+    if (pt_compute(p)) {
+      // its own tail: Coriolis and the vertical terms' vvm0 parts
+      const real dv[3] = {S[0][a], S[1][a], S[2][a]};
+      const real rh = rg[a];
+      vvb[1] += real(2.0) * rh * c.OHM * dv[0];
+      vvb[0] -= real(2.0) * rh * c.OHM * dv[1];
+      if (k + 1 <= kmax) {
+        const real wup = rw[a + ks];
+        for (int q = 0; q < 3; ++q) vvb[q] += (-dv[q] * G.rdgz[k]) * wup * G.bfact[k + 1];
+      }
+      if (k >= kmin + 1) {
+        const real w0 = rw[a];
+        real dn[3];
+        vv3(a - ks, k - 1, dn);
+        real w0b = real(0.0);
+        for (int q = 0; q < 3; ++q) {
+          const real lob = dv[q] * G.rdgz[k];
+          w0b += lob * (G.afact[k] * vvP[q] + G.bfact[k] * dn[q]);
+          vvb[q] += lob * w0 * G.afact[k];
+        }
+        rwb += w0b;
+      }
+      // the up term of level k - 1 and the down term of level k + 1
+      if (k - 1 >= kmin) {
+        const real wup = rw[a];
+        real vm[3];
+        vv3(a - ks, k - 1, vm);
+        real wupb = real(0.0);
+        for (int q = 0; q < 3; ++q) {
+          const real hib = -S[q][a - ks] * G.rdgz[k - 1];
+          wupb += hib * (G.afact[k] * vvP[q] + G.bfact[k] * vm[q]);
+          vvb[q] += hib * wup * G.afact[k];
+        }
+        rwb += wupb;
+      }
+      if (k + 1 <= kmax) {
+        const real w0 = rw[a + ks];
+        for (int q = 0; q < 3; ++q) vvb[q] += (S[q][a + ks] * G.rdgz[k + 1]) * w0 * G.bfact[k + 1];
+      }
     }
+    for (int dd = 0; dd < 3; ++dd) prog_b[(1 + dd) * NF + a] += fb[dd];
+    prog_b[4 * NF + a] += rwb;
+    real wcb = real(0.0);
+    for (int q = 0; q < 3; ++q) {
+      diag_b[(I_VX + q) * NF + a] += vvb[q];
+      wcb += vvb[q] * xp[q] / R;
+    }
+    W[a] = wcb;
+  }
+};
+
+struct AdvMomG3 {
+  static constexpr const char* kName = "adv_mom_adj3";
+  Dims d; Geo G;
+  const real* W;
+  real* wdb;       // bar of diag w
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p)) return;
+    const long a = p.a, ks = p.ks;
+    const int k = p.k;
+    real add = real(0.0);
+    if (k - 1 >= d.kmin && k - 1 <= d.kmax) add += W[a - ks] * G.cfact[k - 1];
+    if (k >= d.kmin && k <= d.kmax) add += W[a] * G.dfact[k];
+    wdb[a] += add;
   }
 };
 

@@ -250,12 +250,25 @@ void stage_large_adj(X& x, const Context& ctx, const real* prog, const real* rho
     x.run_pts(l, d);
   }
   {
-    AdvMomA f;
-    f.d = d; f.c = ctx.c; f.G = ctx.G;
-    f.prog = prog; f.diag = diag;
-    f.ob = afield(ctx, A_GADVB);
-    f.prog_b = prog_b; f.diag_b = diagb;
-    x.run_pts(f, d);
+    // momentum advection transposed as gathers; the damping transposes'
+    // triangle weights are dead here and hold the dvv bars and W
+    real* S[3] = {afield(ctx, A_TRIW), afield(ctx, A_TRIW + 1), afield(ctx, A_TRIW + 2)};
+    real* W = afield(ctx, A_TRIW + 3);
+    AdvMomG1 f1;
+    f1.d = d; f1.c = ctx.c; f1.G = ctx.G;
+    f1.prog = prog; f1.diag = diag;
+    f1.ob = afield(ctx, A_GADVB); f1.prog_b = prog_b;
+    for (int q = 0; q < 3; ++q) f1.S[q] = S[q];
+    x.run_pts(f1, d);
+    AdvMomG2 f2;
+    f2.d = d; f2.c = ctx.c; f2.G = ctx.G;
+    f2.prog = prog; f2.diag = diag;
+    for (int q = 0; q < 3; ++q) f2.S[q] = S[q];
+    f2.prog_b = prog_b; f2.diag_b = diagb; f2.W = W;
+    x.run_pts(f2, d);
+    AdvMomG3 f3;
+    f3.d = d; f3.G = ctx.G; f3.W = W; f3.wdb = diagb + I_W * NF;
+    x.run_pts(f3, d);
   }
   {
     PrePostA f;
