@@ -1404,6 +1404,10 @@ struct ViPath0A {
   real* gadv_b;      // 4 fields
   real* vt_b[6];
   real* dd_b[3];
+  // the bars the horizontal operators send to the stencil, per compute point
+  // and level (zero off the physical levels), gathered by ViPath0G: the two
+  // divergences' (-gT0, -gT5 bars) and the pressure gradient's (3)
+  real* S[5];
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
@@ -1422,6 +1426,8 @@ struct ViPath0A {
     }
     const real geb = getot_b[a];
     getot_b[a] = real(0.0);
+    if (k < kmin || k > kmax)
+      for (int f = 0; f < 5; ++f) S[f][a] = real(0.0);
     double mcb = 0.0, mub = 0.0, mlb = 0.0;
     if (k >= kmin + 1 && k <= kmax) {
       mcb = Mc_b[a]; mub = Mu_b[a]; mlb = Ml_b[a];
@@ -1503,25 +1509,9 @@ struct ViPath0A {
         Aet[2] += wsb * w * G.afact[k];
         Aet[1] += wsb * w * G.bfact[k];
       }
-      // the two divergences
-      const int NN = p.pole ? 6 : 7;
-      for (int m = 0; m < NN; ++m) {
-        const long bn = p.pole ? a + m : a + nb_off(m, d.g);
-        real cdir[3];
-        if (p.pole) {
-          for (int q = 0; q < 3; ++q) cdir[q] = G.cdiv_pl[(m * 2 + p.lp) * 3 + q];
-        } else {
-          for (int q = 0; q < 3; ++q) cdir[q] = rc(G.cdiv, d.nreg, q * 7 + m, p.ci);
-        }
-        const real e = ldc(eth + (bn));
-        real eb = real(0.0);
-        for (int q = 0; q < 3; ++q) {
-          const real t0 = cdir[q] * hdb, t1 = cdir[q] * heb;
-          aadd(&prog_b[(1 + q) * NF + bn], t0 + t1 * e);
-          eb += t1 * ldc(prog + ((1 + q) * NF + bn));
-        }
-        aadd(&eth_b[bn], eb);
-      }
+      // the two divergences: gathered by ViPath0G
+      S[0][a] = hdb;
+      S[1][a] = heb;
       // pressure gradient (level kmin -> kmin+1) and pw
       const long sh = (k == kmin) ? ks : 0;
       real gr[3];
@@ -1551,20 +1541,8 @@ struct ViPath0A {
         aadd(&diag_b[(I_VX + q) * NF + a], pwb * dpg[q]);
       }
       proj_t(xx, R, dpgb);
-      // one atomic add per stencil point (the three directions summed first)
-      if (p.pole) {
-        for (int v = 0; v < 6; ++v) {
-          real sb = real(0.0);
-          for (int dd2 = 0; dd2 < 3; ++dd2) sb += G.cgrad_pl[(v * 2 + p.lp) * 3 + dd2] * dpgb[dd2];
-          aadd(&diag_b[I_PRE * NF + a + v + sh], sb);
-        }
-      } else {
-        for (int m = 0; m < 7; ++m) {
-          real sb = real(0.0);
-          for (int dd2 = 0; dd2 < 3; ++dd2) sb += rc(G.cgrad, d.nreg, dd2 * 7 + m, p.ci) * dpgb[dd2];
-          aadd(&diag_b[I_PRE * NF + a + nb_off(m, d.g) + sh], sb);
-        }
-      }
+      // the gradient's stencil: gathered by ViPath0G
+      for (int dd2 = 0; dd2 < 3; ++dd2) S[2 + dd2][a] = dpgb[dd2];
       // pw: w2c0(k) pwh(k+1) + w2c1(k) pwh(k), pwh(kk) = -g~(kk) rw(kk)
       for (int s2 = 0; s2 < 2; ++s2) {
         const int kk = (s2 == 0) ? k + 1 : k;
@@ -1609,6 +1587,78 @@ struct ViPath0A {
       if (Aet[jw] != real(0.0)) aadd(&eth_b[aw], Aet[jw]);
       if (Arw[jw] != real(0.0)) aadd(&prog_b[4 * NF + aw], Arw[jw]);
     }
+  }
+};
+
+// the horizontal part of ViPath0A's transpose as a gather: each point and
+// level collects, from the compute points whose stencil holds it, the bars of
+// rhogv and eth (the two divergences, at the same level) and of the pressure
+// (the gradient: a point at kmin + 1 also takes the kmin points', which read
+// the pressure there)
+struct ViPath0G {
+  static constexpr const char* kName = "vi_path0_adj2";
+  Dims d; Geo G;
+  const real* prog;
+  const real* eth;
+  const real* S[5];
+  real* prog_b;
+  real* diag_b;
+  real* eth_b;
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p)) return;
+    const long NF = d.NF, a = p.a, ks = p.ks;
+    const int kmin = d.kmin, kmax = d.kmax, k = p.k;
+    if (k < kmin || k > kmax) return;
+    const real e = ldc(eth + a);
+    real pv[3];
+    for (int q = 0; q < 3; ++q) pv[q] = ldc(prog + (1 + q) * NF + a);
+    real rvb[3] = {real(0.0), real(0.0), real(0.0)};
+    real eb = real(0.0), preb = real(0.0);
+    // Q: the compute point, m: this point's place in its stencil
+    auto from = [&](long aQ, const real cdiv3[3], const real cgrad3[3], bool div, bool grad, long gsh) {
+      if (div) {
+        const real hdb = ldc(S[0] + aQ), heb = ldc(S[1] + aQ);
+        for (int q = 0; q < 3; ++q) {
+          const real t0 = cdiv3[q] * hdb, t1 = cdiv3[q] * heb;
+          rvb[q] += t0 + t1 * e;
+          eb += t1 * pv[q];
+        }
+      }
+      if (grad) {
+        real sb = real(0.0);
+        for (int dd2 = 0; dd2 < 3; ++dd2) sb += cgrad3[dd2] * ldc(S[2 + dd2] + aQ - gsh);
+        preb += sb;
+      }
+    };
+    // the gradient's points at this level (none at kmin) and, at kmin + 1, kmin's
+    const bool gk = k >= kmin + 1, gk0 = (k == kmin + 1);
+    if (p.pole) {
+      real cd[3], cg[3];
+      for (int q = 0; q < 3; ++q) {
+        cd[q] = G.cdiv_pl[(p.v * 2 + p.lp) * 3 + q];
+        cg[q] = G.cgrad_pl[(p.v * 2 + p.lp) * 3 + q];
+      }
+      from(a - p.v, cd, cg, true, gk, 0);
+      if (gk0) from(a - p.v, cd, cg, false, true, ks);
+    } else {
+      for (int m = 0; m < 7; ++m) {
+        const int iq = p.i - (((4201 >> (2 * m)) & 3) - 1), jq = p.j - (((421 >> (2 * m)) & 3) - 1);
+        if (iq < 1 || iq > d.g - 2 || jq < 1 || jq > d.g - 2) continue;
+        const long off = nb_off(m, d.g);
+        real cd[3], cg[3];
+        for (int q = 0; q < 3; ++q) {
+          cd[q] = rc(G.cdiv, d.nreg, q * 7 + m, p.ci - off);
+          cg[q] = rc(G.cgrad, d.nreg, q * 7 + m, p.ci - off);
+        }
+        from(a - off, cd, cg, true, gk, 0);
+        if (gk0) from(a - off, cd, cg, false, true, ks);
+      }
+    }
+    // this point's thread is the only one adding to these elements here
+    for (int q = 0; q < 3; ++q) prog_b[(1 + q) * NF + a] += rvb[q];
+    eth_b[a] += eb;
+    diag_b[I_PRE * NF + a] += preb;
   }
 };
 

@@ -212,7 +212,16 @@ void stage_large_adj(X& x, const Context& ctx, const real* prog, const real* rho
     f.prog_b = prog_b; f.diag_b = diagb; f.eth_b = ethb;
     f.gadv_b = afield(ctx, A_GADVB);
     for (int q = 0; q < 3; ++q) f.dd_b[q] = afield(ctx, A_DDB + q);
+    // the stencil bars go through the damping transposes' triangle weights,
+    // which are free until divdamp_adj below
+    for (int q = 0; q < 5; ++q) f.S[q] = afield(ctx, A_TRIW + q);
     x.run_pts(f, d);
+    ViPath0G h;
+    h.d = d; h.G = ctx.G;
+    h.prog = prog; h.eth = ctx.field(W_ETH);
+    for (int q = 0; q < 5; ++q) h.S[q] = afield(ctx, A_TRIW + q);
+    h.prog_b = prog_b; h.diag_b = diagb; h.eth_b = ethb;
+    x.run_pts(h, d);
   }
   {
     real* ob[3] = {afield(ctx, A_DDB), afield(ctx, A_DDB + 1), afield(ctx, A_DDB + 2)};
