@@ -1298,51 +1298,39 @@ struct ViMainPostA {
     };
     rhogkin_k_t(G, kmin, kmax, k, g11, add11, -b2);
   }
-  // one launch per level class k mod 4 (color): the points of one launch
-  // write disjoint bars (their targets k-1 .. k+2 of their own column), so
-  // plain adds suffice.  The bars are summed in registers first; the
-  // (field, level) pairs the transpose can reach are then read together and
-  // written back (a pair the point did not reach gets + 0).
-  // The launch covers only the levels of its colour: plane index
-  // g * nkc + i (g: region 0..lall-1, then the two pole columns) is level
-  // color + 4 i of that region or pole column.
-  int color;
-  NH_HD static int ncolor_levels(const Dims& d, int color) { return (d.kall - color + 3) / 4; }
-  NH_HD void operator()(int cplane, int x) const {
-    const int nkc = ncolor_levels(d, color);
-    const int grp = cplane / nkc, kk = color + 4 * (cplane - grp * nkc);
-    const int plane = (grp < d.lall) ? grp * d.kall + kk : d.lall * d.kall + (grp - d.lall) * d.kall + kk;
+};
+
+// ViMainPostA's transpose as one gather launch: the thread of a compute point
+// at level j re-runs point() for the forward points of its column at levels
+// j - 2 .. j + 1 (those whose windows hold level j) and keeps their adds at
+// level j, so each bar element is read and written once, by one thread.
+struct ViMainPostG {
+  static constexpr const char* kName = "vi_main_post_adj";
+  ViMainPostA pa;
+  NH_HD void operator()(int plane, int x) const {
+    const Dims& d = pa.d;
     Pt p;
     if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
     const long a = p.a, ks = p.ks;
-    real acc[F_N][4];
-    for (int f = 0; f < F_N; ++f)
-      for (int j = 0; j < 4; ++j) acc[f][j] = real(0.0);
-    point(p, [&](int f, int dk, real v) { acc[f][dk + 1] += v; });
-    // reachable pairs: (field, dk + 1)
-    constexpr int np = 37;
-    constexpr signed char pf[np] = {F_P, F_P, F_P, F_P + 1, F_P + 2, F_P + 3, F_P + 4, F_P + 4,
-                                    F_S, F_S, F_S, F_S + 1, F_S + 2, F_S + 3, F_S + 4, F_S + 4,
-                                    F_S + 5, F_S + 5, F_S + 5, F_D, F_D + 1, F_D + 2,
-                                    F_W1, F_W1, F_W1, F_W1, F_H0, F_H0, F_H0, F_H2,
-                                    F_G0, F_G0, F_G0, F_ET, F_ET, F_ET, F_GE};
-    constexpr signed char pj[np] = {0, 1, 2, 1, 1, 1, 1, 2,
-                                    0, 1, 2, 1, 1, 1, 1, 2,
-                                    0, 1, 2, 1, 1, 1,
-                                    0, 1, 2, 3, 0, 1, 2, 1,
-                                    0, 1, 2, 0, 1, 2, 1};
-    const int k = p.k;
-    real cur[np];
+    const int j = p.k;
+    real acc[ViMainPostA::F_N];
+    for (int f = 0; f < ViMainPostA::F_N; ++f) acc[f] = real(0.0);
 #pragma unroll
-    for (int n = 0; n < np; ++n) {
-      const int kk = k + pj[n] - 1;
-      if (kk >= 0 && kk < d.kall) cur[n] = fptr(pf[n])[a + (long)(pj[n] - 1) * ks];
+    for (int dk = -1; dk <= 2; ++dk) {
+      const int k = j - dk;
+      if (k < 0 || k > d.kall - 1) continue;
+      Pt q = p;
+      q.k = k;
+      q.a = a - (long)dk * ks;
+      pa.point(q, [&](int f, int dd, real v) {
+        if (dd == dk) acc[f] += v;
+      });
     }
+    real cur[ViMainPostA::F_N];
 #pragma unroll
-    for (int n = 0; n < np; ++n) {
-      const int kk = k + pj[n] - 1;
-      if (kk >= 0 && kk < d.kall) fptr(pf[n])[a + (long)(pj[n] - 1) * ks] = cur[n] + acc[pf[n]][pj[n]];
-    }
+    for (int f = 0; f < ViMainPostA::F_N; ++f) cur[f] = pa.fptr(f)[a];
+#pragma unroll
+    for (int f = 0; f < ViMainPostA::F_N; ++f) pa.fptr(f)[a] = cur[f] + acc[f];
   }
 };
 

@@ -133,7 +133,7 @@ void small_step_adj(X& x, const Context& ctx, const real* prog, real* const ck[6
     f.we1 = nx[4];
     for (int q = 0; q < 3; ++q) f.we_b[q] = web[q];
     f.prog_b = prog_b; f.ethtot_b = afield(ctx, A_ETHTOTB); f.gT_b = gTb; f.getot_b = afield(ctx, A_GETOTB);
-    for (f.color = 0; f.color < 4; ++f.color) x.run_pts_n(f, d, (d.lall + 2) * ViMainPostA::ncolor_levels(d, f.color));
+    x.run_pts(ViMainPostG{f}, d);
   }
   {
     // transposed solve: y per column, then the bars per point and level
