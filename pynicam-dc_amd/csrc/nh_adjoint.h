@@ -293,7 +293,11 @@ struct NegLapGA {
 
 // second pass, diffused fields: -Kh-weighted diffusion of the temperature and
 // density fields with Kh = rhog * fac * GAMMA_H (the four Laplacian fields
-// go through NegLapGA)
+// go through NegLapGA).  Three launches: HDiffKhTriA stores, per triangle,
+// the bars of its three vertices; HDiffKhG gathers them per point; this
+// functor then scatters, with atomic adds, the compute points the two leave
+// out: the pole centres and the pentagon point (1, 1) of the regions whose
+// stencil there drops a triangle (sgp), one thread each.
 struct HDiffPass2A {
   static constexpr const char* kName = "hdiff_pass2_adj";
   Dims d; Consts c; Geo G;
@@ -302,9 +306,22 @@ struct HDiffPass2A {
   real* ob[6];
   real* ib[6];
   real* rhog_b;
-  NH_HD void operator()(int plane, int x) const {
+  // t < lall * kall: the point (1, 1) of region t / kall at level t % kall
+  // (sgp regions only); then the pole centres, one per pole plane
+  NH_HD long threads() const { return (long)(d.lall + 2) * d.kall; }
+  NH_HD void operator()(long t) const {
+    const long nrp = (long)d.lall * d.kall;
     Pt p;
-    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    if (t < nrp) {
+      if (!G.sgp[t / d.kall]) return;
+      if (!pt_of(d, (int)t, d.g + 1, p)) return;
+    } else if (!pt_of(d, (int)t, 0, p)) {
+      return;
+    }
+    if (!pt_compute(p)) return;
+    point(p);
+  }
+  NH_HD void point(const Pt& p) const {
     const long a = p.a;
     // regional points: the bars of rhog and of each field summed per stencil
     // point first, then one atomic add per stencil point
@@ -434,6 +451,134 @@ struct HDiffPass2A {
       aadd(&sb[a0], b0);
       aadd(&sb[a0 + v], b1);
       aadd(&sb[a0 + vp], b2);
+    }
+  }
+};
+
+// the Kh-weighted diffusion transposed per triangle.  Triangle (c, T) of
+// corner c has the vertices c, c + 1, c + 1 + g (T = 0) or c, c + 1 + g, c + g
+// (T = 1) and enters the stencils of those three points, each through the two
+// arcs from that point to the other two vertices.  One thread per corner and
+// level sums, for both fields and both triangles, the contributions of the
+// triangle's users (interior points other than the sgp pentagon point) to the
+// bar of its gradient, transposes the gradient, and stores the bars of its
+// vertices: S[f * 6 + T * 3 + v] at the corner's address (f = 0, 1 the two
+// fields, f = 2 rhog, whose bar from arc (u, w) lands on u and w).  Corners
+// 0..g-2 in i and j, every level.
+struct HDiffKhTriA {
+  static constexpr const char* kName = "hdiff_pass2_adj";
+  Dims d; Consts c; Geo G;
+  const real* in[2];   // fields 4, 5 of the first pass (halo-filled)
+  const real* ob[2];   // bars of pass 2's outputs 4, 5
+  const real* rhog;
+  real* S[18];
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p) || p.pole || p.i > d.g - 2 || p.j > d.g - 2) return;
+    const long g = d.g, nreg = d.nreg, ac = p.a, cc = p.ci;
+    const bool sgp = G.sgp[p.l] != 0;
+    const real kfac[2] = {c.CVdry * c.GAMMA_H, c.HDIFF_FACT_RHO * c.GAMMA_H};
+    for (int T = 0; T < 2; ++T) {
+      // vertex offsets from the corner
+      const long vo[3] = {0, (T == 0) ? 1 : 1 + g, (T == 0) ? 1 + g : g};
+      const int vi[3] = {0, 1, (T == 0) ? 1 : 0}, vj[3] = {0, (T == 0) ? 0 : 1, 1};
+      real tv[2][3];
+      for (int f = 0; f < 2; ++f) tri_grad(G, nreg, cc, T, in[f], ac, (int)g, tv[f]);
+      real tb[2][3] = {{real(0.0), real(0.0), real(0.0)}, {real(0.0), real(0.0), real(0.0)}};
+      real rb[3] = {real(0.0), real(0.0), real(0.0)};
+      real rv[3];
+      for (int v = 0; v < 3; ++v) rv[v] = rhog[ac + vo[v]];
+      // user u (vertex u) reaches the other two vertices through its arcs
+      // eA (to vertex vA) and eB (to vertex vB)
+      for (int u = 0; u < 3; ++u) {
+        const int ui = p.i + vi[u], uj = p.j + vj[u];
+        if (ui < 1 || ui > d.g - 2 || uj < 1 || uj > d.g - 2) continue;
+        if (sgp && ui == 1 && uj == 1) continue;
+        int eA, eB, vA, vB;
+        if (T == 0) {
+          if (u == 0) { eA = 0; vA = 2; eB = 5; vB = 1; }
+          else if (u == 1) { eA = 2; vA = 0; eB = 1; vB = 2; }
+          else { eA = 4; vA = 1; eB = 3; vB = 0; }
+        } else {
+          if (u == 0) { eA = 1; vA = 2; eB = 0; vB = 1; }
+          else if (u == 1) { eA = 3; vA = 0; eB = 2; vB = 2; }
+          else { eA = 5; vA = 1; eB = 4; vB = 0; }
+        }
+        const long au = ac + vo[u], cu = cc + vo[u];
+        real cA[3], cB[3];
+        for (int dd = 0; dd < 3; ++dd) {
+          cA[dd] = G.cdiff[(long)(dd * 6 + eA) * nreg + cu];
+          cB[dd] = G.cdiff[(long)(dd * 6 + eB) * nreg + cu];
+        }
+        for (int f = 0; f < 2; ++f) {
+          const real o = -ob[f][au];
+          const real kfA = real(0.5) * (rv[u] * kfac[f] + rv[vA] * kfac[f]);
+          const real kfB = real(0.5) * (rv[u] * kfac[f] + rv[vB] * kfac[f]);
+          real kbA = real(0.0), kbB = real(0.0);
+          for (int dd = 0; dd < 3; ++dd) {
+            const real ccA = cA[dd] * o, ccB = cB[dd] * o;
+            tb[f][dd] += ccA * kfA + ccB * kfB;
+            kbA += ccA * tv[f][dd];
+            kbB += ccB * tv[f][dd];
+          }
+          const real hA = real(0.5) * kbA * kfac[f], hB = real(0.5) * kbB * kfac[f];
+          rb[u] += hA + hB;
+          rb[vA] += hA;
+          rb[vB] += hB;
+        }
+      }
+      for (int f = 0; f < 2; ++f) {
+        real b0 = real(0.0), b1 = real(0.0), b2 = real(0.0);
+        for (int dd = 0; dd < 3; ++dd) {
+          const real c1 = rc(G.cintp, nreg, (dd * 3 + 0) * 2 + T, cc);
+          const real c2 = rc(G.cintp, nreg, (dd * 3 + 1) * 2 + T, cc);
+          const real c3 = rc(G.cintp, nreg, (dd * 3 + 2) * 2 + T, cc);
+          const real o = tb[f][dd] / real(3.0);
+          b0 += (real(2.0) * c1 - c2 - c3) * o;
+          b1 += (-c1 + real(2.0) * c2 - c3) * o;
+          b2 += (-c1 - c2 + real(2.0) * c3) * o;
+        }
+        S[f * 6 + T * 3 + 0][ac] = b0;
+        S[f * 6 + T * 3 + 1][ac] = b1;
+        S[f * 6 + T * 3 + 2][ac] = b2;
+      }
+      for (int v = 0; v < 3; ++v) S[12 + T * 3 + v][ac] = rb[v];
+    }
+  }
+};
+
+// the gather of HDiffKhTriA's vertex bars: point x is vertex 0 of triangles
+// (x, 0) and (x, 1), vertex 1 of (x - 1, 0) and (x - 1 - g, 1), vertex 2 of
+// (x - 1 - g, 0) and (x - g, 1); clears the output bars it has consumed
+struct HDiffKhG {
+  static constexpr const char* kName = "hdiff_pass2_adj";
+  Dims d; Geo G;
+  const real* S[18];
+  real* ob[2];
+  real* ib[2];
+  real* rhog_b;
+  NH_HD void operator()(int plane, int x) const {
+    Pt p;
+    if (!pt_of(d, plane, x, p) || p.pole) return;
+    const int gm = d.g - 2;
+    const long g = d.g, a = p.a;
+    // (corner offset, T, vertex, valid)
+    const long co[6] = {0, 0, -1, -1 - g, -1 - g, -g};
+    const int tT[6] = {0, 1, 0, 1, 0, 1}, tv[6] = {0, 0, 1, 1, 2, 2};
+    const bool ok[6] = {p.i <= gm && p.j <= gm, p.i <= gm && p.j <= gm, p.i >= 1 && p.j <= gm,
+                        p.i >= 1 && p.j >= 1, p.i >= 1 && p.j >= 1, p.i <= gm && p.j >= 1};
+    real acc[3] = {real(0.0), real(0.0), real(0.0)};
+    for (int n = 0; n < 6; ++n) {
+      if (!ok[n]) continue;
+      const int sidx = tT[n] * 3 + tv[n];
+      for (int f = 0; f < 3; ++f) acc[f] += S[f * 6 + sidx][a + co[n]];
+    }
+    ib[0][a] += acc[0];
+    ib[1][a] += acc[1];
+    rhog_b[a] += acc[2];
+    if (p.inner && !(p.i == 1 && p.j == 1 && G.sgp[p.l])) {
+      ob[0][a] = real(0.0);
+      ob[1][a] = real(0.0);
     }
   }
 };

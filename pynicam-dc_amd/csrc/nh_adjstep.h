@@ -231,6 +231,27 @@ void stage_large_adj(X& x, const Context& ctx, const real* prog, const real* rho
   // the forward step fills no halo of pass 2's output (path 0 reads it at its
   // compute points only), so its bars stay zero off the compute points
   {
+    // the triangles' vertex bars go through the split-state bars A_PSB ..
+    // A_DDVB, which the small steps of the stage no longer need
+    static_assert(A_TRIW - A_PSB >= 18, "18 scratch fields");
+    HDiffKhTriA t;
+    t.d = d; t.c = ctx.c; t.G = ctx.G;
+    for (int q = 0; q < 2; ++q) {
+      t.in[q] = sf.hda + (4 + q) * NF;
+      t.ob[q] = afield(ctx, A_HDBB + 4 + q);
+    }
+    t.rhog = prog;
+    for (int q = 0; q < 18; ++q) t.S[q] = afield(ctx, A_PSB + q);
+    x.run_pts(t, d);
+    HDiffKhG h;
+    h.d = d; h.G = ctx.G;
+    for (int q = 0; q < 18; ++q) h.S[q] = afield(ctx, A_PSB + q);
+    for (int q = 0; q < 2; ++q) {
+      h.ob[q] = afield(ctx, A_HDBB + 4 + q);
+      h.ib[q] = afield(ctx, A_HDAB + 4 + q);
+    }
+    h.rhog_b = prog_b;
+    x.run_pts(h, d);
     HDiffPass2A f;
     f.d = d; f.c = ctx.c; f.G = ctx.G;
     for (int q = 0; q < 6; ++q) {
@@ -240,7 +261,7 @@ void stage_large_adj(X& x, const Context& ctx, const real* prog, const real* rho
     }
     f.rhog = prog;
     f.rhog_b = prog_b;
-    x.run_pts(f, d);
+    x.run(f, f.threads());
     NegLapGA l;
     l.d = d; l.G = ctx.G; l.nq = 4;
     for (int q = 0; q < 4; ++q) {
