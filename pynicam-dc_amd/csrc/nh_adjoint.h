@@ -1053,8 +1053,13 @@ struct Path1A {
       }
     }
     if (nsrc > 0) ps_b[I_RHOGE][a] += eb;
-    // pointwise terms at the compute points (ghost rows folded into the nearest level)
-    if (!pt_compute(p) || k < kmin || k > kmax) return;
+    // pointwise terms at the compute points (ghost rows folded into the nearest
+    // level); the damping-force bars are stored (zeros elsewhere): nothing
+    // writes them earlier in the reversed small step
+    if (!pt_compute(p) || k < kmin || k > kmax) {
+      for (int q = 0; q < 3; ++q) dd_b[q][a] = real(0.0);
+      return;
+    }
     real vt[3];
     for (int q = 0; q < 3; ++q) vt[q] = dv_b[q][a];
     real rnb = real(0.0);
@@ -1078,7 +1083,7 @@ struct Path1A {
     for (int q = 0; q < 3; ++q) {
       ps_b[1 + q][a] += vt[q];
       gT_b[(1 + q) * NF + a] += vt[q] * dt;
-      dd_b[q][a] += vt[q] * dt;
+      dd_b[q][a] = vt[q] * dt;
     }
   }
 };
@@ -1449,13 +1454,27 @@ struct ViMainPostA {
 // at level j re-runs point() for the forward points of its column at levels
 // j - 2 .. j + 1 (those whose windows hold level j) and keeps their adds at
 // level j, so each bar element is read and written once, by one thread.
+// The bars of the small step's input split state and of the divergences 0, 2
+// are stored, not added (this is the first kernel of the reversed small step
+// to write them), and the divergence bar 1 is cleared: at the compute points
+// and, as zeros, at every other point.
 struct ViMainPostG {
   static constexpr const char* kName = "vi_main_post_adj";
   ViMainPostA pa;
+  NH_HD static bool stored(int f) {
+    return (f >= ViMainPostA::F_S && f < ViMainPostA::F_D) || f == ViMainPostA::F_H0 || f == ViMainPostA::F_H2;
+  }
   NH_HD void operator()(int plane, int x) const {
     const Dims& d = pa.d;
     Pt p;
-    if (!pt_of(d, plane, x, p) || !pt_compute(p)) return;
+    if (!pt_of(d, plane, x, p)) return;
+    if (!pt_compute(p)) {
+#pragma unroll
+      for (int f = 0; f < ViMainPostA::F_N; ++f)
+        if (stored(f)) pa.fptr(f)[p.a] = real(0.0);
+      pa.hv_b[1][p.a] = real(0.0);
+      return;
+    }
     const long a = p.a, ks = p.ks;
     const int j = p.k;
     real acc[ViMainPostA::F_N];
@@ -1473,9 +1492,10 @@ struct ViMainPostG {
     }
     real cur[ViMainPostA::F_N];
 #pragma unroll
-    for (int f = 0; f < ViMainPostA::F_N; ++f) cur[f] = pa.fptr(f)[a];
+    for (int f = 0; f < ViMainPostA::F_N; ++f) cur[f] = stored(f) ? real(0.0) : pa.fptr(f)[a];
 #pragma unroll
     for (int f = 0; f < ViMainPostA::F_N; ++f) pa.fptr(f)[a] = cur[f] + acc[f];
+    pa.hv_b[1][a] = real(0.0);
   }
 };
 
@@ -1865,47 +1885,92 @@ struct SfcPresP {
 // divergence damping, gather form.  Both operators are out(cell) =
 // hex_sum(cell, T) over the six triangles around the cell of triangle values
 // T = tri_div (+ the vertical rhogw term for the first operator).
-// Kernel W: per compute cell the six weights w_n = sum_dd H_n(dd) o_dd that
-// its output bar o sends to its triangles n (H_n: the hex_sum coefficient of
-// triangle n; in the first operator the pentagon corner (1,1) sends triangle
-// 4's share to triangle 5, as the forward substitutes T5 for T4 there).
-// Kernel G: per point x the triangle bars S(T) = sum of the weights of the
-// cells around T, then the tri_div transposes at x's corner of each triangle.
-// The pole centre keeps its five ring-triangle bars in w_0..w_4.
+// Kernel S: per corner b and level the bars of its two triangles, S(TI(b)) and
+// S(TJ(b)), each the sum over the triangle's three cells of the weight
+// w_n = sum_dd H_n(dd) o_dd that the cell's output bar o sends to it (H_n: the
+// hex_sum coefficient of triangle n of the cell; in the first operator the
+// pentagon corner (1,1) of an sgp region sends triangle 4's share to triangle
+// 5, as the forward substitutes T5 for T4 there).  The pole centre stores its
+// five ring-triangle bars in the pole slots 1..5 of the first field.
+// Kernel G: per point x the tri_div transposes at x's corner of each of the
+// six triangles around it (and, first operator, the vertical term from the
+// triangle sums at x's level and the one below).
 // ----------------------------------------------------------------------------
 template <int OP>
-struct DivDampWA {
+struct DivDampSA {
   static constexpr const char* kName = OP == 1 ? "divdamp_3d_adj" : "divdamp_post_adj";
   Dims d; Consts c; Geo G;
   const real* ob[3];
-  real* w;            // 6 fields
+  real* S[2];
+  // the output bar of cell (ii, jj) at the corner's level, projected and
+  // scaled in the second operator; false off the compute cells
+  NH_HD bool cell_o(const Pt& p, int ii, int jj, long a, long ci, real o[3]) const {
+    if (ii < 1 || ii > d.g - 2 || jj < 1 || jj > d.g - 2) return false;
+    for (int q = 0; q < 3; ++q) o[q] = ob[q][a];
+    if (OP == 2) {
+      real xx[3];
+      for (int q = 0; q < 3; ++q) xx[q] = G.gx[q * d.nreg + ci];
+      proj_t(xx, c.RADIUS, o);
+      for (int q = 0; q < 3; ++q) o[q] = c.ALPHA_D * o[q];
+    }
+    return true;
+  }
+  // weight of triangle n of the cell (hex_sum_t's Tb[n])
+  NH_HD real wn(long ci, const real o[3], int n) const {
+    const long nreg = d.nreg;
+    const int m0 = (n == 0) ? 0 : n - 1, m1 = (n == 0) ? 5 : n;
+    real t = real(0.0);
+    for (int dd = 0; dd < 3; ++dd) {
+      const real* cd = G.cdiff + (long)(dd * 6) * nreg + ci;
+      t += (cd[m0 * nreg] + cd[m1 * nreg]) * o[dd];
+    }
+    return t;
+  }
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p)) return;
-    const long NF = d.NF, a = p.a;
-    real wn[6] = {real(0.0), real(0.0), real(0.0), real(0.0), real(0.0), real(0.0)};
-    if (pt_compute(p) && p.k >= d.kmin && p.k <= d.kmax) {
-      real o[3];
-      for (int q = 0; q < 3; ++q) o[q] = ob[q][a];
-      if (OP == 2) {
-        real xx[3];
-        point_pos(G, d, p, xx);
-        proj_t(xx, c.RADIUS, o);
-        for (int q = 0; q < 3; ++q) o[q] = c.ALPHA_D * o[q];
-      }
-      if (p.pole) {
-        real sb[5];
-        pole_ring_t(G, p.lp, o, sb);
-        for (int v = 0; v < 5; ++v) wn[v] = sb[v];
-      } else {
-        hex_sum_t(G, d.nreg, p.ci, o, wn);
-        if (OP == 1 && p.i == 1 && p.j == 1 && G.sgp[p.l]) {
-          wn[5] += wn[4];
-          wn[4] = real(0.0);
+    const bool act = p.k >= d.kmin && p.k <= d.kmax;
+    if (p.pole) {
+      if (p.v != 0) return;
+      real sb[5] = {real(0.0), real(0.0), real(0.0), real(0.0), real(0.0)};
+      if (act) {
+        real o[3];
+        for (int q = 0; q < 3; ++q) o[q] = ob[q][p.a];
+        if (OP == 2) {
+          real xx[3];
+          point_pos(G, d, p, xx);
+          proj_t(xx, c.RADIUS, o);
+          for (int q = 0; q < 3; ++q) o[q] = c.ALPHA_D * o[q];
         }
+        pole_ring_t(G, p.lp, o, sb);
+      }
+      for (int v = 0; v < 5; ++v) S[0][p.a + 1 + v] = sb[v];
+      return;
+    }
+    const int g = d.g, i = p.i, j = p.j;
+    if (i > g - 2 || j > g - 2) return;
+    const long a = p.a, ci = p.ci;
+    // cells b, b+(1,0), b+(1,1), b+(0,1) and the weights they send here
+    real w0 = real(0.0), w1 = real(0.0), w2 = real(0.0), w3 = real(0.0), w4 = real(0.0), w5 = real(0.0);
+    if (act) {
+      const bool sgp = (OP == 1) && G.sgp[p.l];
+      real o[3];
+      if (cell_o(p, i, j, a, ci, o)) {
+        w0 = wn(ci, o, 0);
+        w1 = wn(ci, o, 1);
+      }
+      if (cell_o(p, i + 1, j, a + 1, ci + 1, o)) w2 = wn(ci + 1, o, 2);
+      if (cell_o(p, i + 1, j + 1, a + 1 + g, ci + 1 + g, o)) {
+        w3 = wn(ci + 1 + g, o, 3);
+        w4 = (sgp && i == 0 && j == 0) ? real(0.0) : wn(ci + 1 + g, o, 4);
+      }
+      if (cell_o(p, i, j + 1, a + g, ci + g, o)) {
+        w5 = wn(ci + g, o, 5);
+        if (sgp && i == 1 && j == 0) w5 += wn(ci + g, o, 4);
       }
     }
-    for (int n = 0; n < 6; ++n) w[n * NF + a] = wn[n];
+    S[0][a] = w0 + w2 + w4;
+    S[1][a] = w1 + w3 + w5;
   }
 };
 
@@ -1913,13 +1978,13 @@ template <int OP>
 struct DivDampGA {
   static constexpr const char* kName = OP == 1 ? "divdamp_3d_adj" : "divdamp_post_adj";
   Dims d; Consts c; Geo G;
-  const real* w;      // the weights of DivDampWA
+  const real* S[2];   // the triangle bars of DivDampSA
   real* vb[4];        // OP 1: bars of rhogvx..vz, rhogw (added); OP 2: bars of the first result (set)
-  // sum of the six triangles' bars at level offset (0 or -1) and their tri_div transposes at x
+  // the six triangles' bars at level offset (0 or -1) and their tri_div transposes at x
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p)) return;
-    const long NF = d.NF, a = p.a, ks = p.ks;
+    const long a = p.a, ks = p.ks;
     const int kmin = d.kmin, kmax = d.kmax, k = p.k;
     const real sgn = (OP == 1) ? real(1.0) : -real(1.0);
     real acc[3] = {real(0.0), real(0.0), real(0.0)};
@@ -1929,10 +1994,10 @@ struct DivDampGA {
       const long ac = a - p.v;   // pole centre
       const int lp = p.lp;
       auto tri = [&](int v, int corner) {
-        const real S = w[(v - 1) * NF + ac];
-        for (int dd = 0; dd < 3; ++dd) acc[dd] += G.cintp_pl[((v * 2 + lp) * 3 + dd) * 3 + corner] * sgn * S;
-        ssum += S;
-        if (vert) ssum_dn += w[(v - 1) * NF + ac - ks];
+        const real Sv = S[0][ac + v];
+        for (int dd = 0; dd < 3; ++dd) acc[dd] += G.cintp_pl[((v * 2 + lp) * 3 + dd) * 3 + corner] * sgn * Sv;
+        ssum += Sv;
+        if (vert) ssum_dn += S[0][ac + v - ks];
       };
       if (p.v == 0) {
         for (int v = 1; v <= 5; ++v) tri(v, 0);
@@ -1943,29 +2008,21 @@ struct DivDampGA {
     } else {
       const int g = d.g, i = p.i, j = p.j;
       const long nreg = d.nreg;
-      // weight n of cell (ii, jj) at level offset dk (0 outside the region's array)
-      auto wc = [&](int n, int ii, int jj, int dk) -> real {
-        if (ii < 0 || jj < 0 || ii > g - 1 || jj > g - 1) return real(0.0);
-        return w[n * NF + a + (long)(ii - i) + (long)(jj - j) * g + (long)dk * ks];
-      };
-      // the six triangles with x as a corner: (type, base (bi, bj), corner of x, cells n)
+      // the six triangles with x as a corner: (type, base (bi, bj), corner of x)
       const int tT[6] = {0, 0, 0, 1, 1, 1};
       const int tbi[6] = {0, -1, -1, 0, -1, 0}, tbj[6] = {0, 0, -1, 0, -1, -1};
       const int tcor[6] = {0, 1, 2, 0, 1, 2};
       for (int t = 0; t < 6; ++t) {
         const int bi = i + tbi[t], bj = j + tbj[t];
-        if (bi < 0 || bj < 0 || bi > g - 1 || bj > g - 1) continue;
-        auto S_at = [&](int dk) {
-          // TI(b): cells b (n 0), b+(1,0) (n 2), b+(1,1) (n 4); TJ(b): b (n 1), b+(1,1) (n 3), b+(0,1) (n 5)
-          return (tT[t] == 0) ? wc(0, bi, bj, dk) + wc(2, bi + 1, bj, dk) + wc(4, bi + 1, bj + 1, dk)
-                              : wc(1, bi, bj, dk) + wc(3, bi + 1, bj + 1, dk) + wc(5, bi, bj + 1, dk);
-        };
-        const real S = S_at(0);
+        // bases on the last row or column hold no compute cell
+        if (bi < 0 || bj < 0 || bi > g - 2 || bj > g - 2) continue;
+        const long ab = a + (long)tbi[t] + (long)tbj[t] * g;
+        const real Sv = S[tT[t]][ab];
         const long cj = p.ci + (long)tbi[t] + (long)tbj[t] * g;
         for (int dd = 0; dd < 3; ++dd)
-          acc[dd] += rc(G.cintp, nreg, (dd * 3 + tcor[t]) * 2 + tT[t], cj) * sgn * S;
-        ssum += S;
-        if (vert) ssum_dn += S_at(-1);
+          acc[dd] += rc(G.cintp, nreg, (dd * 3 + tcor[t]) * 2 + tT[t], cj) * sgn * Sv;
+        ssum += Sv;
+        if (vert) ssum_dn += S[tT[t]][ab - ks];
       }
     }
     if (OP == 1) {

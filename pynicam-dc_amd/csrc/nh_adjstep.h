@@ -88,15 +88,15 @@ void halo_block_adj(X& x, const Context& ctx, real* base, int nf) {
 template <class X>
 void divdamp_adj(X& x, const Context& ctx, real* const ob[3], real* const vb[4]) {
   real* tb[3] = {afield(ctx, A_DDTB), afield(ctx, A_DDTB + 1), afield(ctx, A_DDTB + 2)};
-  real* w = afield(ctx, A_TRIW);
-  DivDampWA<2> w2{ctx.d, ctx.c, ctx.G, {ob[0], ob[1], ob[2]}, w};
-  x.run_pts(w2, ctx.d);
-  DivDampGA<2> g2{ctx.d, ctx.c, ctx.G, w, {tb[0], tb[1], tb[2], nullptr}};
+  real* S[2] = {afield(ctx, A_TRIW), afield(ctx, A_TRIW + 1)};
+  DivDampSA<2> s2{ctx.d, ctx.c, ctx.G, {ob[0], ob[1], ob[2]}, {S[0], S[1]}};
+  x.run_pts(s2, ctx.d);
+  DivDampGA<2> g2{ctx.d, ctx.c, ctx.G, {S[0], S[1]}, {tb[0], tb[1], tb[2], nullptr}};
   x.run_pts(g2, ctx.d);
   halo_adj(x, ctx, tb, 3);
-  DivDampWA<1> w1{ctx.d, ctx.c, ctx.G, {tb[0], tb[1], tb[2]}, w};
-  x.run_pts(w1, ctx.d);
-  DivDampGA<1> g1{ctx.d, ctx.c, ctx.G, w, {vb[0], vb[1], vb[2], vb[3]}};
+  DivDampSA<1> s1{ctx.d, ctx.c, ctx.G, {tb[0], tb[1], tb[2]}, {S[0], S[1]}};
+  x.run_pts(s1, ctx.d);
+  DivDampGA<1> g1{ctx.d, ctx.c, ctx.G, {S[0], S[1]}, {vb[0], vb[1], vb[2], vb[3]}};
   x.run_pts(g1, ctx.d);
 }
 
@@ -118,10 +118,8 @@ void small_step_adj(X& x, const Context& ctx, const real* prog, real* const ck[6
   real* web[3] = {psb[0], psb[4], psb[5]};
   real* dvb[3] = {psb[1], psb[2], psb[3]};
   halo_adj(x, ctx, web, 3);
-  // the divergence bars and the damping-force bars (adjacent fields) start at
-  // zero; nothing before Path1A touches the damping-force bars
-  static_assert(A_DDVB == A_HVB + 3, "bars cleared together");
-  zero_fields(x, hvb[0], 6, NF);
+  // ViMainPostG stores psb2 and the divergence bars whole, Path1A the
+  // damping-force bars: none of them is cleared first
   {
     ViMainPostA f;
     f.d = d; f.c = ctx.c; f.G = ctx.G;
@@ -376,7 +374,7 @@ int dyn_step_adj(X& x, const Context& ctx, const real* prog_in, const real* prog
     }
     // adjoint of the stage
     // the stage bars the small steps accumulate into; A_PSB is stored whole by
-    // ProgUpdateA, A_PSB2 .. A_DDVB are cleared inside each small step
+    // ProgUpdateA, A_PSB2 .. A_DDVB by each small step
     zero_fields(x, afield(ctx, A_DIAGB), A_PSB - A_DIAGB, NF);
     zero_bytes(x, adfield(ctx, AD_MCB), 3 * NF * (long)sizeof(double));
     real *psb[6], *psb2[6];
@@ -395,7 +393,6 @@ int dyn_step_adj(X& x, const Context& ctx, const real* prog_in, const real* prog
       real *c0[6], *c1[6];
       ck_slot(it, c0);
       ck_slot(it + 1, c1);
-      zero_fields(x, psb2[0], 6, NF);
       small_step_adj(x, ctx, P, c0, c1, slot_hv(it), dts, prog_b, psb, psb2, ws_stage(ctx));
       for (int q = 0; q < 6; ++q) {
         real* t = psb[q];
@@ -439,7 +436,7 @@ int dyn_step_adj_tape(X& x, const Context& ctx, const void* tape, const real* pr
       x.run_pts(f, d);
     }
     // the stage bars the small steps accumulate into; A_PSB is stored whole by
-    // ProgUpdateA, A_PSB2 .. A_DDVB are cleared inside each small step
+    // ProgUpdateA, A_PSB2 .. A_DDVB by each small step
     zero_fields(x, afield(ctx, A_DIAGB), A_PSB - A_DIAGB, NF);
     zero_bytes(x, adfield(ctx, AD_MCB), 3 * NF * (long)sizeof(double));
     real *psb[6], *psb2[6];
@@ -460,7 +457,6 @@ int dyn_step_adj_tape(X& x, const Context& ctx, const void* tape, const real* pr
         c0[q] = tp.slot(nl, it) + q * NF;
         c1[q] = tp.slot(nl, it + 1) + q * NF;
       }
-      zero_fields(x, psb2[0], 6, NF);
       small_step_adj(x, ctx, P, c0, c1, tp.slot(nl, it) + 6 * NF, dts, prog_b, psb, psb2, sf);
       for (int q = 0; q < 6; ++q) {
         real* t = psb[q];
