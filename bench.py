@@ -501,6 +501,7 @@ def main():
             if f == "float64":
                 ref = r
             adj[FORMS[f][0]] = r
+            adj[FORMS[f][0] + "_stages"] = adjoint_leg(hipdyn, f, store="stages", same=r)
         for r in adj.values():
             r.pop("grad", None)
         print(json.dumps({"metric": "adjoint/forward step cost at gl06 z40 over 18 steps", "adjoint": adj}),

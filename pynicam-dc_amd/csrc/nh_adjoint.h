@@ -265,6 +265,9 @@ NH_HD inline bool src_of(const Dims& d, const Pt& x, int m, long* a, long* ci) {
 
 // -lap transposed, gather form: ib[q](x) += -sum over the compute points a
 // whose stencil holds x of clap(a, m) ob[q](a) (both hyperdiffusion passes)
+// (the interior points run as column tiles, LapTC; this functor's 1-D form
+// does the rest: the region borders, one thread per border point and level,
+// and the pole points)
 struct NegLapGA {
   static constexpr const char* kName = "hdiff_lap_adj";
   Dims d; Geo G;
@@ -274,6 +277,31 @@ struct NegLapGA {
   NH_HD void operator()(int plane, int x) const {
     Pt p;
     if (!pt_of(d, plane, x, p)) return;
+    point(p);
+  }
+  // t < lall * kall * 4 (g - 1): border point e of plane t / (4 (g - 1)),
+  // going round the region (j = 0, i = g - 1, j = g - 1, i = 0); then the
+  // pole points
+  NH_HD long border_threads() const { return (long)d.lall * d.kall * 4 * (d.g - 1) + 2L * d.kall * 6; }
+  NH_HD void operator()(long t) const {
+    const int g = d.g, per = 4 * (g - 1);
+    const long nb = (long)d.lall * d.kall * per;
+    Pt p;
+    if (t < nb) {
+      const int plane = (int)(t / per), e = (int)(t - (long)plane * per);
+      int i, j;
+      if (e < g - 1) { i = e; j = 0; }
+      else if (e < 2 * (g - 1)) { i = g - 1; j = e - (g - 1); }
+      else if (e < 3 * (g - 1)) { i = g - 1 - (e - 2 * (g - 1)); j = g - 1; }
+      else { i = 0; j = g - 1 - (e - 3 * (g - 1)); }
+      if (!pt_of(d, plane, j * g + i, p)) return;
+    } else {
+      const long u = t - nb;
+      if (!pt_of(d, d.lall * d.kall + (int)(u / 6), (int)(u % 6), p)) return;
+    }
+    point(p);
+  }
+  NH_HD void point(const Pt& p) const {
     const long a = p.a;
     real acc[6] = {real(0.0), real(0.0), real(0.0), real(0.0), real(0.0), real(0.0)};
     if (p.pole) {

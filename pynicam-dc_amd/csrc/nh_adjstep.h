@@ -100,6 +100,20 @@ void divdamp_adj(X& x, const Context& ctx, real* const ob[3], real* const vb[4])
   x.run_pts(g1, ctx.d);
 }
 
+// the transposed Laplacian of NegLapGA: interior points as column tiles,
+// then the border and pole points
+template <int N, class X>
+void lap_adj(X& x, const Context& ctx, const NegLapGA& l) {
+  const StGeom sg = st_geom(ctx.d, kColChunks);
+  ColStencil<LapTC<N>> cs;
+  cs.op.d = ctx.d;
+  cs.op.G = ctx.G;
+  for (int q = 0; q < N; ++q) { cs.op.ob[q] = l.ob[q]; cs.op.ib[q] = l.ib[q]; }
+  cs.cg = sg;
+  x.run_cols(cs, sg.nblocks, decltype(cs)::lds_elems(sg));
+  x.run(l, l.border_threads());
+}
+
 // reverse of one small step: ck = its input split state, nx = its output
 // (we0, dv0, dv1, dv2, we1, we2); psb (bar of nx, consumed) -> psb2 (bar of
 // ck, added) and the stage bars
@@ -266,7 +280,7 @@ void stage_large_adj(X& x, const Context& ctx, const real* prog, const real* rho
       l.ob[q] = afield(ctx, A_HDBB + q);
       l.ib[q] = afield(ctx, A_HDAB + q);
     }
-    x.run_pts(l, d);
+    lap_adj<4>(x, ctx, l);
   }
   halo_block_adj(x, ctx, afield(ctx, A_HDAB), 6);
   {
@@ -275,7 +289,7 @@ void stage_large_adj(X& x, const Context& ctx, const real* prog, const real* rho
     for (int q = 0; q < 6; ++q) l.ob[q] = afield(ctx, A_HDAB + q);
     l.ib[0] = diagb + I_VX * NF; l.ib[1] = diagb + I_VY * NF; l.ib[2] = diagb + I_VZ * NF;
     l.ib[3] = diagb + I_W * NF; l.ib[4] = diagb + I_TEM * NF; l.ib[5] = prog_b;
-    x.run_pts(l, d);
+    lap_adj<6>(x, ctx, l);
   }
   {
     // momentum advection transposed as gathers; the damping transposes'

@@ -230,6 +230,52 @@ struct LapC {
   }
 };
 
+// the transposed Laplacian of the hyperdiffusion adjoint at the interior
+// points: ib(x) += -sum_m clap(x - off(m), m) ob(x - off(m)), summed over
+// the interior sources in NegLapGA's order; per column the seven
+// coefficients (zero where the source is not interior) in registers, ob
+// staged per level.  NegLapGA's 1-D form does the border and pole points.
+template <int NFLD>
+struct LapTC {
+  static constexpr const char* kName = "hdiff_lap_adj";
+  static constexpr int kNS = NFLD;
+  Dims d; Geo G;
+  const real* ob[NFLD];
+  real* ib[NFLD];
+  struct Col {
+    real c[7];
+  };
+  struct Stg {};
+  NH_HD void init_stage(long, Stg&) const {}
+  NH_HD bool stage_level(int) const { return true; }
+  NH_HD void stage(int, long a, const Stg&, real* v) const {
+    for (int f = 0; f < NFLD; ++f) v[f] = ob[f][a];
+  }
+  NH_HD void init_col(long ci, Col& col) const {
+    const long gg = (long)d.g * d.g;
+    const long r = ci % gg;
+    const int j = (int)(r / d.g), i = (int)(r - (long)j * d.g);
+    for (int m = 0; m < 7; ++m) {
+      const int di = ((4201 >> (2 * m)) & 3) - 1, dj = ((421 >> (2 * m)) & 3) - 1;
+      const int is = i - di, js = j - dj;
+      const bool in = is >= 1 && is <= d.g - 2 && js >= 1 && js <= d.g - 2;
+      col.c[m] = in ? rc(G.clap, d.nreg, m, ci - (long)di - (long)dj * d.g) : real(0.0);
+    }
+  }
+  NH_HD void pole(int) const {}
+  template <class NB>
+  NH_HD void point(int, long a, const Col& col, NB nb) const {
+    // the source x - off(m) is staged neighbour opp(m)
+    const int opp[7] = {0, 4, 5, 6, 1, 2, 3};
+    for (int f = 0; f < NFLD; ++f) {
+      real acc = real(0.0);
+#pragma unroll
+      for (int m = 0; m < 7; ++m) acc += col.c[m] * nb(opp[m], f);
+      ib[f][a] += -acc;
+    }
+  }
+};
+
 // momentum advection and Coriolis (AdvMomP) as a column operator: the 21
 // divergence coefficients and the column's position in registers; staged per
 // level are the merged velocity v + w_full r/|r| (formed once per point, where
