@@ -348,14 +348,14 @@ int dyn_step_adj(X& x, const Context& ctx, const real* prog_in, const real* prog
   for (int nl = 0; nl < 3 && !stage_in; ++nl) {
     x.d2d(afield(ctx, A_SC + 6 * nl), P, 6 * NF * sizeof(real));
     if (nl == 2) break;
-    stage_large(x, ctx, P, progq, dts, ws_stage(ctx));
+    stage_large(x, ctx, P, progq, dts, ws_stage(ctx), false);
     SmallBufs s;
     for (int q = 0; q < 6; ++q) s.ps[q] = ctx.field(W_SPLIT + q);
     for (int q = 0; q < 3; ++q) s.dv[q] = ctx.field(W_SPLIT + 6 + q);
     for (int q = 0; q < 3; ++q) s.we[q] = ctx.field(W_SPLIT + 9 + q);
     x.run(SplitInit{NF, P, prog0, {s.ps[0], s.ps[1], s.ps[2], s.ps[3], s.ps[4], s.ps[5]}, nl == 0 ? 1 : 0}, NF);
     for (int it = 0; it < nsub[nl]; ++it) {
-      small_step(x, ctx, P, s, dts, 1.0 / (double)nsub[nl], ws_stage(ctx));
+      small_step(x, ctx, P, s, dts, 1.0 / (double)nsub[nl], ws_stage(ctx), false);
       real* old[6] = {s.ps[0], s.ps[1], s.ps[2], s.ps[3], s.ps[4], s.ps[5]};
       s.ps[0] = s.we[0]; s.ps[1] = s.dv[0]; s.ps[2] = s.dv[1]; s.ps[3] = s.dv[2]; s.ps[4] = s.we[1];
       s.ps[5] = s.we[2];
@@ -371,7 +371,7 @@ int dyn_step_adj(X& x, const Context& ctx, const real* prog_in, const real* prog
     if (nl < 2) halo_block_adj(x, ctx, prog_b, 6);
     // recompute the stage, keeping every small step's split state
     x.d2d(P, stage_input(nl), 6 * NF * sizeof(real));
-    stage_large(x, ctx, P, progq, dts, ws_stage(ctx));
+    stage_large(x, ctx, P, progq, dts, ws_stage(ctx), false);
     real* ck[6];
     ck_slot(0, ck);
     x.run(SplitInit{NF, P, prog0, {ck[0], ck[1], ck[2], ck[3], ck[4], ck[5]}, nl == 0 ? 1 : 0}, NF);
@@ -384,7 +384,7 @@ int dyn_step_adj(X& x, const Context& ctx, const real* prog_in, const real* prog
       for (int q = 0; q < 6; ++q) s.ps[q] = c0[q];
       s.we[0] = c1[0]; s.dv[0] = c1[1]; s.dv[1] = c1[2]; s.dv[2] = c1[3]; s.we[1] = c1[4]; s.we[2] = c1[5];
       s.hv = slot_hv(it);
-      small_step(x, ctx, P, s, dts, rweight, ws_stage(ctx));
+      small_step(x, ctx, P, s, dts, rweight, ws_stage(ctx), false);
     }
     // adjoint of the stage
     // the stage bars the small steps accumulate into; A_PSB is stored whole by

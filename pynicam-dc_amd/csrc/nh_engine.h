@@ -606,7 +606,8 @@ void hdiff_passes(X& x, const Context& ctx, const real* prog, const real* diag, 
 // damping of prog, the sources and matrix of the small steps, ethtot and the
 // start of the mean fluxes
 template <class X>
-void stage_large(X& x, const Context& ctx, real* prog, const real* rhogq, double dts, const StageFields& sf) {
+void stage_large(X& x, const Context& ctx, real* prog, const real* rhogq, double dts, const StageFields& sf,
+                 bool means = true) {
   const Dims& d = ctx.d;
   const long NF = d.NF;
   real* diag = ctx.field(W_DIAG);
@@ -645,7 +646,7 @@ void stage_large(X& x, const Context& ctx, real* prog, const real* rhogq, double
     run_staged(x, f);
   }
   {
-    EthTotPMP f{d, ctx.c, ctx.G, prog, eth, sf.ethtot, ctx.field(W_PM)};
+    EthTotPMP f{d, ctx.c, ctx.G, prog, eth, sf.ethtot, means ? ctx.field(W_PM) : nullptr};
     x.run_pts(f, d);
   }
 }
@@ -670,11 +671,11 @@ void small_hv(X& x, const Context& ctx, const real* prog, real* const dv[3], rea
 // momentum, path1, vi_main (horizontal, tridiagonal, mass/energy), halos
 template <class X>
 void small_step(X& x, const Context& ctx, const real* prog, const SmallBufs& s, double dts, double rweight,
-                const StageFields& sf) {
+                const StageFields& sf, bool means = true) {
   const Dims& d = ctx.d;
   const long NF = d.NF;
   real* gT = sf.gT;
-  real* pm = ctx.field(W_PM);
+  real* pm = means ? ctx.field(W_PM) : nullptr;   // the tracer's mean mass fluxes
   real* ddv = s.ddv ? s.ddv : ctx.field(W_DDV);
   real* hvb = s.hv ? s.hv : ctx.field(W_HV);
   divdamp(x, ctx, s.ps[1], s.ps[2], s.ps[3], s.ps[4], ddv, ddv + NF, ddv + 2 * NF);

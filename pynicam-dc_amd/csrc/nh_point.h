@@ -787,9 +787,11 @@ struct EthTotPMP {
     };
     const real kin0 = rhogkin_k(G, d.kmin, d.kmax, k, get);
     ethtot[a] = e + kin0 / rg[1] + G.phi[k];
-    pm[a] = rg[1];
-    for (int q = 0; q < 3; ++q) pm[(1 + q) * NF + a] = rv[q];
-    pm[4 * NF + a] = rw[0];
+    if (pm) {   // null: the mean fluxes are not wanted (adjoint recompute)
+      pm[a] = rg[1];
+      for (int q = 0; q < 3; ++q) pm[(1 + q) * NF + a] = rv[q];
+      pm[4 * NF + a] = rw[0];
+    }
   }
 };
 
@@ -878,8 +880,9 @@ struct Path1P {
     };
     // the mass-flux sums are read before any store (a store to dv could alias
     // them for all the compiler knows, which would put each read behind it)
-    real pmq[3];
-    for (int q = 0; q < 3; ++q) pmq[q] = pm[(1 + q) * NF + a];
+    real pmq[3] = {real(0.0), real(0.0), real(0.0)};
+    if (pm)
+      for (int q = 0; q < 3; ++q) pmq[q] = pm[(1 + q) * NF + a];
     real v[3];
     if (k >= kmin && k <= kmax) {
       level(p, a, grad, v);
@@ -894,7 +897,7 @@ struct Path1P {
     }
     for (int q = 0; q < 3; ++q) {
       dv[q][a] = v[q];
-      pm[(1 + q) * NF + a] = pmq[q] + v[q] * rweight;
+      if (pm) pm[(1 + q) * NF + a] = pmq[q] + v[q] * rweight;
     }
   }
 };
@@ -1113,7 +1116,7 @@ struct ViMainPost {
     const real h2 = hv[2][a], ge = getot[a];
     const int ke = (k < kmin) ? kmin : ((k > kmax) ? kmax : k);
     const real pe_k = ps[I_RHOGE][b + (long)ke * ks];
-    const real pm0 = pm[0 * NF + a], pm4 = pm[4 * NF + a];
+    const real pm0 = pm ? pm[0 * NF + a] : real(0.0), pm4 = pm ? pm[4 * NF + a] : real(0.0);
     // j: window index of level kk (k - 1 + j)
     auto rs1 = [&](int kk, int j) {
       real drhog2 = real(0.0);
@@ -1153,8 +1156,10 @@ struct ViMainPost {
     const real re1 = pe_k + (ge + dtot) * dt + (kin10 - kin11) + (S0[1] - r1) * G.phi[k];
     we[0][a] = r1;
     we[2][a] = re1;
-    pm[0 * NF + a] = pm0 + r1 * rweight;
-    pm[4 * NF + a] = pm4 + RW[1] * rweight;
+    if (pm) {
+      pm[0 * NF + a] = pm0 + r1 * rweight;
+      pm[4 * NF + a] = pm4 + RW[1] * rweight;
+    }
   }
 };
 
