@@ -70,7 +70,7 @@ NH_HD inline void proj_t(const real x[3], real R, real v[3]) {
 struct HaloScatterA {
   static constexpr const char* kName = "halo_adj";
   Dims d;
-  const int* dst;
+  const int* dst;   // level-0 field addresses, as in HaloGather
   const int* src;
   long n;
   real* f[16];
@@ -78,11 +78,11 @@ struct HaloScatterA {
   NH_HD void operator()(long t) const {
     HaloGather h;
     h.d = d;
-    const long hh = t % n;
-    const int k = (int)(t / n);
-    long ks_d, ks_s;
-    const long bd = h.colbase(dst[hh], &ks_d) + k * ks_d;
-    const long bs = h.colbase(src[hh], &ks_s) + k * ks_s;
+    const unsigned tt = (unsigned)t, nn = (unsigned)n;
+    const unsigned k = tt / nn, hh = tt - k * nn;
+    const long bd0 = dst[hh], bs0 = src[hh];
+    const long bd = bd0 + (long)k * h.stride(bd0);
+    const long bs = bs0 + (long)k * h.stride(bs0);
     // every destination bar read before any store or add (one memory round
     // trip rather than one per field)
     real v[16];

@@ -298,34 +298,40 @@ NH_HD inline real div_pl(const Geo& G, int lp, FX fx, FY fy, FZ fz) {
 // ----------------------------------------------------------------------------
 // halo gather
 // ----------------------------------------------------------------------------
+// field address of level 0 of halo-map column c (regional l*g*g + j*g + i,
+// then the 12 pole columns lp*6 + v) and its level stride
+NH_HD inline long halo_colbase(const Dims& d, long c, long* ks) {
+  if (c < d.nreg) {
+    long gg = (long)d.g * d.g;
+    long l = c / gg;
+    long r = c - l * gg;
+    *ks = gg;
+    return l * d.kall * gg + r;
+  }
+  long p = c - d.nreg;
+  long lp = p / 6;
+  *ks = 6;
+  return d.R + lp * d.kall * 6 + (p - lp * 6);
+}
+
 struct HaloGather {
   static constexpr const char* kName = "halo_gather";
   Dims d;
-  const int* dst;   // column ids
+  const int* dst;   // level-0 field addresses of the halo columns (halo_colbase)
   const int* src;
   long n;           // number of halo columns
   real* f[16];
   int nf;
-  NH_HD inline long colbase(long c, long* ks) const {
-    if (c < d.nreg) {
-      long gg = (long)d.g * d.g;
-      long l = c / gg;
-      long r = c - l * gg;
-      *ks = gg;
-      return l * d.kall * gg + r;
-    }
-    long p = c - d.nreg;
-    long lp = p / 6;
-    *ks = 6;
-    return d.R + lp * d.kall * 6 + (p - lp * 6);
-  }
-  // t over n * kall: halo column fastest so a wavefront copies consecutive halo slots
+  // level stride of a column from its level-0 address: regional below R
+  NH_HD inline long stride(long b) const { return (b < d.R) ? (long)d.g * d.g : 6L; }
+  // t over n * kall: halo column fastest so a wavefront copies consecutive
+  // halo slots (32-bit index arithmetic: n * kall < 2^31 is checked at upload)
   NH_HD inline void operator()(long t) const {
-    long h = t % n;
-    int k = (int)(t / n);
-    long ks_d, ks_s;
-    long bd = colbase(dst[h], &ks_d) + k * ks_d;
-    long bs = colbase(src[h], &ks_s) + k * ks_s;
+    const unsigned tt = (unsigned)t, nn = (unsigned)n;
+    const unsigned k = tt / nn, h = tt - k * nn;
+    const long bd0 = dst[h], bs0 = src[h];
+    const long bd = bd0 + (long)k * stride(bd0);
+    const long bs = bs0 + (long)k * stride(bs0);
     for (int q = 0; q < nf; ++q) f[q][bd] = f[q][bs];
   }
 };

@@ -74,7 +74,7 @@ struct Context {
   real* geo_buf = nullptr;      // coefficients and profiles in the field precision
   double* geo64_buf = nullptr;  // FP64 geometry (profiles, areas, latitude/longitude)
   int* int_buf = nullptr;
-  int* halo_dst = nullptr;
+  int* halo_dst = nullptr;      // level-0 field addresses of the halo map
   int* halo_src = nullptr;
   long halo_n = 0;
   real* ws = nullptr;
@@ -267,11 +267,20 @@ int upload_halo(X& x, Context& ctx, const int32_t* dst, const int32_t* src, long
     ctx.halo_adj_ok = ok;
   }
   if (n == 0) return 0;
+  // the kernels index halo slot and level in 32 bits and take level-0 field
+  // addresses in 32 bits
+  if (n * ctx.d.kall >= (1L << 31) || ctx.d.NF >= (1L << 31)) return NH_ERR_ARG;
+  std::vector<int> db((size_t)n), sb((size_t)n);
+  for (long h = 0; h < n; ++h) {
+    long ks;
+    db[(size_t)h] = (int)halo_colbase(ctx.d, dst[h], &ks);
+    sb[(size_t)h] = (int)halo_colbase(ctx.d, src[h], &ks);
+  }
   ctx.halo_dst = (int*)x.alloc(n * sizeof(int));
   ctx.halo_src = (int*)x.alloc(n * sizeof(int));
   if (!ctx.halo_dst || !ctx.halo_src) return NH_ERR_ALLOC;
-  x.h2d(ctx.halo_dst, dst, n * sizeof(int));
-  x.h2d(ctx.halo_src, src, n * sizeof(int));
+  x.h2d(ctx.halo_dst, db.data(), n * sizeof(int));
+  x.h2d(ctx.halo_src, sb.data(), n * sizeof(int));
   return x.check("upload_halo");
 }
 
