@@ -314,7 +314,7 @@ inline long ncols(const Dims& d) { return d.nreg + 12; }
 
 // level chunks of the column tiles (divergence damping, stencil operators)
 constexpr int kDivDampChunks = 4;
-constexpr int kColChunks = 4;
+constexpr int kColChunks = 6;
 
 // ----------------------------------------------------------------------------
 // 3-D divergence damping of (vx, vy, vz, w) into out (3 fields)
