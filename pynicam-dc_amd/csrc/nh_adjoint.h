@@ -1482,12 +1482,15 @@ struct ViMainPostA {
 // at level j re-runs point() for the forward points of its column at levels
 // j - 2 .. j + 1 (those whose windows hold level j) and keeps their adds at
 // level j, so each bar element is read and written once, by one thread.
-// The bars of the small step's input split state and of the divergences 0, 2
-// are stored, not added (this is the first kernel of the reversed small step
-// to write them), and the divergence bar 1 is cleared: at the compute points
-// and, as zeros, at every other point.
+// With 4-byte fields the bars of the small step's input split state and of
+// the divergences 0, 2 are stored, not added (this is the first kernel of the
+// reversed small step to write them), and the divergence bar 1 is cleared: at
+// the compute points and, as zeros, at every other point.  With 8-byte fields
+// the caller clears them first and all bars are added (measured faster there:
+// adjoint step 16.07 -> 15.79 ms at gl06; 9.50 against 9.54 ms in FP32).
 struct ViMainPostG {
   static constexpr const char* kName = "vi_main_post_adj";
+  static constexpr bool kStore = sizeof(real) == 4;
   ViMainPostA pa;
   NH_HD static bool stored(int f) {
     return (f >= ViMainPostA::F_S && f < ViMainPostA::F_D) || f == ViMainPostA::F_H0 || f == ViMainPostA::F_H2;
@@ -1497,10 +1500,12 @@ struct ViMainPostG {
     Pt p;
     if (!pt_of(d, plane, x, p)) return;
     if (!pt_compute(p)) {
+      if (kStore) {
 #pragma unroll
-      for (int f = 0; f < ViMainPostA::F_N; ++f)
-        if (stored(f)) pa.fptr(f)[p.a] = real(0.0);
-      pa.hv_b[1][p.a] = real(0.0);
+        for (int f = 0; f < ViMainPostA::F_N; ++f)
+          if (stored(f)) pa.fptr(f)[p.a] = real(0.0);
+        pa.hv_b[1][p.a] = real(0.0);
+      }
       return;
     }
     const long a = p.a, ks = p.ks;
@@ -1520,10 +1525,10 @@ struct ViMainPostG {
     }
     real cur[ViMainPostA::F_N];
 #pragma unroll
-    for (int f = 0; f < ViMainPostA::F_N; ++f) cur[f] = stored(f) ? real(0.0) : pa.fptr(f)[a];
+    for (int f = 0; f < ViMainPostA::F_N; ++f) cur[f] = (kStore && stored(f)) ? real(0.0) : pa.fptr(f)[a];
 #pragma unroll
     for (int f = 0; f < ViMainPostA::F_N; ++f) pa.fptr(f)[a] = cur[f] + acc[f];
-    pa.hv_b[1][a] = real(0.0);
+    if (kStore) pa.hv_b[1][a] = real(0.0);
   }
 };
 

@@ -132,8 +132,10 @@ void small_step_adj(X& x, const Context& ctx, const real* prog, real* const ck[6
   real* web[3] = {psb[0], psb[4], psb[5]};
   real* dvb[3] = {psb[1], psb[2], psb[3]};
   halo_adj(x, ctx, web, 3);
-  // ViMainPostG stores psb2 and the divergence bars whole, Path1A the
-  // damping-force bars: none of them is cleared first
+  // Path1A stores the damping-force bars whole; ViMainPostG stores psb2 and
+  // the divergence bars with 4-byte fields, and adds to them (cleared here
+  // and by the caller) with 8-byte fields
+  if (!ViMainPostG::kStore) zero_fields(x, hvb[0], 3, NF);
   {
     ViMainPostA f;
     f.d = d; f.c = ctx.c; f.G = ctx.G;
@@ -388,7 +390,8 @@ int dyn_step_adj(X& x, const Context& ctx, const real* prog_in, const real* prog
     }
     // adjoint of the stage
     // the stage bars the small steps accumulate into; A_PSB is stored whole by
-    // ProgUpdateA, A_PSB2 .. A_DDVB by each small step
+    // ProgUpdateA, A_PSB2 .. A_DDVB by each small step (cleared first with
+    // 8-byte fields)
     zero_fields(x, afield(ctx, A_DIAGB), A_PSB - A_DIAGB, NF);
     zero_bytes(x, adfield(ctx, AD_MCB), 3 * NF * (long)sizeof(double));
     real *psb[6], *psb2[6];
@@ -407,6 +410,7 @@ int dyn_step_adj(X& x, const Context& ctx, const real* prog_in, const real* prog
       real *c0[6], *c1[6];
       ck_slot(it, c0);
       ck_slot(it + 1, c1);
+      if (!ViMainPostG::kStore) zero_fields(x, psb2[0], 6, NF);
       small_step_adj(x, ctx, P, c0, c1, slot_hv(it), dts, prog_b, psb, psb2, ws_stage(ctx));
       for (int q = 0; q < 6; ++q) {
         real* t = psb[q];
@@ -450,7 +454,8 @@ int dyn_step_adj_tape(X& x, const Context& ctx, const void* tape, const real* pr
       x.run_pts(f, d);
     }
     // the stage bars the small steps accumulate into; A_PSB is stored whole by
-    // ProgUpdateA, A_PSB2 .. A_DDVB by each small step
+    // ProgUpdateA, A_PSB2 .. A_DDVB by each small step (cleared first with
+    // 8-byte fields)
     zero_fields(x, afield(ctx, A_DIAGB), A_PSB - A_DIAGB, NF);
     zero_bytes(x, adfield(ctx, AD_MCB), 3 * NF * (long)sizeof(double));
     real *psb[6], *psb2[6];
@@ -471,6 +476,7 @@ int dyn_step_adj_tape(X& x, const Context& ctx, const void* tape, const real* pr
         c0[q] = tp.slot(nl, it) + q * NF;
         c1[q] = tp.slot(nl, it + 1) + q * NF;
       }
+      if (!ViMainPostG::kStore) zero_fields(x, psb2[0], 6, NF);
       small_step_adj(x, ctx, P, c0, c1, tp.slot(nl, it) + 6 * NF, dts, prog_b, psb, psb2, sf);
       for (int q = 0; q < 6; ++q) {
         real* t = psb[q];
